@@ -1,0 +1,818 @@
+// Solver context and C ABI (include/coral_amd.h).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/coral_amd.h"
+#include "assembly.hpp"
+#include "kernels.hpp"
+
+using namespace coral;
+
+namespace {
+
+thread_local std::string g_err;
+
+struct HipError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+#define HIP_OK(call)                                                                          \
+    do {                                                                                      \
+        hipError_t e_ = (call);                                                               \
+        if (e_ != hipSuccess)                                                                 \
+            throw HipError(std::string(#call) + ": " + hipGetErrorString(e_));                \
+    } while (0)
+
+template <class T>
+struct DevBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    ~DevBuf() {
+        if (p) (void)hipFree(p);
+    }
+    void alloc(size_t count) {
+        if (p) HIP_OK(hipFree(p));
+        p = nullptr;
+        n = count;
+        if (count) HIP_OK(hipMalloc(&p, count * sizeof(T)));
+    }
+    void upload(const std::vector<T> &v) {
+        alloc(v.size());
+        if (!v.empty()) HIP_OK(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+    }
+};
+
+struct SysDev {
+    SystemTables *t = nullptr;
+    bool kxky = true;
+    int nsys = 0;
+    DevBuf<double2> field, aux;
+    std::vector<std::unique_ptr<DevBuf<double2>>> Khat, Kstd;
+    DevBuf<double2> Lf, Uf;
+    DevBuf<unsigned char> piv;
+    DevBuf<int> info;
+    DevBuf<double> mband, sband;
+    DevBuf<int> mkx, mky, skx, sky;
+    DevBuf<int> pos_eqn, pos_row, eqn_begin, t_qvar, t_kx, t_ky, t_w, t_order, t_woff;
+    DevBuf<double> t_dsca, wv;
+};
+
+struct ExtractDev {
+    int full, kind, system, kx, ky;
+    DevBuf<int> rowptr, col;
+    DevBuf<double> val;
+};
+
+struct TimerRec {
+    std::string name;
+    hipEvent_t a, b;
+};
+
+}  // namespace
+
+struct coral_amd_ctx {
+    Problem P;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    KeptModes km{};
+    std::vector<std::unique_ptr<SysDev>> ks, zs;
+    std::vector<std::unique_ptr<ExtractDev>> ext;
+    std::vector<int> full_has_kxky;
+    DevBuf<double2> spec, Fz, G, H, Hy, Qhat, phys;
+    DevBuf<double> Tz, Cz, zweight;
+    DevBuf<double2> twx, twy;
+    DevBuf<int> qa, qb;
+    DevBuf<unsigned long long> cflmax;
+    FFTPlan px, py;
+    double dt = 0.0;
+    bool factored = false;
+    bool timing = false;
+    std::vector<TimerRec> recs;
+    std::map<std::string, std::pair<double, int>> totals;
+
+    ~coral_amd_ctx() {
+        for (auto &r : recs) {
+            (void)hipEventDestroy(r.a);
+            (void)hipEventDestroy(r.b);
+        }
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+
+    template <class F>
+    void timed(const char *name, F &&f) {
+        if (!timing) {
+            f();
+            return;
+        }
+        TimerRec r;
+        r.name = name;
+        HIP_OK(hipEventCreate(&r.a));
+        HIP_OK(hipEventCreate(&r.b));
+        HIP_OK(hipEventRecord(r.a, stream));
+        f();
+        HIP_OK(hipEventRecord(r.b, stream));
+        recs.push_back(r);
+    }
+
+    void flush_timers() {
+        if (recs.empty()) return;
+        HIP_OK(hipStreamSynchronize(stream));
+        for (auto &r : recs) {
+            float ms = 0.f;
+            HIP_OK(hipEventElapsedTime(&ms, r.a, r.b));
+            auto &t = totals[r.name];
+            t.first += ms;
+            t.second += 1;
+            (void)hipEventDestroy(r.a);
+            (void)hipEventDestroy(r.b);
+        }
+        recs.clear();
+    }
+};
+
+namespace {
+
+FFTPlan make_plan(int N, DevBuf<double2> &tw) {
+    FFTPlan p;
+    p.N = N;
+    int m = N;
+    int nr = 0;
+    while (m % 4 == 0) {
+        p.radix[nr++] = 4;
+        m /= 4;
+    }
+    while (m % 2 == 0) {
+        p.radix[nr++] = 2;
+        m /= 2;
+    }
+    while (m % 3 == 0) {
+        p.radix[nr++] = 3;
+        m /= 3;
+    }
+    if (m != 1 || nr > 16)
+        throw std::runtime_error("FFT length " + std::to_string(N) + " is not of the form 2^a 3^b");
+    p.nrad = nr;
+    std::vector<double2> h(N);
+    for (int k = 0; k < N; ++k) {
+        // exp(-2 pi i k / N) with exact reduction of the angle
+        long long num = 2LL * k;  // angle = -pi * num / N
+        double a = M_PI * double(num % (2LL * N)) / double(N);
+        h[k] = make_double2(std::cos(a), -std::sin(a));
+    }
+    tw.upload(h);
+    p.tw = tw.p;
+    return p;
+}
+
+void fill_system_device(SysDev &d, int nstages) {
+    SystemTables &t = *d.t;
+    const size_t n = size_t(d.nsys) * t.total;
+    d.field.alloc(n);
+    d.aux.alloc(n);
+    HIP_OK(hipMemset(d.field.p, 0, n * sizeof(double2)));
+    HIP_OK(hipMemset(d.aux.p, 0, n * sizeof(double2)));
+    for (int i = 0; i < nstages; ++i) {
+        d.Khat.emplace_back(new DevBuf<double2>());
+        d.Kstd.emplace_back(new DevBuf<double2>());
+        d.Khat.back()->alloc(n);
+        d.Kstd.back()->alloc(n);
+    }
+    d.mband.upload(t.mass_band);
+    d.sband.upload(t.stif_band);
+    std::vector<int> a, b;
+    for (auto &k : t.mass_keys) {
+        a.push_back(k.first);
+        b.push_back(k.second);
+    }
+    d.mkx.upload(a);
+    d.mky.upload(b);
+    a.clear();
+    b.clear();
+    for (auto &k : t.stif_keys) {
+        a.push_back(k.first);
+        b.push_back(k.second);
+    }
+    d.skx.upload(a);
+    d.sky.upload(b);
+    d.pos_eqn.upload(t.pos_eqn);
+    d.pos_row.upload(t.pos_row);
+    d.eqn_begin.upload(t.nl_eqn_begin);
+    std::vector<int> q, kx, ky, w, order, woff;
+    std::vector<double> dsca;
+    for (auto &nt : t.nl_terms) {
+        q.push_back(nt.qvar);
+        kx.push_back(nt.kx);
+        ky.push_back(nt.ky);
+        w.push_back(nt.w);
+        order.push_back(nt.order);
+        woff.push_back(nt.woff);
+        dsca.push_back(nt.dsca);
+    }
+    // never upload empty arrays as null pointers that a kernel may index
+    if (q.empty()) {
+        q.push_back(0);
+        kx.push_back(0);
+        ky.push_back(0);
+        w.push_back(0);
+        order.push_back(0);
+        woff.push_back(0);
+        dsca.push_back(0.0);
+    }
+    d.t_qvar.upload(q);
+    d.t_kx.upload(kx);
+    d.t_ky.upload(ky);
+    d.t_w.upload(w);
+    d.t_order.upload(order);
+    d.t_woff.upload(woff);
+    d.t_dsca.upload(dsca);
+    std::vector<double> wv = t.nl_w;
+    if (wv.empty()) wv.push_back(0.0);
+    d.wv.upload(wv);
+    const int kv = t.nl + t.nu;
+    d.Lf.alloc(size_t(d.nsys) * t.total * std::max(t.nl, 1));
+    d.Uf.alloc(size_t(d.nsys) * t.total * (kv + 1));
+    d.piv.alloc(size_t(d.nsys) * t.total);
+    d.info.alloc(d.nsys);
+}
+
+void fill_zero(hipStream_t st, double2 *p, size_t n);
+
+void factorize(coral_amd_ctx &c, SysDev &d, double dt) {
+    SystemTables &t = *d.t;
+    if (t.mass_keys.size() > 8 || t.stif_keys.size() > 8)
+        throw std::runtime_error("more than 8 distinct (kx, ky) monomials in one operator");
+    const int kv = t.nl + t.nu, ldab = 2 * t.nl + t.nu + 1;
+    const size_t per = size_t(t.total) * ldab;
+    // bound the working band to ~1 GiB per launch
+    int chunk = int(std::max<size_t>(1, (size_t(1) << 30) / (per * sizeof(double2))));
+    chunk = std::min(chunk, d.nsys);
+    DevBuf<double2> work;
+    work.alloc(size_t(chunk) * per);
+    for (int s0 = 0; s0 < d.nsys; s0 += chunk) {
+        FactorArgs a{};
+        a.mband = d.mband.p;
+        a.sband = d.sband.p;
+        a.mkx = d.mkx.p;
+        a.mky = d.mky.p;
+        a.skx = d.skx.p;
+        a.sky = d.sky.p;
+        a.nmg = int(t.mass_keys.size());
+        a.nsg = int(t.stif_keys.size());
+        a.nl = t.nl;
+        a.nu = t.nu;
+        a.total = t.total;
+        a.sdt = -dt * c.P.a_arr[0];
+        a.s0 = s0;
+        a.ns = std::min(chunk, d.nsys - s0);
+        a.identity00 = d.kxky ? 1 : 0;
+        a.km = d.kxky ? c.km : KeptModes{1, 1, 2, 0.0, 0.0};
+        a.work = work.p;
+        a.Lf = d.Lf.p;
+        a.Uf = d.Uf.p;
+        a.piv = d.piv.p;
+        a.info = d.info.p;
+        c.timed("factor", [&] { launch_factor(c.stream, a); });
+    }
+    HIP_OK(hipStreamSynchronize(c.stream));
+    std::vector<int> info(d.nsys);
+    HIP_OK(hipMemcpy(info.data(), d.info.p, d.nsys * sizeof(int), hipMemcpyDeviceToHost));
+    for (int s = 0; s < d.nsys; ++s)
+        if (info[s] != 0)
+            throw std::runtime_error("singular evolution operator (zgbtrf info " + std::to_string(info[s]) +
+                                     ") at wavenumber index " + std::to_string(s));
+    (void)kv;
+}
+
+// rhs / K_std
+void band_apply(coral_amd_ctx &c, SysDev &d, bool mass, const double2 *x, double2 *y,
+                const std::vector<std::pair<const double2 *, double>> &vecs) {
+    SystemTables &t = *d.t;
+    BandApplyArgs a{};
+    a.band = mass ? d.mband.p : d.sband.p;
+    a.gkx = mass ? d.mkx.p : d.skx.p;
+    a.gky = mass ? d.mky.p : d.sky.p;
+    a.ngroups = int(mass ? t.mass_keys.size() : t.stif_keys.size());
+    a.nl = t.nl;
+    a.nu = t.nu;
+    a.total = t.total;
+    a.nsys = d.nsys;
+    a.x = x;
+    if (vecs.size() > 8) throw std::runtime_error("too many IMEX stages");
+    a.nvec = int(vecs.size());
+    for (size_t i = 0; i < vecs.size(); ++i) {
+        a.vec[i] = vecs[i].first;
+        a.alpha[i] = vecs[i].second;
+    }
+    a.zero00 = d.kxky ? 1 : 0;
+    a.km = d.kxky ? c.km : KeptModes{1, 1, 2, 0.0, 0.0};
+    a.y = y;
+    c.timed(mass ? "rhs" : "kstd", [&] { launch_band_apply(c.stream, a); });
+}
+
+void nl_assemble(coral_amd_ctx &c, SysDev &d, double2 *Khat) {
+    SystemTables &t = *d.t;
+    NLArgs a{};
+    a.Qhat = c.Qhat.p;
+    a.nsys = c.P.nsys;
+    a.NZ = c.P.NZ;
+    a.total = t.total;
+    a.mean = d.kxky ? 0 : 1;
+    a.pos_eqn = d.pos_eqn.p;
+    a.pos_row = d.pos_row.p;
+    a.eqn_begin = d.eqn_begin.p;
+    a.t_qvar = d.t_qvar.p;
+    a.t_kx = d.t_kx.p;
+    a.t_ky = d.t_ky.p;
+    a.t_w = d.t_w.p;
+    a.t_order = d.t_order.p;
+    a.t_woff = d.t_woff.p;
+    a.t_dsca = d.t_dsca.p;
+    a.wv = d.wv.p;
+    a.km = c.km;
+    a.Khat = Khat;
+    c.timed(d.kxky ? "nl_assemble" : "nl_mean", [&] { launch_nl(c.stream, a); });
+}
+
+// Chebyshev coefficients of all full variables from the given states, then the
+// backward transforms up to G.
+void backward_to_G(coral_amd_ctx &c, const std::vector<const double2 *> &kst, const std::vector<const double2 *> &zst) {
+    const Problem &P = c.P;
+    const size_t fsz = size_t(P.nsys) * P.NZ;
+    c.timed("extract", [&] {
+        for (int f = 0; f < P.nfull; ++f)
+            if (!c.full_has_kxky[f]) fill_zero(c.stream, c.spec.p + f * fsz, fsz);
+        std::vector<int> started(P.nfull, 0);
+        for (auto &e : c.ext) {
+            if (e->kind != 0) continue;
+            SysDev &d = *c.ks[e->system];
+            launch_extract(c.stream, kst[e->system], d.t->total, P.nsys, P.NZ, e->rowptr.p, e->col.p, e->val.p,
+                           e->kx, e->ky, c.km, c.spec.p + e->full * fsz, started[e->full]);
+            started[e->full] = 1;
+        }
+        for (auto &e : c.ext) {
+            if (e->kind != 1) continue;
+            launch_extract_mean(c.stream, zst[e->system], P.NZ, e->rowptr.p, e->col.p, e->val.p,
+                                c.spec.p + e->full * fsz);
+        }
+    });
+    c.timed("zbwd", [&] { launch_zbwd(c.stream, c.Tz.p, c.spec.p, c.Fz.p, P.NZAA, P.NZ, P.nsys, P.nfull); });
+    c.timed("ybwd", [&] { launch_ybwd(c.stream, c.py, c.Fz.p, c.G.p, P.nfull, P.NZAA, P.nkx, P.nky, P.NY); });
+}
+
+void nl_terms(coral_amd_ctx &c, const std::vector<const double2 *> &kst, const std::vector<const double2 *> &zst,
+              int stage) {
+    const Problem &P = c.P;
+    backward_to_G(c, kst, zst);
+    HIP_OK(hipMemsetAsync(c.cflmax.p, 0, 3 * sizeof(unsigned long long), c.stream));
+    XPassArgs xa{};
+    xa.G = c.G.p;
+    xa.H = c.H.p;
+    xa.nfull = P.nfull;
+    xa.nquad = P.nquad;
+    xa.NZAA = P.NZAA;
+    xa.NYAA = P.NYAA;
+    xa.nkx = P.nkx;
+    xa.qa = c.qa.p;
+    xa.qb = c.qb.p;
+    xa.zweight = c.zweight.p;
+    xa.cflmax = c.cflmax.p;
+    c.timed("xpass", [&] { launch_xpass(c.stream, c.px, xa); });
+    c.timed("yfwd", [&] { launch_yfwd(c.stream, c.py, c.H.p, c.Hy.p, P.nquad, P.NZAA, P.nkx, P.nky, P.NY); });
+    c.timed("zfwd", [&] { launch_zfwd(c.stream, c.Cz.p, c.Hy.p, c.Qhat.p, P.NZ, P.NZAA, P.nsys, P.nquad); });
+    for (int q = 0; q < P.nquad; ++q)
+        if (P.quad_filter[q]) launch_remove_zmean(c.stream, c.Qhat.p + size_t(q) * P.nsys * P.NZ, P.NZ, P.nsys);
+    for (auto &d : c.ks) nl_assemble(c, *d, d->Khat[stage]->p);
+    for (auto &d : c.zs) nl_assemble(c, *d, d->Khat[stage]->p);
+}
+
+void one_step(coral_amd_ctx &c) {
+    const Problem &P = c.P;
+    const int s = P.nstages;
+    const double dt = c.dt;
+    auto A = [&](int i, int j) { return P.a_arr[i * s + j]; };
+    auto Ah = [&](int i, int j) { return P.a_hat[i * (s + 1) + j]; };
+    for (int i = 0; i < s; ++i) {
+        std::vector<const double2 *> kst, zst;
+        for (auto &d : c.ks) kst.push_back(i == 0 ? d->field.p : d->aux.p);
+        for (auto &d : c.zs) zst.push_back(i == 0 ? d->field.p : d->aux.p);
+        nl_terms(c, kst, zst, i);
+        auto linear = [&](SysDev &d) {
+            std::vector<std::pair<const double2 *, double>> vecs;
+            for (int j = 0; j < i; ++j) {
+                vecs.push_back({d.Kstd[j]->p, dt * A(i, j)});
+                vecs.push_back({d.Khat[j]->p, dt * Ah(i + 1, j)});
+            }
+            vecs.push_back({d.Khat[i]->p, dt * Ah(i + 1, i)});
+            band_apply(c, d, true, d.field.p, d.aux.p, vecs);
+            const SystemTables &t = *d.t;
+            c.timed(d.kxky ? "solve" : "solve_mean", [&] {
+                launch_solve(c.stream, d.Lf.p, d.Uf.p, d.piv.p, t.total, t.nl, t.nl + t.nu, d.nsys, d.aux.p,
+                             d.aux.p);
+            });
+            band_apply(c, d, false, d.aux.p, d.Kstd[i]->p, {});
+        };
+        for (auto &d : c.ks) linear(*d);
+        for (auto &d : c.zs) linear(*d);
+    }
+    for (auto &d : c.ks) std::swap(d->field.p, d->aux.p);
+    for (auto &d : c.zs) std::swap(d->field.p, d->aux.p);
+}
+
+int fail(const std::exception &e) {
+    g_err = e.what();
+    return -1;
+}
+
+}  // namespace
+
+// ---- small helper kernels used from the host side -------------------------
+namespace {
+__global__ void k_fill_zero(double2 *p, size_t n) {
+    size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = make_double2(0.0, 0.0);
+}
+void fill_zero(hipStream_t st, double2 *p, size_t n) {
+    if (!n) return;
+    hipLaunchKernelGGL(k_fill_zero, dim3(unsigned((n + 255) / 256)), dim3(256), 0, st, p, n);
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+extern "C" {
+
+int coral_amd_last_error(char *buf, int32_t len) {
+    if (!buf || len <= 0) return -1;
+    std::strncpy(buf, g_err.c_str(), size_t(len) - 1);
+    buf[len - 1] = 0;
+    return 0;
+}
+
+int coral_amd_create(const coral_amd_problem *prob, int device, coral_amd_ctx **out) {
+    if (!prob || !out) {
+        g_err = "null argument";
+        return -1;
+    }
+    try {
+        std::unique_ptr<coral_amd_ctx> c(new coral_amd_ctx());
+        c->P = assemble(*prob);
+        Problem &P = c->P;
+        c->device = device;
+        HIP_OK(hipSetDevice(device));
+        HIP_OK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        c->km = KeptModes{P.nkx, P.nky, P.NY, P.kx_box, P.ky_box};
+        for (auto &t : P.kxky) {
+            c->ks.emplace_back(new SysDev());
+            SysDev &d = *c->ks.back();
+            d.t = &t;
+            d.kxky = true;
+            d.nsys = P.nsys;
+            fill_system_device(d, P.nstages);
+        }
+        for (auto &t : P.zero) {
+            c->zs.emplace_back(new SysDev());
+            SysDev &d = *c->zs.back();
+            d.t = &t;
+            d.kxky = false;
+            d.nsys = 1;
+            fill_system_device(d, P.nstages);
+        }
+        c->full_has_kxky.assign(P.nfull, 0);
+        for (auto &e : P.extract) {
+            c->ext.emplace_back(new ExtractDev());
+            ExtractDev &x = *c->ext.back();
+            x.full = e.full;
+            x.kind = e.kind;
+            x.system = e.system;
+            x.kx = e.kx;
+            x.ky = e.ky;
+            x.rowptr.upload(e.rowptr);
+            std::vector<int> col = e.col;
+            std::vector<double> val = e.val;
+            if (col.empty()) {
+                col.push_back(0);
+                val.push_back(0.0);
+            }
+            x.col.upload(col);
+            x.val.upload(val);
+            if (e.kind == 0) c->full_has_kxky[e.full] = 1;
+        }
+        const size_t nfull = std::max(P.nfull, 1), nquad = std::max(P.nquad, 1);
+        c->spec.alloc(nfull * P.nsys * P.NZ);
+        c->Fz.alloc(nfull * P.NZAA * P.nsys);
+        c->G.alloc(nfull * size_t(P.NZAA) * P.NYAA * P.nkx);
+        c->H.alloc(nquad * size_t(P.NZAA) * P.NYAA * P.nkx);
+        c->Hy.alloc(nquad * size_t(P.NZAA) * P.nsys);
+        c->Qhat.alloc(nquad * size_t(P.nsys) * P.NZ);
+        HIP_OK(hipMemset(c->Qhat.p, 0, c->Qhat.n * sizeof(double2)));
+        c->Tz.upload(P.Tz);
+        c->Cz.upload(P.Cz);
+        c->zweight.upload(P.zweight);
+        std::vector<int> qa = P.quad_a, qb = P.quad_b;
+        if (qa.empty()) {
+            qa.push_back(-1);
+            qb.push_back(-1);
+        }
+        c->qa.upload(qa);
+        c->qb.upload(qb);
+        c->cflmax.alloc(3);
+        HIP_OK(hipMemset(c->cflmax.p, 0, 3 * sizeof(unsigned long long)));
+        c->px = make_plan(P.NXAA, c->twx);
+        c->py = make_plan(P.NYAA, c->twy);
+        HIP_OK(hipDeviceSynchronize());
+        *out = c.release();
+        return 0;
+    } catch (const std::exception &e) {
+        return fail(e);
+    }
+}
+
+int coral_amd_destroy(coral_amd_ctx *ctx) {
+    try {
+        if (ctx) {
+            HIP_OK(hipSetDevice(ctx->device));
+            if (ctx->stream) HIP_OK(hipStreamSynchronize(ctx->stream));
+            delete ctx;
+        }
+        return 0;
+    } catch (const std::exception &e) {
+        return fail(e);
+    }
+}
+
+int coral_amd_set_dt(coral_amd_ctx *c, double dt) {
+    try {
+        if (!c) throw std::runtime_error("null context");
+        if (!(dt > 0.0) || !std::isfinite(dt)) throw std::runtime_error("dt must be positive and finite");
+        HIP_OK(hipSetDevice(c->device));
+        for (auto &d : c->ks) factorize(*c, *d, dt);
+        for (auto &d : c->zs) factorize(*c, *d, dt);
+        c->dt = dt;
+        c->factored = true;
+        return 0;
+    } catch (const std::exception &e) {
+        return fail(e);
+    }
+}
+
+int coral_amd_upload_state(coral_amd_ctx *c, int32_t kind, int32_t isys, const double *data) {
+    try {
+        if (!c || !data) throw std::runtime_error("null argument");
+        HIP_OK(hipSetDevice(c->device));
+        const Problem &P = c->P;
+        if (kind == 0) {
+            if (isys < 0 || isys >= int(c->ks.size())) throw std::runtime_error("bad kxky system index");
+            SysDev &d = *c->ks[isys];
+            const int total = d.t->total;
+            std::vector<double2> h(size_t(P.nsys) * total);
+            for (int ix = 0; ix < P.nkx; ++ix)
+                for (int jy = 0; jy < P.nky; ++jy) {
+                    const int s = ix * P.nky + jy;
+                    const size_t src = (size_t(ix) * P.NYAA + P.ky_slot[jy]) * total;
+                    for (int i = 0; i < total; ++i)
+                        h[size_t(s) * total + i] =
+                            (s == 0) ? make_double2(0.0, 0.0) : make_double2(data[2 * (src + i)], data[2 * (src + i) + 1]);
+                }
+            HIP_OK(hipMemcpy(d.field.p, h.data(), h.size() * sizeof(double2), hipMemcpyHostToDevice));
+        } else if (kind == 1) {
+            if (isys < 0 || isys >= int(c->zs.size())) throw std::runtime_error("bad mean system index");
+            SysDev &d = *c->zs[isys];
+            std::vector<double2> h(d.t->total);
+            for (int i = 0; i < d.t->total; ++i) h[i] = make_double2(data[i], 0.0);
+            HIP_OK(hipMemcpy(d.field.p, h.data(), h.size() * sizeof(double2), hipMemcpyHostToDevice));
+        } else {
+            throw std::runtime_error("kind must be 0 (kxky) or 1 (mean)");
+        }
+        return 0;
+    } catch (const std::exception &e) {
+        return fail(e);
+    }
+}
+
+int coral_amd_download_state(coral_amd_ctx *c, int32_t kind, int32_t isys, double *data) {
+    try {
+        if (!c || !data) throw std::runtime_error("null argument");
+        HIP_OK(hipSetDevice(c->device));
+        HIP_OK(hipStreamSynchronize(c->stream));
+        const Problem &P = c->P;
+        if (kind == 0) {
+            if (isys < 0 || isys >= int(c->ks.size())) throw std::runtime_error("bad kxky system index");
+            SysDev &d = *c->ks[isys];
+            const int total = d.t->total;
+            std::vector<double2> h(size_t(P.nsys) * total);
+            HIP_OK(hipMemcpy(h.data(), d.field.p, h.size() * sizeof(double2), hipMemcpyDeviceToHost));
+            std::memset(data, 0, sizeof(double) * 2 * size_t(P.nkx) * P.NYAA * total);
+            for (int ix = 0; ix < P.nkx; ++ix)
+                for (int jy = 0; jy < P.nky; ++jy) {
+                    const int s = ix * P.nky + jy;
+                    const size_t dst = (size_t(ix) * P.NYAA + P.ky_slot[jy]) * total;
+                    for (int i = 0; i < total; ++i) {
+                        data[2 * (dst + i)] = h[size_t(s) * total + i].x;
+                        data[2 * (dst + i) + 1] = h[size_t(s) * total + i].y;
+                    }
+                }
+        } else if (kind == 1) {
+            if (isys < 0 || isys >= int(c->zs.size())) throw std::runtime_error("bad mean system index");
+            SysDev &d = *c->zs[isys];
+            std::vector<double2> h(d.t->total);
+            HIP_OK(hipMemcpy(h.data(), d.field.p, h.size() * sizeof(double2), hipMemcpyDeviceToHost));
+            for (int i = 0; i < d.t->total; ++i) data[i] = h[i].x;
+        } else {
+            throw std::runtime_error("kind must be 0 (kxky) or 1 (mean)");
+        }
+        return 0;
+    } catch (const std::exception &e) {
+        return fail(e);
+    }
+}
+
+int coral_amd_step(coral_amd_ctx *c, int32_t nsteps) {
+    try {
+        if (!c) throw std::runtime_error("null context");
+        if (!c->factored) throw std::runtime_error("coral_amd_set_dt must be called before stepping");
+        if (nsteps < 0) throw std::runtime_error("nsteps must be >= 0");
+        HIP_OK(hipSetDevice(c->device));
+        for (int n = 0; n < nsteps; ++n) one_step(*c);
+        HIP_OK(hipGetLastError());
+        return 0;
+    } catch (const std::exception &e) {
+        return fail(e);
+    }
+}
+
+int coral_amd_synchronize(coral_amd_ctx *c) {
+    try {
+        if (!c) throw std::runtime_error("null context");
+        HIP_OK(hipSetDevice(c->device));
+        HIP_OK(hipStreamSynchronize(c->stream));
+        return 0;
+    } catch (const std::exception &e) {
+        return fail(e);
+    }
+}
+
+int coral_amd_get_cfl_dt(coral_amd_ctx *c, double *out) {
+    try {
+        if (!c || !out) throw std::runtime_error("null argument");
+        HIP_OK(hipSetDevice(c->device));
+        HIP_OK(hipStreamSynchronize(c->stream));
+        unsigned long long h[3];
+        HIP_OK(hipMemcpy(h, c->cflmax.p, sizeof(h), hipMemcpyDeviceToHost));
+        double m[3];
+        std::memcpy(m, h, sizeof(m));
+        const Problem &P = c->P;
+        double u = m[0] * P.NXAA / P.Lx, v = m[1] * P.NYAA / P.Ly;
+        double w = std::max(v, m[2]);
+        double mx = std::max(u, std::max(v, w));
+        *out = mx > 0 ? 1.0 / mx : INFINITY;
+        return 0;
+    } catch (const std::exception &e) {
+        return fail(e);
+    }
+}
+
+int coral_amd_wavenumbers(coral_amd_ctx *c, int32_t *kx_int, int32_t *ky_int, uint8_t *dealias_x,
+                          uint8_t *dealias_y) {
+    try {
+        if (!c) throw std::runtime_error("null context");
+        const Problem &P = c->P;
+        for (int ix = 0; ix < P.NX / 2; ++ix) {
+            int k = ix + 1;
+            int ki = (k >= P.NXAA / 2 + 1) ? -(P.NXAA + 1 - k) : k - 1;
+            if (kx_int) kx_int[ix] = ki;
+            if (dealias_x) dealias_x[ix] = (std::fabs(ki * P.kx_box) >= P.kx_box * P.NX / 2.0) ? 1 : 0;
+        }
+        for (int iy = 0; iy < P.NYAA; ++iy) {
+            int k = iy + 1;
+            int ki = (k >= P.NYAA / 2 + 1) ? -(P.NYAA + 1 - k) : k - 1;
+            if (ky_int) ky_int[iy] = ki;
+            if (dealias_y) dealias_y[iy] = (std::fabs(ki * P.ky_box) >= P.ky_box * P.NY / 2.0) ? 1 : 0;
+        }
+        return 0;
+    } catch (const std::exception &e) {
+        return fail(e);
+    }
+}
+
+int coral_amd_system_info(coral_amd_ctx *c, int32_t isys, int32_t *total, int32_t *nl, int32_t *nu,
+                          int32_t *nkept) {
+    try {
+        if (!c) throw std::runtime_error("null context");
+        if (isys < 0 || isys >= int(c->P.kxky.size())) throw std::runtime_error("bad kxky system index");
+        const SystemTables &t = c->P.kxky[isys];
+        if (total) *total = t.total;
+        if (nl) *nl = t.nl;
+        if (nu) *nu = t.nu;
+        if (nkept) *nkept = c->P.nsys;
+        return 0;
+    } catch (const std::exception &e) {
+        return fail(e);
+    }
+}
+
+int coral_amd_get_physical(coral_amd_ctx *c, int32_t ifull, double *out) {
+    g_err = "coral_amd_get_physical: not implemented yet";
+    (void)c;
+    (void)ifull;
+    (void)out;
+    return -2;
+}
+
+int coral_amd_bwd_transform(coral_amd_ctx *c, const double *spec, double *phys) {
+    g_err = "coral_amd_bwd_transform: not implemented yet";
+    (void)c;
+    (void)spec;
+    (void)phys;
+    return -2;
+}
+
+int coral_amd_fwd_transform(coral_amd_ctx *c, const double *phys, double *spec) {
+    g_err = "coral_amd_fwd_transform: not implemented yet";
+    (void)c;
+    (void)phys;
+    (void)spec;
+    return -2;
+}
+
+int coral_amd_banded_solve(int32_t device, int32_t n, int32_t kl, int32_t ku, int32_t nsys, const double *ab,
+                           double *x, int32_t *info) {
+    try {
+        if (!ab || !x || !info) throw std::runtime_error("null argument");
+        if (n < 1 || kl < 0 || ku < 0 || nsys < 1 || kl > 255) throw std::runtime_error("bad band dimensions");
+        HIP_OK(hipSetDevice(device));
+        hipStream_t st;
+        HIP_OK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        const int ldab = 2 * kl + ku + 1, kv = kl + ku;
+        DevBuf<double2> work, Lf, Uf, xb;
+        DevBuf<unsigned char> piv;
+        DevBuf<int> inf;
+        work.alloc(size_t(nsys) * n * ldab);
+        Lf.alloc(size_t(nsys) * n * std::max(kl, 1));
+        Uf.alloc(size_t(nsys) * n * (kv + 1));
+        xb.alloc(size_t(nsys) * n);
+        piv.alloc(size_t(nsys) * n);
+        inf.alloc(nsys);
+        HIP_OK(hipMemcpy(work.p, ab, work.n * sizeof(double2), hipMemcpyHostToDevice));
+        HIP_OK(hipMemcpy(xb.p, x, xb.n * sizeof(double2), hipMemcpyHostToDevice));
+        launch_factor_lapack(st, work.p, n, kl, ku, nsys, Lf.p, Uf.p, piv.p, inf.p);
+        launch_solve(st, Lf.p, Uf.p, piv.p, n, kl, kv, nsys, xb.p, xb.p);
+        HIP_OK(hipStreamSynchronize(st));
+        HIP_OK(hipMemcpy(x, xb.p, xb.n * sizeof(double2), hipMemcpyDeviceToHost));
+        HIP_OK(hipMemcpy(info, inf.p, nsys * sizeof(int), hipMemcpyDeviceToHost));
+        HIP_OK(hipStreamDestroy(st));
+        return 0;
+    } catch (const std::exception &e) {
+        return fail(e);
+    }
+}
+
+int coral_amd_timers_enable(coral_amd_ctx *c, int32_t enable) {
+    try {
+        if (!c) throw std::runtime_error("null context");
+        c->flush_timers();
+        c->totals.clear();
+        c->timing = enable != 0;
+        return 0;
+    } catch (const std::exception &e) {
+        return fail(e);
+    }
+}
+
+int coral_amd_timer_read(coral_amd_ctx *c, int32_t i, char *name, int32_t name_len, double *total_ms,
+                         int32_t *count) {
+    try {
+        if (!c) throw std::runtime_error("null context");
+        c->flush_timers();
+        int k = 0;
+        for (auto &kv : c->totals) {
+            if (k == i) {
+                if (name && name_len > 0) {
+                    std::strncpy(name, kv.first.c_str(), size_t(name_len) - 1);
+                    name[name_len - 1] = 0;
+                }
+                if (total_ms) *total_ms = kv.second.first;
+                if (count) *count = kv.second.second;
+                break;
+            }
+            ++k;
+        }
+        return int(c->totals.size());
+    } catch (const std::exception &e) {
+        return fail(e);
+    }
+}
+
+}  // extern "C"

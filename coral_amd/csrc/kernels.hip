@@ -1,0 +1,705 @@
+// Device kernels of the plane-layer IMEX step (CDNA4 / gfx950).
+//
+// Data layout in HBM (one GPU owns the whole domain):
+//   state   [sys][total]              complex, sys = ix*nky + jy over kept modes
+//   spec    [full][sys][NZ]           Chebyshev coefficients of full variables
+//   Fz      [full][NZAA][sys]         after the z transform (Gauss points)
+//   G       [full][NZAA][NYAA][nkx]   after the y transform
+//   H       [quad][NZAA][NYAA][nkx]   products after the x transform (kept kx)
+//   Hy      [quad][NZAA][sys]         after the y transform (kept ky)
+//   Qhat    [quad][sys][NZ]           Chebyshev coefficients of the products
+#include "kernels.hpp"
+
+#include <cstdio>
+
+namespace coral {
+
+#define CHECK_LAUNCH()                                                                    \
+    do {                                                                                  \
+        hipError_t e_ = hipGetLastError();                                                \
+        if (e_ != hipSuccess) {                                                           \
+            fprintf(stderr, "coral_amd: launch failed: %s\n", hipGetErrorString(e_));     \
+        }                                                                                 \
+    } while (0)
+
+static inline unsigned blocks_for(long long n, int threads) { return unsigned((n + threads - 1) / threads); }
+
+// ---------------------------------------------------------------------------
+// extraction (padStencilExtractShuffle . aux, PL_IMEX_timestepping.f90:1199-1209,
+// PL_IMEX_timestepping_marching.f90:430-470)
+// ---------------------------------------------------------------------------
+__global__ void k_extract(const double2 *__restrict__ state, int total, int nsys, int NZ,
+                          const int *__restrict__ rowptr, const int *__restrict__ col,
+                          const double *__restrict__ val, int kx, int ky, KeptModes km,
+                          double2 *__restrict__ spec, int accumulate) {
+    long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long long)nsys * NZ) return;
+    int s = int(idx / NZ), n = int(idx % NZ);
+    const double2 *st = state + (size_t)s * total;
+    double2 acc = make_double2(0.0, 0.0);
+    for (int k = rowptr[n]; k < rowptr[n + 1]; ++k) {
+        double v = val[k];
+        double2 a = st[col[k]];
+        acc.x += v * a.x;
+        acc.y += v * a.y;
+    }
+    if (kx || ky) acc = cmul(monomial(kx, ky, km.kx(s), km.ky(s)), acc);
+    if (accumulate) acc = cadd(spec[idx], acc);
+    spec[idx] = acc;
+}
+
+void launch_extract(hipStream_t st, const double2 *state, int total, int nsys, int NZ, const int *rowptr,
+                    const int *col, const double *val, int kx, int ky, KeptModes km, double2 *spec, int accumulate) {
+    hipLaunchKernelGGL(k_extract, dim3(blocks_for((long long)nsys * NZ, 256)), dim3(256), 0, st, state, total, nsys,
+                       NZ, rowptr, col, val, kx, ky, km, spec, accumulate);
+    CHECK_LAUNCH();
+}
+
+__global__ void k_extract_mean(const double2 *__restrict__ zstate, int NZ, const int *__restrict__ rowptr,
+                               const int *__restrict__ col, const double *__restrict__ val, double2 *spec0) {
+    for (int n = threadIdx.x; n < NZ; n += blockDim.x) {
+        double acc = 0.0;
+        for (int k = rowptr[n]; k < rowptr[n + 1]; ++k) acc += val[k] * zstate[col[k]].x;
+        spec0[n].x += acc;
+    }
+}
+
+void launch_extract_mean(hipStream_t st, const double2 *zstate, int NZ, const int *rowptr, const int *col,
+                         const double *val, double2 *spec0) {
+    hipLaunchKernelGGL(k_extract_mean, dim3(1), dim3(256), 0, st, zstate, NZ, rowptr, col, val, spec0);
+    CHECK_LAUNCH();
+}
+
+// ---------------------------------------------------------------------------
+// z transforms as dense FP64 products (A real, right operand complex)
+// MODE 0: B(k, c) = Bp[((c>>1)*K + k)*2 + (c&1)],  C(m, c) = Cp[m*Ncol + c]
+// MODE 1: B(k, c) = Bp[k*Ncol + c],                C(m, c) = Cp[((c>>1)*M + m)*2 + (c&1)]
+// ---------------------------------------------------------------------------
+template <int MODE>
+__global__ __launch_bounds__(256) void k_gemm(const double *__restrict__ A, int lda, const double *__restrict__ Bp,
+                                              double *__restrict__ Cp, int M, int Ncol, int K, size_t strideB,
+                                              size_t strideC) {
+    constexpr int BM = 64, BN = 64, BK = 16;
+    __shared__ double As[BK][BM + 1];
+    __shared__ double Bs[BK][BN + 1];
+    Bp += blockIdx.z * strideB;
+    Cp += blockIdx.z * strideC;
+    const int m0 = blockIdx.y * BM, c0 = blockIdx.x * BN;
+    const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
+    double acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = 0.0;
+    for (int k0 = 0; k0 < K; k0 += BK) {
+        for (int e = tid; e < BM * BK; e += 256) {
+            int mm = e / BK, kk = e % BK;
+            int gm = m0 + mm, gk = k0 + kk;
+            As[kk][mm] = (gm < M && gk < K) ? A[(size_t)gm * lda + gk] : 0.0;
+        }
+        if (MODE == 0) {
+            for (int e = tid; e < BN * BK; e += 256) {
+                int lo = e & 1, kk = (e >> 1) % BK, ch = (e >> 1) / BK;
+                int c = c0 + 2 * ch + lo, gk = k0 + kk;
+                Bs[kk][2 * ch + lo] = (c < Ncol && gk < K) ? Bp[((size_t)(c >> 1) * K + gk) * 2 + lo] : 0.0;
+            }
+        } else {
+            for (int e = tid; e < BN * BK; e += 256) {
+                int cc = e % BN, kk = e / BN;
+                int c = c0 + cc, gk = k0 + kk;
+                Bs[kk][cc] = (c < Ncol && gk < K) ? Bp[(size_t)gk * Ncol + c] : 0.0;
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < BK; ++kk) {
+            double a[4], b[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) a[i] = As[kk][ty * 4 + i];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) b[j] = Bs[kk][tx + 16 * j];
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) acc[i][j] = fma(a[i], b[j], acc[i][j]);
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        int m = m0 + ty * 4 + i;
+        if (m >= M) continue;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            int c = c0 + tx + 16 * j;
+            if (c >= Ncol) continue;
+            if (MODE == 0)
+                Cp[(size_t)m * Ncol + c] = acc[i][j];
+            else
+                Cp[((size_t)(c >> 1) * M + m) * 2 + (c & 1)] = acc[i][j];
+        }
+    }
+}
+
+void launch_zbwd(hipStream_t st, const double *Tz, const double2 *spec, double2 *Fz, int NZAA, int NZ, int nsys,
+                 int nbatch) {
+    int Ncol = 2 * nsys;
+    dim3 grid(blocks_for(Ncol, 64), blocks_for(NZAA, 64), nbatch);
+    hipLaunchKernelGGL(k_gemm<0>, grid, dim3(256), 0, st, Tz, NZ, (const double *)spec, (double *)Fz, NZAA, Ncol,
+                       NZ, (size_t)nsys * NZ * 2, (size_t)NZAA * nsys * 2);
+    CHECK_LAUNCH();
+}
+
+void launch_zfwd(hipStream_t st, const double *Cz, const double2 *Hy, double2 *Qhat, int NZ, int NZAA, int nsys,
+                 int nbatch) {
+    int Ncol = 2 * nsys;
+    dim3 grid(blocks_for(Ncol, 64), blocks_for(NZ, 64), nbatch);
+    hipLaunchKernelGGL(k_gemm<1>, grid, dim3(256), 0, st, Cz, NZAA, (const double *)Hy, (double *)Qhat, NZ, Ncol,
+                       NZAA, (size_t)NZAA * nsys * 2, (size_t)nsys * NZ * 2);
+    CHECK_LAUNCH();
+}
+
+// ---------------------------------------------------------------------------
+// Stockham FFT of `lines` lines held in LDS (line stride LS), radix 2/3/4.
+// Stage: in index j + r*N/R, out index (j/Ns)*Ns*R + j%Ns + r*Ns, twiddle
+// exp(sign*2*pi*i*(j%Ns)*r/(Ns*R)).  sign = -1 forward, +1 inverse.
+// ---------------------------------------------------------------------------
+struct FFTPlanDev {
+    int N, nrad;
+    int radix[16];
+    const double2 *tw;
+};
+
+template <int MAXB>
+__device__ void fft_lds(double2 *buf, int LS, int lines, const FFTPlanDev &p, double sign) {
+    const int N = p.N;
+    int Ns = 1;
+    for (int st = 0; st < p.nrad; ++st) {
+        const int R = p.radix[st];
+        const int nb = N / R;
+        const int total = nb * lines;
+        const int step = N / (Ns * R);
+        double2 v[MAXB][4];
+        int cnt = 0;
+        for (int b = threadIdx.x; b < total && cnt < MAXB; b += blockDim.x, ++cnt) {
+            int line = b / nb, j = b - line * nb;
+            double2 *L = buf + line * LS;
+            int jm = j % Ns;
+            for (int r = 0; r < R; ++r) {
+                double2 x = L[j + r * nb];
+                if (r > 0) {
+                    double2 w = p.tw[jm * r * step];  // exp(-2 pi i k / N)
+                    if (sign > 0) w.y = -w.y;         // inverse transform: conjugate twiddle
+                    x = cmul(x, w);
+                }
+                v[cnt][r] = x;
+            }
+            double2 *y = v[cnt];
+            if (R == 2) {
+                double2 a = y[0], c = y[1];
+                y[0] = cadd(a, c);
+                y[1] = csub(a, c);
+            } else if (R == 3) {
+                const double c1 = -0.5, s1 = sign * 0.86602540378443864676;
+                double2 a = y[0], b1 = y[1], b2 = y[2];
+                double2 t = cadd(b1, b2), d = csub(b1, b2);
+                y[0] = cadd(a, t);
+                double2 m = make_double2(a.x + c1 * t.x, a.y + c1 * t.y);
+                // i*s1*d = (-s1*d.y, s1*d.x)
+                y[1] = make_double2(m.x - s1 * d.y, m.y + s1 * d.x);
+                y[2] = make_double2(m.x + s1 * d.y, m.y - s1 * d.x);
+            } else {  // R == 4
+                double2 a = y[0], b = y[1], c = y[2], d = y[3];
+                double2 t0 = cadd(a, c), t1 = csub(a, c), t2 = cadd(b, d), t3 = csub(b, d);
+                // w = sign*i ; w*t3 = (-sign*t3.y, sign*t3.x)
+                double2 wt3 = make_double2(-sign * t3.y, sign * t3.x);
+                y[0] = cadd(t0, t2);
+                y[2] = csub(t0, t2);
+                y[1] = cadd(t1, wt3);
+                y[3] = csub(t1, wt3);
+            }
+        }
+        __syncthreads();
+        cnt = 0;
+        for (int b = threadIdx.x; b < total && cnt < MAXB; b += blockDim.x, ++cnt) {
+            int line = b / nb, j = b - line * nb;
+            double2 *L = buf + line * LS;
+            int base = (j / Ns) * Ns * R + (j % Ns);
+            for (int r = 0; r < R; ++r) L[base + r * Ns] = v[cnt][r];
+        }
+        __syncthreads();
+        Ns *= R;
+    }
+}
+
+static FFTPlanDev to_dev(const FFTPlan &p) {
+    FFTPlanDev d;
+    d.N = p.N;
+    d.nrad = p.nrad;
+    for (int i = 0; i < 16; ++i) d.radix[i] = p.radix[i];
+    d.tw = p.tw;
+    return d;
+}
+
+// lines per workgroup of the y passes: lines*N <= 1536 keeps the butterflies of
+// every stage (lines*N/2 at most) within fft_lds<6> capacity (6 per thread x 256)
+static int lines_for(int N) {
+    int lines = 1;
+    while (lines * 2 * N <= 1536) lines *= 2;
+    return lines;
+}
+
+// y inverse FFT: Fz[b][z][ix*nky + jy] -> G[b][z][y][ix]
+__global__ __launch_bounds__(256) void k_ybwd(FFTPlanDev p, const double2 *__restrict__ Fz, double2 *__restrict__ G,
+                                              int nkx, int nky, int NY, int lines) {
+    extern __shared__ double2 sbuf[];
+    const int N = p.N, LS = N + 1;
+    const int ntile = (nkx + lines - 1) / lines;
+    const int tile = blockIdx.x % ntile;
+    const size_t bz = blockIdx.x / ntile;
+    const int ix0 = tile * lines, nl = min(lines, nkx - ix0);
+    for (int e = threadIdx.x; e < lines * LS; e += blockDim.x) sbuf[e] = make_double2(0.0, 0.0);
+    __syncthreads();
+    const double2 *src = Fz + bz * nkx * nky + (size_t)ix0 * nky;
+    for (int e = threadIdx.x; e < nl * nky; e += blockDim.x) {
+        int l = e / nky, jy = e - l * nky;
+        int slot = jy < NY / 2 ? jy : N - (NY - 1) + jy;
+        sbuf[l * LS + slot] = src[e];
+    }
+    __syncthreads();
+    fft_lds<6>(sbuf, LS, lines, p, +1.0);
+    double2 *dst = G + bz * N * nkx;
+    for (int e = threadIdx.x; e < N * nl; e += blockDim.x) {
+        int y = e / nl, l = e - y * nl;
+        dst[(size_t)y * nkx + ix0 + l] = sbuf[l * LS + y];
+    }
+}
+
+void launch_ybwd(hipStream_t st, const FFTPlan &py, const double2 *Fz, double2 *G, int nbatch, int NZAA, int nkx,
+                 int nky, int NY) {
+    int lines = lines_for(py.N);
+    int ntile = (nkx + lines - 1) / lines;
+    size_t shmem = size_t(lines) * (py.N + 1) * sizeof(double2);
+    hipLaunchKernelGGL(k_ybwd, dim3(ntile * nbatch * NZAA), dim3(256), shmem, st, to_dev(py), Fz, G, nkx, nky, NY,
+                       lines);
+    CHECK_LAUNCH();
+}
+
+// y forward FFT: H[b][z][y][ix] -> Hy[b][z][ix*nky + jy]
+__global__ __launch_bounds__(256) void k_yfwd(FFTPlanDev p, const double2 *__restrict__ H, double2 *__restrict__ Hy,
+                                              int nkx, int nky, int NY, int lines) {
+    extern __shared__ double2 sbuf[];
+    const int N = p.N, LS = N + 1;
+    const int ntile = (nkx + lines - 1) / lines;
+    const int tile = blockIdx.x % ntile;
+    const size_t bz = blockIdx.x / ntile;
+    const int ix0 = tile * lines, nl = min(lines, nkx - ix0);
+    const double2 *src = H + bz * N * nkx;
+    for (int e = threadIdx.x; e < N * lines; e += blockDim.x) {
+        int y = e / lines, l = e - y * lines;
+        sbuf[l * LS + y] = (l < nl) ? src[(size_t)y * nkx + ix0 + l] : make_double2(0.0, 0.0);
+    }
+    __syncthreads();
+    fft_lds<6>(sbuf, LS, lines, p, -1.0);
+    double2 *dst = Hy + bz * nkx * nky + (size_t)ix0 * nky;
+    for (int e = threadIdx.x; e < nl * nky; e += blockDim.x) {
+        int l = e / nky, jy = e - l * nky;
+        int slot = jy < NY / 2 ? jy : N - (NY - 1) + jy;
+        dst[e] = sbuf[l * LS + slot];
+    }
+}
+
+void launch_yfwd(hipStream_t st, const FFTPlan &py, const double2 *H, double2 *Hy, int nbatch, int NZAA, int nkx,
+                 int nky, int NY) {
+    int lines = lines_for(py.N);
+    int ntile = (nkx + lines - 1) / lines;
+    size_t shmem = size_t(lines) * (py.N + 1) * sizeof(double2);
+    hipLaunchKernelGGL(k_yfwd, dim3(ntile * nbatch * NZAA), dim3(256), shmem, st, to_dev(py), H, Hy, nkx, nky, NY,
+                       lines);
+    CHECK_LAUNCH();
+}
+
+// x pass: per (z, y) line: c2r of all full variables, products, r2c (kept kx)
+__device__ inline void atomic_max_nonneg(unsigned long long *addr, double v) {
+    atomicMax(addr, (unsigned long long)__double_as_longlong(v));
+}
+
+__global__ __launch_bounds__(256) void k_xpass(FFTPlanDev p, XPassArgs a, int LX) {
+    extern __shared__ double2 sbuf[];
+    const int N = p.N, LS = N + 1;
+    double2 *fb = sbuf;
+    double *phys = (double *)(sbuf + LX * LS);
+    const long long nlines = (long long)a.NZAA * a.NYAA;
+    const long long l0 = (long long)blockIdx.x * LX;
+    const int nl = (int)min((long long)LX, nlines - l0);
+    const int nkx = a.nkx;
+    for (int f = 0; f < a.nfull; ++f) {
+        for (int e = threadIdx.x; e < LX * LS; e += blockDim.x) fb[e] = make_double2(0.0, 0.0);
+        __syncthreads();
+        const double2 *g = a.G + ((size_t)f * nlines + l0) * nkx;
+        for (int e = threadIdx.x; e < nl * nkx; e += blockDim.x) {
+            int l = e / nkx, k = e - l * nkx;
+            double2 v = g[e];
+            if (k == 0) {
+                fb[l * LS] = make_double2(v.x, 0.0);
+            } else {
+                fb[l * LS + k] = v;
+                fb[l * LS + N - k] = cconj(v);
+            }
+        }
+        __syncthreads();
+        fft_lds<6>(fb, LS, LX, p, +1.0);
+        for (int e = threadIdx.x; e < LX * N; e += blockDim.x) {
+            int l = e / N, x = e - l * N;
+            phys[(f * LX + l) * N + x] = fb[l * LS + x].x;
+        }
+        __syncthreads();
+    }
+    // CFL maxima (compute_cfl_based_timestep, marching.f90:200-254)
+    if (a.cflmax && a.nfull >= 3) {
+        double mu = 0.0, mv = 0.0, mw = 0.0;
+        for (int e = threadIdx.x; e < nl * N; e += blockDim.x) {
+            int l = e / N, x = e - l * N;
+            int z = int((l0 + l) / a.NYAA);
+            mu = fmax(mu, fabs(phys[(0 * LX + l) * N + x]));
+            mv = fmax(mv, fabs(phys[(1 * LX + l) * N + x]));
+            mw = fmax(mw, fabs(phys[(2 * LX + l) * N + x] / a.zweight[z]));
+        }
+        for (int off = 32; off > 0; off >>= 1) {
+            mu = fmax(mu, __shfl_xor(mu, off));
+            mv = fmax(mv, __shfl_xor(mv, off));
+            mw = fmax(mw, __shfl_xor(mw, off));
+        }
+        if ((threadIdx.x & 63) == 0) {
+            atomic_max_nonneg(&a.cflmax[0], mu);
+            atomic_max_nonneg(&a.cflmax[1], mv);
+            atomic_max_nonneg(&a.cflmax[2], mw);
+        }
+    }
+    for (int q = 0; q < a.nquad; ++q) {
+        const int fa = a.qa[q], fbi = a.qb[q];
+        for (int e = threadIdx.x; e < LX * N; e += blockDim.x) {
+            int l = e / N, x = e - l * N;
+            double va = fa < 0 ? 1.0 : phys[(fa * LX + l) * N + x];
+            double vb = fbi < 0 ? 1.0 : phys[(fbi * LX + l) * N + x];
+            fb[l * LS + x] = make_double2(va * vb, 0.0);
+        }
+        __syncthreads();
+        fft_lds<6>(fb, LS, LX, p, -1.0);
+        double2 *h = a.H + ((size_t)q * nlines + l0) * nkx;
+        for (int e = threadIdx.x; e < nl * nkx; e += blockDim.x) {
+            int l = e / nkx, k = e - l * nkx;
+            h[e] = fb[l * LS + k];
+        }
+        __syncthreads();
+    }
+}
+
+static size_t xpass_lds(int LX, int N, int nfull) {
+    return size_t(LX) * (N + 1) * sizeof(double2) + size_t(nfull) * LX * N * sizeof(double);
+}
+
+void launch_xpass(hipStream_t st, const FFTPlan &px, const XPassArgs &a) {
+    // lines per workgroup: FFT butterflies must fit fft_lds<6> (LX*N/2 <= 6*256)
+    // and the workgroup's LDS must stay within 64 KiB
+    int LX = 4;
+    while (LX > 1 && (LX * px.N > 3072 || xpass_lds(LX, px.N, a.nfull) > 65536)) LX /= 2;
+    if (px.N > 3072 || xpass_lds(LX, px.N, a.nfull) > 65536) {
+        fprintf(stderr, "coral_amd: x-pass does not fit (N=%d, nfull=%d)\n", px.N, a.nfull);
+        return;
+    }
+    long long nlines = (long long)a.NZAA * a.NYAA;
+    size_t shmem = xpass_lds(LX, px.N, a.nfull);
+    hipLaunchKernelGGL(k_xpass, dim3(blocks_for(nlines, LX)), dim3(256), shmem, st, to_dev(px), a, LX);
+    CHECK_LAUNCH();
+}
+
+// remove_vertical_mean_of_quadratic_var (PL_IMEX_timestepping.f90:223-235)
+__global__ void k_remove_zmean(double2 *Qhat, int NZ, int nsys) {
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nsys) return;
+    double2 *q = Qhat + (size_t)s * NZ;
+    double2 acc = make_double2(0.0, 0.0);
+    for (int k = 1; k <= NZ / 2 - 1; ++k) {
+        double w = 0.5 / (2.0 * k + 1.0) - 0.5 / (2.0 * k - 1.0);
+        acc.x -= q[2 * k].x * w;
+        acc.y -= q[2 * k].y * w;
+    }
+    q[0] = acc;
+}
+
+void launch_remove_zmean(hipStream_t st, double2 *Qhat, int NZ, int nsys) {
+    hipLaunchKernelGGL(k_remove_zmean, dim3(blocks_for(nsys, 256)), dim3(256), 0, st, Qhat, NZ, nsys);
+    CHECK_LAUNCH();
+}
+
+// ---------------------------------------------------------------------------
+// NL assembly (PL_IMEX_timestepping_marching.f90:306-367 kxky, :372-397 mean)
+// ---------------------------------------------------------------------------
+__global__ void k_nl(NLArgs a, int nsys_out, int nsysQ) {
+    long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long long)nsys_out * a.total) return;
+    const int s = int(idx / a.total), pos = int(idx % a.total);
+    const int e = a.pos_eqn[pos], r = a.pos_row[pos];
+    const int ssrc = a.mean ? 0 : s;
+    double2 acc = make_double2(0.0, 0.0);
+    for (int t = a.eqn_begin[e]; t < a.eqn_begin[e + 1]; ++t) {
+        const int q = a.t_qvar[t], w = a.t_w[t], m0 = r + a.t_order[t];
+        const double2 *Q = a.Qhat + ((size_t)q * nsysQ + ssrc) * a.NZ;
+        const double *W = a.wv + a.t_woff[t] + r * (2 * w + 1);
+        double2 sum = make_double2(0.0, 0.0);
+        for (int d = -w; d <= w; ++d) {
+            int m = m0 + d;
+            if (m < 0 || m >= a.NZ) continue;
+            double wv = W[d + w];
+            double2 qv = Q[m];
+            sum.x += wv * qv.x;
+            sum.y += wv * qv.y;
+        }
+        if (a.mean) {
+            acc.x += a.t_dsca[t] * sum.x;
+        } else {
+            double2 c = monomial(a.t_kx[t], a.t_ky[t], a.km.kx(s), a.km.ky(s));
+            acc = cadd(acc, cscale(cmul(c, sum), a.t_dsca[t]));
+        }
+    }
+    if (!a.mean && s == 0) acc = make_double2(0.0, 0.0);
+    a.Khat[idx] = acc;
+}
+
+void launch_nl(hipStream_t st, const NLArgs &a) {
+    int nout = a.mean ? 1 : a.nsys;
+    hipLaunchKernelGGL(k_nl, dim3(blocks_for((long long)nout * a.total, 256)), dim3(256), 0, st, a, nout, a.nsys);
+    CHECK_LAUNCH();
+}
+
+// ---------------------------------------------------------------------------
+// banded operator application + AXPYs (mass_field_to_rhs, add_K_*_to_rhs,
+// stif_aux_to_K_std: PL_IMEX_timestepping_marching.f90:93-171)
+// ---------------------------------------------------------------------------
+__global__ void k_band_apply(BandApplyArgs a) {
+    long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long long)a.nsys * a.total) return;
+    const int s = int(idx / a.total), i = int(idx % a.total);
+    const int w = a.nl + a.nu + 1;
+    const double2 *xs = a.x + (size_t)s * a.total;
+    const int dlo = max(-a.nl, -i), dhi = min(a.nu, a.total - 1 - i);
+    double2 y = make_double2(0.0, 0.0);
+    for (int g = 0; g < a.ngroups; ++g) {
+        const double *row = a.band + ((size_t)g * a.total + i) * w + a.nl;
+        double2 sum = make_double2(0.0, 0.0);
+        for (int d = dlo; d <= dhi; ++d) {
+            double b = row[d];
+            if (b != 0.0) {
+                double2 xv = xs[i + d];
+                sum.x += b * xv.x;
+                sum.y += b * xv.y;
+            }
+        }
+        if (a.gkx[g] || a.gky[g])
+            sum = cmul(monomial(a.gkx[g], a.gky[g], a.km.kx(s), a.km.ky(s)), sum);
+        y = cadd(y, sum);
+    }
+    for (int v = 0; v < a.nvec; ++v) {
+        double2 t = a.vec[v][idx];
+        y.x += a.alpha[v] * t.x;
+        y.y += a.alpha[v] * t.y;
+    }
+    if (a.zero00 && s == 0) y = make_double2(0.0, 0.0);
+    a.y[idx] = y;
+}
+
+void launch_band_apply(hipStream_t st, const BandApplyArgs &a) {
+    hipLaunchKernelGGL(k_band_apply, dim3(blocks_for((long long)a.nsys * a.total, 256)), dim3(256), 0, st, a);
+    CHECK_LAUNCH();
+}
+
+// ---------------------------------------------------------------------------
+// banded LU, one workgroup (one wavefront) per wavenumber: restates LAPACK
+// xGBTF2 as called per (kx,ky) in PL_algebra_z_3D.f90:245-276.  The working
+// band lives in `work` (LAPACK layout), the result is split into column
+// records Lf (multipliers) and Uf (inverse pivot + column of U above it).
+// ---------------------------------------------------------------------------
+__device__ inline double cabs1(double2 z) { return fabs(z.x) + fabs(z.y); }
+
+__device__ void gbtf2_wave(double2 *AB, int n, int kl, int ku, int ldab, unsigned char *piv, int *info) {
+    const int kv = kl + ku;
+    const int tid = threadIdx.x;
+    int ju = 0;
+    for (int j = 0; j < n; ++j) {
+        const int km = min(kl, n - 1 - j);
+        double2 *colj = AB + (size_t)j * ldab;
+        double v = (tid <= km) ? cabs1(colj[kv + tid]) : -1.0;
+        int ix = tid;
+        for (int off = 32; off > 0; off >>= 1) {
+            double ov = __shfl_xor(v, off);
+            int oi = __shfl_xor(ix, off);
+            if (ov > v || (ov == v && oi < ix)) {
+                v = ov;
+                ix = oi;
+            }
+        }
+        const int jp = ix;
+        if (tid == 0) piv[j] = (unsigned char)jp;
+        double2 pv = colj[kv + jp];
+        if (pv.x == 0.0 && pv.y == 0.0) {
+            if (tid == 0 && *info == 0) *info = j + 1;
+            __syncthreads();
+            continue;
+        }
+        ju = max(ju, min(j + ku + jp, n - 1));
+        if (jp != 0) {
+            for (int c = j + tid; c <= ju; c += blockDim.x) {
+                double2 *cc = AB + (size_t)c * ldab;
+                double2 t = cc[kv + jp + j - c];
+                cc[kv + jp + j - c] = cc[kv + j - c];
+                cc[kv + j - c] = t;
+            }
+        }
+        __syncthreads();
+        if (km > 0) {
+            double2 rp = crecip(colj[kv]);
+            if (tid >= 1 && tid <= km) colj[kv + tid] = cmul(colj[kv + tid], rp);
+            __syncthreads();
+            const int ncols = ju - j;
+            for (int e = tid; e < ncols * km; e += blockDim.x) {
+                int c = j + 1 + e / km, i = 1 + e % km;
+                double2 *cc = AB + (size_t)c * ldab;
+                double2 u = cc[kv + j - c];
+                double2 l = colj[kv + i];
+                double2 t = cc[kv + j + i - c];
+                cc[kv + j + i - c] = csub(t, cmul(l, u));
+            }
+        }
+        __syncthreads();
+    }
+}
+
+__device__ void split_factors(const double2 *AB, int n, int kl, int ku, int ldab, double2 *Lf, double2 *Uf) {
+    const int kv = kl + ku;
+    for (int e = threadIdx.x; e < n * kl; e += blockDim.x) {
+        int j = e / kl, k = e % kl;
+        Lf[e] = (k < min(kl, n - 1 - j)) ? AB[(size_t)j * ldab + kv + 1 + k] : make_double2(0.0, 0.0);
+    }
+    for (int e = threadIdx.x; e < n * (kv + 1); e += blockDim.x) {
+        int j = e / (kv + 1), k = e % (kv + 1);
+        double2 v;
+        if (k == 0)
+            v = crecip(AB[(size_t)j * ldab + kv]);
+        else
+            v = (j - k >= 0) ? AB[(size_t)j * ldab + kv - k] : make_double2(0.0, 0.0);
+        Uf[e] = v;
+    }
+}
+
+__global__ __launch_bounds__(64) void k_factor(FactorArgs a) {
+    const int sl = blockIdx.x, s = a.s0 + sl;
+    const int n = a.total, kl = a.nl, ku = a.nu, kv = kl + ku, ldab = 2 * kl + ku + 1, w = a.nl + a.nu + 1;
+    double2 *AB = a.work + (size_t)sl * n * ldab;
+    const bool ident = a.identity00 && s == 0;
+    double2 cm[8], cs[8];
+    for (int g = 0; g < a.nmg && g < 8; ++g) cm[g] = monomial(a.mkx[g], a.mky[g], a.km.kx(s), a.km.ky(s));
+    for (int g = 0; g < a.nsg && g < 8; ++g)
+        cs[g] = cscale(monomial(a.skx[g], a.sky[g], a.km.kx(s), a.km.ky(s)), a.sdt);
+    for (int e = threadIdx.x; e < n * ldab; e += blockDim.x) {
+        int j = e / ldab, rr = e - j * ldab;
+        int i = rr - kv + j;
+        double2 val = make_double2(0.0, 0.0);
+        if (rr >= kl && i >= 0 && i < n) {
+            int d = j - i;
+            if (ident) {
+                val.x = (i == j) ? 1.0 : 0.0;
+            } else {
+                for (int g = 0; g < a.nmg; ++g) {
+                    double b = a.mband[((size_t)g * n + i) * w + d + a.nl];
+                    val.x += b * cm[g].x;
+                    val.y += b * cm[g].y;
+                }
+                for (int g = 0; g < a.nsg; ++g) {
+                    double b = a.sband[((size_t)g * n + i) * w + d + a.nl];
+                    val.x += b * cs[g].x;
+                    val.y += b * cs[g].y;
+                }
+            }
+        }
+        AB[e] = val;
+    }
+    __syncthreads();
+    int *info = a.info + s;
+    if (threadIdx.x == 0) *info = 0;
+    __syncthreads();
+    gbtf2_wave(AB, n, kl, ku, ldab, a.piv + (size_t)s * n, info);
+    __syncthreads();
+    split_factors(AB, n, kl, ku, ldab, a.Lf + (size_t)s * n * kl, a.Uf + (size_t)s * n * (kv + 1));
+}
+
+void launch_factor(hipStream_t st, const FactorArgs &a) {
+    hipLaunchKernelGGL(k_factor, dim3(a.ns), dim3(64), 0, st, a);
+    CHECK_LAUNCH();
+}
+
+__global__ __launch_bounds__(64) void k_factor_lapack(double2 *work, int n, int kl, int ku, double2 *Lf, double2 *Uf,
+                                                      unsigned char *piv, int *info) {
+    const int s = blockIdx.x, kv = kl + ku, ldab = 2 * kl + ku + 1;
+    double2 *AB = work + (size_t)s * n * ldab;
+    if (threadIdx.x == 0) info[s] = 0;
+    __syncthreads();
+    gbtf2_wave(AB, n, kl, ku, ldab, piv + (size_t)s * n, info + s);
+    __syncthreads();
+    split_factors(AB, n, kl, ku, ldab, Lf + (size_t)s * n * kl, Uf + (size_t)s * n * (kv + 1));
+}
+
+void launch_factor_lapack(hipStream_t st, double2 *work, int n, int kl, int ku, int nsys, double2 *Lf, double2 *Uf,
+                          unsigned char *piv, int *info) {
+    hipLaunchKernelGGL(k_factor_lapack, dim3(nsys), dim3(64), 0, st, work, n, kl, ku, Lf, Uf, piv, info);
+    CHECK_LAUNCH();
+}
+
+// ---------------------------------------------------------------------------
+// banded solve (xGBTRS 'N', PL_algebra_z_3D.f90:189-243), one wavefront per
+// wavenumber, right-hand side staged in LDS
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(64) void k_solve(const double2 *__restrict__ Lf, const double2 *__restrict__ Uf,
+                                              const unsigned char *__restrict__ piv, int n, int kl, int kv,
+                                              const double2 *__restrict__ rhs, double2 *__restrict__ x) {
+    extern __shared__ double2 b[];
+    const int s = blockIdx.x, tid = threadIdx.x;
+    const double2 *L = Lf + (size_t)s * n * kl;
+    const double2 *U = Uf + (size_t)s * n * (kv + 1);
+    const unsigned char *P = piv + (size_t)s * n;
+    for (int i = tid; i < n; i += blockDim.x) b[i] = rhs[(size_t)s * n + i];
+    __syncthreads();
+    for (int j = 0; j < n - 1; ++j) {
+        const int p = P[j];
+        if (p != 0) {
+            if (tid == 0) {
+                double2 t = b[j];
+                b[j] = b[j + p];
+                b[j + p] = t;
+            }
+            __syncthreads();
+        }
+        const double2 bj = b[j];
+        const int lm = min(kl, n - 1 - j);
+        for (int k = tid; k < lm; k += blockDim.x) b[j + 1 + k] = csub(b[j + 1 + k], cmul(L[(size_t)j * kl + k], bj));
+        __syncthreads();
+    }
+    for (int j = n - 1; j >= 0; --j) {
+        if (tid == 0) b[j] = cmul(b[j], U[(size_t)j * (kv + 1)]);
+        __syncthreads();
+        const double2 xj = b[j];
+        const int km = min(kv, j);
+        for (int k = tid; k < km; k += blockDim.x)
+            b[j - 1 - k] = csub(b[j - 1 - k], cmul(U[(size_t)j * (kv + 1) + 1 + k], xj));
+        __syncthreads();
+    }
+    for (int i = tid; i < n; i += blockDim.x) x[(size_t)s * n + i] = b[i];
+}
+
+void launch_solve(hipStream_t st, const double2 *Lf, const double2 *Uf, const unsigned char *piv, int n, int kl,
+                  int kv, int nsys, const double2 *rhs, double2 *x) {
+    hipLaunchKernelGGL(k_solve, dim3(nsys), dim3(64), size_t(n) * sizeof(double2), st, Lf, Uf, piv, n, kl, kv, rhs, x);
+    CHECK_LAUNCH();
+}
+
+}  // namespace coral

@@ -1,0 +1,100 @@
+// Kernel launch wrappers (device code in kernels.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "common.hpp"
+
+namespace coral {
+
+// ---- extraction: spec[s][n] (+)= mono(s) * sum val[k] * state[s][col[k]] ----
+void launch_extract(hipStream_t st, const double2 *state, int total, int nsys, int NZ, const int *rowptr,
+                    const int *col, const double *val, int kx, int ky, KeptModes km, double2 *spec, int accumulate);
+// mean contributions into the (0,0) slot: spec0[n] += sum val[k] * Re(zstate[col[k]])
+void launch_extract_mean(hipStream_t st, const double2 *zstate, int NZ, const int *rowptr, const int *col,
+                         const double *val, double2 *spec0);
+
+// ---- dense real GEMMs with complex right operands (z transforms) ----
+// Fz[b][z][s] = sum_n Tz[z][n] spec[b][s][n]            (batch b over full variables)
+void launch_zbwd(hipStream_t st, const double *Tz, const double2 *spec, double2 *Fz, int NZAA, int NZ, int nsys,
+                 int nbatch);
+// Qhat[b][s][n] = sum_z Cz[n][z] Hy[b][z][s]             (batch b over quadratic variables)
+void launch_zfwd(hipStream_t st, const double *Cz, const double2 *Hy, double2 *Qhat, int NZ, int NZAA, int nsys,
+                 int nbatch);
+
+// ---- FFT passes ----
+struct FFTPlan {
+    int N = 0, nrad = 0;
+    int radix[16];
+    const double2 *tw = nullptr;  // device, N entries exp(-2 pi i k / N)
+};
+// y inverse: G[b][z][y][ix] = sum_jy Fz[b][z][ix*nky+jy] e^{+i ky y}
+void launch_ybwd(hipStream_t st, const FFTPlan &py, const double2 *Fz, double2 *G, int nbatch, int NZAA,
+                 int nkx, int nky, int NY);
+// x pass: c2r of every full variable, products, CFL maxima, r2c of every quadratic variable
+struct XPassArgs {
+    const double2 *G;    // [nfull][NZAA][NYAA][nkx]
+    double2 *H;          // [nquad][NZAA][NYAA][nkx]
+    int nfull, nquad, NZAA, NYAA, nkx;
+    const int *qa, *qb;  // quadratic factors (-1 = ONE)
+    const double *zweight;
+    unsigned long long *cflmax;  // [3] bit patterns of non-negative doubles
+};
+void launch_xpass(hipStream_t st, const FFTPlan &px, const XPassArgs &a);
+// y forward: Hy[b][z][ix*nky+jy] = sum_y H[b][z][y][ix] e^{-i ky y}
+void launch_yfwd(hipStream_t st, const FFTPlan &py, const double2 *H, double2 *Hy, int nbatch, int NZAA,
+                 int nkx, int nky, int NY);
+// remove the z-integral of Qhat[s][:] (PL_IMEX_timestepping.f90:223-235)
+void launch_remove_zmean(hipStream_t st, double2 *Qhat, int NZ, int nsys);
+
+// ---- NL assembly into K_hat ----
+struct NLArgs {
+    const double2 *Qhat;      // [nquad][nsys][NZ]
+    int nsys, NZ, total, mean;
+    const int *pos_eqn, *pos_row, *eqn_begin;
+    const int *t_qvar, *t_kx, *t_ky, *t_w, *t_order, *t_woff;
+    const double *t_dsca, *wv;
+    KeptModes km;
+    double2 *Khat;            // [nsys][total]
+};
+void launch_nl(hipStream_t st, const NLArgs &a);
+
+// ---- banded operator application with AXPYs ----
+// y[s][i] = sum_g c_g(s) (B_g x[s])[i] + sum_j alpha_j V_j[s][i]; kxky (0,0): mass -> 0, stif -> 0
+struct BandApplyArgs {
+    const double *band;       // [ngroups][total][w]
+    const int *gkx, *gky;     // exponents per group
+    int ngroups, nl, nu, total, nsys;
+    const double2 *x;
+    int nvec;
+    const double2 *vec[8];
+    double alpha[8];
+    int zero00;               // 1: force y = 0 at s = 0
+    KeptModes km;
+    double2 *y;
+};
+void launch_band_apply(hipStream_t st, const BandApplyArgs &a);
+
+// ---- banded LU factorization and solve ----
+struct FactorArgs {
+    const double *mband, *sband;
+    const int *mkx, *mky, *skx, *sky;
+    int nmg, nsg, nl, nu, total;
+    double sdt;               // -dt * a11
+    int s0, ns;               // systems [s0, s0+ns)
+    int identity00;           // kxky (0,0) -> identity
+    KeptModes km;
+    double2 *work;            // [ns][total][ldab]
+    double2 *Lf;              // [nsys][total][kl]
+    double2 *Uf;              // [nsys][total][kv+1]  (Uf[.][j][0] = 1/U(j,j), Uf[.][j][1+k] = U(j-1-k, j))
+    unsigned char *piv;       // [nsys][total]   pivot offsets
+    int *info;                // [nsys]
+};
+void launch_factor(hipStream_t st, const FactorArgs &a);
+// generic batched factorization of LAPACK band input (for the unit seam)
+void launch_factor_lapack(hipStream_t st, double2 *work, int n, int kl, int ku, int nsys, double2 *Lf, double2 *Uf,
+                          unsigned char *piv, int *info);
+void launch_solve(hipStream_t st, const double2 *Lf, const double2 *Uf, const unsigned char *piv, int n, int kl,
+                  int kv, int nsys, const double2 *rhs, double2 *x);
+
+}  // namespace coral

@@ -1,0 +1,341 @@
+"""Reader for Coral's ``coral.equations`` text interface.
+
+This is the host-side half of the drop-in boundary: it turns the equation text
+into the recipe tables that the C-ABI (``coral_amd_create``) consumes.  The
+grammar, the meaning of each token and the bookkeeping follow Coral's parser:
+
+* line layout ``>>keyword :: payload <<`` and the keyword state machine:
+  ``src/plane_layer/PL_equations.f90:1758-2043`` (``determine_kind_of_information``);
+* atom splitting on ``.`` with the operator tokens ``dx``, ``dy``, ``Iz`` (``dz``
+  inside full-variable definitions) and ``mz``:
+  ``src/plane_layer/PL_string_to_data.f90:53-118``;
+* a signed term's scalar is its sign times the product of the named
+  parameters: ``PL_equations.f90:1510-1535`` (stiffness), ``:1089-1154`` (NL);
+* variables of a coupled set are numbered in the order of
+  ``linearly_coupled_var`` lines and equations in the order of ``new_equation``
+  lines (``PL_equations.f90:706-811``);
+* a quadratic variable is the product of two full variables, ``ONE`` standing
+  for 1 (``PL_equations.f90:890-954``); ``zFiltered_quadratic_variable`` sets
+  ``remove_z_integral`` (``:918-925``).
+
+Features of the grammar that the time-stepper does not implement (sources,
+penalisation, boundary-value extraction, barotropic/baroclinic filters) raise
+``NotImplementedError`` naming the keyword, rather than being silently skipped.
+"""
+from __future__ import annotations
+
+import dataclasses
+import re
+from typing import Dict, List, Optional, Tuple
+
+
+class EquationsError(ValueError):
+    """Malformed or inconsistent ``coral.equations`` input (Coral: ``stop``)."""
+
+
+@dataclasses.dataclass
+class LinearPiece:
+    """One piece ``dsca * (dx)^kx (dy)^ky Iz^iz mz^mz`` acting on variable ``var``
+    inside equation ``eqn`` (both 0-based).  Mirrors ``linear_operator_recipe_T``
+    (``PL_equations.f90:59-68``)."""
+    dsca: float
+    kx: int
+    ky: int
+    iz: int
+    mz: int
+    var: int
+    eqn: int
+
+
+@dataclasses.dataclass
+class NLTerm:
+    """``dsca * (dx)^kx (dy)^ky Iz^iz mz^mz Q[qvar]`` added to equation ``eqn``.
+    Mirrors ``atom_NL_term_T`` (``PL_equations.f90:22-30``)."""
+    dsca: float
+    kx: int
+    ky: int
+    iz: int
+    mz: int
+    qvar: int
+    eqn: int
+
+
+@dataclasses.dataclass
+class CoupledSystem:
+    """A set of linearly coupled variables with one equation per variable."""
+    var_names: List[str]
+    bc_codes: List[int]
+    eqn_orders: List[int]
+    mass: List[LinearPiece]
+    stif: List[LinearPiece]
+    nl: List[NLTerm]
+
+    @property
+    def nvars(self) -> int:
+        return len(self.var_names)
+
+
+@dataclasses.dataclass
+class FullTerm:
+    """``dsca * (dx)^kx (dy)^ky (dz)^dz mz^mz var`` in a full-variable build.
+    ``kind`` is ``'kxky'`` or ``'zero'``; ``system``/``position`` locate the
+    variable inside its coupled set (0-based).  Mirrors
+    ``a_linear_contribution_T`` (``PL_equations.f90:71-79``)."""
+    dsca: float
+    kx: int
+    ky: int
+    dz: int
+    mz: int
+    kind: str
+    system: int
+    position: int
+
+
+@dataclasses.dataclass
+class FullVariable:
+    name: str
+    terms: List[FullTerm]
+
+
+@dataclasses.dataclass
+class QuadraticVariable:
+    """``Q = F1 * F2``; a factor index of ``-1`` stands for the constant ``ONE``."""
+    name: str
+    ivar1: int
+    ivar2: int
+    remove_z_integral: bool = False
+
+
+@dataclasses.dataclass
+class Recipe:
+    """Parsed equations: mirrors ``full_problem_recipe_T``
+    (``PL_equations.f90:143-219``), restricted to what the time step uses."""
+    params: Dict[str, float]
+    kxky_vars: List[str]
+    zero_vars: List[str]
+    full_vars: List[FullVariable]
+    quad_vars: List[QuadraticVariable]
+    kxky_systems: List[CoupledSystem]
+    zero_systems: List[CoupledSystem]
+
+    def var_location(self, name: str) -> Tuple[str, int, int]:
+        for kind, systems in (("kxky", self.kxky_systems), ("zero", self.zero_systems)):
+            for isys, s in enumerate(systems):
+                if name in s.var_names:
+                    return kind, isys, s.var_names.index(name)
+        raise EquationsError(f"variable {name!r} belongs to no coupled set")
+
+
+_LINE = re.compile(r"^>>(.*)<<\s*$")
+
+
+def _fortran_float(text: str) -> float:
+    """Fortran real literal (``1.d0``, ``2.81D5``, ``707.1067811``) to float."""
+    t = text.strip().replace("d", "e").replace("D", "e")
+    try:
+        return float(t)
+    except ValueError as exc:
+        raise EquationsError(f"cannot read a number from {text!r}") from exc
+
+
+def _split_signed_atoms(expr: str) -> List[Tuple[float, str]]:
+    """``"+ a.b - c"`` -> [(+1, 'a.b'), (-1, 'c')].  Coral's
+    ``break_linear_operators_in_pieces`` (``PL_string_to_data.f90:330-466``)
+    splits on the signs that separate atoms; a leading atom without a sign is
+    taken as positive."""
+    expr = expr.strip()
+    out: List[Tuple[float, str]] = []
+    sign = 1.0
+    cur = ""
+    for ch in expr:
+        if ch in "+-":
+            if cur.strip():
+                out.append((sign, cur.strip()))
+            cur = ""
+            sign = 1.0 if ch == "+" else -1.0
+        else:
+            cur += ch
+    if cur.strip():
+        out.append((sign, cur.strip()))
+    if not out:
+        raise EquationsError(f"empty linear expression {expr!r}")
+    return out
+
+
+def _atom_tokens(atom: str, dz_token: str):
+    """Split one atom on '.' and count operator tokens (PL_string_to_data.f90:94-110)."""
+    kx = ky = iz = mz = 0
+    names: List[str] = []
+    for tok in atom.split("."):
+        tok = tok.strip()
+        if tok == "dx":
+            kx += 1
+        elif tok == "dy":
+            ky += 1
+        elif tok == dz_token:
+            iz += 1
+        elif tok == "mz":
+            mz += 1
+        elif tok:
+            names.append(tok)
+    if len(names) > 5:
+        raise EquationsError(f"too many unrecognized tokens in {atom!r}")
+    return kx, ky, iz, mz, names
+
+
+_UNSUPPORTED = (
+    "sourceParam", "enable_penalisation", "penalisation_strength", "penalisation_width",
+    "linear_variable_bottom_boundary_value", "linear_variable_top_boundary_value",
+    "vertical_decomposition", "add_source",
+)
+
+
+def parse_equations(text: str) -> Recipe:
+    """Parse the text of a ``coral.equations`` file into a :class:`Recipe`."""
+    params: Dict[str, float] = {}
+    kxky_vars: List[str] = []
+    zero_vars: List[str] = []
+    full_vars: List[FullVariable] = []
+    quad_specs: List[Tuple[str, str, str, bool]] = []
+    kxky_systems: List[CoupledSystem] = []
+    zero_systems: List[CoupledSystem] = []
+    current: Optional[CoupledSystem] = None
+    current_kind = ""
+    full_build_pending: List[Tuple[int, str]] = []  # (full var index, expression)
+    saw_eof = False
+
+    for raw in text.splitlines():
+        line = raw.rstrip("\n")
+        m = _LINE.match(line.strip())
+        if not m:
+            continue
+        body = m.group(1)
+        if body.strip().startswith("====") or body.strip() == "":
+            continue
+        if body.strip() == "EOF":
+            saw_eof = True
+            break
+        if "::" in body:
+            key, payload = body.split("::", 1)
+        else:
+            key, payload = body, ""
+        key = key.strip()
+        payload = payload.strip()
+        if any(key.startswith(u) for u in _UNSUPPORTED):
+            raise NotImplementedError(f"coral.equations keyword '{key}' is not supported by coral_amd")
+
+        if key == "Param":
+            name, val = payload.split(":=", 1)
+            params[name.strip()] = _fortran_float(val)
+        elif key == "linear_variable_kxky":
+            kxky_vars.append(payload)
+        elif key == "linear_variable_mean":
+            zero_vars.append(payload)
+        elif key == "linear_variable_full":
+            full_vars.append(FullVariable(payload, []))
+        elif key == "linear_variable_full_build":
+            if not full_vars:
+                raise EquationsError("linear_variable_full_build before linear_variable_full")
+            full_build_pending.append((len(full_vars) - 1, payload))
+        elif key in ("quadratic_variable", "zFiltered_quadratic_variable"):
+            qv, rhs = payload.split(":=", 1)
+            parts = rhs.strip().split(".")
+            if len(parts) != 2:
+                raise EquationsError(f"bad quadratic variable definition {payload!r}")
+            quad_specs.append((qv.strip(), parts[0].strip(), parts[1].strip(),
+                               key.startswith("zFiltered")))
+        elif key in ("add_set_of_coupled_kxky_equations", "add_set_of_coupled_zero_equations"):
+            current = CoupledSystem([], [], [], [], [], [])
+            current_kind = "kxky" if "kxky" in key else "zero"
+            (kxky_systems if current_kind == "kxky" else zero_systems).append(current)
+        elif key == "linearly_coupled_var":
+            if current is None:
+                raise EquationsError("linearly_coupled_var outside a coupled set")
+            pool = kxky_vars if current_kind == "kxky" else zero_vars
+            if payload not in pool:
+                raise EquationsError(f"unknown {current_kind} variable {payload!r}")
+            current.var_names.append(payload)
+            current.bc_codes.append(0)
+        elif key == "add_BC_for_this_var":
+            current.bc_codes[-1] = int(payload)
+        elif key == "new_equation":
+            current.eqn_orders.append(int(payload))
+        elif key in ("add_d/dt_term", "add_rhs_linear", "add_rhs_NL"):
+            if current is None or not current.eqn_orders:
+                raise EquationsError(f"{key} outside an equation")
+            eqn = len(current.eqn_orders) - 1
+            for sgn, atom in _split_signed_atoms(payload):
+                kx, ky, iz, mz, names = _atom_tokens(atom, "Iz")
+                if current_kind == "zero" and (kx or ky):
+                    raise EquationsError(f"horizontal derivative in a mean-field equation: {atom!r}")
+                dsca = sgn
+                target = None
+                for nm in names:
+                    if key == "add_rhs_NL":
+                        hit = [i for i, q in enumerate(quad_specs) if q[0] == nm]
+                    else:
+                        hit = [i for i, v in enumerate(current.var_names) if v == nm]
+                    if hit:
+                        target = hit[0]
+                    elif nm in params:
+                        dsca = params[nm] * dsca
+                    else:
+                        raise EquationsError(f"unknown string in equations: {nm!r}")
+                if target is None:
+                    raise EquationsError(f"term {atom!r} names no variable")
+                if key == "add_rhs_NL":
+                    current.nl.append(NLTerm(dsca, kx, ky, iz, mz, target, eqn))
+                else:
+                    piece = LinearPiece(dsca, kx, ky, iz, mz, target, eqn)
+                    (current.mass if key == "add_d/dt_term" else current.stif).append(piece)
+        else:
+            raise EquationsError(f"unknown coral.equations keyword {key!r}")
+
+    if not saw_eof:
+        raise EquationsError("missing >>EOF<< line")
+
+    recipe = Recipe(params, kxky_vars, zero_vars, full_vars, [], kxky_systems, zero_systems)
+
+    # full variables are resolved once all coupled sets are known
+    for ifull, expr in full_build_pending:
+        for sgn, atom in _split_signed_atoms(expr):
+            kx, ky, dz, mz, names = _atom_tokens(atom, "dz")
+            dsca = sgn
+            loc = None
+            for nm in names:
+                if nm in zero_vars or nm in kxky_vars:
+                    loc = recipe.var_location(nm)
+                elif nm in params:
+                    dsca = dsca * params[nm]
+                else:
+                    raise EquationsError(f"unknown string in equations: {nm!r}")
+            if loc is None:
+                raise EquationsError(f"invalid full variable definition {atom!r}")
+            kind, isys, pos = loc
+            full_vars[ifull].terms.append(FullTerm(dsca, kx, ky, dz, mz, kind, isys, pos))
+
+    names = [f.name for f in full_vars]
+    for qv, a, b, filt in quad_specs:
+        i1 = -1 if a == "ONE" else (names.index(a) if a in names else None)
+        i2 = -1 if b == "ONE" else (names.index(b) if b in names else None)
+        if i1 is None or i2 is None:
+            raise EquationsError("bad NL var definition (wrong reference to linear var.)")
+        if i1 == -1 and i2 == -1:
+            raise EquationsError("a quadratic variable cannot be ONE.ONE")
+        recipe.quad_vars.append(QuadraticVariable(qv, i1, i2, filt))
+
+    for kind, systems in (("kxky", kxky_systems), ("zero", zero_systems)):
+        for isys, s in enumerate(systems):
+            if len(s.eqn_orders) != s.nvars:
+                raise EquationsError(
+                    f"{kind} system #{isys + 1}: {len(s.eqn_orders)} equations for {s.nvars} variables")
+            for p in s.mass + s.stif:
+                if p.iz > s.eqn_orders[p.eqn]:
+                    raise EquationsError(
+                        f"{kind} system #{isys + 1}: Iz^{p.iz} exceeds the order of equation {p.eqn + 1}")
+    return recipe
+
+
+def read_equations(path: str) -> Recipe:
+    with open(path, "r") as f:
+        return parse_equations(f.read())

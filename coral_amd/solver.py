@@ -1,0 +1,218 @@
+"""Host-side mirror of Coral's solver object on top of the C ABI.
+
+``Solver`` plays the role of ``full_problem_data_structure_T``
+(``src/plane_layer/PL_IMEX_timestepping.f90:143-219``):
+
+==========================  =============================================
+Coral                        coral_amd
+==========================  =============================================
+``main%init(scheme_id)``      ``Solver(recipe, params, scheme)``
+``factorize_operators(dt)``   ``Solver.factorize_operators(dt)``
+``march_forward(dt)``         ``Solver.march_forward(dt, nsteps=1)``
+``coupled_kxky_set(i)%field`` ``Solver.get_field / set_field``
+==========================  =============================================
+
+States use Coral's layout: a kxky system's field is a complex128 array of C
+shape ``(NX/2, NYAA, total)`` (Fortran ``(total, NYAA, NX/2)``), a mean
+system's field a float64 array ``(total,)``.  Every compute call runs on the
+GPU through ``libcoral_amd.so``.
+"""
+from __future__ import annotations
+
+import ctypes
+from typing import Dict, List, Optional, Tuple
+
+import numpy as np
+
+from . import _lib
+from ._lib import check, lib
+from .equations import Recipe
+from .schemes import tableau
+
+
+def _arr(ctype, values):
+    values = list(values)
+    return (ctype * max(1, len(values)))(*values)
+
+
+class Solver:
+    def __init__(self, recipe: Recipe, NX: int, NY: int, NZ: int, Lx: float, Ly: float,
+                 gap: float = 1.0, center: float = 0.5, scheme: str = "ARS_222", device: int = 0):
+        self.recipe = recipe
+        self.NX, self.NY, self.NZ = NX, NY, NZ
+        self.NXAA, self.NYAA, self.NZAA = 3 * NX // 2, 3 * NY // 2, 3 * NZ // 2
+        self.Lx, self.Ly, self.gap, self.center = Lx, Ly, gap, center
+        self.scheme = scheme
+        A, Ah = tableau(scheme)
+        self.A, self.Ah = A, Ah
+        keep = []  # keep ctypes objects alive during create
+
+        def system(s):
+            bc = _arr(ctypes.c_int32, s.bc_codes)
+            eo = _arr(ctypes.c_int32, s.eqn_orders)
+            mass = _arr(_lib.Piece, [_lib.Piece(p.dsca, p.kx, p.ky, p.iz, p.mz, p.var, p.eqn) for p in s.mass])
+            stif = _arr(_lib.Piece, [_lib.Piece(p.dsca, p.kx, p.ky, p.iz, p.mz, p.var, p.eqn) for p in s.stif])
+            nl = _arr(_lib.NLTerm, [_lib.NLTerm(t.dsca, t.kx, t.ky, t.iz, t.mz, t.qvar, t.eqn) for t in s.nl])
+            keep.extend([bc, eo, mass, stif, nl])
+            return _lib.System(s.nvars, bc, eo, len(s.mass), mass, len(s.stif), stif, len(s.nl), nl)
+
+        ks = _arr(_lib.System, [system(s) for s in recipe.kxky_systems])
+        zs = _arr(_lib.System, [system(s) for s in recipe.zero_systems])
+        fvs = []
+        for f in recipe.full_vars:
+            terms = _arr(_lib.FullTerm, [_lib.FullTerm(t.dsca, t.kx, t.ky, t.dz, t.mz, 0 if t.kind == "kxky" else 1,
+                                                       t.system, t.position) for t in f.terms])
+            keep.append(terms)
+            fvs.append(_lib.FullVar(len(f.terms), terms))
+        fv = _arr(_lib.FullVar, fvs)
+        qv = _arr(_lib.QuadVar, [_lib.QuadVar(q.ivar1, q.ivar2, int(q.remove_z_integral)) for q in recipe.quad_vars])
+        a_arr = _arr(ctypes.c_double, A.ravel())
+        a_hat = _arr(ctypes.c_double, Ah.ravel())
+        keep.extend([ks, zs, fv, qv, a_arr, a_hat])
+        prob = _lib.Problem(NX, NY, NZ, Lx, Ly, gap, center, A.shape[0], a_arr, a_hat,
+                            len(recipe.kxky_systems), ks, len(recipe.zero_systems), zs,
+                            len(recipe.full_vars), fv, len(recipe.quad_vars), qv)
+        ctx = ctypes.c_void_p()
+        check(lib.coral_amd_create(ctypes.byref(prob), device, ctypes.byref(ctx)), "coral_amd_create")
+        self._ctx = ctx
+        self.dt: Optional[float] = None
+        self.totals = [self.system_info(i)[0] for i in range(len(recipe.kxky_systems))]
+        self.zero_totals = [sum(NZ - bc // 10 for bc in s.bc_codes) for s in recipe.zero_systems]
+
+    # --- lifetime -------------------------------------------------------------
+    def close(self):
+        if getattr(self, "_ctx", None) is not None and self._ctx.value:
+            check(lib.coral_amd_destroy(self._ctx), "coral_amd_destroy")
+            self._ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @property
+    def handle(self):
+        return self._ctx
+
+    # --- Coral's verbs ---------------------------------------------------------
+    def factorize_operators(self, dt: float):
+        check(lib.coral_amd_set_dt(self._ctx, float(dt)), "coral_amd_set_dt")
+        self.dt = float(dt)
+
+    def march_forward(self, dt: float, nsteps: int = 1):
+        if self.dt is None or dt != self.dt:
+            self.factorize_operators(dt)
+        check(lib.coral_amd_step(self._ctx, int(nsteps)), "coral_amd_step")
+
+    def synchronize(self):
+        check(lib.coral_amd_synchronize(self._ctx), "coral_amd_synchronize")
+
+    # --- state -----------------------------------------------------------------
+    def set_field(self, kind: str, isys: int, data: np.ndarray):
+        if kind == "kxky":
+            a = np.ascontiguousarray(data, np.complex128)
+            exp = (self.NX // 2, self.NYAA, self.totals[isys])
+            if a.shape != exp:
+                raise ValueError(f"kxky field must have shape {exp}, got {a.shape}")
+            check(lib.coral_amd_upload_state(self._ctx, 0, isys, a.ctypes.data), "coral_amd_upload_state")
+        else:
+            a = np.ascontiguousarray(data, np.float64)
+            if a.shape != (self.zero_totals[isys],):
+                raise ValueError(f"mean field must have shape ({self.zero_totals[isys]},)")
+            check(lib.coral_amd_upload_state(self._ctx, 1, isys, a.ctypes.data), "coral_amd_upload_state")
+
+    def get_field(self, kind: str, isys: int) -> np.ndarray:
+        if kind == "kxky":
+            a = np.empty((self.NX // 2, self.NYAA, self.totals[isys]), np.complex128)
+            check(lib.coral_amd_download_state(self._ctx, 0, isys, a.ctypes.data), "coral_amd_download_state")
+        else:
+            a = np.empty(self.zero_totals[isys], np.float64)
+            check(lib.coral_amd_download_state(self._ctx, 1, isys, a.ctypes.data), "coral_amd_download_state")
+        return a
+
+    def set_state(self, kstates: List[np.ndarray], zstates: List[np.ndarray]):
+        for i, a in enumerate(kstates):
+            self.set_field("kxky", i, a)
+        for i, a in enumerate(zstates):
+            self.set_field("zero", i, a)
+
+    def get_state(self) -> Tuple[List[np.ndarray], List[np.ndarray]]:
+        return ([self.get_field("kxky", i) for i in range(len(self.totals))],
+                [self.get_field("zero", i) for i in range(len(self.zero_totals))])
+
+    # --- diagnostics -------------------------------------------------------------
+    def cfl_dt(self) -> float:
+        out = ctypes.c_double()
+        check(lib.coral_amd_get_cfl_dt(self._ctx, ctypes.byref(out)), "coral_amd_get_cfl_dt")
+        return out.value
+
+    def physical(self, ifull: int) -> np.ndarray:
+        out = np.empty((self.NXAA, self.NYAA, self.NZAA), np.float64)
+        check(lib.coral_amd_get_physical(self._ctx, ifull, out.ctypes.data), "coral_amd_get_physical")
+        return out
+
+    def wavenumbers(self):
+        kx = np.empty(self.NX // 2, np.int32)
+        ky = np.empty(self.NYAA, np.int32)
+        dx = np.empty(self.NX // 2, np.uint8)
+        dy = np.empty(self.NYAA, np.uint8)
+        check(lib.coral_amd_wavenumbers(self._ctx, kx.ctypes.data, ky.ctypes.data, dx.ctypes.data, dy.ctypes.data),
+              "coral_amd_wavenumbers")
+        return kx, ky, dx.astype(bool), dy.astype(bool)
+
+    def system_info(self, isys: int):
+        v = [ctypes.c_int32() for _ in range(4)]
+        check(lib.coral_amd_system_info(self._ctx, isys, *[ctypes.byref(x) for x in v]), "coral_amd_system_info")
+        return tuple(x.value for x in v)  # total, nl, nu, nkept
+
+    def bwd_transform(self, spec: np.ndarray) -> np.ndarray:
+        s = np.ascontiguousarray(spec, np.complex128)
+        if s.shape != (self.NX // 2, self.NYAA, self.NZ):
+            raise ValueError("spectral field must have shape (NX/2, NYAA, NZ)")
+        out = np.empty((self.NXAA, self.NYAA, self.NZAA), np.float64)
+        check(lib.coral_amd_bwd_transform(self._ctx, s.ctypes.data, out.ctypes.data), "coral_amd_bwd_transform")
+        return out
+
+    def fwd_transform(self, phys: np.ndarray) -> np.ndarray:
+        p = np.ascontiguousarray(phys, np.float64)
+        if p.shape != (self.NXAA, self.NYAA, self.NZAA):
+            raise ValueError("physical field must have shape (NXAA, NYAA, NZAA)")
+        out = np.empty((self.NX // 2, self.NYAA, self.NZ), np.complex128)
+        check(lib.coral_amd_fwd_transform(self._ctx, p.ctypes.data, out.ctypes.data), "coral_amd_fwd_transform")
+        return out
+
+    # --- timing ---------------------------------------------------------------------
+    def timers_enable(self, on: bool = True):
+        check(lib.coral_amd_timers_enable(self._ctx, 1 if on else 0), "coral_amd_timers_enable")
+
+    def timers(self) -> Dict[str, Tuple[float, int]]:
+        name = ctypes.create_string_buffer(64)
+        ms = ctypes.c_double()
+        cnt = ctypes.c_int32()
+        n = check(lib.coral_amd_timer_read(self._ctx, 0, name, 64, ctypes.byref(ms), ctypes.byref(cnt)))
+        out = {}
+        for i in range(n):
+            check(lib.coral_amd_timer_read(self._ctx, i, name, 64, ctypes.byref(ms), ctypes.byref(cnt)))
+            out[name.value.decode()] = (ms.value, cnt.value)
+        return out
+
+
+def banded_solve(ab: np.ndarray, kl: int, ku: int, b: np.ndarray, device: int = 0):
+    """Batched LAPACK-band LU + solve on the GPU.  ``ab`` complex (nsys, n, 2kl+ku+1)
+    (column j of system s contiguous), ``b`` complex (nsys, n).  Returns (x, info)."""
+    ab = np.ascontiguousarray(ab, np.complex128)
+    x = np.ascontiguousarray(b, np.complex128).copy()
+    nsys, n, ldab = ab.shape
+    if ldab != 2 * kl + ku + 1:
+        raise ValueError("ab must have 2kl+ku+1 rows per column")
+    info = np.zeros(nsys, np.int32)
+    check(lib.coral_amd_banded_solve(device, n, kl, ku, nsys, ab.ctypes.data, x.ctypes.data, info.ctypes.data),
+          "coral_amd_banded_solve")
+    return x, info

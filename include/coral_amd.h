@@ -1,0 +1,167 @@
+/*
+ * coral_amd -- MI355X-native plane-layer IMEX time-stepper, C ABI.
+ *
+ * The ABI replaces the seams a Coral host crosses on its hot path:
+ *   - the step seam  full_problem_data_structure_T%march_forward(dt) and
+ *     %factorize_operators(dt)  (src/plane_layer/PL_IMEX_timestepping.f90:191,196,
+ *     called from src/plane_layer/plane_layer_main.f90:147,157);
+ *   - the transform seam PS_fields_T%spec_to_phys / %phys_to_spec
+ *     (src/slabs.fftw3mpi/transforms.f90:10-25, :155-301);
+ *   - the algebra seam zOperator_3d_1coupled_T%factorize / %backsolve
+ *     (src/plane_layer/PL_algebra_z_3D.f90:12-36, :189-276).
+ * The problem is handed over as the parsed recipe (full_problem_recipe_T,
+ * src/plane_layer/PL_equations.f90:59-219) plus the geometry of
+ * coral.parameters.in (src/plane_layer/PL_geometry.f90:269-305); the library
+ * assembles every operator itself.
+ *
+ * Conventions
+ *   - every function returns 0 on success and a negative code on failure; the
+ *     message is available from coral_amd_last_error (thread-local);
+ *   - all host buffers are caller owned; device memory and the HIP stream are
+ *     owned by the context; calls enqueue on the context's stream and return
+ *     once host-visible results (downloads) are complete;
+ *   - indices of systems, variables and equations are 0-based;
+ *   - spectral state of a kxky system uses Coral's layout: complex double
+ *     (re, im interleaved) array of Fortran shape (total, NYAA, NX/2), i.e. C
+ *     shape [NX/2][NYAA][total]; mean (zero) systems are real double [total].
+ */
+#ifndef CORAL_AMD_H
+#define CORAL_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct coral_amd_ctx coral_amd_ctx;
+
+/* One linear piece dsca * (dx)^kx (dy)^ky Iz^iz mz^mz var, in equation eqn
+ * (linear_operator_recipe_T, PL_equations.f90:59-68). */
+typedef struct {
+    double dsca;
+    int32_t kx, ky, iz, mz, var, eqn;
+} coral_amd_piece;
+
+/* One nonlinear term dsca * (dx)^kx (dy)^ky Iz^iz mz^mz Q[qvar] in equation eqn
+ * (atom_NL_term_T, PL_equations.f90:22-30). */
+typedef struct {
+    double dsca;
+    int32_t kx, ky, iz, mz, qvar, eqn;
+} coral_amd_nlterm;
+
+/* A coupled set of variables (kxky or mean).  Mean systems must not carry
+ * horizontal derivatives. */
+typedef struct {
+    int32_t nvars;
+    const int32_t *bc_codes;   /* [nvars], Galerkin codes of chebyshev_galerkin_2.f90 */
+    const int32_t *eqn_orders; /* [nvars] */
+    int32_t n_mass;
+    const coral_amd_piece *mass;
+    int32_t n_stif;
+    const coral_amd_piece *stif;
+    int32_t n_nl;
+    const coral_amd_nlterm *nl;
+} coral_amd_system;
+
+/* One term of a full variable (a_linear_contribution_T, PL_equations.f90:71-79).
+ * kind: 0 = kxky variable, 1 = mean variable. */
+typedef struct {
+    double dsca;
+    int32_t kx, ky, dz, mz, kind, system, position;
+} coral_amd_fullterm;
+
+typedef struct {
+    int32_t nterms;
+    const coral_amd_fullterm *terms;
+} coral_amd_fullvar;
+
+/* Quadratic variable Q = F[ivar1] * F[ivar2]; -1 stands for ONE. */
+typedef struct {
+    int32_t ivar1, ivar2, remove_z_integral;
+} coral_amd_quadvar;
+
+typedef struct {
+    int32_t NX, NY, NZ;            /* dealiased mode counts (coral.parameters.in lines 1-3) */
+    double Lx, Ly, gap, center;    /* lines 5-8 */
+    int32_t nstages;               /* implicit stages s of the ARS scheme */
+    const double *a_arr;           /* s x s, row-major (IMEX_schemes.f90 A_arr) */
+    const double *a_hat;           /* (s+1) x (s+1), row-major (A_hat) */
+    int32_t n_kxky;
+    const coral_amd_system *kxky;
+    int32_t n_zero;
+    const coral_amd_system *zero;
+    int32_t n_full;
+    const coral_amd_fullvar *full;
+    int32_t n_quad;
+    const coral_amd_quadvar *quad;
+} coral_amd_problem;
+
+/* Build the solver on HIP device `device`: assembles every operator from the
+ * recipe, allocates all device memory.  Replaces the setup of
+ * full_problem_data_structure_T%init (PL_IMEX_timestepping.f90:510-569). */
+int coral_amd_create(const coral_amd_problem *prob, int device, coral_amd_ctx **out);
+int coral_amd_destroy(coral_amd_ctx *ctx);
+/* Copy the last error message of the calling thread into buf (NUL terminated). */
+int coral_amd_last_error(char *buf, int32_t len);
+
+/* (Re)factorize M - dt*a11*L for every wavenumber on the device
+ * (factorize_operators, PL_IMEX_timestepping.f90:351-422). */
+int coral_amd_set_dt(coral_amd_ctx *ctx, double dt);
+
+/* State transfer in Coral's layout (see header comment).  kind 0 = kxky
+ * system, 1 = mean system.  Upload zeroes dealiased modes and the kxky (0,0)
+ * mode, which the reference keeps at zero (PL_IMEX_timestepping_marching.f90:32-46). */
+int coral_amd_upload_state(coral_amd_ctx *ctx, int32_t kind, int32_t isys, const double *data);
+int coral_amd_download_state(coral_amd_ctx *ctx, int32_t kind, int32_t isys, double *data);
+
+/* march_forward(dt) repeated nsteps times with the factorized dt
+ * (one_step_beyond, PL_IMEX_timestepping_marching.f90:3-48).  Asynchronous. */
+int coral_amd_step(coral_amd_ctx *ctx, int32_t nsteps);
+int coral_amd_synchronize(coral_amd_ctx *ctx);
+
+/* Full variable `ifull` of the current field in physical space, C shape
+ * [NXAA][NYAA][NZAA] (x slowest, z fastest). */
+int coral_amd_get_physical(coral_amd_ctx *ctx, int32_t ifull, double *out);
+/* CFL time step of the last evaluated stage (compute_cfl_based_timestep,
+ * PL_IMEX_timestepping_marching.f90:200-254, cflFactor_along_z = 1). */
+int coral_amd_get_cfl_dt(coral_amd_ctx *ctx, double *out);
+
+/* Integer wavenumbers and 2/3-rule masks of Coral's slab layout
+ * (PL_geometry.f90:179-222): kx_int[NX/2], ky_int[NYAA], dealias_x[NX/2],
+ * dealias_y[NYAA] (1 = mode zeroed). */
+int coral_amd_wavenumbers(coral_amd_ctx *ctx, int32_t *kx_int, int32_t *ky_int,
+                          uint8_t *dealias_x, uint8_t *dealias_y);
+
+/* Sizes: total unknowns of kxky system isys, its band (nl, nu) and the number
+ * of kept wavenumbers (NX/2 * (NY-1)). */
+int coral_amd_system_info(coral_amd_ctx *ctx, int32_t isys, int32_t *total, int32_t *nl,
+                          int32_t *nu, int32_t *nkept);
+
+/* Unit seams on host buffers.
+ * Backward transform of a spectral field, C shape [NX/2][NYAA][NZ] complex, to
+ * physical C shape [NXAA][NYAA][NZAA] (c2r_transform, slabs.fftw3mpi/transforms.f90:155-225);
+ * dealiased slots of the input are ignored.  Forward is its inverse on the kept
+ * modes (r2c_transform, :227-301); dealiased output slots are set to zero. */
+int coral_amd_bwd_transform(coral_amd_ctx *ctx, const double *spec, double *phys);
+int coral_amd_fwd_transform(coral_amd_ctx *ctx, const double *phys, double *spec);
+
+/* Batched complex banded LU with partial pivoting and solve, on the device, for
+ * nsys matrices given in LAPACK band storage (ldab = 2kl+ku+1, column j of
+ * system s at ab[(s*n + j)*ldab], complex interleaved): the zgbtrf/zgbtrs
+ * calls of PL_algebra_z_3D.f90:189-276.  x[s*n + i] is overwritten with the
+ * solution; info[s] receives the LAPACK info of each factorization. */
+int coral_amd_banded_solve(int32_t device, int32_t n, int32_t kl, int32_t ku, int32_t nsys,
+                           const double *ab, double *x, int32_t *info);
+
+/* Per-kernel timing with HIP events on the context's stream.  enable=1 resets
+ * and starts accumulating; names are filled by coral_amd_timer_read. */
+int coral_amd_timers_enable(coral_amd_ctx *ctx, int32_t enable);
+/* Returns the number of kernels timed; for i < n: total ms and launch count. */
+int coral_amd_timer_read(coral_amd_ctx *ctx, int32_t i, char *name, int32_t name_len,
+                         double *total_ms, int32_t *count);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

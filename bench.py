@@ -1,0 +1,193 @@
+#!/usr/bin/env python3
+"""Benchmark: Rayleigh-Benard timesteps/s at 256^2 x 129 Fourier-Chebyshev, FP64.
+
+Workload (one MI355X): rotating Rayleigh-Benard convection in Coral's
+primitive-variable form (coral_amd/inputs/coral.equations.rrbc_ra5e7_ek1e-4:
+Ek = 1e-4, Ra = 5e7, Pr = 1), box 2pi x 2pi x 1, NX = NY = 256, NZ = 129
+(padded grid 384 x 384 x 193), ARS_222, fixed dt = 2e-5, seeded temperature
+perturbation of amplitude 1e-2 on |k| <= 32 and the 8 lowest Galerkin modes.
+A step is one full ARS_222 step (two implicit stages: transforms, products,
+NL assembly, right-hand sides, banded solves).  State and operators are
+resident in HBM before the timed region.
+
+Prints one JSON line (see the bench contract in the repository notes):
+metric/value (timesteps/s), roofline of the dominant kernel from HIP events on
+the library's stream, and a CPU baseline from the oracle on the host cores.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "RB timesteps/s at 256²×129 FP64; sim-time/s vs plain GPU at fixed Nu error"
+PEAK_HBM_GBS = 8000.0
+
+WORKLOADS = {
+    "rrbc256": dict(eq="coral.equations.rrbc_ra5e7_ek1e-4", NX=256, NZ=129, dt=2e-5, amp=1e-2, kmax=32,
+                    nmodes=8, seed=2, steps=500,
+                    name="rotating RBC Ek=1e-4 Ra=5e7 Pr=1, 256x256x129 (384x384x193 padded), ARS_222, dt=2e-5"),
+    "rbc64": dict(eq="coral.equations.rbc_ra1e5", NX=64, NZ=33, dt=1e-4, amp=1e-3, kmax=8, nmodes=8, seed=1,
+                  steps=1000, name="RBC Ra=1e5 Pr=1, 64x64x33 (96x96x49 padded), ARS_222, dt=1e-4"),
+}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def solve_bytes(N, kl, kv, nsys):
+    """Algorithmic HBM bytes of one banded-solve launch: LU factors (16 B per complex
+    entry, kl multipliers + kv+1 U entries per column), 1 B pivot offsets, rhs read,
+    solution written."""
+    return nsys * N * (16 * kl + 16 * (kv + 1) + 1 + 16 + 16)
+
+
+def kernel_bytes(name, P):
+    """Algorithmic bytes per launch of the main kernels (complex = 16 B, real = 8 B)."""
+    N, nsys, NZ, NZAA, NYAA, nkx = P["N"], P["nsys"], P["NZ"], P["NZAA"], P["NYAA"], P["nkx"]
+    nf, nq = P["nfull"], P["nquad"]
+    S_sys = 16 * N * nsys
+    table = {
+        "solve": solve_bytes(N, P["kl"], P["kl"] + P["ku"], nsys),
+        "kstd": 2 * S_sys,                                   # x read, K_std written
+        "extract": S_sys + nf * 16 * NZ * nsys,              # state read, spectra written
+        "zbwd": nf * 16 * (NZ + NZAA) * nsys,
+        "ybwd": nf * 16 * NZAA * (nsys + NYAA * nkx),
+        "xpass": 16 * NZAA * NYAA * nkx * (nf + nq),
+        "yfwd": nq * 16 * NZAA * (nsys + NYAA * nkx),
+        "zfwd": nq * 16 * (NZ + NZAA) * nsys,
+        "nl_assemble": nq * 16 * NZ * nsys + S_sys,
+    }
+    return table.get(name)
+
+
+def cpu_baseline(wl, recipe, k0, z0, nsteps=1):
+    """Oracle (numpy/scipy + OpenMP C) on the host cores: time `nsteps` steps after setup."""
+    from oracle import coral_oracle as O
+    geo = O.Geometry(wl["NX"], wl["NX"], wl["NZ"], 2 * math.pi, 2 * math.pi)
+    ora = O.OracleSolver(recipe, geo, "ARS_222")
+    ora.field, ora.zfield = [a.copy() for a in k0], [a.copy() for a in z0]
+    t0 = time.perf_counter()
+    ora.factorize(wl["dt"])
+    t_setup = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    for _ in range(nsteps):
+        ora.step(wl["dt"])
+    dt = time.perf_counter() - t0
+    return nsteps / dt, t_setup, dt
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=None)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--workload", default="rrbc256", choices=sorted(WORKLOADS))
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-steps", type=int, default=1)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py: this build runs the whole domain on one GPU; multi-GPU runs are not part of it "
+              "(see DESIGN.md for how the path would shard).", file=sys.stderr)
+        sys.exit(2)
+    wl = WORKLOADS[args.workload]
+    steps = args.steps if args.steps is not None else wl["steps"]
+
+    from coral_amd.equations import read_equations
+    from coral_amd.ic import rbc_initial_state
+    from coral_amd.solver import Solver
+    import numpy as np
+
+    recipe = read_equations(os.path.join(ROOT, "coral_amd", "inputs", wl["eq"]))
+    NX, NZ = wl["NX"], wl["NZ"]
+    k0, z0 = rbc_initial_state(recipe, NX, NX, NZ, wl["amp"], wl["kmax"], wl["nmodes"], wl["seed"])
+    g = Solver(recipe, NX, NX, NZ, 2 * math.pi, 2 * math.pi, scheme="ARS_222")
+    total, kl, ku, nsys = g.system_info(0)
+    g.set_state(k0, z0)
+    t0 = time.perf_counter()
+    g.factorize_operators(wl["dt"])
+    log(f"setup: LU factorization of {nsys} systems (N={total}, kl={kl}, ku={ku}) in {time.perf_counter() - t0:.3f} s")
+    g.march_forward(wl["dt"], args.warmup)
+    g.synchronize()
+
+    g.timers_enable(True)
+    g.synchronize()
+    t0 = time.perf_counter()
+    g.march_forward(wl["dt"], steps)
+    g.synchronize()
+    elapsed = time.perf_counter() - t0
+    timers = g.timers()
+    g.timers_enable(False)
+    kfin, _ = g.get_state()
+    finite = bool(np.isfinite(kfin[0]).all())
+    norm = float(np.linalg.norm(kfin[0]))
+    log(f"{steps} steps in {elapsed:.3f} s, state finite={finite}, |state|={norm:.6e}")
+
+    P = dict(N=total, kl=kl, ku=ku, nsys=nsys, NZ=NZ, NZAA=3 * NZ // 2, NYAA=3 * NX // 2, nkx=NX // 2,
+             nfull=len(recipe.full_vars), nquad=len(recipe.quad_vars))
+    per_kernel = {}
+    for name, (ms, cnt) in timers.items():
+        per_kernel[name] = {"ms_per_launch": ms / cnt, "launches_per_step": cnt / steps,
+                            "ms_per_step": ms / steps}
+        b = kernel_bytes(name, P)
+        if b is not None:
+            per_kernel[name]["GBps"] = b / (ms / cnt * 1e-3) / 1e9
+    for name, v in sorted(per_kernel.items(), key=lambda kv: -kv[1]["ms_per_step"]):
+        log(f"  {name:14s} {v['ms_per_step']:9.3f} ms/step  {v['ms_per_launch']:8.3f} ms/launch"
+            + (f"  {v['GBps']:8.1f} GB/s" if "GBps" in v else ""))
+    # dominant kernel with an algorithmic byte count
+    dom = max((n for n in per_kernel if kernel_bytes(n, P) is not None), key=lambda n: per_kernel[n]["ms_per_step"])
+    dom_bytes = kernel_bytes(dom, P)
+    achieved = dom_bytes / (per_kernel[dom]["ms_per_launch"] * 1e-3) / 1e9
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "traffic.json")
+    if os.path.exists(tpath):
+        tr = json.load(open(tpath)).get(args.workload, {}).get(dom)
+        if tr is not None:
+            traffic = tr.get("bytes_per_launch")
+    roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic, "algorithmic_bytes": dom_bytes}
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        try:
+            v, t_setup, t_run = cpu_baseline(wl, recipe, k0, z0, args.cpu_steps)
+            cpu = {"value": round(v, 5), "unit": "timesteps/s", "cores": threads, "kind": "port",
+                   "sample": f"{args.cpu_steps} ARS_222 step(s) of the same workload by the oracle "
+                             f"(numpy/scipy.fft with {threads} workers + OpenMP C banded LU) after setup; "
+                             f"setup {t_setup:.1f} s excluded, timed {t_run:.1f} s"}
+        except MemoryError as e:  # pragma: no cover
+            cpu = {"value": None, "unit": "timesteps/s", "cores": threads, "kind": "port",
+                   "sample": f"failed: {e}"}
+
+    value = steps / elapsed
+    out = {
+        "metric": METRIC,
+        "value": round(value, 4),
+        "unit": "timesteps/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * elapsed / steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded perturbation, no checkpoint)",
+        "config": {"workload": wl["name"], "grid": [NX, NX, NZ], "systems": nsys, "unknowns_per_system": total,
+                   "band": [kl, ku], "state_finite": finite},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "kernels_ms_per_step": {k: round(v["ms_per_step"], 4) for k, v in per_kernel.items()},
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -89,7 +89,8 @@ struct coral_amd_ctx {
     std::vector<std::unique_ptr<SysDev>> ks, zs;
     std::vector<std::unique_ptr<ExtractDev>> ext;
     std::vector<int> full_has_kxky;
-    DevBuf<double2> spec, Fz, G, H, Hy, Qhat, phys;
+    DevBuf<double2> spec, Fz, G, H, Hy, Qhat;
+    DevBuf<double> physd;
     DevBuf<double> Tz, Cz, zweight;
     DevBuf<double2> twx, twy;
     DevBuf<int> qa, qb;
@@ -429,6 +430,23 @@ void one_step(coral_amd_ctx &c) {
     for (auto &d : c.zs) std::swap(d->field.p, d->aux.p);
 }
 
+void ensure_phys(coral_amd_ctx &c) {
+    const size_t n = size_t(c.P.NZAA) * c.P.NYAA * c.P.NXAA;
+    if (c.physd.n != n) c.physd.alloc(n);
+}
+
+// device [z][y][x] -> host [x][y][z]
+void download_phys(coral_amd_ctx &c, double *out) {
+    const Problem &P = c.P;
+    HIP_OK(hipStreamSynchronize(c.stream));
+    std::vector<double> h(c.physd.n);
+    HIP_OK(hipMemcpy(h.data(), c.physd.p, h.size() * sizeof(double), hipMemcpyDeviceToHost));
+    for (int z = 0; z < P.NZAA; ++z)
+        for (int y = 0; y < P.NYAA; ++y)
+            for (int x = 0; x < P.NXAA; ++x)
+                out[(size_t(x) * P.NYAA + y) * P.NZAA + z] = h[(size_t(z) * P.NYAA + y) * P.NXAA + x];
+}
+
 int fail(const std::exception &e) {
     g_err = e.what();
     return -1;
@@ -724,27 +742,86 @@ int coral_amd_system_info(coral_amd_ctx *c, int32_t isys, int32_t *total, int32_
 }
 
 int coral_amd_get_physical(coral_amd_ctx *c, int32_t ifull, double *out) {
-    g_err = "coral_amd_get_physical: not implemented yet";
-    (void)c;
-    (void)ifull;
-    (void)out;
-    return -2;
+    try {
+        if (!c || !out) throw std::runtime_error("null argument");
+        const Problem &P = c->P;
+        if (ifull < 0 || ifull >= P.nfull) throw std::runtime_error("bad full-variable index");
+        HIP_OK(hipSetDevice(c->device));
+        std::vector<const double2 *> kst, zst;
+        for (auto &d : c->ks) kst.push_back(d->field.p);
+        for (auto &d : c->zs) zst.push_back(d->field.p);
+        backward_to_G(*c, kst, zst);
+        const long long nlines = (long long)P.NZAA * P.NYAA;
+        ensure_phys(*c);
+        launch_xbwd(c->stream, c->px, c->G.p + (size_t)ifull * nlines * P.nkx, c->physd.p, 1, nlines, P.nkx);
+        download_phys(*c, out);
+        return 0;
+    } catch (const std::exception &e) {
+        return fail(e);
+    }
 }
 
 int coral_amd_bwd_transform(coral_amd_ctx *c, const double *spec, double *phys) {
-    g_err = "coral_amd_bwd_transform: not implemented yet";
-    (void)c;
-    (void)spec;
-    (void)phys;
-    return -2;
+    try {
+        if (!c || !spec || !phys) throw std::runtime_error("null argument");
+        const Problem &P = c->P;
+        HIP_OK(hipSetDevice(c->device));
+        HIP_OK(hipStreamSynchronize(c->stream));
+        std::vector<double2> h(size_t(P.nsys) * P.NZ);
+        for (int ix = 0; ix < P.nkx; ++ix)
+            for (int jy = 0; jy < P.nky; ++jy) {
+                const size_t src = (size_t(ix) * P.NYAA + P.ky_slot[jy]) * P.NZ;
+                const size_t dst = (size_t(ix) * P.nky + jy) * P.NZ;
+                for (int n = 0; n < P.NZ; ++n) h[dst + n] = make_double2(spec[2 * (src + n)], spec[2 * (src + n) + 1]);
+            }
+        HIP_OK(hipMemcpy(c->spec.p, h.data(), h.size() * sizeof(double2), hipMemcpyHostToDevice));
+        launch_zbwd(c->stream, c->Tz.p, c->spec.p, c->Fz.p, P.NZAA, P.NZ, P.nsys, 1);
+        launch_ybwd(c->stream, c->py, c->Fz.p, c->G.p, 1, P.NZAA, P.nkx, P.nky, P.NY);
+        const long long nlines = (long long)P.NZAA * P.NYAA;
+        ensure_phys(*c);
+        launch_xbwd(c->stream, c->px, c->G.p, c->physd.p, 1, nlines, P.nkx);
+        download_phys(*c, phys);
+        return 0;
+    } catch (const std::exception &e) {
+        return fail(e);
+    }
 }
 
 int coral_amd_fwd_transform(coral_amd_ctx *c, const double *phys, double *spec) {
-    g_err = "coral_amd_fwd_transform: not implemented yet";
-    (void)c;
-    (void)phys;
-    (void)spec;
-    return -2;
+    try {
+        if (!c || !spec || !phys) throw std::runtime_error("null argument");
+        const Problem &P = c->P;
+        HIP_OK(hipSetDevice(c->device));
+        HIP_OK(hipStreamSynchronize(c->stream));
+        ensure_phys(*c);
+        // host [x][y][z] -> device [z][y][x]
+        std::vector<double> h(size_t(P.NXAA) * P.NYAA * P.NZAA);
+        for (int x = 0; x < P.NXAA; ++x)
+            for (int y = 0; y < P.NYAA; ++y)
+                for (int z = 0; z < P.NZAA; ++z)
+                    h[(size_t(z) * P.NYAA + y) * P.NXAA + x] = phys[(size_t(x) * P.NYAA + y) * P.NZAA + z];
+        HIP_OK(hipMemcpy(c->physd.p, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
+        const long long nlines = (long long)P.NZAA * P.NYAA;
+        launch_xfwd(c->stream, c->px, c->physd.p, c->H.p, 1, nlines, P.nkx);
+        launch_yfwd(c->stream, c->py, c->H.p, c->Hy.p, 1, P.NZAA, P.nkx, P.nky, P.NY);
+        launch_zfwd(c->stream, c->Cz.p, c->Hy.p, c->Qhat.p, P.NZ, P.NZAA, P.nsys, 1);
+        HIP_OK(hipStreamSynchronize(c->stream));
+        std::vector<double2> q(size_t(P.nsys) * P.NZ);
+        HIP_OK(hipMemcpy(q.data(), c->Qhat.p, q.size() * sizeof(double2), hipMemcpyDeviceToHost));
+        std::memset(spec, 0, sizeof(double) * 2 * size_t(P.nkx) * P.NYAA * P.NZ);
+        for (int ix = 0; ix < P.nkx; ++ix)
+            for (int jy = 0; jy < P.nky; ++jy) {
+                const size_t dst = (size_t(ix) * P.NYAA + P.ky_slot[jy]) * P.NZ;
+                const size_t src = (size_t(ix) * P.nky + jy) * P.NZ;
+                for (int n = 0; n < P.NZ; ++n) {
+                    spec[2 * (dst + n)] = q[src + n].x;
+                    spec[2 * (dst + n) + 1] = q[src + n].y;
+                }
+            }
+        return 0;
+    } catch (const std::exception &e) {
+        return fail(e);
+    }
 }
 
 int coral_amd_banded_solve(int32_t device, int32_t n, int32_t kl, int32_t ku, int32_t nsys, const double *ab,
@@ -773,6 +850,39 @@ int coral_amd_banded_solve(int32_t device, int32_t n, int32_t kl, int32_t ku, in
         HIP_OK(hipMemcpy(x, xb.p, xb.n * sizeof(double2), hipMemcpyDeviceToHost));
         HIP_OK(hipMemcpy(info, inf.p, nsys * sizeof(int), hipMemcpyDeviceToHost));
         HIP_OK(hipStreamDestroy(st));
+        return 0;
+    } catch (const std::exception &e) {
+        return fail(e);
+    }
+}
+
+int coral_amd_assembled_operator(const coral_amd_problem *prob, int32_t isys, int32_t which, int32_t kx_int,
+                                 int32_t ky_int, double *out, int32_t *total, int32_t *nl, int32_t *nu) {
+    try {
+        if (!prob) throw std::runtime_error("null problem");
+        Problem P = assemble(*prob);
+        if (isys < 0 || isys >= int(P.kxky.size())) throw std::runtime_error("bad kxky system index");
+        const SystemTables &t = P.kxky[isys];
+        if (total) *total = t.total;
+        if (nl) *nl = t.nl;
+        if (nu) *nu = t.nu;
+        if (!out) return 0;
+        const auto &keys = which == 0 ? t.mass_keys : t.stif_keys;
+        const auto &band = which == 0 ? t.mass_band : t.stif_band;
+        const int n = t.total, w = t.nl + t.nu + 1;
+        std::memset(out, 0, sizeof(double) * 2 * size_t(n) * n);
+        const double kx = kx_int * P.kx_box, ky = ky_int * P.ky_box;
+        for (size_t g = 0; g < keys.size(); ++g) {
+            double2 cf = monomial(keys[g].first, keys[g].second, kx, ky);
+            for (int i = 0; i < n; ++i)
+                for (int d = -t.nl; d <= t.nu; ++d) {
+                    int j = i + d;
+                    if (j < 0 || j >= n) continue;
+                    double b = band[(g * n + i) * w + d + t.nl];
+                    out[2 * (size_t(i) * n + j)] += b * cf.x;
+                    out[2 * (size_t(i) * n + j) + 1] += b * cf.y;
+                }
+        }
         return 0;
     } catch (const std::exception &e) {
         return fail(e);

@@ -414,6 +414,82 @@ void launch_xpass(hipStream_t st, const FFTPlan &px, const XPassArgs &a) {
     CHECK_LAUNCH();
 }
 
+// c2r only (diagnostics and the transform seam): G[b][line][kx] -> phys[b][line][x]
+__global__ __launch_bounds__(256) void k_xbwd(FFTPlanDev p, const double2 *__restrict__ G, double *__restrict__ phys,
+                                              long long nlines, int nkx, int LX) {
+    extern __shared__ double2 sbuf[];
+    const int N = p.N, LS = N + 1;
+    const int b = blockIdx.y;
+    const long long l0 = (long long)blockIdx.x * LX;
+    const int nl = (int)min((long long)LX, nlines - l0);
+    for (int e = threadIdx.x; e < LX * LS; e += blockDim.x) sbuf[e] = make_double2(0.0, 0.0);
+    __syncthreads();
+    const double2 *g = G + ((size_t)b * nlines + l0) * nkx;
+    for (int e = threadIdx.x; e < nl * nkx; e += blockDim.x) {
+        int l = e / nkx, k = e - l * nkx;
+        double2 v = g[e];
+        if (k == 0) {
+            sbuf[l * LS] = make_double2(v.x, 0.0);
+        } else {
+            sbuf[l * LS + k] = v;
+            sbuf[l * LS + N - k] = cconj(v);
+        }
+    }
+    __syncthreads();
+    fft_lds<6>(sbuf, LS, LX, p, +1.0);
+    double *o = phys + ((size_t)b * nlines + l0) * N;
+    for (int e = threadIdx.x; e < nl * N; e += blockDim.x) {
+        int l = e / N, x = e - l * N;
+        o[e] = sbuf[l * LS + x].x;
+    }
+}
+
+// r2c only: phys[b][line][x] -> H[b][line][kx] (kept kx)
+__global__ __launch_bounds__(256) void k_xfwd(FFTPlanDev p, const double *__restrict__ phys, double2 *__restrict__ H,
+                                              long long nlines, int nkx, int LX) {
+    extern __shared__ double2 sbuf[];
+    const int N = p.N, LS = N + 1;
+    const int b = blockIdx.y;
+    const long long l0 = (long long)blockIdx.x * LX;
+    const int nl = (int)min((long long)LX, nlines - l0);
+    const double *src = phys + ((size_t)b * nlines + l0) * N;
+    for (int e = threadIdx.x; e < LX * N; e += blockDim.x) {
+        int l = e / N, x = e - l * N;
+        sbuf[l * LS + x] = make_double2(l < nl ? src[e] : 0.0, 0.0);
+    }
+    __syncthreads();
+    fft_lds<6>(sbuf, LS, LX, p, -1.0);
+    double2 *h = H + ((size_t)b * nlines + l0) * nkx;
+    for (int e = threadIdx.x; e < nl * nkx; e += blockDim.x) {
+        int l = e / nkx, k = e - l * nkx;
+        h[e] = sbuf[l * LS + k];
+    }
+}
+
+static int xlines_for(int N) {
+    int LX = 1;
+    while (LX * 2 * N <= 1536) LX *= 2;
+    return LX;
+}
+
+void launch_xbwd(hipStream_t st, const FFTPlan &px, const double2 *G, double *phys, int nbatch, long long nlines,
+                 int nkx) {
+    int LX = xlines_for(px.N);
+    size_t shmem = size_t(LX) * (px.N + 1) * sizeof(double2);
+    hipLaunchKernelGGL(k_xbwd, dim3(blocks_for(nlines, LX), nbatch), dim3(256), shmem, st, to_dev(px), G, phys, nlines,
+                       nkx, LX);
+    CHECK_LAUNCH();
+}
+
+void launch_xfwd(hipStream_t st, const FFTPlan &px, const double *phys, double2 *H, int nbatch, long long nlines,
+                 int nkx) {
+    int LX = xlines_for(px.N);
+    size_t shmem = size_t(LX) * (px.N + 1) * sizeof(double2);
+    hipLaunchKernelGGL(k_xfwd, dim3(blocks_for(nlines, LX), nbatch), dim3(256), shmem, st, to_dev(px), phys, H, nlines,
+                       nkx, LX);
+    CHECK_LAUNCH();
+}
+
 // remove_vertical_mean_of_quadratic_var (PL_IMEX_timestepping.f90:223-235)
 __global__ void k_remove_zmean(double2 *Qhat, int NZ, int nsys) {
     int s = blockIdx.x * blockDim.x + threadIdx.x;

@@ -44,6 +44,11 @@ void launch_xpass(hipStream_t st, const FFTPlan &px, const XPassArgs &a);
 // y forward: Hy[b][z][ix*nky+jy] = sum_y H[b][z][y][ix] e^{-i ky y}
 void launch_yfwd(hipStream_t st, const FFTPlan &py, const double2 *H, double2 *Hy, int nbatch, int NZAA,
                  int nkx, int nky, int NY);
+// c2r / r2c passes alone (diagnostics, transform seam); lines = (z, y) pairs
+void launch_xbwd(hipStream_t st, const FFTPlan &px, const double2 *G, double *phys, int nbatch, long long nlines,
+                 int nkx);
+void launch_xfwd(hipStream_t st, const FFTPlan &px, const double *phys, double2 *H, int nbatch, long long nlines,
+                 int nkx);
 // remove the z-integral of Qhat[s][:] (PL_IMEX_timestepping.f90:223-235)
 void launch_remove_zmean(hipStream_t st, double2 *Qhat, int NZ, int nsys);
 

@@ -11,7 +11,10 @@ every machine, so that the GPU path and any checker start from the same state:
   wavevectors with ``kx_int**2 + ky_int**2 <= kmax**2``;
 * the kx = 0 plane is made Hermitian in ky (``c(-ky) = conj(c(ky))``), as the
   real fields that Coral's forward transform produces are;
-* the (0, 0) mode and every dealiased mode stay zero.
+* the (0, 0) mode and every dealiased mode stay zero;
+* the coefficients are scaled so that their l2 norm over the full Hermitian
+  spectrum equals the requested amplitude (the Galerkin functions have O(1)
+  norms, so the physical rms of the perturbation is of that order).
 
 The state layout is Coral's per-system array ``(total, NYAA, NX/2)`` in Fortran
 order, i.e. a C-contiguous ``(NX/2, NYAA, total)`` complex128 array; interleaved
@@ -101,12 +104,17 @@ def seeded_perturbation(NX: int, NY: int, NZ: int, bc_codes: Sequence[int], var:
                 continue                              # filled by symmetry below / (0,0) stays 0
             for p in pos:
                 a, b = rng.normal_pair()
-                state[ix, iy, p] = amplitude * complex(a, b)
+                state[ix, iy, p] = complex(a, b)
     # Hermitian kx = 0 plane: c(0, -ky) = conj(c(0, ky))
     for iy in range(NYAA):
         if ky[iy] < 0 and kept[0, iy]:
             jy = int(np.nonzero(ky == -ky[iy])[0][0])
             state[0, iy, :] = np.conj(state[0, jy, :])
+    # normalise: the l2 norm of the coefficients over the full (Hermitian) spectrum
+    # equals `amplitude`, so the physical perturbation has an rms of that order
+    norm2 = np.sum(np.abs(state[0]) ** 2) + 2.0 * np.sum(np.abs(state[1:]) ** 2)
+    if norm2 > 0:
+        state *= amplitude / math.sqrt(norm2)
     return state
 
 

@@ -35,6 +35,56 @@ def _arr(ctype, values):
     return (ctype * max(1, len(values)))(*values)
 
 
+def build_problem(recipe: Recipe, NX: int, NY: int, NZ: int, Lx: float, Ly: float, gap: float = 1.0,
+                  center: float = 0.5, scheme: str = "ARS_222"):
+    """ctypes ``coral_amd_problem`` for a recipe; returns (problem, keepalive)."""
+    A, Ah = tableau(scheme)
+    keep = []
+
+    def system(s):
+        bc = _arr(ctypes.c_int32, s.bc_codes)
+        eo = _arr(ctypes.c_int32, s.eqn_orders)
+        mass = _arr(_lib.Piece, [_lib.Piece(p.dsca, p.kx, p.ky, p.iz, p.mz, p.var, p.eqn) for p in s.mass])
+        stif = _arr(_lib.Piece, [_lib.Piece(p.dsca, p.kx, p.ky, p.iz, p.mz, p.var, p.eqn) for p in s.stif])
+        nl = _arr(_lib.NLTerm, [_lib.NLTerm(t.dsca, t.kx, t.ky, t.iz, t.mz, t.qvar, t.eqn) for t in s.nl])
+        keep.extend([bc, eo, mass, stif, nl])
+        return _lib.System(s.nvars, bc, eo, len(s.mass), mass, len(s.stif), stif, len(s.nl), nl)
+
+    ks = _arr(_lib.System, [system(s) for s in recipe.kxky_systems])
+    zs = _arr(_lib.System, [system(s) for s in recipe.zero_systems])
+    fvs = []
+    for f in recipe.full_vars:
+        terms = _arr(_lib.FullTerm, [_lib.FullTerm(t.dsca, t.kx, t.ky, t.dz, t.mz, 0 if t.kind == "kxky" else 1,
+                                                   t.system, t.position) for t in f.terms])
+        keep.append(terms)
+        fvs.append(_lib.FullVar(len(f.terms), terms))
+    fv = _arr(_lib.FullVar, fvs)
+    qv = _arr(_lib.QuadVar, [_lib.QuadVar(q.ivar1, q.ivar2, int(q.remove_z_integral)) for q in recipe.quad_vars])
+    a_arr = _arr(ctypes.c_double, A.ravel())
+    a_hat = _arr(ctypes.c_double, Ah.ravel())
+    keep.extend([ks, zs, fv, qv, a_arr, a_hat])
+    prob = _lib.Problem(NX, NY, NZ, Lx, Ly, gap, center, A.shape[0], a_arr, a_hat,
+                        len(recipe.kxky_systems), ks, len(recipe.zero_systems), zs,
+                        len(recipe.full_vars), fv, len(recipe.quad_vars), qv)
+    return prob, keep
+
+
+def assembled_operator(recipe: Recipe, NX: int, NY: int, NZ: int, Lx: float, Ly: float, isys: int, which: str,
+                       kx_int: int, ky_int: int, gap: float = 1.0, center: float = 0.5):
+    """Dense mass (``which='mass'``) or stiffness operator of kxky system ``isys``
+    at integer wavenumbers, as assembled by the library (no GPU needed).
+    Returns (matrix complex (total, total), nl, nu)."""
+    prob, keep = build_problem(recipe, NX, NY, NZ, Lx, Ly, gap, center)
+    t, nl, nu = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+    w = 0 if which == "mass" else 1
+    check(lib.coral_amd_assembled_operator(ctypes.byref(prob), isys, w, kx_int, ky_int, None, ctypes.byref(t),
+                                           ctypes.byref(nl), ctypes.byref(nu)), "coral_amd_assembled_operator")
+    out = np.zeros((t.value, t.value), np.complex128)
+    check(lib.coral_amd_assembled_operator(ctypes.byref(prob), isys, w, kx_int, ky_int, out.ctypes.data, None,
+                                           None, None), "coral_amd_assembled_operator")
+    return out, nl.value, nu.value
+
+
 class Solver:
     def __init__(self, recipe: Recipe, NX: int, NY: int, NZ: int, Lx: float, Ly: float,
                  gap: float = 1.0, center: float = 0.5, scheme: str = "ARS_222", device: int = 0):
@@ -43,35 +93,8 @@ class Solver:
         self.NXAA, self.NYAA, self.NZAA = 3 * NX // 2, 3 * NY // 2, 3 * NZ // 2
         self.Lx, self.Ly, self.gap, self.center = Lx, Ly, gap, center
         self.scheme = scheme
-        A, Ah = tableau(scheme)
-        self.A, self.Ah = A, Ah
-        keep = []  # keep ctypes objects alive during create
-
-        def system(s):
-            bc = _arr(ctypes.c_int32, s.bc_codes)
-            eo = _arr(ctypes.c_int32, s.eqn_orders)
-            mass = _arr(_lib.Piece, [_lib.Piece(p.dsca, p.kx, p.ky, p.iz, p.mz, p.var, p.eqn) for p in s.mass])
-            stif = _arr(_lib.Piece, [_lib.Piece(p.dsca, p.kx, p.ky, p.iz, p.mz, p.var, p.eqn) for p in s.stif])
-            nl = _arr(_lib.NLTerm, [_lib.NLTerm(t.dsca, t.kx, t.ky, t.iz, t.mz, t.qvar, t.eqn) for t in s.nl])
-            keep.extend([bc, eo, mass, stif, nl])
-            return _lib.System(s.nvars, bc, eo, len(s.mass), mass, len(s.stif), stif, len(s.nl), nl)
-
-        ks = _arr(_lib.System, [system(s) for s in recipe.kxky_systems])
-        zs = _arr(_lib.System, [system(s) for s in recipe.zero_systems])
-        fvs = []
-        for f in recipe.full_vars:
-            terms = _arr(_lib.FullTerm, [_lib.FullTerm(t.dsca, t.kx, t.ky, t.dz, t.mz, 0 if t.kind == "kxky" else 1,
-                                                       t.system, t.position) for t in f.terms])
-            keep.append(terms)
-            fvs.append(_lib.FullVar(len(f.terms), terms))
-        fv = _arr(_lib.FullVar, fvs)
-        qv = _arr(_lib.QuadVar, [_lib.QuadVar(q.ivar1, q.ivar2, int(q.remove_z_integral)) for q in recipe.quad_vars])
-        a_arr = _arr(ctypes.c_double, A.ravel())
-        a_hat = _arr(ctypes.c_double, Ah.ravel())
-        keep.extend([ks, zs, fv, qv, a_arr, a_hat])
-        prob = _lib.Problem(NX, NY, NZ, Lx, Ly, gap, center, A.shape[0], a_arr, a_hat,
-                            len(recipe.kxky_systems), ks, len(recipe.zero_systems), zs,
-                            len(recipe.full_vars), fv, len(recipe.quad_vars), qv)
+        self.A, self.Ah = tableau(scheme)
+        prob, keep = build_problem(recipe, NX, NY, NZ, Lx, Ly, gap, center, scheme)
         ctx = ctypes.c_void_p()
         check(lib.coral_amd_create(ctypes.byref(prob), device, ctypes.byref(ctx)), "coral_amd_create")
         self._ctx = ctx

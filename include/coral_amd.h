@@ -154,6 +154,14 @@ int coral_amd_fwd_transform(coral_amd_ctx *ctx, const double *phys, double *spec
 int coral_amd_banded_solve(int32_t device, int32_t n, int32_t kl, int32_t ku, int32_t nsys,
                            const double *ab, double *x, int32_t *info);
 
+/* Assembly only, no device touched: the dense operator of kxky system isys at
+ * integer wavenumbers (kx_int, ky_int), i.e. the per-wavenumber CSR matrix the
+ * reference builds in PL_IMEX_timestepping.f90:1500-1574 after the change of
+ * basis (PL_algebra_z_3D.f90:40-53).  which = 0 mass M, 1 stiffness L.  out
+ * (complex, row-major total x total) may be NULL to query total/nl/nu only. */
+int coral_amd_assembled_operator(const coral_amd_problem *prob, int32_t isys, int32_t which, int32_t kx_int,
+                                 int32_t ky_int, double *out, int32_t *total, int32_t *nl, int32_t *nu);
+
 /* Per-kernel timing with HIP events on the context's stream.  enable=1 resets
  * and starts accumulating; names are filled by coral_amd_timer_read. */
 int coral_amd_timers_enable(coral_amd_ctx *ctx, int32_t enable);

@@ -38,6 +38,8 @@ import scipy.sparse as sp
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB = None
+# threads for scipy's FFTs (the C parts use OpenMP with OMP_NUM_THREADS)
+WORKERS = int(os.environ.get("ORACLE_THREADS", os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)))
 
 
 def _lib():
@@ -440,24 +442,24 @@ def to_diagonals(m: np.ndarray, nl: int, nu: int) -> np.ndarray:
 def spec_to_phys(S: np.ndarray, geo: Geometry) -> np.ndarray:
     """c2r_transform (slabs.fftw3mpi/transforms.f90:155-225).
     S: complex (nkx=NX/2, NYAA, NZ) -> real (NXAA, NYAA, NZAA), axes (x, y, z)."""
-    A = sfft.ifft(S, axis=1) * geo.NYAA                  # y c2c FFTW_BACKWARD (:172-178)
+    A = sfft.ifft(S, axis=1, workers=WORKERS) * geo.NYAA                  # y c2c FFTW_BACKWARD (:172-178)
     A[:, :, 0] *= 2.0                                    # Chebyshev k=0 row x2 (:184)
     B = np.zeros((geo.NXAA // 2 + 1, geo.NYAA, geo.NZ), np.complex128)
     B[: geo.NX // 2] = A * 0.5                           # pad x and halve (:197-198)
-    C = sfft.irfft(B, n=geo.NXAA, axis=0) * geo.NXAA     # x c2r, unnormalised (:204-210)
+    C = sfft.irfft(B, n=geo.NXAA, axis=0, workers=WORKERS) * geo.NXAA     # x c2r, unnormalised (:204-210)
     D = np.zeros((geo.NXAA, geo.NYAA, geo.NZAA))
     D[:, :, : geo.NZ] = C                                # pad z (:216-217)
-    return sfft.dct(D, type=3, axis=2)                   # REDFT01 (:223)
+    return sfft.dct(D, type=3, axis=2, workers=WORKERS)                   # REDFT01 (:223)
 
 
 def phys_to_spec(P: np.ndarray, geo: Geometry) -> np.ndarray:
     """r2c_transform (slabs.fftw3mpi/transforms.f90:227-301), inverse of the above."""
-    B5 = sfft.dct(P, type=2, axis=2)                     # REDFT10 (:243)
+    B5 = sfft.dct(P, type=2, axis=2, workers=WORKERS)                     # REDFT10 (:243)
     B4 = B5[:, :, : geo.NZ]                              # shrink z (:250)
-    B3 = sfft.rfft(B4, axis=0)                           # x r2c (:256-262)
+    B3 = sfft.rfft(B4, axis=0, workers=WORKERS)                           # x r2c (:256-262)
     B2 = B3[: geo.NX // 2].copy()                        # shrink x (:268)
     B2[:, :, 0] *= 0.5                                   # Chebyshev k=0 row x1/2 (:279)
-    S = sfft.fft(B2, axis=1)                             # y c2c FFTW_FORWARD (:285-292)
+    S = sfft.fft(B2, axis=1, workers=WORKERS)                             # y c2c FFTW_FORWARD (:285-292)
     return S / (geo.NXAA * geo.NYAA * geo.NZAA)          # normalisation (:293-296)
 
 
@@ -530,14 +532,21 @@ class OracleSolver:
             n = ops.total
             ldab = 2 * nl + nu + 1
             ab = np.zeros((len(kept), n, ldab), np.complex128)
-            # A(i, i+d) = sum_g coef_g bands[g, d+nl, i] -> LAPACK column j=i+d, row kl+ku+i-j
-            cvec = coef[kept[:, 0], kept[:, 1], :]          # (nk, ng)
-            full = np.einsum("kg,gdi->kdi", cvec, bands)    # (nk, nl+nu+1, n)
-            for d in range(-nl, nu + 1):
-                i = np.arange(max(0, -d), min(n, n - d))
-                j = i + d
-                ab[:, j, nl + nu + i - j] = full[:, d + nl, i]
-            ipiv, info = zgbtrf_batch(ab, nl, nu)
+            ipiv = np.zeros((len(kept), n), np.int32)
+            info = np.zeros(len(kept), np.int32)
+            cvec_all = coef[kept[:, 0], kept[:, 1], :]      # (nk, ng)
+            chunk = 2048
+            for c0 in range(0, len(kept), chunk):
+                cvec = cvec_all[c0:c0 + chunk]
+                # A(i, i+d) = sum_g coef_g bands[g, d+nl, i] -> LAPACK column j=i+d, row kl+ku+i-j
+                full = np.einsum("kg,gdi->kdi", cvec, bands)    # (nk, nl+nu+1, n)
+                sub = ab[c0:c0 + chunk]
+                for d in range(-nl, nu + 1):
+                    i = np.arange(max(0, -d), min(n, n - d))
+                    j = i + d
+                    sub[:, j, nl + nu + i - j] = full[:, d + nl, i]
+                p, inf = zgbtrf_batch(sub, nl, nu)
+                ipiv[c0:c0 + chunk], info[c0:c0 + chunk] = p, inf
             if np.any(info):
                 raise RuntimeError("singular evolution operator in kxky system %d" % (isys + 1))
             self.klu.append((kept, ab, ipiv, nl, nu))
@@ -597,7 +606,8 @@ class OracleSolver:
         v = np.max(np.abs(phys[1])) * g.NYAA / g.Ly
         w8 = g.gauss_chebyshev_weight()
         w = max(v, float(np.max(np.abs(phys[2]) / w8[None, None, :])))
-        return 1.0 / max(u, v, w)
+        m = max(u, v, w)
+        return 1.0 / m if m > 0 else math.inf
 
     # --- NL terms (marching.f90:257-418) -----------------------------------
     def nl_terms(self, aux, zaux):

@@ -1,0 +1,281 @@
+"""GPU parity: the HIP path (through the C ABI) against the oracle on the same
+seeded inputs.  Tolerance of the spectral state after N steps: relative L2
+<= 1e-10 (FP64 path).  Integer indexing and masks are compared bit for bit."""
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+import scipy.linalg.lapack as lap
+
+from conftest import GOLDEN, product_recipe, recipe
+from coral_amd import ic
+from oracle import coral_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-10
+
+
+@pytest.fixture(scope="module")
+def solver_mod(native_lib):
+    from coral_amd import solver
+    return solver
+
+
+def rel(a, b):
+    nb = np.linalg.norm(b)
+    return float(np.linalg.norm(a - b) / (nb if nb > 0 else 1.0))
+
+
+CASES = [
+    # name, NX, NY, NZ, Lx, Ly, scheme, dt, nsteps
+    ("coral.equations.rbc_ra1e5", 16, 16, 17, 2 * np.pi, 2 * np.pi, "ARS_222", 1e-4, 6),
+    ("coral.equations.rbc_ra1e5", 32, 48, 17, 2.0, 3.0, "ARS_222", 1e-4, 5),
+    ("coral.equations.rrbc_ra5e7_ek1e-4", 32, 32, 25, 2 * np.pi, 2 * np.pi, "ARS_222", 2e-5, 5),
+    ("coral.equations.rbc_ra1e5", 16, 24, 17, 2 * np.pi, 2 * np.pi, "ARS_111", 1e-4, 4),
+    ("coral.equations.rbc_ra1e5", 16, 16, 17, 2 * np.pi, 2 * np.pi, "ARS_443", 1e-4, 3),
+    ("coral.equations.rrbc_ra5e7_ek1e-4", 16, 16, 21, 1.5, 1.5, "ARS_343", 1e-5, 3),
+    ("coral.equations.rbc_ra1e5", 16, 16, 17, 2 * np.pi, 2 * np.pi, "ARS_232", 1e-4, 3),
+    ("coral.equations.rbc_ra1e5", 16, 16, 17, 2 * np.pi, 2 * np.pi, "ARS_233", 1e-4, 3),
+    ("coral.equations.rbc_ra1e5", 16, 16, 17, 2 * np.pi, 2 * np.pi, "ARS_122", 1e-4, 3),
+]
+
+
+@pytest.mark.parametrize("case", CASES, ids=[f"{c[0].split('.')[-1]}-{c[1]}x{c[2]}x{c[3]}-{c[6]}" for c in CASES])
+def test_step_parity(solver_mod, case):
+    name, NX, NY, NZ, Lx, Ly, scheme, dt, nsteps = case
+    print("case", case, flush=True)
+    r = product_recipe(name)
+    k0, z0 = ic.rbc_initial_state(r, NX, NY, NZ, 1e-2, 6, 6, 3)
+    geo = O.Geometry(NX, NY, NZ, Lx, Ly)
+    ora = O.OracleSolver(r, geo, scheme)
+    ora.field, ora.zfield = [a.copy() for a in k0], [a.copy() for a in z0]
+    g = solver_mod.Solver(r, NX, NY, NZ, Lx, Ly, scheme=scheme)
+    try:
+        g.set_state(k0, z0)
+        for it in range(nsteps):
+            ora.step(dt)
+            g.march_forward(dt)
+        k, z = g.get_state()
+        assert rel(k[0], ora.field[0]) <= TOL
+        for a, b in zip(z, ora.zfield):
+            assert rel(a, b) <= TOL
+        assert abs(g.cfl_dt() / ora.cfl_dt - 1.0) <= 1e-10
+        # physical full variables
+        phys = ora.physical_full_variables()
+        for f in range(len(phys)):
+            assert rel(g.physical(f), phys[f]) <= TOL
+    finally:
+        g.close()
+
+
+def test_config1_geometry_parity(solver_mod):
+    """64^2 x 33 Fourier-Chebyshev, Ra = 1e5, dt = 1e-4 (the small CPU-runnable case), 10 steps."""
+    r = product_recipe("coral.equations.rbc_ra1e5")
+    NX, NZ, dt = 64, 33, 1e-4
+    k0, z0 = ic.rbc_initial_state(r, NX, NX, NZ, 1e-3, 8, 8, 1)
+    geo = O.Geometry(NX, NX, NZ, 2 * np.pi, 2 * np.pi)
+    ora = O.OracleSolver(r, geo, "ARS_222")
+    ora.field, ora.zfield = [a.copy() for a in k0], [a.copy() for a in z0]
+    with solver_mod.Solver(r, NX, NX, NZ, 2 * np.pi, 2 * np.pi) as g:
+        g.set_state(k0, z0)
+        for it in range(10):
+            ora.step(dt)
+            g.march_forward(dt)
+            if it in (0, 9):
+                k, z = g.get_state()
+                print("step", it, rel(k[0], ora.field[0]), flush=True)
+                assert rel(k[0], ora.field[0]) <= TOL
+                assert all(rel(a, b) <= TOL for a, b in zip(z, ora.zfield))
+
+
+def test_heat_equation_table_on_gpu(solver_mod, reference_numbers):
+    """The reference's heat-equation error table (error_report.py:4-15), through the GPU mean systems."""
+    r = recipe("heat_equation.coral.equations")
+    setup = reference_numbers["heat_equation_setup"]
+    table = reference_numbers["heat_equation_error_after_10_steps"]
+    NZ = 128
+    geo = O.Geometry(4, 4, NZ, 1.0, 1.0)
+    zg = geo.gauss_chebyshev_grid()
+    n = setup["modes"]["u"]
+    u0 = setup["amplitudes"]["u"] * np.sin((2 * n + 1) * (zg - 1) * np.pi / 2)
+    rate = setup["Prandtl"] * ((2 * n + 1) * np.pi / 2) ** 2
+    ops = O.SystemOps(r.zero_systems[0], NZ, 0.5, 1.0)
+    import scipy.fft as sfft
+    c = sfft.dct(u0, type=2)[:NZ] / geo.NZAA
+    c[0] *= 0.5
+    nc = ops.var_size[0]
+    zf = np.zeros(ops.total)
+    zf[ops.posP[:nc]] = np.linalg.solve(ops.stencils[0][:nc, :], c[:nc])
+    for scheme in ("ARS_111", "ARS_222", "ARS_443"):
+        for idx in (3, 5, 6):
+            dt, ref = table["dt"][idx], table[scheme][idx]
+            with solver_mod.Solver(r, 4, 4, NZ, 1.0, 1.0, scheme=scheme) as g:
+                kz = [np.zeros((2, 6, t), np.complex128) for t in g.totals]
+                g.set_state(kz, [zf, np.zeros(g.zero_totals[1]), np.zeros(g.zero_totals[2])])
+                p0 = g.physical(0)[0, 0, :].copy()
+                g.march_forward(dt, 10)
+                p1 = g.physical(0)[0, 0, :]
+            err = math.sqrt(np.sum((p1 - p0 * np.exp(-rate * 10 * dt)) ** 2) / np.sum(p0 ** 2))
+            print(scheme, dt, err, ref, flush=True)
+            assert abs(err / ref - 1.0) < 2e-3
+
+
+def test_banded_solve_matches_lapack(solver_mod):
+    rng = np.random.default_rng(9)
+    n, kl, ku, ns = 190, 16, 27, 6
+    mats, abs_ = [], []
+    for s in range(ns):
+        A = np.zeros((n, n), np.complex128)
+        for i in range(n):
+            for j in range(max(0, i - kl), min(n, i + ku + 1)):
+                A[i, j] = rng.normal() + 1j * rng.normal()
+        mats.append(A)
+        abs_.append(O.dense_to_lapack_band(A, kl, ku))
+    ab = np.stack(abs_)
+    b = rng.normal(size=(ns, n)) + 1j * rng.normal(size=(ns, n))
+    x, info = solver_mod.banded_solve(ab, kl, ku, b)
+    assert not info.any()
+    for s in range(ns):
+        lu, piv, _ = lap.zgbtrf(abs_[s].T.copy(), kl, ku)
+        xr, _ = lap.zgbtrs(lu, kl, ku, b[s], piv)
+        assert rel(x[s], xr) <= 1e-12
+
+
+def test_banded_solve_known_answer_and_singular(solver_mod, reference_numbers):
+    kat = reference_numbers["driver_algebra_5x5"]
+    A = np.zeros((5, 5), np.complex128)
+    for i, j, v in kat["entries_1based"]:
+        A[i - 1, j - 1] = v
+    ab = O.dense_to_lapack_band(A, 1, 2)[None]
+    x, info = solver_mod.banded_solve(ab, 1, 2, np.array([kat["rhs"]], np.complex128))
+    assert info[0] == 0
+    np.testing.assert_allclose(x[0].real, [-1, 2, -3, 4, -5], atol=1e-13)
+    S = A.copy()
+    S[:, 2] = 0.0   # singular: zgbtrf reports info = 3
+    _, info = solver_mod.banded_solve(O.dense_to_lapack_band(S, 1, 2)[None], 1, 2, np.ones((1, 5), np.complex128))
+    assert info[0] == 3
+
+
+def test_factorization_at_production_size(solver_mod):
+    """Full-size (256^2 x 129, rotating Ra = 5e7) operators: GPU LU solve vs LAPACK on sampled wavenumbers."""
+    r = product_recipe("coral.equations.rrbc_ra5e7_ek1e-4")
+    NX, NZ, dt = 256, 129, 2e-5
+    A = 1.0 - 1.0 / math.sqrt(2.0)
+    rng = np.random.default_rng(2)
+    abs_, bs, refs = [], [], []
+    for kxi, kyi in [(0, 1), (1, 0), (5, -7), (40, 33), (127, -127), (64, 100)]:
+        M, nl, nu = solver_mod.assembled_operator(r, NX, NX, NZ, 2 * np.pi, 2 * np.pi, 0, "mass", kxi, kyi)
+        L, _, _ = solver_mod.assembled_operator(r, NX, NX, NZ, 2 * np.pi, 2 * np.pi, 0, "stif", kxi, kyi)
+        Aop = M - dt * A * L
+        ab = O.dense_to_lapack_band(Aop, nl, nu)
+        b = rng.normal(size=Aop.shape[0]) + 1j * rng.normal(size=Aop.shape[0])
+        lu, piv, _ = lap.zgbtrf(ab.T.copy(), nl, nu)
+        xr, _ = lap.zgbtrs(lu, nl, nu, b, piv)
+        abs_.append(ab), bs.append(b), refs.append(xr)
+    x, info = solver_mod.banded_solve(np.stack(abs_), nl, nu, np.stack(bs))
+    assert not info.any()
+    for s in range(len(refs)):
+        assert rel(x[s], refs[s]) <= 1e-11
+
+
+def test_transforms_match_oracle(solver_mod):
+    r = product_recipe("coral.equations.rbc_ra1e5")
+    for NX, NY, NZ, Lx, Ly in [(16, 16, 17, 2 * np.pi, 2 * np.pi), (64, 48, 33, 2.0, 3.0)]:
+        geo = O.Geometry(NX, NY, NZ, Lx, Ly)
+        rng = np.random.default_rng(NX)
+        S = rng.normal(size=(geo.nkx, geo.nky, NZ)) + 1j * rng.normal(size=(geo.nkx, geo.nky, NZ))
+        S[geo.dealias] = 0.0
+        S[0, 0] = S[0, 0].real
+        for iy in range(geo.nky):
+            k = geo.ky_int[iy]
+            if k < 0 and not geo.dealias_y[iy]:
+                S[0, iy] = np.conj(S[0, list(geo.ky_int).index(-k)])
+        with solver_mod.Solver(r, NX, NY, NZ, Lx, Ly) as g:
+            P = g.bwd_transform(S)
+            Pr = O.spec_to_phys(S, geo)
+            assert rel(P, Pr) <= 1e-13
+            Q = P * P + 0.3 * P
+            Sg = g.fwd_transform(Q)
+            So = O.phys_to_spec(Q, geo)
+            keep = ~geo.dealias
+            assert rel(Sg[keep], So[keep]) <= 1e-13
+            assert not Sg[geo.dealias].any()
+
+
+def test_wavenumbers_bit_exact(solver_mod):
+    gold = json.load(open(os.path.join(GOLDEN, "wavenumbers.json")))
+    r = product_recipe("coral.equations.rbc_ra1e5")
+    for key in ("64x64", "96x64", "16x32"):
+        NX, NY = map(int, key.split("x"))
+        with solver_mod.Solver(r, NX, NY, 9, 2 * np.pi, 3.0) as g:
+            kx, ky, dx, dy = g.wavenumbers()
+        g0 = gold[key]
+        assert kx.tolist() == g0["kx_int"] and ky.tolist() == g0["ky_int"]
+        assert dx.astype(int).tolist() == g0["dealias_x"] and dy.astype(int).tolist() == g0["dealias_y"]
+
+
+def test_state_round_trip_and_masks(solver_mod):
+    r = product_recipe("coral.equations.rbc_ra1e5")
+    NX, NY, NZ = 32, 32, 17
+    rng = np.random.default_rng(1)
+    with solver_mod.Solver(r, NX, NY, NZ, 2 * np.pi, 2 * np.pi) as g:
+        t = g.totals[0]
+        a = rng.normal(size=(NX // 2, 3 * NY // 2, t)) + 1j * rng.normal(size=(NX // 2, 3 * NY // 2, t))
+        g.set_field("kxky", 0, a)
+        b = g.get_field("kxky", 0)
+        kx, ky, kept = ic.wavenumber_indices(NX, NY)
+        kept00 = kept.copy()
+        kept00[0, 0] = False
+        np.testing.assert_array_equal(b[kept00], a[kept00])
+        assert not b[~kept00].any()
+        zr = rng.normal(size=g.zero_totals[0])
+        g.set_field("zero", 0, zr)
+        np.testing.assert_array_equal(g.get_field("zero", 0), zr)
+
+
+def test_error_paths(solver_mod):
+    from coral_amd._lib import CoralError, check, lib
+    r = product_recipe("coral.equations.rbc_ra1e5")
+    with solver_mod.Solver(r, 16, 16, 17, 2 * np.pi, 2 * np.pi) as g:
+        with pytest.raises(CoralError, match="set_dt"):
+            check(lib.coral_amd_step(g.handle, 1), "coral_amd_step")
+        with pytest.raises(CoralError, match="positive"):
+            g.factorize_operators(-1.0)
+        with pytest.raises(ValueError):
+            g.set_field("kxky", 0, np.zeros((3, 3, 3)))
+        with pytest.raises(CoralError, match="full-variable index"):
+            g.physical(99)
+
+
+def test_production_size_properties(solver_mod):
+    """256^2 x 129 rotating Ra = 5e7: size-independent properties of the step --
+    determinism, Hermitian kx = 0 plane, finiteness and linearity in the small-amplitude limit."""
+    r = product_recipe("coral.equations.rrbc_ra5e7_ek1e-4")
+    NX, NZ, dt = 256, 129, 2e-5
+    k0, z0 = ic.rbc_initial_state(r, NX, NX, NZ, 1e-2, 32, 8, 2)
+    runs = []
+    for rep in range(2):
+        with solver_mod.Solver(r, NX, NX, NZ, 2 * np.pi, 2 * np.pi) as g:
+            g.set_state(k0, z0)
+            g.march_forward(dt, 3)
+            runs.append(g.get_state())
+            if rep == 0:
+                # linearity: the quadratic terms scale with the square of the amplitude
+                outs = []
+                for eps in (1e-9, 2e-9):
+                    g.set_state([a * (eps / 1e-2) for a in k0], z0)
+                    g.march_forward(dt, 1)
+                    outs.append(g.get_state()[0][0])
+                assert rel(outs[1], 2.0 * outs[0]) <= 1e-7
+    (ka, za), (kb, zb) = runs
+    np.testing.assert_array_equal(ka[0], kb[0])
+    assert np.isfinite(ka[0]).all()
+    kx, ky, kept = ic.wavenumber_indices(NX, NX)
+    plane = ka[0][0]
+    for iy in range(ky.size):
+        if ky[iy] < 0 and kept[0, iy]:
+            jy = list(ky).index(-ky[iy])
+            assert np.max(np.abs(plane[iy] - np.conj(plane[jy]))) <= 1e-12 * np.max(np.abs(plane))

@@ -1,0 +1,322 @@
+"""Pins of the oracle (and of the product's CPU-side assembly) against what the
+reference's own files hold and what the mathematics fixes.  No GPU needed."""
+import json
+import math
+import os
+
+import numpy as np
+import numpy.polynomial.chebyshev as C
+import pytest
+import scipy.linalg as sl
+import scipy.linalg.lapack as lap
+
+from conftest import GOLDEN, recipe
+from coral_amd import ic
+from oracle import coral_oracle as O
+
+
+# --- integer wavenumbers and dealiasing masks (bit exact) ---------------------
+
+def test_wavenumber_golden_vectors_are_fft_frequencies():
+    gold = json.load(open(os.path.join(GOLDEN, "wavenumbers.json")))
+    for key, g in gold.items():
+        NX, NY = map(int, key.split("x"))
+        fy = np.rint(np.fft.fftfreq(3 * NY // 2) * (3 * NY // 2)).astype(int)
+        fx = np.rint(np.fft.fftfreq(3 * NX // 2) * (3 * NX // 2)).astype(int)[: NX // 2]
+        assert g["ky_int"] == fy.tolist()
+        assert g["kx_int"] == fx.tolist()
+        assert g["dealias_y"] == [int(2 * abs(k) >= NY) for k in fy]
+
+
+def test_oracle_and_host_indexing_match_golden():
+    gold = json.load(open(os.path.join(GOLDEN, "wavenumbers.json")))
+    for key, g in gold.items():
+        NX, NY = map(int, key.split("x"))
+        geo = O.Geometry(NX, NY, 9, 2 * np.pi, 3.0)
+        assert geo.kx_int.tolist() == g["kx_int"] and geo.ky_int.tolist() == g["ky_int"]
+        assert geo.dealias_x.astype(int).tolist() == g["dealias_x"]
+        assert geo.dealias_y.astype(int).tolist() == g["dealias_y"]
+        kx, ky, kept = ic.wavenumber_indices(NX, NY)
+        assert kx.tolist() == g["kx_int"] and ky.tolist() == g["ky_int"]
+        assert (~kept).astype(int).tolist() == (np.array(g["dealias_x"])[:, None] | np.array(g["dealias_y"])[None, :]).tolist()
+
+
+# --- Chebyshev integration / multiplication matrices ---------------------------
+
+@pytest.mark.parametrize("center,gap", [(0.5, 1.0), (0.0, 2.0), (1.3, 0.7)])
+def test_integration_and_multiplication_match_numpy_chebyshev(center, gap):
+    N, pad = 24, 6
+    I, M = O.chebyshev_elementary_matrices(N, center, gap, pad)
+    rng = np.random.default_rng(1)
+    c = np.zeros(N + pad)
+    c[: N - 2] = rng.normal(size=N - 2)
+    # d/dz = (2/gap) d/dx, so int dz = (gap/2) int dx; T_0 coefficient is free
+    ref = (gap / 2.0) * C.chebint(c[: N - 2])
+    got = I @ c
+    np.testing.assert_allclose(got[1 : N - 1], ref[1 : N - 1], rtol=0, atol=1e-13)
+    # z p(z) with z = center + gap/2 x
+    ref = center * c[: N - 1] + (gap / 2.0) * C.chebmulx(c[: N - 2])
+    np.testing.assert_allclose((M @ c)[: N - 1], ref, atol=1e-13)
+
+
+def test_derivative_operator_differentiates_polynomials():
+    NZ, center, gap = 20, 0.5, 1.0
+    D = O.chebyshev_derivative_matrix(NZ, center, gap)
+    c = np.random.default_rng(2).normal(size=NZ)
+    np.testing.assert_allclose(D @ c, np.concatenate([(2.0 / gap) * C.chebder(c), [0.0]]), atol=1e-9)
+
+
+# --- Galerkin stencils: columns satisfy their boundary conditions -------------
+
+def _functionals(N):
+    n = np.arange(N, dtype=float)
+    e = lambda x, d: np.array([C.chebval(x, C.chebder(np.eye(N)[k], d)) if d else C.chebval(x, np.eye(N)[k])
+                               for k in range(N)])
+    return {"f+": e(1.0, 0), "f-": e(-1.0, 0), "d+": e(1.0, 1), "d-": e(-1.0, 1), "dd+": e(1.0, 2), "dd-": e(-1.0, 2)}
+
+
+@pytest.mark.parametrize("bc,conds", [(10, ["f+"]), (11, ["f-"]), (20, ["f+", "f-"]), (21, ["d+", "d-"]),
+                                      (22, ["f+", "d-"]), (23, ["d+", "f-"]), (40, ["f+", "f-", "d+", "d-"]),
+                                      (41, ["f+", "f-", "dd+", "dd-"]), (42, ["f+", "d+", "f-", "dd-"]),
+                                      (43, ["f+", "dd+", "f-", "d-"])])
+def test_stencil_columns_satisfy_boundary_conditions(bc, conds):
+    N = 18
+    S = O.galerkin_stencil(N, bc)
+    assert S.shape == (N, N - bc // 10)
+    F = _functionals(N)
+    for c in conds:
+        scale = max(1.0, float(np.max(np.abs(F[c]))))
+        assert np.max(np.abs(F[c] @ S)) < 1e-11 * scale, (bc, c)
+
+
+# --- structure of the RBC primitive-variable system ---------------------------
+
+@pytest.mark.parametrize("NZ", [17, 33, 49, 65, 129])
+def test_rbc_system_structure(NZ):
+    r = recipe("rbc_vanilla.coral.equations")
+    ops = O.SystemOps(r.kxky_systems[0], NZ, 0.5, 1.0)
+    mats = [ops.piece_matrix(p) for p in ops.sys.mass + ops.sys.stif]
+    assert ops.total == 6 * NZ - 8
+    assert O.bandwidth(mats) == (16, 27)
+    assert sorted(ops.posP.tolist()) == list(range(ops.total))
+    assert sorted(ops.posQ.tolist()) == list(range(ops.total))
+
+
+@pytest.mark.parametrize("name,NZ", [("rbc_vanilla.coral.equations", 17),
+                                     ("rrbc_julien.coral.equations.velocityPressure", 25),
+                                     ("stressfree_rb.coral.equations", 24),
+                                     ("heat_equation.coral.equations", 16)])
+def test_product_assembly_matches_oracle(native_lib, name, NZ):
+    from coral_amd.solver import assembled_operator
+    r = recipe(name)
+    Lx, Ly = 2.0, 3.0
+    for isys, sysr in enumerate(r.kxky_systems):
+        ops = O.SystemOps(sysr, NZ, 0.5, 1.0)
+        for which, pieces in (("mass", sysr.mass), ("stif", sysr.stif)):
+            groups = ops.grouped_pieces(pieces)
+            for kxi, kyi in [(0, 0), (1, 0), (3, -2), (7, 5)]:
+                kx, ky = kxi * 2 * np.pi / Lx, kyi * 2 * np.pi / Ly
+                ref = sum((((1j * kx) ** a) * ((1j * ky) ** b)) * m for (a, b), m in groups)
+                got, nl, nu = assembled_operator(r, 16, 16, NZ, Lx, Ly, isys, which, kxi, kyi)
+                assert np.max(np.abs(got - ref)) <= 1e-13 * max(1.0, np.max(np.abs(ref)))
+                mats = [ops.piece_matrix(p) for p in sysr.mass + sysr.stif]
+                assert (nl, nu) == O.bandwidth(mats)
+
+
+# --- stress-free Rayleigh-Benard growth rate ----------------------------------
+
+def test_stressfree_growth_rate(reference_numbers):
+    ref = reference_numbers["stressfree_rb_max_growth_rate"]
+    Ra, Pr, k = ref["Rayleigh"], ref["Prandtl"], ref["k_opt"]
+    # closed form of linearGrowthrate_numerical_vs_analytical.py:52-55 at the notebook's k_opt
+    D2 = k * k + math.pi ** 2
+    s = 0.5 * (-D2 * (Pr + 1) + math.sqrt((Pr + 1) ** 2 * D2 ** 2 - 4 * (Pr * D2 ** 2 - Ra * Pr * k * k / D2)))
+    assert abs(s - ref["s_max"]) < 1e-12
+    # discrete operator of the benchmark's equations at kx = k_opt
+    r = recipe("stressfree_rb.coral.equations")
+    ops = O.SystemOps(r.kxky_systems[0], 24, 0.5, 1.0)
+    mono = lambda a, b: ((1j * k) ** a) * (0j ** b)   # kx = k_opt, ky = 0
+    M = sum(mono(a, b) * m for (a, b), m in ops.grouped_pieces(ops.sys.mass))
+    L = sum(mono(a, b) * m for (a, b), m in ops.grouped_pieces(ops.sys.stif))
+    w = sl.eig(L, M, right=False)
+    w = w[np.isfinite(w)]
+    assert abs(np.max(w.real) - ref["s_max"]) < 1e-9
+
+
+# --- heat-equation error table (IMEX tableaux + mean operators + solve) --------
+
+def heat_error(scheme, dt, nsteps=10):
+    r = recipe("heat_equation.coral.equations")
+    setup = json.load(open(os.path.join(GOLDEN, "reference_numbers.json")))["heat_equation_setup"]
+    geo = O.Geometry(4, 4, 128, 1.0, 1.0)
+    z = geo.gauss_chebyshev_grid()
+    n = setup["modes"]["u"]
+    u0 = setup["amplitudes"]["u"] * np.sin((2 * n + 1) * (z - 1) * np.pi / 2)
+    rate = setup["Prandtl"] * ((2 * n + 1) * np.pi / 2) ** 2
+    s = O.OracleSolver(r, geo, scheme)
+    ops = s.zops[0]
+    import scipy.fft as sfft
+    c = sfft.dct(u0, type=2)[: geo.NZ] / geo.NZAA
+    c[0] *= 0.5
+    nc = ops.var_size[0]
+    s.zfield[0][ops.posP[:nc]] = np.linalg.solve(ops.stencils[0][:nc, :], c[:nc])
+    p0 = s.physical_full_variables()[0][0, 0, :].copy()
+    s.run(dt, nsteps)
+    p1 = s.physical_full_variables()[0][0, 0, :]
+    return float(np.sqrt(np.sum((p1 - p0 * np.exp(-rate * nsteps * dt)) ** 2) / np.sum(p0 ** 2)))
+
+
+@pytest.mark.parametrize("scheme", ["ARS_111", "ARS_222", "ARS_443"])
+def test_heat_equation_error_table(scheme, reference_numbers):
+    table = reference_numbers["heat_equation_error_after_10_steps"]
+    for idx in (3, 6):  # dt = 1e-4 and 1e-3
+        dt, ref = table["dt"][idx], table[scheme][idx]
+        err = heat_error(scheme, dt)
+        assert abs(err / ref - 1.0) < 2e-3, (scheme, dt, err, ref)
+
+
+# --- banded LU against LAPACK and the 5x5 known answer ------------------------
+
+def _random_band(rng, n, kl, ku, pivoting):
+    A = np.zeros((n, n), np.complex128)
+    for i in range(n):
+        for j in range(max(0, i - kl), min(n, i + ku + 1)):
+            A[i, j] = rng.normal() + 1j * rng.normal()
+        if not pivoting:
+            A[i, i] += 50.0
+    return A
+
+
+@pytest.mark.parametrize("pivoting", [True, False])
+def test_banded_lu_matches_lapack(pivoting):
+    rng = np.random.default_rng(3)
+    n, kl, ku, ns = 120, 16, 27, 6
+    mats = [_random_band(rng, n, kl, ku, pivoting) for _ in range(ns)]
+    ab = np.stack([O.dense_to_lapack_band(A, kl, ku) for A in mats])
+    b = rng.normal(size=(ns, n)) + 1j * rng.normal(size=(ns, n))
+    ipiv, info = O.zgbtrf_batch(ab, kl, ku)
+    assert not info.any()
+    x = b.copy()
+    O.zgbtrs_batch(ab, kl, ku, ipiv, x)
+    for s in range(ns):
+        lu, piv, inf = lap.zgbtrf(O.dense_to_lapack_band(mats[s], kl, ku).T.copy(), kl, ku)
+        np.testing.assert_array_equal(ipiv[s] - 1, piv)
+        xr, _ = lap.zgbtrs(lu, kl, ku, b[s], piv)
+        assert np.linalg.norm(x[s] - xr) <= 1e-12 * np.linalg.norm(xr)
+        assert np.linalg.norm(mats[s] @ x[s] - b[s]) <= 1e-10 * np.linalg.norm(b[s])
+
+
+def test_banded_lu_5x5_known_answer(reference_numbers):
+    kat = reference_numbers["driver_algebra_5x5"]
+    A = np.zeros((5, 5), np.complex128)
+    for i, j, v in kat["entries_1based"]:
+        A[i - 1, j - 1] = v
+    ab = O.dense_to_lapack_band(A, 1, 2)[None].copy()
+    ipiv, info = O.zgbtrf_batch(ab, 1, 2)
+    x = np.array(kat["rhs"], np.complex128)[None].copy()
+    O.zgbtrs_batch(ab, 1, 2, ipiv, x)
+    np.testing.assert_allclose(x[0].real, [-1, 2, -3, 4, -5], atol=1e-13)
+    assert ipiv[0, 0] == 2  # A(1,1) = 0 forces the first interchange
+
+
+# --- transforms ------------------------------------------------------------------
+
+def test_transform_normalisation_single_mode():
+    geo = O.Geometry(8, 8, 9, 2.0, 3.0)
+    x = np.arange(geo.NXAA) * geo.Lx / geo.NXAA
+    y = np.arange(geo.NYAA) * geo.Ly / geo.NYAA
+    z = geo.gauss_chebyshev_grid()
+    t = (z - geo.center) * 2.0 / geo.gap
+    for n in (0, 3):
+        for kxi, kyi in ((0, 0), (2, 0), (1, -3)):
+            kx, ky = kxi * geo.kx_box, kyi * geo.ky_box
+            P = (np.cos(kx * x)[:, None, None] * np.cos(ky * y)[None, :, None] if kyi else
+                 np.cos(kx * x)[:, None, None] * np.ones_like(y)[None, :, None]) * C.chebval(t, np.eye(n + 1)[n])[None, None, :]
+            S = O.phys_to_spec(P, geo)
+            expected = np.zeros_like(S)
+            iy = kyi % geo.NYAA
+            if kxi == 0 and kyi == 0:
+                expected[0, 0, n] = 1.0
+            elif kyi == 0:
+                expected[kxi, 0, n] = 0.5
+            else:
+                # cos(a)cos(b) = [cos(a+b) + cos(a-b)]/2 -> coefficients 1/4 at (kx, ky), (kx, -ky)
+                expected[kxi, iy, n] = 0.25
+                expected[kxi, (-kyi) % geo.NYAA, n] = 0.25
+            np.testing.assert_allclose(S, expected, atol=1e-14)
+
+
+def _random_spectral(geo, rng, hermitian=True):
+    S = rng.normal(size=(geo.nkx, geo.nky, geo.NZ)) + 1j * rng.normal(size=(geo.nkx, geo.nky, geo.NZ))
+    S[geo.dealias] = 0.0
+    if hermitian:
+        S[0, 0, :] = S[0, 0, :].real
+        for iy in range(geo.nky):
+            k = geo.ky_int[iy]
+            if k < 0 and not geo.dealias_y[iy]:
+                S[0, iy, :] = np.conj(S[0, list(geo.ky_int).index(-k), :])
+    return S
+
+
+def test_transform_round_trip():
+    geo = O.Geometry(12, 8, 7, 2.0, 1.0)
+    S = _random_spectral(geo, np.random.default_rng(4))
+    np.testing.assert_allclose(O.phys_to_spec(O.spec_to_phys(S, geo), geo), S, atol=1e-13)
+
+
+def test_dealiased_product_equals_truncated_convolution():
+    """Brute force on a tiny grid: the 3/2-padded pseudo-spectral product of two
+    band-limited fields equals their exact product truncated to the kept modes."""
+    geo = O.Geometry(4, 4, 5, 2 * np.pi, 2 * np.pi)
+    rng = np.random.default_rng(5)
+    A, B = _random_spectral(geo, rng), _random_spectral(geo, rng)
+    got = O.phys_to_spec(O.spec_to_phys(A, geo) * O.spec_to_phys(B, geo), geo)
+
+    def full(S):  # dict over all (kx, ky) with Hermitian extension
+        out = {}
+        for ix in range(geo.nkx):
+            for iy in range(geo.nky):
+                if geo.dealias[ix, iy]:
+                    continue
+                kx, ky = int(geo.kx_int[ix]), int(geo.ky_int[iy])
+                out[(kx, ky)] = S[ix, iy]
+                if kx > 0:
+                    out[(-kx, -ky)] = np.conj(S[ix, iy])
+        return out
+
+    fa, fb = full(A), full(B)
+    exp = np.zeros_like(got)
+    for (ax, ay), ca in fa.items():
+        for (bx, by), cb in fb.items():
+            kx, ky = ax + bx, ay + by
+            if kx < 0 or kx >= geo.nkx or 2 * abs(ky) >= geo.NY:
+                continue
+            iy = ky % geo.NYAA
+            # T_m T_n = (T_{m+n} + T_{|m-n|}) / 2
+            prod = np.zeros(2 * geo.NZ, np.complex128)
+            for m in range(geo.NZ):
+                for n in range(geo.NZ):
+                    v = ca[m] * cb[n] / 2.0
+                    prod[m + n] += v
+                    prod[abs(m - n)] += v
+            exp[kx, iy] += prod[: geo.NZ]
+    keep = ~geo.dealias   # the reference zeroes the 2/3-rule modes of every NL term
+    np.testing.assert_allclose(got[keep], exp[keep], atol=1e-12)
+
+
+# --- seeded initial condition ---------------------------------------------------
+
+def test_initial_condition_properties():
+    r = recipe("rbc_vanilla.coral.equations")
+    k1, z1 = ic.rbc_initial_state(r, 32, 32, 17, 1e-3, 8, 8, 11)
+    k2, _ = ic.rbc_initial_state(r, 32, 32, 17, 1e-3, 8, 8, 11)
+    np.testing.assert_array_equal(k1[0], k2[0])
+    s = k1[0]
+    kx, ky, kept = ic.wavenumber_indices(32, 32)
+    assert not s[~kept].any() and not s[0, 0].any()
+    for iy in range(ky.size):
+        if ky[iy] < 0 and kept[0, iy]:
+            np.testing.assert_array_equal(s[0, iy], np.conj(s[0, list(ky).index(-ky[iy])]))
+    norm = math.sqrt(np.sum(np.abs(s[0]) ** 2) + 2 * np.sum(np.abs(s[1:]) ** 2))
+    assert abs(norm - 1e-3) < 1e-15
+    assert all(not z.any() for z in z1)

@@ -233,7 +233,7 @@ static SystemTables build_system(const coral_amd_system &s, int NZ, double cente
         if (it == im_cache.end()) it = im_cache.emplace(key, cheb_IM(NZ, center, gap, iz, mz)).first;
         return it->second;
     };
-    // dense shuffled matrices per (kx, ky) group
+    // dense shuffled matrices per (kx, ky) exponent pair
     auto assemble_group = [&](const coral_amd_piece *pieces, int n, std::vector<std::pair<int, int>> &keys,
                               std::vector<Dense> &mats) {
         std::map<std::pair<int, int>, int> index;
@@ -243,6 +243,7 @@ static SystemTables build_system(const coral_amd_system &s, int NZ, double cente
                 throw std::runtime_error("piece refers to a variable or equation outside its system");
             if (p.iz > s.eqn_orders[p.eqn])
                 throw std::runtime_error("Iz exponent exceeds the equation order");
+            if (p.iz < 0 || p.mz < 0 || p.kx < 0 || p.ky < 0) throw std::runtime_error("negative operator exponent");
             if (!kxky && (p.kx || p.ky))
                 throw std::runtime_error("horizontal derivative in a mean-field equation");
             auto key = std::make_pair(int(p.kx), int(p.ky));
@@ -267,7 +268,6 @@ static SystemTables build_system(const coral_amd_system &s, int NZ, double cente
                 }
             }
         }
-        // deterministic order of the groups
         std::vector<int> order(keys.size());
         for (size_t i = 0; i < order.size(); ++i) order[i] = int(i);
         std::sort(order.begin(), order.end(), [&](int a, int b) { return keys[a] < keys[b]; });
@@ -280,9 +280,10 @@ static SystemTables build_system(const coral_amd_system &s, int NZ, double cente
         keys.swap(k2);
         mats.swap(m2);
     };
+    std::vector<std::pair<int, int>> mkeys, skeys;
     std::vector<Dense> mm, sm;
-    assemble_group(s.mass, s.n_mass, t.mass_keys, mm);
-    assemble_group(s.stif, s.n_stif, t.stif_keys, sm);
+    assemble_group(s.mass, s.n_mass, mkeys, mm);
+    assemble_group(s.stif, s.n_stif, skeys, sm);
     // band of the union of structures (z_csr2dia, sparse_formats_z.f90:812-829)
     int nl = 0, nu = 0;
     for (auto *v : {&mm, &sm})
@@ -296,18 +297,67 @@ static SystemTables build_system(const coral_amd_system &s, int NZ, double cente
     t.nl = nl;
     t.nu = nu;
     const int w = nl + nu + 1;
-    auto to_band = [&](const std::vector<Dense> &mats, std::vector<double> &band) {
-        band.assign(mats.size() * size_t(t.total) * w, 0.0);
-        for (size_t g = 0; g < mats.size(); ++g)
-            for (int i = 0; i < t.total; ++i)
+    auto build_op = [&](const std::vector<std::pair<int, int>> &keys, const std::vector<Dense> &mats, OpTables &op) {
+        // merge exponent pairs whose matrices are identical
+        std::vector<int> rep;                       // representative matrix of each group
+        std::vector<std::vector<std::pair<int, int>>> monos;
+        for (size_t k = 0; k < keys.size(); ++k) {
+            int found = -1;
+            for (size_t g = 0; g < rep.size(); ++g)
+                if (mats[rep[g]].a == mats[k].a) {
+                    found = int(g);
+                    break;
+                }
+            if (found < 0) {
+                rep.push_back(int(k));
+                monos.push_back({keys[k]});
+            } else {
+                monos[found].push_back(keys[k]);
+            }
+        }
+        op.ngroups = int(rep.size());
+        op.mono_off.assign(1, 0);
+        for (auto &ml : monos) {
+            for (auto &m : ml) {
+                op.mono_a.push_back(m.first);
+                op.mono_b.push_back(m.second);
+            }
+            op.mono_off.push_back(int(op.mono_a.size()));
+        }
+        op.band.assign(rep.size() * size_t(t.total) * w, 0.0);
+        for (size_t g = 0; g < rep.size(); ++g) {
+            const Dense &A = mats[rep[g]];
+            int ew = 1;
+            for (int i = 0; i < t.total; ++i) {
+                int cnt = 0;
                 for (int d = -nl; d <= nu; ++d) {
                     int j = i + d;
                     if (j < 0 || j >= t.total) continue;
-                    band[(g * t.total + i) * w + d + nl] = mats[g](i, j);
+                    op.band[(g * t.total + i) * w + d + nl] = A(i, j);
+                    cnt += A(i, j) != 0.0;
                 }
+                ew = std::max(ew, cnt);
+            }
+            op.ell_off.push_back(int(op.ell_col.size()));
+            op.ell_w.push_back(ew);
+            for (int i = 0; i < t.total; ++i) {
+                int cnt = 0;
+                for (int d = -nl; d <= nu; ++d) {
+                    int j = i + d;
+                    if (j < 0 || j >= t.total || A(i, j) == 0.0) continue;
+                    op.ell_col.push_back(j);
+                    op.ell_val.push_back(A(i, j));
+                    ++cnt;
+                }
+                for (; cnt < ew; ++cnt) {
+                    op.ell_col.push_back(i);
+                    op.ell_val.push_back(0.0);
+                }
+            }
+        }
     };
-    to_band(mm, t.mass_band);
-    to_band(sm, t.stif_band);
+    build_op(mkeys, mm, t.mass);
+    build_op(skeys, sm, t.stif);
     // NL terms, sorted by equation
     std::vector<coral_amd_nlterm> terms(s.nl, s.nl + s.n_nl);
     std::stable_sort(terms.begin(), terms.end(),

@@ -41,15 +41,27 @@ struct NLTable {
     int woff;  // offset into SystemTables::nl_w
 };
 
+// A linear operator as groups of real matrices, each multiplied by a sum of
+// horizontal monomials: Op(kx,ky) = sum_g (sum_{m in g} (i kx)^a_m (i ky)^b_m) B_g.
+// Pieces whose matrices are identical share one group (e.g. the kx^2 and ky^2
+// parts of a Laplacian).
+struct OpTables {
+    int ngroups = 0;
+    std::vector<int> mono_off;         // [ngroups+1] into mono_a/mono_b
+    std::vector<int> mono_a, mono_b;
+    std::vector<double> band;          // [g][total][w]: B_g(i, i+d) at (g*total + i)*w + d + nl
+    std::vector<int> ell_w, ell_off;   // per group: entries per row, offset of the group's block
+    std::vector<int> ell_col;          // [off + i*ell_w + e] column, padded with i
+    std::vector<double> ell_val;       // padded with 0
+};
+
 struct SystemTables {
     bool kxky = true;
     int nvars = 0, total = 0, nl = 0, nu = 0;
     std::vector<int> var_size, eqn_size, var_first, eqn_first, eqn_orders, bc_codes;
     std::vector<int> posP, posQ;          // var-order / eqn-order index -> interleaved position
     std::vector<int> pos_eqn, pos_row;    // interleaved position -> (equation, row in equation)
-    // operator pieces grouped by (kx, ky) exponents; band B(i, i+d) at [(g*total + i)*w + d + nl]
-    std::vector<std::pair<int, int>> mass_keys, stif_keys;
-    std::vector<double> mass_band, stif_band;
+    OpTables mass, stif;
     std::vector<NLTable> nl_terms;        // sorted by equation
     std::vector<int> nl_eqn_begin;        // [nvars+1]
     std::vector<double> nl_w;

@@ -62,8 +62,14 @@ struct SysDev {
     DevBuf<double2> Lf, Uf;
     DevBuf<unsigned char> piv;
     DevBuf<int> info;
-    DevBuf<double> mband, sband;
-    DevBuf<int> mkx, mky, skx, sky;
+    struct OpBufs {
+        DevBuf<int> mono_off, mono_a, mono_b, ell_w, ell_off, ell_col;
+        DevBuf<double> band, ell_val;
+        int ngroups = 0;
+        OpDev dev() const {
+            return OpDev{ngroups, mono_off.p, mono_a.p, mono_b.p, band.p, ell_w.p, ell_off.p, ell_col.p, ell_val.p};
+        }
+    } mass, stif;
     DevBuf<int> pos_eqn, pos_row, eqn_begin, t_qvar, t_kx, t_ky, t_w, t_order, t_woff;
     DevBuf<double> t_dsca, wv;
 };
@@ -189,23 +195,19 @@ void fill_system_device(SysDev &d, int nstages) {
         d.Khat.back()->alloc(n);
         d.Kstd.back()->alloc(n);
     }
-    d.mband.upload(t.mass_band);
-    d.sband.upload(t.stif_band);
-    std::vector<int> a, b;
-    for (auto &k : t.mass_keys) {
-        a.push_back(k.first);
-        b.push_back(k.second);
-    }
-    d.mkx.upload(a);
-    d.mky.upload(b);
-    a.clear();
-    b.clear();
-    for (auto &k : t.stif_keys) {
-        a.push_back(k.first);
-        b.push_back(k.second);
-    }
-    d.skx.upload(a);
-    d.sky.upload(b);
+    auto up = [](SysDev::OpBufs &b, const OpTables &o) {
+        b.ngroups = o.ngroups;
+        b.mono_off.upload(o.mono_off);
+        b.mono_a.upload(o.mono_a);
+        b.mono_b.upload(o.mono_b);
+        b.band.upload(o.band);
+        b.ell_w.upload(o.ell_w);
+        b.ell_off.upload(o.ell_off);
+        b.ell_col.upload(o.ell_col);
+        b.ell_val.upload(o.ell_val);
+    };
+    up(d.mass, t.mass);
+    up(d.stif, t.stif);
     d.pos_eqn.upload(t.pos_eqn);
     d.pos_row.upload(t.pos_row);
     d.eqn_begin.upload(t.nl_eqn_begin);
@@ -251,8 +253,6 @@ void fill_zero(hipStream_t st, double2 *p, size_t n);
 
 void factorize(coral_amd_ctx &c, SysDev &d, double dt) {
     SystemTables &t = *d.t;
-    if (t.mass_keys.size() > 8 || t.stif_keys.size() > 8)
-        throw std::runtime_error("more than 8 distinct (kx, ky) monomials in one operator");
     const int kv = t.nl + t.nu, ldab = 2 * t.nl + t.nu + 1;
     const size_t per = size_t(t.total) * ldab;
     // bound the working band to ~1 GiB per launch
@@ -262,14 +262,8 @@ void factorize(coral_amd_ctx &c, SysDev &d, double dt) {
     work.alloc(size_t(chunk) * per);
     for (int s0 = 0; s0 < d.nsys; s0 += chunk) {
         FactorArgs a{};
-        a.mband = d.mband.p;
-        a.sband = d.sband.p;
-        a.mkx = d.mkx.p;
-        a.mky = d.mky.p;
-        a.skx = d.skx.p;
-        a.sky = d.sky.p;
-        a.nmg = int(t.mass_keys.size());
-        a.nsg = int(t.stif_keys.size());
+        a.mass = d.mass.dev();
+        a.stif = d.stif.dev();
         a.nl = t.nl;
         a.nu = t.nu;
         a.total = t.total;
@@ -300,12 +294,7 @@ void band_apply(coral_amd_ctx &c, SysDev &d, bool mass, const double2 *x, double
                 const std::vector<std::pair<const double2 *, double>> &vecs) {
     SystemTables &t = *d.t;
     BandApplyArgs a{};
-    a.band = mass ? d.mband.p : d.sband.p;
-    a.gkx = mass ? d.mkx.p : d.skx.p;
-    a.gky = mass ? d.mky.p : d.sky.p;
-    a.ngroups = int(mass ? t.mass_keys.size() : t.stif_keys.size());
-    a.nl = t.nl;
-    a.nu = t.nu;
+    a.op = mass ? d.mass.dev() : d.stif.dev();
     a.total = t.total;
     a.nsys = d.nsys;
     a.x = x;
@@ -867,18 +856,19 @@ int coral_amd_assembled_operator(const coral_amd_problem *prob, int32_t isys, in
         if (nl) *nl = t.nl;
         if (nu) *nu = t.nu;
         if (!out) return 0;
-        const auto &keys = which == 0 ? t.mass_keys : t.stif_keys;
-        const auto &band = which == 0 ? t.mass_band : t.stif_band;
+        const OpTables &op = which == 0 ? t.mass : t.stif;
         const int n = t.total, w = t.nl + t.nu + 1;
         std::memset(out, 0, sizeof(double) * 2 * size_t(n) * n);
         const double kx = kx_int * P.kx_box, ky = ky_int * P.ky_box;
-        for (size_t g = 0; g < keys.size(); ++g) {
-            double2 cf = monomial(keys[g].first, keys[g].second, kx, ky);
+        for (int g = 0; g < op.ngroups; ++g) {
+            double2 cf = make_double2(0.0, 0.0);
+            for (int m = op.mono_off[g]; m < op.mono_off[g + 1]; ++m)
+                cf = cadd(cf, monomial(op.mono_a[m], op.mono_b[m], kx, ky));
             for (int i = 0; i < n; ++i)
                 for (int d = -t.nl; d <= t.nu; ++d) {
                     int j = i + d;
                     if (j < 0 || j >= n) continue;
-                    double b = band[(g * n + i) * w + d + t.nl];
+                    double b = op.band[(size_t(g) * n + i) * w + d + t.nl];
                     out[2 * (size_t(i) * n + j)] += b * cf.x;
                     out[2 * (size_t(i) * n + j) + 1] += b * cf.y;
                 }

@@ -170,64 +170,90 @@ struct FFTPlanDev {
     const double2 *tw;
 };
 
-template <int MAXB>
-__device__ void fft_lds(double2 *buf, int LS, int lines, const FFTPlanDev &p, double sign) {
-    const int N = p.N;
-    int Ns = 1;
-    for (int st = 0; st < p.nrad; ++st) {
-        const int R = p.radix[st];
-        const int nb = N / R;
-        const int total = nb * lines;
-        const int step = N / (Ns * R);
-        double2 v[MAXB][4];
-        int cnt = 0;
-        for (int b = threadIdx.x; b < total && cnt < MAXB; b += blockDim.x, ++cnt) {
-            int line = b / nb, j = b - line * nb;
-            double2 *L = buf + line * LS;
-            int jm = j % Ns;
+// R-point DFT in registers, exp(sign 2 pi i r k / R)
+template <int R>
+__device__ inline void dft_small(double2 *y, double sign) {
+    if (R == 2) {
+        double2 a = y[0], c = y[1];
+        y[0] = cadd(a, c);
+        y[1] = csub(a, c);
+    } else if (R == 3) {
+        const double c1 = -0.5, s1 = sign * 0.86602540378443864676;
+        double2 a = y[0], b1 = y[1], b2 = y[2];
+        double2 t = cadd(b1, b2), d = csub(b1, b2);
+        y[0] = cadd(a, t);
+        double2 m = make_double2(a.x + c1 * t.x, a.y + c1 * t.y);
+        y[1] = make_double2(m.x - s1 * d.y, m.y + s1 * d.x);
+        y[2] = make_double2(m.x + s1 * d.y, m.y - s1 * d.x);
+    } else {  // R == 4
+        double2 a = y[0], b = y[1], c = y[2], d = y[3];
+        double2 t0 = cadd(a, c), t1 = csub(a, c), t2 = cadd(b, d), t3 = csub(b, d);
+        double2 wt3 = make_double2(-sign * t3.y, sign * t3.x);
+        y[0] = cadd(t0, t2);
+        y[2] = csub(t0, t2);
+        y[1] = cadd(t1, wt3);
+        y[3] = csub(t1, wt3);
+    }
+}
+
+// one Stockham stage over `lines` lines; every thread owns at most MAXB butterflies
+// (slots are unrolled so the butterfly registers never spill)
+template <int R, int MAXB>
+__device__ inline void fft_stage(double2 *buf, int LS, int lines, int N, int Ns, const double2 *tw, double sign) {
+    const int nb = N / R;
+    const int total = nb * lines;
+    const int step = N / (Ns * R);
+    double2 v[MAXB][R];
+#pragma unroll
+    for (int c = 0; c < MAXB; ++c) {
+        const int b = threadIdx.x + c * blockDim.x;
+        if (b < total) {
+            const int line = b / nb, j = b - line * nb;
+            const double2 *L = buf + line * LS;
+            const int jm = j % Ns;
+#pragma unroll
             for (int r = 0; r < R; ++r) {
                 double2 x = L[j + r * nb];
                 if (r > 0) {
-                    double2 w = p.tw[jm * r * step];  // exp(-2 pi i k / N)
-                    if (sign > 0) w.y = -w.y;         // inverse transform: conjugate twiddle
+                    double2 w = tw[jm * r * step];  // exp(-2 pi i k / N)
+                    if (sign > 0) w.y = -w.y;
                     x = cmul(x, w);
                 }
-                v[cnt][r] = x;
+                v[c][r] = x;
             }
-            double2 *y = v[cnt];
-            if (R == 2) {
-                double2 a = y[0], c = y[1];
-                y[0] = cadd(a, c);
-                y[1] = csub(a, c);
-            } else if (R == 3) {
-                const double c1 = -0.5, s1 = sign * 0.86602540378443864676;
-                double2 a = y[0], b1 = y[1], b2 = y[2];
-                double2 t = cadd(b1, b2), d = csub(b1, b2);
-                y[0] = cadd(a, t);
-                double2 m = make_double2(a.x + c1 * t.x, a.y + c1 * t.y);
-                // i*s1*d = (-s1*d.y, s1*d.x)
-                y[1] = make_double2(m.x - s1 * d.y, m.y + s1 * d.x);
-                y[2] = make_double2(m.x + s1 * d.y, m.y - s1 * d.x);
-            } else {  // R == 4
-                double2 a = y[0], b = y[1], c = y[2], d = y[3];
-                double2 t0 = cadd(a, c), t1 = csub(a, c), t2 = cadd(b, d), t3 = csub(b, d);
-                // w = sign*i ; w*t3 = (-sign*t3.y, sign*t3.x)
-                double2 wt3 = make_double2(-sign * t3.y, sign * t3.x);
-                y[0] = cadd(t0, t2);
-                y[2] = csub(t0, t2);
-                y[1] = cadd(t1, wt3);
-                y[3] = csub(t1, wt3);
-            }
+            dft_small<R>(v[c], sign);
         }
-        __syncthreads();
-        cnt = 0;
-        for (int b = threadIdx.x; b < total && cnt < MAXB; b += blockDim.x, ++cnt) {
-            int line = b / nb, j = b - line * nb;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < MAXB; ++c) {
+        const int b = threadIdx.x + c * blockDim.x;
+        if (b < total) {
+            const int line = b / nb, j = b - line * nb;
             double2 *L = buf + line * LS;
-            int base = (j / Ns) * Ns * R + (j % Ns);
-            for (int r = 0; r < R; ++r) L[base + r * Ns] = v[cnt][r];
+            const int base = (j / Ns) * Ns * R + (j % Ns);
+#pragma unroll
+            for (int r = 0; r < R; ++r) L[base + r * Ns] = v[c][r];
         }
-        __syncthreads();
+    }
+    __syncthreads();
+}
+
+// Stockham FFT of `lines` lines held in LDS (line stride LS), radix 2/3/4.
+// Stage: in index j + r*N/R, out index (j/Ns)*Ns*R + j%Ns + r*Ns, twiddle
+// exp(sign*2*pi*i*(j%Ns)*r/(Ns*R)).  sign = -1 forward, +1 inverse.  With
+// blockDim = 256 and lines*N <= 1536, MAXB = 3 slots hold every butterfly.
+template <int MAXB>
+__device__ void fft_lds(double2 *buf, int LS, int lines, const FFTPlanDev &p, double sign) {
+    int Ns = 1;
+    for (int st = 0; st < p.nrad; ++st) {
+        const int R = p.radix[st];
+        if (R == 4)
+            fft_stage<4, MAXB>(buf, LS, lines, p.N, Ns, p.tw, sign);
+        else if (R == 2)
+            fft_stage<2, MAXB>(buf, LS, lines, p.N, Ns, p.tw, sign);
+        else
+            fft_stage<3, MAXB>(buf, LS, lines, p.N, Ns, p.tw, sign);
         Ns *= R;
     }
 }
@@ -242,7 +268,7 @@ static FFTPlanDev to_dev(const FFTPlan &p) {
 }
 
 // lines per workgroup of the y passes: lines*N <= 1536 keeps the butterflies of
-// every stage (lines*N/2 at most) within fft_lds<6> capacity (6 per thread x 256)
+// every stage (lines*N/2 at most) within fft_lds<3> capacity (3 per thread x 256)
 static int lines_for(int N) {
     int lines = 1;
     while (lines * 2 * N <= 1536) lines *= 2;
@@ -267,7 +293,7 @@ __global__ __launch_bounds__(256) void k_ybwd(FFTPlanDev p, const double2 *__res
         sbuf[l * LS + slot] = src[e];
     }
     __syncthreads();
-    fft_lds<6>(sbuf, LS, lines, p, +1.0);
+    fft_lds<3>(sbuf, LS, lines, p, +1.0);
     double2 *dst = G + bz * N * nkx;
     for (int e = threadIdx.x; e < N * nl; e += blockDim.x) {
         int y = e / nl, l = e - y * nl;
@@ -275,9 +301,16 @@ __global__ __launch_bounds__(256) void k_ybwd(FFTPlanDev p, const double2 *__res
     }
 }
 
+static bool fft_fits(int lines, int N) {
+    if (lines * N <= 1536) return true;
+    fprintf(stderr, "coral_amd: FFT length %d exceeds the LDS FFT capacity\n", N);
+    return false;
+}
+
 void launch_ybwd(hipStream_t st, const FFTPlan &py, const double2 *Fz, double2 *G, int nbatch, int NZAA, int nkx,
                  int nky, int NY) {
     int lines = lines_for(py.N);
+    if (!fft_fits(lines, py.N)) return;
     int ntile = (nkx + lines - 1) / lines;
     size_t shmem = size_t(lines) * (py.N + 1) * sizeof(double2);
     hipLaunchKernelGGL(k_ybwd, dim3(ntile * nbatch * NZAA), dim3(256), shmem, st, to_dev(py), Fz, G, nkx, nky, NY,
@@ -300,7 +333,7 @@ __global__ __launch_bounds__(256) void k_yfwd(FFTPlanDev p, const double2 *__res
         sbuf[l * LS + y] = (l < nl) ? src[(size_t)y * nkx + ix0 + l] : make_double2(0.0, 0.0);
     }
     __syncthreads();
-    fft_lds<6>(sbuf, LS, lines, p, -1.0);
+    fft_lds<3>(sbuf, LS, lines, p, -1.0);
     double2 *dst = Hy + bz * nkx * nky + (size_t)ix0 * nky;
     for (int e = threadIdx.x; e < nl * nky; e += blockDim.x) {
         int l = e / nky, jy = e - l * nky;
@@ -312,6 +345,7 @@ __global__ __launch_bounds__(256) void k_yfwd(FFTPlanDev p, const double2 *__res
 void launch_yfwd(hipStream_t st, const FFTPlan &py, const double2 *H, double2 *Hy, int nbatch, int NZAA, int nkx,
                  int nky, int NY) {
     int lines = lines_for(py.N);
+    if (!fft_fits(lines, py.N)) return;
     int ntile = (nkx + lines - 1) / lines;
     size_t shmem = size_t(lines) * (py.N + 1) * sizeof(double2);
     hipLaunchKernelGGL(k_yfwd, dim3(ntile * nbatch * NZAA), dim3(256), shmem, st, to_dev(py), H, Hy, nkx, nky, NY,
@@ -348,7 +382,7 @@ __global__ __launch_bounds__(256) void k_xpass(FFTPlanDev p, XPassArgs a, int LX
             }
         }
         __syncthreads();
-        fft_lds<6>(fb, LS, LX, p, +1.0);
+        fft_lds<3>(fb, LS, LX, p, +1.0);
         for (int e = threadIdx.x; e < LX * N; e += blockDim.x) {
             int l = e / N, x = e - l * N;
             phys[(f * LX + l) * N + x] = fb[l * LS + x].x;
@@ -385,7 +419,7 @@ __global__ __launch_bounds__(256) void k_xpass(FFTPlanDev p, XPassArgs a, int LX
             fb[l * LS + x] = make_double2(va * vb, 0.0);
         }
         __syncthreads();
-        fft_lds<6>(fb, LS, LX, p, -1.0);
+        fft_lds<3>(fb, LS, LX, p, -1.0);
         double2 *h = a.H + ((size_t)q * nlines + l0) * nkx;
         for (int e = threadIdx.x; e < nl * nkx; e += blockDim.x) {
             int l = e / nkx, k = e - l * nkx;
@@ -400,11 +434,11 @@ static size_t xpass_lds(int LX, int N, int nfull) {
 }
 
 void launch_xpass(hipStream_t st, const FFTPlan &px, const XPassArgs &a) {
-    // lines per workgroup: FFT butterflies must fit fft_lds<6> (LX*N/2 <= 6*256)
+    // lines per workgroup: FFT butterflies must fit fft_lds<3> (LX*N/2 <= 3*256)
     // and the workgroup's LDS must stay within 64 KiB
     int LX = 4;
-    while (LX > 1 && (LX * px.N > 3072 || xpass_lds(LX, px.N, a.nfull) > 65536)) LX /= 2;
-    if (px.N > 3072 || xpass_lds(LX, px.N, a.nfull) > 65536) {
+    while (LX > 1 && (LX * px.N > 1536 || xpass_lds(LX, px.N, a.nfull) > 65536)) LX /= 2;
+    if (LX * px.N > 1536 || xpass_lds(LX, px.N, a.nfull) > 65536) {
         fprintf(stderr, "coral_amd: x-pass does not fit (N=%d, nfull=%d)\n", px.N, a.nfull);
         return;
     }
@@ -436,7 +470,7 @@ __global__ __launch_bounds__(256) void k_xbwd(FFTPlanDev p, const double2 *__res
         }
     }
     __syncthreads();
-    fft_lds<6>(sbuf, LS, LX, p, +1.0);
+    fft_lds<3>(sbuf, LS, LX, p, +1.0);
     double *o = phys + ((size_t)b * nlines + l0) * N;
     for (int e = threadIdx.x; e < nl * N; e += blockDim.x) {
         int l = e / N, x = e - l * N;
@@ -458,7 +492,7 @@ __global__ __launch_bounds__(256) void k_xfwd(FFTPlanDev p, const double *__rest
         sbuf[l * LS + x] = make_double2(l < nl ? src[e] : 0.0, 0.0);
     }
     __syncthreads();
-    fft_lds<6>(sbuf, LS, LX, p, -1.0);
+    fft_lds<3>(sbuf, LS, LX, p, -1.0);
     double2 *h = H + ((size_t)b * nlines + l0) * nkx;
     for (int e = threadIdx.x; e < nl * nkx; e += blockDim.x) {
         int l = e / nkx, k = e - l * nkx;
@@ -475,6 +509,7 @@ static int xlines_for(int N) {
 void launch_xbwd(hipStream_t st, const FFTPlan &px, const double2 *G, double *phys, int nbatch, long long nlines,
                  int nkx) {
     int LX = xlines_for(px.N);
+    if (!fft_fits(LX, px.N)) return;
     size_t shmem = size_t(LX) * (px.N + 1) * sizeof(double2);
     hipLaunchKernelGGL(k_xbwd, dim3(blocks_for(nlines, LX), nbatch), dim3(256), shmem, st, to_dev(px), G, phys, nlines,
                        nkx, LX);
@@ -484,6 +519,7 @@ void launch_xbwd(hipStream_t st, const FFTPlan &px, const double2 *G, double *ph
 void launch_xfwd(hipStream_t st, const FFTPlan &px, const double *phys, double2 *H, int nbatch, long long nlines,
                  int nkx) {
     int LX = xlines_for(px.N);
+    if (!fft_fits(LX, px.N)) return;
     size_t shmem = size_t(LX) * (px.N + 1) * sizeof(double2);
     hipLaunchKernelGGL(k_xfwd, dim3(blocks_for(nlines, LX), nbatch), dim3(256), shmem, st, to_dev(px), phys, H, nlines,
                        nkx, LX);
@@ -553,28 +589,24 @@ void launch_nl(hipStream_t st, const NLArgs &a) {
 // banded operator application + AXPYs (mass_field_to_rhs, add_K_*_to_rhs,
 // stif_aux_to_K_std: PL_IMEX_timestepping_marching.f90:93-171)
 // ---------------------------------------------------------------------------
-__global__ void k_band_apply(BandApplyArgs a) {
+__global__ __launch_bounds__(256) void k_band_apply(BandApplyArgs a) {
     long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= (long long)a.nsys * a.total) return;
     const int s = int(idx / a.total), i = int(idx % a.total);
-    const int w = a.nl + a.nu + 1;
     const double2 *xs = a.x + (size_t)s * a.total;
-    const int dlo = max(-a.nl, -i), dhi = min(a.nu, a.total - 1 - i);
+    const double kx = a.km.kx(s), ky = a.km.ky(s);
     double2 y = make_double2(0.0, 0.0);
-    for (int g = 0; g < a.ngroups; ++g) {
-        const double *row = a.band + ((size_t)g * a.total + i) * w + a.nl;
+    for (int g = 0; g < a.op.ngroups; ++g) {
+        const int W = a.op.ell_w[g];
+        const int base = a.op.ell_off[g] + i * W;
         double2 sum = make_double2(0.0, 0.0);
-        for (int d = dlo; d <= dhi; ++d) {
-            double b = row[d];
-            if (b != 0.0) {
-                double2 xv = xs[i + d];
-                sum.x += b * xv.x;
-                sum.y += b * xv.y;
-            }
+        for (int e = 0; e < W; ++e) {
+            const double b = a.op.ell_val[base + e];
+            const double2 xv = xs[a.op.ell_col[base + e]];
+            sum.x += b * xv.x;
+            sum.y += b * xv.y;
         }
-        if (a.gkx[g] || a.gky[g])
-            sum = cmul(monomial(a.gkx[g], a.gky[g], a.km.kx(s), a.km.ky(s)), sum);
-        y = cadd(y, sum);
+        y = cadd(y, cmul(group_coef(a.op, g, kx, ky), sum));
     }
     for (int v = 0; v < a.nvec; ++v) {
         double2 t = a.vec[v][idx];
@@ -673,10 +705,7 @@ __global__ __launch_bounds__(64) void k_factor(FactorArgs a) {
     const int n = a.total, kl = a.nl, ku = a.nu, kv = kl + ku, ldab = 2 * kl + ku + 1, w = a.nl + a.nu + 1;
     double2 *AB = a.work + (size_t)sl * n * ldab;
     const bool ident = a.identity00 && s == 0;
-    double2 cm[8], cs[8];
-    for (int g = 0; g < a.nmg && g < 8; ++g) cm[g] = monomial(a.mkx[g], a.mky[g], a.km.kx(s), a.km.ky(s));
-    for (int g = 0; g < a.nsg && g < 8; ++g)
-        cs[g] = cscale(monomial(a.skx[g], a.sky[g], a.km.kx(s), a.km.ky(s)), a.sdt);
+    const double kx = a.km.kx(s), ky = a.km.ky(s);
     for (int e = threadIdx.x; e < n * ldab; e += blockDim.x) {
         int j = e / ldab, rr = e - j * ldab;
         int i = rr - kv + j;
@@ -686,15 +715,13 @@ __global__ __launch_bounds__(64) void k_factor(FactorArgs a) {
             if (ident) {
                 val.x = (i == j) ? 1.0 : 0.0;
             } else {
-                for (int g = 0; g < a.nmg; ++g) {
-                    double b = a.mband[((size_t)g * n + i) * w + d + a.nl];
-                    val.x += b * cm[g].x;
-                    val.y += b * cm[g].y;
+                for (int g = 0; g < a.mass.ngroups; ++g) {
+                    double b = a.mass.band[((size_t)g * n + i) * w + d + a.nl];
+                    if (b != 0.0) val = cadd(val, cscale(group_coef(a.mass, g, kx, ky), b));
                 }
-                for (int g = 0; g < a.nsg; ++g) {
-                    double b = a.sband[((size_t)g * n + i) * w + d + a.nl];
-                    val.x += b * cs[g].x;
-                    val.y += b * cs[g].y;
+                for (int g = 0; g < a.stif.ngroups; ++g) {
+                    double b = a.stif.band[((size_t)g * n + i) * w + d + a.nl];
+                    if (b != 0.0) val = cadd(val, cscale(group_coef(a.stif, g, kx, ky), b * a.sdt));
                 }
             }
         }
@@ -772,9 +799,177 @@ __global__ __launch_bounds__(64) void k_solve(const double2 *__restrict__ Lf, co
     for (int i = tid; i < n; i += blockDim.x) x[(size_t)s * n + i] = b[i];
 }
 
+// ---------------------------------------------------------------------------
+// Register-resident banded solve: one wavefront per wavenumber.  Position i of
+// the right-hand side lives in lane i%64 of a 64-entry block; the wave keeps the
+// active block and its neighbour in registers and stages finished blocks of the
+// forward result through the output array.  Pivots and the current unknown are
+// broadcast with v_readlane (uniform scalar).  Factor records are fetched per
+// lane, PF steps ahead, into a ring of registers indexed statically by the
+// unrolled step, so each step's coefficients are already in flight.
+// Requirements: kl < 64 and kl + ku < 64.
+// ---------------------------------------------------------------------------
+__device__ inline double readlane_d(double v, int l) {
+    long long i = __double_as_longlong(v);
+    int lo = __builtin_amdgcn_readlane((int)(i & 0xffffffffLL), l);
+    int hi = __builtin_amdgcn_readlane((int)(i >> 32), l);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+__device__ inline double2 readlane_c(double2 v, int l) { return make_double2(readlane_d(v.x, l), readlane_d(v.y, l)); }
+// y -= c * b (complex)
+__device__ inline double2 csubmul(double2 y, double2 c, double2 b) {
+    y.x = fma(-c.x, b.x, y.x);
+    y.x = fma(c.y, b.y, y.x);
+    y.y = fma(-c.x, b.y, y.y);
+    y.y = fma(-c.y, b.x, y.y);
+    return y;
+}
+
+typedef double nd2 __attribute__((ext_vector_type(2)));
+// streaming (non-temporal) 16-byte load of a complex from global memory
+__device__ inline double2 ld_nt(const double2 *p) {
+    nd2 v = __builtin_nontemporal_load(reinterpret_cast<const nd2 *>(p));
+    return make_double2(v.x, v.y);
+}
+
+// component-wise select: selects on double2 aggregates are lowered through the
+// stack, selects on the two doubles stay in registers
+__device__ inline double2 csel(bool c, double2 a, double2 b) {
+    return make_double2(c ? a.x : b.x, c ? a.y : b.y);
+}
+
+// p[i] if cond else 0, as an unconditional load of an in-bounds element (index
+// clamped to 0) followed by a value select, so no pointer select is generated
+__device__ inline double2 ld_masked(const double2 *p, int i, bool cond) {
+    const double2 v = p[cond ? i : 0];
+    return csel(cond, v, make_double2(0.0, 0.0));
+}
+
+// forward coefficient of lane `lane` for step j: L(j+1+k, j), k = (lane - j%64 - 1) mod 64.
+// The load address is always inside the system's factor block (clamped); invalid
+// lanes and steps get zero, so the update is a no-op there.
+__device__ inline double2 fwd_coef(const double2 *L, int j, int lane, int n, int kl) {
+    const int t = j & 63;
+    const int k = (lane - t - 1) & 63;
+    const int lm = min(kl, n - 1 - j);
+    const int jc = min(j, n - 2);
+    const double2 v = ld_nt(L + (size_t)jc * kl + (k < kl ? k : 0));
+    return csel(k < lm, v, make_double2(0.0, 0.0));
+}
+
+// backward coefficient of lane `lane` for step j: lane j%64 gets 1/U(j,j), the
+// others U(j-1-k, j) with k = (j%64 - 1 - lane) mod 64; zero outside the band or for j >= n.
+__device__ inline double2 bwd_coef(const double2 *U, int j, int lane, int n, int kv) {
+    const int t = j & 63;
+    const int jc = max(0, min(j, n - 1));
+    const int k = (t - 1 - lane) & 63;
+    const bool diag = lane == t;
+    const int off = diag ? 0 : 1 + (k < kv ? k : 0);
+    const double2 v = ld_nt(U + (size_t)jc * (kv + 1) + off);
+    const bool valid = (j >= 0) && (j < n) && (diag || k < min(kv, j));
+    return csel(valid, v, make_double2(0.0, 0.0));
+}
+
+template <int PF>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_solve_reg(const double2 *__restrict__ Lf, const double2 *__restrict__ Uf,
+                                                  const unsigned char *__restrict__ piv, int n, int kl, int kv,
+                                                  const double2 *__restrict__ rhs, double2 *__restrict__ x) {
+    const int s = blockIdx.x;
+    const int lane = threadIdx.x;
+    const double2 *L = Lf + (size_t)s * n * kl;
+    const double2 *U = Uf + (size_t)s * n * (kv + 1);
+    const unsigned char *P = piv + (size_t)s * n;
+    const double2 *b = rhs + (size_t)s * n;
+    double2 *xo = x + (size_t)s * n;
+    const int R = (n + 63) >> 6;
+#define CZERO make_double2(0.0, 0.0)
+
+    // ---- forward: L with interleaved row interchanges (xGBTRS, first loop) ----
+    double2 cur = ld_masked(b, lane, lane < n);
+    double2 nxt = ld_masked(b, 64 + lane, 64 + lane < n);
+    int pv = P[min(lane, n - 1)];
+    pv = (lane < n) ? pv : 0;
+    double2 ring[PF];
+#pragma unroll
+    for (int u = 0; u < PF; ++u) ring[u] = fwd_coef(L, u, lane, n, kl);
+    for (int r = 0; r < R; ++r) {
+        for (int t0 = 0; t0 < 64; t0 += PF) {
+#pragma unroll
+            for (int u = 0; u < PF; ++u) {
+                const int t = t0 + u;
+                const int j = (r << 6) + t;
+                const int p = __builtin_amdgcn_readlane(pv, t);
+                if (p != 0) {
+                    // interchange positions j and j+p (lane tp of this block or lane tp-64 of the next)
+                    const int tp = t + p;
+                    const double2 a = readlane_c(cur, t);
+                    const double2 ca = readlane_c(cur, min(tp, 63));
+                    const double2 cb = readlane_c(nxt, max(tp - 64, 0));
+                    const double2 c2 = csel(tp < 64, ca, cb);
+                    cur = csel(lane == t, c2, cur);
+                    cur = csel(lane == tp, a, cur);
+                    nxt = csel(lane == tp - 64, a, nxt);
+                }
+                const double2 bj = readlane_c(cur, t);
+                const double2 c = ring[u];
+                const bool here = lane > t;
+                cur = csubmul(cur, csel(here, c, CZERO), bj);
+                if (t >= 64 - kl) nxt = csubmul(nxt, csel(here, CZERO, c), bj);
+                ring[u] = fwd_coef(L, j + PF, lane, n, kl);
+            }
+        }
+        const int i0 = (r << 6) + lane;
+        if (i0 < n) xo[i0] = cur;
+        if (r < R - 1) {
+            cur = nxt;
+            const int i2 = ((r + 2) << 6) + lane;
+            nxt = ld_masked(b, i2, i2 < n);
+            const int i1 = ((r + 1) << 6) + lane;
+            pv = P[min(i1, n - 1)];
+            pv = (i1 < n) ? pv : 0;
+        }
+    }
+    __threadfence_block();
+
+    // ---- backward: U with kv super-diagonals, column oriented (xTBSV) ----
+    // cur holds block R-1 of the forward result
+    double2 prv = ld_masked(xo, ((R - 2) << 6) + lane, R >= 2);
+#pragma unroll
+    for (int u = 0; u < PF; ++u) ring[u] = bwd_coef(U, (R << 6) - 1 - u, lane, n, kv);
+    for (int r = R - 1; r >= 0; --r) {
+        for (int t0 = 63; t0 >= 0; t0 -= PF) {
+#pragma unroll
+            for (int u = 0; u < PF; ++u) {
+                const int t = t0 - u;
+                const int j = (r << 6) + t;
+                const double2 c = ring[u];
+                const double2 xj = readlane_c(cmul(cur, c), t);
+                cur = csel(lane == t, xj, cur);
+                const bool here = lane < t;
+                const double2 cu = csel(lane == t, CZERO, c);
+                cur = csubmul(cur, csel(here, cu, CZERO), xj);
+                if (t < kv) prv = csubmul(prv, csel(here, CZERO, cu), xj);
+                ring[u] = bwd_coef(U, j - PF, lane, n, kv);
+            }
+        }
+        const int i0 = (r << 6) + lane;
+        if (i0 < n) xo[i0] = cur;
+        if (r > 0) {
+            cur = prv;
+            prv = ld_masked(xo, ((r - 2) << 6) + lane, r >= 2);
+        }
+    }
+}
+#undef CZERO
+
 void launch_solve(hipStream_t st, const double2 *Lf, const double2 *Uf, const unsigned char *piv, int n, int kl,
                   int kv, int nsys, const double2 *rhs, double2 *x) {
-    hipLaunchKernelGGL(k_solve, dim3(nsys), dim3(64), size_t(n) * sizeof(double2), st, Lf, Uf, piv, n, kl, kv, rhs, x);
+    if (kl >= 1 && kl < 64 && kv < 64 && n >= 64) {
+        hipLaunchKernelGGL(k_solve_reg<16>, dim3(nsys), dim3(64), 0, st, Lf, Uf, piv, n, kl, kv, rhs, x);
+    } else {
+        hipLaunchKernelGGL(k_solve, dim3(nsys), dim3(64), size_t(n) * sizeof(double2), st, Lf, Uf, piv, n, kl, kv, rhs,
+                           x);
+    }
     CHECK_LAUNCH();
 }
 

@@ -64,17 +64,31 @@ struct NLArgs {
 };
 void launch_nl(hipStream_t st, const NLArgs &a);
 
-// ---- banded operator application with AXPYs ----
-// y[s][i] = sum_g c_g(s) (B_g x[s])[i] + sum_j alpha_j V_j[s][i]; kxky (0,0): mass -> 0, stif -> 0
+// ---- operators: groups of real matrices times sums of horizontal monomials ----
+struct OpDev {
+    int ngroups;
+    const int *mono_off, *mono_a, *mono_b;
+    const double *band;                 // [g][total][nl+nu+1]
+    const int *ell_w, *ell_off, *ell_col;
+    const double *ell_val;
+};
+
+__device__ inline double2 group_coef(const OpDev &op, int g, double kx, double ky) {
+    double2 c = make_double2(0.0, 0.0);
+    for (int m = op.mono_off[g]; m < op.mono_off[g + 1]; ++m) c = cadd(c, monomial(op.mono_a[m], op.mono_b[m], kx, ky));
+    return c;
+}
+
+// ---- operator application with AXPYs ----
+// y[s][i] = sum_g c_g(s) (B_g x[s])[i] + sum_j alpha_j V_j[s][i]; zero00: y = 0 at s = 0
 struct BandApplyArgs {
-    const double *band;       // [ngroups][total][w]
-    const int *gkx, *gky;     // exponents per group
-    int ngroups, nl, nu, total, nsys;
+    OpDev op;
+    int total, nsys;
     const double2 *x;
     int nvec;
     const double2 *vec[8];
     double alpha[8];
-    int zero00;               // 1: force y = 0 at s = 0
+    int zero00;
     KeptModes km;
     double2 *y;
 };
@@ -82,9 +96,8 @@ void launch_band_apply(hipStream_t st, const BandApplyArgs &a);
 
 // ---- banded LU factorization and solve ----
 struct FactorArgs {
-    const double *mband, *sband;
-    const int *mkx, *mky, *skx, *sky;
-    int nmg, nsg, nl, nu, total;
+    OpDev mass, stif;
+    int nl, nu, total;
     double sdt;               // -dt * a11
     int s0, ns;               // systems [s0, s0+ns)
     int identity00;           // kxky (0,0) -> identity

@@ -90,7 +90,20 @@ def main():
     ap.add_argument("--workload", default="rrbc256", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run a plain device-to-device copy of MIB MiB (5 times); its FETCH_SIZE/WRITE_SIZE "
+                         "counters calibrate the per-kernel HBM traffic figures")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        import torch
+        n = args.calibrate_copy * (1 << 20) // 8
+        src = torch.arange(n, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"copied {args.calibrate_copy} MiB 5 times")
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs the whole domain on one GPU; multi-GPU runs are not part of it "
               "(see DESIGN.md for how the path would shard).", file=sys.stderr)

@@ -40,11 +40,13 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def solve_bytes(N, kl, kv, nsys):
-    """Algorithmic HBM bytes of one banded-solve launch: LU factors (16 B per complex
-    entry, kl multipliers + kv+1 U entries per column), 1 B pivot offsets, rhs read,
-    solution written."""
-    return nsys * N * (16 * kl + 16 * (kv + 1) + 1 + 16 + 16)
+def solve_bytes(N, kl, kv, nsys, real):
+    """Algorithmic HBM bytes of one banded-solve launch: LU factors (kl multipliers
+    + kv+1 U entries per column; 8 B per entry in real form, 16 B complex), 1 B
+    pivot offset, 1 B rotation code in real form, complex rhs read and solution
+    written."""
+    e = 8 if real else 16
+    return nsys * N * (e * kl + e * (kv + 1) + 1 + (1 if real else 0) + 16 + 16)
 
 
 def kernel_bytes(name, P):
@@ -53,7 +55,7 @@ def kernel_bytes(name, P):
     nf, nq = P["nfull"], P["nquad"]
     S_sys = 16 * N * nsys
     table = {
-        "solve": solve_bytes(N, P["kl"], P["kl"] + P["ku"], nsys),
+        "solve": solve_bytes(N, P["kl"], P["kl"] + P["ku"], nsys, P["real"]),
         "kstd": 2 * S_sys,                                   # x read, K_std written
         "extract": S_sys + nf * 16 * NZ * nsys,              # state read, spectra written
         "zbwd": nf * 16 * (NZ + NZAA) * nsys,
@@ -141,7 +143,7 @@ def main():
     norm = float(np.linalg.norm(kfin[0]))
     log(f"{steps} steps in {elapsed:.3f} s, state finite={finite}, |state|={norm:.6e}")
 
-    P = dict(N=total, kl=kl, ku=ku, nsys=nsys, NZ=NZ, NZAA=3 * NZ // 2, NYAA=3 * NX // 2, nkx=NX // 2,
+    P = dict(N=total, kl=kl, ku=ku, nsys=nsys, real=g.factor_is_real(0), NZ=NZ, NZAA=3 * NZ // 2, NYAA=3 * NX // 2, nkx=NX // 2,
              nfull=len(recipe.full_vars), nquad=len(recipe.quad_vars))
     per_kernel = {}
     for name, (ms, cnt) in timers.items():
@@ -194,7 +196,8 @@ def main():
         "dtype": "f64",
         "data": "synthetic (seeded perturbation, no checkpoint)",
         "config": {"workload": wl["name"], "grid": [NX, NX, NZ], "systems": nsys, "unknowns_per_system": total,
-                   "band": [kl, ku], "state_finite": finite},
+                   "band": [kl, ku], "factors": "real form" if P["real"] else "complex",
+                   "state_finite": finite},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "kernels_ms_per_step": {k: round(v["ms_per_step"], 4) for k, v in per_kernel.items()},

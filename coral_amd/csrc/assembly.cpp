@@ -192,6 +192,50 @@ static double cospi_frac(long long num, long long den) {
     return std::cos(M_PI * double(m) / double(den));
 }
 
+// Parity assignment for the real form (see SystemTables::real_form): nodes
+// 0..nvars-1 are equations, nvars..2nvars-1 variables; a piece of degree a+b
+// joining equation e and variable v asks parity(e) xor parity(v) = (a+b) mod 2.
+static void find_real_form(const coral_amd_system &s, SystemTables &t) {
+    const int nv = s.nvars;
+    std::vector<std::vector<std::pair<int, int>>> adj(2 * nv);
+    for (const auto &grp : {std::make_pair(s.mass, s.n_mass), std::make_pair(s.stif, s.n_stif)})
+        for (int k = 0; k < grp.second; ++k) {
+            const coral_amd_piece &p = grp.first[k];
+            const int par = (p.kx + p.ky) & 1;
+            adj[p.eqn].push_back({nv + p.var, par});
+            adj[nv + p.var].push_back({p.eqn, par});
+        }
+    std::vector<int> val(2 * nv, -1);
+    bool ok = true;
+    for (int root = 0; root < 2 * nv && ok; ++root) {
+        if (val[root] >= 0) continue;
+        val[root] = 0;
+        std::vector<int> stack{root};
+        while (!stack.empty() && ok) {
+            const int u = stack.back();
+            stack.pop_back();
+            for (auto [w, par] : adj[u]) {
+                const int want = val[u] ^ par;
+                if (val[w] < 0) {
+                    val[w] = want;
+                    stack.push_back(w);
+                } else if (val[w] != want) {
+                    ok = false;
+                    break;
+                }
+            }
+        }
+    }
+    t.real_form = ok;
+    t.rot.assign(t.total, 0);
+    if (!ok) return;
+    std::vector<int> var_at(t.total, -1);
+    for (int v = 0; v < nv; ++v)
+        for (int c = 0; c < t.var_size[v]; ++c) var_at[t.posP[t.var_first[v] + c]] = v;
+    for (int p = 0; p < t.total; ++p)
+        t.rot[p] = (unsigned char)(val[t.pos_eqn[p]] | (val[nv + var_at[p]] << 1));
+}
+
 static SystemTables build_system(const coral_amd_system &s, int NZ, double center, double gap, bool kxky,
                                  std::map<std::pair<int, int>, Dense> &im_cache, int nquad) {
     SystemTables t;
@@ -386,6 +430,7 @@ static SystemTables build_system(const coral_amd_system &s, int NZ, double cente
         t.nl_eqn_begin[q.eqn + 1]++;
     }
     for (int e = 0; e < s.nvars; ++e) t.nl_eqn_begin[e + 1] += t.nl_eqn_begin[e];
+    find_real_form(s, t);
     return t;
 }
 

@@ -65,6 +65,13 @@ struct SystemTables {
     std::vector<NLTable> nl_terms;        // sorted by equation
     std::vector<int> nl_eqn_begin;        // [nvars+1]
     std::vector<double> nl_w;
+    // Real form of the evolution operator: with variable v scaled by i^phi_v
+    // and equation e by i^-psi_e (phi, psi in {0, 1}), every piece
+    // (i kx)^a (i ky)^b becomes real when a + b + phi_v + psi_e is even for all
+    // pieces.  rot[p] = psi of the equation at position p | phi of the variable
+    // at position p << 1.
+    bool real_form = false;
+    std::vector<unsigned char> rot;
     int band_width() const { return nl + nu + 1; }
 };
 

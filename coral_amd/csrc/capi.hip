@@ -3,6 +3,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -59,7 +60,10 @@ struct SysDev {
     int nsys = 0;
     DevBuf<double2> field, aux;
     std::vector<std::unique_ptr<DevBuf<double2>>> Khat, Kstd;
-    DevBuf<double2> Lf, Uf;
+    DevBuf<double2> Lf, Uf;      // complex factors
+    bool real = false;           // factored in real form (Lr, Ur, rot)
+    DevBuf<double> Lr, Ur;
+    DevBuf<unsigned char> rot;
     DevBuf<unsigned char> piv;
     DevBuf<int> info;
     struct OpBufs {
@@ -243,8 +247,16 @@ void fill_system_device(SysDev &d, int nstages) {
     if (wv.empty()) wv.push_back(0.0);
     d.wv.upload(wv);
     const int kv = t.nl + t.nu;
-    d.Lf.alloc(size_t(d.nsys) * t.total * std::max(t.nl, 1));
-    d.Uf.alloc(size_t(d.nsys) * t.total * (kv + 1));
+    const char *force = std::getenv("CORAL_AMD_COMPLEX_FACTORS");
+    d.real = t.real_form && !(force && force[0] == '1');
+    if (d.real) {
+        d.Lr.alloc(size_t(d.nsys) * t.total * std::max(t.nl, 1));
+        d.Ur.alloc(size_t(d.nsys) * t.total * (kv + 1));
+        d.rot.upload(t.rot);
+    } else {
+        d.Lf.alloc(size_t(d.nsys) * t.total * std::max(t.nl, 1));
+        d.Uf.alloc(size_t(d.nsys) * t.total * (kv + 1));
+    }
     d.piv.alloc(size_t(d.nsys) * t.total);
     d.info.alloc(d.nsys);
 }
@@ -275,6 +287,9 @@ void factorize(coral_amd_ctx &c, SysDev &d, double dt) {
         a.work = work.p;
         a.Lf = d.Lf.p;
         a.Uf = d.Uf.p;
+        a.rot = d.real ? d.rot.p : nullptr;
+        a.Lr = d.Lr.p;
+        a.Ur = d.Ur.p;
         a.piv = d.piv.p;
         a.info = d.info.p;
         c.timed("factor", [&] { launch_factor(c.stream, a); });
@@ -407,8 +422,12 @@ void one_step(coral_amd_ctx &c) {
             band_apply(c, d, true, d.field.p, d.aux.p, vecs);
             const SystemTables &t = *d.t;
             c.timed(d.kxky ? "solve" : "solve_mean", [&] {
-                launch_solve(c.stream, d.Lf.p, d.Uf.p, d.piv.p, t.total, t.nl, t.nl + t.nu, d.nsys, d.aux.p,
-                             d.aux.p);
+                if (d.real)
+                    launch_solve_real(c.stream, d.Lr.p, d.Ur.p, d.piv.p, d.rot.p, d.kxky ? 1 : 0, t.total, t.nl,
+                                      t.nl + t.nu, d.nsys, d.aux.p, d.aux.p);
+                else
+                    launch_solve(c.stream, d.Lf.p, d.Uf.p, d.piv.p, t.total, t.nl, t.nl + t.nu, d.nsys, d.aux.p,
+                                 d.aux.p);
             });
             band_apply(c, d, false, d.aux.p, d.Kstd[i]->p, {});
         };
@@ -724,6 +743,17 @@ int coral_amd_system_info(coral_amd_ctx *c, int32_t isys, int32_t *total, int32_
         if (nl) *nl = t.nl;
         if (nu) *nu = t.nu;
         if (nkept) *nkept = c->P.nsys;
+        return 0;
+    } catch (const std::exception &e) {
+        return fail(e);
+    }
+}
+
+int coral_amd_factor_kind(coral_amd_ctx *c, int32_t isys, int32_t *is_real) {
+    try {
+        if (!c || !is_real) throw std::runtime_error("null argument");
+        if (isys < 0 || isys >= int(c->ks.size())) throw std::runtime_error("bad kxky system index");
+        *is_real = c->ks[isys]->real ? 1 : 0;
         return 0;
     } catch (const std::exception &e) {
         return fail(e);

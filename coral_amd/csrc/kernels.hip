@@ -434,6 +434,7 @@ static size_t xpass_lds(int LX, int N, int nfull) {
 }
 
 void launch_xpass(hipStream_t st, const FFTPlan &px, const XPassArgs &a) {
+    if (launch_xpass_wave(st, px, a)) return;
     // lines per workgroup: FFT butterflies must fit fft_lds<3> (LX*N/2 <= 3*256)
     // and the workgroup's LDS must stay within 64 KiB
     int LX = 4;
@@ -628,16 +629,43 @@ void launch_band_apply(hipStream_t st, const BandApplyArgs &a) {
 // band lives in `work` (LAPACK layout), the result is split into column
 // records Lf (multipliers) and Uf (inverse pivot + column of U above it).
 // ---------------------------------------------------------------------------
-__device__ inline double cabs1(double2 z) { return fabs(z.x) + fabs(z.y); }
+// Scalar traits of the factor entries: complex (double2), or real (double)
+// for systems in real form.  The real reciprocal is 1/x, which is what the
+// complex reciprocal gives for a zero imaginary part.
+template <typename T>
+struct Fx;
+template <>
+struct Fx<double2> {
+    __device__ static double abs1(double2 z) { return fabs(z.x) + fabs(z.y); }
+    __device__ static bool is_zero(double2 z) { return z.x == 0.0 && z.y == 0.0; }
+    __device__ static double2 zero() { return make_double2(0.0, 0.0); }
+    __device__ static double2 recip(double2 z) { return crecip(z); }
+    __device__ static double2 mul(double2 a, double2 b) { return cmul(a, b); }
+    __device__ static double2 sub(double2 a, double2 b) { return csub(a, b); }
+};
+template <>
+struct Fx<double> {
+    __device__ static double abs1(double z) { return fabs(z); }
+    __device__ static bool is_zero(double z) { return z == 0.0; }
+    __device__ static double zero() { return 0.0; }
+    __device__ static double recip(double z) { return 1.0 / z; }
+    __device__ static double mul(double a, double b) { return a * b; }
+    __device__ static double sub(double a, double b) { return a - b; }
+};
 
-__device__ void gbtf2_wave(double2 *AB, int n, int kl, int ku, int ldab, unsigned char *piv, int *info) {
+// Partial pivoting picks the largest |Re| + |Im| (LAPACK's cabs1); rotating
+// rows and columns by powers of i leaves that measure unchanged, so the real
+// form pivots on the same rows as the complex operator.
+template <typename T>
+__device__ void gbtf2_wave(T *AB, int n, int kl, int ku, int ldab, unsigned char *piv, int *info) {
+    using F = Fx<T>;
     const int kv = kl + ku;
     const int tid = threadIdx.x;
     int ju = 0;
     for (int j = 0; j < n; ++j) {
         const int km = min(kl, n - 1 - j);
-        double2 *colj = AB + (size_t)j * ldab;
-        double v = (tid <= km) ? cabs1(colj[kv + tid]) : -1.0;
+        T *colj = AB + (size_t)j * ldab;
+        double v = (tid <= km) ? F::abs1(colj[kv + tid]) : -1.0;
         int ix = tid;
         for (int off = 32; off > 0; off >>= 1) {
             double ov = __shfl_xor(v, off);
@@ -649,8 +677,8 @@ __device__ void gbtf2_wave(double2 *AB, int n, int kl, int ku, int ldab, unsigne
         }
         const int jp = ix;
         if (tid == 0) piv[j] = (unsigned char)jp;
-        double2 pv = colj[kv + jp];
-        if (pv.x == 0.0 && pv.y == 0.0) {
+        T pv = colj[kv + jp];
+        if (F::is_zero(pv)) {
             if (tid == 0 && *info == 0) *info = j + 1;
             __syncthreads();
             continue;
@@ -658,52 +686,59 @@ __device__ void gbtf2_wave(double2 *AB, int n, int kl, int ku, int ldab, unsigne
         ju = max(ju, min(j + ku + jp, n - 1));
         if (jp != 0) {
             for (int c = j + tid; c <= ju; c += blockDim.x) {
-                double2 *cc = AB + (size_t)c * ldab;
-                double2 t = cc[kv + jp + j - c];
+                T *cc = AB + (size_t)c * ldab;
+                T t = cc[kv + jp + j - c];
                 cc[kv + jp + j - c] = cc[kv + j - c];
                 cc[kv + j - c] = t;
             }
         }
         __syncthreads();
         if (km > 0) {
-            double2 rp = crecip(colj[kv]);
-            if (tid >= 1 && tid <= km) colj[kv + tid] = cmul(colj[kv + tid], rp);
+            T rp = F::recip(colj[kv]);
+            if (tid >= 1 && tid <= km) colj[kv + tid] = F::mul(colj[kv + tid], rp);
             __syncthreads();
             const int ncols = ju - j;
             for (int e = tid; e < ncols * km; e += blockDim.x) {
                 int c = j + 1 + e / km, i = 1 + e % km;
-                double2 *cc = AB + (size_t)c * ldab;
-                double2 u = cc[kv + j - c];
-                double2 l = colj[kv + i];
-                double2 t = cc[kv + j + i - c];
-                cc[kv + j + i - c] = csub(t, cmul(l, u));
+                T *cc = AB + (size_t)c * ldab;
+                T u = cc[kv + j - c];
+                T l = colj[kv + i];
+                T t = cc[kv + j + i - c];
+                cc[kv + j + i - c] = F::sub(t, F::mul(l, u));
             }
         }
         __syncthreads();
     }
 }
 
-__device__ void split_factors(const double2 *AB, int n, int kl, int ku, int ldab, double2 *Lf, double2 *Uf) {
+template <typename T>
+__device__ void split_factors(const T *AB, int n, int kl, int ku, int ldab, T *Lf, T *Uf) {
+    using F = Fx<T>;
     const int kv = kl + ku;
     for (int e = threadIdx.x; e < n * kl; e += blockDim.x) {
         int j = e / kl, k = e % kl;
-        Lf[e] = (k < min(kl, n - 1 - j)) ? AB[(size_t)j * ldab + kv + 1 + k] : make_double2(0.0, 0.0);
+        Lf[e] = (k < min(kl, n - 1 - j)) ? AB[(size_t)j * ldab + kv + 1 + k] : F::zero();
     }
     for (int e = threadIdx.x; e < n * (kv + 1); e += blockDim.x) {
         int j = e / (kv + 1), k = e % (kv + 1);
-        double2 v;
+        T v;
         if (k == 0)
-            v = crecip(AB[(size_t)j * ldab + kv]);
+            v = F::recip(AB[(size_t)j * ldab + kv]);
         else
-            v = (j - k >= 0) ? AB[(size_t)j * ldab + kv - k] : make_double2(0.0, 0.0);
+            v = (j - k >= 0) ? AB[(size_t)j * ldab + kv - k] : F::zero();
         Uf[e] = v;
     }
 }
 
+// Re(z * i^e) for e in {0, 1, 3}
+__device__ inline double real_rot(double2 z, int e) { return e == 0 ? z.x : (e == 1 ? -z.y : z.y); }
+
+template <typename T>
 __global__ __launch_bounds__(64) void k_factor(FactorArgs a) {
+    constexpr bool REAL = sizeof(T) == sizeof(double);
     const int sl = blockIdx.x, s = a.s0 + sl;
     const int n = a.total, kl = a.nl, ku = a.nu, kv = kl + ku, ldab = 2 * kl + ku + 1, w = a.nl + a.nu + 1;
-    double2 *AB = a.work + (size_t)sl * n * ldab;
+    T *AB = reinterpret_cast<T *>(a.work) + (size_t)sl * n * ldab;
     const bool ident = a.identity00 && s == 0;
     const double kx = a.km.kx(s), ky = a.km.ky(s);
     for (int e = threadIdx.x; e < n * ldab; e += blockDim.x) {
@@ -725,19 +760,31 @@ __global__ __launch_bounds__(64) void k_factor(FactorArgs a) {
                 }
             }
         }
-        AB[e] = val;
+        if constexpr (REAL) {
+            // identity systems stay unrotated (their solve skips the rotation too)
+            const int rot = ident ? 0 : (((a.rot[min(max(j, 0), n - 1)] >> 1) - (a.rot[min(max(i, 0), n - 1)] & 1)) & 3);
+            AB[e] = real_rot(val, rot);
+        } else {
+            AB[e] = val;
+        }
     }
     __syncthreads();
     int *info = a.info + s;
     if (threadIdx.x == 0) *info = 0;
     __syncthreads();
-    gbtf2_wave(AB, n, kl, ku, ldab, a.piv + (size_t)s * n, info);
+    gbtf2_wave<T>(AB, n, kl, ku, ldab, a.piv + (size_t)s * n, info);
     __syncthreads();
-    split_factors(AB, n, kl, ku, ldab, a.Lf + (size_t)s * n * kl, a.Uf + (size_t)s * n * (kv + 1));
+    if constexpr (REAL)
+        split_factors<double>(AB, n, kl, ku, ldab, a.Lr + (size_t)s * n * kl, a.Ur + (size_t)s * n * (kv + 1));
+    else
+        split_factors<double2>(AB, n, kl, ku, ldab, a.Lf + (size_t)s * n * kl, a.Uf + (size_t)s * n * (kv + 1));
 }
 
 void launch_factor(hipStream_t st, const FactorArgs &a) {
-    hipLaunchKernelGGL(k_factor, dim3(a.ns), dim3(64), 0, st, a);
+    if (a.rot)
+        hipLaunchKernelGGL(k_factor<double>, dim3(a.ns), dim3(64), 0, st, a);
+    else
+        hipLaunchKernelGGL(k_factor<double2>, dim3(a.ns), dim3(64), 0, st, a);
     CHECK_LAUNCH();
 }
 
@@ -747,9 +794,9 @@ __global__ __launch_bounds__(64) void k_factor_lapack(double2 *work, int n, int 
     double2 *AB = work + (size_t)s * n * ldab;
     if (threadIdx.x == 0) info[s] = 0;
     __syncthreads();
-    gbtf2_wave(AB, n, kl, ku, ldab, piv + (size_t)s * n, info + s);
+    gbtf2_wave<double2>(AB, n, kl, ku, ldab, piv + (size_t)s * n, info + s);
     __syncthreads();
-    split_factors(AB, n, kl, ku, ldab, Lf + (size_t)s * n * kl, Uf + (size_t)s * n * (kv + 1));
+    split_factors<double2>(AB, n, kl, ku, ldab, Lf + (size_t)s * n * kl, Uf + (size_t)s * n * (kv + 1));
 }
 
 void launch_factor_lapack(hipStream_t st, double2 *work, int n, int kl, int ku, int nsys, double2 *Lf, double2 *Uf,
@@ -759,18 +806,64 @@ void launch_factor_lapack(hipStream_t st, double2 *work, int n, int kl, int ku, 
 }
 
 // ---------------------------------------------------------------------------
-// banded solve (xGBTRS 'N', PL_algebra_z_3D.f90:189-243), one wavefront per
-// wavenumber, right-hand side staged in LDS
+// Helpers shared by the solves.  Factor entries are T = double2 (complex) or
+// double (real form); the right-hand side is always complex.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(64) void k_solve(const double2 *__restrict__ Lf, const double2 *__restrict__ Uf,
-                                              const unsigned char *__restrict__ piv, int n, int kl, int kv,
-                                              const double2 *__restrict__ rhs, double2 *__restrict__ x) {
+typedef double nd2 __attribute__((ext_vector_type(2)));
+// streaming (non-temporal) loads of factor entries
+__device__ inline double2 ld_nt(const double2 *p) {
+    nd2 v = __builtin_nontemporal_load(reinterpret_cast<const nd2 *>(p));
+    return make_double2(v.x, v.y);
+}
+__device__ inline double ld_nt(const double *p) { return __builtin_nontemporal_load(p); }
+
+// component-wise select: selects on double2 aggregates are lowered through the
+// stack, selects on the two doubles stay in registers
+__device__ inline double2 csel(bool c, double2 a, double2 b) {
+    return make_double2(c ? a.x : b.x, c ? a.y : b.y);
+}
+__device__ inline double tsel(bool c, double a, double b) { return c ? a : b; }
+__device__ inline double2 tsel(bool c, double2 a, double2 b) { return csel(c, a, b); }
+
+// y - c * b
+__device__ inline double2 submul(double2 y, double2 c, double2 b) {
+    y.x = fma(-c.x, b.x, y.x);
+    y.x = fma(c.y, b.y, y.x);
+    y.y = fma(-c.x, b.y, y.y);
+    y.y = fma(-c.y, b.x, y.y);
+    return y;
+}
+__device__ inline double2 submul(double2 y, double c, double2 b) {
+    y.x = fma(-c, b.x, y.x);
+    y.y = fma(-c, b.y, y.y);
+    return y;
+}
+__device__ inline double2 scale(double2 z, double2 c) { return cmul(z, c); }
+__device__ inline double2 scale(double2 z, double c) { return make_double2(z.x * c, z.y * c); }
+
+// real-form rotations of the right-hand side (z * i^-psi) and of the solution (z * i^phi)
+__device__ inline double2 rot_in(double2 z, int code) { return csel(code & 1, make_double2(z.y, -z.x), z); }
+__device__ inline double2 rot_out(double2 z, int code) { return csel(code & 2, make_double2(-z.y, z.x), z); }
+
+// ---------------------------------------------------------------------------
+// banded solve (xGBTRS 'N', PL_algebra_z_3D.f90:189-243), one wavefront per
+// wavenumber, right-hand side staged in LDS (small systems)
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(64) void k_solve(const T *__restrict__ Lf, const T *__restrict__ Uf,
+                                              const unsigned char *__restrict__ piv,
+                                              const unsigned char *__restrict__ rot, int ident0, int n, int kl,
+                                              int kv, const double2 *__restrict__ rhs, double2 *__restrict__ x) {
     extern __shared__ double2 b[];
     const int s = blockIdx.x, tid = threadIdx.x;
-    const double2 *L = Lf + (size_t)s * n * kl;
-    const double2 *U = Uf + (size_t)s * n * (kv + 1);
+    const bool dorot = rot && !(ident0 && s == 0);
+    const T *L = Lf + (size_t)s * n * kl;
+    const T *U = Uf + (size_t)s * n * (kv + 1);
     const unsigned char *P = piv + (size_t)s * n;
-    for (int i = tid; i < n; i += blockDim.x) b[i] = rhs[(size_t)s * n + i];
+    for (int i = tid; i < n; i += blockDim.x) {
+        const double2 v = rhs[(size_t)s * n + i];
+        b[i] = dorot ? rot_in(v, rot[i]) : v;
+    }
     __syncthreads();
     for (int j = 0; j < n - 1; ++j) {
         const int p = P[j];
@@ -784,19 +877,19 @@ __global__ __launch_bounds__(64) void k_solve(const double2 *__restrict__ Lf, co
         }
         const double2 bj = b[j];
         const int lm = min(kl, n - 1 - j);
-        for (int k = tid; k < lm; k += blockDim.x) b[j + 1 + k] = csub(b[j + 1 + k], cmul(L[(size_t)j * kl + k], bj));
+        for (int k = tid; k < lm; k += blockDim.x) b[j + 1 + k] = submul(b[j + 1 + k], L[(size_t)j * kl + k], bj);
         __syncthreads();
     }
     for (int j = n - 1; j >= 0; --j) {
-        if (tid == 0) b[j] = cmul(b[j], U[(size_t)j * (kv + 1)]);
+        if (tid == 0) b[j] = scale(b[j], U[(size_t)j * (kv + 1)]);
         __syncthreads();
         const double2 xj = b[j];
         const int km = min(kv, j);
         for (int k = tid; k < km; k += blockDim.x)
-            b[j - 1 - k] = csub(b[j - 1 - k], cmul(U[(size_t)j * (kv + 1) + 1 + k], xj));
+            b[j - 1 - k] = submul(b[j - 1 - k], U[(size_t)j * (kv + 1) + 1 + k], xj);
         __syncthreads();
     }
-    for (int i = tid; i < n; i += blockDim.x) x[(size_t)s * n + i] = b[i];
+    for (int i = tid; i < n; i += blockDim.x) x[(size_t)s * n + i] = dorot ? rot_out(b[i], rot[i]) : b[i];
 }
 
 // ---------------------------------------------------------------------------
@@ -816,27 +909,6 @@ __device__ inline double readlane_d(double v, int l) {
     return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 __device__ inline double2 readlane_c(double2 v, int l) { return make_double2(readlane_d(v.x, l), readlane_d(v.y, l)); }
-// y -= c * b (complex)
-__device__ inline double2 csubmul(double2 y, double2 c, double2 b) {
-    y.x = fma(-c.x, b.x, y.x);
-    y.x = fma(c.y, b.y, y.x);
-    y.y = fma(-c.x, b.y, y.y);
-    y.y = fma(-c.y, b.x, y.y);
-    return y;
-}
-
-typedef double nd2 __attribute__((ext_vector_type(2)));
-// streaming (non-temporal) 16-byte load of a complex from global memory
-__device__ inline double2 ld_nt(const double2 *p) {
-    nd2 v = __builtin_nontemporal_load(reinterpret_cast<const nd2 *>(p));
-    return make_double2(v.x, v.y);
-}
-
-// component-wise select: selects on double2 aggregates are lowered through the
-// stack, selects on the two doubles stay in registers
-__device__ inline double2 csel(bool c, double2 a, double2 b) {
-    return make_double2(c ? a.x : b.x, c ? a.y : b.y);
-}
 
 // p[i] if cond else 0, as an unconditional load of an in-bounds element (index
 // clamped to 0) followed by a value select, so no pointer select is generated
@@ -848,48 +920,53 @@ __device__ inline double2 ld_masked(const double2 *p, int i, bool cond) {
 // forward coefficient of lane `lane` for step j: L(j+1+k, j), k = (lane - j%64 - 1) mod 64.
 // The load address is always inside the system's factor block (clamped); invalid
 // lanes and steps get zero, so the update is a no-op there.
-__device__ inline double2 fwd_coef(const double2 *L, int j, int lane, int n, int kl) {
+template <typename T>
+__device__ inline T fwd_coef(const T *L, int j, int lane, int n, int kl) {
     const int t = j & 63;
     const int k = (lane - t - 1) & 63;
     const int lm = min(kl, n - 1 - j);
     const int jc = min(j, n - 2);
-    const double2 v = ld_nt(L + (size_t)jc * kl + (k < kl ? k : 0));
-    return csel(k < lm, v, make_double2(0.0, 0.0));
+    const T v = ld_nt(L + (size_t)jc * kl + (k < kl ? k : 0));
+    return tsel(k < lm, v, Fx<T>::zero());
 }
 
 // backward coefficient of lane `lane` for step j: lane j%64 gets 1/U(j,j), the
 // others U(j-1-k, j) with k = (j%64 - 1 - lane) mod 64; zero outside the band or for j >= n.
-__device__ inline double2 bwd_coef(const double2 *U, int j, int lane, int n, int kv) {
+template <typename T>
+__device__ inline T bwd_coef(const T *U, int j, int lane, int n, int kv) {
     const int t = j & 63;
     const int jc = max(0, min(j, n - 1));
     const int k = (t - 1 - lane) & 63;
     const bool diag = lane == t;
     const int off = diag ? 0 : 1 + (k < kv ? k : 0);
-    const double2 v = ld_nt(U + (size_t)jc * (kv + 1) + off);
+    const T v = ld_nt(U + (size_t)jc * (kv + 1) + off);
     const bool valid = (j >= 0) && (j < n) && (diag || k < min(kv, j));
-    return csel(valid, v, make_double2(0.0, 0.0));
+    return tsel(valid, v, Fx<T>::zero());
 }
 
-template <int PF>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_solve_reg(const double2 *__restrict__ Lf, const double2 *__restrict__ Uf,
-                                                  const unsigned char *__restrict__ piv, int n, int kl, int kv,
-                                                  const double2 *__restrict__ rhs, double2 *__restrict__ x) {
+template <int PF, typename T>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_solve_reg(
+    const T *__restrict__ Lf, const T *__restrict__ Uf, const unsigned char *__restrict__ piv,
+    const unsigned char *__restrict__ rot, int ident0, int n, int kl, int kv, const double2 *__restrict__ rhs,
+    double2 *__restrict__ x) {
     const int s = blockIdx.x;
     const int lane = threadIdx.x;
-    const double2 *L = Lf + (size_t)s * n * kl;
-    const double2 *U = Uf + (size_t)s * n * (kv + 1);
+    const bool dorot = rot && !(ident0 && s == 0);
+    const T *L = Lf + (size_t)s * n * kl;
+    const T *U = Uf + (size_t)s * n * (kv + 1);
     const unsigned char *P = piv + (size_t)s * n;
     const double2 *b = rhs + (size_t)s * n;
     double2 *xo = x + (size_t)s * n;
     const int R = (n + 63) >> 6;
-#define CZERO make_double2(0.0, 0.0)
+    const T zero = Fx<T>::zero();
+    auto code = [&](int i) -> int { return dorot ? (int)rot[min(i, n - 1)] : 0; };
 
     // ---- forward: L with interleaved row interchanges (xGBTRS, first loop) ----
-    double2 cur = ld_masked(b, lane, lane < n);
-    double2 nxt = ld_masked(b, 64 + lane, 64 + lane < n);
+    double2 cur = rot_in(ld_masked(b, lane, lane < n), code(lane));
+    double2 nxt = rot_in(ld_masked(b, 64 + lane, 64 + lane < n), code(64 + lane));
     int pv = P[min(lane, n - 1)];
     pv = (lane < n) ? pv : 0;
-    double2 ring[PF];
+    T ring[PF];
 #pragma unroll
     for (int u = 0; u < PF; ++u) ring[u] = fwd_coef(L, u, lane, n, kl);
     for (int r = 0; r < R; ++r) {
@@ -911,10 +988,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 4))) void
                     nxt = csel(lane == tp - 64, a, nxt);
                 }
                 const double2 bj = readlane_c(cur, t);
-                const double2 c = ring[u];
+                const T c = ring[u];
                 const bool here = lane > t;
-                cur = csubmul(cur, csel(here, c, CZERO), bj);
-                if (t >= 64 - kl) nxt = csubmul(nxt, csel(here, CZERO, c), bj);
+                cur = submul(cur, tsel(here, c, zero), bj);
+                if (t >= 64 - kl) nxt = submul(nxt, tsel(here, zero, c), bj);
                 ring[u] = fwd_coef(L, j + PF, lane, n, kl);
             }
         }
@@ -923,7 +1000,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 4))) void
         if (r < R - 1) {
             cur = nxt;
             const int i2 = ((r + 2) << 6) + lane;
-            nxt = ld_masked(b, i2, i2 < n);
+            nxt = rot_in(ld_masked(b, i2, i2 < n), code(i2));
             const int i1 = ((r + 1) << 6) + lane;
             pv = P[min(i1, n - 1)];
             pv = (i1 < n) ? pv : 0;
@@ -942,35 +1019,48 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 4))) void
             for (int u = 0; u < PF; ++u) {
                 const int t = t0 - u;
                 const int j = (r << 6) + t;
-                const double2 c = ring[u];
-                const double2 xj = readlane_c(cmul(cur, c), t);
+                const T c = ring[u];
+                const double2 xj = readlane_c(scale(cur, c), t);
                 cur = csel(lane == t, xj, cur);
                 const bool here = lane < t;
-                const double2 cu = csel(lane == t, CZERO, c);
-                cur = csubmul(cur, csel(here, cu, CZERO), xj);
-                if (t < kv) prv = csubmul(prv, csel(here, CZERO, cu), xj);
+                const T cu = tsel(lane == t, zero, c);
+                cur = submul(cur, tsel(here, cu, zero), xj);
+                if (t < kv) prv = submul(prv, tsel(here, zero, cu), xj);
                 ring[u] = bwd_coef(U, j - PF, lane, n, kv);
             }
         }
         const int i0 = (r << 6) + lane;
-        if (i0 < n) xo[i0] = cur;
+        if (i0 < n) xo[i0] = rot_out(cur, code(i0));
         if (r > 0) {
             cur = prv;
             prv = ld_masked(xo, ((r - 2) << 6) + lane, r >= 2);
         }
     }
 }
-#undef CZERO
+
+template <typename T>
+static void launch_solve_t(hipStream_t st, const T *Lf, const T *Uf, const unsigned char *piv,
+                           const unsigned char *rot, int ident0, int n, int kl, int kv, int nsys, const double2 *rhs,
+                           double2 *x) {
+    if (kl >= 1 && kl < 64 && kv < 64 && n >= 64) {
+        hipLaunchKernelGGL((k_solve_reg<16, T>), dim3(nsys), dim3(64), 0, st, Lf, Uf, piv, rot, ident0, n, kl, kv,
+                           rhs, x);
+    } else {
+        hipLaunchKernelGGL(k_solve<T>, dim3(nsys), dim3(64), size_t(n) * sizeof(double2), st, Lf, Uf, piv, rot,
+                           ident0, n, kl, kv, rhs, x);
+    }
+    CHECK_LAUNCH();
+}
 
 void launch_solve(hipStream_t st, const double2 *Lf, const double2 *Uf, const unsigned char *piv, int n, int kl,
                   int kv, int nsys, const double2 *rhs, double2 *x) {
-    if (kl >= 1 && kl < 64 && kv < 64 && n >= 64) {
-        hipLaunchKernelGGL(k_solve_reg<16>, dim3(nsys), dim3(64), 0, st, Lf, Uf, piv, n, kl, kv, rhs, x);
-    } else {
-        hipLaunchKernelGGL(k_solve, dim3(nsys), dim3(64), size_t(n) * sizeof(double2), st, Lf, Uf, piv, n, kl, kv, rhs,
-                           x);
-    }
-    CHECK_LAUNCH();
+    launch_solve_t<double2>(st, Lf, Uf, piv, nullptr, 0, n, kl, kv, nsys, rhs, x);
+}
+
+void launch_solve_real(hipStream_t st, const double *Lr, const double *Ur, const unsigned char *piv,
+                       const unsigned char *rot, int ident0, int n, int kl, int kv, int nsys, const double2 *rhs,
+                       double2 *x) {
+    launch_solve_t<double>(st, Lr, Ur, piv, rot, ident0, n, kl, kv, nsys, rhs, x);
 }
 
 }  // namespace coral

@@ -41,6 +41,9 @@ struct XPassArgs {
     unsigned long long *cflmax;  // [3] bit patterns of non-negative doubles
 };
 void launch_xpass(hipStream_t st, const FFTPlan &px, const XPassArgs &a);
+// wavefront-per-line x pass for N in {96, 192, 384, 768} and up to 6 full
+// variables; returns false (nothing launched) for other shapes
+bool launch_xpass_wave(hipStream_t st, const FFTPlan &px, const XPassArgs &a);
 // y forward: Hy[b][z][ix*nky+jy] = sum_y H[b][z][y][ix] e^{-i ky y}
 void launch_yfwd(hipStream_t st, const FFTPlan &py, const double2 *H, double2 *Hy, int nbatch, int NZAA,
                  int nkx, int nky, int NY);
@@ -107,6 +110,10 @@ struct FactorArgs {
     double2 *Uf;              // [nsys][total][kv+1]  (Uf[.][j][0] = 1/U(j,j), Uf[.][j][1+k] = U(j-1-k, j))
     unsigned char *piv;       // [nsys][total]   pivot offsets
     int *info;                // [nsys]
+    // real form (rot != nullptr): entries rotated by i^(phi_col - psi_row) are
+    // real; factors go to Lr/Ur (same layout, real) and work is used as double
+    const unsigned char *rot; // [total] psi | phi << 1
+    double *Lr, *Ur;
 };
 void launch_factor(hipStream_t st, const FactorArgs &a);
 // generic batched factorization of LAPACK band input (for the unit seam)
@@ -114,5 +121,10 @@ void launch_factor_lapack(hipStream_t st, double2 *work, int n, int kl, int ku, 
                           unsigned char *piv, int *info);
 void launch_solve(hipStream_t st, const double2 *Lf, const double2 *Uf, const unsigned char *piv, int n, int kl,
                   int kv, int nsys, const double2 *rhs, double2 *x);
+// real-form solve: x = i^phi . (LU)^-1 . P . i^-psi rhs, with real factors
+// (ident0: system 0 is the identity and is not rotated)
+void launch_solve_real(hipStream_t st, const double *Lr, const double *Ur, const unsigned char *piv,
+                       const unsigned char *rot, int ident0, int n, int kl, int kv, int nsys, const double2 *rhs,
+                       double2 *x);
 
 }  // namespace coral

@@ -1,0 +1,183 @@
+// Wavefront-local Stockham FFTs for the horizontal transforms.
+//
+// One wavefront (64 lanes) owns S complex sequences of length N held in its
+// own slice of LDS; stages exchange data through that slice with wavefront
+// ordering only (no workgroup barrier).  Lengths, radices and stage strides
+// are template constants, so every index is resolved at compile time.
+//
+// Stage with radix R after Ns points have been combined (Ns = product of the
+// earlier radices), T = N / R butterflies per sequence:
+//   butterfly j reads   j + r*T                           (r = 0..R-1)
+//   twiddle             exp(sign * 2 pi i * (j % Ns) * r / (Ns * R))
+//   writes              (j / Ns) * Ns * R + j % Ns + r * Ns
+// The last stage (Ns * R = N) writes j + r*T: the element set a butterfly
+// owns at the end of one transform is the set a first-stage butterfly of the
+// same radix reads, which lets the x pass form products in registers between
+// the inverse and the forward transform.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "common.hpp"
+
+namespace coral {
+namespace wfft {
+
+// LDS traffic of one wavefront is ordered; the fences keep the compiler from
+// moving LDS accesses across a stage boundary.
+__device__ inline void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// z * (SIGN i)
+template <int SIGN>
+__device__ inline double2 mul_si(double2 z) {
+    return SIGN > 0 ? make_double2(-z.y, z.x) : make_double2(z.y, -z.x);
+}
+
+// R-point DFT in registers: X[k] = sum_n x[n] exp(SIGN 2 pi i n k / R)
+template <int R, int SIGN>
+struct Dft;
+
+template <int SIGN>
+struct Dft<2, SIGN> {
+    __device__ static inline void run(double2 *v) {
+        const double2 a = v[0], b = v[1];
+        v[0] = cadd(a, b);
+        v[1] = csub(a, b);
+    }
+};
+
+template <int SIGN>
+struct Dft<3, SIGN> {
+    __device__ static inline void run(double2 *v) {
+        const double s1 = SIGN * 0.86602540378443864676;  // sin(2 pi / 3)
+        const double2 a = v[0], t = cadd(v[1], v[2]), d = csub(v[1], v[2]);
+        const double2 m = make_double2(a.x - 0.5 * t.x, a.y - 0.5 * t.y);
+        v[0] = cadd(a, t);
+        v[1] = make_double2(m.x - s1 * d.y, m.y + s1 * d.x);
+        v[2] = make_double2(m.x + s1 * d.y, m.y - s1 * d.x);
+    }
+};
+
+template <int SIGN>
+struct Dft<4, SIGN> {
+    __device__ static inline void run(double2 *v) {
+        const double2 t0 = cadd(v[0], v[2]), t1 = csub(v[0], v[2]);
+        const double2 t2 = cadd(v[1], v[3]), t3 = mul_si<SIGN>(csub(v[1], v[3]));
+        v[0] = cadd(t0, t2);
+        v[2] = csub(t0, t2);
+        v[1] = cadd(t1, t3);
+        v[3] = csub(t1, t3);
+    }
+};
+
+// 6 = 2 x 3: X[k] = E[k%3] + w^k O[k%3], X[k+3] = E[k] - w^k O[k], w = exp(SIGN pi i / 3)
+template <int SIGN>
+struct Dft<6, SIGN> {
+    __device__ static inline void run(double2 *v) {
+        double2 e[3] = {v[0], v[2], v[4]}, o[3] = {v[1], v[3], v[5]};
+        Dft<3, SIGN>::run(e);
+        Dft<3, SIGN>::run(o);
+        const double s = SIGN * 0.86602540378443864676;
+        const double2 o1 = make_double2(0.5 * o[1].x - s * o[1].y, 0.5 * o[1].y + s * o[1].x);
+        const double2 o2 = make_double2(-0.5 * o[2].x - s * o[2].y, -0.5 * o[2].y + s * o[2].x);
+        v[0] = cadd(e[0], o[0]);
+        v[3] = csub(e[0], o[0]);
+        v[1] = cadd(e[1], o1);
+        v[4] = csub(e[1], o1);
+        v[2] = cadd(e[2], o2);
+        v[5] = csub(e[2], o2);
+    }
+};
+
+// 8 = 2 x 4: X[k] = E[k] + w^k O[k], X[k+4] = E[k] - w^k O[k], w = exp(SIGN pi i / 4)
+template <int SIGN>
+struct Dft<8, SIGN> {
+    __device__ static inline void run(double2 *v) {
+        double2 e[4] = {v[0], v[2], v[4], v[6]}, o[4] = {v[1], v[3], v[5], v[7]};
+        Dft<4, SIGN>::run(e);
+        Dft<4, SIGN>::run(o);
+        const double h = 0.70710678118654752440;
+        const double2 o1 = make_double2(h * (o[1].x - SIGN * o[1].y), h * (o[1].y + SIGN * o[1].x));
+        const double2 o2 = mul_si<SIGN>(o[2]);
+        const double2 o3 = make_double2(-h * (o[3].x + SIGN * o[3].y), h * (SIGN * o[3].x - o[3].y));
+        v[0] = cadd(e[0], o[0]);
+        v[4] = csub(e[0], o[0]);
+        v[1] = cadd(e[1], o1);
+        v[5] = csub(e[1], o1);
+        v[2] = cadd(e[2], o2);
+        v[6] = csub(e[2], o2);
+        v[3] = cadd(e[3], o3);
+        v[7] = csub(e[3], o3);
+    }
+};
+
+// exp(SIGN 2 pi i m / N) from the table tw[k] = exp(-2 pi i k / N)
+template <int SIGN>
+__device__ inline double2 twiddle(const double2 *__restrict__ tw, int m) {
+    const double2 w = tw[m];
+    return SIGN > 0 ? make_double2(w.x, -w.y) : w;
+}
+
+// One LDS -> LDS stage over S sequences (sequence s at buf + s*N); the
+// S*T butterflies are spread over the 64 lanes round by round.
+template <int N, int R, int NS, int SIGN, int S>
+__device__ inline void stage_lds(double2 *buf, const double2 *__restrict__ tw, int lane) {
+    constexpr int T = N / R;
+    constexpr int TOT = S * T;
+    constexpr int ROUNDS = (TOT + 63) / 64;
+    static_assert(N % R == 0, "radix must divide the length");
+    double2 v[ROUNDS][R];
+#pragma unroll
+    for (int c = 0; c < ROUNDS; ++c) {
+        const int b = lane + 64 * c;
+        if (TOT % 64 == 0 || b < TOT) {
+            const int s = b / T, j = b - s * T;
+            const double2 *L = buf + s * N;
+            const int jm = j % NS;
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                double2 x = L[j + r * T];
+                if (NS > 1 && r > 0) x = cmul(x, twiddle<SIGN>(tw, jm * r * (N / (NS * R))));
+                v[c][r] = x;
+            }
+            Dft<R, SIGN>::run(v[c]);
+        }
+    }
+    wave_sync();
+#pragma unroll
+    for (int c = 0; c < ROUNDS; ++c) {
+        const int b = lane + 64 * c;
+        if (TOT % 64 == 0 || b < TOT) {
+            const int s = b / T, j = b - s * T;
+            double2 *L = buf + s * N;
+            const int base = (j / NS) * NS * R + j % NS;
+#pragma unroll
+            for (int r = 0; r < R; ++r) L[base + r * NS] = v[c][r];
+        }
+    }
+    wave_sync();
+}
+
+// A run of LDS stages with radices Rs..., starting after NS points.
+template <int N, int SIGN, int S, int NS, int... Rs>
+struct Stages;
+
+template <int N, int SIGN, int S, int NS>
+struct Stages<N, SIGN, S, NS> {
+    __device__ static inline void run(double2 *, const double2 *__restrict__, int) {}
+};
+
+template <int N, int SIGN, int S, int NS, int R, int... Rs>
+struct Stages<N, SIGN, S, NS, R, Rs...> {
+    __device__ static inline void run(double2 *buf, const double2 *__restrict__ tw, int lane) {
+        stage_lds<N, R, NS, SIGN, S>(buf, tw, lane);
+        Stages<N, SIGN, S, NS * R, Rs...>::run(buf, tw, lane);
+    }
+};
+
+}  // namespace wfft
+}  // namespace coral

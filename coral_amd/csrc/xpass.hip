@@ -1,0 +1,293 @@
+// x pass of the nonlinear step on one wavefront per (z, y) line.
+//
+// For each line: inverse real FFTs of the full variables (two real lines are
+// packed into one complex sequence, f0 + i f1), the quadratic products and
+// the CFL maxima in registers, then forward real FFTs of the products (again
+// two per complex sequence, separated with Hermitian symmetry) truncated to
+// the kept kx.  This is the x-direction part of c2r_transform / r2c_transform
+// (slabs.fftw3mpi/transforms.f90:155-301) fused with the products of
+// PL_IMEX_timestepping_marching.f90:279-300 and compute_cfl_based_timestep
+// (PL_IMEX_timestepping_marching.f90:200-254).
+#include "kernels.hpp"
+#include "wavefft.hpp"
+
+#include <cstdio>
+
+namespace coral {
+
+using wfft::Dft;
+using wfft::Stages;
+
+// Radix plans.  Inverse: first stage R0 reads the packed spectrum from HBM,
+// InvMid runs in LDS, the last stage RL leaves each lane holding the points
+// j + r*N/RL of its lines.  Forward: first stage RL starts from those
+// registers, FwdRest finishes in LDS in natural order.  LPW lines per
+// wavefront make LPW * N / RL a multiple of 64.
+template <int N>
+struct XPlan;
+template <>
+struct XPlan<96> {
+    static constexpr int R0 = 4, RL = 6, LPW = 4;
+    template <int S> using InvMid = Stages<96, +1, S, 4, 4>;
+    template <int S> using FwdRest = Stages<96, -1, S, 6, 4, 4>;
+};
+template <>
+struct XPlan<192> {
+    static constexpr int R0 = 4, RL = 3, LPW = 1;
+    template <int S> using InvMid = Stages<192, +1, S, 4, 4, 4>;
+    template <int S> using FwdRest = Stages<192, -1, S, 3, 4, 4, 4>;
+};
+template <>
+struct XPlan<384> {
+    static constexpr int R0 = 4, RL = 6, LPW = 1;
+    template <int S> using InvMid = Stages<384, +1, S, 4, 4, 4>;
+    template <int S> using FwdRest = Stages<384, -1, S, 6, 4, 4, 4>;
+};
+template <>
+struct XPlan<768> {
+    static constexpr int R0 = 8, RL = 6, LPW = 1;
+    template <int S> using InvMid = Stages<768, +1, S, 8, 4, 4>;
+    template <int S> using FwdRest = Stages<768, -1, S, 6, 4, 4, 8>;
+};
+
+constexpr int XGP = 2;  // product pairs per forward batch
+
+template <int N, int NPI>
+struct XShape {
+    using P = XPlan<N>;
+    static constexpr int LPW = P::LPW, R0 = P::R0, RL = P::RL;
+    static constexpr int T0 = N / R0, TL = N / RL;
+    static constexpr int NT = LPW * TL / 64;  // (line, j) tasks per lane at the register stages
+    static constexpr int SIN = LPW * NPI;     // inverse sequences per wavefront
+    static constexpr int SF = LPW * XGP;      // forward sequences per batch
+    static constexpr int SMAX = SIN > SF ? SIN : SF;
+    static_assert((LPW * TL) % 64 == 0, "register stage must fill the wavefront");
+};
+
+// value of full variable idx (or 1 for idx < 0) at point r of a task
+template <int NPI, int RL>
+__device__ inline double fval(const double2 (&f)[NPI][RL], int idx, int r) {
+    double v = 1.0;
+#pragma unroll
+    for (int p = 0; p < NPI; ++p) {
+        if (idx == 2 * p) v = f[p][r].x;
+        if (idx == 2 * p + 1) v = f[p][r].y;
+    }
+    return v;
+}
+
+// forward transforms of product pairs [pq0, pq0 + G) and the store of the kept kx
+template <int N, int NPI, int G>
+__device__ inline void xfwd_batch(const XPassArgs &a, const double2 (&fv)[XShape<N, NPI>::NT][NPI][XPlan<N>::RL],
+                                  double2 *buf, const double2 *__restrict__ tw, int lane, long long line0,
+                                  long long nlines, int pq0) {
+    using X = XShape<N, NPI>;
+    constexpr int RL = X::RL, TL = X::TL, LPW = X::LPW;
+#pragma unroll
+    for (int c = 0; c < X::NT; ++c) {
+        const int t = lane + 64 * c, l = t / TL, j = t - l * TL;
+#pragma unroll
+        for (int gp = 0; gp < G; ++gp) {
+            const int q0 = 2 * (pq0 + gp), q1 = q0 + 1;
+            const bool has1 = q1 < a.nquad;
+            const int a0 = a.qa[q0], b0 = a.qb[q0];
+            const int a1 = has1 ? a.qa[q1] : -1, b1 = has1 ? a.qb[q1] : -1;
+            double2 y[RL];
+#pragma unroll
+            for (int r = 0; r < RL; ++r) {
+                const double p0 = fval<NPI, RL>(fv[c], a0, r) * fval<NPI, RL>(fv[c], b0, r);
+                const double p1 = has1 ? fval<NPI, RL>(fv[c], a1, r) * fval<NPI, RL>(fv[c], b1, r) : 0.0;
+                y[r] = make_double2(p0, p1);
+            }
+            Dft<RL, -1>::run(y);
+            double2 *L = buf + (l * G + gp) * N + j * RL;
+#pragma unroll
+            for (int r = 0; r < RL; ++r) L[r] = y[r];
+        }
+    }
+    wfft::wave_sync();
+    XPlan<N>::template FwdRest<LPW * G>::run(buf, tw, lane);
+    const int nkx = a.nkx;
+#pragma unroll
+    for (int l = 0; l < LPW; ++l) {
+        const long long line = line0 + l;
+        if (line < nlines) {
+#pragma unroll
+            for (int gp = 0; gp < G; ++gp) {
+                const int q0 = 2 * (pq0 + gp), q1 = q0 + 1;
+                const double2 *L = buf + (l * G + gp) * N;
+                double2 *h0 = a.H + ((size_t)q0 * nlines + line) * nkx;
+                double2 *h1 = a.H + ((size_t)q1 * nlines + line) * nkx;
+                for (int k = lane; k < nkx; k += 64) {
+                    const double2 y = L[k], yc = cconj(L[k == 0 ? 0 : N - k]);
+                    // p0 = (Y + conj Y(-k)) / 2,  p1 = -i (Y - conj Y(-k)) / 2
+                    h0[k] = make_double2(0.5 * (y.x + yc.x), 0.5 * (y.y + yc.y));
+                    if (q1 < a.nquad) h1[k] = make_double2(0.5 * (y.y - yc.y), -0.5 * (y.x - yc.x));
+                }
+            }
+        }
+    }
+    wfft::wave_sync();
+}
+
+template <int N, int NPI>
+__global__ __launch_bounds__(256) void k_xpass_wave(XPassArgs a, const double2 *__restrict__ tw, long long nlines) {
+    using X = XShape<N, NPI>;
+    constexpr int R0 = X::R0, RL = X::RL, T0 = X::T0, TL = X::TL, LPW = X::LPW, SIN = X::SIN;
+    extern __shared__ double2 lds[];
+    __shared__ double cfl_part[4][3];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    double2 *buf = lds + wid * X::SMAX * N;
+    const long long line0 = ((long long)blockIdx.x * (blockDim.x >> 6) + wid) * LPW;
+    const int nkx = a.nkx;
+
+    // ---- inverse, first stage: packed Hermitian spectra straight from HBM ----
+    {
+        constexpr int TOT = SIN * T0, ROUNDS = (TOT + 63) / 64;
+        double2 v[ROUNDS][R0];
+#pragma unroll
+        for (int c = 0; c < ROUNDS; ++c) {
+            const int b = lane + 64 * c;
+            if (TOT % 64 == 0 || b < TOT) {
+                const int s = b / T0, j = b - s * T0, l = s / NPI, p = s - l * NPI;
+                const long long line = line0 + l;
+                const bool valid = line < nlines;
+                const int f0 = 2 * p, f1 = 2 * p + 1;
+                const bool has1 = f1 < a.nfull;
+                const size_t lo = valid ? (size_t)line : 0;
+                const double2 *g0 = a.G + ((size_t)f0 * nlines + lo) * nkx;
+                const double2 *g1 = a.G + ((size_t)(has1 ? f1 : f0) * nlines + lo) * nkx;
+#pragma unroll
+                for (int r = 0; r < R0; ++r) {
+                    const int e = j + r * T0;
+                    const bool pos = e < nkx, neg = e > N - nkx;
+                    const int k = pos ? e : (neg ? N - e : 0);
+                    double2 A = g0[k], B = g1[k];
+                    const double bx = has1 ? B.x : 0.0, by = (has1 && e != 0) ? B.y : 0.0;
+                    const double ay = e != 0 ? A.y : 0.0;
+                    // pos: A + iB;  neg: conj(A) + i conj(B)
+                    double zx = pos ? A.x - by : A.x + by;
+                    double zy = pos ? ay + bx : bx - ay;
+                    const bool keep = valid && (pos || neg);
+                    v[c][r] = make_double2(keep ? zx : 0.0, keep ? zy : 0.0);
+                }
+                Dft<R0, +1>::run(v[c]);
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < ROUNDS; ++c) {
+            const int b = lane + 64 * c;
+            if (TOT % 64 == 0 || b < TOT) {
+                const int s = b / T0, j = b - s * T0;
+#pragma unroll
+                for (int r = 0; r < R0; ++r) buf[s * N + j * R0 + r] = v[c][r];
+            }
+        }
+        wfft::wave_sync();
+    }
+    XPlan<N>::template InvMid<SIN>::run(buf, tw, lane);
+
+    // ---- inverse, last stage into registers: fv[task][pair][r] at x = j + r*TL ----
+    double2 fv[X::NT][NPI][RL];
+#pragma unroll
+    for (int c = 0; c < X::NT; ++c) {
+        const int t = lane + 64 * c, l = t / TL, j = t - l * TL;
+#pragma unroll
+        for (int p = 0; p < NPI; ++p) {
+            const double2 *L = buf + (l * NPI + p) * N;
+#pragma unroll
+            for (int r = 0; r < RL; ++r) {
+                double2 x = L[j + r * TL];
+                if (r > 0) x = cmul(x, wfft::twiddle<+1>(tw, j * r));
+                fv[c][p][r] = x;
+            }
+            Dft<RL, +1>::run(fv[c][p]);
+        }
+    }
+    wfft::wave_sync();
+
+    // ---- CFL maxima over u, v, w / weight_z (full variables 0, 1, 2) ----
+    if constexpr (NPI >= 2) {
+      if (a.cflmax && a.nfull >= 3) {
+        double mu = 0.0, mv = 0.0, mw = 0.0;
+#pragma unroll
+        for (int c = 0; c < X::NT; ++c) {
+            const int t = lane + 64 * c, l = t / TL;
+            const long long line = line0 + l;
+            if (line < nlines) {
+                const double zw = a.zweight[line / a.NYAA];
+#pragma unroll
+                for (int r = 0; r < RL; ++r) {
+                    mu = fmax(mu, fabs(fv[c][0][r].x));
+                    mv = fmax(mv, fabs(fv[c][0][r].y));
+                    mw = fmax(mw, fabs(fv[c][1][r].x / zw));
+                }
+            }
+        }
+        for (int off = 32; off > 0; off >>= 1) {
+            mu = fmax(mu, __shfl_xor(mu, off));
+            mv = fmax(mv, __shfl_xor(mv, off));
+            mw = fmax(mw, __shfl_xor(mw, off));
+        }
+        if (lane == 0) {
+            cfl_part[wid][0] = mu;
+            cfl_part[wid][1] = mv;
+            cfl_part[wid][2] = mw;
+        }
+        __syncthreads();
+        if (threadIdx.x < 3) {
+            double m = 0.0;
+            for (int w = 0; w < (int)(blockDim.x >> 6); ++w) m = fmax(m, cfl_part[w][threadIdx.x]);
+            atomicMax(&a.cflmax[threadIdx.x], (unsigned long long)__double_as_longlong(m));
+        }
+      }
+    }
+
+    // ---- products and forward transforms, XGP pairs at a time ----
+    const int npairs = (a.nquad + 1) / 2;
+    for (int pq0 = 0; pq0 < npairs; pq0 += XGP) {
+        if (npairs - pq0 >= XGP)
+            xfwd_batch<N, NPI, XGP>(a, fv, buf, tw, lane, line0, nlines, pq0);
+        else
+            xfwd_batch<N, NPI, 1>(a, fv, buf, tw, lane, line0, nlines, pq0);
+    }
+}
+
+template <int N, int NPI>
+static bool launch_wave(hipStream_t st, const FFTPlan &px, const XPassArgs &a) {
+    using X = XShape<N, NPI>;
+    const long long nlines = (long long)a.NZAA * a.NYAA;
+    const size_t per_wave = size_t(X::SMAX) * N * sizeof(double2);
+    int wpb = 4;
+    while (wpb > 1 && wpb * per_wave > 65536) wpb /= 2;
+    if (wpb * per_wave > 65536) return false;
+    const long long per_block = (long long)wpb * X::LPW;
+    const unsigned blocks = unsigned((nlines + per_block - 1) / per_block);
+    hipLaunchKernelGGL((k_xpass_wave<N, NPI>), dim3(blocks), dim3(64 * wpb), wpb * per_wave, st, a, px.tw, nlines);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) fprintf(stderr, "coral_amd: launch failed: %s\n", hipGetErrorString(e));
+    return true;
+}
+
+template <int N>
+static bool launch_wave_n(hipStream_t st, const FFTPlan &px, const XPassArgs &a) {
+    switch ((a.nfull + 1) / 2) {
+    case 1: return launch_wave<N, 1>(st, px, a);
+    case 2: return launch_wave<N, 2>(st, px, a);
+    case 3: return launch_wave<N, 3>(st, px, a);
+    default: return false;
+    }
+}
+
+bool launch_xpass_wave(hipStream_t st, const FFTPlan &px, const XPassArgs &a) {
+    if (a.nfull < 1 || 2 * a.nkx > px.N) return false;
+    switch (px.N) {
+    case 96: return launch_wave_n<96>(st, px, a);
+    case 192: return launch_wave_n<192>(st, px, a);
+    case 384: return launch_wave_n<384>(st, px, a);
+    case 768: return launch_wave_n<768>(st, px, a);
+    default: return false;
+    }
+}
+
+}  // namespace coral

@@ -195,6 +195,12 @@ class Solver:
         check(lib.coral_amd_system_info(self._ctx, isys, *[ctypes.byref(x) for x in v]), "coral_amd_system_info")
         return tuple(x.value for x in v)  # total, nl, nu, nkept
 
+    def factor_is_real(self, isys: int = 0) -> bool:
+        """True when kxky system `isys` is factored in real form."""
+        v = ctypes.c_int32()
+        check(lib.coral_amd_factor_kind(self._ctx, isys, ctypes.byref(v)), "coral_amd_factor_kind")
+        return bool(v.value)
+
     def bwd_transform(self, spec: np.ndarray) -> np.ndarray:
         s = np.ascontiguousarray(spec, np.complex128)
         if s.shape != (self.NX // 2, self.NYAA, self.NZ):

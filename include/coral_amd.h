@@ -138,6 +138,15 @@ int coral_amd_wavenumbers(coral_amd_ctx *ctx, int32_t *kx_int, int32_t *ky_int,
 int coral_amd_system_info(coral_amd_ctx *ctx, int32_t isys, int32_t *total, int32_t *nl,
                           int32_t *nu, int32_t *nkept);
 
+/* Arithmetic of the factored evolution operator of kxky system isys: 1 when it
+ * is factored in real form (each variable scaled by i^phi, each equation by
+ * i^-psi, which makes every (i kx)^a (i ky)^b piece real), 0 when complex.
+ * The reference factors the complex operator (zgbtrf, PL_algebra_z_3D.f90:245-276);
+ * pivot rows and solutions agree, since the scaling leaves |Re|+|Im| of each
+ * entry unchanged.  Setting CORAL_AMD_COMPLEX_FACTORS=1 before
+ * coral_amd_create forces the complex form. */
+int coral_amd_factor_kind(coral_amd_ctx *ctx, int32_t isys, int32_t *is_real);
+
 /* Unit seams on host buffers.
  * Backward transform of a spectral field, C shape [NX/2][NYAA][NZ] complex, to
  * physical C shape [NXAA][NYAA][NZAA] (c2r_transform, slabs.fftw3mpi/transforms.f90:155-225);

@@ -55,6 +55,7 @@ def test_step_parity(solver_mod, case):
     g = solver_mod.Solver(r, NX, NY, NZ, Lx, Ly, scheme=scheme)
     try:
         g.set_state(k0, z0)
+        assert g.factor_is_real(0)
         for it in range(nsteps):
             ora.step(dt)
             g.march_forward(dt)
@@ -67,6 +68,65 @@ def test_step_parity(solver_mod, case):
         phys = ora.physical_full_variables()
         for f in range(len(phys)):
             assert rel(g.physical(f), phys[f]) <= TOL
+    finally:
+        g.close()
+
+
+def test_complex_factor_path_parity(solver_mod, monkeypatch):
+    """The complex-arithmetic factor/solve (used for recipes without a real
+    form) on a recipe that has one, forced through CORAL_AMD_COMPLEX_FACTORS."""
+    name, NX, NY, NZ, Lx, Ly, scheme, dt, nsteps = CASES[2]
+    monkeypatch.setenv("CORAL_AMD_COMPLEX_FACTORS", "1")
+    r = product_recipe(name)
+    k0, z0 = ic.rbc_initial_state(r, NX, NY, NZ, 1e-2, 6, 6, 3)
+    geo = O.Geometry(NX, NY, NZ, Lx, Ly)
+    ora = O.OracleSolver(r, geo, scheme)
+    ora.field, ora.zfield = [a.copy() for a in k0], [a.copy() for a in z0]
+    g = solver_mod.Solver(r, NX, NY, NZ, Lx, Ly, scheme=scheme)
+    try:
+        assert not g.factor_is_real(0)
+        g.set_state(k0, z0)
+        for it in range(nsteps):
+            ora.step(dt)
+            g.march_forward(dt)
+        k, z = g.get_state()
+        assert rel(k[0], ora.field[0]) <= TOL
+        for a, b in zip(z, ora.zfield):
+            assert rel(a, b) <= TOL
+    finally:
+        g.close()
+
+
+BROADBAND = [
+    # name, NX, NY, NZ, Lx, Ly, scheme, dt, nsteps: every kept mode of t seeded
+    ("coral.equations.rrbc_ra5e7_ek1e-4", 256, 256, 17, 2 * np.pi, 2 * np.pi, "ARS_222", 2e-5, 2),
+    ("coral.equations.rbc_ra1e5", 128, 96, 17, 2 * np.pi, 3.0, "ARS_222", 1e-4, 2),
+    ("coral.equations.rbc_ra1e5", 64, 256, 17, 2.0, 2 * np.pi, "ARS_443", 1e-4, 1),
+]
+
+
+@pytest.mark.parametrize("case", BROADBAND, ids=[f"{c[0].split('.')[-1]}-{c[1]}x{c[2]}x{c[3]}" for c in BROADBAND])
+def test_broadband_step_parity(solver_mod, case):
+    """All kept horizontal modes populated, at the transform lengths of the
+    production grids (NXAA, NYAA in 96 .. 384)."""
+    name, NX, NY, NZ, Lx, Ly, scheme, dt, nsteps = case
+    print("case", case, flush=True)
+    r = product_recipe(name)
+    k0, z0 = ic.rbc_initial_state(r, NX, NY, NZ, 1e-2, 10 * max(NX, NY), 8, 11)
+    geo = O.Geometry(NX, NY, NZ, Lx, Ly)
+    ora = O.OracleSolver(r, geo, scheme)
+    ora.field, ora.zfield = [a.copy() for a in k0], [a.copy() for a in z0]
+    g = solver_mod.Solver(r, NX, NY, NZ, Lx, Ly, scheme=scheme)
+    try:
+        g.set_state(k0, z0)
+        for it in range(nsteps):
+            ora.step(dt)
+            g.march_forward(dt)
+        k, z = g.get_state()
+        assert rel(k[0], ora.field[0]) <= TOL
+        for a, b in zip(z, ora.zfield):
+            assert rel(a, b) <= TOL
+        assert abs(g.cfl_dt() / ora.cfl_dt - 1.0) <= 1e-10
     finally:
         g.close()
 

@@ -123,6 +123,60 @@ def test_product_assembly_matches_oracle(native_lib, name, NZ):
                 assert (nl, nu) == O.bandwidth(mats)
 
 
+@pytest.mark.parametrize("name,NZ", [("rbc_vanilla.coral.equations", 17),
+                                     ("rrbc_julien.coral.equations.velocityPressure", 25),
+                                     ("stressfree_rb.coral.equations", 24)])
+def test_evolution_operator_has_real_form(native_lib, name, NZ):
+    """The GPU factors M - dt a L in real arithmetic after scaling unknowns and
+    equations by powers of i.  Found here from the matrix entries alone: every
+    nonzero of the assembled complex operator is real or imaginary, and a
+    parity labelling of rows and columns (breadth-first over the nonzeros)
+    makes every entry of D_row^-1 A D_col real."""
+    from coral_amd.solver import assembled_operator
+    r = recipe(name)
+    for kxi, kyi in [(1, 0), (3, -2), (7, 5)]:
+        M, nl, nu = assembled_operator(r, 16, 16, NZ, 2.0, 3.0, 0, "mass", kxi, kyi)
+        L, _, _ = assembled_operator(r, 16, 16, NZ, 2.0, 3.0, 0, "stif", kxi, kyi)
+        A = M - 0.37 * L
+        n = A.shape[0]
+        nz = np.argwhere(np.abs(A) > 0)
+        imag = np.abs(A.imag) > np.abs(A.real)
+        # each entry is (numerically) purely real or purely imaginary
+        small = np.where(imag, np.abs(A.real), np.abs(A.imag))
+        assert np.max(small) <= 1e-14 * np.max(np.abs(A))
+        row, col = -np.ones(n, int), -np.ones(n, int)
+        adj_r = [[] for _ in range(n)]
+        adj_c = [[] for _ in range(n)]
+        for i, j in nz:
+            adj_r[i].append(j)
+            adj_c[j].append(i)
+        for start in range(n):
+            if row[start] >= 0 or not adj_r[start]:
+                continue
+            row[start] = 0
+            stack = [("r", start)]
+            while stack:
+                kind, u = stack.pop()
+                if kind == "r":
+                    for j in adj_r[u]:
+                        want = row[u] ^ int(imag[u, j])
+                        if col[j] < 0:
+                            col[j] = want
+                            stack.append(("c", j))
+                        assert col[j] == want
+                else:
+                    for i in adj_c[u]:
+                        want = col[u] ^ int(imag[i, u])
+                        if row[i] < 0:
+                            row[i] = want
+                            stack.append(("r", i))
+                        assert row[i] == want
+        row[row < 0] = 0
+        col[col < 0] = 0
+        Ar = (1j ** -row)[:, None] * A * (1j ** col)[None, :]
+        assert np.max(np.abs(Ar.imag)) <= 1e-14 * np.max(np.abs(A))
+
+
 # --- stress-free Rayleigh-Benard growth rate ----------------------------------
 
 def test_stressfree_growth_rate(reference_numbers):

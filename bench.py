@@ -92,6 +92,8 @@ def main():
     ap.add_argument("--workload", default="rrbc256", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--amp", type=float, default=None,
+                    help="amplitude of the seeded temperature perturbation (default: the workload's)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB (5 times); its FETCH_SIZE/WRITE_SIZE "
                          "counters calibrate the per-kernel HBM traffic figures")
@@ -120,7 +122,8 @@ def main():
 
     recipe = read_equations(os.path.join(ROOT, "coral_amd", "inputs", wl["eq"]))
     NX, NZ = wl["NX"], wl["NZ"]
-    k0, z0 = rbc_initial_state(recipe, NX, NX, NZ, wl["amp"], wl["kmax"], wl["nmodes"], wl["seed"])
+    amp = args.amp if args.amp is not None else wl["amp"]
+    k0, z0 = rbc_initial_state(recipe, NX, NX, NZ, amp, wl["kmax"], wl["nmodes"], wl["seed"])
     g = Solver(recipe, NX, NX, NZ, 2 * math.pi, 2 * math.pi, scheme="ARS_222")
     total, kl, ku, nsys = g.system_info(0)
     g.set_state(k0, z0)
@@ -138,10 +141,12 @@ def main():
     elapsed = time.perf_counter() - t0
     timers = g.timers()
     g.timers_enable(False)
+    cfl_final = g.cfl_dt()
     kfin, _ = g.get_state()
     finite = bool(np.isfinite(kfin[0]).all())
     norm = float(np.linalg.norm(kfin[0]))
-    log(f"{steps} steps in {elapsed:.3f} s, state finite={finite}, |state|={norm:.6e}")
+    log(f"{steps} steps in {elapsed:.3f} s, state finite={finite}, |state|={norm:.6e}, CFL dt of the last stage "
+        f"{cfl_final:.3e} (dt {wl['dt']:.1e})")
 
     P = dict(N=total, kl=kl, ku=ku, nsys=nsys, real=g.factor_is_real(0), NZ=NZ, NZAA=3 * NZ // 2, NYAA=3 * NX // 2, nkx=NX // 2,
              nfull=len(recipe.full_vars), nquad=len(recipe.quad_vars))
@@ -197,7 +202,7 @@ def main():
         "data": "synthetic (seeded perturbation, no checkpoint)",
         "config": {"workload": wl["name"], "grid": [NX, NX, NZ], "systems": nsys, "unknowns_per_system": total,
                    "band": [kl, ku], "factors": "real form" if P["real"] else "complex",
-                   "state_finite": finite},
+                   "ic_amplitude": amp, "state_finite": finite, "cfl_dt_last_stage": cfl_final},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "kernels_ms_per_step": {k: round(v["ms_per_step"], 4) for k, v in per_kernel.items()},

@@ -62,6 +62,7 @@ struct SysDev {
     std::vector<std::unique_ptr<DevBuf<double2>>> Khat, Kstd;
     DevBuf<double2> Lf, Uf;      // complex factors
     bool real = false;           // factored in real form (Lr, Ur, rot)
+    int ncol16 = 0;              // > 0: 16-lane solve layout (see solve16_ncol)
     DevBuf<double> Lr, Ur;
     DevBuf<unsigned char> rot;
     DevBuf<unsigned char> piv;
@@ -102,6 +103,8 @@ struct coral_amd_ctx {
     DevBuf<double2> spec, Fz, G, H, Hy, Qhat;
     DevBuf<double> physd;
     DevBuf<double> Tz, Cz, zweight;
+    DevBuf<double> zfrag_b, zfrag_f;   // MFMA fragments of the folded z transforms
+    ZFragsDev zdev{};
     DevBuf<double2> twx, twy;
     DevBuf<int> qa, qb;
     DevBuf<unsigned long long> cflmax;
@@ -249,13 +252,20 @@ void fill_system_device(SysDev &d, int nstages) {
     const int kv = t.nl + t.nu;
     const char *force = std::getenv("CORAL_AMD_COMPLEX_FACTORS");
     d.real = t.real_form && !(force && force[0] == '1');
+    // 16-lane solve layout when the band allows it; U arrays padded (zeroed) on
+    // both sides for its out-of-record reads
+    d.ncol16 = solve16_ncol(t.total, t.nl, kv);
+    const size_t lsize = d.ncol16 > 0 ? size_t(d.nsys) * d.ncol16 * 16 : size_t(d.nsys) * t.total * std::max(t.nl, 1);
+    const size_t usize = size_t(d.nsys) * t.total * (kv + 1) + 2 * size_t(SOLVE16_PAD);
     if (d.real) {
-        d.Lr.alloc(size_t(d.nsys) * t.total * std::max(t.nl, 1));
-        d.Ur.alloc(size_t(d.nsys) * t.total * (kv + 1));
+        d.Lr.alloc(lsize);
+        d.Ur.alloc(usize);
+        HIP_OK(hipMemset(d.Ur.p, 0, usize * sizeof(double)));
         d.rot.upload(t.rot);
     } else {
-        d.Lf.alloc(size_t(d.nsys) * t.total * std::max(t.nl, 1));
-        d.Uf.alloc(size_t(d.nsys) * t.total * (kv + 1));
+        d.Lf.alloc(lsize);
+        d.Uf.alloc(usize);
+        HIP_OK(hipMemset(d.Uf.p, 0, usize * sizeof(double2)));
     }
     d.piv.alloc(size_t(d.nsys) * t.total);
     d.info.alloc(d.nsys);
@@ -286,10 +296,11 @@ void factorize(coral_amd_ctx &c, SysDev &d, double dt) {
         a.km = d.kxky ? c.km : KeptModes{1, 1, 2, 0.0, 0.0};
         a.work = work.p;
         a.Lf = d.Lf.p;
-        a.Uf = d.Uf.p;
+        a.Uf = d.Uf.p ? d.Uf.p + SOLVE16_PAD : nullptr;
         a.rot = d.real ? d.rot.p : nullptr;
         a.Lr = d.Lr.p;
-        a.Ur = d.Ur.p;
+        a.Ur = d.Ur.p ? d.Ur.p + SOLVE16_PAD : nullptr;
+        a.ncol16 = d.ncol16;
         a.piv = d.piv.p;
         a.info = d.info.p;
         c.timed("factor", [&] { launch_factor(c.stream, a); });
@@ -323,6 +334,18 @@ void band_apply(coral_amd_ctx &c, SysDev &d, bool mass, const double2 *x, double
     a.km = d.kxky ? c.km : KeptModes{1, 1, 2, 0.0, 0.0};
     a.y = y;
     c.timed(mass ? "rhs" : "kstd", [&] { launch_band_apply(c.stream, a); });
+}
+
+// z transforms: MFMA kernels where a variant covers the shape, plain GEMM otherwise
+void zbwd(coral_amd_ctx &c, const double2 *spec, double2 *Fz, int nb) {
+    const Problem &P = c.P;
+    if (!launch_zbwd_mfma(c.stream, c.zdev, spec, Fz, P.NZ, P.NZAA, P.nsys, nb))
+        launch_zbwd(c.stream, c.Tz.p, spec, Fz, P.NZAA, P.NZ, P.nsys, nb);
+}
+void zfwd(coral_amd_ctx &c, const double2 *Hy, double2 *Qhat, int nb) {
+    const Problem &P = c.P;
+    if (!launch_zfwd_mfma(c.stream, c.zdev, Hy, Qhat, P.NZ, P.NZAA, P.nsys, nb))
+        launch_zfwd(c.stream, c.Cz.p, Hy, Qhat, P.NZ, P.NZAA, P.nsys, nb);
 }
 
 void nl_assemble(coral_amd_ctx &c, SysDev &d, double2 *Khat) {
@@ -367,11 +390,11 @@ void backward_to_G(coral_amd_ctx &c, const std::vector<const double2 *> &kst, co
         }
         for (auto &e : c.ext) {
             if (e->kind != 1) continue;
-            launch_extract_mean(c.stream, zst[e->system], P.NZ, e->rowptr.p, e->col.p, e->val.p,
+            launch_extract_mean(c.stream, zst[e->system], P.NZ, P.nsys, e->rowptr.p, e->col.p, e->val.p,
                                 c.spec.p + e->full * fsz);
         }
     });
-    c.timed("zbwd", [&] { launch_zbwd(c.stream, c.Tz.p, c.spec.p, c.Fz.p, P.NZAA, P.NZ, P.nsys, P.nfull); });
+    c.timed("zbwd", [&] { zbwd(c, c.spec.p, c.Fz.p, P.nfull); });
     c.timed("ybwd", [&] { launch_ybwd(c.stream, c.py, c.Fz.p, c.G.p, P.nfull, P.NZAA, P.nkx, P.nky, P.NY); });
 }
 
@@ -394,7 +417,7 @@ void nl_terms(coral_amd_ctx &c, const std::vector<const double2 *> &kst, const s
     xa.cflmax = c.cflmax.p;
     c.timed("xpass", [&] { launch_xpass(c.stream, c.px, xa); });
     c.timed("yfwd", [&] { launch_yfwd(c.stream, c.py, c.H.p, c.Hy.p, P.nquad, P.NZAA, P.nkx, P.nky, P.NY); });
-    c.timed("zfwd", [&] { launch_zfwd(c.stream, c.Cz.p, c.Hy.p, c.Qhat.p, P.NZ, P.NZAA, P.nsys, P.nquad); });
+    c.timed("zfwd", [&] { zfwd(c, c.Hy.p, c.Qhat.p, P.nquad); });
     for (int q = 0; q < P.nquad; ++q)
         if (P.quad_filter[q]) launch_remove_zmean(c.stream, c.Qhat.p + size_t(q) * P.nsys * P.NZ, P.NZ, P.nsys);
     for (auto &d : c.ks) nl_assemble(c, *d, d->Khat[stage]->p);
@@ -422,11 +445,18 @@ void one_step(coral_amd_ctx &c) {
             band_apply(c, d, true, d.field.p, d.aux.p, vecs);
             const SystemTables &t = *d.t;
             c.timed(d.kxky ? "solve" : "solve_mean", [&] {
-                if (d.real)
-                    launch_solve_real(c.stream, d.Lr.p, d.Ur.p, d.piv.p, d.rot.p, d.kxky ? 1 : 0, t.total, t.nl,
-                                      t.nl + t.nu, d.nsys, d.aux.p, d.aux.p);
+                const int kv = t.nl + t.nu, id0 = d.kxky ? 1 : 0;
+                if (d.ncol16 > 0 && d.real)
+                    launch_solve16_real(c.stream, d.Lr.p, d.ncol16, d.Ur.p + SOLVE16_PAD, d.piv.p, d.rot.p, id0,
+                                        t.total, t.nl, kv, d.nsys, d.aux.p, d.aux.p);
+                else if (d.ncol16 > 0)
+                    launch_solve16_complex(c.stream, d.Lf.p, d.ncol16, d.Uf.p + SOLVE16_PAD, d.piv.p, id0, t.total,
+                                           t.nl, kv, d.nsys, d.aux.p, d.aux.p);
+                else if (d.real)
+                    launch_solve_real(c.stream, d.Lr.p, d.Ur.p + SOLVE16_PAD, d.piv.p, d.rot.p, id0, t.total, t.nl,
+                                      kv, d.nsys, d.aux.p, d.aux.p);
                 else
-                    launch_solve(c.stream, d.Lf.p, d.Uf.p, d.piv.p, t.total, t.nl, t.nl + t.nu, d.nsys, d.aux.p,
+                    launch_solve(c.stream, d.Lf.p, d.Uf.p + SOLVE16_PAD, d.piv.p, t.total, t.nl, kv, d.nsys, d.aux.p,
                                  d.aux.p);
             });
             band_apply(c, d, false, d.aux.p, d.Kstd[i]->p, {});
@@ -546,6 +576,19 @@ int coral_amd_create(const coral_amd_problem *prob, int device, coral_amd_ctx **
         HIP_OK(hipMemset(c->Qhat.p, 0, c->Qhat.n * sizeof(double2)));
         c->Tz.upload(P.Tz);
         c->Cz.upload(P.Cz);
+        {
+            ZFrags zf;
+            make_zfrags(P.NZ, P.NZAA, P.Tz, P.Cz, device, zf);
+            if (zf.bwd.empty()) zf.bwd.push_back(0.0);
+            if (zf.fwd.empty()) zf.fwd.push_back(0.0);
+            c->zfrag_b.upload(zf.bwd);
+            c->zfrag_f.upload(zf.fwd);
+            int ncu = 0;
+            HIP_OK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device));
+            c->zdev = ZFragsDev{zf.TE, zf.TO, zf.NEK, zf.NOK, zf.RE, zf.RO, zf.KEK, zf.KOK,
+                                zf.bwd_ok && !std::getenv("CORAL_AMD_NO_MFMA"), zf.fwd_ok && !std::getenv("CORAL_AMD_NO_MFMA"),
+                                std::max(ncu, 1), c->zfrag_b.p, c->zfrag_f.p};
+        }
         c->zweight.upload(P.zweight);
         std::vector<int> qa = P.quad_a, qb = P.quad_b;
         if (qa.empty()) {
@@ -780,6 +823,167 @@ int coral_amd_get_physical(coral_amd_ctx *c, int32_t ifull, double *out) {
     }
 }
 
+namespace {
+// Maps of one coupled variable: Chebyshev coefficients c = S g of its Galerkin
+// coefficients g (stencil S, NZ x ncol, chebyshev_galerkin_2.f90) at the
+// interleaved positions pos, and the inverse of the square truncation
+// S[0:ncol, 0:ncol] used to go back (prepare_stencils, PL_IMEX_timestepping.f90:663-723).
+struct VarMaps {
+    int ncol = 0;
+    std::vector<int> pos;
+    Dense S;
+    std::vector<double> Sinv;  // ncol x ncol, row-major
+};
+
+VarMaps var_maps(const SystemTables &t, int NZ, int v) {
+    if (v < 0 || v >= t.nvars) throw std::runtime_error("bad variable index");
+    VarMaps m;
+    m.ncol = t.var_size[v];
+    for (int j = 0; j < m.ncol; ++j) m.pos.push_back(t.posP[t.var_first[v] + j]);
+    m.S = galerkin_stencil(NZ, t.bc_codes[v]);
+    // inverse of the square truncation by Gauss-Jordan elimination with partial pivoting
+    const int n = m.ncol;
+    std::vector<double> A(size_t(n) * n), B(size_t(n) * n, 0.0);
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j) A[size_t(i) * n + j] = m.S(i, j);
+    for (int i = 0; i < n; ++i) B[size_t(i) * n + i] = 1.0;
+    for (int col = 0; col < n; ++col) {
+        int piv = col;
+        for (int r = col + 1; r < n; ++r)
+            if (std::fabs(A[size_t(r) * n + col]) > std::fabs(A[size_t(piv) * n + col])) piv = r;
+        if (A[size_t(piv) * n + col] == 0.0) throw std::runtime_error("singular square stencil");
+        if (piv != col)
+            for (int j = 0; j < n; ++j) {
+                std::swap(A[size_t(piv) * n + j], A[size_t(col) * n + j]);
+                std::swap(B[size_t(piv) * n + j], B[size_t(col) * n + j]);
+            }
+        const double d = A[size_t(col) * n + col];
+        for (int j = 0; j < n; ++j) {
+            A[size_t(col) * n + j] /= d;
+            B[size_t(col) * n + j] /= d;
+        }
+        for (int r = 0; r < n; ++r) {
+            if (r == col) continue;
+            const double f = A[size_t(r) * n + col];
+            if (f == 0.0) continue;
+            for (int j = 0; j < n; ++j) {
+                A[size_t(r) * n + j] -= f * A[size_t(col) * n + j];
+                B[size_t(r) * n + j] -= f * B[size_t(col) * n + j];
+            }
+        }
+    }
+    m.Sinv = B;
+    return m;
+}
+}  // namespace
+
+int coral_amd_get_variable_physical(coral_amd_ctx *c, int32_t kind, int32_t isys, int32_t ivar, double *out) {
+    try {
+        if (!c || !out) throw std::runtime_error("null argument");
+        const Problem &P = c->P;
+        HIP_OK(hipSetDevice(c->device));
+        if (kind == 0) {
+            if (isys < 0 || isys >= int(c->ks.size())) throw std::runtime_error("bad kxky system index");
+            SysDev &d = *c->ks[isys];
+            const VarMaps m = var_maps(*d.t, P.NZ, ivar);
+            std::vector<int> rowptr(1, 0), col;
+            std::vector<double> val;
+            for (int n = 0; n < P.NZ; ++n) {
+                for (int j = 0; j < m.ncol; ++j)
+                    if (m.S(n, j) != 0.0) {
+                        col.push_back(m.pos[j]);
+                        val.push_back(m.S(n, j));
+                    }
+                rowptr.push_back(int(col.size()));
+            }
+            DevBuf<int> drow, dcol;
+            DevBuf<double> dval;
+            drow.upload(rowptr);
+            dcol.upload(col);
+            dval.upload(val);
+            launch_extract(c->stream, d.field.p, d.t->total, P.nsys, P.NZ, drow.p, dcol.p, dval.p, 0, 0, c->km,
+                           c->spec.p, 0);
+            zbwd(*c, c->spec.p, c->Fz.p, 1);
+            launch_ybwd(c->stream, c->py, c->Fz.p, c->G.p, 1, P.NZAA, P.nkx, P.nky, P.NY);
+            ensure_phys(*c);
+            launch_xbwd(c->stream, c->px, c->G.p, c->physd.p, 1, (long long)P.NZAA * P.NYAA, P.nkx);
+            download_phys(*c, out);
+        } else if (kind == 1) {
+            if (isys < 0 || isys >= int(c->zs.size())) throw std::runtime_error("bad mean system index");
+            SysDev &d = *c->zs[isys];
+            const VarMaps m = var_maps(*d.t, P.NZ, ivar);
+            HIP_OK(hipStreamSynchronize(c->stream));
+            std::vector<double2> f(d.t->total);
+            HIP_OK(hipMemcpy(f.data(), d.field.p, f.size() * sizeof(double2), hipMemcpyDeviceToHost));
+            std::vector<double> cz(P.NZ, 0.0);
+            for (int n = 0; n < P.NZ; ++n)
+                for (int j = 0; j < m.ncol; ++j) cz[n] += m.S(n, j) * f[m.pos[j]].x;
+            for (int z = 0; z < P.NZAA; ++z) {
+                double v = 0.0;
+                for (int n = 0; n < P.NZ; ++n) v += P.Tz[size_t(z) * P.NZ + n] * cz[n];
+                out[z] = v;
+            }
+        } else {
+            throw std::runtime_error("kind must be 0 (kxky) or 1 (mean)");
+        }
+        return 0;
+    } catch (const std::exception &e) {
+        return fail(e);
+    }
+}
+
+int coral_amd_set_variable_physical(coral_amd_ctx *c, int32_t kind, int32_t isys, int32_t ivar, const double *in) {
+    try {
+        if (!c || !in) throw std::runtime_error("null argument");
+        const Problem &P = c->P;
+        HIP_OK(hipSetDevice(c->device));
+        HIP_OK(hipStreamSynchronize(c->stream));
+        if (kind == 0) {
+            if (isys < 0 || isys >= int(c->ks.size())) throw std::runtime_error("bad kxky system index");
+            SysDev &d = *c->ks[isys];
+            const VarMaps m = var_maps(*d.t, P.NZ, ivar);
+            ensure_phys(*c);
+            std::vector<double> h(size_t(P.NXAA) * P.NYAA * P.NZAA);
+            for (int x = 0; x < P.NXAA; ++x)
+                for (int y = 0; y < P.NYAA; ++y)
+                    for (int z = 0; z < P.NZAA; ++z)
+                        h[(size_t(z) * P.NYAA + y) * P.NXAA + x] = in[(size_t(x) * P.NYAA + y) * P.NZAA + z];
+            HIP_OK(hipMemcpy(c->physd.p, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
+            launch_xfwd(c->stream, c->px, c->physd.p, c->H.p, 1, (long long)P.NZAA * P.NYAA, P.nkx);
+            launch_yfwd(c->stream, c->py, c->H.p, c->Hy.p, 1, P.NZAA, P.nkx, P.nky, P.NY);
+            zfwd(*c, c->Hy.p, c->Qhat.p, 1);
+            DevBuf<double> dsinv;
+            DevBuf<int> dpos;
+            dsinv.upload(m.Sinv);
+            dpos.upload(m.pos);
+            launch_insert_var(c->stream, c->Qhat.p, P.nsys, m.ncol, dsinv.p, dpos.p, d.field.p, d.t->total);
+            HIP_OK(hipStreamSynchronize(c->stream));
+        } else if (kind == 1) {
+            if (isys < 0 || isys >= int(c->zs.size())) throw std::runtime_error("bad mean system index");
+            SysDev &d = *c->zs[isys];
+            const VarMaps m = var_maps(*d.t, P.NZ, ivar);
+            // Chebyshev coefficients of the profile: the z part of the forward transform
+            const double scale = double(P.NXAA) * double(P.NYAA);
+            std::vector<double> cz(P.NZ, 0.0);
+            for (int n = 0; n < P.NZ; ++n)
+                for (int z = 0; z < P.NZAA; ++z) cz[n] += scale * P.Cz[size_t(n) * P.NZAA + z] * in[z];
+            std::vector<double2> f(d.t->total);
+            HIP_OK(hipMemcpy(f.data(), d.field.p, f.size() * sizeof(double2), hipMemcpyDeviceToHost));
+            for (int j = 0; j < m.ncol; ++j) {
+                double g = 0.0;
+                for (int n = 0; n < m.ncol; ++n) g += m.Sinv[size_t(j) * m.ncol + n] * cz[n];
+                f[m.pos[j]] = make_double2(g, 0.0);
+            }
+            HIP_OK(hipMemcpy(d.field.p, f.data(), f.size() * sizeof(double2), hipMemcpyHostToDevice));
+        } else {
+            throw std::runtime_error("kind must be 0 (kxky) or 1 (mean)");
+        }
+        return 0;
+    } catch (const std::exception &e) {
+        return fail(e);
+    }
+}
+
 int coral_amd_bwd_transform(coral_amd_ctx *c, const double *spec, double *phys) {
     try {
         if (!c || !spec || !phys) throw std::runtime_error("null argument");
@@ -790,11 +994,12 @@ int coral_amd_bwd_transform(coral_amd_ctx *c, const double *spec, double *phys) 
         for (int ix = 0; ix < P.nkx; ++ix)
             for (int jy = 0; jy < P.nky; ++jy) {
                 const size_t src = (size_t(ix) * P.NYAA + P.ky_slot[jy]) * P.NZ;
-                const size_t dst = (size_t(ix) * P.nky + jy) * P.NZ;
-                for (int n = 0; n < P.NZ; ++n) h[dst + n] = make_double2(spec[2 * (src + n)], spec[2 * (src + n) + 1]);
+                const size_t sy = size_t(ix) * P.nky + jy;
+                for (int n = 0; n < P.NZ; ++n)
+                    h[size_t(n) * P.nsys + sy] = make_double2(spec[2 * (src + n)], spec[2 * (src + n) + 1]);
             }
         HIP_OK(hipMemcpy(c->spec.p, h.data(), h.size() * sizeof(double2), hipMemcpyHostToDevice));
-        launch_zbwd(c->stream, c->Tz.p, c->spec.p, c->Fz.p, P.NZAA, P.NZ, P.nsys, 1);
+        zbwd(*c, c->spec.p, c->Fz.p, 1);
         launch_ybwd(c->stream, c->py, c->Fz.p, c->G.p, 1, P.NZAA, P.nkx, P.nky, P.NY);
         const long long nlines = (long long)P.NZAA * P.NYAA;
         ensure_phys(*c);
@@ -823,7 +1028,7 @@ int coral_amd_fwd_transform(coral_amd_ctx *c, const double *phys, double *spec) 
         const long long nlines = (long long)P.NZAA * P.NYAA;
         launch_xfwd(c->stream, c->px, c->physd.p, c->H.p, 1, nlines, P.nkx);
         launch_yfwd(c->stream, c->py, c->H.p, c->Hy.p, 1, P.NZAA, P.nkx, P.nky, P.NY);
-        launch_zfwd(c->stream, c->Cz.p, c->Hy.p, c->Qhat.p, P.NZ, P.NZAA, P.nsys, 1);
+        zfwd(*c, c->Hy.p, c->Qhat.p, 1);
         HIP_OK(hipStreamSynchronize(c->stream));
         std::vector<double2> q(size_t(P.nsys) * P.NZ);
         HIP_OK(hipMemcpy(q.data(), c->Qhat.p, q.size() * sizeof(double2), hipMemcpyDeviceToHost));
@@ -831,10 +1036,10 @@ int coral_amd_fwd_transform(coral_amd_ctx *c, const double *phys, double *spec) 
         for (int ix = 0; ix < P.nkx; ++ix)
             for (int jy = 0; jy < P.nky; ++jy) {
                 const size_t dst = (size_t(ix) * P.NYAA + P.ky_slot[jy]) * P.NZ;
-                const size_t src = (size_t(ix) * P.nky + jy) * P.NZ;
+                const size_t sy = size_t(ix) * P.nky + jy;
                 for (int n = 0; n < P.NZ; ++n) {
-                    spec[2 * (dst + n)] = q[src + n].x;
-                    spec[2 * (dst + n) + 1] = q[src + n].y;
+                    spec[2 * (dst + n)] = q[size_t(n) * P.nsys + sy].x;
+                    spec[2 * (dst + n) + 1] = q[size_t(n) * P.nsys + sy].y;
                 }
             }
         return 0;

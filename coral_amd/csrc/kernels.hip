@@ -2,12 +2,12 @@
 //
 // Data layout in HBM (one GPU owns the whole domain):
 //   state   [sys][total]              complex, sys = ix*nky + jy over kept modes
-//   spec    [full][sys][NZ]           Chebyshev coefficients of full variables
+//   spec    [full][NZ][sys]           Chebyshev coefficients of full variables
 //   Fz      [full][NZAA][sys]         after the z transform (Gauss points)
 //   G       [full][NZAA][NYAA][nkx]   after the y transform
 //   H       [quad][NZAA][NYAA][nkx]   products after the x transform (kept kx)
 //   Hy      [quad][NZAA][sys]         after the y transform (kept ky)
-//   Qhat    [quad][sys][NZ]           Chebyshev coefficients of the products
+//   Qhat    [quad][NZ][sys]           Chebyshev coefficients of the products
 #include "kernels.hpp"
 
 #include <cstdio>
@@ -28,54 +28,68 @@ static inline unsigned blocks_for(long long n, int threads) { return unsigned((n
 // extraction (padStencilExtractShuffle . aux, PL_IMEX_timestepping.f90:1199-1209,
 // PL_IMEX_timestepping_marching.f90:430-470)
 // ---------------------------------------------------------------------------
-__global__ void k_extract(const double2 *__restrict__ state, int total, int nsys, int NZ,
-                          const int *__restrict__ rowptr, const int *__restrict__ col,
-                          const double *__restrict__ val, int kx, int ky, KeptModes km,
-                          double2 *__restrict__ spec, int accumulate) {
-    long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= (long long)nsys * NZ) return;
-    int s = int(idx / NZ), n = int(idx % NZ);
-    const double2 *st = state + (size_t)s * total;
-    double2 acc = make_double2(0.0, 0.0);
-    for (int k = rowptr[n]; k < rowptr[n + 1]; ++k) {
-        double v = val[k];
-        double2 a = st[col[k]];
-        acc.x += v * a.x;
-        acc.y += v * a.y;
+// spec[n][s] (+)= mono(s) * sum_k val[k] * state[s][col[k]].  A workgroup owns
+// xt systems: it reads each system's state along n (coalesced within the
+// system) and writes rows of xt consecutive systems through LDS.
+__global__ __launch_bounds__(256) void k_extract(const double2 *__restrict__ state, int total, int nsys, int NZ,
+                                                 const int *__restrict__ rowptr, const int *__restrict__ col,
+                                                 const double *__restrict__ val, int kx, int ky, KeptModes km,
+                                                 double2 *__restrict__ spec, int accumulate, int xt) {
+    extern __shared__ double2 tile[];  // [NZ][xt]
+    const int s0 = blockIdx.x * xt;
+    const int ns = min(xt, nsys - s0);
+    for (int e = threadIdx.x; e < ns * NZ; e += blockDim.x) {
+        const int sl = e / NZ, n = e - sl * NZ, s = s0 + sl;
+        const double2 *st = state + (size_t)s * total;
+        double2 acc = make_double2(0.0, 0.0);
+        for (int k = rowptr[n]; k < rowptr[n + 1]; ++k) {
+            const double v = val[k];
+            const double2 a = st[col[k]];
+            acc.x += v * a.x;
+            acc.y += v * a.y;
+        }
+        if (kx || ky) acc = cmul(monomial(kx, ky, km.kx(s), km.ky(s)), acc);
+        tile[n * xt + sl] = acc;
     }
-    if (kx || ky) acc = cmul(monomial(kx, ky, km.kx(s), km.ky(s)), acc);
-    if (accumulate) acc = cadd(spec[idx], acc);
-    spec[idx] = acc;
+    __syncthreads();
+    for (int e = threadIdx.x; e < ns * NZ; e += blockDim.x) {
+        const int n = e / ns, sl = e - n * ns;
+        double2 *o = spec + (size_t)n * nsys + s0 + sl;
+        const double2 v = tile[n * xt + sl];
+        *o = accumulate ? cadd(*o, v) : v;
+    }
 }
 
 void launch_extract(hipStream_t st, const double2 *state, int total, int nsys, int NZ, const int *rowptr,
                     const int *col, const double *val, int kx, int ky, KeptModes km, double2 *spec, int accumulate) {
-    hipLaunchKernelGGL(k_extract, dim3(blocks_for((long long)nsys * NZ, 256)), dim3(256), 0, st, state, total, nsys,
-                       NZ, rowptr, col, val, kx, ky, km, spec, accumulate);
+    int xt = 16;
+    while (xt > 1 && size_t(NZ) * xt * sizeof(double2) > 49152) xt /= 2;
+    const size_t shm = size_t(NZ) * xt * sizeof(double2);
+    hipLaunchKernelGGL(k_extract, dim3(blocks_for(nsys, xt)), dim3(256), shm, st, state, total, nsys, NZ, rowptr, col,
+                       val, kx, ky, km, spec, accumulate, xt);
     CHECK_LAUNCH();
 }
 
-__global__ void k_extract_mean(const double2 *__restrict__ zstate, int NZ, const int *__restrict__ rowptr,
-                               const int *__restrict__ col, const double *__restrict__ val, double2 *spec0) {
+// mean contributions land in the (0,0) column: spec[n][0]
+__global__ void k_extract_mean(const double2 *__restrict__ zstate, int NZ, int nsys, const int *__restrict__ rowptr,
+                               const int *__restrict__ col, const double *__restrict__ val, double2 *spec) {
     for (int n = threadIdx.x; n < NZ; n += blockDim.x) {
         double acc = 0.0;
         for (int k = rowptr[n]; k < rowptr[n + 1]; ++k) acc += val[k] * zstate[col[k]].x;
-        spec0[n].x += acc;
+        spec[(size_t)n * nsys].x += acc;
     }
 }
 
-void launch_extract_mean(hipStream_t st, const double2 *zstate, int NZ, const int *rowptr, const int *col,
-                         const double *val, double2 *spec0) {
-    hipLaunchKernelGGL(k_extract_mean, dim3(1), dim3(256), 0, st, zstate, NZ, rowptr, col, val, spec0);
+void launch_extract_mean(hipStream_t st, const double2 *zstate, int NZ, int nsys, const int *rowptr, const int *col,
+                         const double *val, double2 *spec) {
+    hipLaunchKernelGGL(k_extract_mean, dim3(1), dim3(256), 0, st, zstate, NZ, nsys, rowptr, col, val, spec);
     CHECK_LAUNCH();
 }
 
 // ---------------------------------------------------------------------------
-// z transforms as dense FP64 products (A real, right operand complex)
-// MODE 0: B(k, c) = Bp[((c>>1)*K + k)*2 + (c&1)],  C(m, c) = Cp[m*Ncol + c]
-// MODE 1: B(k, c) = Bp[k*Ncol + c],                C(m, c) = Cp[((c>>1)*M + m)*2 + (c&1)]
+// z transforms, plain FP64 GEMM (used when no MFMA variant covers the shape):
+// C[b](m, c) = sum_k A(m, k) B[b](k, c), B and C row-major with Ncol columns
 // ---------------------------------------------------------------------------
-template <int MODE>
 __global__ __launch_bounds__(256) void k_gemm(const double *__restrict__ A, int lda, const double *__restrict__ Bp,
                                               double *__restrict__ Cp, int M, int Ncol, int K, size_t strideB,
                                               size_t strideC) {
@@ -97,18 +111,10 @@ __global__ __launch_bounds__(256) void k_gemm(const double *__restrict__ A, int 
             int gm = m0 + mm, gk = k0 + kk;
             As[kk][mm] = (gm < M && gk < K) ? A[(size_t)gm * lda + gk] : 0.0;
         }
-        if (MODE == 0) {
-            for (int e = tid; e < BN * BK; e += 256) {
-                int lo = e & 1, kk = (e >> 1) % BK, ch = (e >> 1) / BK;
-                int c = c0 + 2 * ch + lo, gk = k0 + kk;
-                Bs[kk][2 * ch + lo] = (c < Ncol && gk < K) ? Bp[((size_t)(c >> 1) * K + gk) * 2 + lo] : 0.0;
-            }
-        } else {
-            for (int e = tid; e < BN * BK; e += 256) {
-                int cc = e % BN, kk = e / BN;
-                int c = c0 + cc, gk = k0 + kk;
-                Bs[kk][cc] = (c < Ncol && gk < K) ? Bp[(size_t)gk * Ncol + c] : 0.0;
-            }
+        for (int e = tid; e < BN * BK; e += 256) {
+            int cc = e % BN, kk = e / BN;
+            int c = c0 + cc, gk = k0 + kk;
+            Bs[kk][cc] = (c < Ncol && gk < K) ? Bp[(size_t)gk * Ncol + c] : 0.0;
         }
         __syncthreads();
 #pragma unroll
@@ -132,11 +138,7 @@ __global__ __launch_bounds__(256) void k_gemm(const double *__restrict__ A, int 
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
             int c = c0 + tx + 16 * j;
-            if (c >= Ncol) continue;
-            if (MODE == 0)
-                Cp[(size_t)m * Ncol + c] = acc[i][j];
-            else
-                Cp[((size_t)(c >> 1) * M + m) * 2 + (c & 1)] = acc[i][j];
+            if (c < Ncol) Cp[(size_t)m * Ncol + c] = acc[i][j];
         }
     }
 }
@@ -145,8 +147,8 @@ void launch_zbwd(hipStream_t st, const double *Tz, const double2 *spec, double2 
                  int nbatch) {
     int Ncol = 2 * nsys;
     dim3 grid(blocks_for(Ncol, 64), blocks_for(NZAA, 64), nbatch);
-    hipLaunchKernelGGL(k_gemm<0>, grid, dim3(256), 0, st, Tz, NZ, (const double *)spec, (double *)Fz, NZAA, Ncol,
-                       NZ, (size_t)nsys * NZ * 2, (size_t)NZAA * nsys * 2);
+    hipLaunchKernelGGL(k_gemm, grid, dim3(256), 0, st, Tz, NZ, (const double *)spec, (double *)Fz, NZAA, Ncol, NZ,
+                       (size_t)NZ * Ncol, (size_t)NZAA * Ncol);
     CHECK_LAUNCH();
 }
 
@@ -154,8 +156,8 @@ void launch_zfwd(hipStream_t st, const double *Cz, const double2 *Hy, double2 *Q
                  int nbatch) {
     int Ncol = 2 * nsys;
     dim3 grid(blocks_for(Ncol, 64), blocks_for(NZ, 64), nbatch);
-    hipLaunchKernelGGL(k_gemm<1>, grid, dim3(256), 0, st, Cz, NZAA, (const double *)Hy, (double *)Qhat, NZ, Ncol,
-                       NZAA, (size_t)NZAA * nsys * 2, (size_t)nsys * NZ * 2);
+    hipLaunchKernelGGL(k_gemm, grid, dim3(256), 0, st, Cz, NZAA, (const double *)Hy, (double *)Qhat, NZ, Ncol, NZAA,
+                       (size_t)NZAA * Ncol, (size_t)NZ * Ncol);
     CHECK_LAUNCH();
 }
 
@@ -309,6 +311,7 @@ static bool fft_fits(int lines, int N) {
 
 void launch_ybwd(hipStream_t st, const FFTPlan &py, const double2 *Fz, double2 *G, int nbatch, int NZAA, int nkx,
                  int nky, int NY) {
+    if (launch_ypass_wave(st, false, py, Fz, G, nbatch, NZAA, nkx, nky, NY)) return;
     int lines = lines_for(py.N);
     if (!fft_fits(lines, py.N)) return;
     int ntile = (nkx + lines - 1) / lines;
@@ -344,6 +347,7 @@ __global__ __launch_bounds__(256) void k_yfwd(FFTPlanDev p, const double2 *__res
 
 void launch_yfwd(hipStream_t st, const FFTPlan &py, const double2 *H, double2 *Hy, int nbatch, int NZAA, int nkx,
                  int nky, int NY) {
+    if (launch_ypass_wave(st, true, py, H, Hy, nbatch, NZAA, nkx, nky, NY)) return;
     int lines = lines_for(py.N);
     if (!fft_fits(lines, py.N)) return;
     int ntile = (nkx + lines - 1) / lines;
@@ -531,14 +535,14 @@ void launch_xfwd(hipStream_t st, const FFTPlan &px, const double *phys, double2 
 __global__ void k_remove_zmean(double2 *Qhat, int NZ, int nsys) {
     int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= nsys) return;
-    double2 *q = Qhat + (size_t)s * NZ;
     double2 acc = make_double2(0.0, 0.0);
     for (int k = 1; k <= NZ / 2 - 1; ++k) {
         double w = 0.5 / (2.0 * k + 1.0) - 0.5 / (2.0 * k - 1.0);
-        acc.x -= q[2 * k].x * w;
-        acc.y -= q[2 * k].y * w;
+        const double2 q = Qhat[(size_t)(2 * k) * nsys + s];
+        acc.x -= q.x * w;
+        acc.y -= q.y * w;
     }
-    q[0] = acc;
+    Qhat[s] = acc;
 }
 
 void launch_remove_zmean(hipStream_t st, double2 *Qhat, int NZ, int nsys) {
@@ -546,43 +550,109 @@ void launch_remove_zmean(hipStream_t st, double2 *Qhat, int NZ, int nsys) {
     CHECK_LAUNCH();
 }
 
+// Galerkin coefficients of one coupled variable from the Chebyshev
+// coefficients of its physical values (import_quicksave,
+// PL_IMEX_timestepping_output.f90:340-356): g[j] = sum_{n < ncol} Sinv[j][n] Q[n][s],
+// stored at position pos[j] of every kxky system; the (0,0) column stays zero.
+__global__ void k_insert_var(const double2 *__restrict__ Q, int nsys, int ncol, const double *__restrict__ Sinv,
+                             const int *__restrict__ pos, double2 *__restrict__ field, int total) {
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long long)ncol * nsys) return;
+    const int j = int(idx / nsys), s = int(idx % nsys);
+    double2 g = make_double2(0.0, 0.0);
+    for (int n = 0; n < ncol; ++n) {
+        const double w = Sinv[(size_t)j * ncol + n];
+        const double2 q = Q[(size_t)n * nsys + s];
+        g.x += w * q.x;
+        g.y += w * q.y;
+    }
+    field[(size_t)s * total + pos[j]] = s == 0 ? make_double2(0.0, 0.0) : g;
+}
+
+void launch_insert_var(hipStream_t st, const double2 *Q, int nsys, int ncol, const double *Sinv, const int *pos,
+                       double2 *field, int total) {
+    hipLaunchKernelGGL(k_insert_var, dim3(blocks_for((long long)ncol * nsys, 256)), dim3(256), 0, st, Q, nsys, ncol,
+                       Sinv, pos, field, total);
+    CHECK_LAUNCH();
+}
+
 // ---------------------------------------------------------------------------
 // NL assembly (PL_IMEX_timestepping_marching.f90:306-367 kxky, :372-397 mean)
 // ---------------------------------------------------------------------------
-__global__ void k_nl(NLArgs a, int nsys_out, int nsysQ) {
-    long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= (long long)nsys_out * a.total) return;
-    const int s = int(idx / a.total), pos = int(idx % a.total);
-    const int e = a.pos_eqn[pos], r = a.pos_row[pos];
-    const int ssrc = a.mean ? 0 : s;
-    double2 acc = make_double2(0.0, 0.0);
-    for (int t = a.eqn_begin[e]; t < a.eqn_begin[e + 1]; ++t) {
-        const int q = a.t_qvar[t], w = a.t_w[t], m0 = r + a.t_order[t];
-        const double2 *Q = a.Qhat + ((size_t)q * nsysQ + ssrc) * a.NZ;
-        const double *W = a.wv + a.t_woff[t] + r * (2 * w + 1);
-        double2 sum = make_double2(0.0, 0.0);
-        for (int d = -w; d <= w; ++d) {
-            int m = m0 + d;
-            if (m < 0 || m >= a.NZ) continue;
-            double wv = W[d + w];
-            double2 qv = Q[m];
-            sum.x += wv * qv.x;
-            sum.y += wv * qv.y;
+// kxky systems: a workgroup owns 16 systems x 64 positions.  Lanes run over
+// systems first, so every read of Qhat[q][m][s] covers 16 consecutive
+// systems; the result is transposed through LDS and written as 64-position
+// rows of each system.
+__global__ __launch_bounds__(256) void k_nl(NLArgs a) {
+    __shared__ double2 tile[64][17];
+    const int nsys = a.nsys, total = a.total;
+    const int ntp = (total + 63) / 64;
+    const int s0 = (blockIdx.x / ntp) * 16, p0 = (blockIdx.x % ntp) * 64;
+    const int sl = threadIdx.x & 15, pr = threadIdx.x >> 4;
+    const int s = s0 + sl;
+    const bool sv = s < nsys;
+    const double kx = a.km.kx(sv ? s : 0), ky = a.km.ky(sv ? s : 0);
+    const size_t qstride = (size_t)a.NZ * nsys;
+    for (int i = 0; i < 4; ++i) {
+        const int pl = pr + 16 * i, pos = p0 + pl;
+        double2 acc = make_double2(0.0, 0.0);
+        if (pos < total && sv && s != 0) {
+            const int e = a.pos_eqn[pos], r = a.pos_row[pos];
+            for (int t = a.eqn_begin[e]; t < a.eqn_begin[e + 1]; ++t) {
+                const int w = a.t_w[t], m0 = r + a.t_order[t];
+                const double2 *Q = a.Qhat + a.t_qvar[t] * qstride + s;
+                const double *W = a.wv + a.t_woff[t] + r * (2 * w + 1);
+                double2 sum = make_double2(0.0, 0.0);
+                for (int d = -w; d <= w; ++d) {
+                    const int m = m0 + d;
+                    if (m < 0 || m >= a.NZ) continue;
+                    const double wv = W[d + w];
+                    const double2 qv = Q[(size_t)m * nsys];
+                    sum.x += wv * qv.x;
+                    sum.y += wv * qv.y;
+                }
+                const double2 c = monomial(a.t_kx[t], a.t_ky[t], kx, ky);
+                acc = cadd(acc, cscale(cmul(c, sum), a.t_dsca[t]));
+            }
         }
-        if (a.mean) {
-            acc.x += a.t_dsca[t] * sum.x;
-        } else {
-            double2 c = monomial(a.t_kx[t], a.t_ky[t], a.km.kx(s), a.km.ky(s));
-            acc = cadd(acc, cscale(cmul(c, sum), a.t_dsca[t]));
-        }
+        tile[pl][sl] = acc;
     }
-    if (!a.mean && s == 0) acc = make_double2(0.0, 0.0);
-    a.Khat[idx] = acc;
+    __syncthreads();
+    for (int i = 0; i < 4; ++i) {
+        const int e = threadIdx.x + 256 * i, sl2 = e >> 6, pl = e & 63;
+        const int s2 = s0 + sl2, pos = p0 + pl;
+        if (s2 < nsys && pos < total) a.Khat[(size_t)s2 * total + pos] = tile[pl][sl2];
+    }
+}
+
+// mean systems: real part of the (0,0) column
+__global__ void k_nl_mean(NLArgs a) {
+    const int pos = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pos >= a.total) return;
+    const int e = a.pos_eqn[pos], r = a.pos_row[pos];
+    double acc = 0.0;
+    for (int t = a.eqn_begin[e]; t < a.eqn_begin[e + 1]; ++t) {
+        const int w = a.t_w[t], m0 = r + a.t_order[t];
+        const double2 *Q = a.Qhat + a.t_qvar[t] * (size_t)a.NZ * a.nsys;
+        const double *W = a.wv + a.t_woff[t] + r * (2 * w + 1);
+        double sum = 0.0;
+        for (int d = -w; d <= w; ++d) {
+            const int m = m0 + d;
+            if (m < 0 || m >= a.NZ) continue;
+            sum += W[d + w] * Q[(size_t)m * a.nsys].x;
+        }
+        acc += a.t_dsca[t] * sum;
+    }
+    a.Khat[pos] = make_double2(acc, 0.0);
 }
 
 void launch_nl(hipStream_t st, const NLArgs &a) {
-    int nout = a.mean ? 1 : a.nsys;
-    hipLaunchKernelGGL(k_nl, dim3(blocks_for((long long)nout * a.total, 256)), dim3(256), 0, st, a, nout, a.nsys);
+    if (a.mean) {
+        hipLaunchKernelGGL(k_nl_mean, dim3(blocks_for(a.total, 256)), dim3(256), 0, st, a);
+    } else {
+        const int ntp = (a.total + 63) / 64;
+        hipLaunchKernelGGL(k_nl, dim3(blocks_for(a.nsys, 16) * ntp), dim3(256), 0, st, a);
+    }
     CHECK_LAUNCH();
 }
 
@@ -730,6 +800,31 @@ __device__ void split_factors(const T *AB, int n, int kl, int ku, int ldab, T *L
     }
 }
 
+// Factor records for the 16-lane solve: the multipliers of column j are
+// stored rotated so that lane l (position 16b + l of the block it updates)
+// reads entry l: L16[j][l] = L(j + 1 + k, j), k = (l - j%16 - 1) mod 16, zero
+// for k >= min(kl, n-1-j) and for the padding columns j >= n.  U as in
+// split_factors.
+template <typename T>
+__device__ void split_factors16(const T *AB, int n, int kl, int ku, int ldab, int ncol16, T *L16, T *Uf) {
+    using F = Fx<T>;
+    const int kv = kl + ku;
+    for (int e = threadIdx.x; e < ncol16 * 16; e += blockDim.x) {
+        const int j = e >> 4, l = e & 15;
+        const int k = (l - (j & 15) - 1) & 15;
+        L16[e] = (j < n && k < min(kl, n - 1 - j)) ? AB[(size_t)j * ldab + kv + 1 + k] : F::zero();
+    }
+    for (int e = threadIdx.x; e < n * (kv + 1); e += blockDim.x) {
+        int j = e / (kv + 1), k = e % (kv + 1);
+        T v;
+        if (k == 0)
+            v = F::recip(AB[(size_t)j * ldab + kv]);
+        else
+            v = (j - k >= 0) ? AB[(size_t)j * ldab + kv - k] : F::zero();
+        Uf[e] = v;
+    }
+}
+
 // Re(z * i^e) for e in {0, 1, 3}
 __device__ inline double real_rot(double2 z, int e) { return e == 0 ? z.x : (e == 1 ? -z.y : z.y); }
 
@@ -774,10 +869,13 @@ __global__ __launch_bounds__(64) void k_factor(FactorArgs a) {
     __syncthreads();
     gbtf2_wave<T>(AB, n, kl, ku, ldab, a.piv + (size_t)s * n, info);
     __syncthreads();
-    if constexpr (REAL)
-        split_factors<double>(AB, n, kl, ku, ldab, a.Lr + (size_t)s * n * kl, a.Ur + (size_t)s * n * (kv + 1));
+    T *Lout = REAL ? reinterpret_cast<T *>(a.Lr) : reinterpret_cast<T *>(a.Lf);
+    T *Uout = REAL ? reinterpret_cast<T *>(a.Ur) : reinterpret_cast<T *>(a.Uf);
+    if (a.ncol16 > 0)
+        split_factors16<T>(AB, n, kl, ku, ldab, a.ncol16, Lout + (size_t)s * a.ncol16 * 16,
+                           Uout + (size_t)s * n * (kv + 1));
     else
-        split_factors<double2>(AB, n, kl, ku, ldab, a.Lf + (size_t)s * n * kl, a.Uf + (size_t)s * n * (kv + 1));
+        split_factors<T>(AB, n, kl, ku, ldab, Lout + (size_t)s * n * kl, Uout + (size_t)s * n * (kv + 1));
 }
 
 void launch_factor(hipStream_t st, const FactorArgs &a) {
@@ -1036,6 +1134,203 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 4))) void
             prv = ld_masked(xo, ((r - 2) << 6) + lane, r >= 2);
         }
     }
+}
+
+// ---------------------------------------------------------------------------
+// Banded solve with four systems per wavefront, sixteen lanes each (kl <= 16).
+// Position 16b + l of a system lives in lane l of its 16-lane row, block b in
+// one register.  Forward (xGBTRS first loop): at step j = 16b + t the row
+// interchange j <-> j + p gathers position j + p with ds_bpermute, the value
+// of position j is broadcast with DPP row_newbcast, and rows j+1 .. j+16 are
+// exactly the 16 lanes (lanes > t in block b, lanes <= t in block b+1), so
+// every lane does useful work.  Backward (xTBSV, column oriented): x_j is
+// broadcast and NQ registers (blocks b .. b-NQ+1) take the column update.
+// Factor records: L16 (see split_factors16), U as in split_factors with
+// padding around the array so that out-of-record reads stay in bounds (their
+// values are masked).
+// ---------------------------------------------------------------------------
+template <int T_>
+__device__ inline int bcast16_i(int v) {
+    return __builtin_amdgcn_update_dpp(0, v, 0x150 + T_, 0xF, 0xF, false);
+}
+template <int T_>
+__device__ inline double bcast16_d(double v) {
+    const long long i = __double_as_longlong(v);
+    const int lo = bcast16_i<T_>((int)(i & 0xffffffffLL)), hi = bcast16_i<T_>((int)(i >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+template <int T_>
+__device__ inline double2 bcast16_c(double2 v) { return make_double2(bcast16_d<T_>(v.x), bcast16_d<T_>(v.y)); }
+template <int T_>
+__device__ inline double bcast16(double v) { return bcast16_d<T_>(v); }
+template <int T_>
+__device__ inline double2 bcast16(double2 v) { return bcast16_c<T_>(v); }
+
+__device__ inline double bperm_d(int addr, double v) {
+    const long long i = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_ds_bpermute(addr, (int)(i & 0xffffffffLL));
+    const int hi = __builtin_amdgcn_ds_bpermute(addr, (int)(i >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+__device__ inline double2 bperm_c(int addr, double2 v) { return make_double2(bperm_d(addr, v.x), bperm_d(addr, v.y)); }
+
+template <typename T, int NQ, int PFB>
+struct Solve16 {
+    // forward step t of a block: swap, eliminate, prefetch the next block's coefficient
+    template <int t>
+    __device__ static inline void fwd_step(double2 &cur, double2 &nxt, int pv, T (&ring)[16], const T *Lcol_next,
+                                           int l, int rowbase) {
+        const T zero = Fx<T>::zero();
+        const int p = bcast16_i<t>(pv);
+        const int tp = t + p;
+        const double2 w = csel(tp < 16, cur, nxt);
+        const double2 bj = bperm_c((rowbase + (tp & 15)) << 2, w);
+        const double2 a = bcast16_c<t>(cur);
+        cur = csel(l == tp, a, cur);
+        nxt = csel(l + 16 == tp, a, nxt);
+        cur = csel(l == t, bj, cur);
+        const T c = ring[t];
+        cur = submul(cur, tsel(l > t, c, zero), bj);
+        nxt = submul(nxt, tsel(l > t, zero, c), bj);
+        ring[t] = ld_nt(Lcol_next + t * 16);
+    }
+
+    template <int t>
+    __device__ static inline void fwd_block(double2 &cur, double2 &nxt, int pv, T (&ring)[16], const T *Lcol_next,
+                                            int l, int rowbase) {
+        fwd_step<t>(cur, nxt, pv, ring, Lcol_next, l, rowbase);
+        if constexpr (t < 15) fwd_block<t + 1>(cur, nxt, pv, ring, Lcol_next, l, rowbase);
+    }
+
+    // backward step t (descending) of block blk
+    template <int t>
+    __device__ static inline void bwd_step(double2 (&r)[NQ], T (&dring)[PFB], T (&cring)[PFB][NQ], const T *Ul,
+                                           const T *Ud, int l, int kv, int jnext) {
+        const T zero = Fx<T>::zero();
+        constexpr int slot = t % PFB;
+        const double2 xj = scale(bcast16_c<t>(r[0]), dring[slot]);
+        r[0] = csel(l == t, xj, r[0]);
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+            const int k = t - 1 - l + 16 * q;
+            const bool valid = (k >= 0) && (k < kv);
+            r[q] = submul(r[q], tsel(valid, cring[slot][q], zero), xj);
+        }
+        // prefetch step jnext = j - PFB (its position in the block: (t - PFB) mod 16)
+        constexpr int tn = (t - PFB) & 15;
+        const size_t off = (size_t)((long long)jnext * (kv + 1));
+        dring[slot] = ld_nt(Ud + off);
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) cring[slot][q] = ld_nt(Ul + off + tn + 16 * q);
+    }
+
+    template <int t>
+    __device__ static inline void bwd_block(double2 (&r)[NQ], T (&dring)[PFB], T (&cring)[PFB][NQ], const T *Ul,
+                                            const T *Ud, int l, int kv, int j0) {
+        bwd_step<t>(r, dring, cring, Ul, Ud, l, kv, j0 + t - PFB);
+        if constexpr (t > 0) bwd_block<t - 1>(r, dring, cring, Ul, Ud, l, kv, j0);
+    }
+};
+
+template <typename T, int NQ, int PFB>
+__global__ __launch_bounds__(64) void k_solve16(const T *__restrict__ L16, int ncol16, const T *__restrict__ Ur,
+                                                const unsigned char *__restrict__ piv,
+                                                const unsigned char *__restrict__ rot, int ident0, int n, int kv,
+                                                int nsys, const double2 *__restrict__ rhs, double2 *__restrict__ x) {
+    using S = Solve16<T, NQ, PFB>;
+    const int lane = threadIdx.x, l = lane & 15, rowbase = lane & 48;
+    const int sg = blockIdx.x * 4 + (lane >> 4);
+    const bool live = sg < nsys;
+    const int s = live ? sg : nsys - 1;
+    const bool dorot = rot && !(ident0 && s == 0);
+    const T *Lp = L16 + (size_t)s * ncol16 * 16 + l;
+    const unsigned char *P = piv + (size_t)s * n;
+    const double2 *b = rhs + (size_t)s * n;
+    double2 *y = x + (size_t)s * n;
+    const int B = (n + 15) >> 4;
+    auto code = [&](int i) -> int { return dorot ? (int)rot[min(i, n - 1)] : 0; };
+    auto load_rhs = [&](int i) -> double2 { return rot_in(ld_masked(b, i, i < n), code(i)); };
+    auto load_piv = [&](int i) -> int { const int v = P[min(i, n - 1)]; return i < n ? v : 0; };
+
+    // ---- forward ----
+    double2 cur = load_rhs(l), nxt = load_rhs(16 + l);
+    int pv = load_piv(l);
+    T ring[16];
+#pragma unroll
+    for (int t = 0; t < 16; ++t) ring[t] = ld_nt(Lp + t * 16);
+    for (int blk = 0; blk < B; ++blk) {
+        S::template fwd_block<0>(cur, nxt, pv, ring, Lp + (size_t)(blk * 16 + 16) * 16, l, rowbase);
+        const int i0 = blk * 16 + l;
+        if (live && i0 < n) y[i0] = cur;
+        cur = nxt;
+        nxt = load_rhs(blk * 16 + 32 + l);
+        pv = load_piv(blk * 16 + 16 + l);
+    }
+    __threadfence_block();
+
+    // ---- backward ----
+    const T *Ud = Ur + (size_t)s * n * (kv + 1);   // + j*(kv+1): 1/U(j,j)
+    const T *Ul = Ud - l;                          // + j*(kv+1) + t + 16q: U(j-1-k, j), k = t-1-l+16q
+    double2 r[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+        const int bq = B - 1 - q;
+        r[q] = ld_masked(y, bq * 16 + l, bq >= 0 && bq * 16 + l < n);
+    }
+    T dring[PFB], cring[PFB][NQ];
+#pragma unroll
+    for (int u = 0; u < PFB; ++u) {
+        // step j = 16B - 1 - u lives in slot (15 - u) % PFB
+        const int j = B * 16 - 1 - u;
+        const int t = j & 15;
+        const size_t off = (size_t)((long long)j * (kv + 1));
+        dring[(15 - u) % PFB] = ld_nt(Ud + off);
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) cring[(15 - u) % PFB][q] = ld_nt(Ul + off + t + 16 * q);
+    }
+    for (int blk = B - 1; blk >= 0; --blk) {
+        S::template bwd_block<15>(r, dring, cring, Ul, Ud, l, kv, blk * 16);
+        const int i0 = blk * 16 + l;
+        if (live && i0 < n) y[i0] = rot_out(r[0], code(i0));
+#pragma unroll
+        for (int q = 0; q < NQ - 1; ++q) r[q] = r[q + 1];
+        const int bq = blk - NQ;
+        r[NQ - 1] = ld_masked(y, bq * 16 + l, bq >= 0);
+    }
+}
+
+template <typename T>
+static bool launch_solve16(hipStream_t st, const T *L16, int ncol16, const T *Ur, const unsigned char *piv,
+                           const unsigned char *rot, int ident0, int n, int kl, int kv, int nsys, const double2 *rhs,
+                           double2 *x) {
+    if (kl > 16 || n < 32 || ncol16 < 16 * ((n + 15) / 16 + 1)) return false;
+    const int nq = 1 + (kv + 15) / 16;
+    const dim3 grid((nsys + 3) / 4);
+    switch (nq) {
+    case 2: hipLaunchKernelGGL((k_solve16<T, 2, 4>), grid, dim3(64), 0, st, L16, ncol16, Ur, piv, rot, ident0, n, kv, nsys, rhs, x); break;
+    case 3: hipLaunchKernelGGL((k_solve16<T, 3, 4>), grid, dim3(64), 0, st, L16, ncol16, Ur, piv, rot, ident0, n, kv, nsys, rhs, x); break;
+    case 4: hipLaunchKernelGGL((k_solve16<T, 4, 4>), grid, dim3(64), 0, st, L16, ncol16, Ur, piv, rot, ident0, n, kv, nsys, rhs, x); break;
+    default: return false;
+    }
+    CHECK_LAUNCH();
+    return true;
+}
+
+int solve16_ncol(int n, int kl, int kv) {
+    if (kl > 16 || n < 32 || 1 + (kv + 15) / 16 > 4) return 0;
+    return 16 * ((n + 15) / 16 + 1);
+}
+
+bool launch_solve16_real(hipStream_t st, const double *L16, int ncol16, const double *Ur, const unsigned char *piv,
+                         const unsigned char *rot, int ident0, int n, int kl, int kv, int nsys, const double2 *rhs,
+                         double2 *x) {
+    return launch_solve16<double>(st, L16, ncol16, Ur, piv, rot, ident0, n, kl, kv, nsys, rhs, x);
+}
+
+bool launch_solve16_complex(hipStream_t st, const double2 *L16, int ncol16, const double2 *Ur,
+                            const unsigned char *piv, int ident0, int n, int kl, int kv, int nsys, const double2 *rhs,
+                            double2 *x) {
+    return launch_solve16<double2>(st, L16, ncol16, Ur, piv, nullptr, ident0, n, kl, kv, nsys, rhs, x);
 }
 
 template <typename T>

@@ -5,22 +5,46 @@
 
 #include "common.hpp"
 
+#include <functional>
+#include <vector>
+
 namespace coral {
 
-// ---- extraction: spec[s][n] (+)= mono(s) * sum val[k] * state[s][col[k]] ----
+// ---- extraction: spec[n][s] (+)= mono(s) * sum val[k] * state[s][col[k]] ----
 void launch_extract(hipStream_t st, const double2 *state, int total, int nsys, int NZ, const int *rowptr,
                     const int *col, const double *val, int kx, int ky, KeptModes km, double2 *spec, int accumulate);
-// mean contributions into the (0,0) slot: spec0[n] += sum val[k] * Re(zstate[col[k]])
-void launch_extract_mean(hipStream_t st, const double2 *zstate, int NZ, const int *rowptr, const int *col,
-                         const double *val, double2 *spec0);
+// mean contributions into the (0,0) column: spec[n][0] += sum val[k] * Re(zstate[col[k]])
+void launch_extract_mean(hipStream_t st, const double2 *zstate, int NZ, int nsys, const int *rowptr, const int *col,
+                         const double *val, double2 *spec);
 
-// ---- dense real GEMMs with complex right operands (z transforms) ----
-// Fz[b][z][s] = sum_n Tz[z][n] spec[b][s][n]            (batch b over full variables)
+// ---- z transforms ----
+// Layouts (z-major): spec[b][n][s], Fz[b][z][s], Hy[b][z][s], Qhat[b][n][s] (complex s)
+// Fz[b][z][s] = sum_n Tz[z][n] spec[b][n][s]          (plain FP64 GEMM fallback)
 void launch_zbwd(hipStream_t st, const double *Tz, const double2 *spec, double2 *Fz, int NZAA, int NZ, int nsys,
                  int nbatch);
-// Qhat[b][s][n] = sum_z Cz[n][z] Hy[b][z][s]             (batch b over quadratic variables)
+// Qhat[b][n][s] = sum_z Cz[n][z] Hy[b][z][s]
 void launch_zfwd(hipStream_t st, const double *Cz, const double2 *Hy, double2 *Qhat, int NZ, int NZAA, int nsys,
                  int nbatch);
+// MFMA fragments of the folded (even/odd) transform matrices
+struct ZFrags {
+    int TE = 0, TO = 0, NEK = 0, NOK = 0;  // synthesis: row tiles (even, odd), k steps (even, odd)
+    int RE = 0, RO = 0, KEK = 0, KOK = 0;  // analysis
+    bool bwd_ok = false, fwd_ok = false;    // fragments fit the LDS of one workgroup
+    std::vector<double> bwd, fwd;
+};
+struct ZFragsDev {
+    int TE, TO, NEK, NOK, RE, RO, KEK, KOK;
+    bool bwd_ok, fwd_ok;
+    int ncu;
+    const double *bwd, *fwd;
+};
+bool make_zfrags(int NZ, int NZAA, const std::vector<double> &Tz, const std::vector<double> &Cz, int device,
+                 ZFrags &zf);
+// return false (nothing launched) when no MFMA variant covers the shape
+bool launch_zbwd_mfma(hipStream_t st, const ZFragsDev &z, const double2 *spec, double2 *Fz, int NZ, int NZAA, int nsys,
+                      int nb);
+bool launch_zfwd_mfma(hipStream_t st, const ZFragsDev &z, const double2 *Hy, double2 *Qhat, int NZ, int NZAA,
+                      int nsys, int nb);
 
 // ---- FFT passes ----
 struct FFTPlan {
@@ -44,6 +68,9 @@ void launch_xpass(hipStream_t st, const FFTPlan &px, const XPassArgs &a);
 // wavefront-per-line x pass for N in {96, 192, 384, 768} and up to 6 full
 // variables; returns false (nothing launched) for other shapes
 bool launch_xpass_wave(hipStream_t st, const FFTPlan &px, const XPassArgs &a);
+// wavefront-FFT y passes for NYAA in {96, 192, 384}; false (nothing launched) otherwise
+bool launch_ypass_wave(hipStream_t st, bool fwd, const FFTPlan &py, const double2 *in, double2 *out, int nbatch,
+                       int NZAA, int nkx, int nky, int NY);
 // y forward: Hy[b][z][ix*nky+jy] = sum_y H[b][z][y][ix] e^{-i ky y}
 void launch_yfwd(hipStream_t st, const FFTPlan &py, const double2 *H, double2 *Hy, int nbatch, int NZAA,
                  int nkx, int nky, int NY);
@@ -52,12 +79,15 @@ void launch_xbwd(hipStream_t st, const FFTPlan &px, const double2 *G, double *ph
                  int nkx);
 void launch_xfwd(hipStream_t st, const FFTPlan &px, const double *phys, double2 *H, int nbatch, long long nlines,
                  int nkx);
+// field[s][pos[j]] = sum_{n<ncol} Sinv[j][n] Q[n][s] (state import of one variable)
+void launch_insert_var(hipStream_t st, const double2 *Q, int nsys, int ncol, const double *Sinv, const int *pos,
+                       double2 *field, int total);
 // remove the z-integral of Qhat[s][:] (PL_IMEX_timestepping.f90:223-235)
 void launch_remove_zmean(hipStream_t st, double2 *Qhat, int NZ, int nsys);
 
 // ---- NL assembly into K_hat ----
 struct NLArgs {
-    const double2 *Qhat;      // [nquad][nsys][NZ]
+    const double2 *Qhat;      // [nquad][NZ][nsys]
     int nsys, NZ, total, mean;
     const int *pos_eqn, *pos_row, *eqn_begin;
     const int *t_qvar, *t_kx, *t_ky, *t_w, *t_order, *t_woff;
@@ -114,6 +144,7 @@ struct FactorArgs {
     // real; factors go to Lr/Ur (same layout, real) and work is used as double
     const unsigned char *rot; // [total] psi | phi << 1
     double *Lr, *Ur;
+    int ncol16;               // > 0: multipliers in the 16-lane layout with this many columns per system
 };
 void launch_factor(hipStream_t st, const FactorArgs &a);
 // generic batched factorization of LAPACK band input (for the unit seam)
@@ -122,6 +153,17 @@ void launch_factor_lapack(hipStream_t st, double2 *work, int n, int kl, int ku, 
 void launch_solve(hipStream_t st, const double2 *Lf, const double2 *Uf, const unsigned char *piv, int n, int kl,
                   int kv, int nsys, const double2 *rhs, double2 *x);
 // real-form solve: x = i^phi . (LU)^-1 . P . i^-psi rhs, with real factors
+// 16-lane solve (four systems per wavefront): columns per system of its
+// multiplier records, or 0 when the band does not fit (kl > 16, kv > 47)
+int solve16_ncol(int n, int kl, int kv);
+// U arrays of the 16-lane solve carry this many elements of padding on each side
+constexpr int SOLVE16_PAD = 1024;
+bool launch_solve16_real(hipStream_t st, const double *L16, int ncol16, const double *Ur, const unsigned char *piv,
+                         const unsigned char *rot, int ident0, int n, int kl, int kv, int nsys, const double2 *rhs,
+                         double2 *x);
+bool launch_solve16_complex(hipStream_t st, const double2 *L16, int ncol16, const double2 *Ur,
+                            const unsigned char *piv, int ident0, int n, int kl, int kv, int nsys, const double2 *rhs,
+                            double2 *x);
 // (ident0: system 0 is the identity and is not rotated)
 void launch_solve_real(hipStream_t st, const double *Lr, const double *Ur, const unsigned char *piv,
                        const unsigned char *rot, int ident0, int n, int kl, int kv, int nsys, const double2 *rhs,

@@ -122,9 +122,9 @@ __device__ inline double2 twiddle(const double2 *__restrict__ tw, int m) {
     return SIGN > 0 ? make_double2(w.x, -w.y) : w;
 }
 
-// One LDS -> LDS stage over S sequences (sequence s at buf + s*N); the
+// One LDS -> LDS stage over S sequences (sequence s at buf + s*LS); the
 // S*T butterflies are spread over the 64 lanes round by round.
-template <int N, int R, int NS, int SIGN, int S>
+template <int N, int R, int NS, int SIGN, int S, int LS>
 __device__ inline void stage_lds(double2 *buf, const double2 *__restrict__ tw, int lane) {
     constexpr int T = N / R;
     constexpr int TOT = S * T;
@@ -136,7 +136,7 @@ __device__ inline void stage_lds(double2 *buf, const double2 *__restrict__ tw, i
         const int b = lane + 64 * c;
         if (TOT % 64 == 0 || b < TOT) {
             const int s = b / T, j = b - s * T;
-            const double2 *L = buf + s * N;
+            const double2 *L = buf + s * LS;
             const int jm = j % NS;
 #pragma unroll
             for (int r = 0; r < R; ++r) {
@@ -153,7 +153,7 @@ __device__ inline void stage_lds(double2 *buf, const double2 *__restrict__ tw, i
         const int b = lane + 64 * c;
         if (TOT % 64 == 0 || b < TOT) {
             const int s = b / T, j = b - s * T;
-            double2 *L = buf + s * N;
+            double2 *L = buf + s * LS;
             const int base = (j / NS) * NS * R + j % NS;
 #pragma unroll
             for (int r = 0; r < R; ++r) L[base + r * NS] = v[c][r];
@@ -162,22 +162,27 @@ __device__ inline void stage_lds(double2 *buf, const double2 *__restrict__ tw, i
     wave_sync();
 }
 
-// A run of LDS stages with radices Rs..., starting after NS points.
-template <int N, int SIGN, int S, int NS, int... Rs>
-struct Stages;
+// A run of LDS stages with radices Rs..., starting after NS points; sequences
+// LS elements apart.
+template <int N, int SIGN, int S, int LS, int NS, int... Rs>
+struct StagesLS;
 
-template <int N, int SIGN, int S, int NS>
-struct Stages<N, SIGN, S, NS> {
+template <int N, int SIGN, int S, int LS, int NS>
+struct StagesLS<N, SIGN, S, LS, NS> {
     __device__ static inline void run(double2 *, const double2 *__restrict__, int) {}
 };
 
-template <int N, int SIGN, int S, int NS, int R, int... Rs>
-struct Stages<N, SIGN, S, NS, R, Rs...> {
+template <int N, int SIGN, int S, int LS, int NS, int R, int... Rs>
+struct StagesLS<N, SIGN, S, LS, NS, R, Rs...> {
     __device__ static inline void run(double2 *buf, const double2 *__restrict__ tw, int lane) {
-        stage_lds<N, R, NS, SIGN, S>(buf, tw, lane);
-        Stages<N, SIGN, S, NS * R, Rs...>::run(buf, tw, lane);
+        stage_lds<N, R, NS, SIGN, S, LS>(buf, tw, lane);
+        StagesLS<N, SIGN, S, LS, NS * R, Rs...>::run(buf, tw, lane);
     }
 };
+
+// contiguous sequences (LS = N)
+template <int N, int SIGN, int S, int NS, int... Rs>
+using Stages = StagesLS<N, SIGN, S, N, NS, Rs...>;
 
 }  // namespace wfft
 }  // namespace coral

@@ -1,0 +1,270 @@
+// Chebyshev transforms along z as FP64 MFMA products (v_mfma_f64_16x16x4f64).
+//
+// The Gauss-Chebyshev points are symmetric, x_{NZAA-1-z} = -x_z, and
+// T_n(-x) = (-1)^n T_n(x).  Both transforms are therefore folded into an even
+// and an odd half of about half the size:
+//   synthesis (z bwd):  E[z] = sum_{n even} T_n(x_z) c_n,  O[z] = sum_{n odd} ...
+//                       F[z] = E + O,  F[NZAA-1-z] = E - O          (z < NZAA/2)
+//   analysis  (z fwd):  S[z] = H[z] + H[NZAA-1-z],  D[z] = H[z] - H[NZAA-1-z]
+//                       Q[n even] = sum_z Cz[n][z] S[z],  Q[n odd] = sum_z Cz[n][z] D[z]
+// (the middle point of an odd NZAA enters S once and D not at all).  This is
+// the z-direction part of c2r_transform / r2c_transform (REDFT01 / REDFT10,
+// slabs.fftw3mpi/transforms.f90:155-301) as dense real matrices acting on the
+// real and imaginary parts of every kept (kx, ky) column.
+//
+// Layout: X[b][k][col] and Y[b][m][col], col = 2*s + (re, im), rows contiguous.
+// The matrix operand lives in LDS as MFMA A fragments, frag[(tile*nk + kstep)*64
+// + lane] = A[16*tile + lane%16][4*kstep + lane/16]; a wavefront owns a 16-column
+// strip (B operand: lane = column + 16*k) and every output row of it.
+#include "kernels.hpp"
+
+#include <cstdio>
+
+namespace coral {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+__device__ inline d4 mfma(double a, double b, d4 c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
+
+// D(i, j) of a 16x16 tile sits in lane j + 16*(i/4), element i%4.
+template <int TE, int TO>
+__global__ __launch_bounds__(512) void k_zbwd_mfma(const double *__restrict__ frag, int NEK, int NOK,
+                                                    const double *__restrict__ X, double *__restrict__ F, int NZ,
+                                                    int NZAA, int ncol, int nb) {
+    extern __shared__ double lds[];
+    const int nfrag = (TE * NEK + TO * NOK) * 64;
+    for (int e = threadIdx.x; e < nfrag; e += blockDim.x) lds[e] = frag[e];
+    __syncthreads();
+    const double *GE = lds, *GO = lds + TE * NEK * 64;
+    const int lane = threadIdx.x & 63, nw = blockDim.x >> 6;
+    const int j = lane & 15, kq = lane >> 4;
+    const int ntc = (ncol + 15) >> 4, ntiles = nb * ntc;
+    const int Ke = (NZAA + 1) / 2, Ko = NZAA / 2, NE = (NZ + 1) / 2, NO = NZ / 2;
+    for (int tile = blockIdx.x * nw + (threadIdx.x >> 6); tile < ntiles; tile += gridDim.x * nw) {
+        const int b = tile / ntc, col = (tile - b * ntc) * 16 + j;
+        const bool cv = col < ncol;
+        const double *Xb = X + (size_t)b * NZ * ncol + (cv ? col : 0);
+        d4 ae[TE], ao[TO];
+#pragma unroll
+        for (int r = 0; r < TE; ++r) ae[r] = d4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int r = 0; r < TO; ++r) ao[r] = d4{0.0, 0.0, 0.0, 0.0};
+        // B operands one k-step ahead
+        auto ld_e = [&](int kk) -> double {
+            const int m = 4 * kk + kq;
+            const bool ok = cv && m < NE;
+            const double v = Xb[(size_t)(ok ? 2 * m : 0) * ncol];
+            return ok ? v : 0.0;
+        };
+        auto ld_o = [&](int kk) -> double {
+            const int m = 4 * kk + kq;
+            const bool ok = cv && m < NO;
+            const double v = Xb[(size_t)(ok ? 2 * m + 1 : 0) * ncol];
+            return ok ? v : 0.0;
+        };
+        double be = ld_e(0), bo = ld_o(0);
+        for (int kk = 0; kk < NEK; ++kk) {
+            const double be_n = ld_e(kk + 1), bo_n = ld_o(kk + 1);
+#pragma unroll
+            for (int r = 0; r < TE; ++r) ae[r] = mfma(GE[(r * NEK + kk) * 64 + lane], be, ae[r]);
+            if (kk < NOK) {
+#pragma unroll
+                for (int r = 0; r < TO; ++r) ao[r] = mfma(GO[(r * NOK + kk) * 64 + lane], bo, ao[r]);
+            }
+            be = be_n;
+            bo = bo_n;
+        }
+        if (cv) {
+            double *Fb = F + (size_t)b * NZAA * ncol + col;
+#pragma unroll
+            for (int r = 0; r < TE; ++r)
+#pragma unroll
+                for (int it = 0; it < 4; ++it) {
+                    const int z = 16 * r + 4 * kq + it;
+                    const double E = ae[r][it];
+                    const double O = (r < TO) ? ao[r < TO ? r : 0][it] : 0.0;
+                    if (z < Ke) Fb[(size_t)z * ncol] = E + O;
+                    if (z < Ko) Fb[(size_t)(NZAA - 1 - z) * ncol] = E - O;
+                }
+        }
+    }
+}
+
+template <int RE, int RO>
+__global__ __launch_bounds__(512) void k_zfwd_mfma(const double *__restrict__ frag, int KEK, int KOK,
+                                                    const double *__restrict__ Y, double *__restrict__ Q, int NZ,
+                                                    int NZAA, int ncol, int nb) {
+    extern __shared__ double lds[];
+    const int nfrag = (RE * KEK + RO * KOK) * 64;
+    for (int e = threadIdx.x; e < nfrag; e += blockDim.x) lds[e] = frag[e];
+    __syncthreads();
+    const double *FE = lds, *FO = lds + RE * KEK * 64;
+    const int lane = threadIdx.x & 63, nw = blockDim.x >> 6;
+    const int j = lane & 15, kq = lane >> 4;
+    const int ntc = (ncol + 15) >> 4, ntiles = nb * ntc;
+    const int Ke = (NZAA + 1) / 2, Ko = NZAA / 2;
+    for (int tile = blockIdx.x * nw + (threadIdx.x >> 6); tile < ntiles; tile += gridDim.x * nw) {
+        const int b = tile / ntc, col = (tile - b * ntc) * 16 + j;
+        const bool cv = col < ncol;
+        const double *Yb = Y + (size_t)b * NZAA * ncol + (cv ? col : 0);
+        d4 ae[RE], ao[RO];
+#pragma unroll
+        for (int r = 0; r < RE; ++r) ae[r] = d4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int r = 0; r < RO; ++r) ao[r] = d4{0.0, 0.0, 0.0, 0.0};
+        // folded B operands (S, D) one k-step ahead
+        auto ld = [&](int kk, double &S, double &D) {
+            const int z = 4 * kk + kq;
+            const bool oh = cv && z < Ke, om = cv && z < Ko;
+            const double h0 = Yb[(size_t)(oh ? z : 0) * ncol];
+            const double h1 = Yb[(size_t)(om ? NZAA - 1 - z : 0) * ncol];
+            const double h = oh ? h0 : 0.0, hm = om ? h1 : 0.0;
+            S = h + hm;
+            D = om ? h - hm : 0.0;
+        };
+        double S, D;
+        ld(0, S, D);
+        for (int kk = 0; kk < KEK; ++kk) {
+            double Sn, Dn;
+            ld(kk + 1, Sn, Dn);
+#pragma unroll
+            for (int r = 0; r < RE; ++r) ae[r] = mfma(FE[(r * KEK + kk) * 64 + lane], S, ae[r]);
+            if (kk < KOK) {
+#pragma unroll
+                for (int r = 0; r < RO; ++r) ao[r] = mfma(FO[(r * KOK + kk) * 64 + lane], D, ao[r]);
+            }
+            S = Sn;
+            D = Dn;
+        }
+        if (cv) {
+            double *Qb = Q + (size_t)b * NZ * ncol + col;
+#pragma unroll
+            for (int r = 0; r < RE; ++r)
+#pragma unroll
+                for (int it = 0; it < 4; ++it) {
+                    const int n = 2 * (16 * r + 4 * kq + it);
+                    if (n < NZ) Qb[(size_t)n * ncol] = ae[r][it];
+                }
+#pragma unroll
+            for (int r = 0; r < RO; ++r)
+#pragma unroll
+                for (int it = 0; it < 4; ++it) {
+                    const int n = 2 * (16 * r + 4 * kq + it) + 1;
+                    if (n < NZ) Qb[(size_t)n * ncol] = ao[r][it];
+                }
+        }
+    }
+}
+
+// ---- host side ----
+
+static int lds_limit(int device) {
+    int v = 0;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, device) != hipSuccess) v = 65536;
+    return v;
+}
+
+static void fill_frags(std::vector<double> &out, int ntile, int nk, const std::function<double(int, int)> &A, int M,
+                       int K) {
+    for (int t = 0; t < ntile; ++t)
+        for (int kk = 0; kk < nk; ++kk)
+            for (int lane = 0; lane < 64; ++lane) {
+                const int m = 16 * t + (lane & 15), k = 4 * kk + (lane >> 4);
+                out.push_back((m < M && k < K) ? A(m, k) : 0.0);
+            }
+}
+
+bool make_zfrags(int NZ, int NZAA, const std::vector<double> &Tz, const std::vector<double> &Cz, int device,
+                 ZFrags &zf) {
+    const int Ke = (NZAA + 1) / 2, Ko = NZAA / 2, NE = (NZ + 1) / 2, NO = NZ / 2;
+    zf = ZFrags{};
+    // synthesis: rows z < Ke (even part) / z < Ko (odd part), k over even / odd n
+    zf.TE = (Ke + 15) / 16;
+    zf.TO = (Ko + 15) / 16;
+    zf.NEK = (NE + 3) / 4;
+    zf.NOK = (NO + 3) / 4;
+    fill_frags(zf.bwd, zf.TE, zf.NEK, [&](int z, int m) { return Tz[(size_t)z * NZ + 2 * m]; }, Ke, NE);
+    fill_frags(zf.bwd, zf.TO, zf.NOK, [&](int z, int m) { return Tz[(size_t)z * NZ + 2 * m + 1]; }, Ko, NO);
+    // analysis: rows over even / odd n, k over z < Ke / z < Ko
+    zf.RE = (NE + 15) / 16;
+    zf.RO = (NO + 15) / 16;
+    zf.KEK = (Ke + 3) / 4;
+    zf.KOK = (Ko + 3) / 4;
+    fill_frags(zf.fwd, zf.RE, zf.KEK, [&](int m, int z) { return Cz[(size_t)(2 * m) * NZAA + z]; }, NE, Ke);
+    fill_frags(zf.fwd, zf.RO, zf.KOK, [&](int m, int z) { return Cz[(size_t)(2 * m + 1) * NZAA + z]; }, NO, Ko);
+    const size_t lim = (size_t)lds_limit(device);
+    zf.bwd_ok = zf.bwd.size() * sizeof(double) <= lim;
+    zf.fwd_ok = zf.fwd.size() * sizeof(double) <= lim;
+    return zf.bwd_ok || zf.fwd_ok;
+}
+
+template <int TE, int TO>
+static void go_bwd(hipStream_t st, const ZFragsDev &z, const double *X, double *F, int NZ, int NZAA, int ncol,
+                   int nb, int nblocks) {
+    const size_t shm = size_t(TE * z.NEK + TO * z.NOK) * 64 * sizeof(double);
+    (void)hipFuncSetAttribute((const void *)k_zbwd_mfma<TE, TO>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)shm);
+    hipLaunchKernelGGL((k_zbwd_mfma<TE, TO>), dim3(nblocks), dim3(512), shm, st, z.bwd, z.NEK, z.NOK, X, F, NZ, NZAA,
+                       ncol, nb);
+}
+
+template <int RE, int RO>
+static void go_fwd(hipStream_t st, const ZFragsDev &z, const double *Y, double *Q, int NZ, int NZAA, int ncol,
+                   int nb, int nblocks) {
+    const size_t shm = size_t(RE * z.KEK + RO * z.KOK) * 64 * sizeof(double);
+    (void)hipFuncSetAttribute((const void *)k_zfwd_mfma<RE, RO>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)shm);
+    hipLaunchKernelGGL((k_zfwd_mfma<RE, RO>), dim3(nblocks), dim3(512), shm, st, z.fwd, z.KEK, z.KOK, Y, Q, NZ, NZAA,
+                       ncol, nb);
+}
+
+static int grid_for(int ntiles, int ncu) {
+    const int want = (ntiles + 7) / 8;
+    return want < ncu ? want : ncu;
+}
+
+bool launch_zbwd_mfma(hipStream_t st, const ZFragsDev &z, const double2 *spec, double2 *Fz, int NZ, int NZAA, int nsys,
+                      int nb) {
+    if (!z.bwd_ok) return false;
+    const int ncol = 2 * nsys, ntiles = nb * ((ncol + 15) / 16), g = grid_for(ntiles, z.ncu);
+    const double *X = (const double *)spec;
+    double *F = (double *)Fz;
+    const int key = z.TE * 100 + z.TO;
+    switch (key) {
+    case 101: go_bwd<1, 1>(st, z, X, F, NZ, NZAA, ncol, nb, g); break;
+    case 202: go_bwd<2, 2>(st, z, X, F, NZ, NZAA, ncol, nb, g); break;
+    case 201: go_bwd<2, 1>(st, z, X, F, NZ, NZAA, ncol, nb, g); break;
+    case 303: go_bwd<3, 3>(st, z, X, F, NZ, NZAA, ncol, nb, g); break;
+    case 403: go_bwd<4, 3>(st, z, X, F, NZ, NZAA, ncol, nb, g); break;
+    case 404: go_bwd<4, 4>(st, z, X, F, NZ, NZAA, ncol, nb, g); break;
+    case 505: go_bwd<5, 5>(st, z, X, F, NZ, NZAA, ncol, nb, g); break;
+    case 706: go_bwd<7, 6>(st, z, X, F, NZ, NZAA, ncol, nb, g); break;
+    default: return false;
+    }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) fprintf(stderr, "coral_amd: launch failed: %s\n", hipGetErrorString(e));
+    return true;
+}
+
+bool launch_zfwd_mfma(hipStream_t st, const ZFragsDev &z, const double2 *Hy, double2 *Qhat, int NZ, int NZAA,
+                      int nsys, int nb) {
+    if (!z.fwd_ok) return false;
+    const int ncol = 2 * nsys, ntiles = nb * ((ncol + 15) / 16), g = grid_for(ntiles, z.ncu);
+    const double *Y = (const double *)Hy;
+    double *Q = (double *)Qhat;
+    const int key = z.RE * 100 + z.RO;
+    switch (key) {
+    case 101: go_fwd<1, 1>(st, z, Y, Q, NZ, NZAA, ncol, nb, g); break;
+    case 201: go_fwd<2, 1>(st, z, Y, Q, NZ, NZAA, ncol, nb, g); break;
+    case 202: go_fwd<2, 2>(st, z, Y, Q, NZ, NZAA, ncol, nb, g); break;
+    case 302: go_fwd<3, 2>(st, z, Y, Q, NZ, NZAA, ncol, nb, g); break;
+    case 403: go_fwd<4, 3>(st, z, Y, Q, NZ, NZAA, ncol, nb, g); break;
+    case 504: go_fwd<5, 4>(st, z, Y, Q, NZ, NZAA, ncol, nb, g); break;
+    default: return false;
+    }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) fprintf(stderr, "coral_amd: launch failed: %s\n", hipGetErrorString(e));
+    return true;
+}
+
+}  // namespace coral

@@ -195,6 +195,25 @@ class Solver:
         check(lib.coral_amd_system_info(self._ctx, isys, *[ctypes.byref(x) for x in v]), "coral_amd_system_info")
         return tuple(x.value for x in v)  # total, nl, nu, nkept
 
+    def variable_physical(self, kind: str, isys: int, ivar: int) -> np.ndarray:
+        """Physical values of coupled variable `ivar` of system `isys` (kind
+        "kxky": [NXAA, NYAA, NZAA]; "zero": [NZAA] profile)."""
+        k = 0 if kind == "kxky" else 1
+        shape = (self.NXAA, self.NYAA, self.NZAA) if k == 0 else (self.NZAA,)
+        out = np.empty(shape, np.float64)
+        check(lib.coral_amd_get_variable_physical(self._ctx, k, isys, ivar, out.ctypes.data_as(ctypes.c_void_p)),
+              "coral_amd_get_variable_physical")
+        return out
+
+    def set_variable_physical(self, kind: str, isys: int, ivar: int, values: np.ndarray):
+        """Replace the Galerkin coefficients of one coupled variable from its
+        physical values (the inverse of `variable_physical`)."""
+        k = 0 if kind == "kxky" else 1
+        shape = (self.NXAA, self.NYAA, self.NZAA) if k == 0 else (self.NZAA,)
+        v = np.ascontiguousarray(values, np.float64).reshape(shape)
+        check(lib.coral_amd_set_variable_physical(self._ctx, k, isys, ivar, v.ctypes.data_as(ctypes.c_void_p)),
+              "coral_amd_set_variable_physical")
+
     def factor_is_real(self, isys: int = 0) -> bool:
         """True when kxky system `isys` is factored in real form."""
         v = ctypes.c_int32()

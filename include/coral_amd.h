@@ -147,6 +147,19 @@ int coral_amd_system_info(coral_amd_ctx *ctx, int32_t isys, int32_t *total, int3
  * coral_amd_create forces the complex form. */
 int coral_amd_factor_kind(coral_amd_ctx *ctx, int32_t isys, int32_t *is_real);
 
+/* Physical values of coupled variable ivar of system isys in the current state:
+ * the volumes Coral writes to CheckPoints/QuickSave.{kxky,zero}.sysNNN.varNNN.rollingR
+ * (export_checkpoints, PL_IMEX_timestepping_output.f90:388-455).  kind 0: C
+ * shape [NXAA][NYAA][NZAA] (file order: z fastest, then y, then x; the (0,0)
+ * mode excluded); kind 1: the NZAA-point profile of a mean variable. */
+int coral_amd_get_variable_physical(coral_amd_ctx *ctx, int32_t kind, int32_t isys, int32_t ivar, double *out);
+/* Inverse: forward transform, solve of the square-truncated Galerkin stencil
+ * and insertion of the variable's Galerkin coefficients into the state
+ * (import_quicksave, PL_IMEX_timestepping_output.f90:309-385).  Other variables
+ * keep their coefficients; the kxky (0,0) mode stays zero. */
+int coral_amd_set_variable_physical(coral_amd_ctx *ctx, int32_t kind, int32_t isys, int32_t ivar,
+                                    const double *in);
+
 /* Unit seams on host buffers.
  * Backward transform of a spectral field, C shape [NX/2][NYAA][NZ] complex, to
  * physical C shape [NXAA][NYAA][NZAA] (c2r_transform, slabs.fftw3mpi/transforms.f90:155-225);

@@ -40,6 +40,10 @@ CASES = [
     ("coral.equations.rbc_ra1e5", 16, 16, 17, 2 * np.pi, 2 * np.pi, "ARS_232", 1e-4, 3),
     ("coral.equations.rbc_ra1e5", 16, 16, 17, 2 * np.pi, 2 * np.pi, "ARS_233", 1e-4, 3),
     ("coral.equations.rbc_ra1e5", 16, 16, 17, 2 * np.pi, 2 * np.pi, "ARS_122", 1e-4, 3),
+    # vertical resolutions of the production grids (z transform tile shapes)
+    ("coral.equations.rrbc_ra5e7_ek1e-4", 16, 16, 129, 2 * np.pi, 2 * np.pi, "ARS_222", 2e-5, 2),
+    ("coral.equations.rbc_ra1e5", 32, 32, 65, 2 * np.pi, 2 * np.pi, "ARS_222", 1e-4, 2),
+    ("coral.equations.rbc_ra1e5", 16, 32, 49, 2.0, 2 * np.pi, "ARS_343", 1e-4, 2),
 ]
 
 
@@ -85,6 +89,30 @@ def test_complex_factor_path_parity(solver_mod, monkeypatch):
     g = solver_mod.Solver(r, NX, NY, NZ, Lx, Ly, scheme=scheme)
     try:
         assert not g.factor_is_real(0)
+        g.set_state(k0, z0)
+        for it in range(nsteps):
+            ora.step(dt)
+            g.march_forward(dt)
+        k, z = g.get_state()
+        assert rel(k[0], ora.field[0]) <= TOL
+        for a, b in zip(z, ora.zfield):
+            assert rel(a, b) <= TOL
+    finally:
+        g.close()
+
+
+def test_plain_gemm_z_transform_parity(solver_mod, monkeypatch):
+    """The vector-FMA z transforms (used when no MFMA variant covers NZ),
+    forced through CORAL_AMD_NO_MFMA."""
+    name, NX, NY, NZ, Lx, Ly, scheme, dt, nsteps = CASES[1]
+    monkeypatch.setenv("CORAL_AMD_NO_MFMA", "1")
+    r = product_recipe(name)
+    k0, z0 = ic.rbc_initial_state(r, NX, NY, NZ, 1e-2, 6, 6, 3)
+    geo = O.Geometry(NX, NY, NZ, Lx, Ly)
+    ora = O.OracleSolver(r, geo, scheme)
+    ora.field, ora.zfield = [a.copy() for a in k0], [a.copy() for a in z0]
+    g = solver_mod.Solver(r, NX, NY, NZ, Lx, Ly, scheme=scheme)
+    try:
         g.set_state(k0, z0)
         for it in range(nsteps):
             ora.step(dt)

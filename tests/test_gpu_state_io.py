@@ -1,0 +1,110 @@
+"""GPU tests of the state files (QuickSave volumes), through the C ABI."""
+import math
+import os
+import shutil
+
+import numpy as np
+import pytest
+
+from conftest import product_recipe, recipe
+from coral_amd import ic, state_io
+from oracle import coral_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def solver_mod(native_lib):
+    from coral_amd import solver
+    return solver
+
+
+def rel(a, b):
+    nb = np.linalg.norm(b)
+    return float(np.linalg.norm(a - b) / (nb if nb > 0 else 1.0))
+
+
+def test_quicksave_import_heat_equation(solver_mod, reference_numbers, tmp_path):
+    """Restart files laid out as etc/benchmarks/00_heat_equation/generate_initial_condition.py
+    writes them (zero kxky volumes, sine profiles of the mean variables), imported and
+    checked against the oracle's projection and against the reference's error table."""
+    r = recipe("heat_equation.coral.equations")
+    setup = reference_numbers["heat_equation_setup"]
+    table = reference_numbers["heat_equation_error_after_10_steps"]
+    NX, NZ = 4, 128
+    geo = O.Geometry(NX, NX, NZ, 1.0, 1.0)
+    zg = geo.gauss_chebyshev_grid()
+    restart = tmp_path / "Restart"
+    restart.mkdir()
+    zero = np.zeros((geo.NXAA, geo.NYAA, geo.NZAA))
+    for isys, s in enumerate(r.kxky_systems):
+        for ivar in range(s.nvars):
+            zero.tofile(restart / state_io.quicksave_name("kxky", isys, ivar, 1))
+    prof = {"u": setup["amplitudes"]["u"] * np.sin((2 * setup["modes"]["u"] + 1) * (zg - 1) * np.pi / 2),
+            "v": setup["amplitudes"]["v"] * np.sin((2 * setup["modes"]["v"] + 1) * zg * np.pi / 2),
+            "w": setup["amplitudes"]["w"] * np.sin(setup["modes"]["w"] * zg * np.pi)}
+    names = ["u", "v", "w"]
+    for isys, s in enumerate(r.zero_systems):
+        prof[names[isys]].tofile(restart / state_io.quicksave_name("zero", isys, 0, 1))
+    # the oracle's projection of the first profile (DCT-II of the Gauss values, stencil solve)
+    ops = O.SystemOps(r.zero_systems[0], NZ, 0.5, 1.0)
+    import scipy.fft as sfft
+    c = sfft.dct(prof["u"], type=2)[:NZ] / geo.NZAA
+    c[0] *= 0.5
+    nc = ops.var_size[0]
+    zref = np.zeros(ops.total)
+    zref[ops.posP[:nc]] = np.linalg.solve(ops.stencils[0][:nc, :], c[:nc])
+    n = setup["modes"]["u"]
+    rate = setup["Prandtl"] * ((2 * n + 1) * np.pi / 2) ** 2
+    idx = 3
+    dt, ref = table["dt"][idx], table["ARS_222"][idx]
+    with solver_mod.Solver(r, NX, NX, NZ, 1.0, 1.0, scheme="ARS_222") as g:
+        assert state_io.import_quicksave(g, str(tmp_path)) == 1
+        k, z = g.get_state()
+        assert rel(z[0], zref) <= 1e-12
+        for a in k:
+            assert not np.any(a)
+        p0 = g.physical(0)[0, 0, :].copy()
+        assert rel(p0, prof["u"]) <= 1e-12
+        g.march_forward(dt, 10)
+        p1 = g.physical(0)[0, 0, :]
+    err = math.sqrt(np.sum((p1 - p0 * np.exp(-rate * 10 * dt)) ** 2) / np.sum(p0 ** 2))
+    assert abs(err / ref - 1.0) < 2e-3
+
+
+def test_checkpoint_round_trip_and_consistency(solver_mod, tmp_path):
+    name = "coral.equations.rrbc_ra5e7_ek1e-4"
+    r = product_recipe(name)
+    NX, NY, NZ = 32, 24, 17
+    k0, z0 = ic.rbc_initial_state(r, NX, NY, NZ, 1e-2, 40, 8, 4)
+    rng = np.random.default_rng(4)
+    # populate the mean variables too
+    z0 = [rng.normal(size=a.shape) * 1e-3 for a in z0]
+    with solver_mod.Solver(r, NX, NY, NZ, 2.0, 3.0) as g:
+        g.set_state(k0, z0)
+        g.factorize_operators(2e-5)
+        g.march_forward(2e-5, 2)
+        k1, z1 = g.get_state()
+        paths = state_io.export_checkpoints(g, str(tmp_path), rolling=2)
+        assert len(paths) == sum(s.nvars for s in r.kxky_systems + r.zero_systems)
+        # a full variable is the sum of its kxky and mean parts
+        for ifull, fv in enumerate(r.full_vars):
+            total = np.zeros((g.NXAA, g.NYAA, g.NZAA))
+            for t in fv.terms:
+                if (t.kx, t.ky, t.dz, t.mz) != (0, 0, 0, 0):
+                    break
+                part = g.variable_physical(t.kind, t.system, t.position)
+                total += t.dsca * (part if t.kind == "kxky" else part[None, None, :])
+            else:
+                assert rel(total, g.physical(ifull)) <= 1e-12
+        # state unchanged by the export
+        k2, z2 = g.get_state()
+        assert all(np.array_equal(a, b) for a, b in zip(k1, k2))
+    shutil.move(str(tmp_path / "CheckPoints"), str(tmp_path / "Restart"))
+    with solver_mod.Solver(r, NX, NY, NZ, 2.0, 3.0) as h:
+        assert state_io.import_quicksave(h, str(tmp_path)) == 2
+        k3, z3 = h.get_state()
+    for a, b in zip(k3, k1):
+        assert rel(a, b) <= 1e-12
+    for a, b in zip(z3, z1):
+        assert rel(a, b) <= 1e-12

@@ -63,6 +63,7 @@ EXPORTS = (
     "coral_amd_bwd_transform", "coral_amd_fwd_transform", "coral_amd_banded_solve",
     "coral_amd_timers_enable", "coral_amd_timer_read", "coral_amd_assembled_operator",
     "coral_amd_factor_kind", "coral_amd_get_variable_physical", "coral_amd_set_variable_physical",
+    "coral_amd_horizontal_averages",
 )
 
 

@@ -494,6 +494,11 @@ int fail(const std::exception &e) {
 
 // ---- small helper kernels used from the host side -------------------------
 namespace {
+__global__ void k_gather_strided(const double2 *src, size_t stride, int count, double2 *dst) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) dst[i] = src[(size_t)i * stride];
+}
+
 __global__ void k_fill_zero(double2 *p, size_t n) {
     size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i < n) p[i] = make_double2(0.0, 0.0);
@@ -977,6 +982,68 @@ int coral_amd_set_variable_physical(coral_amd_ctx *c, int32_t kind, int32_t isys
             HIP_OK(hipMemcpy(d.field.p, f.data(), f.size() * sizeof(double2), hipMemcpyHostToDevice));
         } else {
             throw std::runtime_error("kind must be 0 (kxky) or 1 (mean)");
+        }
+        return 0;
+    } catch (const std::exception &e) {
+        return fail(e);
+    }
+}
+
+int coral_amd_horizontal_averages(coral_amd_ctx *c, double *full_avg, double *quad_avg) {
+    try {
+        if (!c) throw std::runtime_error("null context");
+        const Problem &P = c->P;
+        HIP_OK(hipSetDevice(c->device));
+        std::vector<const double2 *> kst, zst;
+        for (auto &d : c->ks) kst.push_back(d->field.p);
+        for (auto &d : c->zs) zst.push_back(d->field.p);
+        DevBuf<double2> col;
+        const int nf = std::max(P.nfull, 1), nq = std::max(P.nquad, 1);
+        col.alloc(size_t(nf) * P.NZ + size_t(nq) * P.NZAA);
+        if (quad_avg) {
+            // products of the current state up to the y transform; column s = 0 is the (0,0) mode
+            backward_to_G(*c, kst, zst);
+            HIP_OK(hipMemsetAsync(c->cflmax.p, 0, 3 * sizeof(unsigned long long), c->stream));
+            XPassArgs xa{};
+            xa.G = c->G.p;
+            xa.H = c->H.p;
+            xa.nfull = P.nfull;
+            xa.nquad = P.nquad;
+            xa.NZAA = P.NZAA;
+            xa.NYAA = P.NYAA;
+            xa.nkx = P.nkx;
+            xa.qa = c->qa.p;
+            xa.qb = c->qb.p;
+            xa.zweight = c->zweight.p;
+            xa.cflmax = c->cflmax.p;
+            launch_xpass(c->stream, c->px, xa);
+            launch_yfwd(c->stream, c->py, c->H.p, c->Hy.p, P.nquad, P.NZAA, P.nkx, P.nky, P.NY);
+            const int n = P.nquad * P.NZAA;
+            hipLaunchKernelGGL(k_gather_strided, dim3((n + 255) / 256), dim3(256), 0, c->stream, c->Hy.p,
+                               size_t(P.nsys), n, col.p + size_t(nf) * P.NZ);
+        } else {
+            backward_to_G(*c, kst, zst);
+        }
+        // full variables: (0,0) Chebyshev coefficients spec[f][n][0]
+        const int m = P.nfull * P.NZ;
+        if (m > 0)
+            hipLaunchKernelGGL(k_gather_strided, dim3((m + 255) / 256), dim3(256), 0, c->stream, c->spec.p,
+                               size_t(P.nsys), m, col.p);
+        HIP_OK(hipStreamSynchronize(c->stream));
+        std::vector<double2> h(col.n);
+        HIP_OK(hipMemcpy(h.data(), col.p, h.size() * sizeof(double2), hipMemcpyDeviceToHost));
+        if (full_avg)
+            for (int f = 0; f < P.nfull; ++f)
+                for (int z = 0; z < P.NZAA; ++z) {
+                    double v = 0.0;
+                    for (int n2 = 0; n2 < P.NZ; ++n2) v += P.Tz[size_t(z) * P.NZ + n2] * h[size_t(f) * P.NZ + n2].x;
+                    full_avg[size_t(f) * P.NZAA + z] = v;
+                }
+        if (quad_avg) {
+            const double inv = 1.0 / (double(P.NXAA) * double(P.NYAA));
+            for (int q = 0; q < P.nquad; ++q)
+                for (int z = 0; z < P.NZAA; ++z)
+                    quad_avg[size_t(q) * P.NZAA + z] = h[size_t(nf) * P.NZ + size_t(q) * P.NZAA + z].x * inv;
         }
         return 0;
     } catch (const std::exception &e) {

@@ -214,6 +214,17 @@ class Solver:
         check(lib.coral_amd_set_variable_physical(self._ctx, k, isys, ivar, v.ctypes.data_as(ctypes.c_void_p)),
               "coral_amd_set_variable_physical")
 
+    def horizontal_averages(self):
+        """(full [nfull, NZAA], quadratic [nquad, NZAA]) horizontal averages of the
+        current state at the Gauss points."""
+        nf, nq = len(self.recipe.full_vars), len(self.recipe.quad_vars)
+        full = np.zeros((max(nf, 1), self.NZAA))
+        quad = np.zeros((max(nq, 1), self.NZAA))
+        check(lib.coral_amd_horizontal_averages(self._ctx, full.ctypes.data_as(ctypes.c_void_p),
+                                                quad.ctypes.data_as(ctypes.c_void_p)),
+              "coral_amd_horizontal_averages")
+        return full[:nf], quad[:nq]
+
     def factor_is_real(self, isys: int = 0) -> bool:
         """True when kxky system `isys` is factored in real form."""
         v = ctypes.c_int32()

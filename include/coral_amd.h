@@ -160,6 +160,15 @@ int coral_amd_get_variable_physical(coral_amd_ctx *ctx, int32_t kind, int32_t is
 int coral_amd_set_variable_physical(coral_amd_ctx *ctx, int32_t kind, int32_t isys, int32_t ivar,
                                     const double *in);
 
+/* Horizontal averages of the current state at the NZAA Gauss points, the
+ * quantities behind Coral's timeseries (output_global_quantities,
+ * PL_IMEX_timestepping_output.f90:5-231: zSlice objects are these values,
+ * volume objects their weighted sums): full_avg [nfull][NZAA], quad_avg
+ * [nquad][NZAA] (either may be NULL).  The averages are the (kx, ky) = (0, 0)
+ * modes, taken after the z transform (full variables) and after the x/y
+ * transforms of the products (quadratic variables). */
+int coral_amd_horizontal_averages(coral_amd_ctx *ctx, double *full_avg, double *quad_avg);
+
 /* Unit seams on host buffers.
  * Backward transform of a spectral field, C shape [NX/2][NYAA][NZ] complex, to
  * physical C shape [NXAA][NYAA][NZAA] (c2r_transform, slabs.fftw3mpi/transforms.f90:155-225);

@@ -22,6 +22,7 @@ INPUTS = {
     "stressfree_rb.coral.parameters.in": "etc/benchmarks/01_Rayleigh_Benard_stressFree/coral.parameters.in",
     "rrbc_julien.coral.equations.velocityPressure": "etc/benchmarks/02_RRBC_julien_JFM96/coral.equations.velocityPressure",
     "rrbc_julien.coral.parameters.in": "etc/benchmarks/02_RRBC_julien_JFM96/coral.parameters.in",
+    "rrbc_julien.coral.timeseries": "etc/benchmarks/02_RRBC_julien_JFM96/coral.timeseries",
     "rbc_vanilla.coral.equations": "etc/coral_equations_examples/coral.equations.RBC_primitiveVars_Vanilla",
     "example.coral.parameters.in": "etc/coral_equations_examples/coral.parameters.in",
 }

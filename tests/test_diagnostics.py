@@ -1,0 +1,54 @@
+"""Diagnostics on the host: timeseries objects, weights and wall fluxes (no GPU)."""
+import math
+import os
+
+import numpy as np
+import pytest
+
+from conftest import INPUTS
+from coral_amd import diagnostics as D
+
+
+def test_parse_reference_timeseries_file():
+    text = open(os.path.join(INPUTS, "rrbc_julien.coral.timeseries")).read()
+    objs = D.parse_timeseries(text)
+    assert [(o.kind, o.name, o.position) for o in objs] == [
+        ("volume", "uu", 0), ("volume", "vv", 0), ("volume", "ww", 0),
+        ("zSlice", "tFull", 1), ("zSlice", "tFull", 24), ("zSlice", "tFull", 48), ("zSlice", "tFull", 72)]
+    # the file names compute_nusselt.py reads (etc/benchmarks/02_RRBC_julien_JFM96/compute_nusselt.py:8-9)
+    assert objs[3].filename == "tFull_XYavg_z00001.dat"
+    assert objs[6].filename == "tFull_XYavg_z00072.dat"
+    assert objs[0].filename == "uu_volAvg.dat"
+    with pytest.raises(ValueError):
+        D.parse_timeseries(">>Volume_sum :: uu <<")
+
+
+@pytest.mark.parametrize("N", [25, 72, 193])
+def test_weights_integrate_low_polynomials(N):
+    # midpoint weights sum to the gap and integrate linear functions exactly
+    gap, center = 1.3, 0.2
+    w = D.gauss_chebyshev_weights(N, gap)
+    z = D.gauss_chebyshev_grid(N, center, gap)
+    assert abs(w.sum() - gap) < 1e-14
+    assert abs(np.dot(w, z) - gap * center) < 1e-13
+
+
+def test_wall_gradients_of_polynomials():
+    N = 96
+    z = D.gauss_chebyshev_grid(N, 0.5, 1.0)
+    bot, top = D.wall_gradients(z ** 3 - 2 * z, 1.0)
+    assert abs(bot + 2.0) < 1e-9 and abs(top - 1.0) < 1e-9
+    bot, top = D.wall_gradients(np.sin(np.pi * z), 1.0)
+    assert abs(bot - np.pi) < 1e-8 and abs(top + np.pi) < 1e-8
+
+
+def test_wall_differences_match_a_conduction_profile():
+    # theta = a (1 - z) near the top wall and a z near the bottom wall: the
+    # one-sided difference over the first Gauss point is exact for linear profiles
+    N, a = 72, 0.7
+    z = D.gauss_chebyshev_grid(N, 0.5, 1.0)
+    top, _ = D.nusselt_wall_differences(a * (1 - z), 1.0)
+    _, bot = D.nusselt_wall_differences(-a * z, 1.0)
+    assert abs(top - a) < 1e-12 and abs(bot - a) < 1e-12
+    # dz of the reference script for NZAA = 72
+    assert abs(0.5 * (1 - math.cos(math.pi / 144)) - (1. - 0.5 - 0.5 * np.cos(np.pi / 2 / 72.))) < 1e-16

@@ -108,3 +108,30 @@ def test_checkpoint_round_trip_and_consistency(solver_mod, tmp_path):
         assert rel(a, b) <= 1e-12
     for a, b in zip(z3, z1):
         assert rel(a, b) <= 1e-12
+
+
+def test_horizontal_averages_match_oracle(solver_mod):
+    """Horizontal averages of full and quadratic variables (the timeseries
+    quantities) against the oracle's physical fields averaged over x and y."""
+    r = product_recipe("coral.equations.rbc_ra1e5")
+    NX, NY, NZ = 32, 24, 17
+    k0, z0 = ic.rbc_initial_state(r, NX, NY, NZ, 1e-2, 40, 8, 6)
+    rng = np.random.default_rng(6)
+    z0 = [rng.normal(size=a.shape) * 1e-2 for a in z0]
+    geo = O.Geometry(NX, NY, NZ, 2.0, 3.0)
+    ora = O.OracleSolver(r, geo, "ARS_222")
+    ora.field, ora.zfield = [a.copy() for a in k0], [a.copy() for a in z0]
+    phys = ora.physical_full_variables()
+    full_ref = [p.mean(axis=(0, 1)) for p in phys]
+    quad_ref = []
+    for q in r.quad_vars:
+        a = phys[q.ivar1] if q.ivar1 >= 0 else 1.0
+        b = phys[q.ivar2] if q.ivar2 >= 0 else 1.0
+        quad_ref.append(np.broadcast_to(a * b, phys[0].shape).mean(axis=(0, 1)))
+    with solver_mod.Solver(r, NX, NY, NZ, 2.0, 3.0) as g:
+        g.set_state(k0, z0)
+        full, quad = g.horizontal_averages()
+    for got, ref in zip(full, full_ref):
+        assert np.max(np.abs(got - ref)) <= 1e-12 * max(1.0, np.max(np.abs(ref)))
+    for got, ref in zip(quad, quad_ref):
+        assert np.max(np.abs(got - ref)) <= 1e-12 * max(1.0, np.max(np.abs(ref)))

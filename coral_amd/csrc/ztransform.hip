@@ -26,7 +26,7 @@ typedef double d4 __attribute__((ext_vector_type(4)));
 
 __device__ inline d4 mfma(double a, double b, d4 c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
 
-// D(i, j) of a 16x16 tile sits in lane j + 16*(i/4), element i%4.
+// D(i, j) of a 16x16 tile sits in lane j + 16*(i%4), element i/4.
 template <int TE, int TO>
 __global__ __launch_bounds__(512) void k_zbwd_mfma(const double *__restrict__ frag, int NEK, int NOK,
                                                     const double *__restrict__ X, double *__restrict__ F, int NZ,
@@ -80,7 +80,7 @@ __global__ __launch_bounds__(512) void k_zbwd_mfma(const double *__restrict__ fr
             for (int r = 0; r < TE; ++r)
 #pragma unroll
                 for (int it = 0; it < 4; ++it) {
-                    const int z = 16 * r + 4 * kq + it;
+                    const int z = 16 * r + 4 * it + kq;
                     const double E = ae[r][it];
                     const double O = (r < TO) ? ao[r < TO ? r : 0][it] : 0.0;
                     if (z < Ke) Fb[(size_t)z * ncol] = E + O;
@@ -142,14 +142,14 @@ __global__ __launch_bounds__(512) void k_zfwd_mfma(const double *__restrict__ fr
             for (int r = 0; r < RE; ++r)
 #pragma unroll
                 for (int it = 0; it < 4; ++it) {
-                    const int n = 2 * (16 * r + 4 * kq + it);
+                    const int n = 2 * (16 * r + 4 * it + kq);
                     if (n < NZ) Qb[(size_t)n * ncol] = ae[r][it];
                 }
 #pragma unroll
             for (int r = 0; r < RO; ++r)
 #pragma unroll
                 for (int it = 0; it < 4; ++it) {
-                    const int n = 2 * (16 * r + 4 * kq + it) + 1;
+                    const int n = 2 * (16 * r + 4 * it + kq) + 1;
                     if (n < NZ) Qb[(size_t)n * ncol] = ao[r][it];
                 }
         }

@@ -60,7 +60,9 @@ def kernel_bytes(name, P):
     S_sys = 16 * N * nsys
     table = {
         "solve": solve_bytes(N, P["kl"], P["kl"] + P["ku"], nsys, P["real"]),
-        "kstd": 2 * S_sys,                                   # x read, K_std written
+        # M x^n + L (dt a x_1) + K_hat terms: ARS_222 reads 2 (stage 1) and 4 (stage 2)
+        # state vectors and writes one, 4 on average per launch
+        "rhs": 4 * S_sys,
         "extract": S_sys + nf * 16 * NZ * nsys,              # state read, spectra written
         "zbwd": nf * 16 * (NZ + NZAA) * nsys,
         "ybwd": nf * 16 * NZAA * (nsys + NYAA * nkx),

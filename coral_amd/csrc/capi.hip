@@ -59,7 +59,8 @@ struct SysDev {
     bool kxky = true;
     int nsys = 0;
     DevBuf<double2> field, aux;
-    std::vector<std::unique_ptr<DevBuf<double2>>> Khat, Kstd;
+    std::vector<std::unique_ptr<DevBuf<double2>>> Khat, X;   // per stage: K_hat and the stage solution
+    DevBuf<double2> cmass, cstif;                              // [nsys][ngroups] group coefficients
     DevBuf<double2> Lf, Uf;      // complex factors
     bool real = false;           // factored in real form (Lr, Ur, rot)
     int ncol16 = 0;              // > 0: 16-lane solve layout (see solve16_ncol)
@@ -77,6 +78,11 @@ struct SysDev {
     } mass, stif;
     DevBuf<int> pos_eqn, pos_row, eqn_begin, t_qvar, t_kx, t_ky, t_w, t_order, t_woff;
     DevBuf<double> t_dsca, wv;
+    // kxky NL program (NLProgArgs)
+    DevBuf<int> nl_pos_ptr, nl_blk_term, nl_blk_ptr, nl_ent_qm;
+    DevBuf<double> nl_ent_w;
+    DevBuf<double2> nl_cmono;
+    int nl_nterms = 0;
 };
 
 struct ExtractDev {
@@ -189,7 +195,7 @@ FFTPlan make_plan(int N, DevBuf<double2> &tw) {
     return p;
 }
 
-void fill_system_device(SysDev &d, int nstages) {
+void fill_system_device(SysDev &d, int nstages, const KeptModes &km, hipStream_t st, int NZ) {
     SystemTables &t = *d.t;
     const size_t n = size_t(d.nsys) * t.total;
     d.field.alloc(n);
@@ -198,9 +204,9 @@ void fill_system_device(SysDev &d, int nstages) {
     HIP_OK(hipMemset(d.aux.p, 0, n * sizeof(double2)));
     for (int i = 0; i < nstages; ++i) {
         d.Khat.emplace_back(new DevBuf<double2>());
-        d.Kstd.emplace_back(new DevBuf<double2>());
+        d.X.emplace_back(new DevBuf<double2>());
         d.Khat.back()->alloc(n);
-        d.Kstd.back()->alloc(n);
+        d.X.back()->alloc(n);
     }
     auto up = [](SysDev::OpBufs &b, const OpTables &o) {
         b.ngroups = o.ngroups;
@@ -215,6 +221,12 @@ void fill_system_device(SysDev &d, int nstages) {
     };
     up(d.mass, t.mass);
     up(d.stif, t.stif);
+    // group coefficients (i kx)^a (i ky)^b summed over each group, per system
+    const KeptModes kmd = d.kxky ? km : KeptModes{1, 1, 2, 0.0, 0.0};
+    d.cmass.alloc(size_t(d.nsys) * std::max(t.mass.ngroups, 1));
+    d.cstif.alloc(size_t(d.nsys) * std::max(t.stif.ngroups, 1));
+    launch_group_coefs(st, d.mass.dev(), kmd, d.nsys, d.cmass.p);
+    launch_group_coefs(st, d.stif.dev(), kmd, d.nsys, d.cstif.p);
     d.pos_eqn.upload(t.pos_eqn);
     d.pos_row.upload(t.pos_row);
     d.eqn_begin.upload(t.nl_eqn_begin);
@@ -249,6 +261,47 @@ void fill_system_device(SysDev &d, int nstages) {
     std::vector<double> wv = t.nl_w;
     if (wv.empty()) wv.push_back(0.0);
     d.wv.upload(wv);
+    if (d.kxky) {
+        // per-position NL program (see NLProgArgs)
+        std::vector<int> pos_ptr(1, 0), blk_term, blk_ptr(1, 0), ent_qm;
+        std::vector<double> ent_w;
+        for (int p = 0; p < t.total; ++p) {
+            const int e = t.pos_eqn[p], r = t.pos_row[p];
+            for (int ti = t.nl_eqn_begin[e]; ti < t.nl_eqn_begin[e + 1]; ++ti) {
+                const NLTable &nt = t.nl_terms[ti];
+                const int w = nt.w, m0 = r + nt.order;
+                for (int dd = -w; dd <= w; ++dd) {
+                    const int m = m0 + dd;
+                    if (m < 0 || m >= NZ) continue;
+                    const double wt = t.nl_w[nt.woff + r * (2 * w + 1) + dd + w];
+                    if (wt == 0.0) continue;
+                    ent_qm.push_back(nt.qvar * NZ + m);
+                    ent_w.push_back(nt.dsca * wt);
+                }
+                if (int(ent_qm.size()) > blk_ptr.back()) {
+                    blk_term.push_back(ti);
+                    blk_ptr.push_back(int(ent_qm.size()));
+                }
+            }
+            pos_ptr.push_back(int(blk_term.size()));
+        }
+        if (blk_term.empty()) {
+            blk_term.push_back(0);
+            blk_ptr.push_back(0);
+        }
+        if (ent_qm.empty()) {
+            ent_qm.push_back(0);
+            ent_w.push_back(0.0);
+        }
+        d.nl_pos_ptr.upload(pos_ptr);
+        d.nl_blk_term.upload(blk_term);
+        d.nl_blk_ptr.upload(blk_ptr);
+        d.nl_ent_qm.upload(ent_qm);
+        d.nl_ent_w.upload(ent_w);
+        d.nl_nterms = std::max(int(t.nl_terms.size()), 1);
+        d.nl_cmono.alloc(size_t(d.nsys) * d.nl_nterms);
+        launch_term_monomials(st, d.t_kx.p, d.t_ky.p, int(t.nl_terms.size()), km, d.nsys, d.nl_cmono.p);
+    }
     const int kv = t.nl + t.nu;
     const char *force = std::getenv("CORAL_AMD_COMPLEX_FACTORS");
     d.real = t.real_form && !(force && force[0] == '1');
@@ -315,25 +368,32 @@ void factorize(coral_amd_ctx &c, SysDev &d, double dt) {
     (void)kv;
 }
 
-// rhs / K_std
-void band_apply(coral_amd_ctx &c, SysDev &d, bool mass, const double2 *x, double2 *y,
-                const std::vector<std::pair<const double2 *, double>> &vecs) {
+// rhs = M field + L (sum xa X) + sum kb K_hat
+void rhs(coral_amd_ctx &c, SysDev &d, const std::vector<std::pair<const double2 *, double>> &xs,
+         const std::vector<std::pair<const double2 *, double>> &ks) {
     SystemTables &t = *d.t;
-    BandApplyArgs a{};
-    a.op = mass ? d.mass.dev() : d.stif.dev();
+    RhsArgs a{};
+    a.mass = d.mass.dev();
+    a.stif = d.stif.dev();
+    a.cmass = d.cmass.p;
+    a.cstif = d.cstif.p;
     a.total = t.total;
     a.nsys = d.nsys;
-    a.x = x;
-    if (vecs.size() > 8) throw std::runtime_error("too many IMEX stages");
-    a.nvec = int(vecs.size());
-    for (size_t i = 0; i < vecs.size(); ++i) {
-        a.vec[i] = vecs[i].first;
-        a.alpha[i] = vecs[i].second;
+    a.x0 = d.field.p;
+    if (xs.size() > 8 || ks.size() > 8) throw std::runtime_error("too many IMEX stages");
+    a.nx = int(xs.size());
+    for (size_t k = 0; k < xs.size(); ++k) {
+        a.xv[k] = xs[k].first;
+        a.xa[k] = xs[k].second;
+    }
+    a.nk = int(ks.size());
+    for (size_t k = 0; k < ks.size(); ++k) {
+        a.kv[k] = ks[k].first;
+        a.kb[k] = ks[k].second;
     }
     a.zero00 = d.kxky ? 1 : 0;
-    a.km = d.kxky ? c.km : KeptModes{1, 1, 2, 0.0, 0.0};
-    a.y = y;
-    c.timed(mass ? "rhs" : "kstd", [&] { launch_band_apply(c.stream, a); });
+    a.y = d.aux.p;
+    c.timed(d.kxky ? "rhs" : "rhs_mean", [&] { launch_rhs(c.stream, a); });
 }
 
 // z transforms: MFMA kernels where a variant covers the shape, plain GEMM otherwise
@@ -350,6 +410,22 @@ void zfwd(coral_amd_ctx &c, const double2 *Hy, double2 *Qhat, int nb) {
 
 void nl_assemble(coral_amd_ctx &c, SysDev &d, double2 *Khat) {
     SystemTables &t = *d.t;
+    if (d.kxky) {
+        NLProgArgs a{};
+        a.pos_ptr = d.nl_pos_ptr.p;
+        a.blk_term = d.nl_blk_term.p;
+        a.blk_ptr = d.nl_blk_ptr.p;
+        a.ent_qm = d.nl_ent_qm.p;
+        a.ent_w = d.nl_ent_w.p;
+        a.cmono = d.nl_cmono.p;
+        a.nterms = d.nl_nterms;
+        a.Qhat = c.Qhat.p;
+        a.nsys = c.P.nsys;
+        a.total = t.total;
+        a.Khat = Khat;
+        c.timed("nl_assemble", [&] { launch_nl_prog(c.stream, a); });
+        return;
+    }
     NLArgs a{};
     a.Qhat = c.Qhat.p;
     a.nsys = c.P.nsys;
@@ -378,16 +454,26 @@ void backward_to_G(coral_amd_ctx &c, const std::vector<const double2 *> &kst, co
     const Problem &P = c.P;
     const size_t fsz = size_t(P.nsys) * P.NZ;
     c.timed("extract", [&] {
-        for (int f = 0; f < P.nfull; ++f)
-            if (!c.full_has_kxky[f]) fill_zero(c.stream, c.spec.p + f * fsz, fsz);
-        std::vector<int> started(P.nfull, 0);
-        for (auto &e : c.ext) {
-            if (e->kind != 0) continue;
-            SysDev &d = *c.ks[e->system];
-            launch_extract(c.stream, kst[e->system], d.t->total, P.nsys, P.NZ, e->rowptr.p, e->col.p, e->val.p,
-                           e->kx, e->ky, c.km, c.spec.p + e->full * fsz, started[e->full]);
-            started[e->full] = 1;
+        ExtractAllArgs a{};
+        a.nfull = P.nfull;
+        a.nsys = P.nsys;
+        a.NZ = P.NZ;
+        a.km = c.km;
+        a.spec = c.spec.p;
+        if (P.nfull > MAX_FULL) throw std::runtime_error("too many full variables");
+        int nt = 0;
+        for (int f = 0; f < P.nfull; ++f) {
+            a.full_first[f] = nt;
+            for (auto &e : c.ext) {
+                if (e->kind != 0 || e->full != f) continue;
+                if (nt >= MAX_EXTRACT_TERMS) throw std::runtime_error("too many full-variable terms");
+                SysDev &d = *c.ks[e->system];
+                a.term[nt++] = ExtractTerm{e->rowptr.p, e->col.p, e->val.p, kst[e->system], d.t->total, e->kx, e->ky};
+            }
         }
+        a.full_first[P.nfull] = nt;
+        a.nterms = nt;
+        if (P.nfull > 0) launch_extract_all(c.stream, a);
         for (auto &e : c.ext) {
             if (e->kind != 1) continue;
             launch_extract_mean(c.stream, zst[e->system], P.NZ, P.nsys, e->rowptr.p, e->col.p, e->val.p,
@@ -424,6 +510,10 @@ void nl_terms(coral_amd_ctx &c, const std::vector<const double2 *> &kst, const s
     for (auto &d : c.zs) nl_assemble(c, *d, d->Khat[stage]->p);
 }
 
+// one_step_beyond (PL_IMEX_timestepping_marching.f90:3-48): stage i takes the
+// NL terms of the previous stage solution, rhs_i = M x^n + sum_{j<i} dt a_ij L x_j
+// + sum_{j<=i} dt ah_{i+1,j} K_hat_j, and solves (M - dt a_ii L) x_i = rhs_i;
+// x^{n+1} = x_s.
 void one_step(coral_amd_ctx &c) {
     const Problem &P = c.P;
     const int s = P.nstages;
@@ -432,40 +522,39 @@ void one_step(coral_amd_ctx &c) {
     auto Ah = [&](int i, int j) { return P.a_hat[i * (s + 1) + j]; };
     for (int i = 0; i < s; ++i) {
         std::vector<const double2 *> kst, zst;
-        for (auto &d : c.ks) kst.push_back(i == 0 ? d->field.p : d->aux.p);
-        for (auto &d : c.zs) zst.push_back(i == 0 ? d->field.p : d->aux.p);
+        for (auto &d : c.ks) kst.push_back(i == 0 ? d->field.p : d->X[i - 1]->p);
+        for (auto &d : c.zs) zst.push_back(i == 0 ? d->field.p : d->X[i - 1]->p);
         nl_terms(c, kst, zst, i);
         auto linear = [&](SysDev &d) {
-            std::vector<std::pair<const double2 *, double>> vecs;
-            for (int j = 0; j < i; ++j) {
-                vecs.push_back({d.Kstd[j]->p, dt * A(i, j)});
-                vecs.push_back({d.Khat[j]->p, dt * Ah(i + 1, j)});
-            }
-            vecs.push_back({d.Khat[i]->p, dt * Ah(i + 1, i)});
-            band_apply(c, d, true, d.field.p, d.aux.p, vecs);
+            std::vector<std::pair<const double2 *, double>> xs, ks;
+            for (int j = 0; j < i; ++j)
+                if (A(i, j) != 0.0) xs.push_back({d.X[j]->p, dt * A(i, j)});
+            for (int j = 0; j <= i; ++j)
+                if (Ah(i + 1, j) != 0.0) ks.push_back({d.Khat[j]->p, dt * Ah(i + 1, j)});
+            rhs(c, d, xs, ks);
             const SystemTables &t = *d.t;
+            double2 *out = d.X[i]->p;
             c.timed(d.kxky ? "solve" : "solve_mean", [&] {
                 const int kv = t.nl + t.nu, id0 = d.kxky ? 1 : 0;
                 if (d.ncol16 > 0 && d.real)
                     launch_solve16_real(c.stream, d.Lr.p, d.ncol16, d.Ur.p + SOLVE16_PAD, d.piv.p, d.rot.p, id0,
-                                        t.total, t.nl, kv, d.nsys, d.aux.p, d.aux.p);
+                                        t.total, t.nl, kv, d.nsys, d.aux.p, out);
                 else if (d.ncol16 > 0)
                     launch_solve16_complex(c.stream, d.Lf.p, d.ncol16, d.Uf.p + SOLVE16_PAD, d.piv.p, id0, t.total,
-                                           t.nl, kv, d.nsys, d.aux.p, d.aux.p);
+                                           t.nl, kv, d.nsys, d.aux.p, out);
                 else if (d.real)
                     launch_solve_real(c.stream, d.Lr.p, d.Ur.p + SOLVE16_PAD, d.piv.p, d.rot.p, id0, t.total, t.nl,
-                                      kv, d.nsys, d.aux.p, d.aux.p);
+                                      kv, d.nsys, d.aux.p, out);
                 else
                     launch_solve(c.stream, d.Lf.p, d.Uf.p + SOLVE16_PAD, d.piv.p, t.total, t.nl, kv, d.nsys, d.aux.p,
-                                 d.aux.p);
+                                 out);
             });
-            band_apply(c, d, false, d.aux.p, d.Kstd[i]->p, {});
         };
         for (auto &d : c.ks) linear(*d);
         for (auto &d : c.zs) linear(*d);
     }
-    for (auto &d : c.ks) std::swap(d->field.p, d->aux.p);
-    for (auto &d : c.zs) std::swap(d->field.p, d->aux.p);
+    for (auto &d : c.ks) std::swap(d->field.p, d->X[s - 1]->p);
+    for (auto &d : c.zs) std::swap(d->field.p, d->X[s - 1]->p);
 }
 
 void ensure_phys(coral_amd_ctx &c) {
@@ -541,7 +630,7 @@ int coral_amd_create(const coral_amd_problem *prob, int device, coral_amd_ctx **
             d.t = &t;
             d.kxky = true;
             d.nsys = P.nsys;
-            fill_system_device(d, P.nstages);
+            fill_system_device(d, P.nstages, c->km, c->stream, P.NZ);
         }
         for (auto &t : P.zero) {
             c->zs.emplace_back(new SysDev());
@@ -549,7 +638,7 @@ int coral_amd_create(const coral_amd_problem *prob, int device, coral_amd_ctx **
             d.t = &t;
             d.kxky = false;
             d.nsys = 1;
-            fill_system_device(d, P.nstages);
+            fill_system_device(d, P.nstages, c->km, c->stream, P.NZ);
         }
         c->full_has_kxky.assign(P.nfull, 0);
         for (auto &e : P.extract) {

@@ -70,6 +70,49 @@ void launch_extract(hipStream_t st, const double2 *state, int total, int nsys, i
     CHECK_LAUNCH();
 }
 
+__global__ __launch_bounds__(256) void k_extract_all(ExtractAllArgs a, int xt) {
+    extern __shared__ double2 tile[];  // [NZ][xt]
+    const int NZ = a.NZ, nsys = a.nsys;
+    const int s0 = blockIdx.x * xt;
+    const int ns = min(xt, nsys - s0);
+    for (int f = 0; f < a.nfull; ++f) {
+        const int t0 = a.full_first[f], t1 = a.full_first[f + 1];
+        for (int e = threadIdx.x; e < ns * NZ; e += blockDim.x) {
+            const int sl = e / NZ, n = e - sl * NZ, s = s0 + sl;
+            double2 acc = make_double2(0.0, 0.0);
+            for (int t = t0; t < t1; ++t) {
+                const ExtractTerm &T = a.term[t];
+                const double2 *st = T.state + (size_t)s * T.total;
+                double2 part = make_double2(0.0, 0.0);
+                for (int k = T.rowptr[n]; k < T.rowptr[n + 1]; ++k) {
+                    const double v = T.val[k];
+                    const double2 x = st[T.col[k]];
+                    part.x += v * x.x;
+                    part.y += v * x.y;
+                }
+                if (T.kx || T.ky) part = cmul(monomial(T.kx, T.ky, a.km.kx(s), a.km.ky(s)), part);
+                acc = cadd(acc, part);
+            }
+            tile[n * xt + sl] = acc;
+        }
+        __syncthreads();
+        double2 *out = a.spec + (size_t)f * NZ * nsys + s0;
+        for (int e = threadIdx.x; e < ns * NZ; e += blockDim.x) {
+            const int n = e / ns, sl = e - n * ns;
+            out[(size_t)n * nsys + sl] = tile[n * xt + sl];
+        }
+        __syncthreads();
+    }
+}
+
+void launch_extract_all(hipStream_t st, const ExtractAllArgs &a) {
+    int xt = 16;
+    while (xt > 1 && size_t(a.NZ) * xt * sizeof(double2) > 49152) xt /= 2;
+    const size_t shm = size_t(a.NZ) * xt * sizeof(double2);
+    hipLaunchKernelGGL(k_extract_all, dim3(blocks_for(a.nsys, xt)), dim3(256), shm, st, a, xt);
+    CHECK_LAUNCH();
+}
+
 // mean contributions land in the (0,0) column: spec[n][0]
 __global__ void k_extract_mean(const double2 *__restrict__ zstate, int NZ, int nsys, const int *__restrict__ rowptr,
                                const int *__restrict__ col, const double *__restrict__ val, double2 *spec) {
@@ -625,6 +668,62 @@ __global__ __launch_bounds__(256) void k_nl(NLArgs a) {
     }
 }
 
+__global__ __launch_bounds__(256) void k_nl_prog(NLProgArgs a) {
+    __shared__ double2 tile[64][17];
+    const int nsys = a.nsys, total = a.total;
+    const int ntp = (total + 63) / 64;
+    const int s0 = (blockIdx.x / ntp) * 16, p0 = (blockIdx.x % ntp) * 64;
+    const int sl = threadIdx.x & 15, pr = threadIdx.x >> 4;
+    const int s = s0 + sl;
+    const bool sv = s < nsys;
+    const double2 *Q = a.Qhat + (sv ? s : 0);
+    const double2 *cm = a.cmono + (size_t)(sv ? s : 0) * a.nterms;
+    for (int i = 0; i < 4; ++i) {
+        const int pl = pr + 16 * i, pos = p0 + pl;
+        double2 acc = make_double2(0.0, 0.0);
+        if (pos < total && sv && s != 0) {
+            for (int b = a.pos_ptr[pos]; b < a.pos_ptr[pos + 1]; ++b) {
+                double2 sum = make_double2(0.0, 0.0);
+                for (int e = a.blk_ptr[b]; e < a.blk_ptr[b + 1]; ++e) {
+                    const double w = a.ent_w[e];
+                    const double2 q = Q[(size_t)a.ent_qm[e] * nsys];
+                    sum.x += w * q.x;
+                    sum.y += w * q.y;
+                }
+                acc = cadd(acc, cmul(cm[a.blk_term[b]], sum));
+            }
+        }
+        tile[pl][sl] = acc;
+    }
+    __syncthreads();
+    for (int i = 0; i < 4; ++i) {
+        const int e = threadIdx.x + 256 * i, sl2 = e >> 6, pl = e & 63;
+        const int s2 = s0 + sl2, pos = p0 + pl;
+        if (s2 < nsys && pos < total) a.Khat[(size_t)s2 * total + pos] = tile[pl][sl2];
+    }
+}
+
+void launch_nl_prog(hipStream_t st, const NLProgArgs &a) {
+    const int ntp = (a.total + 63) / 64;
+    hipLaunchKernelGGL(k_nl_prog, dim3(blocks_for(a.nsys, 16) * ntp), dim3(256), 0, st, a);
+    CHECK_LAUNCH();
+}
+
+__global__ void k_term_monomials(const int *t_kx, const int *t_ky, int nterms, KeptModes km, int nsys, double2 *out) {
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long long)nsys * nterms) return;
+    const int s = int(idx / nterms), t = int(idx % nterms);
+    out[idx] = monomial(t_kx[t], t_ky[t], km.kx(s), km.ky(s));
+}
+
+void launch_term_monomials(hipStream_t st, const int *t_kx, const int *t_ky, int nterms, KeptModes km, int nsys,
+                           double2 *cmono) {
+    if (nterms == 0) return;
+    hipLaunchKernelGGL(k_term_monomials, dim3(blocks_for((long long)nsys * nterms, 256)), dim3(256), 0, st, t_kx,
+                       t_ky, nterms, km, nsys, cmono);
+    CHECK_LAUNCH();
+}
+
 // mean systems: real part of the (0,0) column
 __global__ void k_nl_mean(NLArgs a) {
     const int pos = blockIdx.x * blockDim.x + threadIdx.x;
@@ -657,39 +756,72 @@ void launch_nl(hipStream_t st, const NLArgs &a) {
 }
 
 // ---------------------------------------------------------------------------
-// banded operator application + AXPYs (mass_field_to_rhs, add_K_*_to_rhs,
-// stif_aux_to_K_std: PL_IMEX_timestepping_marching.f90:93-171)
+// right-hand sides (mass_field_to_rhs, add_K_std_to_rhs, add_K_hat_to_rhs,
+// PL_IMEX_timestepping_marching.f90:93-171): one thread per (system, row),
+// operators as ELL rows of the merged groups
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_band_apply(BandApplyArgs a) {
-    long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(256) void k_rhs(RhsArgs a) {
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= (long long)a.nsys * a.total) return;
-    const int s = int(idx / a.total), i = int(idx % a.total);
-    const double2 *xs = a.x + (size_t)s * a.total;
-    const double kx = a.km.kx(s), ky = a.km.ky(s);
+    const int s = int(idx / a.total), i = int(idx - (long long)s * a.total);
+    const size_t base = (size_t)s * a.total;
     double2 y = make_double2(0.0, 0.0);
-    for (int g = 0; g < a.op.ngroups; ++g) {
-        const int W = a.op.ell_w[g];
-        const int base = a.op.ell_off[g] + i * W;
+    const double2 *x0 = a.x0 + base;
+    for (int g = 0; g < a.mass.ngroups; ++g) {
+        const int W = a.mass.ell_w[g], off = a.mass.ell_off[g] + i * W;
         double2 sum = make_double2(0.0, 0.0);
         for (int e = 0; e < W; ++e) {
-            const double b = a.op.ell_val[base + e];
-            const double2 xv = xs[a.op.ell_col[base + e]];
-            sum.x += b * xv.x;
-            sum.y += b * xv.y;
+            const double bv = a.mass.ell_val[off + e];
+            const double2 xv = x0[a.mass.ell_col[off + e]];
+            sum.x += bv * xv.x;
+            sum.y += bv * xv.y;
         }
-        y = cadd(y, cmul(group_coef(a.op, g, kx, ky), sum));
+        y = cadd(y, cmul(a.cmass[(size_t)s * a.mass.ngroups + g], sum));
     }
-    for (int v = 0; v < a.nvec; ++v) {
-        double2 t = a.vec[v][idx];
-        y.x += a.alpha[v] * t.x;
-        y.y += a.alpha[v] * t.y;
+    if (a.nx > 0) {
+        for (int g = 0; g < a.stif.ngroups; ++g) {
+            const int W = a.stif.ell_w[g], off = a.stif.ell_off[g] + i * W;
+            double2 sum = make_double2(0.0, 0.0);
+            for (int e = 0; e < W; ++e) {
+                const double bv = a.stif.ell_val[off + e];
+                const int col = a.stif.ell_col[off + e];
+                double2 v = make_double2(0.0, 0.0);
+                for (int k = 0; k < a.nx; ++k) {
+                    const double2 t = a.xv[k][base + col];
+                    v.x += a.xa[k] * t.x;
+                    v.y += a.xa[k] * t.y;
+                }
+                sum.x += bv * v.x;
+                sum.y += bv * v.y;
+            }
+            y = cadd(y, cmul(a.cstif[(size_t)s * a.stif.ngroups + g], sum));
+        }
+    }
+    for (int k = 0; k < a.nk; ++k) {
+        const double2 t = a.kv[k][idx];
+        y.x += a.kb[k] * t.x;
+        y.y += a.kb[k] * t.y;
     }
     if (a.zero00 && s == 0) y = make_double2(0.0, 0.0);
     a.y[idx] = y;
 }
 
-void launch_band_apply(hipStream_t st, const BandApplyArgs &a) {
-    hipLaunchKernelGGL(k_band_apply, dim3(blocks_for((long long)a.nsys * a.total, 256)), dim3(256), 0, st, a);
+void launch_rhs(hipStream_t st, const RhsArgs &a) {
+    hipLaunchKernelGGL(k_rhs, dim3(blocks_for((long long)a.nsys * a.total, 256)), dim3(256), 0, st, a);
+    CHECK_LAUNCH();
+}
+
+__global__ void k_group_coefs(OpDev op, KeptModes km, int nsys, double2 *out) {
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long long)nsys * op.ngroups) return;
+    const int s = int(idx / op.ngroups), g = int(idx % op.ngroups);
+    out[idx] = group_coef(op, g, km.kx(s), km.ky(s));
+}
+
+void launch_group_coefs(hipStream_t st, const OpDev &op, KeptModes km, int nsys, double2 *out) {
+    if (op.ngroups == 0) return;
+    hipLaunchKernelGGL(k_group_coefs, dim3(blocks_for((long long)nsys * op.ngroups, 256)), dim3(256), 0, st, op, km,
+                       nsys, out);
     CHECK_LAUNCH();
 }
 
@@ -1149,9 +1281,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 4))) void
 // padding around the array so that out-of-record reads stay in bounds (their
 // values are masked).
 // ---------------------------------------------------------------------------
+// row_newbcast writes every lane of the row, so no old value is needed
 template <int T_>
 __device__ inline int bcast16_i(int v) {
-    return __builtin_amdgcn_update_dpp(0, v, 0x150 + T_, 0xF, 0xF, false);
+    return __builtin_amdgcn_mov_dpp(v, 0x150 + T_, 0xF, 0xF, false);
 }
 template <int T_>
 __device__ inline double bcast16_d(double v) {

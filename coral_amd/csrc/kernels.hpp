@@ -13,6 +13,25 @@ namespace coral {
 // ---- extraction: spec[n][s] (+)= mono(s) * sum val[k] * state[s][col[k]] ----
 void launch_extract(hipStream_t st, const double2 *state, int total, int nsys, int NZ, const int *rowptr,
                     const int *col, const double *val, int kx, int ky, KeptModes km, double2 *spec, int accumulate);
+// all kxky terms of all full variables in one pass over the state: a
+// workgroup owns xt systems and produces every full variable for them
+struct ExtractTerm {
+    const int *rowptr, *col;
+    const double *val;
+    const double2 *state;
+    int total, kx, ky;
+};
+constexpr int MAX_EXTRACT_TERMS = 24;
+constexpr int MAX_FULL = 16;
+struct ExtractAllArgs {
+    int nterms, nfull;
+    ExtractTerm term[MAX_EXTRACT_TERMS];    // sorted by full variable
+    int full_first[MAX_FULL + 1];           // terms of full f: [full_first[f], full_first[f+1])
+    int nsys, NZ;
+    KeptModes km;
+    double2 *spec;                          // [nfull][NZ][nsys]
+};
+void launch_extract_all(hipStream_t st, const ExtractAllArgs &a);
 // mean contributions into the (0,0) column: spec[n][0] += sum val[k] * Re(zstate[col[k]])
 void launch_extract_mean(hipStream_t st, const double2 *zstate, int NZ, int nsys, const int *rowptr, const int *col,
                          const double *val, double2 *spec);
@@ -97,6 +116,24 @@ struct NLArgs {
 };
 void launch_nl(hipStream_t st, const NLArgs &a);
 
+// kxky NL assembly from a per-position program: position p runs term blocks
+// [pos_ptr[p], pos_ptr[p+1]); block b adds cmono[s][blk_term[b]] * sum over
+// entries [blk_ptr[b], blk_ptr[b+1]) of ent_w * Qhat[ent_qm][s]
+// (ent_qm = q*NZ + m, ent_w = dsca * trunc(I^c M^d)[row][m]).
+struct NLProgArgs {
+    const int *pos_ptr, *blk_term, *blk_ptr, *ent_qm;
+    const double *ent_w;
+    const double2 *cmono;  // [nsys][nterms]
+    int nterms;
+    const double2 *Qhat;   // [nquad][NZ][nsys]
+    int nsys, total;
+    double2 *Khat;         // [nsys][total]
+};
+void launch_nl_prog(hipStream_t st, const NLProgArgs &a);
+// cmono[s*nterms + t] = (i kx)^kx_t (i ky)^ky_t
+void launch_term_monomials(hipStream_t st, const int *t_kx, const int *t_ky, int nterms, KeptModes km, int nsys,
+                           double2 *cmono);
+
 // ---- operators: groups of real matrices times sums of horizontal monomials ----
 struct OpDev {
     int ngroups;
@@ -112,20 +149,28 @@ __device__ inline double2 group_coef(const OpDev &op, int g, double kx, double k
     return c;
 }
 
-// ---- operator application with AXPYs ----
-// y[s][i] = sum_g c_g(s) (B_g x[s])[i] + sum_j alpha_j V_j[s][i]; zero00: y = 0 at s = 0
-struct BandApplyArgs {
-    OpDev op;
+// ---- right-hand sides ----
+// y[s][i] = sum_g cM_g(s) (M_g x0[s])[i] + sum_g cL_g(s) (L_g v[s])[i] + sum_k kb_k K_k[s][i],
+// v = sum_k xa_k X_k  (mass_field_to_rhs + add_K_std_to_rhs + add_K_hat_to_rhs,
+// PL_IMEX_timestepping_marching.f90:93-171, with L applied to the combination of
+// stage solutions instead of storing K_std = L x for every stage); zero00: y = 0 at s = 0
+struct RhsArgs {
+    OpDev mass, stif;
+    const double2 *cmass, *cstif;  // [nsys][ngroups] group coefficients
     int total, nsys;
-    const double2 *x;
-    int nvec;
-    const double2 *vec[8];
-    double alpha[8];
+    const double2 *x0;
+    int nx;
+    const double2 *xv[8];
+    double xa[8];
+    int nk;
+    const double2 *kv[8];
+    double kb[8];
     int zero00;
-    KeptModes km;
     double2 *y;
 };
-void launch_band_apply(hipStream_t st, const BandApplyArgs &a);
+void launch_rhs(hipStream_t st, const RhsArgs &a);
+// out[s*ngroups + g] = sum over the group's monomials of (i kx)^a (i ky)^b
+void launch_group_coefs(hipStream_t st, const OpDev &op, KeptModes km, int nsys, double2 *out);
 
 // ---- banded LU factorization and solve ----
 struct FactorArgs {

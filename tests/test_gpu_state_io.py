@@ -135,3 +135,46 @@ def test_horizontal_averages_match_oracle(solver_mod):
         assert np.max(np.abs(got - ref)) <= 1e-12 * max(1.0, np.max(np.abs(ref)))
     for got, ref in zip(quad, quad_ref):
         assert np.max(np.abs(got - ref)) <= 1e-12 * max(1.0, np.max(np.abs(ref)))
+
+
+class _OracleAsSolver:
+    """The oracle behind the solver's time-stepping interface."""
+
+    def __init__(self, ora):
+        self.ora = ora
+
+    def factorize_operators(self, dt):
+        self.ora.factorize(dt)
+
+    def march_forward(self, dt, n):
+        for _ in range(n):
+            self.ora.step(dt)
+
+    def cfl_dt(self):
+        return self.ora.cfl_dt
+
+
+def test_adaptive_time_step_matches_oracle(solver_mod):
+    """CFL-driven dt with tolerance-triggered refactorization (timeKeeping.f90),
+    the same rules driving the GPU solver and the oracle."""
+    from coral_amd.timekeeping import Timings, march_adaptive
+    r = product_recipe("coral.equations.rrbc_ra5e7_ek1e-4")
+    NX, NY, NZ = 32, 32, 17
+    k0, z0 = ic.rbc_initial_state(r, NX, NY, NZ, 1e-2, 10, 8, 9)
+    geo = O.Geometry(NX, NY, NZ, 2 * np.pi, 2 * np.pi)
+    ora = O.OracleSolver(r, geo, "ARS_222")
+    ora.field, ora.zfield = [a.copy() for a in k0], [a.copy() for a in z0]
+    t_ora = Timings(dt_max=4e-5, tolerance_factor=0.3, CFL_C=0.1)
+    used_ora = march_adaptive(_OracleAsSolver(ora), t_ora, 24)
+    with solver_mod.Solver(r, NX, NY, NZ, 2 * np.pi, 2 * np.pi) as g:
+        g.set_state(k0, z0)
+        t_gpu = Timings(dt_max=4e-5, tolerance_factor=0.3, CFL_C=0.1)
+        used_gpu = march_adaptive(g, t_gpu, 24)
+        k, z = g.get_state()
+    # the same steps change dt (CFL maxima agree to round-off, far from the 30 % threshold)
+    change = lambda u: [i for i in range(1, len(u)) if u[i] != u[i - 1]]
+    assert change(used_gpu) == change(used_ora)
+    assert np.allclose(used_gpu, used_ora, rtol=1e-10, atol=0.0)
+    assert len(change(used_gpu)) >= 2, "the run should refactorize at least twice"
+    assert rel(k[0], ora.field[0]) <= 1e-10
+    assert abs(t_gpu.absolute_time / t_ora.absolute_time - 1.0) <= 1e-10

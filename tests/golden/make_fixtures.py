@@ -23,6 +23,7 @@ INPUTS = {
     "rrbc_julien.coral.equations.velocityPressure": "etc/benchmarks/02_RRBC_julien_JFM96/coral.equations.velocityPressure",
     "rrbc_julien.coral.parameters.in": "etc/benchmarks/02_RRBC_julien_JFM96/coral.parameters.in",
     "rrbc_julien.coral.timeseries": "etc/benchmarks/02_RRBC_julien_JFM96/coral.timeseries",
+    "rrbc_julien.coral.equations.toroidalPoloidal": "etc/benchmarks/02_RRBC_julien_JFM96/coral.equations.toroidalPoloidal",
     "rbc_vanilla.coral.equations": "etc/coral_equations_examples/coral.equations.RBC_primitiveVars_Vanilla",
     "example.coral.parameters.in": "etc/coral_equations_examples/coral.parameters.in",
 }

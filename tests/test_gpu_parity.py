@@ -367,3 +367,37 @@ def test_production_size_properties(solver_mod):
         if ky[iy] < 0 and kept[0, iy]:
             jy = list(ky).index(-ky[iy])
             assert np.max(np.abs(plane[iy] - np.conj(plane[jy]))) <= 1e-12 * np.max(np.abs(plane))
+
+
+@pytest.mark.parametrize("NX,NY,NZ,nsteps", [(16, 16, 17, 3), (32, 24, 33, 2)])
+def test_toroidal_poloidal_recipe_parity(solver_mod, NX, NY, NZ, nsteps):
+    """The reference's toroidal-poloidal RRBC recipe
+    (etc/benchmarks/02_RRBC_julien_JFM96/coral.equations.toroidalPoloidal): BC 40,
+    fourth-order equations, full variables built with dz and dx dy factors."""
+    r = recipe("rrbc_julien.coral.equations.toroidalPoloidal")
+    k0 = None
+    for seed, var in enumerate(("psi", "phi", "theta")):
+        kv, z0 = ic.rbc_initial_state(r, NX, NY, NZ, 1e-3, 6, 6, 20 + seed, var_name=var)
+        k0 = kv if k0 is None else [a + b for a, b in zip(k0, kv)]
+    rng = np.random.default_rng(NZ)
+    z0 = [rng.normal(size=a.shape) * 1e-3 for a in z0]
+    geo = O.Geometry(NX, NY, NZ, 2.0, 2.0)
+    ora = O.OracleSolver(r, geo, "ARS_222")
+    ora.field, ora.zfield = [a.copy() for a in k0], [a.copy() for a in z0]
+    dt = 1e-4
+    g = solver_mod.Solver(r, NX, NY, NZ, 2.0, 2.0, scheme="ARS_222")
+    try:
+        g.set_state(k0, z0)
+        for it in range(nsteps):
+            ora.step(dt)
+            g.march_forward(dt)
+        k, z = g.get_state()
+        assert np.linalg.norm(ora.field[0]) > 0
+        assert rel(k[0], ora.field[0]) <= TOL
+        for a, b in zip(z, ora.zfield):
+            assert rel(a, b) <= TOL
+        phys = ora.physical_full_variables()
+        for f in range(len(phys)):
+            assert rel(g.physical(f), phys[f]) <= TOL
+    finally:
+        g.close()

@@ -59,7 +59,7 @@ struct SysDev {
     bool kxky = true;
     int nsys = 0;
     DevBuf<double2> field, aux;
-    std::vector<std::unique_ptr<DevBuf<double2>>> Khat, X;   // per stage: K_hat and the stage solution
+    std::vector<std::unique_ptr<DevBuf<double2>>> Khat, X, R;   // per stage: K_hat, solution, right-hand side
     DevBuf<double2> cmass, cstif;                              // [nsys][ngroups] group coefficients
     DevBuf<double2> Lf, Uf;      // complex factors
     bool real = false;           // factored in real form (Lr, Ur, rot)
@@ -205,8 +205,10 @@ void fill_system_device(SysDev &d, int nstages, const KeptModes &km, hipStream_t
     for (int i = 0; i < nstages; ++i) {
         d.Khat.emplace_back(new DevBuf<double2>());
         d.X.emplace_back(new DevBuf<double2>());
+        d.R.emplace_back(new DevBuf<double2>());
         d.Khat.back()->alloc(n);
         d.X.back()->alloc(n);
+        d.R.back()->alloc(n);
     }
     auto up = [](SysDev::OpBufs &b, const OpTables &o) {
         b.ngroups = o.ngroups;
@@ -368,9 +370,10 @@ void factorize(coral_amd_ctx &c, SysDev &d, double dt) {
     (void)kv;
 }
 
-// rhs = M field + L (sum xa X) + sum kb K_hat
-void rhs(coral_amd_ctx &c, SysDev &d, const std::vector<std::pair<const double2 *, double>> &xs,
-         const std::vector<std::pair<const double2 *, double>> &ks) {
+// y = M (sum ma U) + L (sum xa X) + sum kb K
+void rhs(coral_amd_ctx &c, SysDev &d, const std::vector<std::pair<const double2 *, double>> &ms,
+         const std::vector<std::pair<const double2 *, double>> &xs,
+         const std::vector<std::pair<const double2 *, double>> &ks, double2 *y) {
     SystemTables &t = *d.t;
     RhsArgs a{};
     a.mass = d.mass.dev();
@@ -379,8 +382,12 @@ void rhs(coral_amd_ctx &c, SysDev &d, const std::vector<std::pair<const double2 
     a.cstif = d.cstif.p;
     a.total = t.total;
     a.nsys = d.nsys;
-    a.x0 = d.field.p;
-    if (xs.size() > 8 || ks.size() > 8) throw std::runtime_error("too many IMEX stages");
+    if (ms.size() > 8 || xs.size() > 8 || ks.size() > 8) throw std::runtime_error("too many IMEX stages");
+    a.nm = int(ms.size());
+    for (size_t k = 0; k < ms.size(); ++k) {
+        a.mv[k] = ms[k].first;
+        a.ma[k] = ms[k].second;
+    }
     a.nx = int(xs.size());
     for (size_t k = 0; k < xs.size(); ++k) {
         a.xv[k] = xs[k].first;
@@ -392,7 +399,7 @@ void rhs(coral_amd_ctx &c, SysDev &d, const std::vector<std::pair<const double2 
         a.kb[k] = ks[k].second;
     }
     a.zero00 = d.kxky ? 1 : 0;
-    a.y = d.aux.p;
+    a.y = y;
     c.timed(d.kxky ? "rhs" : "rhs_mean", [&] { launch_rhs(c.stream, a); });
 }
 
@@ -526,27 +533,38 @@ void one_step(coral_amd_ctx &c) {
         for (auto &d : c.zs) zst.push_back(i == 0 ? d->field.p : d->X[i - 1]->p);
         nl_terms(c, kst, zst, i);
         auto linear = [&](SysDev &d) {
-            std::vector<std::pair<const double2 *, double>> xs, ks;
-            for (int j = 0; j < i; ++j)
-                if (A(i, j) != 0.0) xs.push_back({d.X[j]->p, dt * A(i, j)});
+            // M x^n + sum_j dt a_ij L x_j + sum_j dt ah_{i+1,j} K_hat_j, with
+            // dt L x_j = (M x_j - rhs_j) / a_jj where a_jj != 0
+            std::vector<std::pair<const double2 *, double>> ms{{d.field.p, 1.0}}, xs, ks;
+            for (int j = 0; j < i; ++j) {
+                if (A(i, j) == 0.0) continue;
+                if (A(j, j) != 0.0) {
+                    const double cij = A(i, j) / A(j, j);
+                    ms.push_back({d.X[j]->p, cij});
+                    ks.push_back({d.R[j]->p, -cij});
+                } else {
+                    xs.push_back({d.X[j]->p, dt * A(i, j)});
+                }
+            }
             for (int j = 0; j <= i; ++j)
                 if (Ah(i + 1, j) != 0.0) ks.push_back({d.Khat[j]->p, dt * Ah(i + 1, j)});
-            rhs(c, d, xs, ks);
+            double2 *rhs_i = d.R[i]->p;
+            rhs(c, d, ms, xs, ks, rhs_i);
             const SystemTables &t = *d.t;
             double2 *out = d.X[i]->p;
             c.timed(d.kxky ? "solve" : "solve_mean", [&] {
                 const int kv = t.nl + t.nu, id0 = d.kxky ? 1 : 0;
                 if (d.ncol16 > 0 && d.real)
                     launch_solve16_real(c.stream, d.Lr.p, d.ncol16, d.Ur.p + SOLVE16_PAD, d.piv.p, d.rot.p, id0,
-                                        t.total, t.nl, kv, d.nsys, d.aux.p, out);
+                                        t.total, t.nl, kv, d.nsys, rhs_i, out);
                 else if (d.ncol16 > 0)
                     launch_solve16_complex(c.stream, d.Lf.p, d.ncol16, d.Uf.p + SOLVE16_PAD, d.piv.p, id0, t.total,
-                                           t.nl, kv, d.nsys, d.aux.p, out);
+                                           t.nl, kv, d.nsys, rhs_i, out);
                 else if (d.real)
                     launch_solve_real(c.stream, d.Lr.p, d.Ur.p + SOLVE16_PAD, d.piv.p, d.rot.p, id0, t.total, t.nl,
-                                      kv, d.nsys, d.aux.p, out);
+                                      kv, d.nsys, rhs_i, out);
                 else
-                    launch_solve(c.stream, d.Lf.p, d.Uf.p + SOLVE16_PAD, d.piv.p, t.total, t.nl, kv, d.nsys, d.aux.p,
+                    launch_solve(c.stream, d.Lf.p, d.Uf.p + SOLVE16_PAD, d.piv.p, t.total, t.nl, kv, d.nsys, rhs_i,
                                  out);
             });
         };

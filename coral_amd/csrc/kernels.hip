@@ -766,15 +766,20 @@ __global__ __launch_bounds__(256) void k_rhs(RhsArgs a) {
     const int s = int(idx / a.total), i = int(idx - (long long)s * a.total);
     const size_t base = (size_t)s * a.total;
     double2 y = make_double2(0.0, 0.0);
-    const double2 *x0 = a.x0 + base;
     for (int g = 0; g < a.mass.ngroups; ++g) {
         const int W = a.mass.ell_w[g], off = a.mass.ell_off[g] + i * W;
         double2 sum = make_double2(0.0, 0.0);
         for (int e = 0; e < W; ++e) {
             const double bv = a.mass.ell_val[off + e];
-            const double2 xv = x0[a.mass.ell_col[off + e]];
-            sum.x += bv * xv.x;
-            sum.y += bv * xv.y;
+            const int col = a.mass.ell_col[off + e];
+            double2 u = make_double2(0.0, 0.0);
+            for (int k = 0; k < a.nm; ++k) {
+                const double2 t = a.mv[k][base + col];
+                u.x += a.ma[k] * t.x;
+                u.y += a.ma[k] * t.y;
+            }
+            sum.x += bv * u.x;
+            sum.y += bv * u.y;
         }
         y = cadd(y, cmul(a.cmass[(size_t)s * a.mass.ngroups + g], sum));
     }

@@ -150,15 +150,19 @@ __device__ inline double2 group_coef(const OpDev &op, int g, double kx, double k
 }
 
 // ---- right-hand sides ----
-// y[s][i] = sum_g cM_g(s) (M_g x0[s])[i] + sum_g cL_g(s) (L_g v[s])[i] + sum_k kb_k K_k[s][i],
-// v = sum_k xa_k X_k  (mass_field_to_rhs + add_K_std_to_rhs + add_K_hat_to_rhs,
-// PL_IMEX_timestepping_marching.f90:93-171, with L applied to the combination of
-// stage solutions instead of storing K_std = L x for every stage); zero00: y = 0 at s = 0
+// y[s][i] = sum_g cM_g(s) (M_g u[s])[i] + sum_g cL_g(s) (L_g v[s])[i] + sum_k kb_k K_k[s][i],
+// u = sum_k ma_k U_k, v = sum_k xa_k X_k  (mass_field_to_rhs + add_K_std_to_rhs +
+// add_K_hat_to_rhs, PL_IMEX_timestepping_marching.f90:93-171).  With a
+// diagonally implicit scheme, dt L x_j = (M x_j - rhs_j) / a_jj, so the stage
+// right-hand sides need M only: rhs_i = M (x^n + sum_j c_ij x_j) - sum_j c_ij rhs_j
+// + sum_j dt ah_{i+1,j} K_hat_j with c_ij = a_ij / a_jj.  zero00: y = 0 at s = 0.
 struct RhsArgs {
     OpDev mass, stif;
     const double2 *cmass, *cstif;  // [nsys][ngroups] group coefficients
     int total, nsys;
-    const double2 *x0;
+    int nm;
+    const double2 *mv[8];
+    double ma[8];
     int nx;
     const double2 *xv[8];
     double xa[8];

@@ -70,46 +70,53 @@ void launch_extract(hipStream_t st, const double2 *state, int total, int nsys, i
     CHECK_LAUNCH();
 }
 
-__global__ __launch_bounds__(256) void k_extract_all(ExtractAllArgs a, int xt) {
-    extern __shared__ double2 tile[];  // [NZ][xt]
-    const int NZ = a.NZ, nsys = a.nsys;
+// One pass over each system's state computes every full variable: for a tile
+// of xt systems the workgroup walks the systems one after another (all threads
+// on one system's CSR rows, so its state is read from HBM once and then hits
+// in cache), keeps the results in LDS as [f][n][sys] and writes rows of xt
+// consecutive systems.
+__global__ __launch_bounds__(512) void k_extract_all(ExtractAllArgs a, int xt) {
+    extern __shared__ double2 tile[];  // [nfull][NZ][xt]
+    const int NZ = a.NZ, nsys = a.nsys, nf = a.nfull;
     const int s0 = blockIdx.x * xt;
     const int ns = min(xt, nsys - s0);
-    for (int f = 0; f < a.nfull; ++f) {
-        const int t0 = a.full_first[f], t1 = a.full_first[f + 1];
-        for (int e = threadIdx.x; e < ns * NZ; e += blockDim.x) {
-            const int sl = e / NZ, n = e - sl * NZ, s = s0 + sl;
-            double2 acc = make_double2(0.0, 0.0);
-            for (int t = t0; t < t1; ++t) {
-                const ExtractTerm &T = a.term[t];
-                const double2 *st = T.state + (size_t)s * T.total;
-                double2 part = make_double2(0.0, 0.0);
-                for (int k = T.rowptr[n]; k < T.rowptr[n + 1]; ++k) {
-                    const double v = T.val[k];
-                    const double2 x = st[T.col[k]];
-                    part.x += v * x.x;
-                    part.y += v * x.y;
-                }
-                if (T.kx || T.ky) part = cmul(monomial(T.kx, T.ky, a.km.kx(s), a.km.ky(s)), part);
-                acc = cadd(acc, part);
+    const int per = nf * NZ;  // outputs per system
+#pragma unroll 2
+    for (int e = threadIdx.x; e < ns * per; e += blockDim.x) {
+        const int sl = e / per, r = e - sl * per, f = r / NZ, n = r - f * NZ, s = s0 + sl;
+        double2 acc = make_double2(0.0, 0.0);
+        for (int t = a.full_first[f]; t < a.full_first[f + 1]; ++t) {
+            const ExtractTerm &T = a.term[t];
+            const double2 *st = T.state + (size_t)s * T.total;
+            double2 part = make_double2(0.0, 0.0);
+            for (int k = T.rowptr[n]; k < T.rowptr[n + 1]; ++k) {
+                const double v = T.val[k];
+                const double2 x = st[T.col[k]];
+                part.x += v * x.x;
+                part.y += v * x.y;
             }
-            tile[n * xt + sl] = acc;
+            if (T.kx || T.ky) part = cmul(monomial(T.kx, T.ky, a.km.kx(s), a.km.ky(s)), part);
+            acc = cadd(acc, part);
         }
-        __syncthreads();
-        double2 *out = a.spec + (size_t)f * NZ * nsys + s0;
-        for (int e = threadIdx.x; e < ns * NZ; e += blockDim.x) {
-            const int n = e / ns, sl = e - n * ns;
-            out[(size_t)n * nsys + sl] = tile[n * xt + sl];
-        }
-        __syncthreads();
+        tile[(size_t)(f * NZ + n) * xt + sl] = acc;
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < nf * NZ * ns; e += blockDim.x) {
+        const int row = e / ns, sl = e - row * ns;  // row = f * NZ + n
+        a.spec[(size_t)row * nsys + s0 + sl] = tile[(size_t)row * xt + sl];
     }
 }
 
 void launch_extract_all(hipStream_t st, const ExtractAllArgs &a) {
-    int xt = 16;
-    while (xt > 1 && size_t(a.NZ) * xt * sizeof(double2) > 49152) xt /= 2;
-    const size_t shm = size_t(a.NZ) * xt * sizeof(double2);
-    hipLaunchKernelGGL(k_extract_all, dim3(blocks_for(a.nsys, xt)), dim3(256), shm, st, a, xt);
+    int lim = 65536;
+    int dev = 0;
+    if (hipGetDevice(&dev) == hipSuccess) (void)hipDeviceGetAttribute(&lim, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
+    // 8 systems (128-byte output rows) leave room for two workgroups per CU
+    int xt = 8;
+    while (xt > 1 && size_t(a.nfull) * a.NZ * xt * sizeof(double2) > size_t(lim) / 2) xt /= 2;
+    const size_t shm = size_t(a.nfull) * a.NZ * xt * sizeof(double2);
+    (void)hipFuncSetAttribute((const void *)k_extract_all, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
+    hipLaunchKernelGGL(k_extract_all, dim3(blocks_for(a.nsys, xt)), dim3(512), shm, st, a, xt);
     CHECK_LAUNCH();
 }
 
@@ -1340,13 +1347,15 @@ struct Solve16 {
         if constexpr (t < 15) fwd_block<t + 1>(cur, nxt, pv, ring, Lcol_next, l, rowbase);
     }
 
-    // backward step t (descending) of block blk
+    // backward step t (descending) of block blk.  Lane t's q = 0 entry of the
+    // ring is record offset 0 of column j, i.e. 1/U(j, j); it is broadcast to
+    // the row instead of being loaded separately.
     template <int t>
-    __device__ static inline void bwd_step(double2 (&r)[NQ], T (&dring)[PFB], T (&cring)[PFB][NQ], const T *Ul,
-                                           const T *Ud, int l, int kv, int jnext) {
+    __device__ static inline void bwd_step(double2 (&r)[NQ], T (&cring)[PFB][NQ], const T *Ul, int l, int kv,
+                                           int jnext) {
         const T zero = Fx<T>::zero();
         constexpr int slot = t % PFB;
-        const double2 xj = scale(bcast16_c<t>(r[0]), dring[slot]);
+        const double2 xj = scale(bcast16_c<t>(r[0]), bcast16<t>(cring[slot][0]));
         r[0] = csel(l == t, xj, r[0]);
 #pragma unroll
         for (int q = 0; q < NQ; ++q) {
@@ -1357,16 +1366,15 @@ struct Solve16 {
         // prefetch step jnext = j - PFB (its position in the block: (t - PFB) mod 16)
         constexpr int tn = (t - PFB) & 15;
         const size_t off = (size_t)((long long)jnext * (kv + 1));
-        dring[slot] = ld_nt(Ud + off);
 #pragma unroll
         for (int q = 0; q < NQ; ++q) cring[slot][q] = ld_nt(Ul + off + tn + 16 * q);
     }
 
     template <int t>
-    __device__ static inline void bwd_block(double2 (&r)[NQ], T (&dring)[PFB], T (&cring)[PFB][NQ], const T *Ul,
-                                            const T *Ud, int l, int kv, int j0) {
-        bwd_step<t>(r, dring, cring, Ul, Ud, l, kv, j0 + t - PFB);
-        if constexpr (t > 0) bwd_block<t - 1>(r, dring, cring, Ul, Ud, l, kv, j0);
+    __device__ static inline void bwd_block(double2 (&r)[NQ], T (&cring)[PFB][NQ], const T *Ul, int l, int kv,
+                                            int j0) {
+        bwd_step<t>(r, cring, Ul, l, kv, j0 + t - PFB);
+        if constexpr (t > 0) bwd_block<t - 1>(r, cring, Ul, l, kv, j0);
     }
 };
 
@@ -1415,19 +1423,18 @@ __global__ __launch_bounds__(64) void k_solve16(const T *__restrict__ L16, int n
         const int bq = B - 1 - q;
         r[q] = ld_masked(y, bq * 16 + l, bq >= 0 && bq * 16 + l < n);
     }
-    T dring[PFB], cring[PFB][NQ];
+    T cring[PFB][NQ];
 #pragma unroll
     for (int u = 0; u < PFB; ++u) {
         // step j = 16B - 1 - u lives in slot (15 - u) % PFB
         const int j = B * 16 - 1 - u;
         const int t = j & 15;
         const size_t off = (size_t)((long long)j * (kv + 1));
-        dring[(15 - u) % PFB] = ld_nt(Ud + off);
 #pragma unroll
         for (int q = 0; q < NQ; ++q) cring[(15 - u) % PFB][q] = ld_nt(Ul + off + t + 16 * q);
     }
     for (int blk = B - 1; blk >= 0; --blk) {
-        S::template bwd_block<15>(r, dring, cring, Ul, Ud, l, kv, blk * 16);
+        S::template bwd_block<15>(r, cring, Ul, l, kv, blk * 16);
         const int i0 = blk * 16 + l;
         if (live && i0 < n) y[i0] = rot_out(r[0], code(i0));
 #pragma unroll
@@ -1445,9 +1452,9 @@ static bool launch_solve16(hipStream_t st, const T *L16, int ncol16, const T *Ur
     const int nq = 1 + (kv + 15) / 16;
     const dim3 grid((nsys + 3) / 4);
     switch (nq) {
-    case 2: hipLaunchKernelGGL((k_solve16<T, 2, 4>), grid, dim3(64), 0, st, L16, ncol16, Ur, piv, rot, ident0, n, kv, nsys, rhs, x); break;
-    case 3: hipLaunchKernelGGL((k_solve16<T, 3, 4>), grid, dim3(64), 0, st, L16, ncol16, Ur, piv, rot, ident0, n, kv, nsys, rhs, x); break;
-    case 4: hipLaunchKernelGGL((k_solve16<T, 4, 4>), grid, dim3(64), 0, st, L16, ncol16, Ur, piv, rot, ident0, n, kv, nsys, rhs, x); break;
+    case 2: hipLaunchKernelGGL((k_solve16<T, 2, 8>), grid, dim3(64), 0, st, L16, ncol16, Ur, piv, rot, ident0, n, kv, nsys, rhs, x); break;
+    case 3: hipLaunchKernelGGL((k_solve16<T, 3, 8>), grid, dim3(64), 0, st, L16, ncol16, Ur, piv, rot, ident0, n, kv, nsys, rhs, x); break;
+    case 4: hipLaunchKernelGGL((k_solve16<T, 4, 8>), grid, dim3(64), 0, st, L16, ncol16, Ur, piv, rot, ident0, n, kv, nsys, rhs, x); break;
     default: return false;
     }
     CHECK_LAUNCH();

@@ -309,17 +309,23 @@ void fill_system_device(SysDev &d, int nstages, const KeptModes &km, hipStream_t
     d.real = t.real_form && !(force && force[0] == '1');
     // 16-lane solve layout when the band allows it; U arrays padded (zeroed) on
     // both sides for its out-of-record reads
+    // (records of four systems interleaved, in groups of four systems and
+    // blocks of 16 columns; the slots past the last system stay zero)
     d.ncol16 = solve16_ncol(t.total, t.nl, kv);
-    const size_t lsize = d.ncol16 > 0 ? size_t(d.nsys) * d.ncol16 * 16 : size_t(d.nsys) * t.total * std::max(t.nl, 1);
-    const size_t usize = size_t(d.nsys) * t.total * (kv + 1) + 2 * size_t(SOLVE16_PAD);
+    const size_t nsys4 = (size_t(d.nsys) + 3) / 4 * 4, nb = 16 * ((size_t(t.total) + 15) / 16);
+    const size_t lsize = d.ncol16 > 0 ? nsys4 * d.ncol16 * 16 : size_t(d.nsys) * t.total * std::max(t.nl, 1);
+    const size_t usize =
+        (d.ncol16 > 0 ? nsys4 * nb : size_t(d.nsys) * t.total) * (kv + 1) + 2 * size_t(SOLVE16_PAD);
     if (d.real) {
         d.Lr.alloc(lsize);
         d.Ur.alloc(usize);
+        HIP_OK(hipMemset(d.Lr.p, 0, lsize * sizeof(double)));
         HIP_OK(hipMemset(d.Ur.p, 0, usize * sizeof(double)));
         d.rot.upload(t.rot);
     } else {
         d.Lf.alloc(lsize);
         d.Uf.alloc(usize);
+        HIP_OK(hipMemset(d.Lf.p, 0, lsize * sizeof(double2)));
         HIP_OK(hipMemset(d.Uf.p, 0, usize * sizeof(double2)));
     }
     d.piv.alloc(size_t(d.nsys) * t.total);
@@ -554,18 +560,20 @@ void one_step(coral_amd_ctx &c) {
             double2 *out = d.X[i]->p;
             c.timed(d.kxky ? "solve" : "solve_mean", [&] {
                 const int kv = t.nl + t.nu, id0 = d.kxky ? 1 : 0;
+                bool done = true;
                 if (d.ncol16 > 0 && d.real)
-                    launch_solve16_real(c.stream, d.Lr.p, d.ncol16, d.Ur.p + SOLVE16_PAD, d.piv.p, d.rot.p, id0,
-                                        t.total, t.nl, kv, d.nsys, rhs_i, out);
+                    done = launch_solve16_real(c.stream, d.Lr.p, d.ncol16, d.Ur.p + SOLVE16_PAD, d.piv.p, d.rot.p,
+                                               id0, t.total, t.nl, kv, d.nsys, rhs_i, out);
                 else if (d.ncol16 > 0)
-                    launch_solve16_complex(c.stream, d.Lf.p, d.ncol16, d.Uf.p + SOLVE16_PAD, d.piv.p, id0, t.total,
-                                           t.nl, kv, d.nsys, rhs_i, out);
+                    done = launch_solve16_complex(c.stream, d.Lf.p, d.ncol16, d.Uf.p + SOLVE16_PAD, d.piv.p, id0,
+                                                  t.total, t.nl, kv, d.nsys, rhs_i, out);
                 else if (d.real)
                     launch_solve_real(c.stream, d.Lr.p, d.Ur.p + SOLVE16_PAD, d.piv.p, d.rot.p, id0, t.total, t.nl,
                                       kv, d.nsys, rhs_i, out);
                 else
                     launch_solve(c.stream, d.Lf.p, d.Uf.p + SOLVE16_PAD, d.piv.p, t.total, t.nl, kv, d.nsys, rhs_i,
                                  out);
+                if (!done) throw std::runtime_error("16-lane solve rejected a band it was factored for");
             });
         };
         for (auto &d : c.ks) linear(*d);

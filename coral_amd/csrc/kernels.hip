@@ -944,19 +944,26 @@ __device__ void split_factors(const T *AB, int n, int kl, int ku, int ldab, T *L
     }
 }
 
-// Factor records for the 16-lane solve: the multipliers of column j are
-// stored rotated so that lane l (position 16b + l of the block it updates)
-// reads entry l: L16[j][l] = L(j + 1 + k, j), k = (l - j%16 - 1) mod 16, zero
-// for k >= min(kl, n-1-j) and for the padding columns j >= n.  U as in
-// split_factors.
+// Factor records for the 16-lane solve.  The four systems one wavefront
+// solves are interleaved column by column, so that the wavefront's loads of
+// one step fall on adjacent records.  For system s (group g = s/4, slot
+// m = s%4):
+//   L16: column j at ((g*ncol16 + j)*4 + m)*16; its multipliers are stored
+//        rotated so that lane l (position 16b + l of the block it updates)
+//        reads entry l: L16[l] = L(j + 1 + k, j), k = (l - j%16 - 1) mod 16,
+//        zero for k >= min(kl, n-1-j) and for the padding columns j >= n;
+//   U:   column j at ((g*nb + j)*4 + m)*(kv+1), nb = 16*ceil(n/16) columns per
+//        system (those beyond n stay zero): entry 0 is 1/U(j,j), entry k the
+//        superdiagonal U(j-k, j).
 template <typename T>
-__device__ void split_factors16(const T *AB, int n, int kl, int ku, int ldab, int ncol16, T *L16, T *Uf) {
+__device__ void split_factors16(const T *AB, int n, int kl, int ku, int ldab, int ncol16, int s, T *L16, T *Ur) {
     using F = Fx<T>;
-    const int kv = kl + ku;
+    const int kv = kl + ku, nb = 16 * ((n + 15) >> 4), g = s >> 2, m = s & 3;
     for (int e = threadIdx.x; e < ncol16 * 16; e += blockDim.x) {
         const int j = e >> 4, l = e & 15;
         const int k = (l - (j & 15) - 1) & 15;
-        L16[e] = (j < n && k < min(kl, n - 1 - j)) ? AB[(size_t)j * ldab + kv + 1 + k] : F::zero();
+        L16[(((size_t)g * ncol16 + j) * 4 + m) * 16 + l] =
+            (j < n && k < min(kl, n - 1 - j)) ? AB[(size_t)j * ldab + kv + 1 + k] : F::zero();
     }
     for (int e = threadIdx.x; e < n * (kv + 1); e += blockDim.x) {
         int j = e / (kv + 1), k = e % (kv + 1);
@@ -965,7 +972,7 @@ __device__ void split_factors16(const T *AB, int n, int kl, int ku, int ldab, in
             v = F::recip(AB[(size_t)j * ldab + kv]);
         else
             v = (j - k >= 0) ? AB[(size_t)j * ldab + kv - k] : F::zero();
-        Uf[e] = v;
+        Ur[(((size_t)g * nb + j) * 4 + m) * (kv + 1) + k] = v;
     }
 }
 
@@ -1016,8 +1023,7 @@ __global__ __launch_bounds__(64) void k_factor(FactorArgs a) {
     T *Lout = REAL ? reinterpret_cast<T *>(a.Lr) : reinterpret_cast<T *>(a.Lf);
     T *Uout = REAL ? reinterpret_cast<T *>(a.Ur) : reinterpret_cast<T *>(a.Uf);
     if (a.ncol16 > 0)
-        split_factors16<T>(AB, n, kl, ku, ldab, a.ncol16, Lout + (size_t)s * a.ncol16 * 16,
-                           Uout + (size_t)s * n * (kv + 1));
+        split_factors16<T>(AB, n, kl, ku, ldab, a.ncol16, s, Lout, Uout);
     else
         split_factors<T>(AB, n, kl, ku, ldab, Lout + (size_t)s * n * kl, Uout + (size_t)s * n * (kv + 1));
 }
@@ -1289,9 +1295,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 4))) void
 // exactly the 16 lanes (lanes > t in block b, lanes <= t in block b+1), so
 // every lane does useful work.  Backward (xTBSV, column oriented): x_j is
 // broadcast and NQ registers (blocks b .. b-NQ+1) take the column update.
-// Factor records: L16 (see split_factors16), U as in split_factors with
-// padding around the array so that out-of-record reads stay in bounds (their
-// values are masked).
+// Factor records: the interleaved layout of split_factors16, with padding
+// around the U array so that out-of-record reads stay in bounds (their values
+// are masked).
 // ---------------------------------------------------------------------------
 // row_newbcast writes every lane of the row, so no old value is needed
 template <int T_>
@@ -1337,7 +1343,7 @@ struct Solve16 {
         const T c = ring[t];
         cur = submul(cur, tsel(l > t, c, zero), bj);
         nxt = submul(nxt, tsel(l > t, zero, c), bj);
-        ring[t] = ld_nt(Lcol_next + t * 16);
+        ring[t] = ld_nt(Lcol_next + t * 64);
     }
 
     template <int t>
@@ -1365,7 +1371,7 @@ struct Solve16 {
         }
         // prefetch step jnext = j - PFB (its position in the block: (t - PFB) mod 16)
         constexpr int tn = (t - PFB) & 15;
-        const size_t off = (size_t)((long long)jnext * (kv + 1));
+        const long long off = (long long)jnext * 4 * (kv + 1);
 #pragma unroll
         for (int q = 0; q < NQ; ++q) cring[slot][q] = ld_nt(Ul + off + tn + 16 * q);
     }
@@ -1389,7 +1395,8 @@ __global__ __launch_bounds__(64) void k_solve16(const T *__restrict__ L16, int n
     const bool live = sg < nsys;
     const int s = live ? sg : nsys - 1;
     const bool dorot = rot && !(ident0 && s == 0);
-    const T *Lp = L16 + (size_t)s * ncol16 * 16 + l;
+    const int g = s >> 2, m = s & 3;
+    const T *Lp = L16 + ((size_t)g * ncol16 * 4 + m) * 16 + l;   // + j*64: column j
     const unsigned char *P = piv + (size_t)s * n;
     const double2 *b = rhs + (size_t)s * n;
     double2 *y = x + (size_t)s * n;
@@ -1403,9 +1410,9 @@ __global__ __launch_bounds__(64) void k_solve16(const T *__restrict__ L16, int n
     int pv = load_piv(l);
     T ring[16];
 #pragma unroll
-    for (int t = 0; t < 16; ++t) ring[t] = ld_nt(Lp + t * 16);
+    for (int t = 0; t < 16; ++t) ring[t] = ld_nt(Lp + t * 64);
     for (int blk = 0; blk < B; ++blk) {
-        S::template fwd_block<0>(cur, nxt, pv, ring, Lp + (size_t)(blk * 16 + 16) * 16, l, rowbase);
+        S::template fwd_block<0>(cur, nxt, pv, ring, Lp + (size_t)(blk * 16 + 16) * 64, l, rowbase);
         const int i0 = blk * 16 + l;
         if (live && i0 < n) y[i0] = cur;
         cur = nxt;
@@ -1415,8 +1422,8 @@ __global__ __launch_bounds__(64) void k_solve16(const T *__restrict__ L16, int n
     __threadfence_block();
 
     // ---- backward ----
-    const T *Ud = Ur + (size_t)s * n * (kv + 1);   // + j*(kv+1): 1/U(j,j)
-    const T *Ul = Ud - l;                          // + j*(kv+1) + t + 16q: U(j-1-k, j), k = t-1-l+16q
+    const T *Ud = Ur + ((size_t)g * B * 64 + m) * (kv + 1);   // + 4j(kv+1): 1/U(j,j)
+    const T *Ul = Ud - l;   // + 4j(kv+1) + t + 16q: U(j-1-k, j), k = t-1-l+16q
     double2 r[NQ];
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
@@ -1429,7 +1436,7 @@ __global__ __launch_bounds__(64) void k_solve16(const T *__restrict__ L16, int n
         // step j = 16B - 1 - u lives in slot (15 - u) % PFB
         const int j = B * 16 - 1 - u;
         const int t = j & 15;
-        const size_t off = (size_t)((long long)j * (kv + 1));
+        const long long off = (long long)j * 4 * (kv + 1);
 #pragma unroll
         for (int q = 0; q < NQ; ++q) cring[(15 - u) % PFB][q] = ld_nt(Ul + off + t + 16 * q);
     }
@@ -1449,6 +1456,9 @@ static bool launch_solve16(hipStream_t st, const T *L16, int ncol16, const T *Ur
                            const unsigned char *rot, int ident0, int n, int kl, int kv, int nsys, const double2 *rhs,
                            double2 *x) {
     if (kl > 16 || n < 32 || ncol16 < 16 * ((n + 15) / 16 + 1)) return false;
+    // the backward prefetch reaches 8 columns (of 4 interleaved records) before
+    // the first record
+    if (8 * 4 * (kv + 1) + 16 > SOLVE16_PAD) return false;
     const int nq = 1 + (kv + 15) / 16;
     const dim3 grid((nsys + 3) / 4);
     switch (nq) {

@@ -205,8 +205,9 @@ void launch_solve(hipStream_t st, const double2 *Lf, const double2 *Uf, const un
 // 16-lane solve (four systems per wavefront): columns per system of its
 // multiplier records, or 0 when the band does not fit (kl > 16, kv > 47)
 int solve16_ncol(int n, int kl, int kv);
-// U arrays of the 16-lane solve carry this many elements of padding on each side
-constexpr int SOLVE16_PAD = 1024;
+// U arrays of the 16-lane solve carry this many elements of padding on each
+// side (its backward prefetch reads up to 8*4*(kv+1) + 15 before the first record)
+constexpr int SOLVE16_PAD = 2048;
 bool launch_solve16_real(hipStream_t st, const double *L16, int ncol16, const double *Ur, const unsigned char *piv,
                          const unsigned char *rot, int ident0, int n, int kl, int kv, int nsys, const double2 *rhs,
                          double2 *x);

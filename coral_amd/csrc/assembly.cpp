@@ -236,6 +236,77 @@ static void find_real_form(const coral_amd_system &s, SystemTables &t) {
         t.rot[p] = (unsigned char)(val[t.pos_eqn[p]] | (val[nv + var_at[p]] << 1));
 }
 
+// Structural blocks (see SystemTables::nblk): union-find over rows (0..n-1)
+// and columns (n..2n-1) joined by every nonzero band entry of every group.
+static void find_blocks(SystemTables &t) {
+    const int n = t.total, w = t.band_width();
+    std::vector<int> parent(2 * n);
+    for (int i = 0; i < 2 * n; ++i) parent[i] = i;
+    auto find = [&](int x) {
+        while (parent[x] != x) x = parent[x] = parent[parent[x]];
+        return x;
+    };
+    std::vector<std::vector<int>> nzcols(n);   // structure, row -> columns
+    for (const OpTables *op : {&t.mass, &t.stif})
+        for (int g = 0; g < op->ngroups; ++g)
+            for (int i = 0; i < n; ++i)
+                for (int d = -t.nl; d <= t.nu; ++d) {
+                    const int j = i + d;
+                    if (j < 0 || j >= n || op->band[(size_t(g) * n + i) * w + d + t.nl] == 0.0) continue;
+                    nzcols[i].push_back(j);
+                    parent[find(i)] = find(n + j);
+                }
+    t.nblk = 1;
+    t.nblk_n = n;
+    t.blk_nl = t.nl;
+    t.blk_nu = t.nu;
+    t.blk_row.clear();
+    t.blk_col.clear();
+    std::map<int, int> id;   // root -> block
+    std::vector<std::vector<int>> rows, cols;
+    for (int x = 0; x < 2 * n; ++x) {
+        const int r = find(x);
+        auto it = id.find(r);
+        if (it == id.end()) {
+            it = id.emplace(r, int(rows.size())).first;
+            rows.emplace_back();
+            cols.emplace_back();
+        }
+        (x < n ? rows : cols)[it->second].push_back(x < n ? x : x - n);
+    }
+    if (rows.size() < 2) return;
+    for (size_t b = 0; b < rows.size(); ++b)
+        if (rows[b].size() != cols[b].size() || rows[b].empty()) return;   // structurally singular: no split
+    const int nb = int(rows.size());
+    int m = 0;
+    for (const auto &r : rows) m = std::max(m, int(r.size()));
+    std::vector<int> sub_of_col(n), blk_of_col(n);
+    for (int b = 0; b < nb; ++b)
+        for (size_t k = 0; k < cols[b].size(); ++k) {
+            sub_of_col[cols[b][k]] = int(k);
+            blk_of_col[cols[b][k]] = b;
+        }
+    int bl = 0, bu = 0;
+    for (int b = 0; b < nb; ++b)
+        for (size_t k = 0; k < rows[b].size(); ++k)
+            for (int j : nzcols[rows[b][k]]) {
+                if (blk_of_col[j] != b) throw std::runtime_error("inconsistent block structure");
+                bl = std::max(bl, int(k) - sub_of_col[j]);
+                bu = std::max(bu, sub_of_col[j] - int(k));
+            }
+    t.nblk = nb;
+    t.nblk_n = m;
+    t.blk_nl = bl;
+    t.blk_nu = bu;
+    t.blk_row.assign(size_t(nb) * m, -1);
+    t.blk_col.assign(size_t(nb) * m, -1);
+    for (int b = 0; b < nb; ++b)
+        for (size_t k = 0; k < rows[b].size(); ++k) {
+            t.blk_row[size_t(b) * m + k] = rows[b][k];
+            t.blk_col[size_t(b) * m + k] = cols[b][k];
+        }
+}
+
 static SystemTables build_system(const coral_amd_system &s, int NZ, double center, double gap, bool kxky,
                                  std::map<std::pair<int, int>, Dense> &im_cache, int nquad) {
     SystemTables t;
@@ -431,6 +502,7 @@ static SystemTables build_system(const coral_amd_system &s, int NZ, double cente
     }
     for (int e = 0; e < s.nvars; ++e) t.nl_eqn_begin[e + 1] += t.nl_eqn_begin[e];
     find_real_form(s, t);
+    find_blocks(t);
     return t;
 }
 

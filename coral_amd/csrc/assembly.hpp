@@ -72,6 +72,16 @@ struct SystemTables {
     // at position p << 1.
     bool real_form = false;
     std::vector<unsigned char> rot;
+    // Structural blocks of the evolution operator M - c L: the bipartite graph
+    // (row i, column j) of entries that are nonzero in some group of M or L
+    // splits into nblk connected components of equal row and column counts, so
+    // that after permutation the operator is block diagonal for every (kx, ky)
+    // and c (for Coral's operators, even and odd Chebyshev degrees).  Block b
+    // holds rows blk_row[b][0..] and columns blk_col[b][0..] in increasing
+    // order, padded to nblk_n entries with -1; inside a block the rows and
+    // columns form a band (blk_nl, blk_nu).  nblk = 1 when there is no split.
+    int nblk = 1, nblk_n = 0, blk_nl = 0, blk_nu = 0;
+    std::vector<int> blk_row, blk_col;   // [nblk][nblk_n]
     int band_width() const { return nl + nu + 1; }
 };
 

@@ -64,6 +64,11 @@ struct SysDev {
     DevBuf<double2> Lf, Uf;      // complex factors
     bool real = false;           // factored in real form (Lr, Ur, rot)
     int ncol16 = 0;              // > 0: 16-lane solve layout (see solve16_ncol)
+    // factored matrices per system: structural blocks of the 16-lane layout
+    // (SystemTables::nblk; 1 = the whole system)
+    int nblk = 1, blk_n = 0, blk_nl = 0, blk_nu = 0;
+    DevBuf<int> blk_row, blk_col;
+    DevBuf<unsigned> words;      // position words of the 16-lane solve (solve16_words)
     DevBuf<double> Lr, Ur;
     DevBuf<unsigned char> rot;
     DevBuf<unsigned char> piv;
@@ -114,6 +119,7 @@ struct coral_amd_ctx {
     DevBuf<double2> twx, twy;
     DevBuf<int> qa, qb;
     DevBuf<unsigned long long> cflmax;
+    DevBuf<double2> sink;   // stores of padding positions in the 16-lane solve
     FFTPlan px, py;
     double dt = 0.0;
     bool factored = false;
@@ -309,13 +315,36 @@ void fill_system_device(SysDev &d, int nstages, const KeptModes &km, hipStream_t
     d.real = t.real_form && !(force && force[0] == '1');
     // 16-lane solve layout when the band allows it; U arrays padded (zeroed) on
     // both sides for its out-of-record reads
-    // (records of four systems interleaved, in groups of four systems and
-    // blocks of 16 columns; the slots past the last system stay zero)
-    d.ncol16 = solve16_ncol(t.total, t.nl, kv);
-    const size_t nsys4 = (size_t(d.nsys) + 3) / 4 * 4, nb = 16 * ((size_t(t.total) + 15) / 16);
-    const size_t lsize = d.ncol16 > 0 ? nsys4 * d.ncol16 * 16 : size_t(d.nsys) * t.total * std::max(t.nl, 1);
+    // (records of four matrices interleaved, in groups of four and blocks of 16
+    // columns; the slots past the last matrix stay zero).  The 16-lane layout
+    // factors the structural blocks of the operator separately when it splits
+    // (CORAL_AMD_NO_BLOCK_SPLIT=1 keeps whole systems).
+    const char *nosplit = std::getenv("CORAL_AMD_NO_BLOCK_SPLIT");
+    const bool split = t.nblk > 1 && !(nosplit && nosplit[0] == '1') &&
+                       solve16_ncol(t.nblk_n, t.blk_nl, t.blk_nl + t.blk_nu) > 0;
+    d.ncol16 = split ? solve16_ncol(t.nblk_n, t.blk_nl, t.blk_nl + t.blk_nu) : solve16_ncol(t.total, t.nl, kv);
+    if (split) {
+        d.nblk = t.nblk;
+        d.blk_n = t.nblk_n;
+        d.blk_nl = t.blk_nl;
+        d.blk_nu = t.blk_nu;
+        d.blk_row.upload(t.blk_row);
+        d.blk_col.upload(t.blk_col);
+    } else {
+        d.nblk = 1;
+        d.blk_n = t.total;
+        d.blk_nl = t.nl;
+        d.blk_nu = t.nu;
+    }
+    if (d.ncol16 > 0)
+        d.words.upload(solve16_words(t.total, d.nblk, d.blk_n, split ? t.blk_row.data() : nullptr,
+                                     split ? t.blk_col.data() : nullptr, d.real ? t.rot.data() : nullptr));
+    const int bkv = d.blk_nl + d.blk_nu;
+    const size_t nmat = size_t(d.nsys) * d.nblk, nmat4 = (nmat + 3) / 4 * 4;
+    const size_t nb = 16 * ((size_t(d.blk_n) + 15) / 16);
+    const size_t lsize = d.ncol16 > 0 ? nmat4 * d.ncol16 * 16 : size_t(d.nsys) * t.total * std::max(t.nl, 1);
     const size_t usize =
-        (d.ncol16 > 0 ? nsys4 * nb : size_t(d.nsys) * t.total) * (kv + 1) + 2 * size_t(SOLVE16_PAD);
+        (d.ncol16 > 0 ? nmat4 * nb * (bkv + 1) : size_t(d.nsys) * t.total * (kv + 1)) + 2 * size_t(SOLVE16_PAD);
     if (d.real) {
         d.Lr.alloc(lsize);
         d.Ur.alloc(usize);
@@ -328,22 +357,26 @@ void fill_system_device(SysDev &d, int nstages, const KeptModes &km, hipStream_t
         HIP_OK(hipMemset(d.Lf.p, 0, lsize * sizeof(double2)));
         HIP_OK(hipMemset(d.Uf.p, 0, usize * sizeof(double2)));
     }
-    d.piv.alloc(size_t(d.nsys) * t.total);
-    d.info.alloc(d.nsys);
+    // (16-lane layout: pivot records padded to whole blocks of 16, zero)
+    const size_t pstride = d.ncol16 > 0 ? nb : size_t(d.blk_n);
+    d.piv.alloc(nmat * pstride);
+    HIP_OK(hipMemset(d.piv.p, 0, nmat * pstride));
+    d.info.alloc(nmat);
 }
 
 void fill_zero(hipStream_t st, double2 *p, size_t n);
 
 void factorize(coral_amd_ctx &c, SysDev &d, double dt) {
     SystemTables &t = *d.t;
-    const int kv = t.nl + t.nu, ldab = 2 * t.nl + t.nu + 1;
-    const size_t per = size_t(t.total) * ldab;
+    // matrices factored: nblk structural blocks of blk_n unknowns per system
+    const int nmat = d.nsys * d.nblk, ldab = 2 * d.blk_nl + d.blk_nu + 1;
+    const size_t per = size_t(d.blk_n) * ldab;
     // bound the working band to ~1 GiB per launch
     int chunk = int(std::max<size_t>(1, (size_t(1) << 30) / (per * sizeof(double2))));
-    chunk = std::min(chunk, d.nsys);
+    chunk = std::min(chunk, nmat);
     DevBuf<double2> work;
     work.alloc(size_t(chunk) * per);
-    for (int s0 = 0; s0 < d.nsys; s0 += chunk) {
+    for (int s0 = 0; s0 < nmat; s0 += chunk) {
         FactorArgs a{};
         a.mass = d.mass.dev();
         a.stif = d.stif.dev();
@@ -352,7 +385,7 @@ void factorize(coral_amd_ctx &c, SysDev &d, double dt) {
         a.total = t.total;
         a.sdt = -dt * c.P.a_arr[0];
         a.s0 = s0;
-        a.ns = std::min(chunk, d.nsys - s0);
+        a.ns = std::min(chunk, nmat - s0);
         a.identity00 = d.kxky ? 1 : 0;
         a.km = d.kxky ? c.km : KeptModes{1, 1, 2, 0.0, 0.0};
         a.work = work.p;
@@ -362,18 +395,24 @@ void factorize(coral_amd_ctx &c, SysDev &d, double dt) {
         a.Lr = d.Lr.p;
         a.Ur = d.Ur.p ? d.Ur.p + SOLVE16_PAD : nullptr;
         a.ncol16 = d.ncol16;
+        a.nblk = d.nblk;
+        a.blk_n = d.blk_n;
+        a.blk_nl = d.blk_nl;
+        a.blk_nu = d.blk_nu;
+        a.blk_row = d.blk_row.p;
+        a.blk_col = d.blk_col.p;
         a.piv = d.piv.p;
         a.info = d.info.p;
         c.timed("factor", [&] { launch_factor(c.stream, a); });
     }
     HIP_OK(hipStreamSynchronize(c.stream));
-    std::vector<int> info(d.nsys);
-    HIP_OK(hipMemcpy(info.data(), d.info.p, d.nsys * sizeof(int), hipMemcpyDeviceToHost));
-    for (int s = 0; s < d.nsys; ++s)
-        if (info[s] != 0)
-            throw std::runtime_error("singular evolution operator (zgbtrf info " + std::to_string(info[s]) +
-                                     ") at wavenumber index " + std::to_string(s));
-    (void)kv;
+    std::vector<int> info(nmat);
+    HIP_OK(hipMemcpy(info.data(), d.info.p, nmat * sizeof(int), hipMemcpyDeviceToHost));
+    for (int m = 0; m < nmat; ++m)
+        if (info[m] != 0)
+            throw std::runtime_error("singular evolution operator (zgbtrf info " + std::to_string(info[m]) +
+                                     (d.nblk > 1 ? " in structural block " + std::to_string(m % d.nblk) : "") +
+                                     ") at wavenumber index " + std::to_string(m / d.nblk));
 }
 
 // y = M (sum ma U) + L (sum xa X) + sum kb K
@@ -561,13 +600,26 @@ void one_step(coral_amd_ctx &c) {
             c.timed(d.kxky ? "solve" : "solve_mean", [&] {
                 const int kv = t.nl + t.nu, id0 = d.kxky ? 1 : 0;
                 bool done = true;
-                if (d.ncol16 > 0 && d.real)
-                    done = launch_solve16_real(c.stream, d.Lr.p, d.ncol16, d.Ur.p + SOLVE16_PAD, d.piv.p, d.rot.p,
-                                               id0, t.total, t.nl, kv, d.nsys, rhs_i, out);
-                else if (d.ncol16 > 0)
-                    done = launch_solve16_complex(c.stream, d.Lf.p, d.ncol16, d.Uf.p + SOLVE16_PAD, d.piv.p, id0,
-                                                  t.total, t.nl, kv, d.nsys, rhs_i, out);
-                else if (d.real)
+                if (d.ncol16 > 0) {
+                    Solve16Args sa{};
+                    sa.L16 = d.real ? (const void *)d.Lr.p : (const void *)d.Lf.p;
+                    sa.U = d.real ? (const void *)(d.Ur.p + SOLVE16_PAD) : (const void *)(d.Uf.p + SOLVE16_PAD);
+                    sa.ncol16 = d.ncol16;
+                    sa.piv = d.piv.p;
+                    sa.words = d.words.p;
+                    sa.rotate = d.real ? 1 : 0;
+                    sa.ident0 = id0;
+                    sa.total = t.total;
+                    sa.n = d.blk_n;
+                    sa.kl = d.blk_nl;
+                    sa.kv = d.blk_nl + d.blk_nu;
+                    sa.nsys = d.nsys;
+                    sa.nblk = d.nblk;
+                    sa.rhs = rhs_i;
+                    sa.x = out;
+                    sa.sink = c.sink.p;
+                    done = d.real ? launch_solve16_real(c.stream, sa) : launch_solve16_complex(c.stream, sa);
+                } else if (d.real)
                     launch_solve_real(c.stream, d.Lr.p, d.Ur.p + SOLVE16_PAD, d.piv.p, d.rot.p, id0, t.total, t.nl,
                                       kv, d.nsys, rhs_i, out);
                 else
@@ -718,6 +770,7 @@ int coral_amd_create(const coral_amd_problem *prob, int device, coral_amd_ctx **
         c->qa.upload(qa);
         c->qb.upload(qb);
         c->cflmax.alloc(3);
+        c->sink.alloc(64);
         HIP_OK(hipMemset(c->cflmax.p, 0, 3 * sizeof(unsigned long long)));
         c->px = make_plan(P.NXAA, c->twx);
         c->py = make_plan(P.NYAA, c->twy);
@@ -917,6 +970,23 @@ int coral_amd_factor_kind(coral_amd_ctx *c, int32_t isys, int32_t *is_real) {
         if (!c || !is_real) throw std::runtime_error("null argument");
         if (isys < 0 || isys >= int(c->ks.size())) throw std::runtime_error("bad kxky system index");
         *is_real = c->ks[isys]->real ? 1 : 0;
+        return 0;
+    } catch (const std::exception &e) {
+        return fail(e);
+    }
+}
+
+int coral_amd_factor_blocks(coral_amd_ctx *c, int32_t kind, int32_t isys, int32_t *nblk, int32_t *blk_n,
+                            int32_t *blk_nl, int32_t *blk_nu) {
+    try {
+        if (!c || !nblk || !blk_n || !blk_nl || !blk_nu) throw std::runtime_error("null argument");
+        const auto &v = kind == 0 ? c->ks : c->zs;
+        if (kind < 0 || kind > 1 || isys < 0 || isys >= int(v.size())) throw std::runtime_error("bad system index");
+        const SysDev &d = *v[isys];
+        *nblk = d.nblk;
+        *blk_n = d.blk_n;
+        *blk_nl = d.blk_nl;
+        *blk_nu = d.blk_nu;
         return 0;
     } catch (const std::exception &e) {
         return fail(e);

@@ -982,50 +982,63 @@ __device__ inline double real_rot(double2 z, int e) { return e == 0 ? z.x : (e =
 template <typename T>
 __global__ __launch_bounds__(64) void k_factor(FactorArgs a) {
     constexpr bool REAL = sizeof(T) == sizeof(double);
-    const int sl = blockIdx.x, s = a.s0 + sl;
-    const int n = a.total, kl = a.nl, ku = a.nu, kv = kl + ku, ldab = 2 * kl + ku + 1, w = a.nl + a.nu + 1;
-    T *AB = reinterpret_cast<T *>(a.work) + (size_t)sl * n * ldab;
+    // one workgroup per factored matrix: a structural block (16-lane layout) or
+    // a whole system
+    const bool l16 = a.ncol16 > 0;
+    const int nb = l16 ? a.nblk : 1;
+    const int Sl = blockIdx.x, S = a.s0 + Sl, s = S / nb, blk = S - s * nb;
+    const int N = a.total, w = a.nl + a.nu + 1;
+    const int n = l16 ? a.blk_n : N, kl = l16 ? a.blk_nl : a.nl, ku = l16 ? a.blk_nu : a.nu;
+    const int kv = kl + ku, ldab = 2 * kl + ku + 1;
+    const int *rmap = l16 && a.blk_row ? a.blk_row + (size_t)blk * n : nullptr;
+    const int *cmap = l16 && a.blk_col ? a.blk_col + (size_t)blk * n : nullptr;
+    T *AB = reinterpret_cast<T *>(a.work) + (size_t)Sl * n * ldab;
     const bool ident = a.identity00 && s == 0;
     const double kx = a.km.kx(s), ky = a.km.ky(s);
     for (int e = threadIdx.x; e < n * ldab; e += blockDim.x) {
-        int j = e / ldab, rr = e - j * ldab;
-        int i = rr - kv + j;
+        const int j = e / ldab, rr = e - j * ldab;
+        const int i = rr - kv + j;
         double2 val = make_double2(0.0, 0.0);
+        int rot = 0;
         if (rr >= kl && i >= 0 && i < n) {
-            int d = j - i;
-            if (ident) {
+            const int I = rmap ? rmap[i] : i, J = cmap ? cmap[j] : j;
+            if (ident || I < 0 || J < 0) {
+                // identity: the (0,0) system and the padding of a block (the
+                // identity system stays unrotated; its solve skips the rotation)
                 val.x = (i == j) ? 1.0 : 0.0;
             } else {
-                for (int g = 0; g < a.mass.ngroups; ++g) {
-                    double b = a.mass.band[((size_t)g * n + i) * w + d + a.nl];
-                    if (b != 0.0) val = cadd(val, cscale(group_coef(a.mass, g, kx, ky), b));
+                const int d = J - I;
+                if (d >= -a.nl && d <= a.nu) {
+                    for (int g = 0; g < a.mass.ngroups; ++g) {
+                        double b = a.mass.band[((size_t)g * N + I) * w + d + a.nl];
+                        if (b != 0.0) val = cadd(val, cscale(group_coef(a.mass, g, kx, ky), b));
+                    }
+                    for (int g = 0; g < a.stif.ngroups; ++g) {
+                        double b = a.stif.band[((size_t)g * N + I) * w + d + a.nl];
+                        if (b != 0.0) val = cadd(val, cscale(group_coef(a.stif, g, kx, ky), b * a.sdt));
+                    }
                 }
-                for (int g = 0; g < a.stif.ngroups; ++g) {
-                    double b = a.stif.band[((size_t)g * n + i) * w + d + a.nl];
-                    if (b != 0.0) val = cadd(val, cscale(group_coef(a.stif, g, kx, ky), b * a.sdt));
-                }
+                if constexpr (REAL) rot = ((a.rot[J] >> 1) - (a.rot[I] & 1)) & 3;
             }
         }
-        if constexpr (REAL) {
-            // identity systems stay unrotated (their solve skips the rotation too)
-            const int rot = ident ? 0 : (((a.rot[min(max(j, 0), n - 1)] >> 1) - (a.rot[min(max(i, 0), n - 1)] & 1)) & 3);
+        if constexpr (REAL)
             AB[e] = real_rot(val, rot);
-        } else {
+        else
             AB[e] = val;
-        }
     }
     __syncthreads();
-    int *info = a.info + s;
+    int *info = a.info + S;
     if (threadIdx.x == 0) *info = 0;
     __syncthreads();
-    gbtf2_wave<T>(AB, n, kl, ku, ldab, a.piv + (size_t)s * n, info);
+    // (16-lane layout: pivot records padded to whole blocks of 16 columns)
+    gbtf2_wave<T>(AB, n, kl, ku, ldab, a.piv + (size_t)S * (l16 ? 16 * ((n + 15) >> 4) : n), info);
     __syncthreads();
     T *Lout = REAL ? reinterpret_cast<T *>(a.Lr) : reinterpret_cast<T *>(a.Lf);
     T *Uout = REAL ? reinterpret_cast<T *>(a.Ur) : reinterpret_cast<T *>(a.Uf);
-    if (a.ncol16 > 0)
-        split_factors16<T>(AB, n, kl, ku, ldab, a.ncol16, s, Lout, Uout);
+    if (l16)
+        split_factors16<T>(AB, n, kl, ku, ldab, a.ncol16, S, Lout, Uout);
     else
-        split_factors<T>(AB, n, kl, ku, ldab, Lout + (size_t)s * n * kl, Uout + (size_t)s * n * (kv + 1));
+        split_factors<T>(AB, n, kl, ku, ldab, Lout + (size_t)S * n * kl, Uout + (size_t)S * n * (kv + 1));
 }
 
 void launch_factor(hipStream_t st, const FactorArgs &a) {
@@ -1325,110 +1338,187 @@ __device__ inline double bperm_d(int addr, double v) {
 }
 __device__ inline double2 bperm_c(int addr, double2 v) { return make_double2(bperm_d(addr, v.x), bperm_d(addr, v.y)); }
 
+// Position words of a block (see Solve16Args::words): row of the system at
+// bits 0-14, column at 15-29 (SOLVE16_NONE: padding), rotation of the
+// right-hand side (psi) at bit 30, of the solution (phi) at bit 31.
+__device__ inline int w_row(unsigned w) { return int(w & 0x7FFFu); }
+__device__ inline int w_col(unsigned w) { return int((w >> 15) & 0x7FFFu); }
+
+// Loads: multipliers are read once and streamed (non-temporal); the U
+// windows of consecutive backward steps overlap in cache lines and use
+// ordinary loads, which keep those lines in the cache for the next step.
+//
+// Scheduling note for the 16-lane solve.  Every value a block starts with
+// (coefficients, right-hand sides, pivots, intermediate results) is loaded
+// one block or PFB steps ahead into a register that the same slot's previous
+// value occupied.  Each reload's address passes through an empty asm that
+// consumes the values the slot's old contents fed, so the load is issued
+// after the old value's last use and can take over its register; otherwise
+// the compiler hoists the load, gives it a fresh register and copies it into
+// place at the end of the loop, and that copy waits for every load issued
+// since.
 template <typename T, int NQ, int PFB>
 struct Solve16 {
-    // forward step t of a block: swap, eliminate, prefetch the next block's coefficient
+    // forward step t of a block: swap, eliminate, prefetch the next block's
+    // coefficient.  lo is the lane's position in its row, opaque to the
+    // compiler so that the lane masks are formed where they are used.
     template <int t>
-    __device__ static inline void fwd_step(double2 &cur, double2 &nxt, int pv, T (&ring)[16], const T *Lcol_next,
-                                           int l, int rowbase) {
+    __device__ static inline void fwd_step(double2 &cur, double2 &nxt, const uint4 &pv, T (&ring)[16], const T *Lg,
+                                           unsigned &lcol, int lo, int rowbase) {
         const T zero = Fx<T>::zero();
-        const int p = bcast16_i<t>(pv);
+        const unsigned pw = t < 4 ? pv.x : t < 8 ? pv.y : t < 12 ? pv.z : pv.w;
+        const int p = int((pw >> (8 * (t & 3))) & 0xffu);
         const int tp = t + p;
         const double2 w = csel(tp < 16, cur, nxt);
         const double2 bj = bperm_c((rowbase + (tp & 15)) << 2, w);
         const double2 a = bcast16_c<t>(cur);
-        cur = csel(l == tp, a, cur);
-        nxt = csel(l + 16 == tp, a, nxt);
-        cur = csel(l == t, bj, cur);
+        cur = csel(lo == tp, a, cur);
+        nxt = csel(lo + 16 == tp, a, nxt);
+        cur = csel(lo == t, bj, cur);
         const T c = ring[t];
-        cur = submul(cur, tsel(l > t, c, zero), bj);
-        nxt = submul(nxt, tsel(l > t, zero, c), bj);
-        ring[t] = ld_nt(Lcol_next + t * 64);
+        const bool below = lo > t;
+        cur = submul(cur, tsel(below, c, zero), bj);
+        nxt = submul(nxt, tsel(below, zero, c), bj);
+        asm volatile("" : "+v"(lcol) : "v"(cur.x), "v"(cur.y), "v"(nxt.x), "v"(nxt.y));
+        ring[t] = ld_nt(Lg + (size_t)lcol + t * 64);
+        __builtin_amdgcn_sched_barrier(0);   // and not later: the prefetch distance stays 16 steps
     }
 
+    // Lg: the wavefront's multiplier records (uniform); lcol: the lane's
+    // offset of the next block's first column
     template <int t>
-    __device__ static inline void fwd_block(double2 &cur, double2 &nxt, int pv, T (&ring)[16], const T *Lcol_next,
-                                            int l, int rowbase) {
-        fwd_step<t>(cur, nxt, pv, ring, Lcol_next, l, rowbase);
-        if constexpr (t < 15) fwd_block<t + 1>(cur, nxt, pv, ring, Lcol_next, l, rowbase);
+    __device__ static inline void fwd_block(double2 &cur, double2 &nxt, const uint4 &pv, T (&ring)[16], const T *Lg,
+                                            unsigned &lcol, int lo, int rowbase) {
+        fwd_step<t>(cur, nxt, pv, ring, Lg, lcol, lo, rowbase);
+        if constexpr (t < 15) fwd_block<t + 1>(cur, nxt, pv, ring, Lg, lcol, lo, rowbase);
     }
 
-    // backward step t (descending) of block blk.  Lane t's q = 0 entry of the
+    // backward step t (descending) of a block.  Lane t's q = 0 entry of the
     // ring is record offset 0 of column j, i.e. 1/U(j, j); it is broadcast to
-    // the row instead of being loaded separately.
+    // the row instead of being loaded separately.  x_j goes to xo; block 0 of
+    // r is updated without a lower mask, since its lanes >= t are finished
+    // (their values live in xo) and are not read again.
+    // Ug: the wavefront's U records less 16 elements (uniform); uofs: the lane's
+    // offset (m(kv+1) - lo + 16 >= 1) of its record of column 0 from Ug
     template <int t>
-    __device__ static inline void bwd_step(double2 (&r)[NQ], T (&cring)[PFB][NQ], const T *Ul, int l, int kv,
-                                           int jnext) {
+    __device__ static inline void bwd_step(double2 (&r)[NQ], double2 &xo, T (&cring)[PFB][NQ], const T *Ug,
+                                           unsigned uofs, int lo, int kv, int jnext) {
         const T zero = Fx<T>::zero();
         constexpr int slot = t % PFB;
         const double2 xj = scale(bcast16_c<t>(r[0]), bcast16<t>(cring[slot][0]));
-        r[0] = csel(l == t, xj, r[0]);
+        xo = csel(lo == t, xj, xo);
 #pragma unroll
         for (int q = 0; q < NQ; ++q) {
-            const int k = t - 1 - l + 16 * q;
-            const bool valid = (k >= 0) && (k < kv);
-            r[q] = submul(r[q], tsel(valid, cring[slot][q], zero), xj);
+            // entry k = t - 1 - lo + 16q of column j; rows with k >= kv are
+            // outside the band (k >= 0 holds for q >= 1)
+            if (q == 0 && NQ > 2) {
+                r[q] = submul(r[q], cring[slot][q], xj);
+            } else {
+                const bool valid = lo > t - 1 + 16 * q - kv;
+                r[q] = submul(r[q], tsel(valid, cring[slot][q], zero), xj);
+            }
         }
-        // prefetch step jnext = j - PFB (its position in the block: (t - PFB) mod 16)
+        // prefetch step jnext = j - PFB (its position in the block: (t - PFB) mod
+        // 16); steps before column 0 are never used and load column 0
         constexpr int tn = (t - PFB) & 15;
-        const long long off = (long long)jnext * 4 * (kv + 1);
+        unsigned o = uofs + unsigned(max(jnext, 0)) * unsigned(4 * (kv + 1));
 #pragma unroll
-        for (int q = 0; q < NQ; ++q) cring[slot][q] = ld_nt(Ul + off + tn + 16 * q);
+        for (int q = 0; q < NQ; ++q) asm volatile("" : "+v"(o) : "v"(r[q].x), "v"(r[q].y));
+        asm volatile("" : "+v"(o) : "v"(xo.x), "v"(xo.y));
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) cring[slot][q] = Ug[(size_t)o + tn + 16 * q];
+        __builtin_amdgcn_sched_barrier(0);   // and not later: the prefetch distance stays PFB steps
     }
 
     template <int t>
-    __device__ static inline void bwd_block(double2 (&r)[NQ], T (&cring)[PFB][NQ], const T *Ul, int l, int kv,
-                                            int j0) {
-        bwd_step<t>(r, cring, Ul, l, kv, j0 + t - PFB);
-        if constexpr (t > 0) bwd_block<t - 1>(r, cring, Ul, l, kv, j0);
+    __device__ static inline void bwd_block(double2 (&r)[NQ], double2 &xo, T (&cring)[PFB][NQ], const T *Ug,
+                                            unsigned uofs, int lo, int kv, int j0) {
+        bwd_step<t>(r, xo, cring, Ug, uofs, lo, kv, j0 + t - PFB);
+        if constexpr (t > 0) bwd_block<t - 1>(r, xo, cring, Ug, uofs, lo, kv, j0);
     }
 };
 
 template <typename T, int NQ, int PFB>
-__global__ __launch_bounds__(64) void k_solve16(const T *__restrict__ L16, int ncol16, const T *__restrict__ Ur,
-                                                const unsigned char *__restrict__ piv,
-                                                const unsigned char *__restrict__ rot, int ident0, int n, int kv,
-                                                int nsys, const double2 *__restrict__ rhs, double2 *__restrict__ x) {
+__global__ __launch_bounds__(64) void k_solve16(Solve16Args a) {
     using S = Solve16<T, NQ, PFB>;
+    extern __shared__ unsigned sW[];   // position words of every block
+    const T *__restrict__ L16 = reinterpret_cast<const T *>(a.L16);
+    const T *__restrict__ Ur = reinterpret_cast<const T *>(a.U);
+    const int n = a.n, kv = a.kv, ncol16 = a.ncol16;
     const int lane = threadIdx.x, l = lane & 15, rowbase = lane & 48;
+    const int wstride = n + 2 * SOLVE16_WPAD;
+    for (int e = lane; e < a.nblk * wstride; e += 64) sW[e] = a.words[e];
+    // factored matrix (structural block) of this 16-lane row
+    const int nmat = a.nsys * a.nblk;
     const int sg = blockIdx.x * 4 + (lane >> 4);
-    const bool live = sg < nsys;
-    const int s = live ? sg : nsys - 1;
-    const bool dorot = rot && !(ident0 && s == 0);
-    const int g = s >> 2, m = s & 3;
-    const T *Lp = L16 + ((size_t)g * ncol16 * 4 + m) * 16 + l;   // + j*64: column j
-    const unsigned char *P = piv + (size_t)s * n;
-    const double2 *b = rhs + (size_t)s * n;
-    double2 *y = x + (size_t)s * n;
+    const bool live = sg < nmat;
+    const int M = live ? sg : nmat - 1;
+    const int sys = M / a.nblk, blk = M - sys * a.nblk;
+    const bool dorot = a.rotate && !(a.ident0 && sys == 0);
+    const int g = M >> 2, m = M & 3;
+    const T *Lg = L16 + (size_t)g * ncol16 * 64;   // wavefront's records; lane's column j at m*16 + l + 64j
     const int B = (n + 15) >> 4;
-    auto code = [&](int i) -> int { return dorot ? (int)rot[min(i, n - 1)] : 0; };
-    auto load_rhs = [&](int i) -> double2 { return rot_in(ld_masked(b, i, i < n), code(i)); };
-    auto load_piv = [&](int i) -> int { const int v = P[min(i, n - 1)]; return i < n ? v : 0; };
+    // pivot offsets of block b: 16 bytes at P[b] (zero past n)
+    const uint4 *P = reinterpret_cast<const uint4 *>(a.piv + (size_t)M * B * 16);
+    const double2 *b = a.rhs + (size_t)sys * a.total;
+    double2 *y = a.x + (size_t)sys * a.total;
+    double2 *sink = a.sink + lane;   // stores of padding positions and idle lanes
+    __syncthreads();
+    const unsigned *W = sW + blk * wstride + SOLVE16_WPAD;   // W[i]: position i (padding outside [0, n))
+    auto code = [&](unsigned w) -> int { return dorot ? int(w >> 30) : 0; };
+    auto row_or0 = [&](unsigned w) -> int { const int r = w_row(w); return r != SOLVE16_NONE ? r : 0; };
+    auto col_or0 = [&](unsigned w) -> int { const int c = w_col(w); return c != SOLVE16_NONE ? c : 0; };
+    auto fix_rhs = [&](double2 v, unsigned w) -> double2 {
+        return rot_in(csel(w_row(w) != SOLVE16_NONE, v, make_double2(0.0, 0.0)), code(w));
+    };
+    auto fix_y = [&](double2 v, unsigned w) -> double2 {
+        return csel(w_col(w) != SOLVE16_NONE, v, make_double2(0.0, 0.0));
+    };
+    auto out_ptr = [&](unsigned w) -> double2 * {
+        const int c = w_col(w);
+        return (live && c != SOLVE16_NONE) ? y + c : sink;
+    };
 
     // ---- forward ----
-    double2 cur = load_rhs(l), nxt = load_rhs(16 + l);
-    int pv = load_piv(l);
-    T ring[16];
+    {
+        const unsigned w0 = W[l], w1 = W[16 + l];
+        double2 cur = fix_rhs(b[row_or0(w0)], w0), nxt = fix_rhs(b[row_or0(w1)], w1);
+        uint4 pv = P[0];
+        T ring[16];
+        const unsigned lbase = unsigned(m * 16 + l);
 #pragma unroll
-    for (int t = 0; t < 16; ++t) ring[t] = ld_nt(Lp + t * 64);
-    for (int blk = 0; blk < B; ++blk) {
-        S::template fwd_block<0>(cur, nxt, pv, ring, Lp + (size_t)(blk * 16 + 16) * 64, l, rowbase);
-        const int i0 = blk * 16 + l;
-        if (live && i0 < n) y[i0] = cur;
-        cur = nxt;
-        nxt = load_rhs(blk * 16 + 32 + l);
-        pv = load_piv(blk * 16 + 16 + l);
+        for (int t = 0; t < 16; ++t) {
+            ring[t] = ld_nt(Lg + lbase + t * 64);
+            __builtin_amdgcn_sched_barrier(0);   // in the order of use
+        }
+        for (int bk = 0; bk < B; ++bk) {
+            int lo = l;
+            asm volatile("" : "+v"(lo));
+            // loaded here, consumed at the end of this block: the right-hand
+            // side of block bk+2 and the pivots of block bk+1
+            const unsigned w2 = W[bk * 16 + 32 + l];
+            const double2 nx2 = b[row_or0(w2)];
+            const uint4 pvn = P[min(bk + 1, B - 1)];
+            unsigned lcol = lbase + unsigned(bk * 16 + 16) * 64u;
+            S::template fwd_block<0>(cur, nxt, pv, ring, Lg, lcol, lo, rowbase);
+            *out_ptr(W[bk * 16 + l]) = cur;   // intermediate, kept at the unknown's slot
+            cur = nxt;
+            nxt = fix_rhs(nx2, w2);
+            pv = pvn;
+        }
     }
     __threadfence_block();
 
     // ---- backward ----
-    const T *Ud = Ur + ((size_t)g * B * 64 + m) * (kv + 1);   // + 4j(kv+1): 1/U(j,j)
-    const T *Ul = Ud - l;   // + 4j(kv+1) + t + 16q: U(j-1-k, j), k = t-1-l+16q
+    // lane's record of column j: Ug + uofs + l + 4j(kv+1) (entry 0: 1/U(j,j));
+    // Ug + uofs + 4j(kv+1) + t + 16q: U(j-1-k, j), k = t-1-l+16q
+    const T *Ug = Ur + (size_t)g * B * 64 * (kv + 1) - 16;
+    const unsigned uofs = unsigned(m * (kv + 1) - l + 16);
     double2 r[NQ];
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
-        const int bq = B - 1 - q;
-        r[q] = ld_masked(y, bq * 16 + l, bq >= 0 && bq * 16 + l < n);
+        const unsigned wq = W[(B - 1 - q) * 16 + l];
+        r[q] = fix_y(y[col_or0(wq)], wq);
     }
     T cring[PFB][NQ];
 #pragma unroll
@@ -1436,56 +1526,77 @@ __global__ __launch_bounds__(64) void k_solve16(const T *__restrict__ L16, int n
         // step j = 16B - 1 - u lives in slot (15 - u) % PFB
         const int j = B * 16 - 1 - u;
         const int t = j & 15;
-        const long long off = (long long)j * 4 * (kv + 1);
+        const unsigned o = uofs + unsigned(j) * unsigned(4 * (kv + 1));
 #pragma unroll
-        for (int q = 0; q < NQ; ++q) cring[(15 - u) % PFB][q] = ld_nt(Ul + off + t + 16 * q);
+        for (int q = 0; q < NQ; ++q) cring[(15 - u) % PFB][q] = Ug[(size_t)o + t + 16 * q];
+        // issued in the order of use: the loop's first wait then counts the
+        // same way whether it is entered from here or from its own end
+        __builtin_amdgcn_sched_barrier(0);
     }
-    for (int blk = B - 1; blk >= 0; --blk) {
-        S::template bwd_block<15>(r, cring, Ul, l, kv, blk * 16);
-        const int i0 = blk * 16 + l;
-        if (live && i0 < n) y[i0] = rot_out(r[0], code(i0));
+    double2 xo = make_double2(0.0, 0.0);
+    for (int bk = B - 1; bk >= 0; --bk) {
+        int lo = l;
+        asm volatile("" : "+v"(lo));
+        // loaded here, consumed at the end of this block: the forward result of
+        // block bk-NQ, which enters the window of blocks being updated
+        const unsigned wy = W[(bk - NQ) * 16 + l];
+        const double2 ynx = y[col_or0(wy)];
+        S::template bwd_block<15>(r, xo, cring, Ug, uofs, lo, kv, bk * 16);
+        const unsigned w0 = W[bk * 16 + l];
+        *out_ptr(w0) = rot_out(xo, code(w0));
 #pragma unroll
         for (int q = 0; q < NQ - 1; ++q) r[q] = r[q + 1];
-        const int bq = blk - NQ;
-        r[NQ - 1] = ld_masked(y, bq * 16 + l, bq >= 0);
+        r[NQ - 1] = fix_y(ynx, wy);
     }
 }
 
 template <typename T>
-static bool launch_solve16(hipStream_t st, const T *L16, int ncol16, const T *Ur, const unsigned char *piv,
-                           const unsigned char *rot, int ident0, int n, int kl, int kv, int nsys, const double2 *rhs,
-                           double2 *x) {
-    if (kl > 16 || n < 32 || ncol16 < 16 * ((n + 15) / 16 + 1)) return false;
-    // the backward prefetch reaches 8 columns (of 4 interleaved records) before
-    // the first record
-    if (8 * 4 * (kv + 1) + 16 > SOLVE16_PAD) return false;
-    const int nq = 1 + (kv + 15) / 16;
-    const dim3 grid((nsys + 3) / 4);
+static bool launch_solve16(hipStream_t st, const Solve16Args &a) {
+    if (a.kl > 16 || a.n < 32 || a.ncol16 < 16 * ((a.n + 15) / 16 + 1)) return false;
+    // reads reach 16 elements before the first record and 64 past the last
+    const int nq = 1 + (a.kv + 15) / 16;
+    const dim3 grid((a.nsys * a.nblk + 3) / 4);
+    const size_t lds = size_t(a.nblk) * (a.n + 2 * SOLVE16_WPAD) * sizeof(unsigned);
+    if (lds > 32768 || !a.sink) return false;
     switch (nq) {
-    case 2: hipLaunchKernelGGL((k_solve16<T, 2, 8>), grid, dim3(64), 0, st, L16, ncol16, Ur, piv, rot, ident0, n, kv, nsys, rhs, x); break;
-    case 3: hipLaunchKernelGGL((k_solve16<T, 3, 8>), grid, dim3(64), 0, st, L16, ncol16, Ur, piv, rot, ident0, n, kv, nsys, rhs, x); break;
-    case 4: hipLaunchKernelGGL((k_solve16<T, 4, 8>), grid, dim3(64), 0, st, L16, ncol16, Ur, piv, rot, ident0, n, kv, nsys, rhs, x); break;
+    case 2: hipLaunchKernelGGL((k_solve16<T, 2, 8>), grid, dim3(64), lds, st, a); break;
+    case 3: hipLaunchKernelGGL((k_solve16<T, 3, 8>), grid, dim3(64), lds, st, a); break;
+    case 4: hipLaunchKernelGGL((k_solve16<T, 4, 8>), grid, dim3(64), lds, st, a); break;
     default: return false;
     }
     CHECK_LAUNCH();
     return true;
 }
 
+std::vector<unsigned> solve16_words(int total, int nblk, int blk_n, const int *blk_row, const int *blk_col,
+                                    const unsigned char *rot) {
+    const size_t stride = size_t(blk_n) + 2 * SOLVE16_WPAD;
+    std::vector<unsigned> w(size_t(nblk) * stride, SOLVE16_NOWORD);
+    for (int b = 0; b < nblk; ++b)
+        for (int i = 0; i < blk_n; ++i) {
+            const size_t e = size_t(b) * blk_n + i;
+            const int r = blk_row ? blk_row[e] : (i < total ? i : -1);
+            const int c = blk_col ? blk_col[e] : (i < total ? i : -1);
+            unsigned v = unsigned(r < 0 ? SOLVE16_NONE : r) | (unsigned(c < 0 ? SOLVE16_NONE : c) << 15);
+            if (rot && r >= 0) v |= unsigned(rot[r] & 1) << 30;
+            if (rot && c >= 0) v |= unsigned((rot[c] >> 1) & 1) << 31;
+            w[size_t(b) * stride + SOLVE16_WPAD + i] = v;
+        }
+    return w;
+}
+
 int solve16_ncol(int n, int kl, int kv) {
-    if (kl > 16 || n < 32 || 1 + (kv + 15) / 16 > 4) return 0;
+    // (positions are packed in 15 bits of the position words)
+    if (kl > 16 || n < 32 || n >= SOLVE16_NONE || 1 + (kv + 15) / 16 > 4) return 0;
     return 16 * ((n + 15) / 16 + 1);
 }
 
-bool launch_solve16_real(hipStream_t st, const double *L16, int ncol16, const double *Ur, const unsigned char *piv,
-                         const unsigned char *rot, int ident0, int n, int kl, int kv, int nsys, const double2 *rhs,
-                         double2 *x) {
-    return launch_solve16<double>(st, L16, ncol16, Ur, piv, rot, ident0, n, kl, kv, nsys, rhs, x);
-}
+bool launch_solve16_real(hipStream_t st, const Solve16Args &a) { return launch_solve16<double>(st, a); }
 
-bool launch_solve16_complex(hipStream_t st, const double2 *L16, int ncol16, const double2 *Ur,
-                            const unsigned char *piv, int ident0, int n, int kl, int kv, int nsys, const double2 *rhs,
-                            double2 *x) {
-    return launch_solve16<double2>(st, L16, ncol16, Ur, piv, nullptr, ident0, n, kl, kv, nsys, rhs, x);
+bool launch_solve16_complex(hipStream_t st, const Solve16Args &a) {
+    Solve16Args c = a;
+    c.rotate = 0;
+    return launch_solve16<double2>(st, c);
 }
 
 template <typename T>

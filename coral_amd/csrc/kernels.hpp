@@ -187,13 +187,20 @@ struct FactorArgs {
     double2 *work;            // [ns][total][ldab]
     double2 *Lf;              // [nsys][total][kl]
     double2 *Uf;              // [nsys][total][kv+1]  (Uf[.][j][0] = 1/U(j,j), Uf[.][j][1+k] = U(j-1-k, j))
-    unsigned char *piv;       // [nsys][total]   pivot offsets
+    unsigned char *piv;       // [nsys][total] pivot offsets ([nmat][16*ceil(blk_n/16)] in the 16-lane layout)
     int *info;                // [nsys]
     // real form (rot != nullptr): entries rotated by i^(phi_col - psi_row) are
     // real; factors go to Lr/Ur (same layout, real) and work is used as double
     const unsigned char *rot; // [total] psi | phi << 1
     double *Lr, *Ur;
     int ncol16;               // > 0: multipliers in the 16-lane layout with this many columns per system
+    // Structural blocks (16-lane layout only): each system is factored as nblk
+    // independent blocks of blk_n unknowns, band (blk_nl, blk_nu); block b takes
+    // rows blk_row[b*blk_n + i] and columns blk_col[b*blk_n + j] of the operator
+    // (-1: padding, identity).  s0/ns, piv, info, Lr/Ur, Lf/Uf then count blocks
+    // (block index = s*nblk + b).  nblk = 1 with null maps: the whole system.
+    int nblk, blk_n, blk_nl, blk_nu;
+    const int *blk_row, *blk_col;
 };
 void launch_factor(hipStream_t st, const FactorArgs &a);
 // generic batched factorization of LAPACK band input (for the unit seam)
@@ -205,15 +212,43 @@ void launch_solve(hipStream_t st, const double2 *Lf, const double2 *Uf, const un
 // 16-lane solve (four systems per wavefront): columns per system of its
 // multiplier records, or 0 when the band does not fit (kl > 16, kv > 47)
 int solve16_ncol(int n, int kl, int kv);
-// U arrays of the 16-lane solve carry this many elements of padding on each
-// side (its backward prefetch reads up to 8*4*(kv+1) + 15 before the first record)
+// U arrays of the solves carry this many elements of padding on each side (the
+// 16-lane solve reads up to 16 elements before the first record and 64 past
+// the last)
 constexpr int SOLVE16_PAD = 2048;
-bool launch_solve16_real(hipStream_t st, const double *L16, int ncol16, const double *Ur, const unsigned char *piv,
-                         const unsigned char *rot, int ident0, int n, int kl, int kv, int nsys, const double2 *rhs,
-                         double2 *x);
-bool launch_solve16_complex(hipStream_t st, const double2 *L16, int ncol16, const double2 *Ur,
-                            const unsigned char *piv, int ident0, int n, int kl, int kv, int nsys, const double2 *rhs,
-                            double2 *x);
+// 16-lane solve of nsys systems of `total` unknowns, each made of nblk
+// structural blocks of n unknowns and band (kl, kv - kl) (see FactorArgs);
+// factors in the layout k_factor writes for ncol16 > 0.
+struct Solve16Args {
+    const void *L16;             // double (real form) or double2
+    const void *U;               // past SOLVE16_PAD elements of padding
+    int ncol16;
+    const unsigned char *piv;    // [nsys*nblk][16*ceil(n/16)], zero past n
+    // position words (solve16_words): row and column of the system at each
+    // position of a block, and their real-form rotations
+    const unsigned *words;
+    int rotate;                  // apply the rotations (real form)
+    int ident0;                  // system 0 is the identity and is not rotated
+    int total, n, kl, kv, nsys, nblk;
+    const double2 *rhs;
+    double2 *x;
+    double2 *sink;               // 64 elements of scratch that absorb stores of padding positions
+};
+// "no row / column" in a position word (padding of a block), and the word of
+// a position outside the block
+constexpr int SOLVE16_NONE = 0x7FFF;
+constexpr unsigned SOLVE16_NOWORD = 0x3FFFFFFFu;
+// words of padding on each side of a block's words (the solve reads up to 96
+// positions before and 79 after the block)
+constexpr int SOLVE16_WPAD = 128;
+// Position words of the nblk blocks of blk_n positions (blk_row/blk_col as in
+// FactorArgs, null for one block of the whole system; rot as in SystemTables,
+// or null): row | col << 15 | psi(row) << 30 | phi(col) << 31, block b at
+// b*(blk_n + 2*SOLVE16_WPAD) + SOLVE16_WPAD, padding SOLVE16_NOWORD.
+std::vector<unsigned> solve16_words(int total, int nblk, int blk_n, const int *blk_row, const int *blk_col,
+                                    const unsigned char *rot);
+bool launch_solve16_real(hipStream_t st, const Solve16Args &a);
+bool launch_solve16_complex(hipStream_t st, const Solve16Args &a);
 // (ident0: system 0 is the identity and is not rotated)
 void launch_solve_real(hipStream_t st, const double *Lr, const double *Ur, const unsigned char *piv,
                        const unsigned char *rot, int ident0, int n, int kl, int kv, int nsys, const double2 *rhs,

@@ -231,6 +231,14 @@ class Solver:
         check(lib.coral_amd_factor_kind(self._ctx, isys, ctypes.byref(v)), "coral_amd_factor_kind")
         return bool(v.value)
 
+    def factor_blocks(self, isys: int = 0, kind: int = 0):
+        """(nblk, blk_n, blk_nl, blk_nu): structural blocks factored per system
+        (kind 0 = kxky, 1 = mean); nblk = 1 when whole systems are factored."""
+        v = [ctypes.c_int32() for _ in range(4)]
+        check(lib.coral_amd_factor_blocks(self._ctx, kind, isys, *[ctypes.byref(x) for x in v]),
+              "coral_amd_factor_blocks")
+        return tuple(x.value for x in v)
+
     def bwd_transform(self, spec: np.ndarray) -> np.ndarray:
         s = np.ascontiguousarray(spec, np.complex128)
         if s.shape != (self.NX // 2, self.NYAA, self.NZ):

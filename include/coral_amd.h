@@ -147,6 +147,18 @@ int coral_amd_system_info(coral_amd_ctx *ctx, int32_t isys, int32_t *total, int3
  * coral_amd_create forces the complex form. */
 int coral_amd_factor_kind(coral_amd_ctx *ctx, int32_t isys, int32_t *is_real);
 
+/* Matrices factored per system (kind 0 = kxky, 1 = mean): the evolution
+ * operator splits into nblk structurally independent blocks (even and odd
+ * Chebyshev degrees for Coral's operators) of blk_n unknowns and band
+ * (blk_nl, blk_nu), each factored and solved on its own; nblk = 1 when the
+ * whole system is factored (blk_n = total).  The reference factors the whole
+ * system (zgbtrf, PL_algebra_z_3D.f90:245-276): partial pivoting never selects
+ * a structurally zero row, so every row receives the same eliminations in the
+ * same order.  CORAL_AMD_NO_BLOCK_SPLIT=1 before coral_amd_create keeps whole
+ * systems. */
+int coral_amd_factor_blocks(coral_amd_ctx *ctx, int32_t kind, int32_t isys, int32_t *nblk, int32_t *blk_n,
+                            int32_t *blk_nl, int32_t *blk_nu);
+
 /* Physical values of coupled variable ivar of system isys in the current state:
  * the volumes Coral writes to CheckPoints/QuickSave.{kxky,zero}.sysNNN.varNNN.rollingR
  * (export_checkpoints, PL_IMEX_timestepping_output.f90:388-455).  kind 0: C

@@ -401,3 +401,40 @@ def test_toroidal_poloidal_recipe_parity(solver_mod, NX, NY, NZ, nsteps):
             assert rel(g.physical(f), phys[f]) <= TOL
     finally:
         g.close()
+
+
+def test_structural_blocks_match_whole_system_factors(solver_mod, monkeypatch):
+    """Factoring the even- and odd-degree blocks of the evolution operator on
+    their own against factoring whole systems (CORAL_AMD_NO_BLOCK_SPLIT): the
+    same eliminations reach every row, so the states agree to rounding; the
+    whole-system path also matches the oracle."""
+    name, NX, NY, NZ, Lx, Ly, scheme, dt, nsteps = CASES[2]
+    r = product_recipe(name)
+    k0, z0 = ic.rbc_initial_state(r, NX, NY, NZ, 1e-2, 6, 6, 3)
+    out = []
+    for nosplit in ("0", "1"):
+        monkeypatch.setenv("CORAL_AMD_NO_BLOCK_SPLIT", nosplit)
+        g = solver_mod.Solver(r, NX, NY, NZ, Lx, Ly, scheme=scheme)
+        try:
+            nblk, blk_n, blk_nl, blk_nu = g.factor_blocks(0)
+            total, nl, nu, _ = g.system_info(0)
+            print("blocks", nosplit, nblk, blk_n, blk_nl, blk_nu, "of", total, nl, nu, flush=True)
+            if nosplit == "1":
+                assert (nblk, blk_n, blk_nl, blk_nu) == (1, total, nl, nu)
+            else:
+                assert nblk == 2 and total <= 2 * blk_n < total + 8 and blk_nl < nl and blk_nu < nu
+            g.set_state(k0, z0)
+            g.march_forward(dt, nsteps)
+            out.append(g.get_state())
+        finally:
+            g.close()
+    (ka, za), (kb, zb) = out
+    assert rel(ka[0], kb[0]) <= 1e-12
+    for a, b in zip(za, zb):
+        assert rel(a, b) <= 1e-12
+    geo = O.Geometry(NX, NY, NZ, Lx, Ly)
+    ora = O.OracleSolver(r, geo, scheme)
+    ora.field, ora.zfield = [a.copy() for a in k0], [a.copy() for a in z0]
+    for it in range(nsteps):
+        ora.step(dt)
+    assert rel(kb[0], ora.field[0]) <= TOL

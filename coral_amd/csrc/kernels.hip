@@ -675,51 +675,80 @@ __global__ __launch_bounds__(256) void k_nl(NLArgs a) {
     }
 }
 
+// One workgroup per 64 consecutive systems (one per lane) and NLP_POS
+// positions (NLP_POS / 4 per wavefront).  A position's program is the same
+// for every system, so the loops are wave-uniform: the program tables are
+// read with scalar loads and every Qhat access is one contiguous row of 64
+// systems; the results go through an LDS tile so that each system's run of
+// positions is stored contiguously.
+constexpr int NLP_POS = 16;
+
 __global__ __launch_bounds__(256) void k_nl_prog(NLProgArgs a) {
-    __shared__ double2 tile[64][17];
+    __shared__ double2 tile[NLP_POS][65];
     const int nsys = a.nsys, total = a.total;
-    const int ntp = (total + 63) / 64;
-    const int s0 = (blockIdx.x / ntp) * 16, p0 = (blockIdx.x % ntp) * 64;
-    const int sl = threadIdx.x & 15, pr = threadIdx.x >> 4;
-    const int s = s0 + sl;
+    const int npb = (total + NLP_POS - 1) / NLP_POS;
+    const int pb = blockIdx.x % npb, sb = blockIdx.x / npb;
+    const int lane = threadIdx.x & 63;
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int s = sb * 64 + lane;
     const bool sv = s < nsys;
-    const double2 *Q = a.Qhat + (sv ? s : 0);
-    const double2 *cm = a.cmono + (size_t)(sv ? s : 0) * a.nterms;
-    for (int i = 0; i < 4; ++i) {
-        const int pl = pr + 16 * i, pos = p0 + pl;
+    const int sc = sv ? s : 0;
+    const double2 *__restrict__ Q = a.Qhat + sc;
+    const int *__restrict__ pos_ptr = a.pos_ptr;
+    const int *__restrict__ blk_ptr = a.blk_ptr;
+    const int *__restrict__ blk_term = a.blk_term;
+    const int *__restrict__ ent_qm = a.ent_qm;
+    const double *__restrict__ ent_w = a.ent_w;
+#pragma unroll 1
+    for (int k = 0; k < NLP_POS / 4; ++k) {
+        const int pl = wv * (NLP_POS / 4) + k;
+        const int pos = pb * NLP_POS + pl;
         double2 acc = make_double2(0.0, 0.0);
-        if (pos < total && sv && s != 0) {
-            for (int b = a.pos_ptr[pos]; b < a.pos_ptr[pos + 1]; ++b) {
+        if (pos < total) {
+            const int b0 = pos_ptr[pos], b1 = pos_ptr[pos + 1];
+            for (int b = b0; b < b1; ++b) {
+                const int e0 = blk_ptr[b], e1 = blk_ptr[b + 1];
                 double2 sum = make_double2(0.0, 0.0);
-                for (int e = a.blk_ptr[b]; e < a.blk_ptr[b + 1]; ++e) {
-                    const double w = a.ent_w[e];
-                    const double2 q = Q[(size_t)a.ent_qm[e] * nsys];
-                    sum.x += w * q.x;
-                    sum.y += w * q.y;
+                int e = e0;
+                for (; e + 4 <= e1; e += 4) {
+                    double2 q[4];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) q[u] = Q[(size_t)ent_qm[e + u] * nsys];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        sum.x = fma(ent_w[e + u], q[u].x, sum.x);
+                        sum.y = fma(ent_w[e + u], q[u].y, sum.y);
+                    }
                 }
-                acc = cadd(acc, cmul(cm[a.blk_term[b]], sum));
+                for (; e < e1; ++e) {
+                    const double2 q = Q[(size_t)ent_qm[e] * nsys];
+                    sum.x = fma(ent_w[e], q.x, sum.x);
+                    sum.y = fma(ent_w[e], q.y, sum.y);
+                }
+                acc = cadd(acc, cmul(a.cmono[(size_t)blk_term[b] * nsys + sc], sum));
             }
         }
-        tile[pl][sl] = acc;
+        // the kxky (0,0) mode stays zero
+        tile[pl][lane] = (sv && s != 0) ? acc : make_double2(0.0, 0.0);
     }
     __syncthreads();
-    for (int i = 0; i < 4; ++i) {
-        const int e = threadIdx.x + 256 * i, sl2 = e >> 6, pl = e & 63;
-        const int s2 = s0 + sl2, pos = p0 + pl;
-        if (s2 < nsys && pos < total) a.Khat[(size_t)s2 * total + pos] = tile[pl][sl2];
+    for (int i = 0; i < 64 * NLP_POS / 256; ++i) {
+        const int e = threadIdx.x + 256 * i, sl = e / NLP_POS, pl = e % NLP_POS;
+        const int s2 = sb * 64 + sl, pos = pb * NLP_POS + pl;
+        if (s2 < nsys && pos < total) a.Khat[(size_t)s2 * total + pos] = tile[pl][sl];
     }
 }
 
 void launch_nl_prog(hipStream_t st, const NLProgArgs &a) {
-    const int ntp = (a.total + 63) / 64;
-    hipLaunchKernelGGL(k_nl_prog, dim3(blocks_for(a.nsys, 16) * ntp), dim3(256), 0, st, a);
+    const int npb = (a.total + NLP_POS - 1) / NLP_POS;
+    hipLaunchKernelGGL(k_nl_prog, dim3(blocks_for(a.nsys, 64) * npb), dim3(256), 0, st, a);
     CHECK_LAUNCH();
 }
 
 __global__ void k_term_monomials(const int *t_kx, const int *t_ky, int nterms, KeptModes km, int nsys, double2 *out) {
     const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= (long long)nsys * nterms) return;
-    const int s = int(idx / nterms), t = int(idx % nterms);
+    const int t = int(idx / nsys), s = int(idx % nsys);
     out[idx] = monomial(t_kx[t], t_ky[t], km.kx(s), km.ky(s));
 }
 

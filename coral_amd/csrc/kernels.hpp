@@ -117,20 +117,20 @@ struct NLArgs {
 void launch_nl(hipStream_t st, const NLArgs &a);
 
 // kxky NL assembly from a per-position program: position p runs term blocks
-// [pos_ptr[p], pos_ptr[p+1]); block b adds cmono[s][blk_term[b]] * sum over
+// [pos_ptr[p], pos_ptr[p+1]); block b adds cmono[blk_term[b]][s] * sum over
 // entries [blk_ptr[b], blk_ptr[b+1]) of ent_w * Qhat[ent_qm][s]
 // (ent_qm = q*NZ + m, ent_w = dsca * trunc(I^c M^d)[row][m]).
 struct NLProgArgs {
     const int *pos_ptr, *blk_term, *blk_ptr, *ent_qm;
     const double *ent_w;
-    const double2 *cmono;  // [nsys][nterms]
+    const double2 *cmono;  // [nterms][nsys]
     int nterms;
     const double2 *Qhat;   // [nquad][NZ][nsys]
     int nsys, total;
     double2 *Khat;         // [nsys][total]
 };
 void launch_nl_prog(hipStream_t st, const NLProgArgs &a);
-// cmono[s*nterms + t] = (i kx)^kx_t (i ky)^ky_t
+// cmono[t*nsys + s] = (i kx)^kx_t (i ky)^ky_t
 void launch_term_monomials(hipStream_t st, const int *t_kx, const int *t_ky, int nterms, KeptModes km, int nsys,
                            double2 *cmono);
 

@@ -342,7 +342,8 @@ void fill_system_device(SysDev &d, int nstages, const KeptModes &km, hipStream_t
     const int bkv = d.blk_nl + d.blk_nu;
     const size_t nmat = size_t(d.nsys) * d.nblk, nmat4 = (nmat + 3) / 4 * 4;
     const size_t nb = 16 * ((size_t(d.blk_n) + 15) / 16);
-    const size_t lsize = d.ncol16 > 0 ? nmat4 * d.ncol16 * 16 : size_t(d.nsys) * t.total * std::max(t.nl, 1);
+    const size_t lsize = d.ncol16 > 0 ? nmat4 * d.ncol16 * solve16_klp(d.blk_nl)
+                                      : size_t(d.nsys) * t.total * std::max(t.nl, 1);
     const size_t usize =
         (d.ncol16 > 0 ? nmat4 * nb * (bkv + 1) : size_t(d.nsys) * t.total * (kv + 1)) + 2 * size_t(SOLVE16_PAD);
     if (d.real) {

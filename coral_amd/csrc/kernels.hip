@@ -973,25 +973,24 @@ __device__ void split_factors(const T *AB, int n, int kl, int ku, int ldab, T *L
     }
 }
 
-// Factor records for the 16-lane solve.  The four systems one wavefront
+// Factor records for the 16-lane solve.  The four matrices one wavefront
 // solves are interleaved column by column, so that the wavefront's loads of
-// one step fall on adjacent records.  For system s (group g = s/4, slot
-// m = s%4):
-//   L16: column j at ((g*ncol16 + j)*4 + m)*16; its multipliers are stored
-//        rotated so that lane l (position 16b + l of the block it updates)
-//        reads entry l: L16[l] = L(j + 1 + k, j), k = (l - j%16 - 1) mod 16,
-//        zero for k >= min(kl, n-1-j) and for the padding columns j >= n;
+// one step fall on adjacent records.  For matrix M (group g = M/4, slot
+// m = M%4), with KLP = solve16_klp(kl):
+//   L:   column j at ((g*ncol16 + j)*4 + m)*KLP: entry k < kl is
+//        L(j + 1 + k, j) (zero for k >= n-1-j and for the padding columns
+//        j >= n), entry kl (when KLP > kl) is a zero the lanes outside the
+//        band read;
 //   U:   column j at ((g*nb + j)*4 + m)*(kv+1), nb = 16*ceil(n/16) columns per
-//        system (those beyond n stay zero): entry 0 is 1/U(j,j), entry k the
+//        matrix (those beyond n stay zero): entry 0 is 1/U(j,j), entry k the
 //        superdiagonal U(j-k, j).
 template <typename T>
 __device__ void split_factors16(const T *AB, int n, int kl, int ku, int ldab, int ncol16, int s, T *L16, T *Ur) {
     using F = Fx<T>;
-    const int kv = kl + ku, nb = 16 * ((n + 15) >> 4), g = s >> 2, m = s & 3;
-    for (int e = threadIdx.x; e < ncol16 * 16; e += blockDim.x) {
-        const int j = e >> 4, l = e & 15;
-        const int k = (l - (j & 15) - 1) & 15;
-        L16[(((size_t)g * ncol16 + j) * 4 + m) * 16 + l] =
+    const int kv = kl + ku, nb = 16 * ((n + 15) >> 4), g = s >> 2, m = s & 3, klp = solve16_klp(kl);
+    for (int e = threadIdx.x; e < ncol16 * klp; e += blockDim.x) {
+        const int j = e / klp, k = e - j * klp;
+        L16[(((size_t)g * ncol16 + j) * 4 + m) * klp + k] =
             (j < n && k < min(kl, n - 1 - j)) ? AB[(size_t)j * ldab + kv + 1 + k] : F::zero();
     }
     for (int e = threadIdx.x; e < n * (kv + 1); e += blockDim.x) {
@@ -1393,7 +1392,7 @@ struct Solve16 {
     // compiler so that the lane masks are formed where they are used.
     template <int t>
     __device__ static inline void fwd_step(double2 &cur, double2 &nxt, const uint4 &pv, T (&ring)[16], const T *Lg,
-                                           unsigned &lcol, int lo, int rowbase) {
+                                           unsigned &lcol, int lo, int rowbase, int klp) {
         const T zero = Fx<T>::zero();
         const unsigned pw = t < 4 ? pv.x : t < 8 ? pv.y : t < 12 ? pv.z : pv.w;
         const int p = int((pw >> (8 * (t & 3))) & 0xffu);
@@ -1409,17 +1408,20 @@ struct Solve16 {
         cur = submul(cur, tsel(below, c, zero), bj);
         nxt = submul(nxt, tsel(below, zero, c), bj);
         asm volatile("" : "+v"(lcol) : "v"(cur.x), "v"(cur.y), "v"(nxt.x), "v"(nxt.y));
-        ring[t] = ld_nt(Lg + (size_t)lcol + t * 64);
+        // entry k = (lo - t - 1) mod 16 of column t of the next block; k >= kl
+        // reads the record's zero (or, for KLP = 16, is inside the band)
+        const unsigned kk = min(unsigned((lo + 15 - t) & 15), unsigned(klp - 1));
+        ring[t] = ld_nt(Lg + (size_t)(lcol + kk) + t * 4 * klp);
         __builtin_amdgcn_sched_barrier(0);   // and not later: the prefetch distance stays 16 steps
     }
 
-    // Lg: the wavefront's multiplier records (uniform); lcol: the lane's
-    // offset of the next block's first column
+    // Lg: the wavefront's multiplier records (uniform); lcol: the offset of
+    // the lane's record of the next block's first column
     template <int t>
     __device__ static inline void fwd_block(double2 &cur, double2 &nxt, const uint4 &pv, T (&ring)[16], const T *Lg,
-                                            unsigned &lcol, int lo, int rowbase) {
-        fwd_step<t>(cur, nxt, pv, ring, Lg, lcol, lo, rowbase);
-        if constexpr (t < 15) fwd_block<t + 1>(cur, nxt, pv, ring, Lg, lcol, lo, rowbase);
+                                            unsigned &lcol, int lo, int rowbase, int klp) {
+        fwd_step<t>(cur, nxt, pv, ring, Lg, lcol, lo, rowbase, klp);
+        if constexpr (t < 15) fwd_block<t + 1>(cur, nxt, pv, ring, Lg, lcol, lo, rowbase, klp);
     }
 
     // backward step t (descending) of a block.  Lane t's q = 0 entry of the
@@ -1485,7 +1487,8 @@ __global__ __launch_bounds__(64) void k_solve16(Solve16Args a) {
     const int sys = M / a.nblk, blk = M - sys * a.nblk;
     const bool dorot = a.rotate && !(a.ident0 && sys == 0);
     const int g = M >> 2, m = M & 3;
-    const T *Lg = L16 + (size_t)g * ncol16 * 64;   // wavefront's records; lane's column j at m*16 + l + 64j
+    const int klp = solve16_klp(a.kl);
+    const T *Lg = L16 + (size_t)g * ncol16 * 4 * klp;   // wavefront's records; matrix m's column j at (4j + m)*klp
     const int B = (n + 15) >> 4;
     // pivot offsets of block b: 16 bytes at P[b] (zero past n)
     const uint4 *P = reinterpret_cast<const uint4 *>(a.piv + (size_t)M * B * 16);
@@ -1514,10 +1517,11 @@ __global__ __launch_bounds__(64) void k_solve16(Solve16Args a) {
         double2 cur = fix_rhs(b[row_or0(w0)], w0), nxt = fix_rhs(b[row_or0(w1)], w1);
         uint4 pv = P[0];
         T ring[16];
-        const unsigned lbase = unsigned(m * 16 + l);
+        const unsigned lbase = unsigned(m * klp);
 #pragma unroll
         for (int t = 0; t < 16; ++t) {
-            ring[t] = ld_nt(Lg + lbase + t * 64);
+            const unsigned kk = min(unsigned((l + 15 - t) & 15), unsigned(klp - 1));
+            ring[t] = ld_nt(Lg + lbase + kk + t * 4 * klp);
             __builtin_amdgcn_sched_barrier(0);   // in the order of use
         }
         for (int bk = 0; bk < B; ++bk) {
@@ -1528,8 +1532,8 @@ __global__ __launch_bounds__(64) void k_solve16(Solve16Args a) {
             const unsigned w2 = W[bk * 16 + 32 + l];
             const double2 nx2 = b[row_or0(w2)];
             const uint4 pvn = P[min(bk + 1, B - 1)];
-            unsigned lcol = lbase + unsigned(bk * 16 + 16) * 64u;
-            S::template fwd_block<0>(cur, nxt, pv, ring, Lg, lcol, lo, rowbase);
+            unsigned lcol = lbase + unsigned(bk * 16 + 16) * unsigned(4 * klp);
+            S::template fwd_block<0>(cur, nxt, pv, ring, Lg, lcol, lo, rowbase, klp);
             *out_ptr(W[bk * 16 + l]) = cur;   // intermediate, kept at the unknown's slot
             cur = nxt;
             nxt = fix_rhs(nx2, w2);

@@ -50,9 +50,8 @@ struct XPlan<768> {
     template <int S> using FwdRest = Stages<768, -1, S, 6, 4, 4, 8>;
 };
 
-constexpr int XGP = 2;  // product pairs per forward batch
-
-template <int N, int NPI>
+// XGP: product pairs per forward batch
+template <int N, int NPI, int XGP>
 struct XShape {
     using P = XPlan<N>;
     static constexpr int LPW = P::LPW, R0 = P::R0, RL = P::RL;
@@ -64,24 +63,49 @@ struct XShape {
     static_assert((LPW * TL) % 64 == 0, "register stage must fill the wavefront");
 };
 
-// value of full variable idx (or 1 for idx < 0) at point r of a task
-template <int NPI, int RL>
-__device__ inline double fval(const double2 (&f)[NPI][RL], int idx, int r) {
-    double v = 1.0;
+// value of full variable A (1 for A < 0) at point r of a task
+template <int A, int NPI, int RL>
+__device__ inline double fget(const double2 (&f)[NPI][RL], int r) {
+    if constexpr (A < 0)
+        return 1.0;
+    else
+        return (A & 1) ? f[A >> 1][r].y : f[A >> 1][r].x;
+}
+
+// o[r] = F_a at the RL points of a task.  a is the same for the whole
+// wavefront: a scalar branch picks the copy (the empty asm keeps the branches
+// from being turned into per-element selects, one v_cndmask per candidate).
+template <int NPI, int RL, int K = 0>
+__device__ inline void pick(const double2 (&f)[NPI][RL], int a, double (&o)[RL]) {
+    if constexpr (K < 2 * NPI) {
+        if (a == K) {
+            asm volatile("");
 #pragma unroll
-    for (int p = 0; p < NPI; ++p) {
-        if (idx == 2 * p) v = f[p][r].x;
-        if (idx == 2 * p + 1) v = f[p][r].y;
+            for (int r = 0; r < RL; ++r) o[r] = fget<K, NPI, RL>(f, r);
+            return;
+        }
+        pick<NPI, RL, K + 1>(f, a, o);
+    } else {
+#pragma unroll
+        for (int r = 0; r < RL; ++r) o[r] = 1.0;   // a < 0: the constant ONE
     }
-    return v;
+}
+
+template <int NPI, int RL>
+__device__ inline void product(const double2 (&f)[NPI][RL], int a, int b, double (&p)[RL]) {
+    double fa[RL], fb[RL];
+    pick<NPI, RL>(f, a, fa);
+    pick<NPI, RL>(f, b, fb);
+#pragma unroll
+    for (int r = 0; r < RL; ++r) p[r] = fa[r] * fb[r];
 }
 
 // forward transforms of product pairs [pq0, pq0 + G) and the store of the kept kx
-template <int N, int NPI, int G>
-__device__ inline void xfwd_batch(const XPassArgs &a, const double2 (&fv)[XShape<N, NPI>::NT][NPI][XPlan<N>::RL],
+template <int N, int NPI, int XGP, int G>
+__device__ inline void xfwd_batch(const XPassArgs &a, const double2 (&fv)[XShape<N, NPI, XGP>::NT][NPI][XPlan<N>::RL],
                                   double2 *buf, const double2 *__restrict__ tw, int lane, long long line0,
                                   long long nlines, int pq0) {
-    using X = XShape<N, NPI>;
+    using X = XShape<N, NPI, XGP>;
     constexpr int RL = X::RL, TL = X::TL, LPW = X::LPW;
 #pragma unroll
     for (int c = 0; c < X::NT; ++c) {
@@ -90,15 +114,19 @@ __device__ inline void xfwd_batch(const XPassArgs &a, const double2 (&fv)[XShape
         for (int gp = 0; gp < G; ++gp) {
             const int q0 = 2 * (pq0 + gp), q1 = q0 + 1;
             const bool has1 = q1 < a.nquad;
-            const int a0 = a.qa[q0], b0 = a.qb[q0];
-            const int a1 = has1 ? a.qa[q1] : -1, b1 = has1 ? a.qb[q1] : -1;
+            const int a0 = __builtin_amdgcn_readfirstlane(a.qa[q0]), b0 = __builtin_amdgcn_readfirstlane(a.qb[q0]);
+            const int a1 = has1 ? __builtin_amdgcn_readfirstlane(a.qa[q1]) : -1;
+            const int b1 = has1 ? __builtin_amdgcn_readfirstlane(a.qb[q1]) : -1;
+            double p0[RL], p1[RL];
+            product<NPI, RL>(fv[c], a0, b0, p0);
+            if (has1)
+                product<NPI, RL>(fv[c], a1, b1, p1);
+            else
+#pragma unroll
+                for (int r = 0; r < RL; ++r) p1[r] = 0.0;
             double2 y[RL];
 #pragma unroll
-            for (int r = 0; r < RL; ++r) {
-                const double p0 = fval<NPI, RL>(fv[c], a0, r) * fval<NPI, RL>(fv[c], b0, r);
-                const double p1 = has1 ? fval<NPI, RL>(fv[c], a1, r) * fval<NPI, RL>(fv[c], b1, r) : 0.0;
-                y[r] = make_double2(p0, p1);
-            }
+            for (int r = 0; r < RL; ++r) y[r] = make_double2(p0[r], p1[r]);
             Dft<RL, -1>::run(y);
             double2 *L = buf + (l * G + gp) * N + j * RL;
 #pragma unroll
@@ -130,9 +158,9 @@ __device__ inline void xfwd_batch(const XPassArgs &a, const double2 (&fv)[XShape
     wfft::wave_sync();
 }
 
-template <int N, int NPI>
+template <int N, int NPI, int XGP>
 __global__ __launch_bounds__(256) void k_xpass_wave(XPassArgs a, const double2 *__restrict__ tw, long long nlines) {
-    using X = XShape<N, NPI>;
+    using X = XShape<N, NPI, XGP>;
     constexpr int R0 = X::R0, RL = X::RL, T0 = X::T0, TL = X::TL, LPW = X::LPW, SIN = X::SIN;
     extern __shared__ double2 lds[];
     __shared__ double cfl_part[4][3];
@@ -247,15 +275,15 @@ __global__ __launch_bounds__(256) void k_xpass_wave(XPassArgs a, const double2 *
     const int npairs = (a.nquad + 1) / 2;
     for (int pq0 = 0; pq0 < npairs; pq0 += XGP) {
         if (npairs - pq0 >= XGP)
-            xfwd_batch<N, NPI, XGP>(a, fv, buf, tw, lane, line0, nlines, pq0);
+            xfwd_batch<N, NPI, XGP, XGP>(a, fv, buf, tw, lane, line0, nlines, pq0);
         else
-            xfwd_batch<N, NPI, 1>(a, fv, buf, tw, lane, line0, nlines, pq0);
+            xfwd_batch<N, NPI, XGP, 1>(a, fv, buf, tw, lane, line0, nlines, pq0);
     }
 }
 
-template <int N, int NPI>
+template <int N, int NPI, int XGP>
 static bool launch_wave(hipStream_t st, const FFTPlan &px, const XPassArgs &a) {
-    using X = XShape<N, NPI>;
+    using X = XShape<N, NPI, XGP>;
     const long long nlines = (long long)a.NZAA * a.NYAA;
     const size_t per_wave = size_t(X::SMAX) * N * sizeof(double2);
     int wpb = 4;
@@ -263,7 +291,7 @@ static bool launch_wave(hipStream_t st, const FFTPlan &px, const XPassArgs &a) {
     if (wpb * per_wave > 65536) return false;
     const long long per_block = (long long)wpb * X::LPW;
     const unsigned blocks = unsigned((nlines + per_block - 1) / per_block);
-    hipLaunchKernelGGL((k_xpass_wave<N, NPI>), dim3(blocks), dim3(64 * wpb), wpb * per_wave, st, a, px.tw, nlines);
+    hipLaunchKernelGGL((k_xpass_wave<N, NPI, XGP>), dim3(blocks), dim3(64 * wpb), wpb * per_wave, st, a, px.tw, nlines);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) fprintf(stderr, "coral_amd: launch failed: %s\n", hipGetErrorString(e));
     return true;
@@ -271,10 +299,12 @@ static bool launch_wave(hipStream_t st, const FFTPlan &px, const XPassArgs &a) {
 
 template <int N>
 static bool launch_wave_n(hipStream_t st, const FFTPlan &px, const XPassArgs &a) {
+    // one product pair per forward batch: two pairs would fill the lanes of
+    // the forward stages better, but their registers cost a wave per SIMD
     switch ((a.nfull + 1) / 2) {
-    case 1: return launch_wave<N, 1>(st, px, a);
-    case 2: return launch_wave<N, 2>(st, px, a);
-    case 3: return launch_wave<N, 3>(st, px, a);
+    case 1: return launch_wave<N, 1, 1>(st, px, a);
+    case 2: return launch_wave<N, 2, 1>(st, px, a);
+    case 3: return launch_wave<N, 3, 1>(st, px, a);
     default: return false;
     }
 }

@@ -44,6 +44,7 @@ CASES = [
     ("coral.equations.rrbc_ra5e7_ek1e-4", 16, 16, 129, 2 * np.pi, 2 * np.pi, "ARS_222", 2e-5, 2),
     ("coral.equations.rbc_ra1e5", 32, 32, 65, 2 * np.pi, 2 * np.pi, "ARS_222", 1e-4, 2),
     ("coral.equations.rbc_ra1e5", 16, 32, 49, 2.0, 2 * np.pi, "ARS_343", 1e-4, 2),
+    ("coral.equations.rrbc_ra5e7_ek1e-4", 16, 16, 257, 2 * np.pi, 2 * np.pi, "ARS_222", 5e-6, 2),
 ]
 
 
@@ -130,6 +131,8 @@ BROADBAND = [
     ("coral.equations.rrbc_ra5e7_ek1e-4", 256, 256, 17, 2 * np.pi, 2 * np.pi, "ARS_222", 2e-5, 2),
     ("coral.equations.rbc_ra1e5", 128, 96, 17, 2 * np.pi, 3.0, "ARS_222", 1e-4, 2),
     ("coral.equations.rbc_ra1e5", 64, 256, 17, 2.0, 2 * np.pi, "ARS_443", 1e-4, 1),
+    # horizontal transform lengths of the 512x512 grid (768-point lines)
+    ("coral.equations.rrbc_ra5e7_ek1e-4", 512, 512, 17, 2 * np.pi, 2 * np.pi, "ARS_222", 5e-6, 1),
 ]
 
 

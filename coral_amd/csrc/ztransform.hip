@@ -24,11 +24,14 @@ namespace coral {
 
 typedef double d4 __attribute__((ext_vector_type(4)));
 
+// k-steps of B operands in flight (synthesis / analysis), threads per workgroup
+constexpr int ZPF_BWD = 4, ZPF_FWD = 8, ZTHREADS = 512;
+
 __device__ inline d4 mfma(double a, double b, d4 c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
 
 // D(i, j) of a 16x16 tile sits in lane j + 16*(i%4), element i/4.
 template <int TE, int TO>
-__global__ __launch_bounds__(512) void k_zbwd_mfma(const double *__restrict__ frag, int NEK, int NOK,
+__global__ __launch_bounds__(ZTHREADS) void k_zbwd_mfma(const double *__restrict__ frag, int NEK, int NOK,
                                                     const double *__restrict__ X, double *__restrict__ F, int NZ,
                                                     int NZAA, int ncol, int nb) {
     extern __shared__ double lds[];
@@ -49,30 +52,41 @@ __global__ __launch_bounds__(512) void k_zbwd_mfma(const double *__restrict__ fr
         for (int r = 0; r < TE; ++r) ae[r] = d4{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
         for (int r = 0; r < TO; ++r) ao[r] = d4{0.0, 0.0, 0.0, 0.0};
-        // B operands one k-step ahead
+        // B operands ZPF_BWD k-steps ahead, in a ring whose slot is the step's
+        // position in an unrolled group of ZPF_BWD steps; loaded values stay raw
+        // until their step
         auto ld_e = [&](int kk) -> double {
             const int m = 4 * kk + kq;
-            const bool ok = cv && m < NE;
-            const double v = Xb[(size_t)(ok ? 2 * m : 0) * ncol];
-            return ok ? v : 0.0;
+            return Xb[(size_t)(cv && m < NE ? 2 * m : 0) * ncol];
         };
         auto ld_o = [&](int kk) -> double {
             const int m = 4 * kk + kq;
-            const bool ok = cv && m < NO;
-            const double v = Xb[(size_t)(ok ? 2 * m + 1 : 0) * ncol];
-            return ok ? v : 0.0;
+            return Xb[(size_t)(cv && m < NO ? 2 * m + 1 : 0) * ncol];
         };
-        double be = ld_e(0), bo = ld_o(0);
-        for (int kk = 0; kk < NEK; ++kk) {
-            const double be_n = ld_e(kk + 1), bo_n = ld_o(kk + 1);
+        double re[ZPF_BWD], ro[ZPF_BWD];
 #pragma unroll
-            for (int r = 0; r < TE; ++r) ae[r] = mfma(GE[(r * NEK + kk) * 64 + lane], be, ae[r]);
-            if (kk < NOK) {
+        for (int u = 0; u < ZPF_BWD; ++u) {
+            re[u] = ld_e(u);
+            ro[u] = ld_o(u);
+        }
+        for (int k0 = 0; k0 < NEK; k0 += ZPF_BWD) {
 #pragma unroll
-                for (int r = 0; r < TO; ++r) ao[r] = mfma(GO[(r * NOK + kk) * 64 + lane], bo, ao[r]);
+            for (int u = 0; u < ZPF_BWD; ++u) {
+                const int kk = k0 + u;
+                if (kk < NEK) {
+                    const int m = 4 * kk + kq;
+                    const double be = (cv && m < NE) ? re[u] : 0.0;
+                    const double bo = (cv && m < NO) ? ro[u] : 0.0;
+#pragma unroll
+                    for (int r = 0; r < TE; ++r) ae[r] = mfma(GE[(r * NEK + kk) * 64 + lane], be, ae[r]);
+                    if (kk < NOK) {
+#pragma unroll
+                        for (int r = 0; r < TO; ++r) ao[r] = mfma(GO[(r * NOK + kk) * 64 + lane], bo, ao[r]);
+                    }
+                }
+                re[u] = ld_e(kk + ZPF_BWD);
+                ro[u] = ld_o(kk + ZPF_BWD);
             }
-            be = be_n;
-            bo = bo_n;
         }
         if (cv) {
             double *Fb = F + (size_t)b * NZAA * ncol + col;
@@ -91,7 +105,7 @@ __global__ __launch_bounds__(512) void k_zbwd_mfma(const double *__restrict__ fr
 }
 
 template <int RE, int RO>
-__global__ __launch_bounds__(512) void k_zfwd_mfma(const double *__restrict__ frag, int KEK, int KOK,
+__global__ __launch_bounds__(ZTHREADS) void k_zfwd_mfma(const double *__restrict__ frag, int KEK, int KOK,
                                                     const double *__restrict__ Y, double *__restrict__ Q, int NZ,
                                                     int NZAA, int ncol, int nb) {
     extern __shared__ double lds[];
@@ -112,29 +126,41 @@ __global__ __launch_bounds__(512) void k_zfwd_mfma(const double *__restrict__ fr
         for (int r = 0; r < RE; ++r) ae[r] = d4{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
         for (int r = 0; r < RO; ++r) ao[r] = d4{0.0, 0.0, 0.0, 0.0};
-        // folded B operands (S, D) one k-step ahead
-        auto ld = [&](int kk, double &S, double &D) {
+        // rows z and NZAA-1-z of the B operand ZPF_FWD k-steps ahead (ring as in
+        // the synthesis), folded into S and D at their step
+        auto ld0 = [&](int kk) -> double {
             const int z = 4 * kk + kq;
-            const bool oh = cv && z < Ke, om = cv && z < Ko;
-            const double h0 = Yb[(size_t)(oh ? z : 0) * ncol];
-            const double h1 = Yb[(size_t)(om ? NZAA - 1 - z : 0) * ncol];
-            const double h = oh ? h0 : 0.0, hm = om ? h1 : 0.0;
-            S = h + hm;
-            D = om ? h - hm : 0.0;
+            return Yb[(size_t)(cv && z < Ke ? z : 0) * ncol];
         };
-        double S, D;
-        ld(0, S, D);
-        for (int kk = 0; kk < KEK; ++kk) {
-            double Sn, Dn;
-            ld(kk + 1, Sn, Dn);
+        auto ld1 = [&](int kk) -> double {
+            const int z = 4 * kk + kq;
+            return Yb[(size_t)(cv && z < Ko ? NZAA - 1 - z : 0) * ncol];
+        };
+        double r0[ZPF_FWD], r1[ZPF_FWD];
 #pragma unroll
-            for (int r = 0; r < RE; ++r) ae[r] = mfma(FE[(r * KEK + kk) * 64 + lane], S, ae[r]);
-            if (kk < KOK) {
+        for (int u = 0; u < ZPF_FWD; ++u) {
+            r0[u] = ld0(u);
+            r1[u] = ld1(u);
+        }
+        for (int k0 = 0; k0 < KEK; k0 += ZPF_FWD) {
 #pragma unroll
-                for (int r = 0; r < RO; ++r) ao[r] = mfma(FO[(r * KOK + kk) * 64 + lane], D, ao[r]);
+            for (int u = 0; u < ZPF_FWD; ++u) {
+                const int kk = k0 + u;
+                if (kk < KEK) {
+                    const int z = 4 * kk + kq;
+                    const bool oh = cv && z < Ke, om = cv && z < Ko;
+                    const double h = oh ? r0[u] : 0.0, hm = om ? r1[u] : 0.0;
+                    const double S = h + hm, D = om ? h - hm : 0.0;
+#pragma unroll
+                    for (int r = 0; r < RE; ++r) ae[r] = mfma(FE[(r * KEK + kk) * 64 + lane], S, ae[r]);
+                    if (kk < KOK) {
+#pragma unroll
+                        for (int r = 0; r < RO; ++r) ao[r] = mfma(FO[(r * KOK + kk) * 64 + lane], D, ao[r]);
+                    }
+                }
+                r0[u] = ld0(kk + ZPF_FWD);
+                r1[u] = ld1(kk + ZPF_FWD);
             }
-            S = Sn;
-            D = Dn;
         }
         if (cv) {
             double *Qb = Q + (size_t)b * NZ * ncol + col;
@@ -204,7 +230,7 @@ static void go_bwd(hipStream_t st, const ZFragsDev &z, const double *X, double *
     const size_t shm = size_t(TE * z.NEK + TO * z.NOK) * 64 * sizeof(double);
     (void)hipFuncSetAttribute((const void *)k_zbwd_mfma<TE, TO>, hipFuncAttributeMaxDynamicSharedMemorySize,
                               (int)shm);
-    hipLaunchKernelGGL((k_zbwd_mfma<TE, TO>), dim3(nblocks), dim3(512), shm, st, z.bwd, z.NEK, z.NOK, X, F, NZ, NZAA,
+    hipLaunchKernelGGL((k_zbwd_mfma<TE, TO>), dim3(nblocks), dim3(ZTHREADS), shm, st, z.bwd, z.NEK, z.NOK, X, F, NZ, NZAA,
                        ncol, nb);
 }
 
@@ -214,12 +240,12 @@ static void go_fwd(hipStream_t st, const ZFragsDev &z, const double *Y, double *
     const size_t shm = size_t(RE * z.KEK + RO * z.KOK) * 64 * sizeof(double);
     (void)hipFuncSetAttribute((const void *)k_zfwd_mfma<RE, RO>, hipFuncAttributeMaxDynamicSharedMemorySize,
                               (int)shm);
-    hipLaunchKernelGGL((k_zfwd_mfma<RE, RO>), dim3(nblocks), dim3(512), shm, st, z.fwd, z.KEK, z.KOK, Y, Q, NZ, NZAA,
+    hipLaunchKernelGGL((k_zfwd_mfma<RE, RO>), dim3(nblocks), dim3(ZTHREADS), shm, st, z.fwd, z.KEK, z.KOK, Y, Q, NZ, NZAA,
                        ncol, nb);
 }
 
 static int grid_for(int ntiles, int ncu) {
-    const int want = (ntiles + 7) / 8;
+    const int want = (ntiles + ZTHREADS / 64 - 1) / (ZTHREADS / 64);
     return want < ncu ? want : ncu;
 }
 

@@ -44,16 +44,18 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def solve_bytes(N, nsys, real, blocks):
+def solve_bytes(N, nsys, real, blocks, u_entries):
     """Algorithmic HBM bytes of one banded-solve launch.  Each system is factored
     as nblk structural blocks of blk_n unknowns with band (bl, bu): per column
-    bl multipliers, bl+bu+1 U entries (8 B per entry in real form, 16 B complex)
-    and a 1 B pivot offset; per unknown the complex right-hand side read and the
-    solution written.  (Row/column maps and rotation codes are shared by all
-    systems and not counted.)"""
+    bl multipliers, u_entries U entries (the mean length of the U records,
+    each cut to the longest nonzero prefix its column has over all
+    wavenumbers; at most bl+bu+1), 8 B per entry in real form, 16 B complex,
+    and a 1 B pivot offset; per unknown the complex right-hand side read and
+    the solution written.  (Row/column maps and rotation codes are shared by
+    all systems and not counted.)"""
     nblk, blk_n, bl, bu = blocks
     e = 8 if real else 16
-    return nsys * (nblk * blk_n * (e * bl + e * (bl + bu + 1) + 1) + N * (16 + 16))
+    return int(nsys * (nblk * blk_n * (e * bl + e * u_entries + 1) + N * (16 + 16)))
 
 
 def kernel_bytes(name, P):
@@ -62,7 +64,7 @@ def kernel_bytes(name, P):
     nf, nq = P["nfull"], P["nquad"]
     S_sys = 16 * N * nsys
     table = {
-        "solve": solve_bytes(N, nsys, P["real"], P["blocks"]),
+        "solve": solve_bytes(N, nsys, P["real"], P["blocks"], P["u_entries"]),
         # M x^n + L (dt a x_1) + K_hat terms: ARS_222 reads 2 (stage 1) and 4 (stage 2)
         # state vectors and writes one, 4 on average per launch
         "rhs": 4 * S_sys,
@@ -157,7 +159,8 @@ def main():
     log(f"{steps} steps in {elapsed:.3f} s, state finite={finite}, |state|={norm:.6e}, CFL dt of the last stage "
         f"{cfl_final:.3e} (dt {wl['dt']:.1e})")
 
-    P = dict(N=total, kl=kl, ku=ku, nsys=nsys, real=g.factor_is_real(0), blocks=g.factor_blocks(0), NZ=NZ,
+    P = dict(N=total, kl=kl, ku=ku, nsys=nsys, real=g.factor_is_real(0), blocks=g.factor_blocks(0),
+             u_entries=g.u_entries_per_column(0), NZ=NZ,
              NZAA=3 * NZ // 2, NYAA=3 * NX // 2, nkx=NX // 2,
              nfull=len(recipe.full_vars), nquad=len(recipe.quad_vars))
     per_kernel = {}
@@ -213,7 +216,8 @@ def main():
         "config": {"workload": wl["name"], "grid": [NX, NX, NZ], "systems": nsys, "unknowns_per_system": total,
                    "band": [kl, ku], "factors": "real form" if P["real"] else "complex",
                    "factored_blocks": {"per_system": P["blocks"][0], "unknowns": P["blocks"][1],
-                                       "band": [P["blocks"][2], P["blocks"][3]]},
+                                       "band": [P["blocks"][2], P["blocks"][3]],
+                                       "u_entries_per_column": round(P["u_entries"], 3)},
                    "ic_amplitude": amp, "state_finite": finite, "cfl_dt_last_stage": cfl_final},
         "roofline": roofline,
         "cpu_baseline": cpu,

@@ -86,7 +86,7 @@ def _load():
     lib.coral_amd_get_cfl_dt.argtypes = [P, ctypes.POINTER(c_double)]
     lib.coral_amd_wavenumbers.argtypes = [P, P, P, P, P]
     lib.coral_amd_system_info.argtypes = [P, c_int32, P, P, P, P]
-    lib.coral_amd_factor_blocks.argtypes = [P, c_int32, c_int32, P, P, P, P]
+    lib.coral_amd_factor_blocks.argtypes = [P, c_int32, c_int32, P, P, P, P, P]
     lib.coral_amd_bwd_transform.argtypes = [P, P, P]
     lib.coral_amd_fwd_transform.argtypes = [P, P, P]
     lib.coral_amd_banded_solve.argtypes = [c_int32, c_int32, c_int32, c_int32, c_int32, P, P, P]

@@ -69,6 +69,14 @@ struct SysDev {
     int nblk = 1, blk_n = 0, blk_nl = 0, blk_nu = 0;
     DevBuf<int> blk_row, blk_col;
     DevBuf<unsigned> words;      // position words of the 16-lane solve (solve16_words)
+    // trimmed U records of the 16-lane solve (solve16_ucol), rebuilt at every
+    // factorization
+    DevBuf<double> Urt;
+    DevBuf<double2> Uft;
+    DevBuf<int2> ucol;
+    DevBuf<int> ulen;
+    long long ustride = 0;
+    double u_mean = 0.0;         // mean trimmed U record length over the block's columns
     DevBuf<double> Lr, Ur;
     DevBuf<unsigned char> rot;
     DevBuf<unsigned char> piv;
@@ -406,6 +414,28 @@ void factorize(coral_amd_ctx &c, SysDev &d, double dt) {
         a.info = d.info.p;
         c.timed("factor", [&] { launch_factor(c.stream, a); });
     }
+    if (d.ncol16 > 0) {
+        const int bkv = d.blk_nl + d.blk_nu, elem = d.real ? 8 : 16, nb = 16 * ((d.blk_n + 15) / 16);
+        const void *Ufull = d.real ? (const void *)(d.Ur.p + SOLVE16_PAD) : (const void *)(d.Uf.p + SOLVE16_PAD);
+        if (d.ulen.n < size_t(nb)) d.ulen.alloc(nb);
+        std::vector<int2> ucol;
+        solve16_ucol(c.stream, Ufull, elem, d.blk_n, bkv, nmat, d.ulen.p, ucol, &d.ustride);
+        double lsum = 0.0;
+        for (int j = 0; j < d.blk_n; ++j) lsum += ucol[j].y;
+        d.u_mean = lsum / d.blk_n;
+        const size_t need = size_t((nmat + 3) / 4) * size_t(d.ustride);
+        void *Ut;
+        if (d.real) {
+            if (d.Urt.n < need) d.Urt.alloc(need);
+            Ut = d.Urt.p;
+        } else {
+            if (d.Uft.n < need) d.Uft.alloc(need);
+            Ut = d.Uft.p;
+        }
+        HIP_OK(hipMemsetAsync(Ut, 0, need * elem, c.stream));
+        d.ucol.upload(ucol);
+        c.timed("factor", [&] { solve16_trim_u_copy(c.stream, Ufull, elem, d.blk_n, bkv, nmat, d.ucol.p, d.ustride, Ut); });
+    }
     HIP_OK(hipStreamSynchronize(c.stream));
     std::vector<int> info(nmat);
     HIP_OK(hipMemcpy(info.data(), d.info.p, nmat * sizeof(int), hipMemcpyDeviceToHost));
@@ -604,7 +634,9 @@ void one_step(coral_amd_ctx &c) {
                 if (d.ncol16 > 0) {
                     Solve16Args sa{};
                     sa.L16 = d.real ? (const void *)d.Lr.p : (const void *)d.Lf.p;
-                    sa.U = d.real ? (const void *)(d.Ur.p + SOLVE16_PAD) : (const void *)(d.Uf.p + SOLVE16_PAD);
+                    sa.U = d.real ? (const void *)d.Urt.p : (const void *)d.Uft.p;
+                    sa.ucol = d.ucol.p;
+                    sa.ustride = d.ustride;
                     sa.ncol16 = d.ncol16;
                     sa.piv = d.piv.p;
                     sa.words = d.words.p;
@@ -978,7 +1010,7 @@ int coral_amd_factor_kind(coral_amd_ctx *c, int32_t isys, int32_t *is_real) {
 }
 
 int coral_amd_factor_blocks(coral_amd_ctx *c, int32_t kind, int32_t isys, int32_t *nblk, int32_t *blk_n,
-                            int32_t *blk_nl, int32_t *blk_nu) {
+                            int32_t *blk_nl, int32_t *blk_nu, double *u_entries) {
     try {
         if (!c || !nblk || !blk_n || !blk_nl || !blk_nu) throw std::runtime_error("null argument");
         const auto &v = kind == 0 ? c->ks : c->zs;
@@ -988,6 +1020,7 @@ int coral_amd_factor_blocks(coral_amd_ctx *c, int32_t kind, int32_t isys, int32_
         *blk_n = d.blk_n;
         *blk_nl = d.blk_nl;
         *blk_nu = d.blk_nu;
+        if (u_entries) *u_entries = d.ncol16 > 0 ? d.u_mean : double(d.blk_nl + d.blk_nu + 1);
         return 0;
     } catch (const std::exception &e) {
         return fail(e);

@@ -11,6 +11,7 @@
 #include "kernels.hpp"
 
 #include <cstdio>
+#include <stdexcept>
 
 namespace coral {
 
@@ -1426,46 +1427,43 @@ struct Solve16 {
 
     // backward step t (descending) of a block.  Lane t's q = 0 entry of the
     // ring is record offset 0 of column j, i.e. 1/U(j, j); it is broadcast to
-    // the row instead of being loaded separately.  x_j goes to xo; block 0 of
-    // r is updated without a lower mask, since its lanes >= t are finished
-    // (their values live in xo) and are not read again.
-    // Ug: the wavefront's U records less 16 elements (uniform); uofs: the lane's
-    // offset (m(kv+1) - lo + 16 >= 1) of its record of column 0 from Ug
+    // the row instead of being loaded separately.  x_j goes to xo.  Entries
+    // outside a column's record were loaded from the group's zeros, so the
+    // updates need no masks; block 0's lanes >= t are finished (their values
+    // live in xo) and may take any update.
+    // Ug: the wavefront's trimmed U records (uniform); ucol: per column (offset
+    // of its four records in the group, record length); m: the lane's matrix
+    // slot.  Entry e = t - lo + 16q of column j is U(j-1-k, j), k = e - 1.
     template <int t>
     __device__ static inline void bwd_step(double2 (&r)[NQ], double2 &xo, T (&cring)[PFB][NQ], const T *Ug,
-                                           unsigned uofs, int lo, int kv, int jnext) {
-        const T zero = Fx<T>::zero();
+                                           const int2 *ucol, int m, int lo, int jnext) {
         constexpr int slot = t % PFB;
         const double2 xj = scale(bcast16_c<t>(r[0]), bcast16<t>(cring[slot][0]));
         xo = csel(lo == t, xj, xo);
 #pragma unroll
-        for (int q = 0; q < NQ; ++q) {
-            // entry k = t - 1 - lo + 16q of column j; rows with k >= kv are
-            // outside the band (k >= 0 holds for q >= 1)
-            if (q == 0 && NQ > 2) {
-                r[q] = submul(r[q], cring[slot][q], xj);
-            } else {
-                const bool valid = lo > t - 1 + 16 * q - kv;
-                r[q] = submul(r[q], tsel(valid, cring[slot][q], zero), xj);
-            }
-        }
+        for (int q = 0; q < NQ; ++q) r[q] = submul(r[q], cring[slot][q], xj);
         // prefetch step jnext = j - PFB (its position in the block: (t - PFB) mod
         // 16); steps before column 0 are never used and load column 0
         constexpr int tn = (t - PFB) & 15;
-        unsigned o = uofs + unsigned(max(jnext, 0)) * unsigned(4 * (kv + 1));
+        const int2 cl = ucol[max(jnext, 0)];   // uniform
+        int lb = cl.x + m * cl.y;
 #pragma unroll
-        for (int q = 0; q < NQ; ++q) asm volatile("" : "+v"(o) : "v"(r[q].x), "v"(r[q].y));
-        asm volatile("" : "+v"(o) : "v"(xo.x), "v"(xo.y));
+        for (int q = 0; q < NQ; ++q) asm volatile("" : "+v"(lb) : "v"(r[q].x), "v"(r[q].y));
+        asm volatile("" : "+v"(lb) : "v"(xo.x), "v"(xo.y));
 #pragma unroll
-        for (int q = 0; q < NQ; ++q) cring[slot][q] = Ug[(size_t)o + tn + 16 * q];
+        for (int q = 0; q < NQ; ++q) {
+            const int e = tn + 16 * q - lo;
+            const unsigned off = unsigned(e) < unsigned(cl.y) ? unsigned(lb + e) : 0u;
+            cring[slot][q] = Ug[off];
+        }
         __builtin_amdgcn_sched_barrier(0);   // and not later: the prefetch distance stays PFB steps
     }
 
     template <int t>
     __device__ static inline void bwd_block(double2 (&r)[NQ], double2 &xo, T (&cring)[PFB][NQ], const T *Ug,
-                                            unsigned uofs, int lo, int kv, int j0) {
-        bwd_step<t>(r, xo, cring, Ug, uofs, lo, kv, j0 + t - PFB);
-        if constexpr (t > 0) bwd_block<t - 1>(r, xo, cring, Ug, uofs, lo, kv, j0);
+                                            const int2 *ucol, int m, int lo, int j0) {
+        bwd_step<t>(r, xo, cring, Ug, ucol, m, lo, j0 + t - PFB);
+        if constexpr (t > 0) bwd_block<t - 1>(r, xo, cring, Ug, ucol, m, lo, j0);
     }
 };
 
@@ -1479,6 +1477,11 @@ __global__ __launch_bounds__(64) void k_solve16(Solve16Args a) {
     const int lane = threadIdx.x, l = lane & 15, rowbase = lane & 48;
     const int wstride = n + 2 * SOLVE16_WPAD;
     for (int e = lane; e < a.nblk * wstride; e += 64) sW[e] = a.words[e];
+    // U column table after the words (read with uniform addresses in the
+    // backward sweep; in LDS, so that those reads never wait for global loads)
+    const int nbc = 16 * ((n + 15) >> 4);
+    int2 *sU = reinterpret_cast<int2 *>(sW + ((a.nblk * wstride + 1) & ~1));
+    for (int e = lane; e < nbc; e += 64) sU[e] = a.ucol[e];
     // factored matrix (structural block) of this 16-lane row
     const int nmat = a.nsys * a.nblk;
     const int sg = blockIdx.x * 4 + (lane >> 4);
@@ -1543,10 +1546,9 @@ __global__ __launch_bounds__(64) void k_solve16(Solve16Args a) {
     __threadfence_block();
 
     // ---- backward ----
-    // lane's record of column j: Ug + uofs + l + 4j(kv+1) (entry 0: 1/U(j,j));
-    // Ug + uofs + 4j(kv+1) + t + 16q: U(j-1-k, j), k = t-1-l+16q
-    const T *Ug = Ur + (size_t)g * B * 64 * (kv + 1) - 16;
-    const unsigned uofs = unsigned(m * (kv + 1) - l + 16);
+    // trimmed U records of the wavefront's group (see Solve16Args::U)
+    const T *Ug = Ur + (size_t)g * a.ustride;
+    const int2 *ucol = sU;
     double2 r[NQ];
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
@@ -1559,9 +1561,12 @@ __global__ __launch_bounds__(64) void k_solve16(Solve16Args a) {
         // step j = 16B - 1 - u lives in slot (15 - u) % PFB
         const int j = B * 16 - 1 - u;
         const int t = j & 15;
-        const unsigned o = uofs + unsigned(j) * unsigned(4 * (kv + 1));
+        const int2 cl = ucol[j];
 #pragma unroll
-        for (int q = 0; q < NQ; ++q) cring[(15 - u) % PFB][q] = Ug[(size_t)o + t + 16 * q];
+        for (int q = 0; q < NQ; ++q) {
+            const int e = t + 16 * q - l;
+            cring[(15 - u) % PFB][q] = Ug[unsigned(e) < unsigned(cl.y) ? unsigned(cl.x + m * cl.y + e) : 0u];
+        }
         // issued in the order of use: the loop's first wait then counts the
         // same way whether it is entered from here or from its own end
         __builtin_amdgcn_sched_barrier(0);
@@ -1574,7 +1579,7 @@ __global__ __launch_bounds__(64) void k_solve16(Solve16Args a) {
         // block bk-NQ, which enters the window of blocks being updated
         const unsigned wy = W[(bk - NQ) * 16 + l];
         const double2 ynx = y[col_or0(wy)];
-        S::template bwd_block<15>(r, xo, cring, Ug, uofs, lo, kv, bk * 16);
+        S::template bwd_block<15>(r, xo, cring, Ug, ucol, m, lo, bk * 16);
         const unsigned w0 = W[bk * 16 + l];
         *out_ptr(w0) = rot_out(xo, code(w0));
 #pragma unroll
@@ -1585,11 +1590,11 @@ __global__ __launch_bounds__(64) void k_solve16(Solve16Args a) {
 
 template <typename T>
 static bool launch_solve16(hipStream_t st, const Solve16Args &a) {
-    if (a.kl > 16 || a.n < 32 || a.ncol16 < 16 * ((a.n + 15) / 16 + 1)) return false;
-    // reads reach 16 elements before the first record and 64 past the last
+    if (a.kl > 16 || a.n < 32 || a.ncol16 < 16 * ((a.n + 15) / 16 + 1) || !a.ucol) return false;
     const int nq = 1 + (a.kv + 15) / 16;
     const dim3 grid((a.nsys * a.nblk + 3) / 4);
-    const size_t lds = size_t(a.nblk) * (a.n + 2 * SOLVE16_WPAD) * sizeof(unsigned);
+    const size_t lds = (size_t(a.nblk) * (a.n + 2 * SOLVE16_WPAD) + 1) / 2 * 2 * sizeof(unsigned) +
+                       size_t(16 * ((a.n + 15) / 16)) * sizeof(int2);
     if (lds > 32768 || !a.sink) return false;
     switch (nq) {
     case 2: hipLaunchKernelGGL((k_solve16<T, 2, 8>), grid, dim3(64), lds, st, a); break;
@@ -1616,6 +1621,75 @@ std::vector<unsigned> solve16_words(int total, int nblk, int blk_n, const int *b
             w[size_t(b) * stride + SOLVE16_WPAD + i] = v;
         }
     return w;
+}
+
+// record length of column j: one past the last nonzero entry over all matrices
+template <typename T>
+__global__ void k_ucol_len(const T *__restrict__ U, int n, int kv, int nmat, int *len) {
+    const int nb = 16 * ((n + 15) >> 4);
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long long)nmat * nb) return;
+    const int M = int(idx / nb), j = int(idx % nb);
+    const T *rec = U + ((size_t)((M >> 2) * nb + j) * 4 + (M & 3)) * (kv + 1);
+    int last = 0;
+    for (int e = kv; e > 0; --e)
+        if (!Fx<T>::is_zero(rec[e])) {
+            last = e;
+            break;
+        }
+    if (last > 0) atomicMax(len + j, last + 1);
+}
+
+template <typename T>
+__global__ void k_u_trim(const T *__restrict__ U, int n, int kv, int nmat, const int2 *__restrict__ ucol,
+                         long long ustride, T *__restrict__ Ut) {
+    const int nb = 16 * ((n + 15) >> 4);
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long long)nmat * nb) return;
+    const int M = int(idx / nb), j = int(idx % nb), g = M >> 2, m = M & 3;
+    const T *rec = U + ((size_t)(g * nb + j) * 4 + m) * (kv + 1);
+    const int2 cl = ucol[j];
+    T *out = Ut + (size_t)g * ustride + cl.x + m * cl.y;
+    for (int e = 0; e < cl.y; ++e) out[e] = rec[e];
+}
+
+void solve16_ucol(hipStream_t st, const void *Ufull, int elem, int n, int kv, int nmat, int *scratch_len,
+                  std::vector<int2> &ucol, long long *ustride) {
+    const int nb = 16 * ((n + 15) >> 4);
+    const unsigned grid = unsigned(((long long)nmat * nb + 255) / 256);
+    std::vector<int> len(nb, 1);   // the diagonal entry at least
+    if (hipMemcpyAsync(scratch_len, len.data(), nb * sizeof(int), hipMemcpyHostToDevice, st) != hipSuccess)
+        throw std::runtime_error("upload of U column lengths failed");
+    if (elem == 8)
+        hipLaunchKernelGGL(k_ucol_len<double>, dim3(grid), dim3(256), 0, st, (const double *)Ufull, n, kv, nmat,
+                           scratch_len);
+    else
+        hipLaunchKernelGGL(k_ucol_len<double2>, dim3(grid), dim3(256), 0, st, (const double2 *)Ufull, n, kv, nmat,
+                           scratch_len);
+    CHECK_LAUNCH();
+    if (hipMemcpyAsync(len.data(), scratch_len, nb * sizeof(int), hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess)
+        throw std::runtime_error("download of U column lengths failed");
+    ucol.assign(nb, make_int2(0, 0));
+    long long off = 16;   // [0, 16) of every group stays zero
+    for (int j = 0; j < nb; ++j) {
+        ucol[j] = make_int2(int(off), len[j]);
+        off += 4LL * len[j];
+    }
+    *ustride = off;
+}
+
+void solve16_trim_u_copy(hipStream_t st, const void *Ufull, int elem, int n, int kv, int nmat, const int2 *ucol_dev,
+                         long long ustride, void *Utrim) {
+    const int nb = 16 * ((n + 15) >> 4);
+    const unsigned grid = unsigned(((long long)nmat * nb + 255) / 256);
+    if (elem == 8)
+        hipLaunchKernelGGL(k_u_trim<double>, dim3(grid), dim3(256), 0, st, (const double *)Ufull, n, kv, nmat, ucol_dev,
+                           ustride, (double *)Utrim);
+    else
+        hipLaunchKernelGGL(k_u_trim<double2>, dim3(grid), dim3(256), 0, st, (const double2 *)Ufull, n, kv, nmat,
+                           ucol_dev, ustride, (double2 *)Utrim);
+    CHECK_LAUNCH();
 }
 
 int solve16_ncol(int n, int kl, int kv) {

@@ -223,7 +223,12 @@ constexpr int SOLVE16_PAD = 2048;
 // factors in the layout k_factor writes for ncol16 > 0.
 struct Solve16Args {
     const void *L16;             // double (real form) or double2
-    const void *U;               // past SOLVE16_PAD elements of padding
+    // U records trimmed per column (solve16_trim_u): for wavefront group g,
+    // column j, matrix slot m and entry e < len_j at U + g*ustride + cb_j +
+    // m*len_j + e, with ucol[j] = (cb_j, len_j); U + g*ustride + [0, 16) is zero
+    const void *U;
+    const int2 *ucol;
+    long long ustride;
     int ncol16;
     const unsigned char *piv;    // [nsys*nblk][16*ceil(n/16)], zero past n
     // position words (solve16_words): row and column of the system at each
@@ -236,6 +241,17 @@ struct Solve16Args {
     double2 *x;
     double2 *sink;               // 64 elements of scratch that absorb stores of padding positions
 };
+// Trimmed U records of the 16-lane layout: the full records of kv+1 entries
+// at Ufull (as k_factor writes them, nmat matrices of n unknowns, elem bytes
+// per entry) are cut to the longest nonzero prefix each column has in any
+// matrix.  solve16_ucol fills ucol (16*ceil(n/16) columns: offset of the
+// column's four records in a group, record length) and the group stride
+// (scratch_len: that many ints of device memory); solve16_trim_u_copy then
+// repacks into Utrim (ngroups * ustride elements, cleared by the caller).
+void solve16_ucol(hipStream_t st, const void *Ufull, int elem, int n, int kv, int nmat, int *scratch_len,
+                  std::vector<int2> &ucol, long long *ustride);
+void solve16_trim_u_copy(hipStream_t st, const void *Ufull, int elem, int n, int kv, int nmat, const int2 *ucol_dev,
+                         long long ustride, void *Utrim);
 // "no row / column" in a position word (padding of a block), and the word of
 // a position outside the block
 constexpr int SOLVE16_NONE = 0x7FFF;

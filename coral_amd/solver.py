@@ -235,9 +235,18 @@ class Solver:
         """(nblk, blk_n, blk_nl, blk_nu): structural blocks factored per system
         (kind 0 = kxky, 1 = mean); nblk = 1 when whole systems are factored."""
         v = [ctypes.c_int32() for _ in range(4)]
-        check(lib.coral_amd_factor_blocks(self._ctx, kind, isys, *[ctypes.byref(x) for x in v]),
+        check(lib.coral_amd_factor_blocks(self._ctx, kind, isys, *[ctypes.byref(x) for x in v], None),
               "coral_amd_factor_blocks")
         return tuple(x.value for x in v)
+
+    def u_entries_per_column(self, isys: int = 0, kind: int = 0) -> float:
+        """Mean number of U entries per column the solve reads (records cut to
+        their longest nonzero prefix over all wavenumbers; after factorization)."""
+        v = [ctypes.c_int32() for _ in range(4)]
+        u = ctypes.c_double()
+        check(lib.coral_amd_factor_blocks(self._ctx, kind, isys, *[ctypes.byref(x) for x in v], ctypes.byref(u)),
+              "coral_amd_factor_blocks")
+        return u.value
 
     def bwd_transform(self, spec: np.ndarray) -> np.ndarray:
         s = np.ascontiguousarray(spec, np.complex128)

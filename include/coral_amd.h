@@ -155,9 +155,12 @@ int coral_amd_factor_kind(coral_amd_ctx *ctx, int32_t isys, int32_t *is_real);
  * system (zgbtrf, PL_algebra_z_3D.f90:245-276): partial pivoting never selects
  * a structurally zero row, so every row receives the same eliminations in the
  * same order.  CORAL_AMD_NO_BLOCK_SPLIT=1 before coral_amd_create keeps whole
- * systems. */
+ * systems.
+ * u_entries (may be NULL) receives the mean number of U entries per column the
+ * solve reads after the last factorization: each column's record is cut to
+ * the longest nonzero prefix it has over all wavenumbers (diagonal included). */
 int coral_amd_factor_blocks(coral_amd_ctx *ctx, int32_t kind, int32_t isys, int32_t *nblk, int32_t *blk_n,
-                            int32_t *blk_nl, int32_t *blk_nu);
+                            int32_t *blk_nl, int32_t *blk_nu, double *u_entries);
 
 /* Physical values of coupled variable ivar of system isys in the current state:
  * the volumes Coral writes to CheckPoints/QuickSave.{kxky,zero}.sysNNN.varNNN.rollingR

@@ -428,6 +428,9 @@ def test_structural_blocks_match_whole_system_factors(solver_mod, monkeypatch):
                 assert nblk == 2 and total <= 2 * blk_n < total + 8 and blk_nl < nl and blk_nu < nu
             g.set_state(k0, z0)
             g.march_forward(dt, nsteps)
+            # U records are cut to their longest nonzero prefix over all wavenumbers
+            ue = g.u_entries_per_column(0)
+            assert 1.0 <= ue <= blk_nl + blk_nu + 1
             out.append(g.get_state())
         finally:
             g.close()

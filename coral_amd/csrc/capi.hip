@@ -458,6 +458,8 @@ void rhs(coral_amd_ctx &c, SysDev &d, const std::vector<std::pair<const double2 
     a.cstif = d.cstif.p;
     a.total = t.total;
     a.nsys = d.nsys;
+    a.bl = t.nl;
+    a.bu = t.nu;
     if (ms.size() > 8 || xs.size() > 8 || ks.size() > 8) throw std::runtime_error("too many IMEX stages");
     a.nm = int(ms.size());
     for (size_t k = 0; k < ms.size(); ++k) {

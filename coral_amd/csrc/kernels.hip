@@ -797,26 +797,46 @@ void launch_nl(hipStream_t st, const NLArgs &a) {
 // PL_IMEX_timestepping_marching.f90:93-171): one thread per (system, row),
 // operators as ELL rows of the merged groups
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_rhs(RhsArgs a) {
-    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= (long long)a.nsys * a.total) return;
-    const int s = int(idx / a.total), i = int(idx - (long long)s * a.total);
+// A workgroup covers RHS_ROWS consecutive rows of one system: the combined
+// vectors X = sum_k ma_k mv_k and V = sum_k xa_k xv_k are staged once in LDS
+// over the rows and the operator band around them, so the ELL products read
+// LDS instead of issuing one global load per entry and vector.
+constexpr int RHS_ROWS = 256;
+
+__global__ __launch_bounds__(RHS_ROWS) void k_rhs(RhsArgs a) {
+    extern __shared__ double2 xs[];   // [2][RHS_ROWS + bl + bu]
+    const int nchunk = (a.total + RHS_ROWS - 1) / RHS_ROWS;
+    const int s = blockIdx.x / nchunk, i0 = (blockIdx.x - s * nchunk) * RHS_ROWS;
+    const int lo = max(i0 - a.bl, 0), hi = min(i0 + RHS_ROWS + a.bu, a.total), span = RHS_ROWS + a.bl + a.bu;
     const size_t base = (size_t)s * a.total;
+    double2 *X = xs, *V = xs + span;
+    for (int c = lo + threadIdx.x; c < hi; c += blockDim.x) {
+        double2 u = make_double2(0.0, 0.0), v = make_double2(0.0, 0.0);
+        for (int k = 0; k < a.nm; ++k) {
+            const double2 t = a.mv[k][base + c];
+            u.x = fma(a.ma[k], t.x, u.x);
+            u.y = fma(a.ma[k], t.y, u.y);
+        }
+        for (int k = 0; k < a.nx; ++k) {
+            const double2 t = a.xv[k][base + c];
+            v.x = fma(a.xa[k], t.x, v.x);
+            v.y = fma(a.xa[k], t.y, v.y);
+        }
+        X[c - lo] = u;
+        V[c - lo] = v;
+    }
+    __syncthreads();
+    const int i = i0 + threadIdx.x;
+    if (i >= a.total) return;
     double2 y = make_double2(0.0, 0.0);
     for (int g = 0; g < a.mass.ngroups; ++g) {
         const int W = a.mass.ell_w[g], off = a.mass.ell_off[g] + i * W;
         double2 sum = make_double2(0.0, 0.0);
         for (int e = 0; e < W; ++e) {
             const double bv = a.mass.ell_val[off + e];
-            const int col = a.mass.ell_col[off + e];
-            double2 u = make_double2(0.0, 0.0);
-            for (int k = 0; k < a.nm; ++k) {
-                const double2 t = a.mv[k][base + col];
-                u.x += a.ma[k] * t.x;
-                u.y += a.ma[k] * t.y;
-            }
-            sum.x += bv * u.x;
-            sum.y += bv * u.y;
+            const double2 u = X[a.mass.ell_col[off + e] - lo];
+            sum.x = fma(bv, u.x, sum.x);
+            sum.y = fma(bv, u.y, sum.y);
         }
         y = cadd(y, cmul(a.cmass[(size_t)s * a.mass.ngroups + g], sum));
     }
@@ -826,30 +846,27 @@ __global__ __launch_bounds__(256) void k_rhs(RhsArgs a) {
             double2 sum = make_double2(0.0, 0.0);
             for (int e = 0; e < W; ++e) {
                 const double bv = a.stif.ell_val[off + e];
-                const int col = a.stif.ell_col[off + e];
-                double2 v = make_double2(0.0, 0.0);
-                for (int k = 0; k < a.nx; ++k) {
-                    const double2 t = a.xv[k][base + col];
-                    v.x += a.xa[k] * t.x;
-                    v.y += a.xa[k] * t.y;
-                }
-                sum.x += bv * v.x;
-                sum.y += bv * v.y;
+                const double2 v = V[a.stif.ell_col[off + e] - lo];
+                sum.x = fma(bv, v.x, sum.x);
+                sum.y = fma(bv, v.y, sum.y);
             }
             y = cadd(y, cmul(a.cstif[(size_t)s * a.stif.ngroups + g], sum));
         }
     }
+    const size_t idx = base + i;
     for (int k = 0; k < a.nk; ++k) {
         const double2 t = a.kv[k][idx];
-        y.x += a.kb[k] * t.x;
-        y.y += a.kb[k] * t.y;
+        y.x = fma(a.kb[k], t.x, y.x);
+        y.y = fma(a.kb[k], t.y, y.y);
     }
     if (a.zero00 && s == 0) y = make_double2(0.0, 0.0);
     a.y[idx] = y;
 }
 
 void launch_rhs(hipStream_t st, const RhsArgs &a) {
-    hipLaunchKernelGGL(k_rhs, dim3(blocks_for((long long)a.nsys * a.total, 256)), dim3(256), 0, st, a);
+    const int nchunk = (a.total + RHS_ROWS - 1) / RHS_ROWS;
+    const size_t shm = 2 * size_t(RHS_ROWS + a.bl + a.bu) * sizeof(double2);
+    hipLaunchKernelGGL(k_rhs, dim3((unsigned)((long long)a.nsys * nchunk)), dim3(RHS_ROWS), shm, st, a);
     CHECK_LAUNCH();
 }
 

@@ -160,6 +160,7 @@ struct RhsArgs {
     OpDev mass, stif;
     const double2 *cmass, *cstif;  // [nsys][ngroups] group coefficients
     int total, nsys;
+    int bl, bu;                    // band of the operators: ELL columns of row i lie in [i - bl, i + bu]
     int nm;
     const double2 *mv[8];
     double ma[8];

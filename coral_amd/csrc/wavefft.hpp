@@ -122,10 +122,36 @@ __device__ inline double2 twiddle(const double2 *__restrict__ tw, int m) {
     return SIGN > 0 ? make_double2(w.x, -w.y) : w;
 }
 
+// LDS bank swizzles (used by runs that opt in with SW).  A wavefront's
+// 16-byte accesses are served eight lanes
+// at a time, conflict-free when the eight elements fall in distinct 16-byte
+// slots of a 128-byte row (element index mod 8).  Reads of a stage are runs
+// of eight consecutive elements (conflict-free under any permutation inside
+// rows of eight); the writes of the early stages are strided.  Each stage
+// therefore stores element e of a sequence at swz<W>(e), W chosen for its
+// write pattern, and the next stage reads through the same map:
+//   W = 1: stride-4 writes (radix 4, NS = 1):  slot ^= row & 3
+//   W = 2: radix 4 after NS = 4:               slot ^= 4 * ((row >> 1) & 1)
+//   W = 3: stride-6 writes (radix 6, NS = 1):  slot ^= row & 1
+// (row = e / 8); other stages write whole rows and keep W = 0.
+template <int W>
+__device__ inline int swz(int e) {
+    if constexpr (W == 1) return e ^ ((e >> 3) & 3);
+    else if constexpr (W == 2) return e ^ (((e >> 4) & 1) << 2);
+    else if constexpr (W == 3) return e ^ ((e >> 3) & 1);
+    else return e;
+}
+// swizzle a stage with radix R after NS points writes with
+template <int R, int NS>
+constexpr int stage_swz() { return (NS == 1 && R == 4) ? 1 : (NS == 4 && R == 4) ? 2 : (NS == 1 && R == 6) ? 3 : 0; }
+
 // One LDS -> LDS stage over S sequences (sequence s at buf + s*LS); the
-// S*T butterflies are spread over the 64 lanes round by round.
-template <int N, int R, int NS, int SIGN, int S, int LS>
+// S*T butterflies are spread over the 64 lanes round by round.  The input was
+// written with swizzle WI, the output is written with stage_swz<R, NS>() when
+// SW is set, unswizzled otherwise.
+template <int N, int R, int NS, int SIGN, int S, int LS, int WI = 0, bool SW = false>
 __device__ inline void stage_lds(double2 *buf, const double2 *__restrict__ tw, int lane) {
+    constexpr int WO = SW ? stage_swz<R, NS>() : 0;
     constexpr int T = N / R;
     constexpr int TOT = S * T;
     constexpr int ROUNDS = (TOT + 63) / 64;
@@ -140,7 +166,7 @@ __device__ inline void stage_lds(double2 *buf, const double2 *__restrict__ tw, i
             const int jm = j % NS;
 #pragma unroll
             for (int r = 0; r < R; ++r) {
-                double2 x = L[j + r * T];
+                double2 x = L[swz<WI>(j + r * T)];
                 if (NS > 1 && r > 0) x = cmul(x, twiddle<SIGN>(tw, jm * r * (N / (NS * R))));
                 v[c][r] = x;
             }
@@ -156,30 +182,40 @@ __device__ inline void stage_lds(double2 *buf, const double2 *__restrict__ tw, i
             double2 *L = buf + s * LS;
             const int base = (j / NS) * NS * R + j % NS;
 #pragma unroll
-            for (int r = 0; r < R; ++r) L[base + r * NS] = v[c][r];
+            for (int r = 0; r < R; ++r) L[swz<WO>(base + r * NS)] = v[c][r];
         }
     }
     wave_sync();
 }
 
-// A run of LDS stages with radices Rs..., starting after NS points; sequences
-// LS elements apart.
-template <int N, int SIGN, int S, int LS, int NS, int... Rs>
-struct StagesLS;
+// A run of LDS stages with radices Rs..., starting after NS points on input
+// written with swizzle WI; sequences LS elements apart; SW: the stages write
+// swizzled.  OUT_SWZ is the swizzle of the run's output (what its reader must
+// apply).
+template <int N, int SIGN, int S, int LS, int WI, bool SW, int NS, int... Rs>
+struct StagesLSW;
 
-template <int N, int SIGN, int S, int LS, int NS>
-struct StagesLS<N, SIGN, S, LS, NS> {
+template <int N, int SIGN, int S, int LS, int WI, bool SW, int NS>
+struct StagesLSW<N, SIGN, S, LS, WI, SW, NS> {
+    static constexpr int OUT_SWZ = WI;
     __device__ static inline void run(double2 *, const double2 *__restrict__, int) {}
 };
 
-template <int N, int SIGN, int S, int LS, int NS, int R, int... Rs>
-struct StagesLS<N, SIGN, S, LS, NS, R, Rs...> {
+template <int N, int SIGN, int S, int LS, int WI, bool SW, int NS, int R, int... Rs>
+struct StagesLSW<N, SIGN, S, LS, WI, SW, NS, R, Rs...> {
+    using Next = StagesLSW<N, SIGN, S, LS, (SW ? stage_swz<R, NS>() : 0), SW, NS * R, Rs...>;
+    static constexpr int OUT_SWZ = Next::OUT_SWZ;
     __device__ static inline void run(double2 *buf, const double2 *__restrict__ tw, int lane) {
-        stage_lds<N, R, NS, SIGN, S, LS>(buf, tw, lane);
-        StagesLS<N, SIGN, S, LS, NS * R, Rs...>::run(buf, tw, lane);
+        stage_lds<N, R, NS, SIGN, S, LS, WI, SW>(buf, tw, lane);
+        Next::run(buf, tw, lane);
     }
 };
 
+// runs on unswizzled input, without (StagesLS) or with (StagesLSZ) swizzled writes
+template <int N, int SIGN, int S, int LS, int NS, int... Rs>
+using StagesLS = StagesLSW<N, SIGN, S, LS, 0, false, NS, Rs...>;
+template <int N, int SIGN, int S, int LS, int NS, int... Rs>
+using StagesLSZ = StagesLSW<N, SIGN, S, LS, 0, true, NS, Rs...>;
 // contiguous sequences (LS = N)
 template <int N, int SIGN, int S, int NS, int... Rs>
 using Stages = StagesLS<N, SIGN, S, N, NS, Rs...>;

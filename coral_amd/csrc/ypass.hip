@@ -13,7 +13,8 @@
 
 namespace coral {
 
-using wfft::StagesLS;
+using wfft::StagesLSZ;
+using wfft::swz;
 
 constexpr int YL = 8;        // kx columns per workgroup
 constexpr int YWAVES = 4;    // wavefronts per workgroup (YL / YWAVES lines each)
@@ -22,19 +23,19 @@ template <int N>
 struct YPlan;
 template <>
 struct YPlan<96> {
-    template <int SIGN, int S, int LS> using Run = StagesLS<96, SIGN, S, LS, 1, 4, 4, 6>;
+    template <int SIGN, int S, int LS> using Run = StagesLSZ<96, SIGN, S, LS, 1, 4, 4, 6>;
 };
 template <>
 struct YPlan<192> {
-    template <int SIGN, int S, int LS> using Run = StagesLS<192, SIGN, S, LS, 1, 4, 4, 4, 3>;
+    template <int SIGN, int S, int LS> using Run = StagesLSZ<192, SIGN, S, LS, 1, 4, 4, 4, 3>;
 };
 template <>
 struct YPlan<384> {
-    template <int SIGN, int S, int LS> using Run = StagesLS<384, SIGN, S, LS, 1, 4, 4, 4, 6>;
+    template <int SIGN, int S, int LS> using Run = StagesLSZ<384, SIGN, S, LS, 1, 4, 4, 4, 6>;
 };
 template <>
 struct YPlan<768> {
-    template <int SIGN, int S, int LS> using Run = StagesLS<768, SIGN, S, LS, 1, 8, 4, 4, 6>;
+    template <int SIGN, int S, int LS> using Run = StagesLSZ<768, SIGN, S, LS, 1, 8, 4, 4, 6>;
 };
 
 // kept ky index jy -> slot of the NYAA-long line
@@ -68,7 +69,7 @@ __global__ __launch_bounds__(64 * YWAVES) void k_ybwd_w(const double2 *__restric
     double2 *dst = G + bz * N * nkx + ix0;
     for (int e = threadIdx.x; e < N * YL; e += blockDim.x) {
         const int y = e / YL, l = e - y * YL;
-        if (l < nl) dst[(size_t)y * nkx + l] = buf[l * LS + y];
+        if (l < nl) dst[(size_t)y * nkx + l] = buf[l * LS + swz<YPlan<N>::template Run<+1, LPW, LS>::OUT_SWZ>(y)];
     }
 }
 
@@ -95,7 +96,7 @@ __global__ __launch_bounds__(64 * YWAVES) void k_yfwd_w(const double2 *__restric
     double2 *dst = Hy + bz * nsys + (size_t)ix0 * nky;
     for (int e = threadIdx.x; e < nl * nky; e += blockDim.x) {
         const int l = e / nky, jy = e - l * nky;
-        dst[e] = buf[l * LS + ky_slot(jy, NY, N)];
+        dst[e] = buf[l * LS + swz<YPlan<N>::template Run<-1, LPW, LS>::OUT_SWZ>(ky_slot(jy, NY, N))];
     }
 }
 

@@ -122,6 +122,16 @@ __device__ inline double2 twiddle(const double2 *__restrict__ tw, int m) {
     return SIGN > 0 ? make_double2(w.x, -w.y) : w;
 }
 
+// p[r] = w^r for r = 1..R-1, by repeated products: one table load per
+// butterfly instead of R-1 (the twiddle loads share the vector memory path
+// with the HBM traffic); the product chain adds a few ulp at most
+template <int R>
+__device__ inline void powers(double2 w, double2 (&p)[R]) {
+    p[1] = w;
+#pragma unroll
+    for (int r = 2; r < R; ++r) p[r] = cmul(p[r - 1], w);
+}
+
 // LDS bank swizzles (used by runs that opt in with SW).  A wavefront's
 // 16-byte accesses are served eight lanes
 // at a time, conflict-free when the eight elements fall in distinct 16-byte
@@ -164,10 +174,12 @@ __device__ inline void stage_lds(double2 *buf, const double2 *__restrict__ tw, i
             const int s = b / T, j = b - s * T;
             const double2 *L = buf + s * LS;
             const int jm = j % NS;
+            double2 wp[R];
+            if constexpr (NS > 1) powers<R>(twiddle<SIGN>(tw, jm * (N / (NS * R))), wp);
 #pragma unroll
             for (int r = 0; r < R; ++r) {
                 double2 x = L[swz<WI>(j + r * T)];
-                if (NS > 1 && r > 0) x = cmul(x, twiddle<SIGN>(tw, jm * r * (N / (NS * R))));
+                if (NS > 1 && r > 0) x = cmul(x, wp[r]);
                 v[c][r] = x;
             }
             Dft<R, SIGN>::run(v[c]);

@@ -39,9 +39,9 @@ struct XPlan<192> {
 };
 template <>
 struct XPlan<384> {
-    static constexpr int R0 = 4, RL = 6, LPW = 1;
-    template <int S> using InvMid = Stages<384, +1, S, 4, 4, 4>;
-    template <int S> using FwdRest = Stages<384, -1, S, 6, 4, 4, 4>;
+    static constexpr int R0 = 8, RL = 6, LPW = 1;
+    template <int S> using InvMid = Stages<384, +1, S, 8, 8>;
+    template <int S> using FwdRest = Stages<384, -1, S, 6, 8, 8>;
 };
 template <>
 struct XPlan<768> {
@@ -57,7 +57,7 @@ struct XShape {
     static constexpr int LPW = P::LPW, R0 = P::R0, RL = P::RL;
     static constexpr int T0 = N / R0, TL = N / RL;
     static constexpr int NT = LPW * TL / 64;  // (line, j) tasks per lane at the register stages
-    static constexpr int SIN = LPW * NPI;     // inverse sequences per wavefront
+    static constexpr int SIN = LPW;           // inverse sequences in LDS at once (one pair per line)
     static constexpr int SF = LPW * XGP;      // forward sequences per batch
     static constexpr int SMAX = SIN > SF ? SIN : SF;
     static_assert((LPW * TL) % 64 == 0, "register stage must fill the wavefront");
@@ -169,70 +169,74 @@ __global__ __launch_bounds__(256) void k_xpass_wave(XPassArgs a, const double2 *
     const long long line0 = ((long long)blockIdx.x * (blockDim.x >> 6) + wid) * LPW;
     const int nkx = a.nkx;
 
-    // ---- inverse, first stage: packed Hermitian spectra straight from HBM ----
-    {
-        constexpr int TOT = SIN * T0, ROUNDS = (TOT + 63) / 64;
-        double2 v[ROUNDS][R0];
-#pragma unroll
-        for (int c = 0; c < ROUNDS; ++c) {
-            const int b = lane + 64 * c;
-            if (TOT % 64 == 0 || b < TOT) {
-                const int s = b / T0, j = b - s * T0, l = s / NPI, p = s - l * NPI;
-                const long long line = line0 + l;
-                const bool valid = line < nlines;
-                const int f0 = 2 * p, f1 = 2 * p + 1;
-                const bool has1 = f1 < a.nfull;
-                const size_t lo = valid ? (size_t)line : 0;
-                const double2 *g0 = a.G + ((size_t)f0 * nlines + lo) * nkx;
-                const double2 *g1 = a.G + ((size_t)(has1 ? f1 : f0) * nlines + lo) * nkx;
-#pragma unroll
-                for (int r = 0; r < R0; ++r) {
-                    const int e = j + r * T0;
-                    const bool pos = e < nkx, neg = e > N - nkx;
-                    const int k = pos ? e : (neg ? N - e : 0);
-                    double2 A = g0[k], B = g1[k];
-                    const double bx = has1 ? B.x : 0.0, by = (has1 && e != 0) ? B.y : 0.0;
-                    const double ay = e != 0 ? A.y : 0.0;
-                    // pos: A + iB;  neg: conj(A) + i conj(B)
-                    double zx = pos ? A.x - by : A.x + by;
-                    double zy = pos ? ay + bx : bx - ay;
-                    const bool keep = valid && (pos || neg);
-                    v[c][r] = make_double2(keep ? zx : 0.0, keep ? zy : 0.0);
-                }
-                Dft<R0, +1>::run(v[c]);
-            }
-        }
-#pragma unroll
-        for (int c = 0; c < ROUNDS; ++c) {
-            const int b = lane + 64 * c;
-            if (TOT % 64 == 0 || b < TOT) {
-                const int s = b / T0, j = b - s * T0;
-#pragma unroll
-                for (int r = 0; r < R0; ++r) buf[s * N + j * R0 + r] = v[c][r];
-            }
-        }
-        wfft::wave_sync();
-    }
-    XPlan<N>::template InvMid<SIN>::run(buf, tw, lane);
-
-    // ---- inverse, last stage into registers: fv[task][pair][r] at x = j + r*TL ----
+    // ---- inverse transforms, one packed pair of full variables at a time
+    // (half the LDS of both pairs at once, which buys a wavefront per SIMD);
+    // the last stage leaves fv[task][pair][r] at x = j + r*TL in registers ----
     double2 fv[X::NT][NPI][RL];
 #pragma unroll
-    for (int c = 0; c < X::NT; ++c) {
-        const int t = lane + 64 * c, l = t / TL, j = t - l * TL;
+    for (int p = 0; p < NPI; ++p) {
+        // first stage: packed Hermitian spectra straight from HBM
+        {
+            constexpr int TOT = SIN * T0, ROUNDS = (TOT + 63) / 64;
+            double2 v[ROUNDS][R0];
 #pragma unroll
-        for (int p = 0; p < NPI; ++p) {
-            const double2 *L = buf + (l * NPI + p) * N;
+            for (int c = 0; c < ROUNDS; ++c) {
+                const int b = lane + 64 * c;
+                if (TOT % 64 == 0 || b < TOT) {
+                    const int l = b / T0, j = b - l * T0;
+                    const long long line = line0 + l;
+                    const bool valid = line < nlines;
+                    const int f0 = 2 * p, f1 = 2 * p + 1;
+                    const bool has1 = f1 < a.nfull;
+                    const size_t lo = valid ? (size_t)line : 0;
+                    const double2 *g0 = a.G + ((size_t)f0 * nlines + lo) * nkx;
+                    const double2 *g1 = a.G + ((size_t)(has1 ? f1 : f0) * nlines + lo) * nkx;
+#pragma unroll
+                    for (int r = 0; r < R0; ++r) {
+                        const int e = j + r * T0;
+                        const bool pos = e < nkx, neg = e > N - nkx;
+                        const int k = pos ? e : (neg ? N - e : 0);
+                        double2 A = g0[k], B = g1[k];
+                        const double bx = has1 ? B.x : 0.0, by = (has1 && e != 0) ? B.y : 0.0;
+                        const double ay = e != 0 ? A.y : 0.0;
+                        // pos: A + iB;  neg: conj(A) + i conj(B)
+                        double zx = pos ? A.x - by : A.x + by;
+                        double zy = pos ? ay + bx : bx - ay;
+                        const bool keep = valid && (pos || neg);
+                        v[c][r] = make_double2(keep ? zx : 0.0, keep ? zy : 0.0);
+                    }
+                    Dft<R0, +1>::run(v[c]);
+                }
+            }
+#pragma unroll
+            for (int c = 0; c < ROUNDS; ++c) {
+                const int b = lane + 64 * c;
+                if (TOT % 64 == 0 || b < TOT) {
+                    const int l = b / T0, j = b - l * T0;
+#pragma unroll
+                    for (int r = 0; r < R0; ++r) buf[l * N + j * R0 + r] = v[c][r];
+                }
+            }
+            wfft::wave_sync();
+        }
+        XPlan<N>::template InvMid<SIN>::run(buf, tw, lane);
+        // last stage into registers
+#pragma unroll
+        for (int c = 0; c < X::NT; ++c) {
+            const int t = lane + 64 * c, l = t / TL, j = t - l * TL;
+            const double2 *L = buf + l * N;
+            double2 wp[RL];
+            wfft::powers<RL>(wfft::twiddle<+1>(tw, j), wp);
 #pragma unroll
             for (int r = 0; r < RL; ++r) {
                 double2 x = L[j + r * TL];
-                if (r > 0) x = cmul(x, wfft::twiddle<+1>(tw, j * r));
+                if (r > 0) x = cmul(x, wp[r]);
                 fv[c][p][r] = x;
             }
             Dft<RL, +1>::run(fv[c][p]);
         }
+        wfft::wave_sync();
     }
-    wfft::wave_sync();
 
     // ---- CFL maxima over u, v, w / weight_z (full variables 0, 1, 2) ----
     if constexpr (NPI >= 2) {

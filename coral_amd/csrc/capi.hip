@@ -553,7 +553,15 @@ void backward_to_G(coral_amd_ctx &c, const std::vector<const double2 *> &kst, co
                 if (e->kind != 0 || e->full != f) continue;
                 if (nt >= MAX_EXTRACT_TERMS) throw std::runtime_error("too many full-variable terms");
                 SysDev &d = *c.ks[e->system];
-                a.term[nt++] = ExtractTerm{e->rowptr.p, e->col.p, e->val.p, kst[e->system], d.t->total, e->kx, e->ky};
+                int q = 0, qoff = 0;
+                while (q < a.nstate && a.state[q] != kst[e->system]) qoff += a.stotal[q++];
+                if (q == a.nstate && q < MAX_EXTRACT_STATES) {
+                    a.state[q] = kst[e->system];
+                    a.stotal[q] = d.t->total;
+                    ++a.nstate;
+                }
+                a.term[nt++] = ExtractTerm{e->rowptr.p, e->col.p, e->val.p, kst[e->system], d.t->total, e->kx, e->ky,
+                                           q < MAX_EXTRACT_STATES ? q : -1, qoff};
             }
         }
         a.full_first[P.nfull] = nt;

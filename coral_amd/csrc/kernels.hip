@@ -108,7 +108,72 @@ __global__ __launch_bounds__(512) void k_extract_all(ExtractAllArgs a, int xt) {
     }
 }
 
+// Same outputs as k_extract_all, with the XT systems' state vectors staged in
+// LDS first: they are contiguous in HBM (one coalesced copy per state), while
+// the sparse rows gather entries about one interleave stride (6 positions)
+// apart, which as direct loads touch a cache line per lane.  Output rows of XT
+// systems are written straight from registers (XT * 16 contiguous bytes).
+template <int XT>
+__global__ __launch_bounds__(256) void k_extract_lds(ExtractAllArgs a) {
+    extern __shared__ double2 sst[];   // state q: [XT][stotal[q]] from XT * (sum of earlier stotal)
+    const int NZ = a.NZ, nsys = a.nsys, nf = a.nfull;
+    const int s0 = blockIdx.x * XT;
+    const int ns = min(XT, nsys - s0);
+    int off = 0;
+    for (int q = 0; q < a.nstate; ++q) {
+        const int cnt = ns * a.stotal[q];
+        const double2 *src = a.state[q] + (size_t)s0 * a.stotal[q];
+        double2 *dst = sst + off;
+        off += XT * a.stotal[q];
+        int e = threadIdx.x;
+        for (; e + 3 * 256 < cnt; e += 4 * 256) {
+            const double2 v0 = src[e], v1 = src[e + 256], v2 = src[e + 512], v3 = src[e + 768];
+            dst[e] = v0;
+            dst[e + 256] = v1;
+            dst[e + 512] = v2;
+            dst[e + 768] = v3;
+        }
+        for (; e < cnt; e += 256) dst[e] = src[e];
+    }
+    __syncthreads();
+    const int per = nf * NZ;
+    for (int e = threadIdx.x; e < per * XT; e += 256) {
+        const int row = e / XT, sl = e - row * XT;   // row = f * NZ + n
+        if (sl >= ns) continue;
+        const int f = row / NZ, n = row - f * NZ, s = s0 + sl;
+        double2 acc = make_double2(0.0, 0.0);
+        for (int t = a.full_first[f]; t < a.full_first[f + 1]; ++t) {
+            const ExtractTerm &T = a.term[t];
+            const double2 *st = sst + XT * T.soff + sl * T.total;
+            double2 part = make_double2(0.0, 0.0);
+            for (int k = T.rowptr[n]; k < T.rowptr[n + 1]; ++k) {
+                const double v = T.val[k];
+                const double2 x = st[T.col[k]];
+                part.x += v * x.x;
+                part.y += v * x.y;
+            }
+            if (T.kx || T.ky) part = cmul(monomial(T.kx, T.ky, a.km.kx(s), a.km.ky(s)), part);
+            acc = cadd(acc, part);
+        }
+        a.spec[(size_t)row * nsys + s] = acc;
+    }
+}
+
 void launch_extract_all(hipStream_t st, const ExtractAllArgs &a) {
+    // staged variant: every term's state known, XT systems of all states in LDS
+    {
+        bool ok = a.nstate > 0;
+        size_t words = 0;
+        for (int t = 0; t < a.nterms; ++t) ok = ok && a.term[t].sidx >= 0 && a.term[t].sidx < a.nstate;
+        for (int q = 0; q < a.nstate; ++q) words += size_t(a.stotal[q]);
+        constexpr int XT = 4;
+        const size_t shm = words * XT * sizeof(double2);
+        if (ok && shm <= 65536) {
+            hipLaunchKernelGGL(k_extract_lds<XT>, dim3(blocks_for(a.nsys, XT)), dim3(256), shm, st, a);
+            CHECK_LAUNCH();
+            return;
+        }
+    }
     int lim = 65536;
     int dev = 0;
     if (hipGetDevice(&dev) == hipSuccess) (void)hipDeviceGetAttribute(&lim, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);

@@ -20,7 +20,10 @@ struct ExtractTerm {
     const double *val;
     const double2 *state;
     int total, kx, ky;
+    int sidx;                               // index of state among ExtractAllArgs::state
+    int soff;                               // sum of stotal over the states before it
 };
+constexpr int MAX_EXTRACT_STATES = 4;
 constexpr int MAX_EXTRACT_TERMS = 24;
 constexpr int MAX_FULL = 16;
 struct ExtractAllArgs {
@@ -30,6 +33,10 @@ struct ExtractAllArgs {
     int nsys, NZ;
     KeptModes km;
     double2 *spec;                          // [nfull][NZ][nsys]
+    // distinct state arrays read by the terms ((total, NY, NX/2) each)
+    int nstate;
+    const double2 *state[MAX_EXTRACT_STATES];
+    int stotal[MAX_EXTRACT_STATES];
 };
 void launch_extract_all(hipStream_t st, const ExtractAllArgs &a);
 // mean contributions into the (0,0) column: spec[n][0] += sum val[k] * Re(zstate[col[k]])

@@ -122,14 +122,15 @@ __device__ inline double2 twiddle(const double2 *__restrict__ tw, int m) {
     return SIGN > 0 ? make_double2(w.x, -w.y) : w;
 }
 
-// p[r] = w^r for r = 1..R-1, by repeated products: one table load per
-// butterfly instead of R-1 (the twiddle loads share the vector memory path
-// with the HBM traffic); the product chain adds a few ulp at most
-template <int R>
-__device__ inline void powers(double2 w, double2 (&p)[R]) {
-    p[1] = w;
+// p[r] = exp(SIGN 2 pi i r m / N) for r = 1..R-1 (r m < N): one table load,
+// the other powers by a chain of products (each adds an ulp or so).  Loads of
+// more powers were measured slower: the twiddle loads sit at the head of
+// every stage.
+template <int R, int SIGN>
+__device__ inline void twiddle_powers(const double2 *__restrict__ tw, int m, double2 (&p)[R]) {
+    p[1] = twiddle<SIGN>(tw, m);
 #pragma unroll
-    for (int r = 2; r < R; ++r) p[r] = cmul(p[r - 1], w);
+    for (int r = 2; r < R; ++r) p[r] = cmul(p[r - 1], p[1]);
 }
 
 // LDS bank swizzles (used by runs that opt in with SW).  A wavefront's
@@ -175,7 +176,7 @@ __device__ inline void stage_lds(double2 *buf, const double2 *__restrict__ tw, i
             const double2 *L = buf + s * LS;
             const int jm = j % NS;
             double2 wp[R];
-            if constexpr (NS > 1) powers<R>(twiddle<SIGN>(tw, jm * (N / (NS * R))), wp);
+            if constexpr (NS > 1) twiddle_powers<R, SIGN>(tw, jm * (N / (NS * R)), wp);
 #pragma unroll
             for (int r = 0; r < R; ++r) {
                 double2 x = L[swz<WI>(j + r * T)];

@@ -158,7 +158,9 @@ __device__ inline void xfwd_batch(const XPassArgs &a, const double2 (&fv)[XShape
     wfft::wave_sync();
 }
 
-template <int N, int NPI, int XGP>
+// NK: the kept kx count when it is known at compile time (N/3, the 3/2 rule),
+// 0 when it is read from the arguments
+template <int N, int NPI, int XGP, int NK>
 __global__ __launch_bounds__(256) void k_xpass_wave(XPassArgs a, const double2 *__restrict__ tw, long long nlines) {
     using X = XShape<N, NPI, XGP>;
     constexpr int R0 = X::R0, RL = X::RL, T0 = X::T0, TL = X::TL, LPW = X::LPW, SIN = X::SIN;
@@ -167,7 +169,7 @@ __global__ __launch_bounds__(256) void k_xpass_wave(XPassArgs a, const double2 *
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     double2 *buf = lds + wid * X::SMAX * N;
     const long long line0 = ((long long)blockIdx.x * (blockDim.x >> 6) + wid) * LPW;
-    const int nkx = a.nkx;
+    const int nkx = NK > 0 ? NK : a.nkx;
 
     // ---- inverse transforms, one packed pair of full variables at a time
     // (half the LDS of both pairs at once, which buys a wavefront per SIMD);
@@ -194,15 +196,28 @@ __global__ __launch_bounds__(256) void k_xpass_wave(XPassArgs a, const double2 *
 #pragma unroll
                     for (int r = 0; r < R0; ++r) {
                         const int e = j + r * T0;
-                        const bool pos = e < nkx, neg = e > N - nkx;
+                        // with NK known, the slots [r T0, (r+1) T0) that hold
+                        // only positive, only negative or only dealiased kx
+                        // are known once the loop is unrolled
+                        const int lo = r * T0, hi = lo + T0 - 1;
+                        const bool all_pos = NK > 0 && hi < NK;
+                        const bool all_neg = NK > 0 && lo > N - NK;
+                        const bool all_zero = NK > 0 && lo >= NK && hi <= N - NK;
+                        if (all_zero) {
+                            v[c][r] = make_double2(0.0, 0.0);
+                            continue;
+                        }
+                        const bool pos = all_pos || (!all_neg && e < nkx);
+                        const bool neg = all_neg || (!all_pos && e > N - nkx);
                         const int k = pos ? e : (neg ? N - e : 0);
                         double2 A = g0[k], B = g1[k];
-                        const double bx = has1 ? B.x : 0.0, by = (has1 && e != 0) ? B.y : 0.0;
-                        const double ay = e != 0 ? A.y : 0.0;
+                        const bool e0 = r == 0 && j == 0;   // kx = 0: imaginary parts dropped
+                        const double bx = has1 ? B.x : 0.0, by = (has1 && !e0) ? B.y : 0.0;
+                        const double ay = !e0 ? A.y : 0.0;
                         // pos: A + iB;  neg: conj(A) + i conj(B)
                         double zx = pos ? A.x - by : A.x + by;
                         double zy = pos ? ay + bx : bx - ay;
-                        const bool keep = valid && (pos || neg);
+                        const bool keep = valid && (all_pos || all_neg || pos || neg);
                         v[c][r] = make_double2(keep ? zx : 0.0, keep ? zy : 0.0);
                     }
                     Dft<R0, +1>::run(v[c]);
@@ -226,7 +241,7 @@ __global__ __launch_bounds__(256) void k_xpass_wave(XPassArgs a, const double2 *
             const int t = lane + 64 * c, l = t / TL, j = t - l * TL;
             const double2 *L = buf + l * N;
             double2 wp[RL];
-            wfft::powers<RL>(wfft::twiddle<+1>(tw, j), wp);
+            wfft::twiddle_powers<RL, +1>(tw, j, wp);
 #pragma unroll
             for (int r = 0; r < RL; ++r) {
                 double2 x = L[j + r * TL];
@@ -247,13 +262,16 @@ __global__ __launch_bounds__(256) void k_xpass_wave(XPassArgs a, const double2 *
             const int t = lane + 64 * c, l = t / TL;
             const long long line = line0 + l;
             if (line < nlines) {
-                const double zw = a.zweight[line / a.NYAA];
+                // max |w| / zw = (max |w|) / zw: rounded division by a
+                // positive number keeps the order, so one division per task
+                double wm = 0.0;
 #pragma unroll
                 for (int r = 0; r < RL; ++r) {
                     mu = fmax(mu, fabs(fv[c][0][r].x));
                     mv = fmax(mv, fabs(fv[c][0][r].y));
-                    mw = fmax(mw, fabs(fv[c][1][r].x / zw));
+                    wm = fmax(wm, fabs(fv[c][1][r].x));
                 }
+                mw = fmax(mw, wm / a.zweight[line / a.NYAA]);
             }
         }
         for (int off = 32; off > 0; off >>= 1) {
@@ -285,7 +303,7 @@ __global__ __launch_bounds__(256) void k_xpass_wave(XPassArgs a, const double2 *
     }
 }
 
-template <int N, int NPI, int XGP>
+template <int N, int NPI, int XGP, int NK>
 static bool launch_wave(hipStream_t st, const FFTPlan &px, const XPassArgs &a) {
     using X = XShape<N, NPI, XGP>;
     const long long nlines = (long long)a.NZAA * a.NYAA;
@@ -295,7 +313,7 @@ static bool launch_wave(hipStream_t st, const FFTPlan &px, const XPassArgs &a) {
     if (wpb * per_wave > 65536) return false;
     const long long per_block = (long long)wpb * X::LPW;
     const unsigned blocks = unsigned((nlines + per_block - 1) / per_block);
-    hipLaunchKernelGGL((k_xpass_wave<N, NPI, XGP>), dim3(blocks), dim3(64 * wpb), wpb * per_wave, st, a, px.tw, nlines);
+    hipLaunchKernelGGL((k_xpass_wave<N, NPI, XGP, NK>), dim3(blocks), dim3(64 * wpb), wpb * per_wave, st, a, px.tw, nlines);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) fprintf(stderr, "coral_amd: launch failed: %s\n", hipGetErrorString(e));
     return true;
@@ -305,10 +323,11 @@ template <int N>
 static bool launch_wave_n(hipStream_t st, const FFTPlan &px, const XPassArgs &a) {
     // one product pair per forward batch: two pairs would fill the lanes of
     // the forward stages better, but their registers cost a wave per SIMD
+    const bool third = 3 * a.nkx == N;
     switch ((a.nfull + 1) / 2) {
-    case 1: return launch_wave<N, 1, 1>(st, px, a);
-    case 2: return launch_wave<N, 2, 1>(st, px, a);
-    case 3: return launch_wave<N, 3, 1>(st, px, a);
+    case 1: return third ? launch_wave<N, 1, 1, N / 3>(st, px, a) : launch_wave<N, 1, 1, 0>(st, px, a);
+    case 2: return third ? launch_wave<N, 2, 1, N / 3>(st, px, a) : launch_wave<N, 2, 1, 0>(st, px, a);
+    case 3: return third ? launch_wave<N, 3, 1, N / 3>(st, px, a) : launch_wave<N, 3, 1, 0>(st, px, a);
     default: return false;
     }
 }

@@ -136,26 +136,43 @@ __global__ __launch_bounds__(256) void k_extract_lds(ExtractAllArgs a) {
         for (; e < cnt; e += 256) dst[e] = src[e];
     }
     __syncthreads();
+    // one output row (f, n) per thread for all XT systems: the row's sparse
+    // entries are read once for the XT systems
     const int per = nf * NZ;
-    for (int e = threadIdx.x; e < per * XT; e += 256) {
-        const int row = e / XT, sl = e - row * XT;   // row = f * NZ + n
-        if (sl >= ns) continue;
-        const int f = row / NZ, n = row - f * NZ, s = s0 + sl;
-        double2 acc = make_double2(0.0, 0.0);
+    for (int row = threadIdx.x; row < per; row += 256) {
+        const int f = row / NZ, n = row - f * NZ;
+        double2 acc[XT];
+#pragma unroll
+        for (int sl = 0; sl < XT; ++sl) acc[sl] = make_double2(0.0, 0.0);
         for (int t = a.full_first[f]; t < a.full_first[f + 1]; ++t) {
             const ExtractTerm &T = a.term[t];
-            const double2 *st = sst + XT * T.soff + sl * T.total;
-            double2 part = make_double2(0.0, 0.0);
+            const double2 *st = sst + XT * T.soff;
+            double2 part[XT];
+#pragma unroll
+            for (int sl = 0; sl < XT; ++sl) part[sl] = make_double2(0.0, 0.0);
             for (int k = T.rowptr[n]; k < T.rowptr[n + 1]; ++k) {
                 const double v = T.val[k];
-                const double2 x = st[T.col[k]];
-                part.x += v * x.x;
-                part.y += v * x.y;
+                const int c = T.col[k];
+#pragma unroll
+                for (int sl = 0; sl < XT; ++sl) {
+                    const double2 x = st[sl * T.total + c];
+                    part[sl].x += v * x.x;
+                    part[sl].y += v * x.y;
+                }
             }
-            if (T.kx || T.ky) part = cmul(monomial(T.kx, T.ky, a.km.kx(s), a.km.ky(s)), part);
-            acc = cadd(acc, part);
+#pragma unroll
+            for (int sl = 0; sl < XT; ++sl) {
+                if (T.kx || T.ky) {
+                    const int s = min(s0 + sl, nsys - 1);
+                    part[sl] = cmul(monomial(T.kx, T.ky, a.km.kx(s), a.km.ky(s)), part[sl]);
+                }
+                acc[sl] = cadd(acc[sl], part[sl]);
+            }
         }
-        a.spec[(size_t)row * nsys + s] = acc;
+        double2 *out = a.spec + (size_t)row * nsys + s0;
+#pragma unroll
+        for (int sl = 0; sl < XT; ++sl)
+            if (sl < ns) out[sl] = acc[sl];
     }
 }
 

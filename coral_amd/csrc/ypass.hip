@@ -18,6 +18,7 @@ using wfft::swz;
 
 constexpr int YL = 8;        // kx columns per workgroup
 constexpr int YWAVES = 4;    // wavefronts per workgroup (YL / YWAVES lines each)
+constexpr int YT = 64 * YWAVES;   // threads per workgroup
 
 template <int N>
 struct YPlan;
@@ -57,19 +58,48 @@ __global__ __launch_bounds__(64 * YWAVES) void k_ybwd_w(const double2 *__restric
         const int l = e / (N - nky), k = e - l * (N - nky);
         buf[l * LS + NY / 2 + k] = make_double2(0.0, 0.0);
     }
+    // all of a thread's loads are issued before the first LDS store (a loop
+    // of load, wait, store would make them one HBM round trip each)
     const double2 *src = Fz + bz * nsys + (size_t)ix0 * nky;
-    for (int e = threadIdx.x; e < YL * nky; e += blockDim.x) {
-        const int l = e / nky, jy = e - l * nky;
-        buf[l * LS + ky_slot(jy, NY, N)] = l < nl ? src[e] : make_double2(0.0, 0.0);
+    {
+        constexpr int PER = (YL * N + YT - 1) / YT;   // nky <= N
+        const int cnt = nl * nky;
+        double2 v[PER];
+#pragma unroll
+        for (int i = 0; i < PER; ++i) {
+            const int e = threadIdx.x + i * YT;
+            v[i] = e < cnt ? src[e] : make_double2(0.0, 0.0);
+        }
+#pragma unroll
+        for (int i = 0; i < PER; ++i) {
+            const int e = threadIdx.x + i * YT;
+            if (e < YL * nky) {
+                const int l = e / nky, jy = e - l * nky;
+                buf[l * LS + ky_slot(jy, NY, N)] = v[i];
+            }
+        }
     }
     __syncthreads();
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     YPlan<N>::template Run<+1, LPW, LS>::run(buf + w * LPW * LS, tw, lane);
     __syncthreads();
     double2 *dst = G + bz * N * nkx + ix0;
-    for (int e = threadIdx.x; e < N * YL; e += blockDim.x) {
-        const int y = e / YL, l = e - y * YL;
-        if (l < nl) dst[(size_t)y * nkx + l] = buf[l * LS + swz<YPlan<N>::template Run<+1, LPW, LS>::OUT_SWZ>(y)];
+    {
+        constexpr int PER = N * YL / YT;
+        static_assert(N * YL % YT == 0, "tile must split evenly over the threads");
+        double2 v[PER];
+#pragma unroll
+        for (int i = 0; i < PER; ++i) {
+            const int e = threadIdx.x + i * YT;
+            const int y = e / YL, l = e - y * YL;
+            v[i] = buf[l * LS + swz<YPlan<N>::template Run<+1, LPW, LS>::OUT_SWZ>(y)];
+        }
+#pragma unroll
+        for (int i = 0; i < PER; ++i) {
+            const int e = threadIdx.x + i * YT;
+            const int y = e / YL, l = e - y * YL;
+            if (l < nl) dst[(size_t)y * nkx + l] = v[i];
+        }
     }
 }
 
@@ -84,19 +114,46 @@ __global__ __launch_bounds__(64 * YWAVES) void k_yfwd_w(const double2 *__restric
     const size_t bz = blockIdx.x / ntile;
     const int ix0 = tile * YL, nl = min(YL, nkx - ix0);
     const int nsys = nkx * nky;
+    // all of a thread's loads are issued before the first LDS store
     const double2 *src = H + bz * N * nkx + ix0;
-    for (int e = threadIdx.x; e < N * YL; e += blockDim.x) {
-        const int y = e / YL, l = e - y * YL;
-        buf[l * LS + y] = l < nl ? src[(size_t)y * nkx + l] : make_double2(0.0, 0.0);
+    {
+        constexpr int PER = N * YL / YT;
+        static_assert(N * YL % YT == 0, "tile must split evenly over the threads");
+        double2 v[PER];
+#pragma unroll
+        for (int i = 0; i < PER; ++i) {
+            const int e = threadIdx.x + i * YT;
+            const int y = e / YL, l = e - y * YL;
+            v[i] = l < nl ? src[(size_t)y * nkx + l] : make_double2(0.0, 0.0);
+        }
+#pragma unroll
+        for (int i = 0; i < PER; ++i) {
+            const int e = threadIdx.x + i * YT;
+            const int y = e / YL, l = e - y * YL;
+            buf[l * LS + y] = v[i];
+        }
     }
     __syncthreads();
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     YPlan<N>::template Run<-1, LPW, LS>::run(buf + w * LPW * LS, tw, lane);
     __syncthreads();
     double2 *dst = Hy + bz * nsys + (size_t)ix0 * nky;
-    for (int e = threadIdx.x; e < nl * nky; e += blockDim.x) {
-        const int l = e / nky, jy = e - l * nky;
-        dst[e] = buf[l * LS + swz<YPlan<N>::template Run<-1, LPW, LS>::OUT_SWZ>(ky_slot(jy, NY, N))];
+    {
+        constexpr int PER = (YL * N + YT - 1) / YT;   // nky <= N
+        const int cnt = nl * nky;
+        double2 v[PER];
+#pragma unroll
+        for (int i = 0; i < PER; ++i) {
+            const int e = threadIdx.x + i * YT;
+            const int l = e / nky, jy = e - l * nky;
+            v[i] = e < cnt ? buf[l * LS + swz<YPlan<N>::template Run<-1, LPW, LS>::OUT_SWZ>(ky_slot(jy, NY, N))]
+                           : make_double2(0.0, 0.0);
+        }
+#pragma unroll
+        for (int i = 0; i < PER; ++i) {
+            const int e = threadIdx.x + i * YT;
+            if (e < cnt) dst[e] = v[i];
+        }
     }
 }
 

@@ -945,7 +945,129 @@ __global__ __launch_bounds__(RHS_ROWS) void k_rhs(RhsArgs a) {
     a.y[idx] = y;
 }
 
+// Whole-system variant for total <= 256 * RPT: one workgroup per system, RPT
+// rows per thread.  No halo rows are re-read, and every vector's loads for the
+// thread's RPT rows are issued together (one HBM round trip per vector, not
+// per element).
+template <int RPT>
+__global__ __launch_bounds__(256) void k_rhs_sys(RhsArgs a) {
+    extern __shared__ double2 xs[];   // X [total], V [total]
+    const int total = a.total, s = blockIdx.x, tid = threadIdx.x;
+    const size_t base = (size_t)s * total;
+    double2 *X = xs, *V = xs + total;
+    {
+        double2 u[RPT], v[RPT];
+#pragma unroll
+        for (int r = 0; r < RPT; ++r) u[r] = v[r] = make_double2(0.0, 0.0);
+        for (int k = 0; k < a.nm; ++k) {
+            const double2 *src = a.mv[k] + base;
+            const double c = a.ma[k];
+            double2 t[RPT];
+#pragma unroll
+            for (int r = 0; r < RPT; ++r) {
+                const int i = tid + 256 * r;
+                t[r] = i < total ? src[i] : make_double2(0.0, 0.0);
+            }
+#pragma unroll
+            for (int r = 0; r < RPT; ++r) {
+                u[r].x = fma(c, t[r].x, u[r].x);
+                u[r].y = fma(c, t[r].y, u[r].y);
+            }
+        }
+        for (int k = 0; k < a.nx; ++k) {
+            const double2 *src = a.xv[k] + base;
+            const double c = a.xa[k];
+            double2 t[RPT];
+#pragma unroll
+            for (int r = 0; r < RPT; ++r) {
+                const int i = tid + 256 * r;
+                t[r] = i < total ? src[i] : make_double2(0.0, 0.0);
+            }
+#pragma unroll
+            for (int r = 0; r < RPT; ++r) {
+                v[r].x = fma(c, t[r].x, v[r].x);
+                v[r].y = fma(c, t[r].y, v[r].y);
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < RPT; ++r) {
+            const int i = tid + 256 * r;
+            if (i < total) {
+                X[i] = u[r];
+                V[i] = v[r];
+            }
+        }
+    }
+    __syncthreads();
+    double2 y[RPT];
+#pragma unroll
+    for (int r = 0; r < RPT; ++r) {
+        const int i = min(tid + 256 * r, total - 1);
+        double2 acc = make_double2(0.0, 0.0);
+        for (int g = 0; g < a.mass.ngroups; ++g) {
+            const int W = a.mass.ell_w[g], off = a.mass.ell_off[g] + i * W;
+            double2 sum = make_double2(0.0, 0.0);
+            for (int e = 0; e < W; ++e) {
+                const double bv = a.mass.ell_val[off + e];
+                const double2 u = X[a.mass.ell_col[off + e]];
+                sum.x = fma(bv, u.x, sum.x);
+                sum.y = fma(bv, u.y, sum.y);
+            }
+            acc = cadd(acc, cmul(a.cmass[(size_t)s * a.mass.ngroups + g], sum));
+        }
+        if (a.nx > 0) {
+            for (int g = 0; g < a.stif.ngroups; ++g) {
+                const int W = a.stif.ell_w[g], off = a.stif.ell_off[g] + i * W;
+                double2 sum = make_double2(0.0, 0.0);
+                for (int e = 0; e < W; ++e) {
+                    const double bv = a.stif.ell_val[off + e];
+                    const double2 v = V[a.stif.ell_col[off + e]];
+                    sum.x = fma(bv, v.x, sum.x);
+                    sum.y = fma(bv, v.y, sum.y);
+                }
+                acc = cadd(acc, cmul(a.cstif[(size_t)s * a.stif.ngroups + g], sum));
+            }
+        }
+        y[r] = acc;
+    }
+    for (int k = 0; k < a.nk; ++k) {
+        const double2 *src = a.kv[k] + base;
+        const double c = a.kb[k];
+        double2 t[RPT];
+#pragma unroll
+        for (int r = 0; r < RPT; ++r) {
+            const int i = tid + 256 * r;
+            t[r] = i < total ? src[i] : make_double2(0.0, 0.0);
+        }
+#pragma unroll
+        for (int r = 0; r < RPT; ++r) {
+            y[r].x = fma(c, t[r].x, y[r].x);
+            y[r].y = fma(c, t[r].y, y[r].y);
+        }
+    }
+    const bool zero = a.zero00 && s == 0;
+#pragma unroll
+    for (int r = 0; r < RPT; ++r) {
+        const int i = tid + 256 * r;
+        if (i < total) a.y[base + i] = zero ? make_double2(0.0, 0.0) : y[r];
+    }
+}
+
 void launch_rhs(hipStream_t st, const RhsArgs &a) {
+    {
+        const int rpt = (a.total + 255) / 256;
+        const size_t shm = 2 * size_t(a.total) * sizeof(double2);
+        const dim3 grid(a.nsys), block(256);
+        switch (rpt) {
+        case 1: hipLaunchKernelGGL(k_rhs_sys<1>, grid, block, shm, st, a); CHECK_LAUNCH(); return;
+        case 2: hipLaunchKernelGGL(k_rhs_sys<2>, grid, block, shm, st, a); CHECK_LAUNCH(); return;
+        case 3: hipLaunchKernelGGL(k_rhs_sys<3>, grid, block, shm, st, a); CHECK_LAUNCH(); return;
+        case 4: hipLaunchKernelGGL(k_rhs_sys<4>, grid, block, shm, st, a); CHECK_LAUNCH(); return;
+        case 5: hipLaunchKernelGGL(k_rhs_sys<5>, grid, block, shm, st, a); CHECK_LAUNCH(); return;
+        case 6: hipLaunchKernelGGL(k_rhs_sys<6>, grid, block, shm, st, a); CHECK_LAUNCH(); return;
+        default: break;   // longer systems: row chunks with halos
+        }
+    }
     const int nchunk = (a.total + RHS_ROWS - 1) / RHS_ROWS;
     const size_t shm = 2 * size_t(RHS_ROWS + a.bl + a.bu) * sizeof(double2);
     hipLaunchKernelGGL(k_rhs, dim3((unsigned)((long long)a.nsys * nchunk)), dim3(RHS_ROWS), shm, st, a);

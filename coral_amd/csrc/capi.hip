@@ -115,6 +115,13 @@ struct coral_amd_ctx {
     Problem P;
     int device = 0;
     hipStream_t stream = nullptr;
+    // the mean systems' NL terms, right-hand sides and solves run on a second
+    // stream, overlapped with the kxky work: they wait for Qhat (ev_qhat) and
+    // the kxky side waits for their solutions (ev_mean) before the next
+    // stage's mean extraction
+    hipStream_t side = nullptr;
+    hipEvent_t ev_qhat = nullptr, ev_mean = nullptr;
+    bool mean_pending = false;
     KeptModes km{};
     std::vector<std::unique_ptr<SysDev>> ks, zs;
     std::vector<std::unique_ptr<ExtractDev>> ext;
@@ -140,6 +147,9 @@ struct coral_amd_ctx {
             (void)hipEventDestroy(r.a);
             (void)hipEventDestroy(r.b);
         }
+        if (ev_qhat) (void)hipEventDestroy(ev_qhat);
+        if (ev_mean) (void)hipEventDestroy(ev_mean);
+        if (side) (void)hipStreamDestroy(side);
         if (stream) (void)hipStreamDestroy(stream);
     }
 
@@ -533,6 +543,26 @@ void nl_assemble(coral_amd_ctx &c, SysDev &d, double2 *Khat) {
     c.timed(d.kxky ? "nl_assemble" : "nl_mean", [&] { launch_nl(c.stream, a); });
 }
 
+// the main stream waits for the mean systems' pending work on the side stream
+void join_mean(coral_amd_ctx &c) {
+    if (!c.mean_pending) return;
+    HIP_OK(hipStreamWaitEvent(c.stream, c.ev_mean, 0));
+    c.mean_pending = false;
+}
+
+// f's launches (and timer events) go to the side stream
+template <class F>
+void on_side(coral_amd_ctx &c, F &&f) {
+    std::swap(c.stream, c.side);
+    try {
+        f();
+    } catch (...) {
+        std::swap(c.stream, c.side);
+        throw;
+    }
+    std::swap(c.stream, c.side);
+}
+
 // Chebyshev coefficients of all full variables from the given states, then the
 // backward transforms up to G.
 void backward_to_G(coral_amd_ctx &c, const std::vector<const double2 *> &kst, const std::vector<const double2 *> &zst) {
@@ -567,6 +597,7 @@ void backward_to_G(coral_amd_ctx &c, const std::vector<const double2 *> &kst, co
         a.full_first[P.nfull] = nt;
         a.nterms = nt;
         if (P.nfull > 0) launch_extract_all(c.stream, a);
+        join_mean(c);   // the mean states below come from the side stream
         for (auto &e : c.ext) {
             if (e->kind != 1) continue;
             launch_extract_mean(c.stream, zst[e->system], P.NZ, P.nsys, e->rowptr.p, e->col.p, e->val.p,
@@ -600,6 +631,13 @@ void nl_terms(coral_amd_ctx &c, const std::vector<const double2 *> &kst, const s
     for (int q = 0; q < P.nquad; ++q)
         if (P.quad_filter[q]) launch_remove_zmean(c.stream, c.Qhat.p + size_t(q) * P.nsys * P.NZ, P.NZ, P.nsys);
     for (auto &d : c.ks) nl_assemble(c, *d, d->Khat[stage]->p);
+    if (!c.zs.empty()) HIP_OK(hipEventRecord(c.ev_qhat, c.stream));
+}
+
+// NL terms of the mean systems from the Qhat of nl_terms (on the stream they
+// are launched on, after ev_qhat)
+void nl_terms_mean(coral_amd_ctx &c, int stage) {
+    HIP_OK(hipStreamWaitEvent(c.stream, c.ev_qhat, 0));
     for (auto &d : c.zs) nl_assemble(c, *d, d->Khat[stage]->p);
 }
 
@@ -671,9 +709,17 @@ void one_step(coral_amd_ctx &c) {
                 if (!done) throw std::runtime_error("16-lane solve rejected a band it was factored for");
             });
         };
+        if (!c.zs.empty()) {
+            on_side(c, [&] {
+                nl_terms_mean(c, i);
+                for (auto &d : c.zs) linear(*d);
+                HIP_OK(hipEventRecord(c.ev_mean, c.stream));
+            });
+            c.mean_pending = true;
+        }
         for (auto &d : c.ks) linear(*d);
-        for (auto &d : c.zs) linear(*d);
     }
+    join_mean(c);
     for (auto &d : c.ks) std::swap(d->field.p, d->X[s - 1]->p);
     for (auto &d : c.zs) std::swap(d->field.p, d->X[s - 1]->p);
 }
@@ -744,6 +790,9 @@ int coral_amd_create(const coral_amd_problem *prob, int device, coral_amd_ctx **
         c->device = device;
         HIP_OK(hipSetDevice(device));
         HIP_OK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        HIP_OK(hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
+        HIP_OK(hipEventCreateWithFlags(&c->ev_qhat, hipEventDisableTiming));
+        HIP_OK(hipEventCreateWithFlags(&c->ev_mean, hipEventDisableTiming));
         c->km = KeptModes{P.nkx, P.nky, P.NY, P.kx_box, P.ky_box};
         for (auto &t : P.kxky) {
             c->ks.emplace_back(new SysDev());

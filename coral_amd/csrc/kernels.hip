@@ -759,14 +759,16 @@ __global__ __launch_bounds__(256) void k_nl(NLArgs a) {
 }
 
 // One workgroup per 64 consecutive systems (one per lane) and NLP_POS
-// positions (NLP_POS / 4 per wavefront).  A position's program is the same
-// for every system, so the loops are wave-uniform: the program tables are
-// read with scalar loads and every Qhat access is one contiguous row of 64
-// systems; the results go through an LDS tile so that each system's run of
-// positions is stored contiguously.
-constexpr int NLP_POS = 16;
+// positions (NLP_POS / NLP_WAVES per wavefront).  A position's program is the
+// same for every system, so the loops are wave-uniform: the program tables
+// are read with scalar loads and every Qhat access is one contiguous row of
+// 64 systems; the results go through an LDS tile so that each system's run of
+// positions is stored contiguously.  Neighbouring positions read overlapping
+// Qhat rows; 64 positions over 16 wavefronts keep those re-reads inside one
+// CU (4 positions per wavefront, as few sequential programs as possible).
+constexpr int NLP_POS = 64, NLP_WAVES = 16, NLP_THREADS = 64 * NLP_WAVES;
 
-__global__ __launch_bounds__(256) void k_nl_prog(NLProgArgs a) {
+__global__ __launch_bounds__(NLP_THREADS) void k_nl_prog(NLProgArgs a) {
     __shared__ double2 tile[NLP_POS][65];
     const int nsys = a.nsys, total = a.total;
     const int npb = (total + NLP_POS - 1) / NLP_POS;
@@ -783,8 +785,8 @@ __global__ __launch_bounds__(256) void k_nl_prog(NLProgArgs a) {
     const int *__restrict__ ent_qm = a.ent_qm;
     const double *__restrict__ ent_w = a.ent_w;
 #pragma unroll 1
-    for (int k = 0; k < NLP_POS / 4; ++k) {
-        const int pl = wv * (NLP_POS / 4) + k;
+    for (int k = 0; k < NLP_POS / NLP_WAVES; ++k) {
+        const int pl = wv * (NLP_POS / NLP_WAVES) + k;
         const int pos = pb * NLP_POS + pl;
         double2 acc = make_double2(0.0, 0.0);
         if (pos < total) {
@@ -815,8 +817,8 @@ __global__ __launch_bounds__(256) void k_nl_prog(NLProgArgs a) {
         tile[pl][lane] = (sv && s != 0) ? acc : make_double2(0.0, 0.0);
     }
     __syncthreads();
-    for (int i = 0; i < 64 * NLP_POS / 256; ++i) {
-        const int e = threadIdx.x + 256 * i, sl = e / NLP_POS, pl = e % NLP_POS;
+    for (int i = 0; i < 64 * NLP_POS / NLP_THREADS; ++i) {
+        const int e = threadIdx.x + NLP_THREADS * i, sl = e / NLP_POS, pl = e % NLP_POS;
         const int s2 = sb * 64 + sl, pos = pb * NLP_POS + pl;
         if (s2 < nsys && pos < total) a.Khat[(size_t)s2 * total + pos] = tile[pl][sl];
     }
@@ -824,7 +826,7 @@ __global__ __launch_bounds__(256) void k_nl_prog(NLProgArgs a) {
 
 void launch_nl_prog(hipStream_t st, const NLProgArgs &a) {
     const int npb = (a.total + NLP_POS - 1) / NLP_POS;
-    hipLaunchKernelGGL(k_nl_prog, dim3(blocks_for(a.nsys, 64) * npb), dim3(256), 0, st, a);
+    hipLaunchKernelGGL(k_nl_prog, dim3(blocks_for(a.nsys, 64) * npb), dim3(NLP_THREADS), 0, st, a);
     CHECK_LAUNCH();
 }
 

@@ -125,15 +125,22 @@ __global__ __launch_bounds__(256) void k_extract_lds(ExtractAllArgs a) {
         const double2 *src = a.state[q] + (size_t)s0 * a.stotal[q];
         double2 *dst = sst + off;
         off += XT * a.stotal[q];
-        int e = threadIdx.x;
-        for (; e + 3 * 256 < cnt; e += 4 * 256) {
-            const double2 v0 = src[e], v1 = src[e + 256], v2 = src[e + 512], v3 = src[e + 768];
-            dst[e] = v0;
-            dst[e + 256] = v1;
-            dst[e + 512] = v2;
-            dst[e + 768] = v3;
+        // up to 16 loads per thread in flight before the first LDS store
+        // (12 for four systems of 766 unknowns)
+        constexpr int BATCH = 16;
+        for (int e0 = 0; e0 < cnt; e0 += BATCH * 256) {
+            double2 v[BATCH];
+#pragma unroll
+            for (int i = 0; i < BATCH; ++i) {
+                const int e = e0 + threadIdx.x + 256 * i;
+                v[i] = e < cnt ? src[e] : make_double2(0.0, 0.0);
+            }
+#pragma unroll
+            for (int i = 0; i < BATCH; ++i) {
+                const int e = e0 + threadIdx.x + 256 * i;
+                if (e < cnt) dst[e] = v[i];
+            }
         }
-        for (; e < cnt; e += 256) dst[e] = src[e];
     }
     __syncthreads();
     // one output row (f, n) per thread for all XT systems: the row's sparse

@@ -955,48 +955,62 @@ __global__ __launch_bounds__(RHS_ROWS) void k_rhs(RhsArgs a) {
 }
 
 // Whole-system variant for total <= 256 * RPT: one workgroup per system, RPT
-// rows per thread.  No halo rows are re-read, and every vector's loads for the
-// thread's RPT rows are issued together (one HBM round trip per vector, not
-// per element).
+// rows per thread.  No halo rows are re-read, and the loads of the thread's
+// RPT rows of the input vectors are issued together.
 template <int RPT>
 __global__ __launch_bounds__(256) void k_rhs_sys(RhsArgs a) {
     extern __shared__ double2 xs[];   // X [total], V [total]
     const int total = a.total, s = blockIdx.x, tid = threadIdx.x;
     const size_t base = (size_t)s * total;
     double2 *X = xs, *V = xs + total;
+    // rows tid + 256 r of vector src (zero past the end)
+    auto load_rows = [&](const double2 *src, double2 (&t)[RPT]) {
+#pragma unroll
+        for (int r = 0; r < RPT; ++r) {
+            const int i = tid + 256 * r;
+            t[r] = i < total ? src[base + i] : make_double2(0.0, 0.0);
+        }
+    };
+    auto axpy_rows = [&](double c, const double2 (&t)[RPT], double2 (&acc)[RPT]) {
+#pragma unroll
+        for (int r = 0; r < RPT; ++r) {
+            acc[r].x = fma(c, t[r].x, acc[r].x);
+            acc[r].y = fma(c, t[r].y, acc[r].y);
+        }
+    };
+    // the loads of the first RM mass-side, RX stiffness-side and RK K vectors
+    // are all issued before any is used (one HBM round trip, not one per
+    // vector); further vectors are loaded one at a time
+    constexpr int RM = 2, RX = 1, RK = 3;
+    double2 mr[RM][RPT], xr[RX][RPT], kr[RK][RPT];
+#pragma unroll
+    for (int k = 0; k < RM; ++k)
+        if (k < a.nm) load_rows(a.mv[k], mr[k]);
+#pragma unroll
+    for (int k = 0; k < RX; ++k)
+        if (k < a.nx) load_rows(a.xv[k], xr[k]);
+#pragma unroll
+    for (int k = 0; k < RK; ++k)
+        if (k < a.nk) load_rows(a.kv[k], kr[k]);
     {
         double2 u[RPT], v[RPT];
 #pragma unroll
         for (int r = 0; r < RPT; ++r) u[r] = v[r] = make_double2(0.0, 0.0);
-        for (int k = 0; k < a.nm; ++k) {
-            const double2 *src = a.mv[k] + base;
-            const double c = a.ma[k];
+#pragma unroll
+        for (int k = 0; k < RM; ++k)
+            if (k < a.nm) axpy_rows(a.ma[k], mr[k], u);
+        for (int k = RM; k < a.nm; ++k) {
             double2 t[RPT];
-#pragma unroll
-            for (int r = 0; r < RPT; ++r) {
-                const int i = tid + 256 * r;
-                t[r] = i < total ? src[i] : make_double2(0.0, 0.0);
-            }
-#pragma unroll
-            for (int r = 0; r < RPT; ++r) {
-                u[r].x = fma(c, t[r].x, u[r].x);
-                u[r].y = fma(c, t[r].y, u[r].y);
-            }
+            load_rows(a.mv[k], t);
+            axpy_rows(a.ma[k], t, u);
         }
-        for (int k = 0; k < a.nx; ++k) {
-            const double2 *src = a.xv[k] + base;
-            const double c = a.xa[k];
+#pragma unroll
+        for (int k = 0; k < RX; ++k)
+            if (k < a.nx) axpy_rows(a.xa[k], xr[k], v);
+        for (int k = RX; k < a.nx; ++k) {
             double2 t[RPT];
-#pragma unroll
-            for (int r = 0; r < RPT; ++r) {
-                const int i = tid + 256 * r;
-                t[r] = i < total ? src[i] : make_double2(0.0, 0.0);
-            }
-#pragma unroll
-            for (int r = 0; r < RPT; ++r) {
-                v[r].x = fma(c, t[r].x, v[r].x);
-                v[r].y = fma(c, t[r].y, v[r].y);
-            }
+            load_rows(a.xv[k], t);
+            axpy_rows(a.xa[k], t, v);
         }
 #pragma unroll
         for (int r = 0; r < RPT; ++r) {
@@ -1039,20 +1053,13 @@ __global__ __launch_bounds__(256) void k_rhs_sys(RhsArgs a) {
         }
         y[r] = acc;
     }
-    for (int k = 0; k < a.nk; ++k) {
-        const double2 *src = a.kv[k] + base;
-        const double c = a.kb[k];
+#pragma unroll
+    for (int k = 0; k < RK; ++k)
+        if (k < a.nk) axpy_rows(a.kb[k], kr[k], y);
+    for (int k = RK; k < a.nk; ++k) {
         double2 t[RPT];
-#pragma unroll
-        for (int r = 0; r < RPT; ++r) {
-            const int i = tid + 256 * r;
-            t[r] = i < total ? src[i] : make_double2(0.0, 0.0);
-        }
-#pragma unroll
-        for (int r = 0; r < RPT; ++r) {
-            y[r].x = fma(c, t[r].x, y[r].x);
-            y[r].y = fma(c, t[r].y, y[r].y);
-        }
+        load_rows(a.kv[k], t);
+        axpy_rows(a.kb[k], t, y);
     }
     const bool zero = a.zero00 && s == 0;
 #pragma unroll

@@ -187,6 +187,16 @@ struct coral_amd_ctx {
 
 namespace {
 
+// Clears the CFL maxima with a kernel rather than hipMemsetAsync, so that a
+// caller who captures the step into a HIP graph records no memset node.
+__global__ void k_zero_u64(unsigned long long *p, int n) {
+    if ((int)threadIdx.x < n) p[threadIdx.x] = 0ull;
+}
+void zero_cflmax(hipStream_t st, unsigned long long *p) {
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, st, p, 3);
+    HIP_OK(hipGetLastError());
+}
+
 FFTPlan make_plan(int N, DevBuf<double2> &tw) {
     FFTPlan p;
     p.N = N;
@@ -612,7 +622,7 @@ void nl_terms(coral_amd_ctx &c, const std::vector<const double2 *> &kst, const s
               int stage) {
     const Problem &P = c.P;
     backward_to_G(c, kst, zst);
-    HIP_OK(hipMemsetAsync(c.cflmax.p, 0, 3 * sizeof(unsigned long long), c.stream));
+    zero_cflmax(c.stream, c.cflmax.p);
     XPassArgs xa{};
     xa.G = c.G.p;
     xa.H = c.H.p;
@@ -1281,7 +1291,7 @@ int coral_amd_horizontal_averages(coral_amd_ctx *c, double *full_avg, double *qu
         if (quad_avg) {
             // products of the current state up to the y transform; column s = 0 is the (0,0) mode
             backward_to_G(*c, kst, zst);
-            HIP_OK(hipMemsetAsync(c->cflmax.p, 0, 3 * sizeof(unsigned long long), c->stream));
+            zero_cflmax(c->stream, c->cflmax.p);
             XPassArgs xa{};
             xa.G = c->G.p;
             xa.H = c->H.p;

@@ -27,6 +27,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "RB timesteps/s at 256²×129 FP64; sim-time/s vs plain GPU at fixed Nu error"
 PEAK_HBM_GBS = 8000.0
+PEAK_FP64_TFLOPS = 78.6   # MI355X dense FP64 (matrix)
 
 WORKLOADS = {
     # amplitude: the stated 1e-2 leaves the advective CFL bound of dt = 2e-5
@@ -77,6 +78,17 @@ def kernel_bytes(name, P):
         "nl_assemble": nq * 16 * NZ * nsys + S_sys,
     }
     return table.get(name)
+
+
+def kernel_flops(name, P):
+    """Floating-point operations per launch of the z transforms as executed on
+    FP64 MFMA: each of the re/im columns of every (quadratic or full) variable
+    and wavenumber is a product with the NZ x NZAA Chebyshev matrix, folded
+    into even and odd halves (half of 2*NZ*NZAA)."""
+    flops = {"zfwd": P["nquad"], "zbwd": P["nfull"]}
+    if name not in flops:
+        return None
+    return flops[name] * 2 * P["nsys"] * P["NZ"] * P["NZAA"]
 
 
 def cpu_baseline(wl, recipe, k0, z0, nsteps=1):
@@ -185,6 +197,21 @@ def main():
             traffic = tr.get("bytes_per_launch")
     roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                 "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic, "algorithmic_bytes": dom_bytes}
+    # per-kernel figures: algorithmic HBM GB/s of every main kernel, FP64 MFMA
+    # rate and utilisation of the z transforms, banded-solve time per wavenumber
+    kernels = {}
+    for name, v in per_kernel.items():
+        b, f = kernel_bytes(name, P), kernel_flops(name, P)
+        if b is None:
+            continue
+        k = {"ms_per_launch": round(v["ms_per_launch"], 4), "GBps": round(v["GBps"], 1),
+             "hbm_frac": round(v["GBps"] / PEAK_HBM_GBS, 4)}
+        if f is not None:
+            tf = f / (v["ms_per_launch"] * 1e-3) / 1e12
+            k.update({"TFLOPs": round(tf, 2), "mfma_fp64_frac": round(tf / PEAK_FP64_TFLOPS, 4)})
+        if name == "solve":
+            k["ns_per_wavenumber"] = round(v["ms_per_launch"] * 1e6 / nsys, 2)
+        kernels[name] = k
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -222,6 +249,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "kernels_ms_per_step": {k: round(v["ms_per_step"], 4) for k, v in per_kernel.items()},
+        "kernels": kernels,
     }
     print(json.dumps(out), flush=True)
 

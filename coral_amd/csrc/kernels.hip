@@ -1217,8 +1217,7 @@ __device__ void split_factors(const T *AB, int n, int kl, int ku, int ldab, T *L
 // m = M%4), with KLP = solve16_klp(kl):
 //   L:   column j at ((g*ncol16 + j)*4 + m)*KLP: entry k < kl is
 //        L(j + 1 + k, j) (zero for k >= n-1-j and for the padding columns
-//        j >= n), entry kl (when KLP > kl) is a zero the lanes outside the
-//        band read;
+//        j >= n);
 //   U:   column j at ((g*nb + j)*4 + m)*(kv+1), nb = 16*ceil(n/16) columns per
 //        matrix (those beyond n stay zero): entry 0 is 1/U(j,j), entry k the
 //        superdiagonal U(j-k, j).
@@ -1641,13 +1640,15 @@ struct Solve16 {
         cur = csel(lo == tp, a, cur);
         nxt = csel(lo + 16 == tp, a, nxt);
         cur = csel(lo == t, bj, cur);
-        const T c = ring[t];
+        // lanes whose row is past the band (entry (lo - t - 1) mod 16 >= kl)
+        // loaded the record's last entry and take no update
+        const T c = tsel(unsigned((lo + 15 - t) & 15) < unsigned(klp), ring[t], zero);
         const bool below = lo > t;
         cur = submul(cur, tsel(below, c, zero), bj);
         nxt = submul(nxt, tsel(below, zero, c), bj);
         asm volatile("" : "+v"(lcol) : "v"(cur.x), "v"(cur.y), "v"(nxt.x), "v"(nxt.y));
         // entry k = (lo - t - 1) mod 16 of column t of the next block; k >= kl
-        // reads the record's zero (or, for KLP = 16, is inside the band)
+        // reads entry kl - 1 (masked at its step)
         const unsigned kk = min(unsigned((lo + 15 - t) & 15), unsigned(klp - 1));
         ring[t] = ld_nt(Lg + (size_t)(lcol + kk) + t * 4 * klp);
         __builtin_amdgcn_sched_barrier(0);   // and not later: the prefetch distance stays 16 steps

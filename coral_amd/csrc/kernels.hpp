@@ -220,8 +220,9 @@ void launch_solve(hipStream_t st, const double2 *Lf, const double2 *Uf, const un
 // 16-lane solve (four systems per wavefront): columns per system of its
 // multiplier records, or 0 when the band does not fit (kl > 16, kv > 47)
 int solve16_ncol(int n, int kl, int kv);
-// entries per multiplier record of the 16-lane solve: kl and a zero, or 16
-__host__ __device__ inline int solve16_klp(int kl) { return kl >= 16 ? 16 : kl + 1; }
+// entries per multiplier record of the 16-lane solve: the kl multipliers of a
+// column (at least one entry)
+__host__ __device__ inline int solve16_klp(int kl) { return kl < 1 ? 1 : kl; }
 // U arrays of the solves carry this many elements of padding on each side (the
 // 16-lane solve reads up to 16 elements before the first record and 64 past
 // the last)

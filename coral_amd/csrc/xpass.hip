@@ -32,6 +32,12 @@ struct XPlan<96> {
     template <int S> using FwdRest = Stages<96, -1, S, 6, 4, 4>;
 };
 template <>
+struct XPlan<144> {
+    static constexpr int R0 = 4, RL = 3, LPW = 4;
+    template <int S> using InvMid = Stages<144, +1, S, 4, 4, 3>;
+    template <int S> using FwdRest = Stages<144, -1, S, 3, 3, 4, 4>;
+};
+template <>
 struct XPlan<192> {
     static constexpr int R0 = 4, RL = 3, LPW = 1;
     template <int S> using InvMid = Stages<192, +1, S, 4, 4, 4>;
@@ -336,6 +342,7 @@ bool launch_xpass_wave(hipStream_t st, const FFTPlan &px, const XPassArgs &a) {
     if (a.nfull < 1 || 2 * a.nkx > px.N) return false;
     switch (px.N) {
     case 96: return launch_wave_n<96>(st, px, a);
+    case 144: return launch_wave_n<144>(st, px, a);
     case 192: return launch_wave_n<192>(st, px, a);
     case 384: return launch_wave_n<384>(st, px, a);
     case 768: return launch_wave_n<768>(st, px, a);

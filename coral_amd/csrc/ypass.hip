@@ -27,6 +27,10 @@ struct YPlan<96> {
     template <int SIGN, int S, int LS> using Run = StagesLSZ<96, SIGN, S, LS, 1, 4, 4, 6>;
 };
 template <>
+struct YPlan<144> {
+    template <int SIGN, int S, int LS> using Run = StagesLSZ<144, SIGN, S, LS, 1, 4, 4, 3, 3>;
+};
+template <>
 struct YPlan<192> {
     template <int SIGN, int S, int LS> using Run = StagesLSZ<192, SIGN, S, LS, 1, 4, 4, 4, 3>;
 };
@@ -85,20 +89,21 @@ __global__ __launch_bounds__(64 * YWAVES) void k_ybwd_w(const double2 *__restric
     __syncthreads();
     double2 *dst = G + bz * N * nkx + ix0;
     {
-        constexpr int PER = N * YL / YT;
-        static_assert(N * YL % YT == 0, "tile must split evenly over the threads");
+        constexpr int PER = (N * YL + YT - 1) / YT;   // the last round partial when YT does not divide N*YL
         double2 v[PER];
 #pragma unroll
         for (int i = 0; i < PER; ++i) {
             const int e = threadIdx.x + i * YT;
             const int y = e / YL, l = e - y * YL;
-            v[i] = buf[l * LS + swz<YPlan<N>::template Run<+1, LPW, LS>::OUT_SWZ>(y)];
+            v[i] = (N * YL % YT == 0 || e < N * YL)
+                       ? buf[l * LS + swz<YPlan<N>::template Run<+1, LPW, LS>::OUT_SWZ>(y)]
+                       : make_double2(0.0, 0.0);
         }
 #pragma unroll
         for (int i = 0; i < PER; ++i) {
             const int e = threadIdx.x + i * YT;
             const int y = e / YL, l = e - y * YL;
-            if (l < nl) dst[(size_t)y * nkx + l] = v[i];
+            if ((N * YL % YT == 0 || e < N * YL) && l < nl) dst[(size_t)y * nkx + l] = v[i];
         }
     }
 }
@@ -117,20 +122,19 @@ __global__ __launch_bounds__(64 * YWAVES) void k_yfwd_w(const double2 *__restric
     // all of a thread's loads are issued before the first LDS store
     const double2 *src = H + bz * N * nkx + ix0;
     {
-        constexpr int PER = N * YL / YT;
-        static_assert(N * YL % YT == 0, "tile must split evenly over the threads");
+        constexpr int PER = (N * YL + YT - 1) / YT;   // the last round partial when YT does not divide N*YL
         double2 v[PER];
 #pragma unroll
         for (int i = 0; i < PER; ++i) {
             const int e = threadIdx.x + i * YT;
             const int y = e / YL, l = e - y * YL;
-            v[i] = l < nl ? src[(size_t)y * nkx + l] : make_double2(0.0, 0.0);
+            v[i] = ((N * YL % YT == 0 || e < N * YL) && l < nl) ? src[(size_t)y * nkx + l] : make_double2(0.0, 0.0);
         }
 #pragma unroll
         for (int i = 0; i < PER; ++i) {
             const int e = threadIdx.x + i * YT;
             const int y = e / YL, l = e - y * YL;
-            buf[l * LS + y] = v[i];
+            if (N * YL % YT == 0 || e < N * YL) buf[l * LS + y] = v[i];
         }
     }
     __syncthreads();
@@ -175,6 +179,7 @@ bool launch_ypass_wave(hipStream_t st, bool fwd, const FFTPlan &py, const double
     if (nky > py.N || NY / 2 + (py.N - nky) > py.N) return false;
     switch (py.N) {
     case 96: go<96>(st, fwd, py, in, out, nbatch, NZAA, nkx, nky, NY); return true;
+    case 144: go<144>(st, fwd, py, in, out, nbatch, NZAA, nkx, nky, NY); return true;
     case 192: go<192>(st, fwd, py, in, out, nbatch, NZAA, nkx, nky, NY); return true;
     case 384: go<384>(st, fwd, py, in, out, nbatch, NZAA, nkx, nky, NY); return true;
     default: return false;  // 768: the 8-line tile exceeds 64 KiB of LDS

@@ -45,6 +45,9 @@ CASES = [
     ("coral.equations.rbc_ra1e5", 32, 32, 65, 2 * np.pi, 2 * np.pi, "ARS_222", 1e-4, 2),
     ("coral.equations.rbc_ra1e5", 16, 32, 49, 2.0, 2 * np.pi, "ARS_343", 1e-4, 2),
     ("coral.equations.rrbc_ra5e7_ek1e-4", 16, 16, 257, 2 * np.pi, 2 * np.pi, "ARS_222", 5e-6, 2),
+    # the case of Coral's published timing (etc/benchmarks/02_RRBC_julien_JFM96:
+    # 96x96x48 modes, 144-point horizontal lines, box 2x2x1, Ra=2.81e5, 1/Ek=707.1)
+    ("coral.equations.rrbc_ra2.81e5_iek707", 96, 96, 48, 2.0, 2.0, "ARS_222", 1e-3, 2),
 ]
 
 

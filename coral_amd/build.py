@@ -77,13 +77,14 @@ def build_fortran_host() -> str:
     with amdflang and links the Fortran test programs under ``tests/fortran``
     against the library (run ``build`` first): ``binding_layout`` (layout of
     the bind(C) types), ``banded_kat`` (a Fortran host calling the banded
-    solve), and the C layout probe ``c_layout``."""
+    solve), ``step_driver`` (a Fortran host running whole time steps), and the
+    C layout probe ``c_layout``."""
     os.makedirs(FORTRAN_BUILD, exist_ok=True)
     mod_obj = os.path.join(FORTRAN_BUILD, "coral_amd_binding.o")
     _run([FC, "-O2", "-c", os.path.join(REPO, "include", "coral_amd_binding.f90"), "-J", FORTRAN_BUILD,
           "-o", mod_obj])
     link = ["-L", LIBDIR, "-lcoral_amd", "-Wl,-rpath,$ORIGIN/../../../coral_amd/lib"]
-    for prog in ("binding_layout", "banded_kat"):
+    for prog in ("binding_layout", "banded_kat", "step_driver"):
         _run([FC, "-O2", "-I", FORTRAN_BUILD, os.path.join(FORTRAN_DIR, prog + ".f90"), mod_obj, *link,
               "-o", os.path.join(FORTRAN_BUILD, prog)])
     _run([CC, "-std=c11", "-Wall", "-I", os.path.join(REPO, "include"), os.path.join(FORTRAN_DIR, "c_layout.c"),

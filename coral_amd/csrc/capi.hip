@@ -139,6 +139,20 @@ struct coral_amd_ctx {
     double dt = 0.0;
     bool factored = false;
     bool timing = false;
+    // HIP graphs of one step: one_step swaps each system's field and last
+    // stage buffers, so two graphs (one per buffer assignment) alternate;
+    // gpar is the graph whose assignment the host pointers hold now
+    bool use_graphs = false;
+    hipGraphExec_t gexec[2] = {nullptr, nullptr};
+    int gpar = 0;
+    void drop_graphs() {
+        for (auto &g : gexec)
+            if (g) {
+                (void)hipGraphExecDestroy(g);
+                g = nullptr;
+            }
+        gpar = 0;
+    }
     std::vector<TimerRec> recs;
     std::map<std::string, std::pair<double, int>> totals;
 
@@ -147,6 +161,7 @@ struct coral_amd_ctx {
             (void)hipEventDestroy(r.a);
             (void)hipEventDestroy(r.b);
         }
+        drop_graphs();
         if (ev_qhat) (void)hipEventDestroy(ev_qhat);
         if (ev_mean) (void)hipEventDestroy(ev_mean);
         if (side) (void)hipStreamDestroy(side);
@@ -560,6 +575,14 @@ void join_mean(coral_amd_ctx &c) {
     c.mean_pending = false;
 }
 
+// x^{n+1} = x_s: the last stage's buffer becomes the state, the state's
+// buffer the next step's last stage
+void swap_step_buffers(coral_amd_ctx &c) {
+    const int s = c.P.nstages;
+    for (auto &d : c.ks) std::swap(d->field.p, d->X[s - 1]->p);
+    for (auto &d : c.zs) std::swap(d->field.p, d->X[s - 1]->p);
+}
+
 // f's launches (and timer events) go to the side stream
 template <class F>
 void on_side(coral_amd_ctx &c, F &&f) {
@@ -730,8 +753,36 @@ void one_step(coral_amd_ctx &c) {
         for (auto &d : c.ks) linear(*d);
     }
     join_mean(c);
-    for (auto &d : c.ks) std::swap(d->field.p, d->X[s - 1]->p);
-    for (auto &d : c.zs) std::swap(d->field.p, d->X[s - 1]->p);
+    swap_step_buffers(c);
+}
+
+// Two graphs of one step each, captured from the stream (the side stream
+// joins through its events).  Capturing runs one_step's host logic, which
+// swaps the step buffers; after both captures they are back where they were.
+// The step holds no memset and no allocation, so the graphs replay it
+// exactly; they are dropped when dt changes.
+void capture_step_graphs(coral_amd_ctx &c) {
+    c.drop_graphs();
+    for (int p = 0; p < 2; ++p) {
+        HIP_OK(hipStreamBeginCapture(c.stream, hipStreamCaptureModeRelaxed));
+        hipGraph_t g = nullptr;
+        try {
+            one_step(c);
+        } catch (...) {
+            (void)hipStreamEndCapture(c.stream, &g);
+            if (g) (void)hipGraphDestroy(g);
+            c.drop_graphs();
+            throw;
+        }
+        HIP_OK(hipStreamEndCapture(c.stream, &g));
+        const hipError_t e = hipGraphInstantiate(&c.gexec[p], g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e != hipSuccess) {
+            c.drop_graphs();
+            throw HipError(std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+        }
+    }
+    c.gpar = 0;
 }
 
 void ensure_phys(coral_amd_ctx &c) {
@@ -902,6 +953,7 @@ int coral_amd_set_dt(coral_amd_ctx *c, double dt) {
         if (!c) throw std::runtime_error("null context");
         if (!(dt > 0.0) || !std::isfinite(dt)) throw std::runtime_error("dt must be positive and finite");
         HIP_OK(hipSetDevice(c->device));
+        c->drop_graphs();   // the factors may be reallocated and dt enters the launches
         for (auto &d : c->ks) factorize(*c, *d, dt);
         for (auto &d : c->zs) factorize(*c, *d, dt);
         c->dt = dt;
@@ -916,6 +968,10 @@ int coral_amd_upload_state(coral_amd_ctx *c, int32_t kind, int32_t isys, const d
     try {
         if (!c || !data) throw std::runtime_error("null argument");
         HIP_OK(hipSetDevice(c->device));
+        // the copies below go through the null stream, which does not wait
+        // for the context's non-blocking streams: finish queued steps first
+        HIP_OK(hipStreamSynchronize(c->stream));
+        HIP_OK(hipStreamSynchronize(c->side));
         const Problem &P = c->P;
         if (kind == 0) {
             if (isys < 0 || isys >= int(c->ks.size())) throw std::runtime_error("bad kxky system index");
@@ -989,7 +1045,16 @@ int coral_amd_step(coral_amd_ctx *c, int32_t nsteps) {
         if (!c->factored) throw std::runtime_error("coral_amd_set_dt must be called before stepping");
         if (nsteps < 0) throw std::runtime_error("nsteps must be >= 0");
         HIP_OK(hipSetDevice(c->device));
-        for (int n = 0; n < nsteps; ++n) one_step(*c);
+        if (c->use_graphs && !c->timing && nsteps > 0) {
+            if (!c->gexec[0]) capture_step_graphs(*c);
+            for (int n = 0; n < nsteps; ++n) {
+                HIP_OK(hipGraphLaunch(c->gexec[c->gpar], c->stream));
+                swap_step_buffers(*c);
+                c->gpar ^= 1;
+            }
+        } else {
+            for (int n = 0; n < nsteps; ++n) one_step(*c);
+        }
         HIP_OK(hipGetLastError());
         return 0;
     } catch (const std::exception &e) {
@@ -1463,6 +1528,19 @@ int coral_amd_assembled_operator(const coral_amd_problem *prob, int32_t isys, in
                     out[2 * (size_t(i) * n + j) + 1] += b * cf.y;
                 }
         }
+        return 0;
+    } catch (const std::exception &e) {
+        return fail(e);
+    }
+}
+
+int coral_amd_use_graphs(coral_amd_ctx *c, int32_t enable) {
+    try {
+        if (!c) throw std::runtime_error("null context");
+        HIP_OK(hipSetDevice(c->device));
+        HIP_OK(hipStreamSynchronize(c->stream));
+        c->use_graphs = enable != 0;
+        c->drop_graphs();
         return 0;
     } catch (const std::exception &e) {
         return fail(e);

@@ -265,6 +265,11 @@ class Solver:
         return out
 
     # --- timing ---------------------------------------------------------------------
+    def use_graphs(self, on: bool = True):
+        """Replay steps as HIP graphs (captured at the next step); results are
+        the eager ones bit for bit."""
+        check(lib.coral_amd_use_graphs(self._ctx, 1 if on else 0), "coral_amd_use_graphs")
+
     def timers_enable(self, on: bool = True):
         check(lib.coral_amd_timers_enable(self._ctx, 1 if on else 0), "coral_amd_timers_enable")
 

@@ -208,6 +208,13 @@ int coral_amd_banded_solve(int32_t device, int32_t n, int32_t kl, int32_t ku, in
 int coral_amd_assembled_operator(const coral_amd_problem *prob, int32_t isys, int32_t which, int32_t kx_int,
                                  int32_t ky_int, double *out, int32_t *total, int32_t *nl, int32_t *nu);
 
+/* enable=1: coral_amd_step replays one step as a HIP graph (captured at the
+ * next step, dropped by coral_amd_set_dt) instead of launching its kernels one
+ * by one; same results bit for bit, less launch overhead on small grids.  No
+ * counterpart in the reference (its time loop calls march_forward,
+ * plane_layer_main.f90:157).  While timers are enabled steps run eagerly. */
+int coral_amd_use_graphs(coral_amd_ctx *ctx, int32_t enable);
+
 /* Per-kernel timing with HIP events on the context's stream.  enable=1 resets
  * and starts accumulating; names are filled by coral_amd_timer_read. */
 int coral_amd_timers_enable(coral_amd_ctx *ctx, int32_t enable);

@@ -90,6 +90,12 @@ module coral_amd_binding
        import
        type(c_ptr), value :: ctx
      end function
+     ! steps replayed as HIP graphs (same results, less launch overhead)
+     integer(c_int) function coral_amd_use_graphs(ctx, enable) bind(C)
+       import
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: enable
+     end function
      integer(c_int) function coral_amd_get_cfl_dt(ctx, dt) bind(C)
        import
        type(c_ptr), value :: ctx

@@ -447,3 +447,37 @@ def test_structural_blocks_match_whole_system_factors(solver_mod, monkeypatch):
     for it in range(nsteps):
         ora.step(dt)
     assert rel(kb[0], ora.field[0]) <= TOL
+
+
+def test_graph_replay_matches_eager(solver_mod):
+    """Steps replayed as HIP graphs (Solver.use_graphs) against eager steps, bit
+    for bit: 2 + 3 replays (both buffer assignments), state downloads between
+    them, a re-upload, then a change of dt (which drops and recaptures the
+    graphs); the mean systems' side stream is part of every graph."""
+    name, NX, NY, NZ, L, dt = "coral.equations.rrbc_ra5e7_ek1e-4", 64, 64, 33, 2 * np.pi, 2e-5
+    r = product_recipe(name)
+    k0, z0 = ic.rbc_initial_state(r, NX, NY, NZ, 1e-2, 10 * NX, 8, 11)
+    plan = [("run", dt, 2), ("run", dt, 3), ("reset", None, None), ("run", dt, 1), ("run", dt / 2, 2)]
+    results = []
+    for graphs in (False, True):
+        out = []
+        with solver_mod.Solver(r, NX, NY, NZ, L, L, scheme="ARS_222") as g:
+            if graphs:
+                g.use_graphs(True)
+            g.set_state(k0, z0)
+            for what, h, n in plan:
+                if what == "reset":
+                    g.set_state(k0, z0)
+                    continue
+                print("graphs", graphs, "dt", h, "steps", n, flush=True)
+                g.march_forward(h, n)
+                k, z = g.get_state()
+                out.append((k, z, g.cfl_dt()))
+        results.append(out)
+    for (ke, ze, ce), (kg, zg, cg) in zip(*results):
+        for a, b in zip(ke, kg):
+            assert np.array_equal(a, b)
+        for a, b in zip(ze, zg):
+            assert np.array_equal(a, b)
+        assert ce == cg
+    assert np.all(np.isfinite(results[1][-1][0][0]))

@@ -144,12 +144,14 @@ __device__ inline void twiddle_powers(const double2 *__restrict__ tw, int m, dou
 //   W = 1: stride-4 writes (radix 4, NS = 1):  slot ^= row & 3
 //   W = 2: radix 4 after NS = 4:               slot ^= 4 * ((row >> 1) & 1)
 //   W = 3: stride-6 writes (radix 6, NS = 1):  slot ^= row & 1
+//   W = 4: stride-8 writes (radix 8, NS = 1):  slot ^= row & 7
 // (row = e / 8); other stages write whole rows and keep W = 0.
 template <int W>
 __device__ inline int swz(int e) {
     if constexpr (W == 1) return e ^ ((e >> 3) & 3);
     else if constexpr (W == 2) return e ^ (((e >> 4) & 1) << 2);
     else if constexpr (W == 3) return e ^ ((e >> 3) & 1);
+    else if constexpr (W == 4) return e ^ ((e >> 3) & 7);
     else return e;
 }
 // swizzle a stage with radix R after NS points writes with

@@ -22,36 +22,42 @@ using wfft::Stages;
 // InvMid runs in LDS, the last stage RL leaves each lane holding the points
 // j + r*N/RL of its lines.  Forward: first stage RL starts from those
 // registers, FwdRest finishes in LDS in natural order.  LPW lines per
-// wavefront make LPW * N / RL a multiple of 64.
+// wavefront make LPW * N / RL a multiple of 64.  W0 / WL: the swizzles
+// (wfft::swz) of the first inverse and first forward stages' LDS writes,
+// which InvMid / FwdRest read through.
 template <int N>
 struct XPlan;
 template <>
 struct XPlan<96> {
-    static constexpr int R0 = 4, RL = 6, LPW = 4;
+    static constexpr int R0 = 4, RL = 6, LPW = 4, W0 = 0, WL = 0;
     template <int S> using InvMid = Stages<96, +1, S, 4, 4>;
     template <int S> using FwdRest = Stages<96, -1, S, 6, 4, 4>;
 };
 template <>
 struct XPlan<144> {
-    static constexpr int R0 = 4, RL = 3, LPW = 4;
+    static constexpr int R0 = 4, RL = 3, LPW = 4, W0 = 0, WL = 0;
     template <int S> using InvMid = Stages<144, +1, S, 4, 4, 3>;
     template <int S> using FwdRest = Stages<144, -1, S, 3, 3, 4, 4>;
 };
 template <>
 struct XPlan<192> {
-    static constexpr int R0 = 4, RL = 3, LPW = 1;
+    static constexpr int R0 = 4, RL = 3, LPW = 1, W0 = 0, WL = 0;
     template <int S> using InvMid = Stages<192, +1, S, 4, 4, 4>;
     template <int S> using FwdRest = Stages<192, -1, S, 3, 4, 4, 4>;
 };
 template <>
 struct XPlan<384> {
-    static constexpr int R0 = 8, RL = 6, LPW = 1;
-    template <int S> using InvMid = Stages<384, +1, S, 8, 8>;
-    template <int S> using FwdRest = Stages<384, -1, S, 6, 8, 8>;
+    // the stride-8 writes of the first inverse stage put all 8 lanes of a
+    // row of eight on the same banks: swizzled (0.613 -> 0.599 ms per
+    // launch); the 2-way conflicts of the first forward stage's stride-6
+    // writes cost less than their swizzle (0.624 ms with it)
+    static constexpr int R0 = 8, RL = 6, LPW = 1, W0 = 4, WL = 0;
+    template <int S> using InvMid = wfft::StagesLSW<384, +1, S, 384, W0, false, 8, 8>;
+    template <int S> using FwdRest = wfft::StagesLSW<384, -1, S, 384, WL, false, 6, 8, 8>;
 };
 template <>
 struct XPlan<768> {
-    static constexpr int R0 = 8, RL = 6, LPW = 1;
+    static constexpr int R0 = 8, RL = 6, LPW = 1, W0 = 0, WL = 0;
     template <int S> using InvMid = Stages<768, +1, S, 8, 4, 4>;
     template <int S> using FwdRest = Stages<768, -1, S, 6, 4, 4, 8>;
 };
@@ -134,9 +140,9 @@ __device__ inline void xfwd_batch(const XPassArgs &a, const double2 (&fv)[XShape
 #pragma unroll
             for (int r = 0; r < RL; ++r) y[r] = make_double2(p0[r], p1[r]);
             Dft<RL, -1>::run(y);
-            double2 *L = buf + (l * G + gp) * N + j * RL;
+            double2 *L = buf + (l * G + gp) * N;
 #pragma unroll
-            for (int r = 0; r < RL; ++r) L[r] = y[r];
+            for (int r = 0; r < RL; ++r) L[wfft::swz<XPlan<N>::WL>(j * RL + r)] = y[r];
         }
     }
     wfft::wave_sync();
@@ -235,7 +241,7 @@ __global__ __launch_bounds__(256) void k_xpass_wave(XPassArgs a, const double2 *
                 if (TOT % 64 == 0 || b < TOT) {
                     const int l = b / T0, j = b - l * T0;
 #pragma unroll
-                    for (int r = 0; r < R0; ++r) buf[l * N + j * R0 + r] = v[c][r];
+                    for (int r = 0; r < R0; ++r) buf[l * N + wfft::swz<XPlan<N>::W0>(j * R0 + r)] = v[c][r];
                 }
             }
             wfft::wave_sync();

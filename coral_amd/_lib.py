@@ -61,7 +61,7 @@ EXPORTS = (
     "coral_amd_upload_state", "coral_amd_download_state", "coral_amd_step", "coral_amd_synchronize",
     "coral_amd_get_physical", "coral_amd_get_cfl_dt", "coral_amd_wavenumbers", "coral_amd_system_info",
     "coral_amd_bwd_transform", "coral_amd_fwd_transform", "coral_amd_banded_solve",
-    "coral_amd_timers_enable", "coral_amd_timer_read", "coral_amd_assembled_operator", "coral_amd_use_graphs",
+    "coral_amd_timers_enable", "coral_amd_timer_read", "coral_amd_assembled_operator", "coral_amd_use_graphs", "coral_amd_products",
     "coral_amd_factor_kind", "coral_amd_get_variable_physical", "coral_amd_set_variable_physical",
     "coral_amd_horizontal_averages", "coral_amd_factor_blocks",
 )
@@ -92,6 +92,7 @@ def _load():
     lib.coral_amd_banded_solve.argtypes = [c_int32, c_int32, c_int32, c_int32, c_int32, P, P, P]
     lib.coral_amd_timers_enable.argtypes = [P, c_int32]
     lib.coral_amd_use_graphs.argtypes = [P, c_int32]
+    lib.coral_amd_products.argtypes = [P, P]
     lib.coral_amd_timer_read.argtypes = [P, c_int32, ctypes.c_char_p, c_int32, ctypes.POINTER(c_double),
                                          ctypes.POINTER(c_int32)]
     lib.coral_amd_assembled_operator.argtypes = [ctypes.POINTER(Problem), c_int32, c_int32, c_int32, c_int32, P,

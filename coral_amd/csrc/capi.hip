@@ -641,8 +641,12 @@ void backward_to_G(coral_amd_ctx &c, const std::vector<const double2 *> &kst, co
     c.timed("ybwd", [&] { launch_ybwd(c.stream, c.py, c.Fz.p, c.G.p, P.nfull, P.NZAA, P.nkx, P.nky, P.NY); });
 }
 
-void nl_terms(coral_amd_ctx &c, const std::vector<const double2 *> &kst, const std::vector<const double2 *> &zst,
-              int stage) {
+// The quadratic variables of the given states in Qhat ([quad][NZ][s]): full
+// variables to the 3/2 grid, products, forward transforms
+// (marching.f90:279-300), z-integral removal (PL_IMEX_timestepping.f90:223-235);
+// the CFL maxima of the full variables on the way.
+void products_to_Qhat(coral_amd_ctx &c, const std::vector<const double2 *> &kst,
+                      const std::vector<const double2 *> &zst) {
     const Problem &P = c.P;
     backward_to_G(c, kst, zst);
     zero_cflmax(c.stream, c.cflmax.p);
@@ -663,6 +667,11 @@ void nl_terms(coral_amd_ctx &c, const std::vector<const double2 *> &kst, const s
     c.timed("zfwd", [&] { zfwd(c, c.Hy.p, c.Qhat.p, P.nquad); });
     for (int q = 0; q < P.nquad; ++q)
         if (P.quad_filter[q]) launch_remove_zmean(c.stream, c.Qhat.p + size_t(q) * P.nsys * P.NZ, P.NZ, P.nsys);
+}
+
+void nl_terms(coral_amd_ctx &c, const std::vector<const double2 *> &kst, const std::vector<const double2 *> &zst,
+              int stage) {
+    products_to_Qhat(c, kst, zst);
     for (auto &d : c.ks) nl_assemble(c, *d, d->Khat[stage]->p);
     if (!c.zs.empty()) HIP_OK(hipEventRecord(c.ev_qhat, c.stream));
 }
@@ -1398,6 +1407,40 @@ int coral_amd_horizontal_averages(coral_amd_ctx *c, double *full_avg, double *qu
                 for (int z = 0; z < P.NZAA; ++z)
                     quad_avg[size_t(q) * P.NZAA + z] = h[size_t(nf) * P.NZ + size_t(q) * P.NZAA + z].x * inv;
         }
+        return 0;
+    } catch (const std::exception &e) {
+        return fail(e);
+    }
+}
+
+int coral_amd_products(coral_amd_ctx *c, double *out) {
+    try {
+        if (!c || !out) throw std::runtime_error("null argument");
+        const Problem &P = c->P;
+        HIP_OK(hipSetDevice(c->device));
+        if (P.nquad == 0) return 0;
+        std::vector<const double2 *> kst, zst;
+        for (auto &d : c->ks) kst.push_back(d->field.p);
+        for (auto &d : c->zs) zst.push_back(d->field.p);
+        products_to_Qhat(*c, kst, zst);
+        join_mean(*c);
+        HIP_OK(hipStreamSynchronize(c->stream));
+        std::vector<double2> h(size_t(P.nquad) * P.NZ * P.nsys);
+        HIP_OK(hipMemcpy(h.data(), c->Qhat.p, h.size() * sizeof(double2), hipMemcpyDeviceToHost));
+        // [q][ix][y slot][n], dealiased ky slots zero
+        const size_t per_q = size_t(P.nkx) * P.NYAA * P.NZ;
+        std::fill(out, out + 2 * per_q * P.nquad, 0.0);
+        for (int q = 0; q < P.nquad; ++q)
+            for (int ix = 0; ix < P.nkx; ++ix)
+                for (int jy = 0; jy < P.nky; ++jy) {
+                    const size_t sy = size_t(ix) * P.nky + jy;
+                    const size_t dst = size_t(q) * per_q + (size_t(ix) * P.NYAA + P.ky_slot[jy]) * P.NZ;
+                    for (int n = 0; n < P.NZ; ++n) {
+                        const double2 v = h[(size_t(q) * P.NZ + n) * P.nsys + sy];
+                        out[2 * (dst + n)] = v.x;
+                        out[2 * (dst + n) + 1] = v.y;
+                    }
+                }
         return 0;
     } catch (const std::exception &e) {
         return fail(e);

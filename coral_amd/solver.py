@@ -265,6 +265,14 @@ class Solver:
         return out
 
     # --- timing ---------------------------------------------------------------------
+    def products(self) -> np.ndarray:
+        """Spectra of the quadratic variables of the current state,
+        complex (nquad, NX/2, NYAA, NZ); dealiased ky slots are zero."""
+        nq = len(self.recipe.quad_vars)
+        out = np.zeros((nq, self.NX // 2, self.NYAA, self.NZ), np.complex128)
+        check(lib.coral_amd_products(self._ctx, out.ctypes.data), "coral_amd_products")
+        return out
+
     def use_graphs(self, on: bool = True):
         """Replay steps as HIP graphs (captured at the next step); results are
         the eager ones bit for bit."""

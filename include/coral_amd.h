@@ -189,6 +189,13 @@ int coral_amd_horizontal_averages(coral_amd_ctx *ctx, double *full_avg, double *
  * physical C shape [NXAA][NYAA][NZAA] (c2r_transform, slabs.fftw3mpi/transforms.f90:155-225);
  * dealiased slots of the input are ignored.  Forward is its inverse on the kept
  * modes (r2c_transform, :227-301); dealiased output slots are set to zero. */
+/* Quadratic variables of the current state, as the step forms them: full
+ * variables on the 3/2 grid, their products (marching.f90:279-300), forward
+ * transforms (r2c_transform, slabs.fftw3mpi/transforms.f90:227-301) and the
+ * z-integral removal (PL_IMEX_timestepping.f90:223-235).  out: complex,
+ * [nquad][NX/2][NYAA][NZ] (the reference's spectral layout), dealiased ky
+ * slots zero. */
+int coral_amd_products(coral_amd_ctx *ctx, double *out);
 int coral_amd_bwd_transform(coral_amd_ctx *ctx, const double *spec, double *phys);
 int coral_amd_fwd_transform(coral_amd_ctx *ctx, const double *phys, double *spec);
 

@@ -610,7 +610,10 @@ class OracleSolver:
         return 1.0 / m if m > 0 else math.inf
 
     # --- NL terms (marching.f90:257-418) -----------------------------------
-    def nl_terms(self, aux, zaux):
+    def quad_spectra(self, aux, zaux):
+        """Spectra of the quadratic variables (marching.f90:279-300 and the
+        z-integral removal of PL_IMEX_timestepping.f90:223-235), complex
+        (nkx, NYAA, NZ) each; sets cfl_dt from the full variables."""
         g = self.geo
         specs = self.full_variables_spec(aux, zaux)
         phys = [spec_to_phys(s, g) for s in specs]
@@ -630,6 +633,11 @@ class OracleSolver:
                 for k in range(1, g.NZ // 2):
                     s[:, :, 0] -= s[:, :, 2 * k] * (0.5 / (2.0 * k + 1.0) - 0.5 / (2.0 * k - 1.0))
             qhat.append(s)
+        return qhat
+
+    def nl_terms(self, aux, zaux):
+        g = self.geo
+        qhat = self.quad_spectra(aux, zaux)
         khat = []
         for isys, ops in enumerate(self.kops):
             K = np.zeros((g.nkx, g.nky, ops.total), np.complex128)

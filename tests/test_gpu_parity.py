@@ -481,3 +481,27 @@ def test_graph_replay_matches_eager(solver_mod):
             assert np.array_equal(a, b)
         assert ce == cg
     assert np.all(np.isfinite(results[1][-1][0][0]))
+
+
+@pytest.mark.parametrize("case", [CASES[2], BROADBAND[1]], ids=["rrbc-32x32x25", "rbc-128x96x17"])
+def test_products_seam_matches_oracle(solver_mod, case):
+    """coral_amd_products: the quadratic variables' spectra of a state (full
+    variables to the 3/2 grid, products, forward transforms, z-integral
+    removal) against the oracle's, at every kept mode; dealiased ky slots are
+    zero."""
+    name, NX, NY, NZ, Lx, Ly, scheme, dt, nsteps = case
+    r = product_recipe(name)
+    k0, z0 = ic.rbc_initial_state(r, NX, NY, NZ, 1e-2, 10 * max(NX, NY), 8, 11)
+    geo = O.Geometry(NX, NY, NZ, Lx, Ly)
+    ora = O.OracleSolver(r, geo, scheme)
+    qref = ora.quad_spectra([a.copy() for a in k0], [a.copy() for a in z0])
+    with solver_mod.Solver(r, NX, NY, NZ, Lx, Ly, scheme=scheme) as g:
+        g.set_state(k0, z0)
+        q = g.products()
+    wn = np.zeros(q.shape[2], bool)
+    wn[: NY // 2] = True
+    wn[q.shape[2] - (NY // 2 - 1):] = True
+    assert len(qref) == q.shape[0] == len(r.quad_vars)
+    for i in range(q.shape[0]):
+        assert rel(q[i][:, wn], qref[i][:, wn]) <= TOL
+        assert not np.any(q[i][:, ~wn])

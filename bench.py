@@ -77,7 +77,7 @@ def kernel_bytes(name, P):
     S_sys = 16 * N * nsys
     table = {
         "solve": solve_bytes(N, nsys, P["real"], P["blocks"], P["u_entries"]),
-        # M x^n + L (dt a x_1) + K_hat terms: ARS_222 reads 2 (stage 1) and 4 (stage 2)
+        # M (a x^n + b x_1) + K_hat terms: ARS_222 reads 2 (stage 1) and 4 (stage 2)
         # state vectors and writes one, 4 on average per launch
         "rhs": 4 * S_sys,
         "extract": S_sys + nf * 16 * NZ * nsys,              # state read, spectra written

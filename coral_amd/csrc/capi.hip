@@ -693,27 +693,44 @@ void one_step(coral_amd_ctx &c) {
     const double dt = c.dt;
     auto A = [&](int i, int j) { return P.a_arr[i * s + j]; };
     auto Ah = [&](int i, int j) { return P.a_hat[i * (s + 1) + j]; };
+    // rhs_i = M x^n + sum_{j<i} dt a_ij L x_j + sum_{j<=i} dt ah_{i+1,j} K_hat_j.
+    // Where a_jj != 0, dt L x_j = (M x_j - rhs_j) / a_jj, and rhs_j itself is
+    // such a combination, so every rhs_i is a combination of M x^n, M x_j,
+    // L x_j (only where a_jj = 0) and K_hat_j alone: no earlier rhs_j is read
+    // back.  mc[0] multiplies M x^n, mc[1 + j] M x_j, xc[j] L x_j, kc[j] K_hat_j.
+    std::vector<std::vector<double>> Rm(s, std::vector<double>(s + 1, 0.0)), Rx(s, std::vector<double>(s, 0.0)),
+        Rk(s, std::vector<double>(s, 0.0));
+    for (int i = 0; i < s; ++i) {
+        std::vector<double> &mc = Rm[i], &xc = Rx[i], &kc = Rk[i];
+        mc[0] = 1.0;
+        for (int j = 0; j < i; ++j) {
+            if (A(i, j) == 0.0) continue;
+            if (A(j, j) != 0.0) {
+                const double cij = A(i, j) / A(j, j);
+                mc[1 + j] += cij;
+                for (int q = 0; q <= s; ++q) mc[q] -= cij * Rm[j][q];
+                for (int q = 0; q < s; ++q) xc[q] -= cij * Rx[j][q];
+                for (int q = 0; q < s; ++q) kc[q] -= cij * Rk[j][q];
+            } else {
+                xc[j] += dt * A(i, j);
+            }
+        }
+        for (int j = 0; j <= i; ++j) kc[j] += dt * Ah(i + 1, j);
+    }
     for (int i = 0; i < s; ++i) {
         std::vector<const double2 *> kst, zst;
         for (auto &d : c.ks) kst.push_back(i == 0 ? d->field.p : d->X[i - 1]->p);
         for (auto &d : c.zs) zst.push_back(i == 0 ? d->field.p : d->X[i - 1]->p);
         nl_terms(c, kst, zst, i);
         auto linear = [&](SysDev &d) {
-            // M x^n + sum_j dt a_ij L x_j + sum_j dt ah_{i+1,j} K_hat_j, with
-            // dt L x_j = (M x_j - rhs_j) / a_jj where a_jj != 0
-            std::vector<std::pair<const double2 *, double>> ms{{d.field.p, 1.0}}, xs, ks;
+            std::vector<std::pair<const double2 *, double>> ms, xs, ks;
+            if (Rm[i][0] != 0.0) ms.push_back({d.field.p, Rm[i][0]});
             for (int j = 0; j < i; ++j) {
-                if (A(i, j) == 0.0) continue;
-                if (A(j, j) != 0.0) {
-                    const double cij = A(i, j) / A(j, j);
-                    ms.push_back({d.X[j]->p, cij});
-                    ks.push_back({d.R[j]->p, -cij});
-                } else {
-                    xs.push_back({d.X[j]->p, dt * A(i, j)});
-                }
+                if (Rm[i][1 + j] != 0.0) ms.push_back({d.X[j]->p, Rm[i][1 + j]});
+                if (Rx[i][j] != 0.0) xs.push_back({d.X[j]->p, Rx[i][j]});
             }
             for (int j = 0; j <= i; ++j)
-                if (Ah(i + 1, j) != 0.0) ks.push_back({d.Khat[j]->p, dt * Ah(i + 1, j)});
+                if (Rk[i][j] != 0.0) ks.push_back({d.Khat[j]->p, Rk[i][j]});
             double2 *rhs_i = d.R[i]->p;
             rhs(c, d, ms, xs, ks, rhs_i);
             const SystemTables &t = *d.t;

@@ -107,6 +107,12 @@ module coral_amd_binding
        integer(c_int32_t), value :: ifull
        type(c_ptr), value :: out
      end function
+     ! spectra of the quadratic variables of the current state, complex
+     ! (NZ, NYAA, NX/2, nquad) in Fortran order
+     integer(c_int) function coral_amd_products(ctx, out) bind(C)
+       import
+       type(c_ptr), value :: ctx, out
+     end function
      ! batched zgbtrf + zgbtrs on the device (PL_algebra_z_3D.f90:189-276):
      ! ab holds nsys complex band matrices in LAPACK storage (ldab = 2kl+ku+1),
      ! x the right-hand sides, overwritten with the solutions

@@ -58,8 +58,9 @@ struct SysDev {
     SystemTables *t = nullptr;
     bool kxky = true;
     int nsys = 0;
-    DevBuf<double2> field, aux;
-    std::vector<std::unique_ptr<DevBuf<double2>>> Khat, X, R;   // per stage: K_hat, solution, right-hand side
+    DevBuf<double2> field;
+    std::vector<std::unique_ptr<DevBuf<double2>>> Khat, X;   // per stage: K_hat, solution
+    DevBuf<double2> R;   // right-hand side of the current stage (never read back by a later stage)
     DevBuf<double2> cmass, cstif;                              // [nsys][ngroups] group coefficients
     DevBuf<double2> Lf, Uf;      // complex factors
     bool real = false;           // factored in real form (Lr, Ur, rot)
@@ -248,16 +249,13 @@ void fill_system_device(SysDev &d, int nstages, const KeptModes &km, hipStream_t
     SystemTables &t = *d.t;
     const size_t n = size_t(d.nsys) * t.total;
     d.field.alloc(n);
-    d.aux.alloc(n);
     HIP_OK(hipMemset(d.field.p, 0, n * sizeof(double2)));
-    HIP_OK(hipMemset(d.aux.p, 0, n * sizeof(double2)));
+    d.R.alloc(n);
     for (int i = 0; i < nstages; ++i) {
         d.Khat.emplace_back(new DevBuf<double2>());
         d.X.emplace_back(new DevBuf<double2>());
-        d.R.emplace_back(new DevBuf<double2>());
         d.Khat.back()->alloc(n);
         d.X.back()->alloc(n);
-        d.R.back()->alloc(n);
     }
     auto up = [](SysDev::OpBufs &b, const OpTables &o) {
         b.ngroups = o.ngroups;
@@ -731,7 +729,7 @@ void one_step(coral_amd_ctx &c) {
             }
             for (int j = 0; j <= i; ++j)
                 if (Rk[i][j] != 0.0) ks.push_back({d.Khat[j]->p, Rk[i][j]});
-            double2 *rhs_i = d.R[i]->p;
+            double2 *rhs_i = d.R.p;
             rhs(c, d, ms, xs, ks, rhs_i);
             const SystemTables &t = *d.t;
             double2 *out = d.X[i]->p;

@@ -505,3 +505,26 @@ def test_products_seam_matches_oracle(solver_mod, case):
     for i in range(q.shape[0]):
         assert rel(q[i][:, wn], qref[i][:, wn]) <= TOL
         assert not np.any(q[i][:, ~wn])
+
+
+def test_config1_200_steps_graph_replays(solver_mod):
+    """The 64^2 x 33 Ra = 1e5 case over 200 steps of dt = 1e-4 (a fifth of the
+    1000-step run), the GPU side as HIP graph replays in blocks of 50: the
+    spectral state stays within 1e-10 (relative L2) of the oracle's."""
+    r = product_recipe("coral.equations.rbc_ra1e5")
+    NX, NZ, dt = 64, 33, 1e-4
+    k0, z0 = ic.rbc_initial_state(r, NX, NX, NZ, 1e-3, 8, 8, 1)
+    geo = O.Geometry(NX, NX, NZ, 2 * np.pi, 2 * np.pi)
+    ora = O.OracleSolver(r, geo, "ARS_222")
+    ora.field, ora.zfield = [a.copy() for a in k0], [a.copy() for a in z0]
+    with solver_mod.Solver(r, NX, NX, NZ, 2 * np.pi, 2 * np.pi) as g:
+        g.use_graphs(True)
+        g.set_state(k0, z0)
+        for block in range(4):
+            g.march_forward(dt, 50)
+            ora.run(dt, 50)
+            k, z = g.get_state()
+            e = rel(k[0], ora.field[0])
+            print("steps", 50 * (block + 1), "rel L2", e, flush=True)
+            assert e <= TOL
+            assert all(rel(a, b) <= TOL for a, b in zip(z, ora.zfield))

@@ -1624,6 +1624,9 @@ __device__ inline int w_col(unsigned w) { return int((w >> 15) & 0x7FFFu); }
 // since.
 template <typename T, int NQ, int PFB>
 struct Solve16 {
+    // step j's U entries sit in ring slot (j mod 16) mod PFB and are reloaded
+    // for step j - PFB, whose slot must be the same
+    static_assert(PFB > 0 && 16 % PFB == 0, "the backward prefetch distance must divide 16");
     // forward step t of a block: swap, eliminate, prefetch the next block's
     // coefficient.  lo is the lane's position in its row, opaque to the
     // compiler so that the lane masks are formed where they are used.

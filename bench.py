@@ -38,6 +38,11 @@ WORKLOADS = {
                     name="rotating RBC Ek=1e-4 Ra=5e7 Pr=1, 256x256x129 (384x384x193 padded), ARS_222, dt=2e-5"),
     "rbc64": dict(eq="coral.equations.rbc_ra1e5", NX=64, NZ=33, dt=1e-4, amp=1e-3, kmax=8, nmodes=8, seed=1,
                   steps=1000, name="RBC Ra=1e5 Pr=1, 64x64x33 (96x96x49 padded), ARS_222, dt=1e-4"),
+    # the FP64 path at 512x512x257 (768x768x385 padded): rotating RBC, Ek=3e-5,
+    # Ra=2e8, Pr=1, dt=5e-6
+    "rrbc512": dict(eq="coral.equations.rrbc_ra2e8_ek3e-5", NX=512, NZ=257, dt=5e-6, amp=1e-30, kmax=64,
+                    nmodes=8, seed=4, steps=20,
+                    name="rotating RBC Ek=3e-5 Ra=2e8 Pr=1, 512x512x257 (768x768x385 padded), ARS_222, dt=5e-6"),
     # the case of Coral's own published timing: rotating RBC, Pr=1, Ra=2.81e5,
     # 1/Ek=707.1, 96x96x48 modes on a 2x2x1 box, 0.27 s per ARS_222 step on 20
     # Skylake cores (etc/benchmarks/02_RRBC_julien_JFM96/benchmark_02_RRBC_julien96.ipynb:78,

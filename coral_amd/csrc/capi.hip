@@ -936,7 +936,11 @@ int coral_amd_create(const coral_amd_problem *prob, int device, coral_amd_ctx **
             HIP_OK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device));
             c->zdev = ZFragsDev{zf.TE, zf.TO, zf.NEK, zf.NOK, zf.RE, zf.RO, zf.KEK, zf.KOK,
                                 zf.bwd_ok && !std::getenv("CORAL_AMD_NO_MFMA"), zf.fwd_ok && !std::getenv("CORAL_AMD_NO_MFMA"),
-                                std::max(ncu, 1), c->zfrag_b.p, c->zfrag_f.p};
+                                std::max(ncu, 1), c->zfrag_b.p, c->zfrag_f.p, zf.nbp, zf.nfp, {}, {}};
+            for (int p = 0; p < ZMAXPARTS; ++p) {
+                c->zdev.bp[p] = zf.bp[p];
+                c->zdev.fp[p] = zf.fp[p];
+            }
         }
         c->zweight.upload(P.zweight);
         std::vector<int> qa = P.quad_a, qb = P.quad_b;

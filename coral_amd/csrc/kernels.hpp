@@ -52,10 +52,19 @@ void launch_zbwd(hipStream_t st, const double *Tz, const double2 *spec, double2 
 void launch_zfwd(hipStream_t st, const double *Cz, const double2 *Hy, double2 *Qhat, int NZ, int NZAA, int nsys,
                  int nbatch);
 // MFMA fragments of the folded (even/odd) transform matrices
+// A launch of a z transform over a part of the output row tiles: even tiles
+// [e0, e0 + ne), odd tiles [o0, o0 + no).  All tiles go in one part when their
+// fragments fit one workgroup's LDS, else in parts that do.
+struct ZPart {
+    int e0, ne, o0, no;
+};
+constexpr int ZMAXPARTS = 8;
 struct ZFrags {
     int TE = 0, TO = 0, NEK = 0, NOK = 0;  // synthesis: row tiles (even, odd), k steps (even, odd)
     int RE = 0, RO = 0, KEK = 0, KOK = 0;  // analysis
-    bool bwd_ok = false, fwd_ok = false;    // fragments fit the LDS of one workgroup
+    bool bwd_ok = false, fwd_ok = false;    // an MFMA plan exists (one part or several)
+    int nbp = 0, nfp = 0;                   // parts of the synthesis / analysis
+    ZPart bp[ZMAXPARTS], fp[ZMAXPARTS];
     std::vector<double> bwd, fwd;
 };
 struct ZFragsDev {
@@ -63,6 +72,8 @@ struct ZFragsDev {
     bool bwd_ok, fwd_ok;
     int ncu;
     const double *bwd, *fwd;
+    int nbp, nfp;
+    ZPart bp[ZMAXPARTS], fp[ZMAXPARTS];
 };
 bool make_zfrags(int NZ, int NZAA, const std::vector<double> &Tz, const std::vector<double> &Cz, int device,
                  ZFrags &zf);

@@ -30,13 +30,17 @@ constexpr int ZPF_BWD = 4, ZPF_FWD = 8, ZTHREADS = 512;
 __device__ inline d4 mfma(double a, double b, d4 c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
 
 // D(i, j) of a 16x16 tile sits in lane j + 16*(i%4), element i/4.
+// fragE / fragO: the fragments of the TE even and TO odd row tiles this launch
+// computes, output row tiles r0 .. r0 + TE - 1 (a part of all tiles when they
+// do not fit one workgroup's LDS together)
 template <int TE, int TO>
-__global__ __launch_bounds__(ZTHREADS) void k_zbwd_mfma(const double *__restrict__ frag, int NEK, int NOK,
+__global__ __launch_bounds__(ZTHREADS) void k_zbwd_mfma(const double *__restrict__ frag,
+                                                    const double *__restrict__ fragO, int NEK, int NOK,
                                                     const double *__restrict__ X, double *__restrict__ F, int NZ,
-                                                    int NZAA, int ncol, int nb) {
+                                                    int NZAA, int ncol, int nb, int r0) {
     extern __shared__ double lds[];
-    const int nfrag = (TE * NEK + TO * NOK) * 64;
-    for (int e = threadIdx.x; e < nfrag; e += blockDim.x) lds[e] = frag[e];
+    const int nfe = TE * NEK * 64, nfrag = nfe + TO * NOK * 64;
+    for (int e = threadIdx.x; e < nfrag; e += blockDim.x) lds[e] = e < nfe ? frag[e] : fragO[e - nfe];
     __syncthreads();
     const double *GE = lds, *GO = lds + TE * NEK * 64;
     const int lane = threadIdx.x & 63, nw = blockDim.x >> 6;
@@ -94,7 +98,7 @@ __global__ __launch_bounds__(ZTHREADS) void k_zbwd_mfma(const double *__restrict
             for (int r = 0; r < TE; ++r)
 #pragma unroll
                 for (int it = 0; it < 4; ++it) {
-                    const int z = 16 * r + 4 * it + kq;
+                    const int z = 16 * (r0 + r) + 4 * it + kq;
                     const double E = ae[r][it];
                     const double O = (r < TO) ? ao[r < TO ? r : 0][it] : 0.0;
                     if (z < Ke) Fb[(size_t)z * ncol] = E + O;
@@ -104,13 +108,16 @@ __global__ __launch_bounds__(ZTHREADS) void k_zbwd_mfma(const double *__restrict
     }
 }
 
+// fragE / fragO: the fragments of the RE even and RO odd output row tiles this
+// launch computes, starting at tiles re0 / ro0
 template <int RE, int RO>
-__global__ __launch_bounds__(ZTHREADS) void k_zfwd_mfma(const double *__restrict__ frag, int KEK, int KOK,
+__global__ __launch_bounds__(ZTHREADS) void k_zfwd_mfma(const double *__restrict__ frag,
+                                                    const double *__restrict__ fragO, int KEK, int KOK,
                                                     const double *__restrict__ Y, double *__restrict__ Q, int NZ,
-                                                    int NZAA, int ncol, int nb) {
+                                                    int NZAA, int ncol, int nb, int re0, int ro0) {
     extern __shared__ double lds[];
-    const int nfrag = (RE * KEK + RO * KOK) * 64;
-    for (int e = threadIdx.x; e < nfrag; e += blockDim.x) lds[e] = frag[e];
+    const int nfe = RE * KEK * 64, nfrag = nfe + RO * KOK * 64;
+    for (int e = threadIdx.x; e < nfrag; e += blockDim.x) lds[e] = e < nfe ? frag[e] : fragO[e - nfe];
     __syncthreads();
     const double *FE = lds, *FO = lds + RE * KEK * 64;
     const int lane = threadIdx.x & 63, nw = blockDim.x >> 6;
@@ -168,14 +175,14 @@ __global__ __launch_bounds__(ZTHREADS) void k_zfwd_mfma(const double *__restrict
             for (int r = 0; r < RE; ++r)
 #pragma unroll
                 for (int it = 0; it < 4; ++it) {
-                    const int n = 2 * (16 * r + 4 * it + kq);
+                    const int n = 2 * (16 * (re0 + r) + 4 * it + kq);
                     if (n < NZ) Qb[(size_t)n * ncol] = ae[r][it];
                 }
 #pragma unroll
             for (int r = 0; r < RO; ++r)
 #pragma unroll
                 for (int it = 0; it < 4; ++it) {
-                    const int n = 2 * (16 * r + 4 * it + kq) + 1;
+                    const int n = 2 * (16 * (ro0 + r) + 4 * it + kq) + 1;
                     if (n < NZ) Qb[(size_t)n * ncol] = ao[r][it];
                 }
         }
@@ -200,6 +207,23 @@ static void fill_frags(std::vector<double> &out, int ntile, int nk, const std::f
             }
 }
 
+// the (even, odd) row-tile counts a launch can be compiled for
+#define ZBWD_SHAPES(X) X(1, 1) X(2, 1) X(2, 2) X(3, 2) X(3, 3) X(4, 3) X(4, 4) X(5, 5) X(7, 6)
+#define ZFWD_SHAPES(X) X(1, 1) X(2, 1) X(2, 2) X(3, 2) X(3, 3) X(4, 3) X(5, 4)
+
+static bool zbwd_shape_ok(int ne, int no) {
+#define ZS(a, b) if (ne == a && no == b) return true;
+    ZBWD_SHAPES(ZS)
+#undef ZS
+    return false;
+}
+static bool zfwd_shape_ok(int ne, int no) {
+#define ZS(a, b) if (ne == a && no == b) return true;
+    ZFWD_SHAPES(ZS)
+#undef ZS
+    return false;
+}
+
 bool make_zfrags(int NZ, int NZAA, const std::vector<double> &Tz, const std::vector<double> &Cz, int device,
                  ZFrags &zf) {
     const int Ke = (NZAA + 1) / 2, Ko = NZAA / 2, NE = (NZ + 1) / 2, NO = NZ / 2;
@@ -219,29 +243,60 @@ bool make_zfrags(int NZ, int NZAA, const std::vector<double> &Tz, const std::vec
     fill_frags(zf.fwd, zf.RE, zf.KEK, [&](int m, int z) { return Cz[(size_t)(2 * m) * NZAA + z]; }, NE, Ke);
     fill_frags(zf.fwd, zf.RO, zf.KOK, [&](int m, int z) { return Cz[(size_t)(2 * m + 1) * NZAA + z]; }, NO, Ko);
     const size_t lim = (size_t)lds_limit(device);
-    zf.bwd_ok = zf.bwd.size() * sizeof(double) <= lim;
-    zf.fwd_ok = zf.fwd.size() * sizeof(double) <= lim;
+    // synthesis: part p takes row tiles [r0, r1) of both halves (an even and
+    // an odd tile of the same r feed the same mirrored output rows)
+    for (int P = 1; P <= ZMAXPARTS && !zf.bwd_ok; ++P) {
+        bool ok = true;
+        for (int p = 0; p < P && ok; ++p) {
+            const int r0 = p * zf.TE / P, r1 = (p + 1) * zf.TE / P;
+            const int ne = r1 - r0, no = std::max(0, std::min(r1, zf.TO) - r0);
+            ok = ne > 0 && no > 0 && zbwd_shape_ok(ne, no) &&
+                 size_t(ne * zf.NEK + no * zf.NOK) * 64 * sizeof(double) <= lim;
+            zf.bp[p] = ZPart{r0, ne, r0, no};
+        }
+        if (ok) {
+            zf.bwd_ok = true;
+            zf.nbp = P;
+        }
+    }
+    // analysis: even and odd output rows are independent; part p takes its
+    // share of each
+    for (int P = 1; P <= ZMAXPARTS && !zf.fwd_ok; ++P) {
+        bool ok = true;
+        for (int p = 0; p < P && ok; ++p) {
+            const int e0 = p * zf.RE / P, e1 = (p + 1) * zf.RE / P, o0 = p * zf.RO / P, o1 = (p + 1) * zf.RO / P;
+            ok = e1 > e0 && o1 > o0 && zfwd_shape_ok(e1 - e0, o1 - o0) &&
+                 size_t((e1 - e0) * zf.KEK + (o1 - o0) * zf.KOK) * 64 * sizeof(double) <= lim;
+            zf.fp[p] = ZPart{e0, e1 - e0, o0, o1 - o0};
+        }
+        if (ok) {
+            zf.fwd_ok = true;
+            zf.nfp = P;
+        }
+    }
     return zf.bwd_ok || zf.fwd_ok;
 }
 
 template <int TE, int TO>
-static void go_bwd(hipStream_t st, const ZFragsDev &z, const double *X, double *F, int NZ, int NZAA, int ncol,
-                   int nb, int nblocks) {
+static void go_bwd(hipStream_t st, const ZFragsDev &z, const ZPart &pt, const double *X, double *F, int NZ, int NZAA,
+                   int ncol, int nb, int nblocks) {
     const size_t shm = size_t(TE * z.NEK + TO * z.NOK) * 64 * sizeof(double);
     (void)hipFuncSetAttribute((const void *)k_zbwd_mfma<TE, TO>, hipFuncAttributeMaxDynamicSharedMemorySize,
                               (int)shm);
-    hipLaunchKernelGGL((k_zbwd_mfma<TE, TO>), dim3(nblocks), dim3(ZTHREADS), shm, st, z.bwd, z.NEK, z.NOK, X, F, NZ, NZAA,
-                       ncol, nb);
+    const double *fe = z.bwd + size_t(pt.e0) * z.NEK * 64, *fo = z.bwd + size_t(z.TE * z.NEK + pt.o0 * z.NOK) * 64;
+    hipLaunchKernelGGL((k_zbwd_mfma<TE, TO>), dim3(nblocks), dim3(ZTHREADS), shm, st, fe, fo, z.NEK, z.NOK, X, F, NZ,
+                       NZAA, ncol, nb, pt.e0);
 }
 
 template <int RE, int RO>
-static void go_fwd(hipStream_t st, const ZFragsDev &z, const double *Y, double *Q, int NZ, int NZAA, int ncol,
-                   int nb, int nblocks) {
+static void go_fwd(hipStream_t st, const ZFragsDev &z, const ZPart &pt, const double *Y, double *Q, int NZ, int NZAA,
+                   int ncol, int nb, int nblocks) {
     const size_t shm = size_t(RE * z.KEK + RO * z.KOK) * 64 * sizeof(double);
     (void)hipFuncSetAttribute((const void *)k_zfwd_mfma<RE, RO>, hipFuncAttributeMaxDynamicSharedMemorySize,
                               (int)shm);
-    hipLaunchKernelGGL((k_zfwd_mfma<RE, RO>), dim3(nblocks), dim3(ZTHREADS), shm, st, z.fwd, z.KEK, z.KOK, Y, Q, NZ, NZAA,
-                       ncol, nb);
+    const double *fe = z.fwd + size_t(pt.e0) * z.KEK * 64, *fo = z.fwd + size_t(z.RE * z.KEK + pt.o0 * z.KOK) * 64;
+    hipLaunchKernelGGL((k_zfwd_mfma<RE, RO>), dim3(nblocks), dim3(ZTHREADS), shm, st, fe, fo, z.KEK, z.KOK, Y, Q, NZ,
+                       NZAA, ncol, nb, pt.e0, pt.o0);
 }
 
 static int grid_for(int ntiles, int ncu) {
@@ -255,17 +310,13 @@ bool launch_zbwd_mfma(hipStream_t st, const ZFragsDev &z, const double2 *spec, d
     const int ncol = 2 * nsys, ntiles = nb * ((ncol + 15) / 16), g = grid_for(ntiles, z.ncu);
     const double *X = (const double *)spec;
     double *F = (double *)Fz;
-    const int key = z.TE * 100 + z.TO;
-    switch (key) {
-    case 101: go_bwd<1, 1>(st, z, X, F, NZ, NZAA, ncol, nb, g); break;
-    case 202: go_bwd<2, 2>(st, z, X, F, NZ, NZAA, ncol, nb, g); break;
-    case 201: go_bwd<2, 1>(st, z, X, F, NZ, NZAA, ncol, nb, g); break;
-    case 303: go_bwd<3, 3>(st, z, X, F, NZ, NZAA, ncol, nb, g); break;
-    case 403: go_bwd<4, 3>(st, z, X, F, NZ, NZAA, ncol, nb, g); break;
-    case 404: go_bwd<4, 4>(st, z, X, F, NZ, NZAA, ncol, nb, g); break;
-    case 505: go_bwd<5, 5>(st, z, X, F, NZ, NZAA, ncol, nb, g); break;
-    case 706: go_bwd<7, 6>(st, z, X, F, NZ, NZAA, ncol, nb, g); break;
-    default: return false;
+    for (int p = 0; p < z.nbp; ++p) {
+        const ZPart &pt = z.bp[p];
+#define ZS(a, b) \
+        if (pt.ne == a && pt.no == b) { go_bwd<a, b>(st, z, pt, X, F, NZ, NZAA, ncol, nb, g); continue; }
+        ZBWD_SHAPES(ZS)
+#undef ZS
+        return false;   // not reached: the plan holds compiled shapes only
     }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) fprintf(stderr, "coral_amd: launch failed: %s\n", hipGetErrorString(e));
@@ -278,15 +329,13 @@ bool launch_zfwd_mfma(hipStream_t st, const ZFragsDev &z, const double2 *Hy, dou
     const int ncol = 2 * nsys, ntiles = nb * ((ncol + 15) / 16), g = grid_for(ntiles, z.ncu);
     const double *Y = (const double *)Hy;
     double *Q = (double *)Qhat;
-    const int key = z.RE * 100 + z.RO;
-    switch (key) {
-    case 101: go_fwd<1, 1>(st, z, Y, Q, NZ, NZAA, ncol, nb, g); break;
-    case 201: go_fwd<2, 1>(st, z, Y, Q, NZ, NZAA, ncol, nb, g); break;
-    case 202: go_fwd<2, 2>(st, z, Y, Q, NZ, NZAA, ncol, nb, g); break;
-    case 302: go_fwd<3, 2>(st, z, Y, Q, NZ, NZAA, ncol, nb, g); break;
-    case 403: go_fwd<4, 3>(st, z, Y, Q, NZ, NZAA, ncol, nb, g); break;
-    case 504: go_fwd<5, 4>(st, z, Y, Q, NZ, NZAA, ncol, nb, g); break;
-    default: return false;
+    for (int p = 0; p < z.nfp; ++p) {
+        const ZPart &pt = z.fp[p];
+#define ZS(a, b) \
+        if (pt.ne == a && pt.no == b) { go_fwd<a, b>(st, z, pt, Y, Q, NZ, NZAA, ncol, nb, g); continue; }
+        ZFWD_SHAPES(ZS)
+#undef ZS
+        return false;   // not reached: the plan holds compiled shapes only
     }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) fprintf(stderr, "coral_amd: launch failed: %s\n", hipGetErrorString(e));

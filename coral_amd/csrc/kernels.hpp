@@ -105,7 +105,7 @@ void launch_xpass(hipStream_t st, const FFTPlan &px, const XPassArgs &a);
 // wavefront-per-line x pass for N in {96, 144, 192, 384, 768} and up to 6 full
 // variables; returns false (nothing launched) for other shapes
 bool launch_xpass_wave(hipStream_t st, const FFTPlan &px, const XPassArgs &a);
-// wavefront-FFT y passes for NYAA in {96, 144, 192, 384}; false (nothing launched) otherwise
+// wavefront-FFT y passes for NYAA in {96, 144, 192, 384, 768}; false (nothing launched) otherwise
 bool launch_ypass_wave(hipStream_t st, bool fwd, const FFTPlan &py, const double2 *in, double2 *out, int nbatch,
                        int NZAA, int nkx, int nky, int NY);
 // y forward: Hy[b][z][ix*nky+jy] = sum_y H[b][z][y][ix] e^{-i ky y}

@@ -51,7 +51,7 @@ template <int N>
 __global__ __launch_bounds__(64 * YWAVES) void k_ybwd_w(const double2 *__restrict__ Fz, double2 *__restrict__ G,
                                                        const double2 *__restrict__ tw, int nkx, int nky, int NY) {
     constexpr int LS = N + 1, LPW = YL / YWAVES;
-    __shared__ double2 buf[YL * LS];
+    extern __shared__ double2 buf[];   // YL * LS (beyond 64 KiB for N = 768)
     const int ntile = (nkx + YL - 1) / YL;
     const int tile = blockIdx.x % ntile;
     const size_t bz = blockIdx.x / ntile;
@@ -113,7 +113,7 @@ template <int N>
 __global__ __launch_bounds__(64 * YWAVES) void k_yfwd_w(const double2 *__restrict__ H, double2 *__restrict__ Hy,
                                                        const double2 *__restrict__ tw, int nkx, int nky, int NY) {
     constexpr int LS = N + 1, LPW = YL / YWAVES;
-    __shared__ double2 buf[YL * LS];
+    extern __shared__ double2 buf[];   // YL * LS (beyond 64 KiB for N = 768)
     const int ntile = (nkx + YL - 1) / YL;
     const int tile = blockIdx.x % ntile;
     const size_t bz = blockIdx.x / ntile;
@@ -166,10 +166,16 @@ static void go(hipStream_t st, bool fwd, const FFTPlan &py, const double2 *in, d
                int nkx, int nky, int NY) {
     const int ntile = (nkx + YL - 1) / YL;
     const dim3 grid(ntile * nbatch * NZAA);
-    if (fwd)
-        hipLaunchKernelGGL(k_yfwd_w<N>, grid, dim3(64 * YWAVES), 0, st, in, out, py.tw, nkx, nky, NY);
-    else
-        hipLaunchKernelGGL(k_ybwd_w<N>, grid, dim3(64 * YWAVES), 0, st, in, out, py.tw, nkx, nky, NY);
+    const size_t shm = size_t(YL) * (N + 1) * sizeof(double2);
+    if (fwd) {
+        if (shm > 65536)
+            (void)hipFuncSetAttribute((const void *)k_yfwd_w<N>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
+        hipLaunchKernelGGL(k_yfwd_w<N>, grid, dim3(64 * YWAVES), shm, st, in, out, py.tw, nkx, nky, NY);
+    } else {
+        if (shm > 65536)
+            (void)hipFuncSetAttribute((const void *)k_ybwd_w<N>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
+        hipLaunchKernelGGL(k_ybwd_w<N>, grid, dim3(64 * YWAVES), shm, st, in, out, py.tw, nkx, nky, NY);
+    }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) fprintf(stderr, "coral_amd: launch failed: %s\n", hipGetErrorString(e));
 }
@@ -182,7 +188,8 @@ bool launch_ypass_wave(hipStream_t st, bool fwd, const FFTPlan &py, const double
     case 144: go<144>(st, fwd, py, in, out, nbatch, NZAA, nkx, nky, NY); return true;
     case 192: go<192>(st, fwd, py, in, out, nbatch, NZAA, nkx, nky, NY); return true;
     case 384: go<384>(st, fwd, py, in, out, nbatch, NZAA, nkx, nky, NY); return true;
-    default: return false;  // 768: the 8-line tile exceeds 64 KiB of LDS
+    case 768: go<768>(st, fwd, py, in, out, nbatch, NZAA, nkx, nky, NY); return true;   // 98 KB of LDS
+    default: return false;
     }
 }
 

@@ -210,7 +210,7 @@ __global__ void k_zero_u64(unsigned long long *p, int n) {
 }
 void zero_cflmax(hipStream_t st, unsigned long long *p) {
     hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, st, p, 3);
-    HIP_OK(hipGetLastError());
+    check_launch("zero_cflmax");
 }
 
 FFTPlan make_plan(int N, DevBuf<double2> &tw) {
@@ -574,11 +574,14 @@ void join_mean(coral_amd_ctx &c) {
 }
 
 // x^{n+1} = x_s: the last stage's buffer becomes the state, the state's
-// buffer the next step's last stage
+// buffer the next step's last stage.  Every swap, eager or replayed, flips
+// gpar, so the graph that matches the host pointers is known after any mix of
+// eager steps (timers on) and replays.
 void swap_step_buffers(coral_amd_ctx &c) {
     const int s = c.P.nstages;
     for (auto &d : c.ks) std::swap(d->field.p, d->X[s - 1]->p);
     for (auto &d : c.zs) std::swap(d->field.p, d->X[s - 1]->p);
+    c.gpar ^= 1;
 }
 
 // f's launches (and timer events) go to the side stream
@@ -681,6 +684,43 @@ void nl_terms_mean(coral_amd_ctx &c, int stage) {
     for (auto &d : c.zs) nl_assemble(c, *d, d->Khat[stage]->p);
 }
 
+// (M - dt a11 L) out = rhs for every wavenumber of a system, with the factors
+// of the last set_dt (backsolve_to_aux, PL_IMEX_timestepping_marching.f90:57-58)
+void solve_system(coral_amd_ctx &c, SysDev &d, const double2 *rhs, double2 *out) {
+    const SystemTables &t = *d.t;
+    c.timed(d.kxky ? "solve" : "solve_mean", [&] {
+        const int kv = t.nl + t.nu, id0 = d.kxky ? 1 : 0;
+        bool done = true;
+        if (d.ncol16 > 0) {
+            Solve16Args sa{};
+            sa.L16 = d.real ? (const void *)d.Lr.p : (const void *)d.Lf.p;
+            sa.U = d.real ? (const void *)d.Urt.p : (const void *)d.Uft.p;
+            sa.ucol = d.ucol.p;
+            sa.ustride = d.ustride;
+            sa.ncol16 = d.ncol16;
+            sa.piv = d.piv.p;
+            sa.words = d.words.p;
+            sa.rotate = d.real ? 1 : 0;
+            sa.ident0 = id0;
+            sa.total = t.total;
+            sa.n = d.blk_n;
+            sa.kl = d.blk_nl;
+            sa.kv = d.blk_nl + d.blk_nu;
+            sa.nsys = d.nsys;
+            sa.nblk = d.nblk;
+            sa.rhs = rhs;
+            sa.x = out;
+            sa.sink = c.sink.p;
+            done = d.real ? launch_solve16_real(c.stream, sa) : launch_solve16_complex(c.stream, sa);
+        } else if (d.real)
+            launch_solve_real(c.stream, d.Lr.p, d.Ur.p + SOLVE16_PAD, d.piv.p, d.rot.p, id0, t.total, t.nl, kv,
+                              d.nsys, rhs, out);
+        else
+            launch_solve(c.stream, d.Lf.p, d.Uf.p + SOLVE16_PAD, d.piv.p, t.total, t.nl, kv, d.nsys, rhs, out);
+        if (!done) throw std::runtime_error("16-lane solve rejected a band it was factored for");
+    });
+}
+
 // one_step_beyond (PL_IMEX_timestepping_marching.f90:3-48): stage i takes the
 // NL terms of the previous stage solution, rhs_i = M x^n + sum_{j<i} dt a_ij L x_j
 // + sum_{j<=i} dt ah_{i+1,j} K_hat_j, and solves (M - dt a_ii L) x_i = rhs_i;
@@ -731,40 +771,7 @@ void one_step(coral_amd_ctx &c) {
                 if (Rk[i][j] != 0.0) ks.push_back({d.Khat[j]->p, Rk[i][j]});
             double2 *rhs_i = d.R.p;
             rhs(c, d, ms, xs, ks, rhs_i);
-            const SystemTables &t = *d.t;
-            double2 *out = d.X[i]->p;
-            c.timed(d.kxky ? "solve" : "solve_mean", [&] {
-                const int kv = t.nl + t.nu, id0 = d.kxky ? 1 : 0;
-                bool done = true;
-                if (d.ncol16 > 0) {
-                    Solve16Args sa{};
-                    sa.L16 = d.real ? (const void *)d.Lr.p : (const void *)d.Lf.p;
-                    sa.U = d.real ? (const void *)d.Urt.p : (const void *)d.Uft.p;
-                    sa.ucol = d.ucol.p;
-                    sa.ustride = d.ustride;
-                    sa.ncol16 = d.ncol16;
-                    sa.piv = d.piv.p;
-                    sa.words = d.words.p;
-                    sa.rotate = d.real ? 1 : 0;
-                    sa.ident0 = id0;
-                    sa.total = t.total;
-                    sa.n = d.blk_n;
-                    sa.kl = d.blk_nl;
-                    sa.kv = d.blk_nl + d.blk_nu;
-                    sa.nsys = d.nsys;
-                    sa.nblk = d.nblk;
-                    sa.rhs = rhs_i;
-                    sa.x = out;
-                    sa.sink = c.sink.p;
-                    done = d.real ? launch_solve16_real(c.stream, sa) : launch_solve16_complex(c.stream, sa);
-                } else if (d.real)
-                    launch_solve_real(c.stream, d.Lr.p, d.Ur.p + SOLVE16_PAD, d.piv.p, d.rot.p, id0, t.total, t.nl,
-                                      kv, d.nsys, rhs_i, out);
-                else
-                    launch_solve(c.stream, d.Lf.p, d.Uf.p + SOLVE16_PAD, d.piv.p, t.total, t.nl, kv, d.nsys, rhs_i,
-                                 out);
-                if (!done) throw std::runtime_error("16-lane solve rejected a band it was factored for");
-            });
+            solve_system(c, d, rhs_i, d.X[i]->p);
         };
         if (!c.zs.empty()) {
             on_side(c, [&] {
@@ -847,6 +854,7 @@ __global__ void k_fill_zero(double2 *p, size_t n) {
 void fill_zero(hipStream_t st, double2 *p, size_t n) {
     if (!n) return;
     hipLaunchKernelGGL(k_fill_zero, dim3(unsigned((n + 255) / 256)), dim3(256), 0, st, p, n);
+    check_launch("fill_zero");
 }
 
 }  // namespace
@@ -1078,7 +1086,6 @@ int coral_amd_step(coral_amd_ctx *c, int32_t nsteps) {
             for (int n = 0; n < nsteps; ++n) {
                 HIP_OK(hipGraphLaunch(c->gexec[c->gpar], c->stream));
                 swap_step_buffers(*c);
-                c->gpar ^= 1;
             }
         } else {
             for (int n = 0; n < nsteps; ++n) one_step(*c);
@@ -1402,6 +1409,7 @@ int coral_amd_horizontal_averages(coral_amd_ctx *c, double *full_avg, double *qu
             const int n = P.nquad * P.NZAA;
             hipLaunchKernelGGL(k_gather_strided, dim3((n + 255) / 256), dim3(256), 0, c->stream, c->Hy.p,
                                size_t(P.nsys), n, col.p + size_t(nf) * P.NZ);
+            check_launch("horizontal_averages");
         } else {
             backward_to_G(*c, kst, zst);
         }
@@ -1410,6 +1418,7 @@ int coral_amd_horizontal_averages(coral_amd_ctx *c, double *full_avg, double *qu
         if (m > 0)
             hipLaunchKernelGGL(k_gather_strided, dim3((m + 255) / 256), dim3(256), 0, c->stream, c->spec.p,
                                size_t(P.nsys), m, col.p);
+        if (m > 0) check_launch("horizontal_averages");
         HIP_OK(hipStreamSynchronize(c->stream));
         std::vector<double2> h(col.n);
         HIP_OK(hipMemcpy(h.data(), col.p, h.size() * sizeof(double2), hipMemcpyDeviceToHost));
@@ -1530,32 +1539,132 @@ int coral_amd_fwd_transform(coral_amd_ctx *c, const double *phys, double *spec) 
     }
 }
 
+namespace {
+// The stream of the banded-solve seam, destroyed on every path out.
+struct OwnedStream {
+    hipStream_t s = nullptr;
+    ~OwnedStream() {
+        if (s) (void)hipStreamDestroy(s);
+    }
+};
+}  // namespace
+
 int coral_amd_banded_solve(int32_t device, int32_t n, int32_t kl, int32_t ku, int32_t nsys, const double *ab,
                            double *x, int32_t *info) {
     try {
         if (!ab || !x || !info) throw std::runtime_error("null argument");
         if (n < 1 || kl < 0 || ku < 0 || nsys < 1 || kl > 255) throw std::runtime_error("bad band dimensions");
         HIP_OK(hipSetDevice(device));
-        hipStream_t st;
-        HIP_OK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        OwnedStream os;
+        HIP_OK(hipStreamCreateWithFlags(&os.s, hipStreamNonBlocking));
+        hipStream_t st = os.s;
         const int ldab = 2 * kl + ku + 1, kv = kl + ku;
-        DevBuf<double2> work, Lf, Uf, xb;
-        DevBuf<unsigned char> piv;
+        DevBuf<double2> work, xb, rb;
         DevBuf<int> inf;
         work.alloc(size_t(nsys) * n * ldab);
-        Lf.alloc(size_t(nsys) * n * std::max(kl, 1));
-        Uf.alloc(size_t(nsys) * n * (kv + 1));
         xb.alloc(size_t(nsys) * n);
-        piv.alloc(size_t(nsys) * n);
         inf.alloc(nsys);
         HIP_OK(hipMemcpy(work.p, ab, work.n * sizeof(double2), hipMemcpyHostToDevice));
-        HIP_OK(hipMemcpy(xb.p, x, xb.n * sizeof(double2), hipMemcpyHostToDevice));
-        launch_factor_lapack(st, work.p, n, kl, ku, nsys, Lf.p, Uf.p, piv.p, inf.p);
-        launch_solve(st, Lf.p, Uf.p, piv.p, n, kl, kv, nsys, xb.p, xb.p);
-        HIP_OK(hipStreamSynchronize(st));
+        const int ncol16 = solve16_ncol(n, kl, kv);
+        if (ncol16 > 0) {
+            // the records, U trimming and 16-lane solve kernel of the time step
+            // (one block per system, complex, no rotation)
+            const size_t nmat4 = (size_t(nsys) + 3) / 4 * 4, nb = 16 * ((size_t(n) + 15) / 16);
+            DevBuf<double2> L16, Uf, Ut, sink;
+            DevBuf<unsigned char> piv;
+            DevBuf<int> ulen;
+            DevBuf<int2> ucol;
+            DevBuf<unsigned> words;
+            L16.alloc(nmat4 * ncol16 * solve16_klp(kl));
+            Uf.alloc(nmat4 * nb * (kv + 1) + 2 * size_t(SOLVE16_PAD));
+            piv.alloc(size_t(nsys) * nb);
+            HIP_OK(hipMemsetAsync(L16.p, 0, L16.n * sizeof(double2), st));
+            HIP_OK(hipMemsetAsync(Uf.p, 0, Uf.n * sizeof(double2), st));
+            HIP_OK(hipMemsetAsync(piv.p, 0, piv.n, st));
+            launch_factor_lapack16(st, work.p, n, kl, ku, nsys, ncol16, L16.p, Uf.p + SOLVE16_PAD, piv.p, inf.p);
+            ulen.alloc(nb);
+            std::vector<int2> uc;
+            long long ustride = 0;
+            solve16_ucol(st, Uf.p + SOLVE16_PAD, 16, n, kv, nsys, ulen.p, uc, &ustride);
+            Ut.alloc(size_t((nsys + 3) / 4) * size_t(ustride));
+            HIP_OK(hipMemsetAsync(Ut.p, 0, Ut.n * sizeof(double2), st));
+            ucol.upload(uc);
+            solve16_trim_u_copy(st, Uf.p + SOLVE16_PAD, 16, n, kv, nsys, ucol.p, ustride, Ut.p);
+            words.upload(solve16_words(n, 1, n, nullptr, nullptr, nullptr));
+            rb.alloc(size_t(nsys) * n);
+            sink.alloc(64);
+            HIP_OK(hipMemcpy(rb.p, x, rb.n * sizeof(double2), hipMemcpyHostToDevice));
+            Solve16Args sa{};
+            sa.L16 = L16.p;
+            sa.U = Ut.p;
+            sa.ucol = ucol.p;
+            sa.ustride = ustride;
+            sa.ncol16 = ncol16;
+            sa.piv = piv.p;
+            sa.words = words.p;
+            sa.rotate = 0;
+            sa.ident0 = 0;
+            sa.total = n;
+            sa.n = n;
+            sa.kl = kl;
+            sa.kv = kv;
+            sa.nsys = nsys;
+            sa.nblk = 1;
+            sa.rhs = rb.p;
+            sa.x = xb.p;
+            sa.sink = sink.p;
+            if (!launch_solve16_complex(st, sa)) throw std::runtime_error("16-lane solve rejected the band");
+            HIP_OK(hipStreamSynchronize(st));
+        } else {
+            DevBuf<double2> Lf, Uf;
+            DevBuf<unsigned char> piv;
+            Lf.alloc(size_t(nsys) * n * std::max(kl, 1));
+            Uf.alloc(size_t(nsys) * n * (kv + 1));
+            piv.alloc(size_t(nsys) * n);
+            HIP_OK(hipMemcpy(xb.p, x, xb.n * sizeof(double2), hipMemcpyHostToDevice));
+            launch_factor_lapack(st, work.p, n, kl, ku, nsys, Lf.p, Uf.p, piv.p, inf.p);
+            launch_solve(st, Lf.p, Uf.p, piv.p, n, kl, kv, nsys, xb.p, xb.p);
+            HIP_OK(hipStreamSynchronize(st));
+        }
         HIP_OK(hipMemcpy(x, xb.p, xb.n * sizeof(double2), hipMemcpyDeviceToHost));
         HIP_OK(hipMemcpy(info, inf.p, nsys * sizeof(int), hipMemcpyDeviceToHost));
-        HIP_OK(hipStreamDestroy(st));
+        return 0;
+    } catch (const std::exception &e) {
+        return fail(e);
+    }
+}
+
+int coral_amd_system_solve(coral_amd_ctx *c, int32_t isys, const double *rhs, double *x) {
+    try {
+        if (!c || !rhs || !x) throw std::runtime_error("null argument");
+        if (!c->factored) throw std::runtime_error("coral_amd_set_dt must be called before solving");
+        if (isys < 0 || isys >= int(c->ks.size())) throw std::runtime_error("bad kxky system index");
+        HIP_OK(hipSetDevice(c->device));
+        HIP_OK(hipStreamSynchronize(c->stream));
+        HIP_OK(hipStreamSynchronize(c->side));
+        const Problem &P = c->P;
+        SysDev &d = *c->ks[isys];
+        const int total = d.t->total;
+        std::vector<double2> h(size_t(P.nsys) * total);
+        for (int ix = 0; ix < P.nkx; ++ix)
+            for (int jy = 0; jy < P.nky; ++jy) {
+                const size_t s = size_t(ix) * P.nky + jy, src = (size_t(ix) * P.NYAA + P.ky_slot[jy]) * total;
+                for (int i = 0; i < total; ++i)
+                    h[s * total + i] = make_double2(rhs[2 * (src + i)], rhs[2 * (src + i) + 1]);
+            }
+        HIP_OK(hipMemcpy(d.R.p, h.data(), h.size() * sizeof(double2), hipMemcpyHostToDevice));
+        solve_system(*c, d, d.R.p, d.X[0]->p);
+        HIP_OK(hipStreamSynchronize(c->stream));
+        HIP_OK(hipMemcpy(h.data(), d.X[0]->p, h.size() * sizeof(double2), hipMemcpyDeviceToHost));
+        std::memset(x, 0, sizeof(double) * 2 * size_t(P.nkx) * P.NYAA * total);
+        for (int ix = 0; ix < P.nkx; ++ix)
+            for (int jy = 0; jy < P.nky; ++jy) {
+                const size_t s = size_t(ix) * P.nky + jy, dst = (size_t(ix) * P.NYAA + P.ky_slot[jy]) * total;
+                for (int i = 0; i < total; ++i) {
+                    x[2 * (dst + i)] = h[s * total + i].x;
+                    x[2 * (dst + i) + 1] = h[s * total + i].y;
+                }
+            }
         return 0;
     } catch (const std::exception &e) {
         return fail(e);

@@ -3,7 +3,37 @@
 
 #include <hip/hip_runtime.h>
 
+#include <stdexcept>
+#include <string>
+
 namespace coral {
+
+// A failed launch becomes an exception; the C ABI returns it as an error code
+// with the message.  (hipGetLastError clears the error, so it is read here,
+// once, right after the launch.)
+inline void check_launch(const char *where) {
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess)
+        throw std::runtime_error(std::string("kernel launch failed in ") + where + ": " + hipGetErrorString(e));
+}
+
+// Largest dynamic LDS a workgroup may request on the current device.
+inline size_t lds_limit_current() {
+    int dev = 0, v = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
+        v = 65536;
+    return size_t(v);
+}
+
+// Opts a kernel into `bytes` of dynamic LDS (needed above 64 KiB); throws when
+// the runtime refuses.
+inline void set_dynamic_lds(const void *fn, size_t bytes, const char *where) {
+    const hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, int(bytes));
+    if (e != hipSuccess)
+        throw std::runtime_error(std::string("dynamic LDS of ") + std::to_string(bytes) + " bytes refused in " + where +
+                                 ": " + hipGetErrorString(e));
+}
 
 __host__ __device__ inline double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
 __host__ __device__ inline double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }

@@ -15,13 +15,7 @@
 
 namespace coral {
 
-#define CHECK_LAUNCH()                                                                    \
-    do {                                                                                  \
-        hipError_t e_ = hipGetLastError();                                                \
-        if (e_ != hipSuccess) {                                                           \
-            fprintf(stderr, "coral_amd: launch failed: %s\n", hipGetErrorString(e_));     \
-        }                                                                                 \
-    } while (0)
+#define CHECK_LAUNCH() check_launch(__func__)
 
 static inline unsigned blocks_for(long long n, int threads) { return unsigned((n + threads - 1) / threads); }
 
@@ -205,7 +199,7 @@ void launch_extract_all(hipStream_t st, const ExtractAllArgs &a) {
     int xt = 8;
     while (xt > 1 && size_t(a.nfull) * a.NZ * xt * sizeof(double2) > size_t(lim) / 2) xt /= 2;
     const size_t shm = size_t(a.nfull) * a.NZ * xt * sizeof(double2);
-    (void)hipFuncSetAttribute((const void *)k_extract_all, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
+    if (shm > 65536) set_dynamic_lds((const void *)k_extract_all, shm, __func__);
     hipLaunchKernelGGL(k_extract_all, dim3(blocks_for(a.nsys, xt)), dim3(512), shm, st, a, xt);
     CHECK_LAUNCH();
 }
@@ -445,8 +439,7 @@ __global__ __launch_bounds__(256) void k_ybwd(FFTPlanDev p, const double2 *__res
 
 static bool fft_fits(int lines, int N) {
     if (lines * N <= 1536) return true;
-    fprintf(stderr, "coral_amd: FFT length %d exceeds the LDS FFT capacity\n", N);
-    return false;
+    throw std::runtime_error("FFT length " + std::to_string(N) + " exceeds the LDS FFT capacity");
 }
 
 void launch_ybwd(hipStream_t st, const FFTPlan &py, const double2 *Fz, double2 *G, int nbatch, int NZAA, int nkx,
@@ -584,8 +577,8 @@ void launch_xpass(hipStream_t st, const FFTPlan &px, const XPassArgs &a) {
     int LX = 4;
     while (LX > 1 && (LX * px.N > 1536 || xpass_lds(LX, px.N, a.nfull) > 65536)) LX /= 2;
     if (LX * px.N > 1536 || xpass_lds(LX, px.N, a.nfull) > 65536) {
-        fprintf(stderr, "coral_amd: x-pass does not fit (N=%d, nfull=%d)\n", px.N, a.nfull);
-        return;
+        throw std::runtime_error("x pass does not fit (N=" + std::to_string(px.N) + ", nfull=" +
+                                 std::to_string(a.nfull) + ")");
     }
     long long nlines = (long long)a.NZAA * a.NYAA;
     size_t shmem = xpass_lds(LX, px.N, a.nfull);
@@ -1328,6 +1321,25 @@ __global__ __launch_bounds__(64) void k_factor_lapack(double2 *work, int n, int 
 void launch_factor_lapack(hipStream_t st, double2 *work, int n, int kl, int ku, int nsys, double2 *Lf, double2 *Uf,
                           unsigned char *piv, int *info) {
     hipLaunchKernelGGL(k_factor_lapack, dim3(nsys), dim3(64), 0, st, work, n, kl, ku, Lf, Uf, piv, info);
+    CHECK_LAUNCH();
+}
+
+// LAPACK band input factored into the records of the 16-lane solve (the
+// layout k_factor writes for ncol16 > 0, one block per system)
+__global__ __launch_bounds__(64) void k_factor_lapack16(double2 *work, int n, int kl, int ku, int ncol16, double2 *L16,
+                                                        double2 *Ufull, unsigned char *piv, int *info) {
+    const int s = blockIdx.x, ldab = 2 * kl + ku + 1;
+    double2 *AB = work + (size_t)s * n * ldab;
+    if (threadIdx.x == 0) info[s] = 0;
+    __syncthreads();
+    gbtf2_wave<double2>(AB, n, kl, ku, ldab, piv + (size_t)s * 16 * ((n + 15) >> 4), info + s);
+    __syncthreads();
+    split_factors16<double2>(AB, n, kl, ku, ldab, ncol16, s, L16, Ufull);
+}
+
+void launch_factor_lapack16(hipStream_t st, double2 *work, int n, int kl, int ku, int nsys, int ncol16, double2 *L16,
+                            double2 *Ufull, unsigned char *piv, int *info) {
+    hipLaunchKernelGGL(k_factor_lapack16, dim3(nsys), dim3(64), 0, st, work, n, kl, ku, ncol16, L16, Ufull, piv, info);
     CHECK_LAUNCH();
 }
 

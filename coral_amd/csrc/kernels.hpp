@@ -225,6 +225,12 @@ void launch_factor(hipStream_t st, const FactorArgs &a);
 // generic batched factorization of LAPACK band input (for the unit seam)
 void launch_factor_lapack(hipStream_t st, double2 *work, int n, int kl, int ku, int nsys, double2 *Lf, double2 *Uf,
                           unsigned char *piv, int *info);
+// the same factorization into the records of the 16-lane solve (ncol16 =
+// solve16_ncol(n, kl, kl + ku) > 0): L16 [ceil(nsys/4)*4][ncol16][kl],
+// Ufull with SOLVE16_PAD elements before it, pivots [nsys][16*ceil(n/16)];
+// all three cleared by the caller
+void launch_factor_lapack16(hipStream_t st, double2 *work, int n, int kl, int ku, int nsys, int ncol16, double2 *L16,
+                            double2 *Ufull, unsigned char *piv, int *info);
 void launch_solve(hipStream_t st, const double2 *Lf, const double2 *Uf, const unsigned char *piv, int n, int kl,
                   int kv, int nsys, const double2 *rhs, double2 *x);
 // real-form solve: x = i^phi . (LU)^-1 . P . i^-psi rhs, with real factors

@@ -326,8 +326,7 @@ static bool launch_wave(hipStream_t st, const FFTPlan &px, const XPassArgs &a) {
     const long long per_block = (long long)wpb * X::LPW;
     const unsigned blocks = unsigned((nlines + per_block - 1) / per_block);
     hipLaunchKernelGGL((k_xpass_wave<N, NPI, XGP, NK>), dim3(blocks), dim3(64 * wpb), wpb * per_wave, st, a, px.tw, nlines);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) fprintf(stderr, "coral_amd: launch failed: %s\n", hipGetErrorString(e));
+    check_launch("x pass");
     return true;
 }
 

@@ -161,28 +161,31 @@ __global__ __launch_bounds__(64 * YWAVES) void k_yfwd_w(const double2 *__restric
     }
 }
 
+// LDS of a workgroup of the wavefront y pass at length N
+static size_t ywave_lds(int N) { return size_t(YL) * (N + 1) * sizeof(double2); }
+
 template <int N>
 static void go(hipStream_t st, bool fwd, const FFTPlan &py, const double2 *in, double2 *out, int nbatch, int NZAA,
                int nkx, int nky, int NY) {
     const int ntile = (nkx + YL - 1) / YL;
     const dim3 grid(ntile * nbatch * NZAA);
-    const size_t shm = size_t(YL) * (N + 1) * sizeof(double2);
+    const size_t shm = ywave_lds(N);
     if (fwd) {
-        if (shm > 65536)
-            (void)hipFuncSetAttribute((const void *)k_yfwd_w<N>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
+        if (shm > 65536) set_dynamic_lds((const void *)k_yfwd_w<N>, shm, "y pass (forward)");
         hipLaunchKernelGGL(k_yfwd_w<N>, grid, dim3(64 * YWAVES), shm, st, in, out, py.tw, nkx, nky, NY);
     } else {
-        if (shm > 65536)
-            (void)hipFuncSetAttribute((const void *)k_ybwd_w<N>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
+        if (shm > 65536) set_dynamic_lds((const void *)k_ybwd_w<N>, shm, "y pass (inverse)");
         hipLaunchKernelGGL(k_ybwd_w<N>, grid, dim3(64 * YWAVES), shm, st, in, out, py.tw, nkx, nky, NY);
     }
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) fprintf(stderr, "coral_amd: launch failed: %s\n", hipGetErrorString(e));
+    check_launch("y pass");
 }
 
 bool launch_ypass_wave(hipStream_t st, bool fwd, const FFTPlan &py, const double2 *in, double2 *out, int nbatch,
                        int NZAA, int nkx, int nky, int NY) {
     if (nky > py.N || NY / 2 + (py.N - nky) > py.N) return false;
+    // the tile of a workgroup must fit the device's LDS (98 KB at N = 768);
+    // otherwise the workgroup-level LDS FFT takes the lines
+    if (ywave_lds(py.N) > lds_limit_current()) return false;
     switch (py.N) {
     case 96: go<96>(st, fwd, py, in, out, nbatch, NZAA, nkx, nky, NY); return true;
     case 144: go<144>(st, fwd, py, in, out, nbatch, NZAA, nkx, nky, NY); return true;

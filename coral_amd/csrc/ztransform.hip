@@ -281,8 +281,7 @@ template <int TE, int TO>
 static void go_bwd(hipStream_t st, const ZFragsDev &z, const ZPart &pt, const double *X, double *F, int NZ, int NZAA,
                    int ncol, int nb, int nblocks) {
     const size_t shm = size_t(TE * z.NEK + TO * z.NOK) * 64 * sizeof(double);
-    (void)hipFuncSetAttribute((const void *)k_zbwd_mfma<TE, TO>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                              (int)shm);
+    if (shm > 65536) set_dynamic_lds((const void *)k_zbwd_mfma<TE, TO>, shm, "z synthesis");
     const double *fe = z.bwd + size_t(pt.e0) * z.NEK * 64, *fo = z.bwd + size_t(z.TE * z.NEK + pt.o0 * z.NOK) * 64;
     hipLaunchKernelGGL((k_zbwd_mfma<TE, TO>), dim3(nblocks), dim3(ZTHREADS), shm, st, fe, fo, z.NEK, z.NOK, X, F, NZ,
                        NZAA, ncol, nb, pt.e0);
@@ -292,8 +291,7 @@ template <int RE, int RO>
 static void go_fwd(hipStream_t st, const ZFragsDev &z, const ZPart &pt, const double *Y, double *Q, int NZ, int NZAA,
                    int ncol, int nb, int nblocks) {
     const size_t shm = size_t(RE * z.KEK + RO * z.KOK) * 64 * sizeof(double);
-    (void)hipFuncSetAttribute((const void *)k_zfwd_mfma<RE, RO>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                              (int)shm);
+    if (shm > 65536) set_dynamic_lds((const void *)k_zfwd_mfma<RE, RO>, shm, "z analysis");
     const double *fe = z.fwd + size_t(pt.e0) * z.KEK * 64, *fo = z.fwd + size_t(z.RE * z.KEK + pt.o0 * z.KOK) * 64;
     hipLaunchKernelGGL((k_zfwd_mfma<RE, RO>), dim3(nblocks), dim3(ZTHREADS), shm, st, fe, fo, z.KEK, z.KOK, Y, Q, NZ,
                        NZAA, ncol, nb, pt.e0, pt.o0);
@@ -318,8 +316,7 @@ bool launch_zbwd_mfma(hipStream_t st, const ZFragsDev &z, const double2 *spec, d
 #undef ZS
         return false;   // not reached: the plan holds compiled shapes only
     }
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) fprintf(stderr, "coral_amd: launch failed: %s\n", hipGetErrorString(e));
+    check_launch("z transform");
     return true;
 }
 
@@ -337,8 +334,7 @@ bool launch_zfwd_mfma(hipStream_t st, const ZFragsDev &z, const double2 *Hy, dou
 #undef ZS
         return false;   // not reached: the plan holds compiled shapes only
     }
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) fprintf(stderr, "coral_amd: launch failed: %s\n", hipGetErrorString(e));
+    check_launch("z transform");
     return true;
 }
 

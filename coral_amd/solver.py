@@ -264,6 +264,18 @@ class Solver:
         check(lib.coral_amd_fwd_transform(self._ctx, p.ctypes.data, out.ctypes.data), "coral_amd_fwd_transform")
         return out
 
+    def solve_system(self, isys: int, rhs: np.ndarray) -> np.ndarray:
+        """(M - dt a11 L) x = rhs for every kept wavenumber of kxky system
+        `isys` with the factors of the last factorization and the solve kernel
+        of the time step.  rhs, x: complex (NX/2, NYAA, total)."""
+        r = np.ascontiguousarray(rhs, np.complex128)
+        exp = (self.NX // 2, self.NYAA, self.totals[isys])
+        if r.shape != exp:
+            raise ValueError(f"rhs must have shape {exp}, got {r.shape}")
+        x = np.empty_like(r)
+        check(lib.coral_amd_system_solve(self._ctx, isys, r.ctypes.data, x.ctypes.data), "coral_amd_system_solve")
+        return x
+
     # --- timing ---------------------------------------------------------------------
     def products(self) -> np.ndarray:
         """Spectra of the quadratic variables of the current state,

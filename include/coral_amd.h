@@ -203,9 +203,24 @@ int coral_amd_fwd_transform(coral_amd_ctx *ctx, const double *phys, double *spec
  * nsys matrices given in LAPACK band storage (ldab = 2kl+ku+1, column j of
  * system s at ab[(s*n + j)*ldab], complex interleaved): the zgbtrf/zgbtrs
  * calls of PL_algebra_z_3D.f90:189-276.  x[s*n + i] is overwritten with the
- * solution; info[s] receives the LAPACK info of each factorization. */
+ * solution; info[s] receives the LAPACK info of each factorization.  Bands
+ * with kl <= 16 and kl + ku <= 47 (Coral's operators: 16 and 43) go through
+ * the factor records, U-record trimming and 16-lane solve kernel of the time
+ * step; other bands through a generic one-wavefront-per-system solve. */
 int coral_amd_banded_solve(int32_t device, int32_t n, int32_t kl, int32_t ku, int32_t nsys,
                            const double *ab, double *x, int32_t *info);
+
+/* Solve (M - dt*a11*L) x = rhs for every kept wavenumber of kxky system isys
+ * with the factors of the last coral_amd_set_dt, through the factor layout and
+ * solve kernel the time step uses: the backsolve of
+ * zOperator_3d_1coupled_T%backsolve (PL_algebra_z_3D.f90:189-243), called as
+ * backsolve_to_aux at PL_IMEX_timestepping_marching.f90:57-58.  rhs and x are
+ * complex in the state layout (Fortran (total, NYAA, NX/2)); dealiased modes
+ * of rhs are ignored and set to zero in x; the kxky (0,0) system is the
+ * identity (factorize_operators, PL_IMEX_timestepping.f90:387-406), so x
+ * equals rhs there.  The state is left unchanged (the stage buffers are used
+ * as work space). */
+int coral_amd_system_solve(coral_amd_ctx *ctx, int32_t isys, const double *rhs, double *x);
 
 /* Assembly only, no device touched: the dense operator of kxky system isys at
  * integer wavenumbers (kx_int, ky_int), i.e. the per-wavenumber CSR matrix the

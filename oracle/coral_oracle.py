@@ -467,6 +467,18 @@ def phys_to_spec(P: np.ndarray, geo: Geometry) -> np.ndarray:
 # The solver (PL_IMEX_timestepping.f90 + PL_IMEX_timestepping_marching.f90)
 # ----------------------------------------------------------------------------
 
+def remove_z_integral(s, NZ):
+    """In place: the Chebyshev row 0 of spectra s[..., n] set so that the
+    z-integral of the series vanishes, summing the even rows 2q for
+    q = 1 .. NZ/2 - 1 (remove_vertical_mean_of_quadratic_var,
+    PL_IMEX_timestepping.f90:223-235; for odd NZ the loop bound leaves row
+    NZ - 1 out)."""
+    s[..., 0] = 0.0
+    for k in range(1, NZ // 2):
+        s[..., 0] -= s[..., 2 * k] * (0.5 / (2.0 * k + 1.0) - 0.5 / (2.0 * k - 1.0))
+    return s
+
+
 class OracleSolver:
     """State and one_step_beyond of ``full_problem_data_structure_T``.
 
@@ -628,10 +640,8 @@ class OracleSolver:
             else:
                 prod = phys[q.ivar1] * phys[q.ivar2]
             s = phys_to_spec(prod, g)
-            if q.remove_z_integral:  # PL_IMEX_timestepping.f90:223-235
-                s[:, :, 0] = 0.0
-                for k in range(1, g.NZ // 2):
-                    s[:, :, 0] -= s[:, :, 2 * k] * (0.5 / (2.0 * k + 1.0) - 0.5 / (2.0 * k - 1.0))
+            if q.remove_z_integral:
+                remove_z_integral(s, g.NZ)
             qhat.append(s)
         return qhat
 

@@ -275,6 +275,122 @@ def test_factorization_at_production_size(solver_mod):
         assert rel(x[s], refs[s]) <= 1e-11
 
 
+def test_system_solve_matches_lapack_at_production_size(solver_mod):
+    """coral_amd_system_solve: the factors the time step uses (real form,
+    even/odd structural blocks, U records trimmed to the longest prefix over
+    all 32,640 wavenumbers) and its 16-lane solve kernel, at 256^2 x 129
+    (rotating Ra = 5e7, dt = 2e-5), against LAPACK zgbtrf/zgbtrs of the
+    assembled complex operator M - dt a11 L (PL_algebra_z_3D.f90:189-276) at
+    sampled wavenumbers: the corners of the kept set, the axes and random ones."""
+    r = product_recipe("coral.equations.rrbc_ra5e7_ek1e-4")
+    NX, NZ, dt, L = 256, 129, 2e-5, 2 * np.pi
+    a11 = 1.0 - 1.0 / math.sqrt(2.0)
+    kx, ky, kept = ic.wavenumber_indices(NX, NX)
+    with solver_mod.Solver(r, NX, NX, NZ, L, L) as g:
+        g.factorize_operators(dt)
+        rng = np.random.default_rng(5)
+        shape = (NX // 2, 3 * NX // 2, g.totals[0])
+        b = rng.normal(size=shape) + 1j * rng.normal(size=shape)
+        x = g.solve_system(0, b)
+    np.testing.assert_array_equal(x[0, 0], b[0, 0])     # the (0,0) system is the identity
+    assert not x[~kept].any()
+    slots = [(0, 1), (1, 0), (0, 127), (0, 384 - 127), (127, 1), (127, 127), (127, 384 - 127), (64, 0), (1, 1)]
+    kept_idx = np.argwhere(kept)
+    slots += [tuple(kept_idx[i]) for i in rng.choice(len(kept_idx), 15, replace=False)]
+    worst = 0.0
+    for ix, iy in slots:
+        if (ix, iy) == (0, 0):
+            continue
+        M, nl, nu = solver_mod.assembled_operator(r, NX, NX, NZ, L, L, 0, "mass", int(kx[ix]), int(ky[iy]))
+        S, _, _ = solver_mod.assembled_operator(r, NX, NX, NZ, L, L, 0, "stif", int(kx[ix]), int(ky[iy]))
+        ab = O.dense_to_lapack_band(M - dt * a11 * S, nl, nu)
+        lu, piv, info = lap.zgbtrf(ab.T.copy(), nl, nu)
+        assert info == 0
+        xr, _ = lap.zgbtrs(lu, nl, nu, b[ix, iy], piv)
+        worst = max(worst, rel(x[ix, iy], xr))
+    print("worst rel L2 over", len(slots), "wavenumbers:", worst, flush=True)
+    assert worst <= 1e-11
+
+
+def test_production_workload_two_steps_match_oracle(solver_mod):
+    """The 256^2 x 129 rotating workload (Ek = 1e-4, Ra = 5e7, dt = 2e-5) at
+    its stated amplitude 1e-2 on |k| <= 32: two ARS_222 steps of the HIP path
+    against the oracle, every kept mode, relative L2 <= 1e-10 (the full-size
+    solve, trimmed U records, transforms and products together)."""
+    r = product_recipe("coral.equations.rrbc_ra5e7_ek1e-4")
+    NX, NZ, dt, L = 256, 129, 2e-5, 2 * np.pi
+    k0, z0 = ic.rbc_initial_state(r, NX, NX, NZ, 1e-2, 32, 8, 2)
+    with solver_mod.Solver(r, NX, NX, NZ, L, L) as g:
+        g.set_state(k0, z0)
+        g.march_forward(dt, 2)
+        k, z = g.get_state()
+        cfl = g.cfl_dt()
+    geo = O.Geometry(NX, NX, NZ, L, L)
+    ora = O.OracleSolver(r, geo, "ARS_222")
+    ora.field, ora.zfield = [a.copy() for a in k0], [a.copy() for a in z0]
+    ora.run(dt, 2)
+    e = rel(k[0], ora.field[0])
+    print("256^2 x 129, 2 steps: rel L2", e, "cfl", cfl, ora.cfl_dt, flush=True)
+    assert np.isfinite(k[0]).all() and np.linalg.norm(k[0] - k0[0]) > 0
+    assert e <= TOL
+    for a, b in zip(z, ora.zfield):
+        assert rel(a, b) <= TOL
+    assert abs(cfl / ora.cfl_dt - 1.0) <= 1e-10
+
+
+def _zfiltered_recipe(filtered=("uw", "tw", "vv")):
+    """The RBC recipe with some quadratic variables declared as
+    zFiltered_quadratic_variable (PL_equations.f90:1884-1905), whose z-integral
+    the step removes (PL_IMEX_timestepping.f90:223-235)."""
+    from coral_amd.equations import parse_equations
+    from conftest import ROOT
+    text = open(os.path.join(ROOT, "coral_amd", "inputs", "coral.equations.rbc_ra1e5")).read()
+    for q in filtered:
+        text = text.replace(f">>quadratic_variable :: {q} :=", f">>zFiltered_quadratic_variable :: {q} :=")
+    r = parse_equations(text)
+    assert sum(q.remove_z_integral for q in r.quad_vars) == len(filtered)
+    return r
+
+
+@pytest.mark.parametrize("NX,NY,NZ", [(16, 16, 17), (32, 24, 32)])
+def test_zfiltered_quadratics_match_oracle(solver_mod, NX, NY, NZ):
+    """z-integral removal of zFiltered quadratic variables (k_remove_zmean):
+    the products seam and three steps against the oracle, at odd and even NZ;
+    the filtered spectra differ from the unfiltered ones in row 0 only."""
+    r = _zfiltered_recipe()
+    r0 = product_recipe("coral.equations.rbc_ra1e5")
+    k0, z0 = ic.rbc_initial_state(r, NX, NY, NZ, 1e-2, 6, 6, 7)
+    rng = np.random.default_rng(NZ)
+    z0 = [a + 1e-2 * rng.normal(size=a.shape) for a in z0]
+    geo = O.Geometry(NX, NY, NZ, 2 * np.pi, 2 * np.pi)
+    ora = O.OracleSolver(r, geo, "ARS_222")
+    qref = ora.quad_spectra([a.copy() for a in k0], [a.copy() for a in z0])
+    with solver_mod.Solver(r0, NX, NY, NZ, 2 * np.pi, 2 * np.pi) as g0:
+        g0.set_state(k0, z0)
+        q0 = g0.products()
+    with solver_mod.Solver(r, NX, NY, NZ, 2 * np.pi, 2 * np.pi) as g:
+        g.set_state(k0, z0)
+        q = g.products()
+        wn = np.zeros(q.shape[2], bool)
+        wn[: NY // 2] = True
+        wn[q.shape[2] - (NY // 2 - 1):] = True
+        for i, qv in enumerate(r.quad_vars):
+            assert rel(q[i][:, wn], qref[i][:, wn]) <= TOL
+            if qv.remove_z_integral:
+                assert np.array_equal(q[i][..., 1:], q0[i][..., 1:])
+                assert not np.allclose(q[i][..., 0], q0[i][..., 0])
+            else:
+                assert np.array_equal(q[i], q0[i])
+        ora.field, ora.zfield = [a.copy() for a in k0], [a.copy() for a in z0]
+        for it in range(3):
+            ora.step(1e-4)
+            g.march_forward(1e-4)
+        k, z = g.get_state()
+    assert rel(k[0], ora.field[0]) <= TOL
+    for a, b in zip(z, ora.zfield):
+        assert rel(a, b) <= TOL
+
+
 def test_transforms_match_oracle(solver_mod):
     r = product_recipe("coral.equations.rbc_ra1e5")
     for NX, NY, NZ, Lx, Ly in [(16, 16, 17, 2 * np.pi, 2 * np.pi), (64, 48, 33, 2.0, 3.0)]:
@@ -451,13 +567,17 @@ def test_structural_blocks_match_whole_system_factors(solver_mod, monkeypatch):
 
 def test_graph_replay_matches_eager(solver_mod):
     """Steps replayed as HIP graphs (Solver.use_graphs) against eager steps, bit
-    for bit: 2 + 3 replays (both buffer assignments), state downloads between
-    them, a re-upload, then a change of dt (which drops and recaptures the
-    graphs); the mean systems' side stream is part of every graph."""
+    for bit: 2 + 3 replays (both buffer assignments), an eager step with the
+    timers on between replays, state downloads between them, a re-upload, then
+    a change of dt (which drops and recaptures the graphs); the mean systems'
+    side stream is part of every graph."""
     name, NX, NY, NZ, L, dt = "coral.equations.rrbc_ra5e7_ek1e-4", 64, 64, 33, 2 * np.pi, 2e-5
     r = product_recipe(name)
     k0, z0 = ic.rbc_initial_state(r, NX, NY, NZ, 1e-2, 10 * NX, 8, 11)
-    plan = [("run", dt, 2), ("run", dt, 3), ("reset", None, None), ("run", dt, 1), ("run", dt / 2, 2)]
+    # timers on make steps run eagerly between replays (one step: the buffer
+    # assignment flips), which the replays that follow must see
+    plan = [("run", dt, 2), ("run", dt, 3), ("timers", True, None), ("run", dt, 1), ("timers", False, None),
+            ("run", dt, 3), ("reset", None, None), ("run", dt, 1), ("run", dt / 2, 2)]
     results = []
     for graphs in (False, True):
         out = []
@@ -468,6 +588,9 @@ def test_graph_replay_matches_eager(solver_mod):
             for what, h, n in plan:
                 if what == "reset":
                     g.set_state(k0, z0)
+                    continue
+                if what == "timers":
+                    g.timers_enable(h)
                     continue
                 print("graphs", graphs, "dt", h, "steps", n, flush=True)
                 g.march_forward(h, n)

@@ -197,6 +197,72 @@ def test_stressfree_growth_rate(reference_numbers):
     assert abs(np.max(w.real) - ref["s_max"]) < 1e-9
 
 
+def stability_rate(scheme, dt, lam):
+    """Growth rate per unit time of the ARS implicit stages for dx/dt = lam x:
+    x_i = x^n + dt sum_{j<=i} a_ij lam x_j, x^{n+1} = x_s, i.e.
+    log(e_s^T (I - dt lam A)^-1 1) / dt (IMEX_schemes.f90 A_arr)."""
+    from coral_amd.schemes import tableau
+    A, _ = tableau(scheme)
+    R = np.linalg.solve(np.eye(A.shape[0]) - dt * lam * A, np.ones(A.shape[0]))[-1]
+    return math.log(R) / dt
+
+
+def stepped_growth_rate(step, state_norm, dt, n0, n1):
+    """Growth rate fitted from the norm of one wavenumber's state between
+    steps n0 and n1 of a run (step() advances one step)."""
+    norms = {}
+    for n in range(1, n1 + 1):
+        step()
+        if n in (n0, n1):
+            norms[n] = state_norm()
+    return math.log(norms[n1] / norms[n0]) / ((n1 - n0) * dt)
+
+
+@pytest.mark.parametrize("scheme,dt,n0,n1", [("ARS_222", 1e-3, 400, 800), ("ARS_443", 2e-3, 200, 400)])
+def test_stressfree_growth_rate_by_time_stepping(reference_numbers, scheme, dt, n0, n1):
+    """The oracle's kxky time stepping (factorisation, right-hand sides, solves
+    and the nonlinear path at negligible amplitude) on the stress-free
+    benchmark (Ra = 2000, Pr = 7) in a box whose first x wavenumber is the
+    notebook's k_opt: the temperature perturbation of that wavenumber grows
+    at exactly the rate the scheme's stability function gives for the
+    published growth rate 24.137487 (benchmark_01...ipynb:68).  Other
+    eigenvalues of the pencil (the next is -39.4) have decayed by the fit
+    window.  A wrong tableau entry, sign or operator term moves the rate."""
+    ref = reference_numbers["stressfree_rb_max_growth_rate"]
+    r = recipe("stressfree_rb.coral.equations")
+    L = 2 * math.pi / ref["k_opt"]
+    NX, NZ = 8, 24
+    geo = O.Geometry(NX, NX, NZ, L, L)
+    k0, z0 = ic.rbc_initial_state(r, NX, NX, NZ, 1e-30, 1, 8, 5, var_name="t")
+    ora = O.OracleSolver(r, geo, scheme)
+    ora.field, ora.zfield = [a.copy() for a in k0], [a.copy() for a in z0]
+    rate = stepped_growth_rate(lambda: ora.step(dt), lambda: np.linalg.norm(ora.field[0][1, 0]), dt, n0, n1)
+    expect = stability_rate(scheme, dt, ref["s_max"])
+    print(scheme, "stepped", rate, "stability function", expect, "continuous", ref["s_max"])
+    assert abs(rate / expect - 1.0) < 1e-10
+    # and the scheme's time error is what separates it from the continuous rate
+    assert abs(rate - ref["s_max"]) < 1e-3 * ref["s_max"]
+
+
+# --- z-integral removal of zFiltered quadratic variables ----------------------
+
+@pytest.mark.parametrize("NZ", [16, 17, 32, 33])
+def test_z_integral_removal_zeroes_the_integral(NZ):
+    """remove_z_integral (PL_IMEX_timestepping.f90:223-235): the Chebyshev
+    series integrates to zero over [-1, 1] afterwards (numpy's Chebyshev
+    integral), except that for odd NZ the reference's loop stops at row
+    NZ - 3, so row NZ - 1 keeps its share of the integral: 2 c / (1 - (NZ-1)^2)."""
+    rng = np.random.default_rng(NZ)
+    s = rng.normal(size=(3, NZ)) + 1j * rng.normal(size=(3, NZ))
+    f = O.remove_z_integral(s.copy(), NZ)
+    assert np.array_equal(f[:, 1:], s[:, 1:])
+    for row in f:
+        integ = C.chebint(row)
+        total = C.chebval(1.0, integ) - C.chebval(-1.0, integ)
+        left = 0.0 if NZ % 2 == 0 else 2.0 * row[NZ - 1] / (1.0 - (NZ - 1) ** 2)
+        assert abs(total - left) <= 1e-13 * np.abs(row).sum()
+
+
 # --- heat-equation error table (IMEX tableaux + mean operators + solve) --------
 
 def heat_error(scheme, dt, nsteps=10):

@@ -64,6 +64,7 @@ EXPORTS = (
     "coral_amd_timers_enable", "coral_amd_timer_read", "coral_amd_assembled_operator", "coral_amd_use_graphs", "coral_amd_products",
     "coral_amd_factor_kind", "coral_amd_get_variable_physical", "coral_amd_set_variable_physical",
     "coral_amd_horizontal_averages", "coral_amd_factor_blocks", "coral_amd_system_solve",
+    "coral_amd_set_precision",
 )
 
 
@@ -90,6 +91,7 @@ def _load():
     lib.coral_amd_bwd_transform.argtypes = [P, P, P]
     lib.coral_amd_fwd_transform.argtypes = [P, P, P]
     lib.coral_amd_system_solve.argtypes = [P, c_int32, P, P]
+    lib.coral_amd_set_precision.argtypes = [P, c_int32, c_int32]
     lib.coral_amd_banded_solve.argtypes = [c_int32, c_int32, c_int32, c_int32, c_int32, P, P, P]
     lib.coral_amd_timers_enable.argtypes = [P, c_int32]
     lib.coral_amd_use_graphs.argtypes = [P, c_int32]

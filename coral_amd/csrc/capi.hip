@@ -82,6 +82,8 @@ struct SysDev {
     DevBuf<unsigned char> rot;
     DevBuf<unsigned char> piv;
     DevBuf<int> info;
+    // residual refinement: A x - rhs and the correction solved from it
+    DevBuf<double2> Rres, Fcor;
     struct OpBufs {
         DevBuf<int> mono_off, mono_a, mono_b, ell_w, ell_off, ell_col;
         DevBuf<double> band, ell_val;
@@ -137,6 +139,16 @@ struct coral_amd_ctx {
     DevBuf<unsigned long long> cflmax;
     DevBuf<double2> sink;   // stores of padding positions in the 16-lane solve
     FFTPlan px, py;
+    // Precision of the step's transform chain: 0 = FP64; 1 = FP32 transforms
+    // and products (z products on FP32 MFMA with compensated accumulation),
+    // NL sums, right-hand sides, factors and solves in FP64.
+    int precision = 0;
+    // FP64 residual-refinement sweeps after the solves: 0 none, 1 after the
+    // last stage of each step, 2 after every stage
+    int refine = 0;
+    DevBuf<float2> spec32, Fz32, G32, H32, Hy32, Qhat32, twx32, twy32;
+    DevBuf<float> zfrag32_b, zfrag32_f;
+    ZFrags32Dev z32{};
     double dt = 0.0;
     bool factored = false;
     bool timing = false;
@@ -538,6 +550,7 @@ void nl_assemble(coral_amd_ctx &c, SysDev &d, double2 *Khat) {
         a.cmono = d.nl_cmono.p;
         a.nterms = d.nl_nterms;
         a.Qhat = c.Qhat.p;
+        a.Qhat32 = c.precision == 1 ? c.Qhat32.p : nullptr;
         a.nsys = c.P.nsys;
         a.total = t.total;
         a.Khat = Khat;
@@ -546,6 +559,7 @@ void nl_assemble(coral_amd_ctx &c, SysDev &d, double2 *Khat) {
     }
     NLArgs a{};
     a.Qhat = c.Qhat.p;
+    a.Qhat32 = c.precision == 1 ? c.Qhat32.p : nullptr;
     a.nsys = c.P.nsys;
     a.NZ = c.P.NZ;
     a.total = t.total;
@@ -599,7 +613,9 @@ void on_side(coral_amd_ctx &c, F &&f) {
 
 // Chebyshev coefficients of all full variables from the given states, then the
 // backward transforms up to G.
-void backward_to_G(coral_amd_ctx &c, const std::vector<const double2 *> &kst, const std::vector<const double2 *> &zst) {
+// fp32: the FP32 chain into spec32 / Fz32 / G32 (the step in precision mode 1)
+void backward_to_G(coral_amd_ctx &c, const std::vector<const double2 *> &kst, const std::vector<const double2 *> &zst,
+                   bool fp32 = false) {
     const Problem &P = c.P;
     const size_t fsz = size_t(P.nsys) * P.NZ;
     c.timed("extract", [&] {
@@ -630,14 +646,26 @@ void backward_to_G(coral_amd_ctx &c, const std::vector<const double2 *> &kst, co
         }
         a.full_first[P.nfull] = nt;
         a.nterms = nt;
+        if (fp32) a.spec32 = c.spec32.p;
         if (P.nfull > 0) launch_extract_all(c.stream, a);
         join_mean(c);   // the mean states below come from the side stream
         for (auto &e : c.ext) {
             if (e->kind != 1) continue;
-            launch_extract_mean(c.stream, zst[e->system], P.NZ, P.nsys, e->rowptr.p, e->col.p, e->val.p,
-                                c.spec.p + e->full * fsz);
+            if (fp32)
+                launch_extract_mean32(c.stream, zst[e->system], P.NZ, P.nsys, e->rowptr.p, e->col.p, e->val.p,
+                                      c.spec32.p + e->full * fsz);
+            else
+                launch_extract_mean(c.stream, zst[e->system], P.NZ, P.nsys, e->rowptr.p, e->col.p, e->val.p,
+                                    c.spec.p + e->full * fsz);
         }
     });
+    if (fp32) {
+        c.timed("zbwd", [&] { launch_zbwd32(c.stream, c.z32, c.spec32.p, c.Fz32.p, P.NZ, P.NZAA, P.nsys, P.nfull); });
+        c.timed("ybwd", [&] {
+            launch_ypass32(c.stream, false, c.py, c.Fz32.p, c.G32.p, P.nfull, P.NZAA, P.nkx, P.nky, P.NY);
+        });
+        return;
+    }
     c.timed("zbwd", [&] { zbwd(c, c.spec.p, c.Fz.p, P.nfull); });
     c.timed("ybwd", [&] { launch_ybwd(c.stream, c.py, c.Fz.p, c.G.p, P.nfull, P.NZAA, P.nkx, P.nky, P.NY); });
 }
@@ -647,13 +675,14 @@ void backward_to_G(coral_amd_ctx &c, const std::vector<const double2 *> &kst, co
 // (marching.f90:279-300), z-integral removal (PL_IMEX_timestepping.f90:223-235);
 // the CFL maxima of the full variables on the way.
 void products_to_Qhat(coral_amd_ctx &c, const std::vector<const double2 *> &kst,
-                      const std::vector<const double2 *> &zst) {
+                      const std::vector<const double2 *> &zst, bool fp32 = false) {
     const Problem &P = c.P;
-    backward_to_G(c, kst, zst);
+    backward_to_G(c, kst, zst, fp32);
     zero_cflmax(c.stream, c.cflmax.p);
     XPassArgs xa{};
-    xa.G = c.G.p;
-    xa.H = c.H.p;
+    xa.G = fp32 ? reinterpret_cast<const double2 *>(c.G32.p) : c.G.p;
+    xa.H = fp32 ? reinterpret_cast<double2 *>(c.H32.p) : c.H.p;
+    xa.fp32 = fp32 ? 1 : 0;
     xa.nfull = P.nfull;
     xa.nquad = P.nquad;
     xa.NZAA = P.NZAA;
@@ -664,6 +693,16 @@ void products_to_Qhat(coral_amd_ctx &c, const std::vector<const double2 *> &kst,
     xa.zweight = c.zweight.p;
     xa.cflmax = c.cflmax.p;
     c.timed("xpass", [&] { launch_xpass(c.stream, c.px, xa); });
+    if (fp32) {
+        c.timed("yfwd", [&] {
+            launch_ypass32(c.stream, true, c.py, c.H32.p, c.Hy32.p, P.nquad, P.NZAA, P.nkx, P.nky, P.NY);
+        });
+        c.timed("zfwd", [&] { launch_zfwd32(c.stream, c.z32, c.Hy32.p, c.Qhat32.p, P.NZ, P.NZAA, P.nsys, P.nquad); });
+        for (int q = 0; q < P.nquad; ++q)
+            if (P.quad_filter[q])
+                launch_remove_zmean32(c.stream, c.Qhat32.p + size_t(q) * P.nsys * P.NZ, P.NZ, P.nsys);
+        return;
+    }
     c.timed("yfwd", [&] { launch_yfwd(c.stream, c.py, c.H.p, c.Hy.p, P.nquad, P.NZAA, P.nkx, P.nky, P.NY); });
     c.timed("zfwd", [&] { zfwd(c, c.Hy.p, c.Qhat.p, P.nquad); });
     for (int q = 0; q < P.nquad; ++q)
@@ -672,7 +711,7 @@ void products_to_Qhat(coral_amd_ctx &c, const std::vector<const double2 *> &kst,
 
 void nl_terms(coral_amd_ctx &c, const std::vector<const double2 *> &kst, const std::vector<const double2 *> &zst,
               int stage) {
-    products_to_Qhat(c, kst, zst);
+    products_to_Qhat(c, kst, zst, c.precision == 1);
     for (auto &d : c.ks) nl_assemble(c, *d, d->Khat[stage]->p);
     if (!c.zs.empty()) HIP_OK(hipEventRecord(c.ev_qhat, c.stream));
 }
@@ -686,9 +725,9 @@ void nl_terms_mean(coral_amd_ctx &c, int stage) {
 
 // (M - dt a11 L) out = rhs for every wavenumber of a system, with the factors
 // of the last set_dt (backsolve_to_aux, PL_IMEX_timestepping_marching.f90:57-58)
-void solve_system(coral_amd_ctx &c, SysDev &d, const double2 *rhs, double2 *out) {
+void solve_system(coral_amd_ctx &c, SysDev &d, const double2 *rhs, double2 *out, const char *name = nullptr) {
     const SystemTables &t = *d.t;
-    c.timed(d.kxky ? "solve" : "solve_mean", [&] {
+    c.timed(name ? name : (d.kxky ? "solve" : "solve_mean"), [&] {
         const int kv = t.nl + t.nu, id0 = d.kxky ? 1 : 0;
         bool done = true;
         if (d.ncol16 > 0) {
@@ -718,6 +757,25 @@ void solve_system(coral_amd_ctx &c, SysDev &d, const double2 *rhs, double2 *out)
         else
             launch_solve(c.stream, d.Lf.p, d.Uf.p + SOLVE16_PAD, d.piv.p, t.total, t.nl, kv, d.nsys, rhs, out);
         if (!done) throw std::runtime_error("16-lane solve rejected a band it was factored for");
+    });
+}
+
+__global__ void k_sub_inplace(double2 *x, const double2 *f, size_t n) {
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) x[i] = csub(x[i], f[i]);
+}
+
+// One sweep of FP64 residual refinement of (M - dt a_ii L) x = rhs:
+// e = (M - dt a_ii L) x - rhs (the right-hand-side kernel with x as both the
+// mass and the stiffness operand), solve for f with the same factors, x -= f.
+void refine_solution(coral_amd_ctx &c, SysDev &d, const double2 *rhs_i, double2 *x, double aii) {
+    if (!d.Rres.p || !d.Fcor.p) throw std::runtime_error("refinement buffers missing");
+    rhs(c, d, {{x, 1.0}}, {{x, -c.dt * aii}}, {{rhs_i, -1.0}}, d.Rres.p);
+    solve_system(c, d, d.Rres.p, d.Fcor.p, d.kxky ? "refine_solve" : "refine_solve_mean");
+    const size_t n = size_t(d.nsys) * d.t->total;
+    c.timed(d.kxky ? "refine_update" : "refine_update_mean", [&] {
+        hipLaunchKernelGGL(k_sub_inplace, dim3(unsigned((n + 255) / 256)), dim3(256), 0, c.stream, x, d.Fcor.p, n);
+        check_launch("refine_update");
     });
 }
 
@@ -772,6 +830,7 @@ void one_step(coral_amd_ctx &c) {
             double2 *rhs_i = d.R.p;
             rhs(c, d, ms, xs, ks, rhs_i);
             solve_system(c, d, rhs_i, d.X[i]->p);
+            if (c.refine == 2 || (c.refine == 1 && i == s - 1)) refine_solution(c, d, rhs_i, d.X[i]->p, A(i, i));
         };
         if (!c.zs.empty()) {
             on_side(c, [&] {
@@ -858,6 +917,11 @@ void fill_zero(hipStream_t st, double2 *p, size_t n) {
 }
 
 }  // namespace
+
+namespace coral {
+// error message of the calling thread, for entry points in other sources
+void set_last_error(const std::string &msg) { g_err = msg; }
+}  // namespace coral
 
 // ---------------------------------------------------------------------------
 // C ABI
@@ -1450,11 +1514,18 @@ int coral_amd_products(coral_amd_ctx *c, double *out) {
         std::vector<const double2 *> kst, zst;
         for (auto &d : c->ks) kst.push_back(d->field.p);
         for (auto &d : c->zs) zst.push_back(d->field.p);
-        products_to_Qhat(*c, kst, zst);
+        const bool fp32 = c->precision == 1;
+        products_to_Qhat(*c, kst, zst, fp32);
         join_mean(*c);
         HIP_OK(hipStreamSynchronize(c->stream));
         std::vector<double2> h(size_t(P.nquad) * P.NZ * P.nsys);
-        HIP_OK(hipMemcpy(h.data(), c->Qhat.p, h.size() * sizeof(double2), hipMemcpyDeviceToHost));
+        if (fp32) {
+            std::vector<float2> hf(h.size());
+            HIP_OK(hipMemcpy(hf.data(), c->Qhat32.p, hf.size() * sizeof(float2), hipMemcpyDeviceToHost));
+            for (size_t i = 0; i < h.size(); ++i) h[i] = make_double2(hf[i].x, hf[i].y);
+        } else {
+            HIP_OK(hipMemcpy(h.data(), c->Qhat.p, h.size() * sizeof(double2), hipMemcpyDeviceToHost));
+        }
         // [q][ix][y slot][n], dealiased ky slots zero
         const size_t per_q = size_t(P.nkx) * P.NYAA * P.NZ;
         std::fill(out, out + 2 * per_q * P.nquad, 0.0);
@@ -1699,6 +1770,71 @@ int coral_amd_assembled_operator(const coral_amd_problem *prob, int32_t isys, in
                     out[2 * (size_t(i) * n + j) + 1] += b * cf.y;
                 }
         }
+        return 0;
+    } catch (const std::exception &e) {
+        return fail(e);
+    }
+}
+
+int coral_amd_set_precision(coral_amd_ctx *c, int32_t mode, int32_t refine) {
+    try {
+        if (!c) throw std::runtime_error("null context");
+        if (mode < 0 || mode > 1) throw std::runtime_error("precision mode must be 0 (FP64) or 1 (FP32 transforms)");
+        if (refine < 0 || refine > 2) throw std::runtime_error("refine must be 0, 1 (last stage) or 2 (every stage)");
+        HIP_OK(hipSetDevice(c->device));
+        HIP_OK(hipStreamSynchronize(c->stream));
+        HIP_OK(hipStreamSynchronize(c->side));
+        Problem &P = c->P;
+        if (mode == 1 && !c->spec32.p) {
+            ZFrags32 zf;
+            if (!make_zfrags32(P.NZ, P.NZAA, P.Tz, P.Cz, lds_limit_current(), zf))
+                throw std::runtime_error("no FP32 MFMA plan for the z transforms at NZ = " + std::to_string(P.NZ));
+            auto f32 = [](const FFTPlan &pl, DevBuf<float2> &buf) {
+                std::vector<double2> h(pl.N);
+                HIP_OK(hipMemcpy(h.data(), pl.tw, pl.N * sizeof(double2), hipMemcpyDeviceToHost));
+                std::vector<float2> f(pl.N);
+                for (int k = 0; k < pl.N; ++k) f[k] = make_float2(float(h[k].x), float(h[k].y));
+                buf.upload(f);
+            };
+            f32(c->px, c->twx32);
+            f32(c->py, c->twy32);
+            c->px.twf = c->twx32.p;
+            c->py.twf = c->twy32.p;
+            const size_t nfull = std::max(P.nfull, 1), nquad = std::max(P.nquad, 1);
+            c->spec32.alloc(nfull * P.nsys * P.NZ);
+            c->Fz32.alloc(nfull * P.NZAA * P.nsys);
+            c->G32.alloc(nfull * size_t(P.NZAA) * P.NYAA * P.nkx);
+            c->H32.alloc(nquad * size_t(P.NZAA) * P.NYAA * P.nkx);
+            c->Hy32.alloc(nquad * size_t(P.NZAA) * P.nsys);
+            c->Qhat32.alloc(nquad * size_t(P.nsys) * P.NZ);
+            HIP_OK(hipMemset(c->Qhat32.p, 0, c->Qhat32.n * sizeof(float2)));
+            c->zfrag32_b.upload(zf.bwd);
+            c->zfrag32_f.upload(zf.fwd);
+            int ncu = 0;
+            HIP_OK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->device));
+            ZFrags32Dev z{};
+            z.TE = zf.TE, z.TO = zf.TO, z.NEK = zf.NEK, z.NOK = zf.NOK;
+            z.RE = zf.RE, z.RO = zf.RO, z.KEK = zf.KEK, z.KOK = zf.KOK;
+            z.bwd_ok = zf.bwd_ok, z.fwd_ok = zf.fwd_ok;
+            z.ncu = std::max(ncu, 1);
+            z.bwd = c->zfrag32_b.p, z.fwd = c->zfrag32_f.p;
+            z.nbp = zf.nbp, z.nfp = zf.nfp;
+            for (int q = 0; q < ZMAXPARTS; ++q) {
+                z.bp[q] = zf.bp[q];
+                z.fp[q] = zf.fp[q];
+            }
+            c->z32 = z;
+        }
+        if (refine > 0)
+            for (auto *v : {&c->ks, &c->zs})
+                for (auto &d : *v)
+                    if (!d->Rres.p) {
+                        d->Rres.alloc(size_t(d->nsys) * d->t->total);
+                        d->Fcor.alloc(size_t(d->nsys) * d->t->total);
+                    }
+        c->drop_graphs();
+        c->precision = mode;
+        c->refine = refine;
         return 0;
     } catch (const std::exception &e) {
         return fail(e);

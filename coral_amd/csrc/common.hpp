@@ -52,6 +52,33 @@ __host__ __device__ inline double2 crecip(double2 z) {
     return make_double2(r / d, -1.0 / d);
 }
 
+// single-precision complex (the FP32 transform chain)
+__host__ __device__ inline float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__host__ __device__ inline float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__host__ __device__ inline float2 cmul(float2 a, float2 b) {
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__host__ __device__ inline float2 cscale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
+__host__ __device__ inline float2 cconj(float2 a) { return make_float2(a.x, -a.y); }
+
+// Complex element types of the transform kernels: the real type and a maker.
+template <class C>
+struct CxT;
+template <>
+struct CxT<double2> {
+    using R = double;
+    __host__ __device__ static inline double2 make(double x, double y) { return make_double2(x, y); }
+};
+template <>
+struct CxT<float2> {
+    using R = float;
+    __host__ __device__ static inline float2 make(float x, float y) { return make_float2(x, y); }
+};
+template <class C>
+__host__ __device__ inline C mkc(typename CxT<C>::R x, typename CxT<C>::R y) {
+    return CxT<C>::make(x, y);
+}
+
 // (i k)^a for integer a >= 0: the wavenumber factor px^a of
 // recipe_to_prefactors_array (PL_IMEX_timestepping.f90:1558-1574); k^a is
 // formed by repeated multiplication, as the complex power of (0, k) is.

@@ -98,7 +98,11 @@ __global__ __launch_bounds__(512) void k_extract_all(ExtractAllArgs a, int xt) {
     __syncthreads();
     for (int e = threadIdx.x; e < nf * NZ * ns; e += blockDim.x) {
         const int row = e / ns, sl = e - row * ns;  // row = f * NZ + n
-        a.spec[(size_t)row * nsys + s0 + sl] = tile[(size_t)row * xt + sl];
+        const double2 v = tile[(size_t)row * xt + sl];
+        if (a.spec32)
+            a.spec32[(size_t)row * nsys + s0 + sl] = make_float2(float(v.x), float(v.y));
+        else
+            a.spec[(size_t)row * nsys + s0 + sl] = v;
     }
 }
 
@@ -170,10 +174,17 @@ __global__ __launch_bounds__(256) void k_extract_lds(ExtractAllArgs a) {
                 acc[sl] = cadd(acc[sl], part[sl]);
             }
         }
-        double2 *out = a.spec + (size_t)row * nsys + s0;
+        if (a.spec32) {
+            float2 *out = a.spec32 + (size_t)row * nsys + s0;
 #pragma unroll
-        for (int sl = 0; sl < XT; ++sl)
-            if (sl < ns) out[sl] = acc[sl];
+            for (int sl = 0; sl < XT; ++sl)
+                if (sl < ns) out[sl] = make_float2(float(acc[sl].x), float(acc[sl].y));
+        } else {
+            double2 *out = a.spec + (size_t)row * nsys + s0;
+#pragma unroll
+            for (int sl = 0; sl < XT; ++sl)
+                if (sl < ns) out[sl] = acc[sl];
+        }
     }
 }
 
@@ -204,19 +215,28 @@ void launch_extract_all(hipStream_t st, const ExtractAllArgs &a) {
     CHECK_LAUNCH();
 }
 
-// mean contributions land in the (0,0) column: spec[n][0]
+// mean contributions land in the (0,0) column: spec[n][0] (C = double2, or
+// float2 in the FP32 chain, where the FP64 sum is rounded once)
+template <class C>
 __global__ void k_extract_mean(const double2 *__restrict__ zstate, int NZ, int nsys, const int *__restrict__ rowptr,
-                               const int *__restrict__ col, const double *__restrict__ val, double2 *spec) {
+                               const int *__restrict__ col, const double *__restrict__ val, C *spec) {
     for (int n = threadIdx.x; n < NZ; n += blockDim.x) {
         double acc = 0.0;
         for (int k = rowptr[n]; k < rowptr[n + 1]; ++k) acc += val[k] * zstate[col[k]].x;
-        spec[(size_t)n * nsys].x += acc;
+        C &o = spec[(size_t)n * nsys];
+        o.x = typename CxT<C>::R(double(o.x) + acc);
     }
 }
 
 void launch_extract_mean(hipStream_t st, const double2 *zstate, int NZ, int nsys, const int *rowptr, const int *col,
                          const double *val, double2 *spec) {
-    hipLaunchKernelGGL(k_extract_mean, dim3(1), dim3(256), 0, st, zstate, NZ, nsys, rowptr, col, val, spec);
+    hipLaunchKernelGGL(k_extract_mean<double2>, dim3(1), dim3(256), 0, st, zstate, NZ, nsys, rowptr, col, val, spec);
+    CHECK_LAUNCH();
+}
+
+void launch_extract_mean32(hipStream_t st, const double2 *zstate, int NZ, int nsys, const int *rowptr, const int *col,
+                           const double *val, float2 *spec) {
+    hipLaunchKernelGGL(k_extract_mean<float2>, dim3(1), dim3(256), 0, st, zstate, NZ, nsys, rowptr, col, val, spec);
     CHECK_LAUNCH();
 }
 
@@ -572,6 +592,7 @@ static size_t xpass_lds(int LX, int N, int nfull) {
 
 void launch_xpass(hipStream_t st, const FFTPlan &px, const XPassArgs &a) {
     if (launch_xpass_wave(st, px, a)) return;
+    if (a.fp32) throw std::runtime_error("the FP32 x pass needs a wavefront plan (N = 96, 144, 192, 384, 768)");
     // lines per workgroup: FFT butterflies must fit fft_lds<3> (LX*N/2 <= 3*256)
     // and the workgroup's LDS must stay within 64 KiB
     int LX = 4;
@@ -665,21 +686,27 @@ void launch_xfwd(hipStream_t st, const FFTPlan &px, const double *phys, double2 
 }
 
 // remove_vertical_mean_of_quadratic_var (PL_IMEX_timestepping.f90:223-235)
-__global__ void k_remove_zmean(double2 *Qhat, int NZ, int nsys) {
+template <class C>
+__global__ void k_remove_zmean(C *Qhat, int NZ, int nsys) {
     int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= nsys) return;
     double2 acc = make_double2(0.0, 0.0);
     for (int k = 1; k <= NZ / 2 - 1; ++k) {
         double w = 0.5 / (2.0 * k + 1.0) - 0.5 / (2.0 * k - 1.0);
-        const double2 q = Qhat[(size_t)(2 * k) * nsys + s];
-        acc.x -= q.x * w;
-        acc.y -= q.y * w;
+        const C q = Qhat[(size_t)(2 * k) * nsys + s];
+        acc.x -= double(q.x) * w;
+        acc.y -= double(q.y) * w;
     }
-    Qhat[s] = acc;
+    Qhat[s] = mkc<C>(typename CxT<C>::R(acc.x), typename CxT<C>::R(acc.y));
 }
 
 void launch_remove_zmean(hipStream_t st, double2 *Qhat, int NZ, int nsys) {
-    hipLaunchKernelGGL(k_remove_zmean, dim3(blocks_for(nsys, 256)), dim3(256), 0, st, Qhat, NZ, nsys);
+    hipLaunchKernelGGL(k_remove_zmean<double2>, dim3(blocks_for(nsys, 256)), dim3(256), 0, st, Qhat, NZ, nsys);
+    CHECK_LAUNCH();
+}
+
+void launch_remove_zmean32(hipStream_t st, float2 *Qhat, int NZ, int nsys) {
+    hipLaunchKernelGGL(k_remove_zmean<float2>, dim3(blocks_for(nsys, 256)), dim3(256), 0, st, Qhat, NZ, nsys);
     CHECK_LAUNCH();
 }
 
@@ -768,7 +795,12 @@ __global__ __launch_bounds__(256) void k_nl(NLArgs a) {
 // CU (4 positions per wavefront, as few sequential programs as possible).
 constexpr int NLP_POS = 64, NLP_WAVES = 16, NLP_THREADS = 64 * NLP_WAVES;
 
-__global__ __launch_bounds__(NLP_THREADS) void k_nl_prog(NLProgArgs a) {
+__device__ inline double2 to_d2(double2 v) { return v; }
+__device__ inline double2 to_d2(float2 v) { return make_double2(double(v.x), double(v.y)); }
+
+// QT: element type of Qhat (double2, or float2 in the FP32 chain; the sums are FP64)
+template <class QT>
+__global__ __launch_bounds__(NLP_THREADS) void k_nl_prog(NLProgArgs a, const QT *__restrict__ Qsrc) {
     __shared__ double2 tile[NLP_POS][65];
     const int nsys = a.nsys, total = a.total;
     const int npb = (total + NLP_POS - 1) / NLP_POS;
@@ -778,7 +810,7 @@ __global__ __launch_bounds__(NLP_THREADS) void k_nl_prog(NLProgArgs a) {
     const int s = sb * 64 + lane;
     const bool sv = s < nsys;
     const int sc = sv ? s : 0;
-    const double2 *__restrict__ Q = a.Qhat + sc;
+    const QT *__restrict__ Q = Qsrc + sc;
     const int *__restrict__ pos_ptr = a.pos_ptr;
     const int *__restrict__ blk_ptr = a.blk_ptr;
     const int *__restrict__ blk_term = a.blk_term;
@@ -798,7 +830,7 @@ __global__ __launch_bounds__(NLP_THREADS) void k_nl_prog(NLProgArgs a) {
                 for (; e + 4 <= e1; e += 4) {
                     double2 q[4];
 #pragma unroll
-                    for (int u = 0; u < 4; ++u) q[u] = Q[(size_t)ent_qm[e + u] * nsys];
+                    for (int u = 0; u < 4; ++u) q[u] = to_d2(Q[(size_t)ent_qm[e + u] * nsys]);
 #pragma unroll
                     for (int u = 0; u < 4; ++u) {
                         sum.x = fma(ent_w[e + u], q[u].x, sum.x);
@@ -806,7 +838,7 @@ __global__ __launch_bounds__(NLP_THREADS) void k_nl_prog(NLProgArgs a) {
                     }
                 }
                 for (; e < e1; ++e) {
-                    const double2 q = Q[(size_t)ent_qm[e] * nsys];
+                    const double2 q = to_d2(Q[(size_t)ent_qm[e] * nsys]);
                     sum.x = fma(ent_w[e], q.x, sum.x);
                     sum.y = fma(ent_w[e], q.y, sum.y);
                 }
@@ -826,7 +858,12 @@ __global__ __launch_bounds__(NLP_THREADS) void k_nl_prog(NLProgArgs a) {
 
 void launch_nl_prog(hipStream_t st, const NLProgArgs &a) {
     const int npb = (a.total + NLP_POS - 1) / NLP_POS;
-    hipLaunchKernelGGL(k_nl_prog, dim3(blocks_for(a.nsys, 64) * npb), dim3(NLP_THREADS), 0, st, a);
+    if (a.Qhat32)
+        hipLaunchKernelGGL(k_nl_prog<float2>, dim3(blocks_for(a.nsys, 64) * npb), dim3(NLP_THREADS), 0, st, a,
+                           a.Qhat32);
+    else
+        hipLaunchKernelGGL(k_nl_prog<double2>, dim3(blocks_for(a.nsys, 64) * npb), dim3(NLP_THREADS), 0, st, a,
+                           a.Qhat);
     CHECK_LAUNCH();
 }
 
@@ -853,13 +890,14 @@ __global__ void k_nl_mean(NLArgs a) {
     double acc = 0.0;
     for (int t = a.eqn_begin[e]; t < a.eqn_begin[e + 1]; ++t) {
         const int w = a.t_w[t], m0 = r + a.t_order[t];
-        const double2 *Q = a.Qhat + a.t_qvar[t] * (size_t)a.NZ * a.nsys;
+        const size_t qoff = a.t_qvar[t] * (size_t)a.NZ * a.nsys;
         const double *W = a.wv + a.t_woff[t] + r * (2 * w + 1);
         double sum = 0.0;
         for (int d = -w; d <= w; ++d) {
             const int m = m0 + d;
             if (m < 0 || m >= a.NZ) continue;
-            sum += W[d + w] * Q[(size_t)m * a.nsys].x;
+            const size_t i = qoff + (size_t)m * a.nsys;
+            sum += W[d + w] * (a.Qhat32 ? double(a.Qhat32[i].x) : a.Qhat[i].x);
         }
         acc += a.t_dsca[t] * sum;
     }
@@ -867,6 +905,7 @@ __global__ void k_nl_mean(NLArgs a) {
 }
 
 void launch_nl(hipStream_t st, const NLArgs &a) {
+    if (a.Qhat32 && !a.mean) throw std::runtime_error("FP32 Qhat: the kxky NL terms run on their per-position program");
     if (a.mean) {
         hipLaunchKernelGGL(k_nl_mean, dim3(blocks_for(a.total, 256)), dim3(256), 0, st, a);
     } else {

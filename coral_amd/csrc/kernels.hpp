@@ -33,6 +33,7 @@ struct ExtractAllArgs {
     int nsys, NZ;
     KeptModes km;
     double2 *spec;                          // [nfull][NZ][nsys]
+    float2 *spec32;                         // when set, the output goes here in FP32 instead
     // distinct state arrays read by the terms ((total, NY, NX/2) each)
     int nstate;
     const double2 *state[MAX_EXTRACT_STATES];
@@ -42,6 +43,8 @@ void launch_extract_all(hipStream_t st, const ExtractAllArgs &a);
 // mean contributions into the (0,0) column: spec[n][0] += sum val[k] * Re(zstate[col[k]])
 void launch_extract_mean(hipStream_t st, const double2 *zstate, int NZ, int nsys, const int *rowptr, const int *col,
                          const double *val, double2 *spec);
+void launch_extract_mean32(hipStream_t st, const double2 *zstate, int NZ, int nsys, const int *rowptr, const int *col,
+                           const double *val, float2 *spec);
 
 // ---- z transforms ----
 // Layouts (z-major): spec[b][n][s], Fz[b][z][s], Hy[b][z][s], Qhat[b][n][s] (complex s)
@@ -83,11 +86,37 @@ bool launch_zbwd_mfma(hipStream_t st, const ZFragsDev &z, const double2 *spec, d
 bool launch_zfwd_mfma(hipStream_t st, const ZFragsDev &z, const double2 *Hy, double2 *Qhat, int NZ, int NZAA,
                       int nsys, int nb);
 
+// FP32 z transforms on MFMA with compensated accumulation (ztransform32.hip):
+// the same folded products in single precision; fragments are float
+struct ZFrags32 {
+    int TE = 0, TO = 0, NEK = 0, NOK = 0, RE = 0, RO = 0, KEK = 0, KOK = 0;
+    bool bwd_ok = false, fwd_ok = false;
+    int nbp = 0, nfp = 0;
+    ZPart bp[ZMAXPARTS], fp[ZMAXPARTS];
+    std::vector<float> bwd, fwd;
+};
+struct ZFrags32Dev {
+    int TE, TO, NEK, NOK, RE, RO, KEK, KOK;
+    bool bwd_ok, fwd_ok;
+    int ncu;
+    const float *bwd, *fwd;
+    int nbp, nfp;
+    ZPart bp[ZMAXPARTS], fp[ZMAXPARTS];
+};
+// false when no compiled shape covers NZ (both directions needed)
+bool make_zfrags32(int NZ, int NZAA, const std::vector<double> &Tz, const std::vector<double> &Cz, size_t lds_limit,
+                   ZFrags32 &zf);
+void launch_zbwd32(hipStream_t st, const ZFrags32Dev &z, const float2 *spec, float2 *Fz, int NZ, int NZAA, int nsys,
+                   int nb);
+void launch_zfwd32(hipStream_t st, const ZFrags32Dev &z, const float2 *Hy, float2 *Qhat, int NZ, int NZAA, int nsys,
+                   int nb);
+
 // ---- FFT passes ----
 struct FFTPlan {
     int N = 0, nrad = 0;
     int radix[16];
     const double2 *tw = nullptr;  // device, N entries exp(-2 pi i k / N)
+    const float2 *twf = nullptr;  // the same rounded to FP32 (FP32 transform chain)
 };
 // y inverse: G[b][z][y][ix] = sum_jy Fz[b][z][ix*nky+jy] e^{+i ky y}
 void launch_ybwd(hipStream_t st, const FFTPlan &py, const double2 *Fz, double2 *G, int nbatch, int NZAA,
@@ -96,6 +125,7 @@ void launch_ybwd(hipStream_t st, const FFTPlan &py, const double2 *Fz, double2 *
 struct XPassArgs {
     const double2 *G;    // [nfull][NZAA][NYAA][nkx]
     double2 *H;          // [nquad][NZAA][NYAA][nkx]
+    int fp32;            // G and H hold float2 (the FP32 chain; wavefront plans only)
     int nfull, nquad, NZAA, NYAA, nkx;
     const int *qa, *qb;  // quadratic factors (-1 = ONE)
     const double *zweight;
@@ -108,6 +138,10 @@ bool launch_xpass_wave(hipStream_t st, const FFTPlan &px, const XPassArgs &a);
 // wavefront-FFT y passes for NYAA in {96, 144, 192, 384, 768}; false (nothing launched) otherwise
 bool launch_ypass_wave(hipStream_t st, bool fwd, const FFTPlan &py, const double2 *in, double2 *out, int nbatch,
                        int NZAA, int nkx, int nky, int NY);
+// the same passes on float2 arrays (the FP32 chain; twiddles py.twf), forward
+// or inverse; throws when no wavefront plan covers NYAA
+void launch_ypass32(hipStream_t st, bool fwd, const FFTPlan &py, const float2 *in, float2 *out, int nbatch, int NZAA,
+                    int nkx, int nky, int NY);
 // y forward: Hy[b][z][ix*nky+jy] = sum_y H[b][z][y][ix] e^{-i ky y}
 void launch_yfwd(hipStream_t st, const FFTPlan &py, const double2 *H, double2 *Hy, int nbatch, int NZAA,
                  int nkx, int nky, int NY);
@@ -121,10 +155,12 @@ void launch_insert_var(hipStream_t st, const double2 *Q, int nsys, int ncol, con
                        double2 *field, int total);
 // remove the z-integral of Qhat[s][:] (PL_IMEX_timestepping.f90:223-235)
 void launch_remove_zmean(hipStream_t st, double2 *Qhat, int NZ, int nsys);
+void launch_remove_zmean32(hipStream_t st, float2 *Qhat, int NZ, int nsys);
 
 // ---- NL assembly into K_hat ----
 struct NLArgs {
     const double2 *Qhat;      // [nquad][NZ][nsys]
+    const float2 *Qhat32;     // when set, Qhat is read from this FP32 copy (mean systems only)
     int nsys, NZ, total, mean;
     const int *pos_eqn, *pos_row, *eqn_begin;
     const int *t_qvar, *t_kx, *t_ky, *t_w, *t_order, *t_woff;
@@ -144,6 +180,7 @@ struct NLProgArgs {
     const double2 *cmono;  // [nterms][nsys]
     int nterms;
     const double2 *Qhat;   // [nquad][NZ][nsys]
+    const float2 *Qhat32;  // when set, Qhat is read from this FP32 copy (sums stay FP64)
     int nsys, total;
     double2 *Khat;         // [nsys][total]
 };

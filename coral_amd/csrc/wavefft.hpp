@@ -32,19 +32,21 @@ __device__ inline void wave_sync() {
 }
 
 // z * (SIGN i)
-template <int SIGN>
-__device__ inline double2 mul_si(double2 z) {
-    return SIGN > 0 ? make_double2(-z.y, z.x) : make_double2(z.y, -z.x);
+template <int SIGN, class C>
+__device__ inline C mul_si(C z) {
+    return SIGN > 0 ? mkc<C>(-z.y, z.x) : mkc<C>(z.y, -z.x);
 }
 
-// R-point DFT in registers: X[k] = sum_n x[n] exp(SIGN 2 pi i n k / R)
+// R-point DFT in registers: X[k] = sum_n x[n] exp(SIGN 2 pi i n k / R), on
+// double2 or float2 elements (constants in the element's precision)
 template <int R, int SIGN>
 struct Dft;
 
 template <int SIGN>
 struct Dft<2, SIGN> {
-    __device__ static inline void run(double2 *v) {
-        const double2 a = v[0], b = v[1];
+    template <class C>
+    __device__ static inline void run(C *v) {
+        const C a = v[0], b = v[1];
         v[0] = cadd(a, b);
         v[1] = csub(a, b);
     }
@@ -52,21 +54,24 @@ struct Dft<2, SIGN> {
 
 template <int SIGN>
 struct Dft<3, SIGN> {
-    __device__ static inline void run(double2 *v) {
-        const double s1 = SIGN * 0.86602540378443864676;  // sin(2 pi / 3)
-        const double2 a = v[0], t = cadd(v[1], v[2]), d = csub(v[1], v[2]);
-        const double2 m = make_double2(a.x - 0.5 * t.x, a.y - 0.5 * t.y);
+    template <class C>
+    __device__ static inline void run(C *v) {
+        using T = typename CxT<C>::R;
+        const T s1 = T(SIGN * 0.86602540378443864676), h = T(0.5);  // sin(2 pi / 3)
+        const C a = v[0], t = cadd(v[1], v[2]), d = csub(v[1], v[2]);
+        const C m = mkc<C>(a.x - h * t.x, a.y - h * t.y);
         v[0] = cadd(a, t);
-        v[1] = make_double2(m.x - s1 * d.y, m.y + s1 * d.x);
-        v[2] = make_double2(m.x + s1 * d.y, m.y - s1 * d.x);
+        v[1] = mkc<C>(m.x - s1 * d.y, m.y + s1 * d.x);
+        v[2] = mkc<C>(m.x + s1 * d.y, m.y - s1 * d.x);
     }
 };
 
 template <int SIGN>
 struct Dft<4, SIGN> {
-    __device__ static inline void run(double2 *v) {
-        const double2 t0 = cadd(v[0], v[2]), t1 = csub(v[0], v[2]);
-        const double2 t2 = cadd(v[1], v[3]), t3 = mul_si<SIGN>(csub(v[1], v[3]));
+    template <class C>
+    __device__ static inline void run(C *v) {
+        const C t0 = cadd(v[0], v[2]), t1 = csub(v[0], v[2]);
+        const C t2 = cadd(v[1], v[3]), t3 = mul_si<SIGN>(csub(v[1], v[3]));
         v[0] = cadd(t0, t2);
         v[2] = csub(t0, t2);
         v[1] = cadd(t1, t3);
@@ -77,13 +82,15 @@ struct Dft<4, SIGN> {
 // 6 = 2 x 3: X[k] = E[k%3] + w^k O[k%3], X[k+3] = E[k] - w^k O[k], w = exp(SIGN pi i / 3)
 template <int SIGN>
 struct Dft<6, SIGN> {
-    __device__ static inline void run(double2 *v) {
-        double2 e[3] = {v[0], v[2], v[4]}, o[3] = {v[1], v[3], v[5]};
+    template <class C>
+    __device__ static inline void run(C *v) {
+        using T = typename CxT<C>::R;
+        C e[3] = {v[0], v[2], v[4]}, o[3] = {v[1], v[3], v[5]};
         Dft<3, SIGN>::run(e);
         Dft<3, SIGN>::run(o);
-        const double s = SIGN * 0.86602540378443864676;
-        const double2 o1 = make_double2(0.5 * o[1].x - s * o[1].y, 0.5 * o[1].y + s * o[1].x);
-        const double2 o2 = make_double2(-0.5 * o[2].x - s * o[2].y, -0.5 * o[2].y + s * o[2].x);
+        const T s = T(SIGN * 0.86602540378443864676), h = T(0.5);
+        const C o1 = mkc<C>(h * o[1].x - s * o[1].y, h * o[1].y + s * o[1].x);
+        const C o2 = mkc<C>(-h * o[2].x - s * o[2].y, -h * o[2].y + s * o[2].x);
         v[0] = cadd(e[0], o[0]);
         v[3] = csub(e[0], o[0]);
         v[1] = cadd(e[1], o1);
@@ -96,14 +103,16 @@ struct Dft<6, SIGN> {
 // 8 = 2 x 4: X[k] = E[k] + w^k O[k], X[k+4] = E[k] - w^k O[k], w = exp(SIGN pi i / 4)
 template <int SIGN>
 struct Dft<8, SIGN> {
-    __device__ static inline void run(double2 *v) {
-        double2 e[4] = {v[0], v[2], v[4], v[6]}, o[4] = {v[1], v[3], v[5], v[7]};
+    template <class C>
+    __device__ static inline void run(C *v) {
+        using T = typename CxT<C>::R;
+        C e[4] = {v[0], v[2], v[4], v[6]}, o[4] = {v[1], v[3], v[5], v[7]};
         Dft<4, SIGN>::run(e);
         Dft<4, SIGN>::run(o);
-        const double h = 0.70710678118654752440;
-        const double2 o1 = make_double2(h * (o[1].x - SIGN * o[1].y), h * (o[1].y + SIGN * o[1].x));
-        const double2 o2 = mul_si<SIGN>(o[2]);
-        const double2 o3 = make_double2(-h * (o[3].x + SIGN * o[3].y), h * (SIGN * o[3].x - o[3].y));
+        const T h = T(0.70710678118654752440), sg = T(SIGN);
+        const C o1 = mkc<C>(h * (o[1].x - sg * o[1].y), h * (o[1].y + sg * o[1].x));
+        const C o2 = mul_si<SIGN>(o[2]);
+        const C o3 = mkc<C>(-h * (o[3].x + sg * o[3].y), h * (sg * o[3].x - o[3].y));
         v[0] = cadd(e[0], o[0]);
         v[4] = csub(e[0], o[0]);
         v[1] = cadd(e[1], o1);
@@ -116,18 +125,18 @@ struct Dft<8, SIGN> {
 };
 
 // exp(SIGN 2 pi i m / N) from the table tw[k] = exp(-2 pi i k / N)
-template <int SIGN>
-__device__ inline double2 twiddle(const double2 *__restrict__ tw, int m) {
-    const double2 w = tw[m];
-    return SIGN > 0 ? make_double2(w.x, -w.y) : w;
+template <int SIGN, class C>
+__device__ inline C twiddle(const C *__restrict__ tw, int m) {
+    const C w = tw[m];
+    return SIGN > 0 ? mkc<C>(w.x, -w.y) : w;
 }
 
 // p[r] = exp(SIGN 2 pi i r m / N) for r = 1..R-1 (r m < N): one table load,
 // the other powers by a chain of products (each adds an ulp or so).  Loads of
 // more powers were measured slower: the twiddle loads sit at the head of
 // every stage.
-template <int R, int SIGN>
-__device__ inline void twiddle_powers(const double2 *__restrict__ tw, int m, double2 (&p)[R]) {
+template <int R, int SIGN, class C>
+__device__ inline void twiddle_powers(const C *__restrict__ tw, int m, C (&p)[R]) {
     p[1] = twiddle<SIGN>(tw, m);
 #pragma unroll
     for (int r = 2; r < R; ++r) p[r] = cmul(p[r - 1], p[1]);
@@ -162,26 +171,26 @@ constexpr int stage_swz() { return (NS == 1 && R == 4) ? 1 : (NS == 4 && R == 4)
 // S*T butterflies are spread over the 64 lanes round by round.  The input was
 // written with swizzle WI, the output is written with stage_swz<R, NS>() when
 // SW is set, unswizzled otherwise.
-template <int N, int R, int NS, int SIGN, int S, int LS, int WI = 0, bool SW = false>
-__device__ inline void stage_lds(double2 *buf, const double2 *__restrict__ tw, int lane) {
+template <int N, int R, int NS, int SIGN, int S, int LS, int WI = 0, bool SW = false, class C>
+__device__ inline void stage_lds(C *buf, const C *__restrict__ tw, int lane) {
     constexpr int WO = SW ? stage_swz<R, NS>() : 0;
     constexpr int T = N / R;
     constexpr int TOT = S * T;
     constexpr int ROUNDS = (TOT + 63) / 64;
     static_assert(N % R == 0, "radix must divide the length");
-    double2 v[ROUNDS][R];
+    C v[ROUNDS][R];
 #pragma unroll
     for (int c = 0; c < ROUNDS; ++c) {
         const int b = lane + 64 * c;
         if (TOT % 64 == 0 || b < TOT) {
             const int s = b / T, j = b - s * T;
-            const double2 *L = buf + s * LS;
+            const C *L = buf + s * LS;
             const int jm = j % NS;
-            double2 wp[R];
+            C wp[R];
             if constexpr (NS > 1) twiddle_powers<R, SIGN>(tw, jm * (N / (NS * R)), wp);
 #pragma unroll
             for (int r = 0; r < R; ++r) {
-                double2 x = L[swz<WI>(j + r * T)];
+                C x = L[swz<WI>(j + r * T)];
                 if (NS > 1 && r > 0) x = cmul(x, wp[r]);
                 v[c][r] = x;
             }
@@ -194,7 +203,7 @@ __device__ inline void stage_lds(double2 *buf, const double2 *__restrict__ tw, i
         const int b = lane + 64 * c;
         if (TOT % 64 == 0 || b < TOT) {
             const int s = b / T, j = b - s * T;
-            double2 *L = buf + s * LS;
+            C *L = buf + s * LS;
             const int base = (j / NS) * NS * R + j % NS;
 #pragma unroll
             for (int r = 0; r < R; ++r) L[swz<WO>(base + r * NS)] = v[c][r];
@@ -213,14 +222,16 @@ struct StagesLSW;
 template <int N, int SIGN, int S, int LS, int WI, bool SW, int NS>
 struct StagesLSW<N, SIGN, S, LS, WI, SW, NS> {
     static constexpr int OUT_SWZ = WI;
-    __device__ static inline void run(double2 *, const double2 *__restrict__, int) {}
+    template <class C>
+    __device__ static inline void run(C *, const C *__restrict__, int) {}
 };
 
 template <int N, int SIGN, int S, int LS, int WI, bool SW, int NS, int R, int... Rs>
 struct StagesLSW<N, SIGN, S, LS, WI, SW, NS, R, Rs...> {
     using Next = StagesLSW<N, SIGN, S, LS, (SW ? stage_swz<R, NS>() : 0), SW, NS * R, Rs...>;
     static constexpr int OUT_SWZ = Next::OUT_SWZ;
-    __device__ static inline void run(double2 *buf, const double2 *__restrict__ tw, int lane) {
+    template <class C>
+    __device__ static inline void run(C *buf, const C *__restrict__ tw, int lane) {
         stage_lds<N, R, NS, SIGN, S, LS, WI, SW>(buf, tw, lane);
         Next::run(buf, tw, lane);
     }

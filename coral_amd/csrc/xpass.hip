@@ -76,10 +76,10 @@ struct XShape {
 };
 
 // value of full variable A (1 for A < 0) at point r of a task
-template <int A, int NPI, int RL>
-__device__ inline double fget(const double2 (&f)[NPI][RL], int r) {
+template <int A, int NPI, int RL, class C>
+__device__ inline typename CxT<C>::R fget(const C (&f)[NPI][RL], int r) {
     if constexpr (A < 0)
-        return 1.0;
+        return 1;
     else
         return (A & 1) ? f[A >> 1][r].y : f[A >> 1][r].x;
 }
@@ -87,8 +87,8 @@ __device__ inline double fget(const double2 (&f)[NPI][RL], int r) {
 // o[r] = F_a at the RL points of a task.  a is the same for the whole
 // wavefront: a scalar branch picks the copy (the empty asm keeps the branches
 // from being turned into per-element selects, one v_cndmask per candidate).
-template <int NPI, int RL, int K = 0>
-__device__ inline void pick(const double2 (&f)[NPI][RL], int a, double (&o)[RL]) {
+template <int NPI, int RL, int K = 0, class C>
+__device__ inline void pick(const C (&f)[NPI][RL], int a, typename CxT<C>::R (&o)[RL]) {
     if constexpr (K < 2 * NPI) {
         if (a == K) {
             asm volatile("");
@@ -99,13 +99,13 @@ __device__ inline void pick(const double2 (&f)[NPI][RL], int a, double (&o)[RL])
         pick<NPI, RL, K + 1>(f, a, o);
     } else {
 #pragma unroll
-        for (int r = 0; r < RL; ++r) o[r] = 1.0;   // a < 0: the constant ONE
+        for (int r = 0; r < RL; ++r) o[r] = 1;   // a < 0: the constant ONE
     }
 }
 
-template <int NPI, int RL>
-__device__ inline void product(const double2 (&f)[NPI][RL], int a, int b, double (&p)[RL]) {
-    double fa[RL], fb[RL];
+template <int NPI, int RL, class C>
+__device__ inline void product(const C (&f)[NPI][RL], int a, int b, typename CxT<C>::R (&p)[RL]) {
+    typename CxT<C>::R fa[RL], fb[RL];
     pick<NPI, RL>(f, a, fa);
     pick<NPI, RL>(f, b, fb);
 #pragma unroll
@@ -113,11 +113,13 @@ __device__ inline void product(const double2 (&f)[NPI][RL], int a, int b, double
 }
 
 // forward transforms of product pairs [pq0, pq0 + G) and the store of the kept kx
-template <int N, int NPI, int XGP, int G>
-__device__ inline void xfwd_batch(const XPassArgs &a, const double2 (&fv)[XShape<N, NPI, XGP>::NT][NPI][XPlan<N>::RL],
-                                  double2 *buf, const double2 *__restrict__ tw, int lane, long long line0,
-                                  long long nlines, int pq0) {
+template <int N, int NPI, int XGP, int G, class C>
+__device__ inline void xfwd_batch(const XPassArgs &a, const C (&fv)[XShape<N, NPI, XGP>::NT][NPI][XPlan<N>::RL],
+                                  C *buf, const C *__restrict__ tw, int lane, long long line0, long long nlines,
+                                  int pq0) {
     using X = XShape<N, NPI, XGP>;
+    using T = typename CxT<C>::R;
+    C *H = reinterpret_cast<C *>(a.H);
     constexpr int RL = X::RL, TL = X::TL, LPW = X::LPW;
 #pragma unroll
     for (int c = 0; c < X::NT; ++c) {
@@ -129,18 +131,18 @@ __device__ inline void xfwd_batch(const XPassArgs &a, const double2 (&fv)[XShape
             const int a0 = __builtin_amdgcn_readfirstlane(a.qa[q0]), b0 = __builtin_amdgcn_readfirstlane(a.qb[q0]);
             const int a1 = has1 ? __builtin_amdgcn_readfirstlane(a.qa[q1]) : -1;
             const int b1 = has1 ? __builtin_amdgcn_readfirstlane(a.qb[q1]) : -1;
-            double p0[RL], p1[RL];
+            T p0[RL], p1[RL];
             product<NPI, RL>(fv[c], a0, b0, p0);
             if (has1)
                 product<NPI, RL>(fv[c], a1, b1, p1);
             else
 #pragma unroll
-                for (int r = 0; r < RL; ++r) p1[r] = 0.0;
-            double2 y[RL];
+                for (int r = 0; r < RL; ++r) p1[r] = 0;
+            C y[RL];
 #pragma unroll
-            for (int r = 0; r < RL; ++r) y[r] = make_double2(p0[r], p1[r]);
+            for (int r = 0; r < RL; ++r) y[r] = mkc<C>(p0[r], p1[r]);
             Dft<RL, -1>::run(y);
-            double2 *L = buf + (l * G + gp) * N;
+            C *L = buf + (l * G + gp) * N;
 #pragma unroll
             for (int r = 0; r < RL; ++r) L[wfft::swz<XPlan<N>::WL>(j * RL + r)] = y[r];
         }
@@ -155,14 +157,15 @@ __device__ inline void xfwd_batch(const XPassArgs &a, const double2 (&fv)[XShape
 #pragma unroll
             for (int gp = 0; gp < G; ++gp) {
                 const int q0 = 2 * (pq0 + gp), q1 = q0 + 1;
-                const double2 *L = buf + (l * G + gp) * N;
-                double2 *h0 = a.H + ((size_t)q0 * nlines + line) * nkx;
-                double2 *h1 = a.H + ((size_t)q1 * nlines + line) * nkx;
+                const C *L = buf + (l * G + gp) * N;
+                C *h0 = H + ((size_t)q0 * nlines + line) * nkx;
+                C *h1 = H + ((size_t)q1 * nlines + line) * nkx;
+                const T hf = T(0.5);
                 for (int k = lane; k < nkx; k += 64) {
-                    const double2 y = L[k], yc = cconj(L[k == 0 ? 0 : N - k]);
+                    const C y = L[k], yc = cconj(L[k == 0 ? 0 : N - k]);
                     // p0 = (Y + conj Y(-k)) / 2,  p1 = -i (Y - conj Y(-k)) / 2
-                    h0[k] = make_double2(0.5 * (y.x + yc.x), 0.5 * (y.y + yc.y));
-                    if (q1 < a.nquad) h1[k] = make_double2(0.5 * (y.y - yc.y), -0.5 * (y.x - yc.x));
+                    h0[k] = mkc<C>(hf * (y.x + yc.x), hf * (y.y + yc.y));
+                    if (q1 < a.nquad) h1[k] = mkc<C>(hf * (y.y - yc.y), -hf * (y.x - yc.x));
                 }
             }
         }
@@ -172,27 +175,31 @@ __device__ inline void xfwd_batch(const XPassArgs &a, const double2 (&fv)[XShape
 
 // NK: the kept kx count when it is known at compile time (N/3, the 3/2 rule),
 // 0 when it is read from the arguments
-template <int N, int NPI, int XGP, int NK>
-__global__ __launch_bounds__(256) void k_xpass_wave(XPassArgs a, const double2 *__restrict__ tw, long long nlines) {
+// C: element type of G, H and the FFTs (double2, or float2 for the FP32 chain)
+template <int N, int NPI, int XGP, int NK, class C>
+__global__ __launch_bounds__(256) void k_xpass_wave(XPassArgs a, const C *__restrict__ tw, long long nlines) {
     using X = XShape<N, NPI, XGP>;
+    using T = typename CxT<C>::R;
     constexpr int R0 = X::R0, RL = X::RL, T0 = X::T0, TL = X::TL, LPW = X::LPW, SIN = X::SIN;
-    extern __shared__ double2 lds[];
+    extern __shared__ unsigned char lds_raw[];
+    C *lds = reinterpret_cast<C *>(lds_raw);
     __shared__ double cfl_part[4][3];
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    double2 *buf = lds + wid * X::SMAX * N;
+    C *buf = lds + wid * X::SMAX * N;
+    const C *G = reinterpret_cast<const C *>(a.G);
     const long long line0 = ((long long)blockIdx.x * (blockDim.x >> 6) + wid) * LPW;
     const int nkx = NK > 0 ? NK : a.nkx;
 
     // ---- inverse transforms, one packed pair of full variables at a time
     // (half the LDS of both pairs at once, which buys a wavefront per SIMD);
     // the last stage leaves fv[task][pair][r] at x = j + r*TL in registers ----
-    double2 fv[X::NT][NPI][RL];
+    C fv[X::NT][NPI][RL];
 #pragma unroll
     for (int p = 0; p < NPI; ++p) {
         // first stage: packed Hermitian spectra straight from HBM
         {
             constexpr int TOT = SIN * T0, ROUNDS = (TOT + 63) / 64;
-            double2 v[ROUNDS][R0];
+            C v[ROUNDS][R0];
 #pragma unroll
             for (int c = 0; c < ROUNDS; ++c) {
                 const int b = lane + 64 * c;
@@ -203,8 +210,8 @@ __global__ __launch_bounds__(256) void k_xpass_wave(XPassArgs a, const double2 *
                     const int f0 = 2 * p, f1 = 2 * p + 1;
                     const bool has1 = f1 < a.nfull;
                     const size_t lo = valid ? (size_t)line : 0;
-                    const double2 *g0 = a.G + ((size_t)f0 * nlines + lo) * nkx;
-                    const double2 *g1 = a.G + ((size_t)(has1 ? f1 : f0) * nlines + lo) * nkx;
+                    const C *g0 = G + ((size_t)f0 * nlines + lo) * nkx;
+                    const C *g1 = G + ((size_t)(has1 ? f1 : f0) * nlines + lo) * nkx;
 #pragma unroll
                     for (int r = 0; r < R0; ++r) {
                         const int e = j + r * T0;
@@ -216,21 +223,21 @@ __global__ __launch_bounds__(256) void k_xpass_wave(XPassArgs a, const double2 *
                         const bool all_neg = NK > 0 && lo > N - NK;
                         const bool all_zero = NK > 0 && lo >= NK && hi <= N - NK;
                         if (all_zero) {
-                            v[c][r] = make_double2(0.0, 0.0);
+                            v[c][r] = mkc<C>(0, 0);
                             continue;
                         }
                         const bool pos = all_pos || (!all_neg && e < nkx);
                         const bool neg = all_neg || (!all_pos && e > N - nkx);
                         const int k = pos ? e : (neg ? N - e : 0);
-                        double2 A = g0[k], B = g1[k];
+                        C A = g0[k], B = g1[k];
                         const bool e0 = r == 0 && j == 0;   // kx = 0: imaginary parts dropped
-                        const double bx = has1 ? B.x : 0.0, by = (has1 && !e0) ? B.y : 0.0;
-                        const double ay = !e0 ? A.y : 0.0;
+                        const T bx = has1 ? B.x : T(0), by = (has1 && !e0) ? B.y : T(0);
+                        const T ay = !e0 ? A.y : T(0);
                         // pos: A + iB;  neg: conj(A) + i conj(B)
-                        double zx = pos ? A.x - by : A.x + by;
-                        double zy = pos ? ay + bx : bx - ay;
+                        T zx = pos ? A.x - by : A.x + by;
+                        T zy = pos ? ay + bx : bx - ay;
                         const bool keep = valid && (all_pos || all_neg || pos || neg);
-                        v[c][r] = make_double2(keep ? zx : 0.0, keep ? zy : 0.0);
+                        v[c][r] = mkc<C>(keep ? zx : T(0), keep ? zy : T(0));
                     }
                     Dft<R0, +1>::run(v[c]);
                 }
@@ -251,12 +258,12 @@ __global__ __launch_bounds__(256) void k_xpass_wave(XPassArgs a, const double2 *
 #pragma unroll
         for (int c = 0; c < X::NT; ++c) {
             const int t = lane + 64 * c, l = t / TL, j = t - l * TL;
-            const double2 *L = buf + l * N;
-            double2 wp[RL];
+            const C *L = buf + l * N;
+            C wp[RL];
             wfft::twiddle_powers<RL, +1>(tw, j, wp);
 #pragma unroll
             for (int r = 0; r < RL; ++r) {
-                double2 x = L[j + r * TL];
+                C x = L[j + r * TL];
                 if (r > 0) x = cmul(x, wp[r]);
                 fv[c][p][r] = x;
             }
@@ -279,9 +286,9 @@ __global__ __launch_bounds__(256) void k_xpass_wave(XPassArgs a, const double2 *
                 double wm = 0.0;
 #pragma unroll
                 for (int r = 0; r < RL; ++r) {
-                    mu = fmax(mu, fabs(fv[c][0][r].x));
-                    mv = fmax(mv, fabs(fv[c][0][r].y));
-                    wm = fmax(wm, fabs(fv[c][1][r].x));
+                    mu = fmax(mu, fabs(double(fv[c][0][r].x)));
+                    mv = fmax(mv, fabs(double(fv[c][0][r].y)));
+                    wm = fmax(wm, fabs(double(fv[c][1][r].x)));
                 }
                 mw = fmax(mw, wm / a.zweight[line / a.NYAA]);
             }
@@ -315,32 +322,42 @@ __global__ __launch_bounds__(256) void k_xpass_wave(XPassArgs a, const double2 *
     }
 }
 
-template <int N, int NPI, int XGP, int NK>
-static bool launch_wave(hipStream_t st, const FFTPlan &px, const XPassArgs &a) {
+template <int N, int NPI, int XGP, int NK, class C>
+static bool launch_wave(hipStream_t st, const C *tw, const XPassArgs &a) {
     using X = XShape<N, NPI, XGP>;
     const long long nlines = (long long)a.NZAA * a.NYAA;
-    const size_t per_wave = size_t(X::SMAX) * N * sizeof(double2);
+    const size_t per_wave = size_t(X::SMAX) * N * sizeof(C);
     int wpb = 4;
     while (wpb > 1 && wpb * per_wave > 65536) wpb /= 2;
     if (wpb * per_wave > 65536) return false;
     const long long per_block = (long long)wpb * X::LPW;
     const unsigned blocks = unsigned((nlines + per_block - 1) / per_block);
-    hipLaunchKernelGGL((k_xpass_wave<N, NPI, XGP, NK>), dim3(blocks), dim3(64 * wpb), wpb * per_wave, st, a, px.tw, nlines);
+    hipLaunchKernelGGL((k_xpass_wave<N, NPI, XGP, NK, C>), dim3(blocks), dim3(64 * wpb), wpb * per_wave, st, a, tw,
+                       nlines);
     check_launch("x pass");
     return true;
 }
 
-template <int N>
-static bool launch_wave_n(hipStream_t st, const FFTPlan &px, const XPassArgs &a) {
+template <int N, class C>
+static bool launch_wave_c(hipStream_t st, const C *tw, const XPassArgs &a) {
     // one product pair per forward batch: two pairs would fill the lanes of
     // the forward stages better, but their registers cost a wave per SIMD
     const bool third = 3 * a.nkx == N;
     switch ((a.nfull + 1) / 2) {
-    case 1: return third ? launch_wave<N, 1, 1, N / 3>(st, px, a) : launch_wave<N, 1, 1, 0>(st, px, a);
-    case 2: return third ? launch_wave<N, 2, 1, N / 3>(st, px, a) : launch_wave<N, 2, 1, 0>(st, px, a);
-    case 3: return third ? launch_wave<N, 3, 1, N / 3>(st, px, a) : launch_wave<N, 3, 1, 0>(st, px, a);
+    case 1: return third ? launch_wave<N, 1, 1, N / 3>(st, tw, a) : launch_wave<N, 1, 1, 0>(st, tw, a);
+    case 2: return third ? launch_wave<N, 2, 1, N / 3>(st, tw, a) : launch_wave<N, 2, 1, 0>(st, tw, a);
+    case 3: return third ? launch_wave<N, 3, 1, N / 3>(st, tw, a) : launch_wave<N, 3, 1, 0>(st, tw, a);
     default: return false;
     }
+}
+
+template <int N>
+static bool launch_wave_n(hipStream_t st, const FFTPlan &px, const XPassArgs &a) {
+    if (a.fp32) {
+        if (!px.twf) return false;
+        return launch_wave_c<N>(st, px.twf, a);
+    }
+    return launch_wave_c<N>(st, px.tw, a);
 }
 
 bool launch_xpass_wave(hipStream_t st, const FFTPlan &px, const XPassArgs &a) {

@@ -16,7 +16,6 @@ namespace coral {
 using wfft::StagesLSZ;
 using wfft::swz;
 
-constexpr int YL = 8;        // kx columns per workgroup
 constexpr int YWAVES = 4;    // wavefronts per workgroup (YL / YWAVES lines each)
 constexpr int YT = 64 * YWAVES;   // threads per workgroup
 
@@ -46,12 +45,18 @@ struct YPlan<768> {
 // kept ky index jy -> slot of the NYAA-long line
 __device__ inline int ky_slot(int jy, int NY, int N) { return jy < NY / 2 ? jy : N - (NY - 1) + jy; }
 
+// kx columns per workgroup for element type C: one 128-byte segment
+template <class C>
+constexpr int ycols() { return int(128 / sizeof(C)); }
+
 // G[b][z][y][ix] = sum_jy Fz[b][z][ix*nky + jy] exp(+i ky y)
-template <int N>
-__global__ __launch_bounds__(64 * YWAVES) void k_ybwd_w(const double2 *__restrict__ Fz, double2 *__restrict__ G,
-                                                       const double2 *__restrict__ tw, int nkx, int nky, int NY) {
+template <int N, class C>
+__global__ __launch_bounds__(64 * YWAVES) void k_ybwd_w(const C *__restrict__ Fz, C *__restrict__ G,
+                                                       const C *__restrict__ tw, int nkx, int nky, int NY) {
+    constexpr int YL = ycols<C>();
     constexpr int LS = N + 1, LPW = YL / YWAVES;
-    extern __shared__ double2 buf[];   // YL * LS (beyond 64 KiB for N = 768)
+    extern __shared__ unsigned char buf_raw[];   // YL * LS elements (beyond 64 KiB for N = 768)
+    C *buf = reinterpret_cast<C *>(buf_raw);
     const int ntile = (nkx + YL - 1) / YL;
     const int tile = blockIdx.x % ntile;
     const size_t bz = blockIdx.x / ntile;
@@ -60,19 +65,19 @@ __global__ __launch_bounds__(64 * YWAVES) void k_ybwd_w(const double2 *__restric
     // dealiased slots are zero
     for (int e = threadIdx.x; e < YL * (N - nky); e += blockDim.x) {
         const int l = e / (N - nky), k = e - l * (N - nky);
-        buf[l * LS + NY / 2 + k] = make_double2(0.0, 0.0);
+        buf[l * LS + NY / 2 + k] = mkc<C>(0, 0);
     }
     // all of a thread's loads are issued before the first LDS store (a loop
     // of load, wait, store would make them one HBM round trip each)
-    const double2 *src = Fz + bz * nsys + (size_t)ix0 * nky;
+    const C *src = Fz + bz * nsys + (size_t)ix0 * nky;
     {
         constexpr int PER = (YL * N + YT - 1) / YT;   // nky <= N
         const int cnt = nl * nky;
-        double2 v[PER];
+        C v[PER];
 #pragma unroll
         for (int i = 0; i < PER; ++i) {
             const int e = threadIdx.x + i * YT;
-            v[i] = e < cnt ? src[e] : make_double2(0.0, 0.0);
+            v[i] = e < cnt ? src[e] : mkc<C>(0, 0);
         }
 #pragma unroll
         for (int i = 0; i < PER; ++i) {
@@ -87,17 +92,17 @@ __global__ __launch_bounds__(64 * YWAVES) void k_ybwd_w(const double2 *__restric
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     YPlan<N>::template Run<+1, LPW, LS>::run(buf + w * LPW * LS, tw, lane);
     __syncthreads();
-    double2 *dst = G + bz * N * nkx + ix0;
+    C *dst = G + bz * N * nkx + ix0;
     {
         constexpr int PER = (N * YL + YT - 1) / YT;   // the last round partial when YT does not divide N*YL
-        double2 v[PER];
+        C v[PER];
 #pragma unroll
         for (int i = 0; i < PER; ++i) {
             const int e = threadIdx.x + i * YT;
             const int y = e / YL, l = e - y * YL;
             v[i] = (N * YL % YT == 0 || e < N * YL)
                        ? buf[l * LS + swz<YPlan<N>::template Run<+1, LPW, LS>::OUT_SWZ>(y)]
-                       : make_double2(0.0, 0.0);
+                       : mkc<C>(0, 0);
         }
 #pragma unroll
         for (int i = 0; i < PER; ++i) {
@@ -109,26 +114,28 @@ __global__ __launch_bounds__(64 * YWAVES) void k_ybwd_w(const double2 *__restric
 }
 
 // Hy[b][z][ix*nky + jy] = sum_y H[b][z][y][ix] exp(-i ky y)
-template <int N>
-__global__ __launch_bounds__(64 * YWAVES) void k_yfwd_w(const double2 *__restrict__ H, double2 *__restrict__ Hy,
-                                                       const double2 *__restrict__ tw, int nkx, int nky, int NY) {
+template <int N, class C>
+__global__ __launch_bounds__(64 * YWAVES) void k_yfwd_w(const C *__restrict__ H, C *__restrict__ Hy,
+                                                       const C *__restrict__ tw, int nkx, int nky, int NY) {
+    constexpr int YL = ycols<C>();
     constexpr int LS = N + 1, LPW = YL / YWAVES;
-    extern __shared__ double2 buf[];   // YL * LS (beyond 64 KiB for N = 768)
+    extern __shared__ unsigned char buf_raw[];   // YL * LS elements (beyond 64 KiB for N = 768)
+    C *buf = reinterpret_cast<C *>(buf_raw);
     const int ntile = (nkx + YL - 1) / YL;
     const int tile = blockIdx.x % ntile;
     const size_t bz = blockIdx.x / ntile;
     const int ix0 = tile * YL, nl = min(YL, nkx - ix0);
     const int nsys = nkx * nky;
     // all of a thread's loads are issued before the first LDS store
-    const double2 *src = H + bz * N * nkx + ix0;
+    const C *src = H + bz * N * nkx + ix0;
     {
         constexpr int PER = (N * YL + YT - 1) / YT;   // the last round partial when YT does not divide N*YL
-        double2 v[PER];
+        C v[PER];
 #pragma unroll
         for (int i = 0; i < PER; ++i) {
             const int e = threadIdx.x + i * YT;
             const int y = e / YL, l = e - y * YL;
-            v[i] = ((N * YL % YT == 0 || e < N * YL) && l < nl) ? src[(size_t)y * nkx + l] : make_double2(0.0, 0.0);
+            v[i] = ((N * YL % YT == 0 || e < N * YL) && l < nl) ? src[(size_t)y * nkx + l] : mkc<C>(0, 0);
         }
 #pragma unroll
         for (int i = 0; i < PER; ++i) {
@@ -141,17 +148,17 @@ __global__ __launch_bounds__(64 * YWAVES) void k_yfwd_w(const double2 *__restric
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     YPlan<N>::template Run<-1, LPW, LS>::run(buf + w * LPW * LS, tw, lane);
     __syncthreads();
-    double2 *dst = Hy + bz * nsys + (size_t)ix0 * nky;
+    C *dst = Hy + bz * nsys + (size_t)ix0 * nky;
     {
         constexpr int PER = (YL * N + YT - 1) / YT;   // nky <= N
         const int cnt = nl * nky;
-        double2 v[PER];
+        C v[PER];
 #pragma unroll
         for (int i = 0; i < PER; ++i) {
             const int e = threadIdx.x + i * YT;
             const int l = e / nky, jy = e - l * nky;
             v[i] = e < cnt ? buf[l * LS + swz<YPlan<N>::template Run<-1, LPW, LS>::OUT_SWZ>(ky_slot(jy, NY, N))]
-                           : make_double2(0.0, 0.0);
+                           : mkc<C>(0, 0);
         }
 #pragma unroll
         for (int i = 0; i < PER; ++i) {
@@ -162,38 +169,52 @@ __global__ __launch_bounds__(64 * YWAVES) void k_yfwd_w(const double2 *__restric
 }
 
 // LDS of a workgroup of the wavefront y pass at length N
-static size_t ywave_lds(int N) { return size_t(YL) * (N + 1) * sizeof(double2); }
+template <class C>
+static size_t ywave_lds(int N) { return size_t(ycols<C>()) * (N + 1) * sizeof(C); }
 
-template <int N>
-static void go(hipStream_t st, bool fwd, const FFTPlan &py, const double2 *in, double2 *out, int nbatch, int NZAA,
-               int nkx, int nky, int NY) {
+template <int N, class C>
+static void go(hipStream_t st, bool fwd, const C *tw, const C *in, C *out, int nbatch, int NZAA, int nkx, int nky,
+               int NY) {
+    constexpr int YL = ycols<C>();
     const int ntile = (nkx + YL - 1) / YL;
     const dim3 grid(ntile * nbatch * NZAA);
-    const size_t shm = ywave_lds(N);
+    const size_t shm = ywave_lds<C>(N);
     if (fwd) {
-        if (shm > 65536) set_dynamic_lds((const void *)k_yfwd_w<N>, shm, "y pass (forward)");
-        hipLaunchKernelGGL(k_yfwd_w<N>, grid, dim3(64 * YWAVES), shm, st, in, out, py.tw, nkx, nky, NY);
+        if (shm > 65536) set_dynamic_lds((const void *)k_yfwd_w<N, C>, shm, "y pass (forward)");
+        hipLaunchKernelGGL((k_yfwd_w<N, C>), grid, dim3(64 * YWAVES), shm, st, in, out, tw, nkx, nky, NY);
     } else {
-        if (shm > 65536) set_dynamic_lds((const void *)k_ybwd_w<N>, shm, "y pass (inverse)");
-        hipLaunchKernelGGL(k_ybwd_w<N>, grid, dim3(64 * YWAVES), shm, st, in, out, py.tw, nkx, nky, NY);
+        if (shm > 65536) set_dynamic_lds((const void *)k_ybwd_w<N, C>, shm, "y pass (inverse)");
+        hipLaunchKernelGGL((k_ybwd_w<N, C>), grid, dim3(64 * YWAVES), shm, st, in, out, tw, nkx, nky, NY);
     }
     check_launch("y pass");
 }
 
-bool launch_ypass_wave(hipStream_t st, bool fwd, const FFTPlan &py, const double2 *in, double2 *out, int nbatch,
-                       int NZAA, int nkx, int nky, int NY) {
-    if (nky > py.N || NY / 2 + (py.N - nky) > py.N) return false;
+template <class C>
+static bool ypass_wave(hipStream_t st, bool fwd, int N, const C *tw, const C *in, C *out, int nbatch, int NZAA,
+                       int nkx, int nky, int NY) {
+    if (!tw || nky > N || NY / 2 + (N - nky) > N) return false;
     // the tile of a workgroup must fit the device's LDS (98 KB at N = 768);
     // otherwise the workgroup-level LDS FFT takes the lines
-    if (ywave_lds(py.N) > lds_limit_current()) return false;
-    switch (py.N) {
-    case 96: go<96>(st, fwd, py, in, out, nbatch, NZAA, nkx, nky, NY); return true;
-    case 144: go<144>(st, fwd, py, in, out, nbatch, NZAA, nkx, nky, NY); return true;
-    case 192: go<192>(st, fwd, py, in, out, nbatch, NZAA, nkx, nky, NY); return true;
-    case 384: go<384>(st, fwd, py, in, out, nbatch, NZAA, nkx, nky, NY); return true;
-    case 768: go<768>(st, fwd, py, in, out, nbatch, NZAA, nkx, nky, NY); return true;   // 98 KB of LDS
+    if (ywave_lds<C>(N) > lds_limit_current()) return false;
+    switch (N) {
+    case 96: go<96>(st, fwd, tw, in, out, nbatch, NZAA, nkx, nky, NY); return true;
+    case 144: go<144>(st, fwd, tw, in, out, nbatch, NZAA, nkx, nky, NY); return true;
+    case 192: go<192>(st, fwd, tw, in, out, nbatch, NZAA, nkx, nky, NY); return true;
+    case 384: go<384>(st, fwd, tw, in, out, nbatch, NZAA, nkx, nky, NY); return true;
+    case 768: go<768>(st, fwd, tw, in, out, nbatch, NZAA, nkx, nky, NY); return true;
     default: return false;
     }
+}
+
+bool launch_ypass_wave(hipStream_t st, bool fwd, const FFTPlan &py, const double2 *in, double2 *out, int nbatch,
+                       int NZAA, int nkx, int nky, int NY) {
+    return ypass_wave<double2>(st, fwd, py.N, py.tw, in, out, nbatch, NZAA, nkx, nky, NY);
+}
+
+void launch_ypass32(hipStream_t st, bool fwd, const FFTPlan &py, const float2 *in, float2 *out, int nbatch, int NZAA,
+                    int nkx, int nky, int NY) {
+    if (!ypass_wave<float2>(st, fwd, py.N, py.twf, in, out, nbatch, NZAA, nkx, nky, NY))
+        throw std::runtime_error("the FP32 y pass needs a wavefront plan (N = 96, 144, 192, 384, 768) that fits LDS");
 }
 
 }  // namespace coral

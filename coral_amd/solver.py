@@ -285,6 +285,15 @@ class Solver:
         check(lib.coral_amd_products(self._ctx, out.ctypes.data), "coral_amd_products")
         return out
 
+    def set_precision(self, mode: str = "fp64", refine: int = 0):
+        """``"fp64"``: the FP64 step; ``"mixed"``: FP32 transforms and products
+        (compensated FP32 MFMA z products), FP64 NL sums, right-hand sides,
+        factors and solves.  ``refine``: FP64 residual-refinement sweeps after
+        the solves (0 none, 1 after the last stage of each step, 2 after every
+        stage)."""
+        m = {"fp64": 0, "mixed": 1}[mode]
+        check(lib.coral_amd_set_precision(self._ctx, m, int(refine)), "coral_amd_set_precision")
+
     def use_graphs(self, on: bool = True):
         """Replay steps as HIP graphs (captured at the next step); results are
         the eager ones bit for bit."""

@@ -230,6 +230,21 @@ int coral_amd_system_solve(coral_amd_ctx *ctx, int32_t isys, const double *rhs, 
 int coral_amd_assembled_operator(const coral_amd_problem *prob, int32_t isys, int32_t which, int32_t kx_int,
                                  int32_t ky_int, double *out, int32_t *total, int32_t *nl, int32_t *nu);
 
+/* Precision of the time step (no counterpart in the reference, which steps in
+ * FP64 throughout).  mode 0: FP64 everywhere (default).  mode 1: the
+ * transforms and the quadratic products run in FP32: extraction rounds the
+ * Chebyshev coefficients of the full variables to FP32, the z transforms are
+ * FP32 MFMA products with compensated (TwoSum) accumulation, the y and x FFTs
+ * and the products are FP32; the NL sums into K_hat, the right-hand sides, the
+ * banded factors and solves and the implicit update stay FP64.  refine: FP64
+ * residual-refinement sweeps after the banded solves (e = A x - rhs, A f = e,
+ * x -= f with the same factors): 0 none, 1 after the last stage of each step,
+ * 2 after every stage.  Mode 1 needs wavefront FFT plans for NXAA and NYAA
+ * (96, 144, 192, 384, 768) and an FP32 MFMA plan for NZ, and allocates FP32
+ * work arrays (about half the size of the FP64 ones) on first use.  The
+ * coral_amd_products seam follows the mode; the other seams stay FP64. */
+int coral_amd_set_precision(coral_amd_ctx *ctx, int32_t mode, int32_t refine);
+
 /* enable=1: coral_amd_step replays one step as a HIP graph (captured at the
  * next step, dropped by coral_amd_set_dt) instead of launching its kernels one
  * by one; same results bit for bit, less launch overhead on small grids.  No

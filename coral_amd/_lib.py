@@ -64,7 +64,7 @@ EXPORTS = (
     "coral_amd_timers_enable", "coral_amd_timer_read", "coral_amd_assembled_operator", "coral_amd_use_graphs", "coral_amd_products",
     "coral_amd_factor_kind", "coral_amd_get_variable_physical", "coral_amd_set_variable_physical",
     "coral_amd_horizontal_averages", "coral_amd_factor_blocks", "coral_amd_system_solve",
-    "coral_amd_set_precision",
+    "coral_amd_set_precision", "coral_amd_set_kx0_projection",
 )
 
 
@@ -95,6 +95,7 @@ def _load():
     lib.coral_amd_banded_solve.argtypes = [c_int32, c_int32, c_int32, c_int32, c_int32, P, P, P]
     lib.coral_amd_timers_enable.argtypes = [P, c_int32]
     lib.coral_amd_use_graphs.argtypes = [P, c_int32]
+    lib.coral_amd_set_kx0_projection.argtypes = [P, c_int32]
     lib.coral_amd_products.argtypes = [P, P]
     lib.coral_amd_timer_read.argtypes = [P, c_int32, ctypes.c_char_p, c_int32, ctypes.POINTER(c_double),
                                          ctypes.POINTER(c_int32)]

@@ -146,6 +146,9 @@ struct coral_amd_ctx {
     // FP64 residual-refinement sweeps after the solves: 0 none, 1 after the
     // last stage of each step, 2 after every stage
     int refine = 0;
+    // replace the kx = 0 plane of every kxky state by its Hermitian part after
+    // each step (coral_amd_set_kx0_projection)
+    bool project_kx0 = false;
     DevBuf<float2> spec32, Fz32, G32, H32, Hy32, Qhat32, twx32, twy32;
     DevBuf<float> zfrag32_b, zfrag32_f;
     ZFrags32Dev z32{};
@@ -779,6 +782,35 @@ void refine_solution(coral_amd_ctx &c, SysDev &d, const double2 *rhs_i, double2 
     });
 }
 
+// Hermitian part of the kx = 0 plane: for ky = m > 0 (kept slot jy = m) and its
+// partner ky = -m (slot NY - 1 - m), c(m) <- (c(m) + conj c(-m)) / 2 and
+// c(-m) <- conj c(m).  The physical fields see only this part (the x c2r
+// drops the imaginary part of kx = 0 after the y transform), so it changes
+// nothing they show; it removes the anti-Hermitian part, on which the
+// Hermitian NL terms never act and which the linear operator alone evolves.
+// One workgroup per ky pair, threads over the positions of the system.
+__global__ void k_project_kx0(double2 *x, int total, int NY) {
+    const int m = blockIdx.x + 1, jm = NY - 1 - m;
+    double2 *a = x + (size_t)m * total, *b = x + (size_t)jm * total;
+    for (int i = threadIdx.x; i < total; i += blockDim.x) {
+        const double2 u = a[i], v = b[i];
+        const double2 h = make_double2(0.5 * (u.x + v.x), 0.5 * (u.y - v.y));
+        a[i] = h;
+        b[i] = make_double2(h.x, -h.y);
+    }
+}
+
+void project_kx0(coral_amd_ctx &c) {
+    const Problem &P = c.P;
+    if (P.NY / 2 - 1 < 1) return;
+    for (auto &d : c.ks)
+        c.timed("project_kx0", [&] {
+            hipLaunchKernelGGL(k_project_kx0, dim3(unsigned(P.NY / 2 - 1)), dim3(256), 0, c.stream, d->field.p,
+                               d->t->total, P.NY);
+            check_launch("project_kx0");
+        });
+}
+
 // one_step_beyond (PL_IMEX_timestepping_marching.f90:3-48): stage i takes the
 // NL terms of the previous stage solution, rhs_i = M x^n + sum_{j<i} dt a_ij L x_j
 // + sum_{j<=i} dt ah_{i+1,j} K_hat_j, and solves (M - dt a_ii L) x_i = rhs_i;
@@ -844,6 +876,7 @@ void one_step(coral_amd_ctx &c) {
     }
     join_mean(c);
     swap_step_buffers(c);
+    if (c.project_kx0) project_kx0(c);
 }
 
 // Two graphs of one step each, captured from the stream (the side stream
@@ -1835,6 +1868,19 @@ int coral_amd_set_precision(coral_amd_ctx *c, int32_t mode, int32_t refine) {
         c->drop_graphs();
         c->precision = mode;
         c->refine = refine;
+        return 0;
+    } catch (const std::exception &e) {
+        return fail(e);
+    }
+}
+
+int coral_amd_set_kx0_projection(coral_amd_ctx *c, int32_t enable) {
+    try {
+        if (!c) throw std::runtime_error("null context");
+        HIP_OK(hipSetDevice(c->device));
+        HIP_OK(hipStreamSynchronize(c->stream));
+        c->drop_graphs();
+        c->project_kx0 = enable != 0;
         return 0;
     } catch (const std::exception &e) {
         return fail(e);

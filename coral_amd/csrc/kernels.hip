@@ -1780,7 +1780,11 @@ __global__ __launch_bounds__(64) void k_solve16(Solve16Args a) {
     const bool live = sg < nmat;
     const int M = live ? sg : nmat - 1;
     const int sys = M / a.nblk, blk = M - sys * a.nblk;
-    const bool dorot = a.rotate && !(a.ident0 && sys == 0);
+    // the kxky (0,0) system is factored as the identity in block coordinates,
+    // which maps block position i to row w_row and column w_col; it solves
+    // x = rhs when its right-hand side is read at the columns too
+    const bool idsys = a.ident0 && sys == 0;
+    const bool dorot = a.rotate && !idsys;
     const int g = M >> 2, m = M & 3;
     const int klp = solve16_klp(a.kl);
     const T *Lg = L16 + (size_t)g * ncol16 * 4 * klp;   // wavefront's records; matrix m's column j at (4j + m)*klp
@@ -1793,10 +1797,11 @@ __global__ __launch_bounds__(64) void k_solve16(Solve16Args a) {
     __syncthreads();
     const unsigned *W = sW + blk * wstride + SOLVE16_WPAD;   // W[i]: position i (padding outside [0, n))
     auto code = [&](unsigned w) -> int { return dorot ? int(w >> 30) : 0; };
-    auto row_or0 = [&](unsigned w) -> int { const int r = w_row(w); return r != SOLVE16_NONE ? r : 0; };
+    auto src_row = [&](unsigned w) -> int { return idsys ? w_col(w) : w_row(w); };
+    auto row_or0 = [&](unsigned w) -> int { const int r = src_row(w); return r != SOLVE16_NONE ? r : 0; };
     auto col_or0 = [&](unsigned w) -> int { const int c = w_col(w); return c != SOLVE16_NONE ? c : 0; };
     auto fix_rhs = [&](double2 v, unsigned w) -> double2 {
-        return rot_in(csel(w_row(w) != SOLVE16_NONE, v, make_double2(0.0, 0.0)), code(w));
+        return rot_in(csel(src_row(w) != SOLVE16_NONE, v, make_double2(0.0, 0.0)), code(w));
     };
     auto fix_y = [&](double2 v, unsigned w) -> double2 {
         return csel(w_col(w) != SOLVE16_NONE, v, make_double2(0.0, 0.0));

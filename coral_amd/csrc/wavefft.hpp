@@ -20,6 +20,8 @@
 
 #include "common.hpp"
 
+#include <type_traits>
+
 namespace coral {
 namespace wfft {
 
@@ -131,15 +133,22 @@ __device__ inline C twiddle(const C *__restrict__ tw, int m) {
     return SIGN > 0 ? mkc<C>(w.x, -w.y) : w;
 }
 
-// p[r] = exp(SIGN 2 pi i r m / N) for r = 1..R-1 (r m < N): one table load,
-// the other powers by a chain of products (each adds an ulp or so).  Loads of
-// more powers were measured slower: the twiddle loads sit at the head of
-// every stage.
+// p[r] = exp(SIGN 2 pi i r m / N) for r = 1..R-1 (r m < N).  FP64: one table
+// load, the other powers by a chain of products (each adds an ulp or so,
+// invisible at FP64); loads of more powers were measured slower, since the
+// twiddle loads sit at the head of every stage.  FP32: every power from the
+// table, each rounded once, since the chain's ulps would double the FFT's
+// rounding error at that precision.
 template <int R, int SIGN, class C>
 __device__ inline void twiddle_powers(const C *__restrict__ tw, int m, C (&p)[R]) {
-    p[1] = twiddle<SIGN>(tw, m);
+    if constexpr (std::is_same<C, float2>::value) {
 #pragma unroll
-    for (int r = 2; r < R; ++r) p[r] = cmul(p[r - 1], p[1]);
+        for (int r = 1; r < R; ++r) p[r] = twiddle<SIGN>(tw, r * m);
+    } else {
+        p[1] = twiddle<SIGN>(tw, m);
+#pragma unroll
+        for (int r = 2; r < R; ++r) p[r] = cmul(p[r - 1], p[1]);
+    }
 }
 
 // LDS bank swizzles (used by runs that opt in with SW).  A wavefront's

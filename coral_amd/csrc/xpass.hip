@@ -112,15 +112,106 @@ __device__ inline void product(const C (&f)[NPI][RL], int a, int b, typename CxT
     for (int r = 0; r < RL; ++r) p[r] = fa[r] * fb[r];
 }
 
+// Per-line balancing of packed pairs (the FP32 chain).  Two real lines packed
+// into one complex FFT share its rounding: each comes out with an error of
+// about eps times the larger of the two, so a line much smaller than its
+// partner (w beside t, a temperature product beside a velocity product)
+// would lose most of its digits.  In FP32 each line of a pair is therefore
+// scaled by the power of two that brings its largest component into
+// [0.5, 1) before the FFT, and scaled back after it; both scalings are
+// exact.  The maxima are reduced per line through LDS (bit patterns of
+// non-negative floats order as unsigned integers).
+template <class C>
+struct Balanced {
+    static constexpr bool value = false;
+};
+template <>
+struct Balanced<float2> {
+    static constexpr bool value = true;
+};
+constexpr int XBAL_SLOTS = 16;   // per wavefront: 2 lines x (lines per wavefront x pairs per batch)
+
+__device__ inline unsigned absmax_bits(float2 v) { return __float_as_uint(fmaxf(fabsf(v.x), fabsf(v.y))); }
+__device__ inline unsigned absmax_bits(float v) { return __float_as_uint(fabsf(v)); }
+
+// power of two bringing a positive maximum into [0.5, 1) (1 for zero), and
+// its inverse; exponents are clamped to the normal range
+__device__ inline float bal_scale(unsigned mbits) {
+    const float m = __uint_as_float(mbits);
+    if (!(m > 0.f)) return 1.f;
+    int e;
+    frexpf(m, &e);
+    e = e < -125 ? -125 : (e > 125 ? 125 : e);
+    return ldexpf(1.f, -e);
+}
+__device__ inline float bal_unscale(unsigned mbits) {
+    const float m = __uint_as_float(mbits);
+    if (!(m > 0.f)) return 1.f;
+    int e;
+    frexpf(m, &e);
+    e = e < -125 ? -125 : (e > 125 ? 125 : e);
+    return ldexpf(1.f, e);
+}
+
 // forward transforms of product pairs [pq0, pq0 + G) and the store of the kept kx
 template <int N, int NPI, int XGP, int G, class C>
 __device__ inline void xfwd_batch(const XPassArgs &a, const C (&fv)[XShape<N, NPI, XGP>::NT][NPI][XPlan<N>::RL],
                                   C *buf, const C *__restrict__ tw, int lane, long long line0, long long nlines,
-                                  int pq0) {
+                                  int pq0, unsigned *xmax) {
     using X = XShape<N, NPI, XGP>;
     using T = typename CxT<C>::R;
     C *H = reinterpret_cast<C *>(a.H);
     constexpr int RL = X::RL, TL = X::TL, LPW = X::LPW;
+    if constexpr (Balanced<C>::value) {
+        static_assert(2 * LPW * G <= XBAL_SLOTS, "balancing slots");
+        // products of every task first, their per-line maxima, then the
+        // scaled FFTs (slot 2 (l G + gp) + {0, 1}: the pair's two lines)
+        T p0[X::NT][G][RL], p1[X::NT][G][RL];
+        if (lane < 2 * LPW * G) xmax[lane] = 0u;
+        wfft::wave_sync();
+#pragma unroll
+        for (int c = 0; c < X::NT; ++c) {
+            const int t = lane + 64 * c, l = t / TL;
+#pragma unroll
+            for (int gp = 0; gp < G; ++gp) {
+                const int q0 = 2 * (pq0 + gp), q1 = q0 + 1;
+                const bool has1 = q1 < a.nquad;
+                const int a0 = __builtin_amdgcn_readfirstlane(a.qa[q0]), b0 = __builtin_amdgcn_readfirstlane(a.qb[q0]);
+                const int a1 = has1 ? __builtin_amdgcn_readfirstlane(a.qa[q1]) : -1;
+                const int b1 = has1 ? __builtin_amdgcn_readfirstlane(a.qb[q1]) : -1;
+                product<NPI, RL>(fv[c], a0, b0, p0[c][gp]);
+                if (has1)
+                    product<NPI, RL>(fv[c], a1, b1, p1[c][gp]);
+                else
+#pragma unroll
+                    for (int r = 0; r < RL; ++r) p1[c][gp][r] = 0;
+                unsigned m0 = 0u, m1 = 0u;
+#pragma unroll
+                for (int r = 0; r < RL; ++r) {
+                    m0 = max(m0, absmax_bits(p0[c][gp][r]));
+                    m1 = max(m1, absmax_bits(p1[c][gp][r]));
+                }
+                atomicMax(&xmax[2 * (l * G + gp)], m0);
+                atomicMax(&xmax[2 * (l * G + gp) + 1], m1);
+            }
+        }
+        wfft::wave_sync();
+#pragma unroll
+        for (int c = 0; c < X::NT; ++c) {
+            const int t = lane + 64 * c, l = t / TL, j = t - l * TL;
+#pragma unroll
+            for (int gp = 0; gp < G; ++gp) {
+                const T s0 = bal_scale(xmax[2 * (l * G + gp)]), s1 = bal_scale(xmax[2 * (l * G + gp) + 1]);
+                C y[RL];
+#pragma unroll
+                for (int r = 0; r < RL; ++r) y[r] = mkc<C>(p0[c][gp][r] * s0, p1[c][gp][r] * s1);
+                Dft<RL, -1>::run(y);
+                C *L = buf + (l * G + gp) * N;
+#pragma unroll
+                for (int r = 0; r < RL; ++r) L[wfft::swz<XPlan<N>::WL>(j * RL + r)] = y[r];
+            }
+        }
+    } else {
 #pragma unroll
     for (int c = 0; c < X::NT; ++c) {
         const int t = lane + 64 * c, l = t / TL, j = t - l * TL;
@@ -147,6 +238,7 @@ __device__ inline void xfwd_batch(const XPassArgs &a, const C (&fv)[XShape<N, NP
             for (int r = 0; r < RL; ++r) L[wfft::swz<XPlan<N>::WL>(j * RL + r)] = y[r];
         }
     }
+    }
     wfft::wave_sync();
     XPlan<N>::template FwdRest<LPW * G>::run(buf, tw, lane);
     const int nkx = a.nkx;
@@ -160,12 +252,16 @@ __device__ inline void xfwd_batch(const XPassArgs &a, const C (&fv)[XShape<N, NP
                 const C *L = buf + (l * G + gp) * N;
                 C *h0 = H + ((size_t)q0 * nlines + line) * nkx;
                 C *h1 = H + ((size_t)q1 * nlines + line) * nkx;
-                const T hf = T(0.5);
+                T hf0 = T(0.5), hf1 = T(0.5);
+                if constexpr (Balanced<C>::value) {
+                    hf0 *= bal_unscale(xmax[2 * (l * G + gp)]);
+                    hf1 *= bal_unscale(xmax[2 * (l * G + gp) + 1]);
+                }
                 for (int k = lane; k < nkx; k += 64) {
                     const C y = L[k], yc = cconj(L[k == 0 ? 0 : N - k]);
                     // p0 = (Y + conj Y(-k)) / 2,  p1 = -i (Y - conj Y(-k)) / 2
-                    h0[k] = mkc<C>(hf * (y.x + yc.x), hf * (y.y + yc.y));
-                    if (q1 < a.nquad) h1[k] = mkc<C>(hf * (y.y - yc.y), -hf * (y.x - yc.x));
+                    h0[k] = mkc<C>(hf0 * (y.x + yc.x), hf0 * (y.y + yc.y));
+                    if (q1 < a.nquad) h1[k] = mkc<C>(hf1 * (y.y - yc.y), -hf1 * (y.x - yc.x));
                 }
             }
         }
@@ -184,7 +280,9 @@ __global__ __launch_bounds__(256) void k_xpass_wave(XPassArgs a, const C *__rest
     extern __shared__ unsigned char lds_raw[];
     C *lds = reinterpret_cast<C *>(lds_raw);
     __shared__ double cfl_part[4][3];
+    __shared__ unsigned xmax_all[4][XBAL_SLOTS];   // per-line maxima of packed pairs (FP32 only)
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    unsigned *xmax = xmax_all[wid];
     C *buf = lds + wid * X::SMAX * N;
     const C *G = reinterpret_cast<const C *>(a.G);
     const long long line0 = ((long long)blockIdx.x * (blockDim.x >> 6) + wid) * LPW;
@@ -197,7 +295,79 @@ __global__ __launch_bounds__(256) void k_xpass_wave(XPassArgs a, const C *__rest
 #pragma unroll
     for (int p = 0; p < NPI; ++p) {
         // first stage: packed Hermitian spectra straight from HBM
-        {
+        if constexpr (Balanced<C>::value) {
+            static_assert(2 * LPW <= XBAL_SLOTS, "balancing slots");
+            constexpr int TOT = SIN * T0, ROUNDS = (TOT + 63) / 64;
+            // the two lines' values first (kept slots only, kx = 0 imaginary
+            // parts dropped), their per-line maxima, then the scaled packing
+            C va[ROUNDS][R0], vb[ROUNDS][R0];
+            if (lane < 2 * LPW) xmax[lane] = 0u;
+            wfft::wave_sync();
+#pragma unroll
+            for (int c = 0; c < ROUNDS; ++c) {
+                const int b = lane + 64 * c;
+                if (TOT % 64 == 0 || b < TOT) {
+                    const int l = b / T0, j = b - l * T0;
+                    const long long line = line0 + l;
+                    const bool valid = line < nlines;
+                    const int f0 = 2 * p, f1 = 2 * p + 1;
+                    const bool has1 = f1 < a.nfull;
+                    const size_t lo = valid ? (size_t)line : 0;
+                    const C *g0 = G + ((size_t)f0 * nlines + lo) * nkx;
+                    const C *g1 = G + ((size_t)(has1 ? f1 : f0) * nlines + lo) * nkx;
+                    unsigned ma = 0u, mb = 0u;
+#pragma unroll
+                    for (int r = 0; r < R0; ++r) {
+                        const int e = j + r * T0;
+                        const int lo = r * T0, hi = lo + T0 - 1;
+                        const bool all_pos = NK > 0 && hi < NK;
+                        const bool all_neg = NK > 0 && lo > N - NK;
+                        const bool all_zero = NK > 0 && lo >= NK && hi <= N - NK;
+                        if (all_zero) {
+                            va[c][r] = vb[c][r] = mkc<C>(0, 0);
+                            continue;
+                        }
+                        const bool pos = all_pos || (!all_neg && e < nkx);
+                        const bool neg = all_neg || (!all_pos && e > N - nkx);
+                        const int k = pos ? e : (neg ? N - e : 0);
+                        const C A = g0[k], B = g1[k];
+                        const bool e0 = r == 0 && j == 0;
+                        const bool keep = valid && (all_pos || all_neg || pos || neg);
+                        va[c][r] = mkc<C>(keep ? A.x : T(0), (keep && !e0) ? A.y : T(0));
+                        vb[c][r] = mkc<C>((keep && has1) ? B.x : T(0), (keep && has1 && !e0) ? B.y : T(0));
+                        ma = max(ma, absmax_bits(va[c][r]));
+                        mb = max(mb, absmax_bits(vb[c][r]));
+                    }
+                    atomicMax(&xmax[2 * l], ma);
+                    atomicMax(&xmax[2 * l + 1], mb);
+                }
+            }
+            wfft::wave_sync();
+#pragma unroll
+            for (int c = 0; c < ROUNDS; ++c) {
+                const int b = lane + 64 * c;
+                if (TOT % 64 == 0 || b < TOT) {
+                    const int l = b / T0, j = b - l * T0;
+                    const T sa = bal_scale(xmax[2 * l]), sb = bal_scale(xmax[2 * l + 1]);
+#pragma unroll
+                    for (int r = 0; r < R0; ++r) {
+                        const int e = j + r * T0;
+                        const int lo = r * T0;
+                        const bool all_neg = NK > 0 && lo > N - NK;
+                        const bool all_pos = NK > 0 && lo + T0 - 1 < NK;
+                        const bool pos = all_pos || (!all_neg && e < nkx);
+                        const T ax = va[c][r].x * sa, ay = va[c][r].y * sa;
+                        const T bx = vb[c][r].x * sb, by = vb[c][r].y * sb;
+                        // pos: A + iB;  neg: conj(A) + i conj(B)
+                        va[c][r] = mkc<C>(pos ? ax - by : ax + by, pos ? ay + bx : bx - ay);
+                    }
+                    Dft<R0, +1>::run(va[c]);
+#pragma unroll
+                    for (int r = 0; r < R0; ++r) buf[l * N + wfft::swz<XPlan<N>::W0>(j * R0 + r)] = va[c][r];
+                }
+            }
+            wfft::wave_sync();
+        } else {
             constexpr int TOT = SIN * T0, ROUNDS = (TOT + 63) / 64;
             C v[ROUNDS][R0];
 #pragma unroll
@@ -268,6 +438,11 @@ __global__ __launch_bounds__(256) void k_xpass_wave(XPassArgs a, const C *__rest
                 fv[c][p][r] = x;
             }
             Dft<RL, +1>::run(fv[c][p]);
+            if constexpr (Balanced<C>::value) {
+                const T ua = bal_unscale(xmax[2 * l]), ub = bal_unscale(xmax[2 * l + 1]);
+#pragma unroll
+                for (int r = 0; r < RL; ++r) fv[c][p][r] = mkc<C>(fv[c][p][r].x * ua, fv[c][p][r].y * ub);
+            }
         }
         wfft::wave_sync();
     }
@@ -316,9 +491,9 @@ __global__ __launch_bounds__(256) void k_xpass_wave(XPassArgs a, const C *__rest
     const int npairs = (a.nquad + 1) / 2;
     for (int pq0 = 0; pq0 < npairs; pq0 += XGP) {
         if (npairs - pq0 >= XGP)
-            xfwd_batch<N, NPI, XGP, XGP>(a, fv, buf, tw, lane, line0, nlines, pq0);
+            xfwd_batch<N, NPI, XGP, XGP>(a, fv, buf, tw, lane, line0, nlines, pq0, xmax);
         else
-            xfwd_batch<N, NPI, XGP, 1>(a, fv, buf, tw, lane, line0, nlines, pq0);
+            xfwd_batch<N, NPI, XGP, 1>(a, fv, buf, tw, lane, line0, nlines, pq0, xmax);
     }
 }
 

@@ -151,3 +151,38 @@ def nusselt_wall_differences(profile: np.ndarray, gap: float) -> Tuple[float, fl
     N = profile.shape[-1]
     dz = 0.5 * gap * (1.0 - math.cos(math.pi / (2 * N)))
     return float(profile[0] / dz), float(-profile[-1] / dz)
+
+
+def nusselt_quarters(time: np.ndarray, t_top: np.ndarray, t_bot: np.ndarray, NZAA: int, gap: float = 1.0):
+    """Nusselt number of a run from its timeseries, as
+    etc/benchmarks/02_RRBC_julien_JFM96/compute_nusselt.py:16-57 forms it.
+
+    ``time`` are the sample times and ``t_top`` / ``t_bot`` the horizontal
+    averages of the temperature at the first (next to the top wall) and last
+    Gauss point.  The series is cut into four quarters of
+    q = (len(time) - 1) // 4 samples (the script takes len(time) // 4 with two
+    time samples more than temperature samples; here every sample k < 4q has
+    its weight dt_k); in quarter a the wall flux is the time-weighted mean
+    sum(dt_k T_k) / sum(dt_k) over k in [a q, (a+1) q) with dt_k = time[k+1] -
+    time[k], divided by the distance dz of the Gauss point to its wall (top:
+    +<T>/dz, bottom: -<T>/dz).  Returns (quarters, mean, std): the (top,
+    bottom) pair of each quarter, and the mean and population standard
+    deviation of the six values of quarters 2 to 4 (the first quarter is the
+    transient)."""
+    time = np.asarray(time, np.float64)
+    t_top = np.asarray(t_top, np.float64)
+    t_bot = np.asarray(t_bot, np.float64)
+    q = (time.shape[0] - 1) // 4
+    if q < 1 or t_top.shape[0] < 4 * q or t_bot.shape[0] < 4 * q:
+        raise ValueError("need at least four samples of time and both temperatures")
+    dz = 0.5 * gap * (1.0 - math.cos(math.pi / (2 * NZAA)))
+    d = np.diff(time)
+    quarters = []
+    for a in range(4):
+        sl = slice(a * q, (a + 1) * q)
+        w = np.sum(d[sl])
+        quarters.append((float(np.sum(d[sl] * t_top[sl]) / w / dz), float(-np.sum(d[sl] * t_bot[sl]) / w / dz)))
+    vals = np.array([v for a in (1, 2, 3) for v in quarters[a]])
+    mean = float(np.mean(vals))
+    std = float(np.sqrt(np.mean((vals - mean) ** 2)))
+    return quarters, mean, std

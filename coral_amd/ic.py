@@ -132,3 +132,54 @@ def rbc_initial_state(recipe, NX: int, NY: int, NZ: int, amplitude: float, kmax:
             kstates.append(np.zeros((NX // 2, (3 * NY) // 2, total), np.complex128))
     zstates = [np.zeros(sum(NZ - bc // 10 for bc in s.bc_codes)) for s in recipe.zero_systems]
     return kstates, zstates
+
+
+def noisy_initial_state(recipe, NX: int, NY: int, NZ: int, Lx: float, Ly: float, amp_kxky: float,
+                        amp_zero: float, seed: int):
+    """Coral's default noisy initial condition (``noisy_initial_conditions``,
+    ``src/plane_layer/PL_IMEX_timestepping.f90:425-486``) with seeded phases.
+
+    Every Galerkin coefficient of position ``n`` (counted inside its variable)
+    of every kxky variable at wavevector |k| gets two random-phase bumps,
+    ``exp(-(n-3.5)^2/8) exp(-(|k|-6 k_box)^2/8) amp`` and
+    ``3 exp(-(n-1.5)^2/8) exp(-(|k|-3 k_box)^2/32) amp``, with k_box = 2 pi / Lx
+    (the reference's ``kx_box``) and |k| from the physical wavenumbers; mean
+    variables get ``exp(-(n-1.5)^2/8) cos(p) amp_zero`` at their storage
+    position p (1-based).  The phases come from SplitMix64 uniforms instead of
+    the unseeded ``random_number``; the kx = 0 plane is made Hermitian in ky
+    and the (0, 0) and dealiased modes are zero (the step keeps them so)."""
+    kx, ky, kept = wavenumber_indices(NX, NY)
+    NYAA = ky.size
+    kx_box, ky_box = 2.0 * math.pi / Lx, 2.0 * math.pi / Ly
+    kabs = np.sqrt((kx[:, None] * kx_box) ** 2 + (ky[None, :] * ky_box) ** 2)
+    rng = SplitMix64(seed)
+    kstates = []
+    for s in recipe.kxky_systems:
+        sizes = [NZ - bc // 10 for bc in s.bc_codes]
+        state = np.zeros((NX // 2, NYAA, sum(sizes)), np.complex128)
+        for v in range(len(sizes)):
+            pos = interleaved_positions(len(sizes), sizes, v)
+            for n, p in enumerate(pos):
+                ph1 = np.array([[rng.uniform() for _ in range(NYAA)] for _ in range(NX // 2)])
+                ph2 = np.array([[rng.uniform() for _ in range(NYAA)] for _ in range(NX // 2)])
+                b1 = math.exp(-(n - 3.5) ** 2 / 8.0) * np.exp(-(kabs - 6.0 * kx_box) ** 2 / 8.0)
+                b2 = 3.0 * math.exp(-(n - 1.5) ** 2 / 8.0) * np.exp(-(kabs - 3.0 * kx_box) ** 2 / 32.0)
+                state[:, :, p] = amp_kxky * (b1 * np.exp(2j * math.pi * ph1) + b2 * np.exp(2j * math.pi * ph2))
+        state[~kept] = 0.0
+        state[0, 0] = 0.0
+        for iy in range(NYAA):
+            if ky[iy] < 0 and kept[0, iy]:
+                jy = int(np.nonzero(ky == -ky[iy])[0][0])
+                state[0, iy, :] = np.conj(state[0, jy, :])
+        kstates.append(state)
+    zstates = []
+    for s in recipe.zero_systems:
+        sizes = [NZ - bc // 10 for bc in s.bc_codes]
+        z = np.zeros(sum(sizes))
+        first = 0
+        for v, size in enumerate(sizes):
+            for n in range(size):
+                z[first + n] = math.exp(-(n - 1.5) ** 2 / 8.0) * math.cos(float(first + n + 1)) * amp_zero
+            first += size
+        zstates.append(z)
+    return kstates, zstates

@@ -294,6 +294,13 @@ class Solver:
         m = {"fp64": 0, "mixed": 1}[mode]
         check(lib.coral_amd_set_precision(self._ctx, m, int(refine)), "coral_amd_set_precision")
 
+    def set_kx0_projection(self, on: bool = True):
+        """Replace the kx = 0 plane by its Hermitian part after every step
+        (``coral_amd_set_kx0_projection``): removes the component the physical
+        fields cannot see, which otherwise grows without bound in long
+        supercritical runs."""
+        check(lib.coral_amd_set_kx0_projection(self._ctx, int(bool(on))), "coral_amd_set_kx0_projection")
+
     def use_graphs(self, on: bool = True):
         """Replay steps as HIP graphs (captured at the next step); results are
         the eager ones bit for bit."""

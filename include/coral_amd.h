@@ -245,6 +245,20 @@ int coral_amd_assembled_operator(const coral_amd_problem *prob, int32_t isys, in
  * coral_amd_products seam follows the mode; the other seams stay FP64. */
 int coral_amd_set_precision(coral_amd_ctx *ctx, int32_t mode, int32_t refine);
 
+/* enable=1: after each step, replace the kx = 0 plane of every kxky state by
+ * its Hermitian part, c(0, ky) = (c(0, ky) + conj c(0, -ky)) / 2.  No
+ * counterpart in the reference, whose steps (and the default here, enable=0)
+ * carry the anti-Hermitian part along.  The physical fields never see that
+ * part: the x c2r drops the imaginary part of kx = 0 after the y transform
+ * (c2r_transform, slabs.fftw3mpi/transforms.f90:204-210), so the Hermitian NL
+ * terms never act on it and it evolves under the linear operator alone.  Where
+ * that operator has unstable kx = 0 modes (supercritical convection) it grows
+ * from rounding level until rounding leaks it back into the fields, which
+ * ends the run (the 96x96x48 rotating case of
+ * etc/benchmarks/02_RRBC_julien_JFM96 after about 15,000 steps).  The
+ * projection changes no physical field. */
+int coral_amd_set_kx0_projection(coral_amd_ctx *ctx, int32_t enable);
+
 /* enable=1: coral_amd_step replays one step as a HIP graph (captured at the
  * next step, dropped by coral_amd_set_dt) instead of launching its kernels one
  * by one; same results bit for bit, less launch overhead on small grids.  No

@@ -441,26 +441,29 @@ def to_diagonals(m: np.ndarray, nl: int, nu: int) -> np.ndarray:
 
 def spec_to_phys(S: np.ndarray, geo: Geometry) -> np.ndarray:
     """c2r_transform (slabs.fftw3mpi/transforms.f90:155-225).
-    S: complex (nkx=NX/2, NYAA, NZ) -> real (NXAA, NYAA, NZAA), axes (x, y, z)."""
+    S: complex (nkx=NX/2, NYAA, NZ) -> real (NXAA, NYAA, NZAA), axes (x, y, z).
+    A complex64 S runs the whole chain in single precision (scipy's FFTs keep
+    the input precision); complex128 in double."""
     A = sfft.ifft(S, axis=1, workers=WORKERS) * geo.NYAA                  # y c2c FFTW_BACKWARD (:172-178)
     A[:, :, 0] *= 2.0                                    # Chebyshev k=0 row x2 (:184)
-    B = np.zeros((geo.NXAA // 2 + 1, geo.NYAA, geo.NZ), np.complex128)
+    B = np.zeros((geo.NXAA // 2 + 1, geo.NYAA, geo.NZ), S.dtype)
     B[: geo.NX // 2] = A * 0.5                           # pad x and halve (:197-198)
     C = sfft.irfft(B, n=geo.NXAA, axis=0, workers=WORKERS) * geo.NXAA     # x c2r, unnormalised (:204-210)
-    D = np.zeros((geo.NXAA, geo.NYAA, geo.NZAA))
+    D = np.zeros((geo.NXAA, geo.NYAA, geo.NZAA), C.dtype)
     D[:, :, : geo.NZ] = C                                # pad z (:216-217)
     return sfft.dct(D, type=3, axis=2, workers=WORKERS)                   # REDFT01 (:223)
 
 
 def phys_to_spec(P: np.ndarray, geo: Geometry) -> np.ndarray:
-    """r2c_transform (slabs.fftw3mpi/transforms.f90:227-301), inverse of the above."""
+    """r2c_transform (slabs.fftw3mpi/transforms.f90:227-301), inverse of the above
+    (in the precision of P: float32 or float64)."""
     B5 = sfft.dct(P, type=2, axis=2, workers=WORKERS)                     # REDFT10 (:243)
     B4 = B5[:, :, : geo.NZ]                              # shrink z (:250)
     B3 = sfft.rfft(B4, axis=0, workers=WORKERS)                           # x r2c (:256-262)
     B2 = B3[: geo.NX // 2].copy()                        # shrink x (:268)
     B2[:, :, 0] *= 0.5                                   # Chebyshev k=0 row x1/2 (:279)
     S = sfft.fft(B2, axis=1, workers=WORKERS)                             # y c2c FFTW_FORWARD (:285-292)
-    return S / (geo.NXAA * geo.NYAA * geo.NZAA)          # normalisation (:293-296)
+    return S / S.real.dtype.type(geo.NXAA * geo.NYAA * geo.NZAA)   # normalisation (:293-296)
 
 
 # ----------------------------------------------------------------------------
@@ -477,6 +480,22 @@ def remove_z_integral(s, NZ):
     for k in range(1, NZ // 2):
         s[..., 0] -= s[..., 2 * k] * (0.5 / (2.0 * k + 1.0) - 0.5 / (2.0 * k - 1.0))
     return s
+
+
+def project_kx0_plane(f, geo):
+    """In place: the kx = 0 plane of a kxky state (nkx, NYAA, total) replaced by
+    its Hermitian part, c(0, ky) = (c(0, ky) + conj c(0, -ky)) / 2 over the
+    kept ky (no counterpart in the reference; see OracleSolver.project_kx0)."""
+    ky = geo.ky_int
+    for iy in range(geo.nky):
+        m = int(ky[iy])
+        if m <= 0 or geo.dealias[0, iy]:
+            continue
+        jy = int(np.nonzero(ky == -m)[0][0])
+        h = 0.5 * (f[0, iy] + np.conj(f[0, jy]))
+        f[0, iy] = h
+        f[0, jy] = np.conj(h)
+    return f
 
 
 class OracleSolver:
@@ -517,6 +536,13 @@ class OracleSolver:
         self.dt = None
         self.D = chebyshev_derivative_matrix(NZ, geo.center, geo.gap)
         self.cfl_dt = None
+        # precision of the transform chain and the products (marching.f90:279-300):
+        # np.float64 as the reference, or np.float32 (scipy's single-precision
+        # FFTs, products rounded to FP32), the yardstick of a mixed-precision step
+        self.transform_dtype = np.float64
+        # after each step, the kx = 0 plane replaced by its Hermitian part (the
+        # HIP path's coral_amd_set_kx0_projection); False restates the reference
+        self.project_kx0 = False
 
     # --- per-mode monomials (recipe_to_prefactors_array, :1558-1574) -------
     def monomials(self, keys) -> np.ndarray:
@@ -628,7 +654,8 @@ class OracleSolver:
         (nkx, NYAA, NZ) each; sets cfl_dt from the full variables."""
         g = self.geo
         specs = self.full_variables_spec(aux, zaux)
-        phys = [spec_to_phys(s, g) for s in specs]
+        cdt = np.complex64 if self.transform_dtype == np.float32 else np.complex128
+        phys = [spec_to_phys(s.astype(cdt), g) for s in specs]
         if len(phys) >= 3:
             self.cfl_dt = self.cfl(phys)
         qhat = []
@@ -639,7 +666,7 @@ class OracleSolver:
                 prod = phys[q.ivar1]
             else:
                 prod = phys[q.ivar1] * phys[q.ivar2]
-            s = phys_to_spec(prod, g)
+            s = phys_to_spec(prod, g).astype(np.complex128)
             if q.remove_z_integral:
                 remove_z_integral(s, g.NZ)
             qhat.append(s)
@@ -718,6 +745,8 @@ class OracleSolver:
             f = aux[isys]
             f[~self.keep] = 0.0
             f[0, 0, :] = 0.0
+            if self.project_kx0:
+                project_kx0_plane(f, self.geo)
             self.field[isys] = f
         self.zfield = zaux
 

@@ -59,3 +59,17 @@ def recipe(name):
 def product_recipe(name):
     from coral_amd.equations import read_equations
     return read_equations(os.path.join(ROOT, "coral_amd", "inputs", name))
+
+
+def seeded_state(r, NX, NY, NZ, kmax, nmodes, seed, amps=(("u", 1.0), ("v", 1.0), ("w", 1.0), ("t", 1e-3))):
+    """Seeded perturbations of several variables of the first kxky system at
+    the given amplitudes (by default the velocities at unit amplitude and the
+    temperature a thousand times smaller, as in convection, where velocities
+    in thermal-diffusion units are large); mean fields zero."""
+    from coral_amd import ic
+    k0, z0 = ic.rbc_initial_state(r, NX, NY, NZ, 0.0, kmax, nmodes, seed)
+    s = r.kxky_systems[0]
+    for i, (name, amp) in enumerate(amps):
+        k0[0] = k0[0] + ic.seeded_perturbation(NX, NY, NZ, s.bc_codes, s.var_names.index(name), amp,
+                                               kmax, nmodes, seed + 17 * i)
+    return k0, z0

@@ -52,3 +52,29 @@ def test_wall_differences_match_a_conduction_profile():
     assert abs(top - a) < 1e-12 and abs(bot - a) < 1e-12
     # dz of the reference script for NZAA = 72
     assert abs(0.5 * (1 - math.cos(math.pi / 144)) - (1. - 0.5 - 0.5 * np.cos(np.pi / 2 / 72.))) < 1e-16
+
+
+def test_nusselt_quarters_follows_compute_nusselt():
+    """compute_nusselt.py:16-57 on series whose answer is known: a constant
+    profile gives T/dz in every quarter; a temperature that jumps between
+    quarters gives each quarter's own value, time-weighted, and the mean and
+    population spread of quarters 2-4 only."""
+    NZAA, gap = 72, 1.0
+    dz = 1.0 - 0.5 - 0.5 * np.cos(np.pi / 2 / 72.0)   # the script's own expression
+    n = 401
+    time = np.cumsum(np.linspace(1e-5, 3e-5, n))
+    quarters, mean, std = D.nusselt_quarters(time, np.full(n, 0.5 * dz), np.full(n, -0.25 * dz), NZAA, gap)
+    for top, bot in quarters:
+        assert top == pytest.approx(0.5, rel=1e-12) and bot == pytest.approx(0.25, rel=1e-12)
+    q = (n - 1) // 4
+    level = np.append(np.repeat([9.0, 4.0, 5.0, 6.0], q), 0.0) * dz
+    quarters, mean, std = D.nusselt_quarters(time, level, -level, NZAA, gap)
+    assert [round(t, 9) for t, _ in quarters] == [9.0, 4.0, 5.0, 6.0]
+    assert mean == pytest.approx(5.0, rel=1e-12)
+    assert std == pytest.approx(np.sqrt(2.0 / 3.0), rel=1e-12)
+    # the time weights: one large step in a quarter dominates its mean
+    t2 = time.copy()
+    t2[q + 1:] += 1.0                      # dt_q (the sample at index q) is huge
+    ramp = np.arange(n, dtype=float) * dz
+    quarters, _, _ = D.nusselt_quarters(t2, ramp, -ramp, NZAA, gap)
+    assert quarters[1][0] == pytest.approx(float(q), rel=1e-3)

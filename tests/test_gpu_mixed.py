@@ -2,16 +2,19 @@
 with the z products on FP32 MFMA with compensated accumulation; FP64 NL sums,
 right-hand sides, banded factors and solves, and FP64 residual refinement.
 
-Tolerances: the FP32 chain rounds the Chebyshev coefficients to FP32 (relative
-6e-8) and runs three transforms each way in FP32 (lengths <= 768; error grows
-like log N per FFT stage); the quadratic spectra are therefore compared with
-the FP64 oracle at relative L2 <= 2e-6.  The state after a few steps moves by
-dt times the NL terms, so its error against FP64 is smaller still (<= 1e-7
-relative here).  FP64 with refinement stays within 1e-12 of FP64 without."""
+The yardstick of the FP32 chain is the oracle's own transform chain run in
+single precision (scipy's FP32 FFTs and DCTs, products rounded to FP32,
+``OracleSolver.transform_dtype = np.float32``): against the FP64 oracle the
+GPU's FP32 chain must be no worse than twice that error.  The states seed the
+velocities at unit amplitude and the temperature a thousand times smaller, as
+in convection (velocities in thermal-diffusion units are large), so the
+packed pairs of the x pass (w beside t, velocity products beside
+temperature products) are far from balanced.  FP64 with refinement stays
+within 1e-12 of FP64 without."""
 import numpy as np
 import pytest
 
-from conftest import product_recipe
+from conftest import product_recipe, seeded_state
 from coral_amd import ic
 from oracle import coral_oracle as O
 
@@ -38,14 +41,18 @@ PRODUCT_CASES = [
 
 @pytest.mark.parametrize("case", PRODUCT_CASES, ids=[f"{c[1]}x{c[2]}x{c[3]}" for c in PRODUCT_CASES])
 def test_mixed_products_match_oracle(solver_mod, case):
-    """The quadratic variables' spectra of the FP32 chain (broadband state,
-    every kept mode seeded) against the FP64 oracle."""
+    """The quadratic variables' spectra of the FP32 chain (every kept mode of
+    u, v, w, t seeded) against the FP64 oracle, per variable, within twice the
+    error of the oracle's FP32 chain."""
     name, NX, NY, NZ, Lx, Ly = case
     print("case", case, flush=True)
     r = product_recipe(name)
-    k0, z0 = ic.rbc_initial_state(r, NX, NY, NZ, 1e-2, 10 * max(NX, NY), 8, 11)
+    k0, z0 = seeded_state(r, NX, NY, NZ, 10 * max(NX, NY), 8, 11)
     geo = O.Geometry(NX, NY, NZ, Lx, Ly)
-    qref = O.OracleSolver(r, geo, "ARS_222").quad_spectra([a.copy() for a in k0], [a.copy() for a in z0])
+    ora = O.OracleSolver(r, geo, "ARS_222")
+    qref = ora.quad_spectra([a.copy() for a in k0], [a.copy() for a in z0])
+    ora.transform_dtype = np.float32
+    q32 = ora.quad_spectra([a.copy() for a in k0], [a.copy() for a in z0])
     with solver_mod.Solver(r, NX, NY, NZ, Lx, Ly) as g:
         g.set_state(k0, z0)
         g.set_precision("mixed")
@@ -55,35 +62,56 @@ def test_mixed_products_match_oracle(solver_mod, case):
     wn = np.zeros(q.shape[2], bool)
     wn[: NY // 2] = True
     wn[q.shape[2] - (NY // 2 - 1):] = True
-    worst = max(rel(q[i][:, wn], qref[i][:, wn]) for i in range(q.shape[0]))
-    worst64 = max(rel(q64[i][:, wn], qref[i][:, wn]) for i in range(q.shape[0]))
-    print("FP32 chain worst rel L2", worst, "FP64", worst64, flush=True)
-    assert worst <= 2e-6
-    assert worst > 10 * worst64          # the FP32 chain really ran
-    assert worst64 <= 1e-10
+    for i in range(q.shape[0]):
+        assert np.linalg.norm(qref[i][:, wn]) > 0
+        e, e32, e64 = (rel(x[i][:, wn], qref[i][:, wn]) for x in (q, q32, q64))
+        print(f"quad {i}: FP32 chain {e:.3e}, oracle FP32 {e32:.3e}, FP64 {e64:.3e}", flush=True)
+        assert e <= 2.0 * e32
+        assert e > 10 * e64          # the FP32 chain really ran
+        assert e64 <= 1e-10
 
 
-def test_mixed_steps_close_to_oracle(solver_mod):
-    """Five ARS_222 steps of the 64^2 x 33 case in mixed precision: the state
-    stays within 1e-7 (relative L2) of the FP64 oracle; with one refinement
-    sweep per step the same."""
-    r = product_recipe("coral.equations.rrbc_ra5e7_ek1e-4")
-    NX, NZ, dt, L = 64, 33, 2e-5, 2 * np.pi
-    k0, z0 = ic.rbc_initial_state(r, NX, NX, NZ, 1e-2, 10 * NX, 8, 11)
-    ora = O.OracleSolver(r, O.Geometry(NX, NX, NZ, L, L), "ARS_222")
-    ora.field, ora.zfield = [a.copy() for a in k0], [a.copy() for a in z0]
-    ora.run(dt, 5)
+STEP_CASES = [
+    # recipe, NX, NZ, dt, amplitude of u, v, w and t, steps
+    ("coral.equations.rbc_ra1e5", 64, 33, 1e-4, 1e-2, 8),
+    ("coral.equations.rrbc_ra5e7_ek1e-4", 64, 33, 2e-5, 1e-4, 6),
+]
+
+
+@pytest.mark.parametrize("case", STEP_CASES, ids=[c[0].split(".")[-1] for c in STEP_CASES])
+def test_mixed_steps_close_to_oracle(solver_mod, case):
+    """Mixed-precision steps against the FP64 oracle: the state's error is at
+    most twice that of the oracle stepped with its FP32 transform chain, with
+    and without a refinement sweep per step.  The cases stay well inside their
+    CFL limit (the oracle's CFL time step is over 100 times dt), where the
+    FP32 error of the NL terms accumulates linearly; past that limit any
+    rounding difference is amplified by the integration itself."""
+    name, NX, NZ, dt, amp, nsteps = case
+    r = product_recipe(name)
+    L = 2 * np.pi
+    k0, z0 = seeded_state(r, NX, NX, NZ, 10 * NX, 8, 11, amps=(("u", amp), ("v", amp), ("w", amp), ("t", amp)))
+    ref = {}
+    for dtype in (np.float64, np.float32):
+        ora = O.OracleSolver(r, O.Geometry(NX, NX, NZ, L, L), "ARS_222")
+        ora.transform_dtype = dtype
+        ora.field, ora.zfield = [a.copy() for a in k0], [a.copy() for a in z0]
+        ora.run(dt, nsteps)
+        ref[dtype] = (ora.field, ora.zfield, ora.cfl_dt)
+    kref, zref, cfl = ref[np.float64]
+    assert cfl > 100 * dt
+    e32 = (rel(ref[np.float32][0][0], kref[0]), max(rel(a, b) for a, b in zip(ref[np.float32][1], zref)))
     errs = []
     for refine in (0, 1):
         with solver_mod.Solver(r, NX, NX, NZ, L, L) as g:
             g.set_state(k0, z0)
             g.set_precision("mixed", refine)
-            g.march_forward(dt, 5)
+            g.march_forward(dt, nsteps)
             k, z = g.get_state()
-        errs.append((rel(k[0], ora.field[0]), max(rel(a, b) for a, b in zip(z, ora.zfield))))
-    print("mixed 5 steps rel L2 (kxky, mean) without / with refinement:", errs, flush=True)
+        errs.append((rel(k[0], kref[0]), max(rel(a, b) for a, b in zip(z, zref))))
+    print("oracle FP32 chain (kxky, mean):", e32, flush=True)
+    print(f"mixed {nsteps} steps rel L2 (kxky, mean) without / with refinement:", errs, flush=True)
     for ek, ez in errs:
-        assert ek <= 1e-7 and ez <= 1e-7
+        assert ek <= 2.0 * e32[0] and ez <= 2.0 * e32[1]
     assert errs[0][0] > 1e-13
 
 

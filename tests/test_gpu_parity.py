@@ -9,7 +9,7 @@ import numpy as np
 import pytest
 import scipy.linalg.lapack as lap
 
-from conftest import GOLDEN, product_recipe, recipe
+from conftest import GOLDEN, product_recipe, recipe, seeded_state
 from coral_amd import ic
 from oracle import coral_oracle as O
 
@@ -359,7 +359,8 @@ def test_zfiltered_quadratics_match_oracle(solver_mod, NX, NY, NZ):
     the filtered spectra differ from the unfiltered ones in row 0 only."""
     r = _zfiltered_recipe()
     r0 = product_recipe("coral.equations.rbc_ra1e5")
-    k0, z0 = ic.rbc_initial_state(r, NX, NY, NZ, 1e-2, 6, 6, 7)
+    # velocities and temperature seeded: every product has a nonzero row 0
+    k0, z0 = seeded_state(r, NX, NY, NZ, 6, 6, 7, amps=(("u", 1e-2), ("v", 1e-2), ("w", 1e-2), ("t", 1e-2)))
     rng = np.random.default_rng(NZ)
     z0 = [a + 1e-2 * rng.normal(size=a.shape) for a in z0]
     geo = O.Geometry(NX, NY, NZ, 2 * np.pi, 2 * np.pi)
@@ -651,3 +652,43 @@ def test_config1_200_steps_graph_replays(solver_mod):
             print("steps", 50 * (block + 1), "rel L2", e, flush=True)
             assert e <= TOL
             assert all(rel(a, b) <= TOL for a, b in zip(z, ora.zfield))
+
+
+def test_kx0_projection_matches_oracle(solver_mod):
+    """coral_amd_set_kx0_projection: from a state whose kx = 0 plane carries an
+    anti-Hermitian part, five steps with the projection match the oracle's
+    projected steps (<= 1e-10), leave the plane exactly Hermitian, and give the
+    same physical fields as steps without it."""
+    r = product_recipe("coral.equations.rrbc_ra5e7_ek1e-4")
+    NX, NY, NZ, dt = 32, 24, 17, 2e-5
+    geo = O.Geometry(NX, NY, NZ, 2 * np.pi, 3.0)
+    k0, z0 = seeded_state(r, NX, NY, NZ, 8, 6, 4, amps=(("u", 1e-2), ("v", 1e-2), ("w", 1e-2), ("t", 1e-2)))
+    rng = np.random.default_rng(12)
+    noise = 1e-2 * (rng.normal(size=k0[0][0].shape) + 1j * rng.normal(size=k0[0][0].shape))
+    noise[geo.dealias[0]] = 0.0     # kept modes only, (0, 0) excluded, as an upload keeps them
+    noise[0] = 0.0
+    k0[0][0] += noise
+    ora = O.OracleSolver(r, geo, "ARS_222")
+    ora.project_kx0 = True
+    ora.field, ora.zfield = [a.copy() for a in k0], [a.copy() for a in z0]
+    ora.run(dt, 5)
+    phys = []
+    for proj in (True, False):
+        with solver_mod.Solver(r, NX, NY, NZ, 2 * np.pi, 3.0) as g:
+            g.set_state(k0, z0)
+            g.set_kx0_projection(proj)
+            g.march_forward(dt, 5)
+            k, z = g.get_state()
+            phys.append([g.physical(i) for i in range(4)])
+            if proj:
+                print("projected 5 steps vs oracle:", rel(k[0], ora.field[0]), flush=True)
+                assert rel(k[0], ora.field[0]) <= 1e-10
+                for a, b in zip(z, ora.zfield):
+                    assert rel(a, b) <= 1e-10
+                ky = geo.ky_int
+                for iy in range(geo.nky):
+                    if ky[iy] > 0 and not geo.dealias[0, iy]:
+                        jy = int(np.nonzero(ky == -ky[iy])[0][0])
+                        assert np.array_equal(k[0][0, jy], np.conj(k[0][0, iy]))
+    for a, b in zip(phys[0], phys[1]):
+        assert rel(a, b) <= 1e-12

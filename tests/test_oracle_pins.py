@@ -384,6 +384,28 @@ def test_transform_round_trip():
     np.testing.assert_allclose(O.phys_to_spec(O.spec_to_phys(S, geo), geo), S, atol=1e-13)
 
 
+def test_fp32_transform_chain_is_single_precision():
+    """The oracle's FP32 transform chain (the yardstick of the mixed-precision
+    step): the brute-force dealiased product of the test below, computed with
+    complex64 spectra, stays within a few FP32 ulps of the exact product, well
+    above FP64 rounding (so it really ran in single precision), and its round
+    trip returns the spectra to FP32 accuracy."""
+    geo = O.Geometry(4, 4, 5, 2 * np.pi, 2 * np.pi)
+    rng = np.random.default_rng(5)
+    A, B = _random_spectral(geo, rng), _random_spectral(geo, rng)
+    exact = O.phys_to_spec(O.spec_to_phys(A, geo) * O.spec_to_phys(B, geo), geo)
+    pa, pb = O.spec_to_phys(A.astype(np.complex64), geo), O.spec_to_phys(B.astype(np.complex64), geo)
+    assert pa.dtype == np.float32
+    got = O.phys_to_spec(pa * pb, geo)
+    assert got.dtype == np.complex64
+    keep = ~geo.dealias
+    err = np.linalg.norm(got[keep] - exact[keep]) / np.linalg.norm(exact[keep])
+    assert 1e-9 < err < 1e-6
+    S = _random_spectral(geo, np.random.default_rng(4)).astype(np.complex64)
+    rt = O.phys_to_spec(O.spec_to_phys(S, geo), geo)
+    assert np.linalg.norm(rt[keep] - S[keep]) / np.linalg.norm(S[keep]) < 1e-6
+
+
 def test_dealiased_product_equals_truncated_convolution():
     """Brute force on a tiny grid: the 3/2-padded pseudo-spectral product of two
     band-limited fields equals their exact product truncated to the kept modes."""
@@ -440,3 +462,31 @@ def test_initial_condition_properties():
     norm = math.sqrt(np.sum(np.abs(s[0]) ** 2) + 2 * np.sum(np.abs(s[1:]) ** 2))
     assert abs(norm - 1e-3) < 1e-15
     assert all(not z.any() for z in z1)
+
+
+def test_kx0_projection_keeps_physical_fields():
+    """The Hermitian projection of the kx = 0 plane (OracleSolver.project_kx0)
+    leaves the physical fields unchanged and the plane exactly Hermitian: the
+    anti-Hermitian part it removes is invisible to the c2r transform."""
+    r = recipe("rbc_vanilla.coral.equations")
+    NX, NY, NZ = 16, 12, 13
+    geo = O.Geometry(NX, NY, NZ, 2 * np.pi, 3.0)
+    ora = O.OracleSolver(r, geo, "ARS_222")
+    k0, z0 = ic.rbc_initial_state(r, NX, NY, NZ, 1e-2, 6, 6, 3)
+    rng = np.random.default_rng(8)
+    k0[0][0] += 1e-2 * (rng.normal(size=k0[0][0].shape) + 1j * rng.normal(size=k0[0][0].shape))
+    k0[0][geo.dealias] = 0.0
+    k0[0][0, 0] = 0.0
+    ora.field, ora.zfield = [k0[0].copy()], [a.copy() for a in z0]
+    before = ora.physical_full_variables()
+    f = O.project_kx0_plane(k0[0].copy(), geo)
+    ora.field = [f]
+    after = ora.physical_full_variables()
+    for a, b in zip(after, before):
+        np.testing.assert_allclose(a, b, atol=1e-15 * max(1.0, np.abs(b).max()))
+    ky = geo.ky_int
+    for iy in range(geo.nky):
+        if ky[iy] > 0 and not geo.dealias[0, iy]:
+            jy = int(np.nonzero(ky == -ky[iy])[0][0])
+            assert np.array_equal(f[0, jy], np.conj(f[0, iy]))
+    assert not np.allclose(f[0], k0[0][0])

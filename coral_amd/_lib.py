@@ -64,7 +64,8 @@ EXPORTS = (
     "coral_amd_timers_enable", "coral_amd_timer_read", "coral_amd_assembled_operator", "coral_amd_use_graphs", "coral_amd_products",
     "coral_amd_factor_kind", "coral_amd_get_variable_physical", "coral_amd_set_variable_physical",
     "coral_amd_horizontal_averages", "coral_amd_factor_blocks", "coral_amd_system_solve",
-    "coral_amd_set_precision", "coral_amd_set_kx0_projection",
+    "coral_amd_set_precision", "coral_amd_set_kx0_projection", "coral_amd_banded_solve_noalloc",
+    "coral_amd_banded_solve_noalloc_bytes",
 )
 
 
@@ -101,8 +102,11 @@ def _load():
                                          ctypes.POINTER(c_int32)]
     lib.coral_amd_assembled_operator.argtypes = [ctypes.POINTER(Problem), c_int32, c_int32, c_int32, c_int32, P,
                                                  P, P, P]
+    lib.coral_amd_banded_solve_noalloc.argtypes = [c_int32, c_int32, c_int32, c_int32, c_int32, P, P, P, P]
+    lib.coral_amd_banded_solve_noalloc_bytes.argtypes = [c_int32, c_int32, c_int32, c_int32]
     for name in EXPORTS:
         getattr(lib, name).restype = ctypes.c_int
+    lib.coral_amd_banded_solve_noalloc_bytes.restype = ctypes.c_size_t
     return lib
 
 

@@ -28,6 +28,7 @@
 #ifndef CORAL_AMD_H
 #define CORAL_AMD_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -209,6 +210,30 @@ int coral_amd_fwd_transform(coral_amd_ctx *ctx, const double *phys, double *spec
  * step; other bands through a generic one-wavefront-per-system solve. */
 int coral_amd_banded_solve(int32_t device, int32_t n, int32_t kl, int32_t ku, int32_t nsys,
                            const double *ab, double *x, int32_t *info);
+
+/* Allocation-free variant of coral_amd_banded_solve.  It has no counterpart in
+ * the reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  ab, x and info are DEVICE pointers on `device`: ab (LAPACK band
+ * storage, as for coral_amd_banded_solve) is overwritten with its LU factors,
+ * as zgbtrf overwrites AB; x holds the right-hand sides on entry and the
+ * solutions on return; info[s] receives the LAPACK info of system s.  The work
+ * space (multipliers, U records and pivots; coral_amd_banded_solve_noalloc_bytes
+ * gives its size) lives in one fixed device array of
+ * CORAL_AMD_NOALLOC_SCRATCH_BYTES per GPU, held until the process ends; a
+ * batch that needs more is rejected with an error (split it).  The work is
+ * enqueued on `stream` (a hipStream_t, NULL for the null stream) and the call
+ * returns without waiting for it.  The first call on a device looks up the
+ * array's device address (make that call outside graph capture); no call
+ * allocates.  Every call on one GPU uses the same array, so calls whose work
+ * overlaps in time on one GPU race.  The solve is the one-wavefront-per-system
+ * kernel, so results equal coral_amd_banded_solve's to rounding. */
+#define CORAL_AMD_NOALLOC_SCRATCH_BYTES (64u * 1024u * 1024u)
+int coral_amd_banded_solve_noalloc(int32_t device, int32_t n, int32_t kl, int32_t ku, int32_t nsys, double *ab,
+                                   double *x, int32_t *info, void *stream);
+/* Bytes of work space coral_amd_banded_solve_noalloc needs for a batch (0 for
+ * invalid dimensions). */
+size_t coral_amd_banded_solve_noalloc_bytes(int32_t n, int32_t kl, int32_t ku, int32_t nsys);
 
 /* Solve (M - dt*a11*L) x = rhs for every kept wavenumber of kxky system isys
  * with the factors of the last coral_amd_set_dt, through the factor layout and

@@ -124,12 +124,196 @@ def cpu_baseline(wl, recipe, k0, z0, nsteps=1):
     return nsteps / dt, t_setup, dt
 
 
+def horizontal_spectrum(k_state, positions, NX, NY, Lx, Ly):
+    """Energy of the state's Galerkin coefficients of the given positions (e.g.
+    the velocity variables) summed per horizontal wavenumber shell
+    |k| in [m, m+1) (integer shells of k / (2 pi / Lx)); kx > 0 counted twice
+    (the conjugate half of the spectrum)."""
+    import numpy as np
+    from coral_amd.ic import wavenumber_indices
+    kx, ky, kept = wavenumber_indices(NX, NY)
+    e = np.sum(np.abs(k_state[:, :, positions]) ** 2, axis=2)
+    e = np.where(kept, e, 0.0) * np.where(kx[:, None] > 0, 2.0, 1.0)
+    kk = np.sqrt((kx[:, None] * 2 * np.pi / Lx) ** 2 + (ky[None, :] * 2 * np.pi / Ly) ** 2) / (2 * np.pi / Lx)
+    shell = np.floor(kk + 0.5).astype(int)
+    return np.bincount(shell.ravel(), weights=e.ravel())
+
+
+def bench_ensemble(args):
+    """Config 3: 48 independent Rayleigh-Benard runs drawn from a seed (NX = NY
+    in {64, 96, 128}, NZ in {33, 49, 65}, Ra log-uniform in [1e5, 1e7], Pr in
+    {0.7, 1, 7}, each with its own operators and dt), stepped as one ensemble
+    (one HIP graph launch per step, tuned launch shapes and branches), against
+    the same runs stepped one after another on their own (each with its own
+    step graphs).  Every run's state after the ensemble steps is compared with
+    its solo state."""
+    import numpy as np
+    from coral_amd import ensemble as E
+    from coral_amd.ic import rbc_initial_state
+    from coral_amd.solver import Solver
+    nrun, seed = 48, 3
+    steps = args.steps if args.steps is not None else 200
+    specs = E.draw_runs(nrun, seed)
+    t0 = time.perf_counter()
+    runs = []
+    for sp in specs:
+        r = E.rbc_recipe(sp.Ra, sp.Pr)
+        k0, z0 = rbc_initial_state(r, sp.NX, sp.NX, sp.NZ, 1e-4, 8, 6, sp.seed)
+        g = Solver(r, sp.NX, sp.NX, sp.NZ, 2 * math.pi, 2 * math.pi)
+        g.set_state(k0, z0)
+        g.factorize_operators(sp.dt)
+        runs.append((g, sp, k0, z0))
+        if len(runs) % 8 == 0:
+            log(f"ensemble: {len(runs)} runs created ({time.perf_counter() - t0:.1f} s)")
+    # solo: each run alone, step graphs, one after another
+    solo_ms, solo_state = [], []
+    for g, sp, k0, z0 in runs:
+        g.use_graphs(True)
+        g.march_forward(sp.dt, 2)
+        g.set_state(k0, z0)
+        g.synchronize()
+        t0 = time.perf_counter()
+        g.march_forward(sp.dt, steps)
+        g.synchronize()
+        solo_ms.append(1e3 * (time.perf_counter() - t0) / steps)
+        solo_state.append(g.get_state())
+        g.use_graphs(False)
+        g.set_state(k0, z0)
+    solo_total = sum(solo_ms)
+    log(f"solo runs one after another: {solo_total:.3f} ms per step of all runs")
+    with E.Ensemble([g for g, _, _, _ in runs]) as ens:
+        t0 = time.perf_counter()
+        nb, tuned_ms = ens.tune(6)
+        log(f"tuner: {nb} branches, {tuned_ms:.3f} ms per ensemble step, {time.perf_counter() - t0:.1f} s")
+        ens.step(2)
+        for g, _, k0, z0 in runs:
+            g.set_state(k0, z0)
+        ens.synchronize()
+        t0 = time.perf_counter()
+        ens.step(steps)
+        ens.synchronize()
+        elapsed = time.perf_counter() - t0
+    worst, finite = 0.0, True
+    for (g, _, _, _), (ks, zs) in zip(runs, solo_state):
+        k, z = g.get_state()
+        for a, b in zip(k + z, ks + zs):
+            finite = finite and bool(np.isfinite(a).all())
+            worst = max(worst, float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)))
+    shapes = {}
+    for g, sp, _, _ in runs:
+        shapes.setdefault(f"{sp.NX}x{sp.NZ}", []).append(list(g.launch_shape()))
+    value = steps / elapsed
+    sim = sum(sp.dt for _, sp, _, _ in runs)
+    out = {
+        "metric": "ensemble steps/s (one step of every run), 48 uneven Rayleigh-Benard runs, FP64",
+        "value": round(value, 3), "unit": "ensemble steps/s", "n_gpus": 1, "steps": steps,
+        "ms_per_step": round(1e3 * elapsed / steps, 4), "higher_is_better": True, "dtype": "f64",
+        "data": "synthetic (seeded perturbations)",
+        "config": {"workload": f"{nrun} RBC runs drawn from seed {seed}: NX=NY in {{64,96,128}}, NZ in {{33,49,65}}, "
+                               "Ra log-uniform [1e5,1e7], Pr in {0.7,1,7}, dt = 5e-5 (Ra Pr/1e5)^-1/2, ARS_222",
+                   "runs": [[sp.NX, sp.NZ, round(sp.Ra, 1), sp.Pr, sp.dt] for _, sp, _, _ in runs],
+                   "branches": nb, "launch_shapes": shapes},
+        "run_steps_per_s": round(nrun * value, 1),
+        "solo_sequential": {"ms_per_step_all_runs": round(solo_total, 4), "ensemble_steps_per_s": round(1e3 / solo_total, 3),
+                            "per_run_ms": [round(x, 4) for x in solo_ms]},
+        "speedup_vs_solo": round(value * solo_total / 1e3, 3),
+        "sim_time_per_s": {"ensemble": round(value * sim, 6), "solo_sequential": round(1e3 / solo_total * sim, 6)},
+        "parity_vs_solo": {"steps": steps, "worst_rel_l2": worst, "finite": finite},
+    }
+    print(json.dumps(out), flush=True)
+
+
+def bench_mixed(args):
+    """Config 4: the rotating 512^2 x 257 workload (Ek = 3e-5, Ra = 2e8, Pr = 1,
+    dt = 5e-6, 300 steps) in FP64 and in mixed precision (FP32 transforms and
+    products, FP64 solves, one FP64 refinement sweep per step) from the same
+    seeded state: sim-time/s of both, and the mixed run's errors against the
+    FP64 run in kinetic energy (the volume average of uu + vv + ww, Coral's
+    KE_display), the Nusselt number (compute_nusselt.py's wall estimator on
+    the mean temperature) and the horizontal spectrum of the velocity state."""
+    import numpy as np
+    from coral_amd import diagnostics as D
+    from coral_amd.equations import read_equations
+    from coral_amd.ic import interleaved_positions, rbc_initial_state
+    from coral_amd.solver import Solver
+    wl = WORKLOADS["rrbc512"]
+    steps = args.steps if args.steps is not None else 300
+    # amplitude: the perturbation grows about 4x per 25 steps at this Ra and dt;
+    # from 1e-8 the CFL time step falls below dt near step 250, from 1e-11 the
+    # 300 steps stay inside it (the final CFL time step is reported)
+    amp = args.amp if args.amp is not None else 1e-11
+    recipe = read_equations(os.path.join(ROOT, "coral_amd", "inputs", wl["eq"]))
+    NX, NZ, dt = wl["NX"], wl["NZ"], wl["dt"]
+    k0, z0 = rbc_initial_state(recipe, NX, NX, NZ, amp, wl["kmax"], wl["nmodes"], wl["seed"])
+    s = recipe.kxky_systems[0]
+    sizes = [NZ - bc // 10 for bc in s.bc_codes]
+    vel = np.concatenate([interleaved_positions(len(sizes), sizes, s.var_names.index(v)) for v in ("u", "v", "w")])
+    tsys = [i for i, z in enumerate(recipe.zero_systems) if "tMean" in z.var_names][0]
+    tvar = recipe.zero_systems[tsys].var_names.index("tMean")
+    res = {}
+    with Solver(recipe, NX, NX, NZ, 2 * math.pi, 2 * math.pi) as g:
+        diag = D.Diagnostics(g)
+        for mode, refine in (("fp64", 0), ("mixed", 0), ("mixed_refined", 1)):
+            g.set_precision("mixed" if mode != "fp64" else "fp64", refine)
+            g.use_graphs(True)
+            g.set_state(k0, z0)
+            g.factorize_operators(dt)
+            g.march_forward(dt, 2)
+            g.set_state(k0, z0)
+            g.synchronize()
+            t0 = time.perf_counter()
+            g.march_forward(dt, steps)
+            g.synchronize()
+            el = time.perf_counter() - t0
+            _, ke = diag.sample()
+            prof = g.variable_physical("zero", tsys, tvar)
+            nu = D.nusselt_wall_differences(prof, g.gap)
+            k, _ = g.get_state()
+            spec = horizontal_spectrum(k[0], vel, NX, NX, 2 * math.pi, 2 * math.pi)
+            res[mode] = dict(elapsed=el, ke=ke, nu=nu, spec=spec, cfl=g.cfl_dt(), finite=bool(np.isfinite(k[0]).all()))
+            log(f"{mode}: {steps} steps in {el:.3f} s, KE {ke:.6e}, Nu (top, bottom) {nu}, CFL dt {res[mode]['cfl']:.3e}")
+            g.use_graphs(False)
+            # per-kernel figures of 10 eager steps
+            g.timers_enable(True)
+            g.march_forward(dt, 10)
+            g.synchronize()
+            res[mode]["kernels"] = {n: round(ms / 10, 3) for n, (ms, c) in g.timers().items()}
+            g.timers_enable(False)
+            log(f"{mode} kernels (ms per step): {res[mode]['kernels']}")
+    a, b, ar = res["mixed"], res["fp64"], res["mixed_refined"]
+
+    def errs(x):
+        return {"kinetic_energy_rel": abs(x["ke"] / b["ke"] - 1),
+                "nusselt_rel": max(abs(p / q - 1) for p, q in zip(x["nu"], b["nu"])),
+                "horizontal_spectrum_rel_l2": float(np.linalg.norm(x["spec"] - b["spec"]) / np.linalg.norm(b["spec"]))}
+    out = {
+        "metric": "sim-time/s, mixed precision vs plain FP64 GPU, 512²×257 rotating RBC",
+        "value": round(steps * dt / a["elapsed"], 8), "unit": "simulated time units/s", "n_gpus": 1, "steps": steps,
+        "ms_per_step": round(1e3 * a["elapsed"] / steps, 4), "higher_is_better": True, "dtype": "f32 transforms, f64 solves",
+        "data": "synthetic (seeded perturbation)",
+        "config": {"workload": wl["name"] + f", {steps} steps", "ic_amplitude": amp, "refine": 0},
+        "fp64": {"sim_time_per_s": round(steps * dt / b["elapsed"], 8), "ms_per_step": round(1e3 * b["elapsed"] / steps, 4),
+                 "kernels_ms_per_step": b["kernels"]},
+        "speedup_vs_fp64": round(b["elapsed"] / a["elapsed"], 4),
+        "errors_vs_fp64": errs(a),
+        "kernels_ms_per_step": a["kernels"],
+        "mixed_refined": {"sim_time_per_s": round(steps * dt / ar["elapsed"], 8),
+                          "ms_per_step": round(1e3 * ar["elapsed"] / steps, 4),
+                          "speedup_vs_fp64": round(b["elapsed"] / ar["elapsed"], 4), "errors_vs_fp64": errs(ar)},
+        "values": {"ke_fp64": b["ke"], "ke_mixed": a["ke"], "nu_fp64": b["nu"], "nu_mixed": a["nu"],
+                   "cfl_dt_fp64": b["cfl"], "finite": a["finite"] and b["finite"] and ar["finite"]},
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--workload", default="rrbc256", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="rrbc256", choices=sorted(WORKLOADS) + ["ensemble48", "mixed512"],
+                    help="ensemble48: 48 uneven runs as one ensemble against solo runs; mixed512: mixed precision "
+                         "against FP64 at 512^2 x 257 (sim-time/s and KE, Nu, spectrum errors)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--eager-only", action="store_true",
                     help="skip the second timed pass of the same steps replayed as HIP graphs")
@@ -157,6 +341,10 @@ def main():
         print("bench.py: this build runs the whole domain on one GPU; multi-GPU runs are not part of it "
               "(the systems would shard by kx slabs, each transform with one all-to-all).", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "ensemble48":
+        return bench_ensemble(args)
+    if args.workload == "mixed512":
+        return bench_mixed(args)
     wl = WORKLOADS[args.workload]
     steps = args.steps if args.steps is not None else wl["steps"]
 
@@ -229,6 +417,26 @@ def main():
         graphs_line = {"value": round(steps / gel, 4), "ms_per_step": round(1e3 * gel / steps, 4), "steps": steps,
                        "state_finite": bool(np.isfinite(kg[0]).all())}
         log(f"{steps} steps as HIP graph replays in {gel:.3f} s")
+
+    value_check = None
+    if args.workload == "rrbc256" and not args.graphs:
+        # the step's work does not depend on the values: the same 20 eager
+        # steps from the stated amplitude 1e-2 (finite through step ~39) and
+        # from the headline's amplitude
+        vc = {}
+        for a_ in (1e-2, amp):
+            ka, za = rbc_initial_state(recipe, NX, NX, NZ, a_, wl["kmax"], wl["nmodes"], wl["seed"])
+            g.set_state(ka, za)
+            g.synchronize()
+            t0 = time.perf_counter()
+            g.march_forward(wl["dt"], 20)
+            g.synchronize()
+            kc, _ = g.get_state()
+            vc[a_] = (1e3 * (time.perf_counter() - t0) / 20, bool(np.isfinite(kc[0]).all()))
+        value_check = {"steps": 20, "launch": "eager, no events",
+                       "amp_1e-2": {"ms_per_step": round(vc[1e-2][0], 4), "state_finite": vc[1e-2][1]},
+                       f"amp_{amp:g}": {"ms_per_step": round(vc[amp][0], 4), "state_finite": vc[amp][1]}}
+        log(f"value check: 20 steps at amplitude 1e-2 {vc[1e-2][0]:.3f} ms/step, at {amp:g} {vc[amp][0]:.3f} ms/step")
 
     P = dict(N=total, kl=kl, ku=ku, nsys=nsys, real=g.factor_is_real(0), blocks=g.factor_blocks(0),
              u_entries=g.u_entries_per_column(0), NZ=NZ,
@@ -311,6 +519,7 @@ def main():
         "kernels_ms_per_step": {k: round(v["ms_per_step"], 4) for k, v in per_kernel.items()},
         "kernels": kernels,
         "hip_graphs": graphs_line,
+        "value_independence": value_check,
     }
     if "published_s_per_step" in wl:
         # Coral's own timing of this case (other hardware: 20 Skylake cores, MPI)

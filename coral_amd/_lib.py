@@ -65,7 +65,9 @@ EXPORTS = (
     "coral_amd_factor_kind", "coral_amd_get_variable_physical", "coral_amd_set_variable_physical",
     "coral_amd_horizontal_averages", "coral_amd_factor_blocks", "coral_amd_system_solve",
     "coral_amd_set_precision", "coral_amd_set_kx0_projection", "coral_amd_banded_solve_noalloc",
-    "coral_amd_banded_solve_noalloc_bytes",
+    "coral_amd_banded_solve_noalloc_bytes", "coral_amd_set_launch_shape", "coral_amd_get_launch_shape",
+    "coral_amd_ensemble_create", "coral_amd_ensemble_destroy", "coral_amd_ensemble_set_schedule",
+    "coral_amd_ensemble_tune", "coral_amd_ensemble_step", "coral_amd_ensemble_synchronize",
 )
 
 
@@ -104,6 +106,14 @@ def _load():
                                                  P, P, P]
     lib.coral_amd_banded_solve_noalloc.argtypes = [c_int32, c_int32, c_int32, c_int32, c_int32, P, P, P, P]
     lib.coral_amd_banded_solve_noalloc_bytes.argtypes = [c_int32, c_int32, c_int32, c_int32]
+    lib.coral_amd_set_launch_shape.argtypes = [P, c_int32, c_int32]
+    lib.coral_amd_get_launch_shape.argtypes = [P, P, P]
+    lib.coral_amd_ensemble_create.argtypes = [P, c_int32, PP]
+    lib.coral_amd_ensemble_destroy.argtypes = [P]
+    lib.coral_amd_ensemble_set_schedule.argtypes = [P, c_int32, P]
+    lib.coral_amd_ensemble_tune.argtypes = [P, c_int32, P, P]
+    lib.coral_amd_ensemble_step.argtypes = [P, c_int32]
+    lib.coral_amd_ensemble_synchronize.argtypes = [P]
     for name in EXPORTS:
         getattr(lib, name).restype = ctypes.c_int
     lib.coral_amd_banded_solve_noalloc_bytes.restype = ctypes.c_size_t

@@ -149,6 +149,18 @@ struct coral_amd_ctx {
     // replace the kx = 0 plane of every kxky state by its Hermitian part after
     // each step (coral_amd_set_kx0_projection)
     bool project_kx0 = false;
+    // launch shapes (coral_amd_set_launch_shape; picked by the ensemble's tuner):
+    // wavefronts per workgroup of the x pass (0 = the largest that fits), and
+    // the z transforms as vector GEMMs instead of MFMA products
+    int xpass_wpb = 0;
+    bool z_plain = false;
+    // incremented whenever the launches of a step change (dt, precision,
+    // projection, launch shapes): graphs that hold this run's step (its own or
+    // an ensemble's) are stale when it differs from their capture's
+    unsigned long long gen = 0;
+    // buffer swaps since creation (never reset): its parity is the state /
+    // last-stage buffer assignment, which captured graphs bake in
+    unsigned long long nswap = 0;
     DevBuf<float2> spec32, Fz32, G32, H32, Hy32, Qhat32, twx32, twy32;
     DevBuf<float> zfrag32_b, zfrag32_f;
     ZFrags32Dev z32{};
@@ -162,6 +174,8 @@ struct coral_amd_ctx {
     hipGraphExec_t gexec[2] = {nullptr, nullptr};
     int gpar = 0;
     void drop_graphs() {
+        // an executable graph is destroyed only once its launches have ended
+        if ((gexec[0] || gexec[1]) && stream) (void)hipStreamSynchronize(stream);
         for (auto &g : gexec)
             if (g) {
                 (void)hipGraphExecDestroy(g);
@@ -532,12 +546,12 @@ void rhs(coral_amd_ctx &c, SysDev &d, const std::vector<std::pair<const double2 
 // z transforms: MFMA kernels where a variant covers the shape, plain GEMM otherwise
 void zbwd(coral_amd_ctx &c, const double2 *spec, double2 *Fz, int nb) {
     const Problem &P = c.P;
-    if (!launch_zbwd_mfma(c.stream, c.zdev, spec, Fz, P.NZ, P.NZAA, P.nsys, nb))
+    if (c.z_plain || !launch_zbwd_mfma(c.stream, c.zdev, spec, Fz, P.NZ, P.NZAA, P.nsys, nb))
         launch_zbwd(c.stream, c.Tz.p, spec, Fz, P.NZAA, P.NZ, P.nsys, nb);
 }
 void zfwd(coral_amd_ctx &c, const double2 *Hy, double2 *Qhat, int nb) {
     const Problem &P = c.P;
-    if (!launch_zfwd_mfma(c.stream, c.zdev, Hy, Qhat, P.NZ, P.NZAA, P.nsys, nb))
+    if (c.z_plain || !launch_zfwd_mfma(c.stream, c.zdev, Hy, Qhat, P.NZ, P.NZAA, P.nsys, nb))
         launch_zfwd(c.stream, c.Cz.p, Hy, Qhat, P.NZ, P.NZAA, P.nsys, nb);
 }
 
@@ -599,6 +613,7 @@ void swap_step_buffers(coral_amd_ctx &c) {
     for (auto &d : c.ks) std::swap(d->field.p, d->X[s - 1]->p);
     for (auto &d : c.zs) std::swap(d->field.p, d->X[s - 1]->p);
     c.gpar ^= 1;
+    ++c.nswap;
 }
 
 // f's launches (and timer events) go to the side stream
@@ -695,6 +710,7 @@ void products_to_Qhat(coral_amd_ctx &c, const std::vector<const double2 *> &kst,
     xa.qb = c.qb.p;
     xa.zweight = c.zweight.p;
     xa.cflmax = c.cflmax.p;
+    xa.max_wpb = c.xpass_wpb;
     c.timed("xpass", [&] { launch_xpass(c.stream, c.px, xa); });
     if (fp32) {
         c.timed("yfwd", [&] {
@@ -1086,7 +1102,8 @@ int coral_amd_set_dt(coral_amd_ctx *c, double dt) {
         if (!c) throw std::runtime_error("null context");
         if (!(dt > 0.0) || !std::isfinite(dt)) throw std::runtime_error("dt must be positive and finite");
         HIP_OK(hipSetDevice(c->device));
-        c->drop_graphs();   // the factors may be reallocated and dt enters the launches
+        c->drop_graphs();
+        ++c->gen;   // the factors may be reallocated and dt enters the launches
         for (auto &d : c->ks) factorize(*c, *d, dt);
         for (auto &d : c->zs) factorize(*c, *d, dt);
         c->dt = dt;
@@ -1866,6 +1883,7 @@ int coral_amd_set_precision(coral_amd_ctx *c, int32_t mode, int32_t refine) {
                         d->Fcor.alloc(size_t(d->nsys) * d->t->total);
                     }
         c->drop_graphs();
+        ++c->gen;
         c->precision = mode;
         c->refine = refine;
         return 0;
@@ -1880,6 +1898,7 @@ int coral_amd_set_kx0_projection(coral_amd_ctx *c, int32_t enable) {
         HIP_OK(hipSetDevice(c->device));
         HIP_OK(hipStreamSynchronize(c->stream));
         c->drop_graphs();
+        ++c->gen;
         c->project_kx0 = enable != 0;
         return 0;
     } catch (const std::exception &e) {
@@ -1933,6 +1952,430 @@ int coral_amd_timer_read(coral_amd_ctx *c, int32_t i, char *name, int32_t name_l
         return int(c->totals.size());
     } catch (const std::exception &e) {
         return fail(e);
+    }
+}
+
+
+int coral_amd_set_launch_shape(coral_amd_ctx *c, int32_t xpass_wpb, int32_t z_plain) {
+    try {
+        if (!c) throw std::runtime_error("null context");
+        if (xpass_wpb < 0 || xpass_wpb > 4) throw std::runtime_error("x pass wavefronts per workgroup must be 0..4");
+        HIP_OK(hipSetDevice(c->device));
+        HIP_OK(hipStreamSynchronize(c->stream));
+        c->drop_graphs();
+        ++c->gen;
+        c->xpass_wpb = xpass_wpb;
+        c->z_plain = z_plain != 0;
+        return 0;
+    } catch (const std::exception &e) {
+        return fail(e);
+    }
+}
+
+int coral_amd_get_launch_shape(coral_amd_ctx *c, int32_t *xpass_wpb, int32_t *z_plain) {
+    if (!c) {
+        g_err = "null context";
+        return -1;
+    }
+    if (xpass_wpb) *xpass_wpb = c->xpass_wpb;
+    if (z_plain) *z_plain = c->z_plain ? 1 : 0;
+    return 0;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Ensembles: independent runs (each its own context, operators, dt and state)
+// stepped together.  One step of every run is one HIP graph: the runs are
+// spread over nbranch parallel branches (a fork from the ensemble's stream,
+// each branch a stream of its own, joined at the end), so the small kernels
+// of different runs overlap on the device; inside a branch its runs step one
+// after the other.  Each run's step swaps its state and last-stage buffers,
+// so two graphs alternate, as for a single run.
+// ---------------------------------------------------------------------------
+struct coral_amd_ensemble {
+    std::vector<coral_amd_ctx *> runs;
+    int device = 0;
+    hipStream_t master = nullptr;
+    std::vector<hipStream_t> branch;
+    std::vector<int> assign;          // run -> branch
+    int nbranch = 1;
+    hipEvent_t ev_fork = nullptr, ev_done = nullptr;
+    std::vector<hipEvent_t> ev_join, ev_run;
+    // per branch, two graphs (one per buffer assignment) of one step of the
+    // branch's runs; each branch replays on its own stream
+    std::vector<hipGraphExec_t> gexec[2];
+    int epar = 0;                     // graph to replay next
+    std::vector<unsigned long long> cap_nswap;   // each run's swap count after the capture
+    std::vector<unsigned long long> cap_gen;
+    std::vector<double> run_ms;       // per-run step time measured by the tuner (0 = not measured)
+    double tuned_ms = 0.0;            // ensemble step time of the tuned schedule
+
+    void drop() {
+        // an executable graph is destroyed only once its launches have ended
+        if (!gexec[0].empty()) {
+            for (auto &b : branch) (void)hipStreamSynchronize(b);
+            if (master) (void)hipStreamSynchronize(master);
+        }
+        for (auto &v : gexec) {
+            for (auto &g : v)
+                if (g) (void)hipGraphExecDestroy(g);
+            v.clear();
+        }
+        epar = 0;
+    }
+    ~coral_amd_ensemble() {
+        drop();
+        for (auto &b : branch) (void)hipStreamDestroy(b);
+        for (auto &ev : ev_join) (void)hipEventDestroy(ev);
+        for (auto &ev : ev_run) (void)hipEventDestroy(ev);
+        if (ev_fork) (void)hipEventDestroy(ev_fork);
+        if (ev_done) (void)hipEventDestroy(ev_done);
+        if (master) (void)hipStreamDestroy(master);
+    }
+};
+
+namespace {
+
+void ens_set_branches(coral_amd_ensemble &e, int nb, const std::vector<int> &assign) {
+    e.drop();
+    while (int(e.branch.size()) < nb) {
+        hipStream_t st = nullptr;
+        HIP_OK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        e.branch.push_back(st);
+        hipEvent_t ev = nullptr;
+        HIP_OK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+        e.ev_join.push_back(ev);
+    }
+    e.nbranch = nb;
+    e.assign = assign;
+}
+
+// Longest-processing-time schedule: runs in decreasing step time, each to the
+// branch with the least time so far.
+std::vector<int> lpt_assign(const std::vector<double> &ms, int nb) {
+    std::vector<int> order(ms.size()), assign(ms.size(), 0);
+    for (size_t i = 0; i < order.size(); ++i) order[i] = int(i);
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return ms[a] > ms[b]; });
+    std::vector<double> load(nb, 0.0);
+    for (int r : order) {
+        const int b = int(std::min_element(load.begin(), load.end()) - load.begin());
+        assign[r] = b;
+        load[b] += ms[r] > 0.0 ? ms[r] : 1.0;
+    }
+    return assign;
+}
+
+void ens_capture(coral_amd_ensemble &e) {
+    e.drop();
+    for (auto *c : e.runs) {
+        if (c->timing) throw std::runtime_error("ensemble steps need timers off on every run");
+        if (!c->factored) throw std::runtime_error("coral_amd_set_dt must be called on every run before stepping");
+    }
+    for (int p = 0; p < 2; ++p) {
+        e.gexec[p].assign(e.nbranch, nullptr);
+        for (int b = 0; b < e.nbranch; ++b) {
+            hipStream_t st = e.branch[b];
+            HIP_OK(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed));
+            hipGraph_t g = nullptr;
+            try {
+                for (size_t r = 0; r < e.runs.size(); ++r) {
+                    if (e.assign[r] != b) continue;
+                    coral_amd_ctx &c = *e.runs[r];
+                    hipStream_t own = c.stream;
+                    c.stream = st;
+                    try {
+                        one_step(c);
+                    } catch (...) {
+                        c.stream = own;
+                        throw;
+                    }
+                    c.stream = own;
+                }
+            } catch (...) {
+                (void)hipStreamEndCapture(st, &g);
+                if (g) (void)hipGraphDestroy(g);
+                e.drop();
+                throw;
+            }
+            HIP_OK(hipStreamEndCapture(st, &g));
+            const hipError_t err = hipGraphInstantiate(&e.gexec[p][b], g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (err != hipSuccess) {
+                e.drop();
+                throw HipError(std::string("hipGraphInstantiate: ") + hipGetErrorString(err));
+            }
+        }
+    }
+    // each capture swapped every run's buffers once: they are back, parity too
+    e.cap_nswap.resize(e.runs.size());
+    e.cap_gen.resize(e.runs.size());
+    for (size_t r = 0; r < e.runs.size(); ++r) {
+        e.cap_nswap[r] = e.runs[r]->nswap;
+        e.cap_gen[r] = e.runs[r]->gen;
+    }
+    e.epar = 0;
+}
+
+// one ensemble step: every branch's graph on its own stream, forked from and
+// joined into the ensemble's stream
+void ens_launch(coral_amd_ensemble &e) {
+    HIP_OK(hipEventRecord(e.ev_fork, e.master));
+    for (int b = 0; b < e.nbranch; ++b) {
+        HIP_OK(hipStreamWaitEvent(e.branch[b], e.ev_fork, 0));
+        HIP_OK(hipGraphLaunch(e.gexec[e.epar][b], e.branch[b]));
+        HIP_OK(hipEventRecord(e.ev_join[b], e.branch[b]));
+        HIP_OK(hipStreamWaitEvent(e.master, e.ev_join[b], 0));
+    }
+    for (auto *c : e.runs) swap_step_buffers(*c);
+    e.epar ^= 1;
+}
+
+// the captured graphs still hold every run's current launches and buffer assignment
+bool ens_valid(const coral_amd_ensemble &e) {
+    if (e.gexec[0].empty()) return false;
+    for (size_t r = 0; r < e.runs.size(); ++r)
+        if (e.runs[r]->gen != e.cap_gen[r] || int((e.runs[r]->nswap - e.cap_nswap[r]) & 1) != e.epar) return false;
+    return true;
+}
+
+// nsteps ensemble steps on the ensemble's stream, ordered after each run's
+// queued work and before anything queued on the runs afterwards
+void ens_steps(coral_amd_ensemble &e, int nsteps) {
+    for (size_t r = 0; r < e.runs.size(); ++r) {
+        HIP_OK(hipEventRecord(e.ev_run[r], e.runs[r]->stream));
+        HIP_OK(hipStreamWaitEvent(e.master, e.ev_run[r], 0));
+    }
+    for (int n = 0; n < nsteps; ++n) {
+        if (!ens_valid(e)) ens_capture(e);
+        ens_launch(e);
+    }
+    HIP_OK(hipEventRecord(e.ev_done, e.master));
+    for (auto *c : e.runs) HIP_OK(hipStreamWaitEvent(c->stream, e.ev_done, 0));
+}
+
+// device copies of every system's state, to run tuning steps and put the state back
+struct StateBackup {
+    std::vector<std::unique_ptr<DevBuf<double2>>> bufs;
+    void save(coral_amd_ctx &c) {
+        bufs.clear();
+        for (auto *v : {&c.ks, &c.zs})
+            for (auto &d : *v) {
+                bufs.emplace_back(new DevBuf<double2>());
+                const size_t n = size_t(d->nsys) * d->t->total;
+                bufs.back()->alloc(n);
+                HIP_OK(hipMemcpyAsync(bufs.back()->p, d->field.p, n * sizeof(double2), hipMemcpyDeviceToDevice,
+                                      c.stream));
+            }
+        HIP_OK(hipStreamSynchronize(c.stream));
+    }
+    void restore(coral_amd_ctx &c) {
+        HIP_OK(hipStreamSynchronize(c.stream));
+        size_t k = 0;
+        for (auto *v : {&c.ks, &c.zs})
+            for (auto &d : *v) {
+                const size_t n = size_t(d->nsys) * d->t->total;
+                HIP_OK(hipMemcpyAsync(d->field.p, bufs[k++]->p, n * sizeof(double2), hipMemcpyDeviceToDevice,
+                                      c.stream));
+            }
+        HIP_OK(hipStreamSynchronize(c.stream));
+    }
+};
+
+// milliseconds per step of `steps` graph replays of one run (an even count
+// leaves its buffer assignment where it was)
+double time_run_steps(coral_amd_ctx &c, int steps) {
+    const bool g0 = c.use_graphs;
+    c.use_graphs = true;
+    c.drop_graphs();
+    capture_step_graphs(c);
+    for (int n = 0; n < 2; ++n) {   // warm-up
+        HIP_OK(hipGraphLaunch(c.gexec[c.gpar], c.stream));
+        swap_step_buffers(c);
+    }
+    hipEvent_t a = nullptr, b = nullptr;
+    HIP_OK(hipEventCreate(&a));
+    HIP_OK(hipEventCreate(&b));
+    HIP_OK(hipEventRecord(a, c.stream));
+    for (int n = 0; n < steps; ++n) {
+        HIP_OK(hipGraphLaunch(c.gexec[c.gpar], c.stream));
+        swap_step_buffers(c);
+    }
+    HIP_OK(hipEventRecord(b, c.stream));
+    HIP_OK(hipEventSynchronize(b));
+    float ms = 0.f;
+    HIP_OK(hipEventElapsedTime(&ms, a, b));
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    c.drop_graphs();
+    c.use_graphs = g0;
+    return double(ms) / steps;
+}
+
+double time_ens_steps(coral_amd_ensemble &e, int steps) {
+    ens_steps(e, 2);   // capture and warm-up
+    hipEvent_t a = nullptr, b = nullptr;
+    HIP_OK(hipEventCreate(&a));
+    HIP_OK(hipEventCreate(&b));
+    HIP_OK(hipEventRecord(a, e.master));
+    for (int n = 0; n < steps; ++n) ens_launch(e);
+    HIP_OK(hipEventRecord(b, e.master));
+    HIP_OK(hipEventSynchronize(b));
+    float ms = 0.f;
+    HIP_OK(hipEventElapsedTime(&ms, a, b));
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    return double(ms) / steps;
+}
+
+}  // namespace
+
+extern "C" {
+
+int coral_amd_ensemble_create(coral_amd_ctx *const *runs, int32_t nrun, coral_amd_ensemble **out) {
+    try {
+        if (!runs || !out || nrun < 1) throw std::runtime_error("null argument or empty ensemble");
+        std::unique_ptr<coral_amd_ensemble> e(new coral_amd_ensemble());
+        for (int r = 0; r < nrun; ++r) {
+            if (!runs[r]) throw std::runtime_error("null run context");
+            for (int q = 0; q < r; ++q)
+                if (runs[q] == runs[r]) throw std::runtime_error("a context appears twice in the ensemble");
+            if (runs[r]->device != runs[0]->device) throw std::runtime_error("every run must be on one device");
+            e->runs.push_back(runs[r]);
+        }
+        e->device = runs[0]->device;
+        HIP_OK(hipSetDevice(e->device));
+        HIP_OK(hipStreamCreateWithFlags(&e->master, hipStreamNonBlocking));
+        HIP_OK(hipEventCreateWithFlags(&e->ev_fork, hipEventDisableTiming));
+        HIP_OK(hipEventCreateWithFlags(&e->ev_done, hipEventDisableTiming));
+        e->ev_run.resize(nrun, nullptr);
+        for (auto &ev : e->ev_run) HIP_OK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+        e->run_ms.assign(nrun, 0.0);
+        ens_set_branches(*e, 1, std::vector<int>(nrun, 0));
+        *out = e.release();
+        return 0;
+    } catch (const std::exception &e) {
+        return fail(e);
+    }
+}
+
+int coral_amd_ensemble_destroy(coral_amd_ensemble *e) {
+    try {
+        if (!e) return 0;
+        HIP_OK(hipSetDevice(e->device));
+        HIP_OK(hipStreamSynchronize(e->master));
+        delete e;
+        return 0;
+    } catch (const std::exception &ex) {
+        return fail(ex);
+    }
+}
+
+int coral_amd_ensemble_set_schedule(coral_amd_ensemble *e, int32_t nbranch, const int32_t *assign) {
+    try {
+        if (!e) throw std::runtime_error("null ensemble");
+        if (nbranch < 1 || nbranch > int(e->runs.size())) throw std::runtime_error("nbranch must be 1..nrun");
+        HIP_OK(hipSetDevice(e->device));
+        HIP_OK(hipStreamSynchronize(e->master));
+        std::vector<int> a(e->runs.size());
+        if (assign) {
+            for (size_t r = 0; r < a.size(); ++r) {
+                if (assign[r] < 0 || assign[r] >= nbranch) throw std::runtime_error("branch index out of range");
+                a[r] = assign[r];
+            }
+        } else {
+            a = lpt_assign(e->run_ms, nbranch);
+        }
+        ens_set_branches(*e, nbranch, a);
+        return 0;
+    } catch (const std::exception &ex) {
+        return fail(ex);
+    }
+}
+
+int coral_amd_ensemble_step(coral_amd_ensemble *e, int32_t nsteps) {
+    try {
+        if (!e) throw std::runtime_error("null ensemble");
+        if (nsteps < 0) throw std::runtime_error("nsteps must be >= 0");
+        HIP_OK(hipSetDevice(e->device));
+        ens_steps(*e, nsteps);
+        HIP_OK(hipGetLastError());
+        return 0;
+    } catch (const std::exception &ex) {
+        return fail(ex);
+    }
+}
+
+int coral_amd_ensemble_synchronize(coral_amd_ensemble *e) {
+    try {
+        if (!e) throw std::runtime_error("null ensemble");
+        HIP_OK(hipSetDevice(e->device));
+        HIP_OK(hipStreamSynchronize(e->master));
+        return 0;
+    } catch (const std::exception &ex) {
+        return fail(ex);
+    }
+}
+
+int coral_amd_ensemble_tune(coral_amd_ensemble *e, int32_t trial_steps, int32_t *nbranch_out, double *ms_out) {
+    try {
+        if (!e) throw std::runtime_error("null ensemble");
+        const int steps = std::max(2, (trial_steps + 1) / 2 * 2);
+        HIP_OK(hipSetDevice(e->device));
+        HIP_OK(hipStreamSynchronize(e->master));
+        const size_t nr = e->runs.size();
+        std::vector<StateBackup> saved(nr);
+        for (size_t r = 0; r < nr; ++r) {
+            coral_amd_ctx &c = *e->runs[r];
+            if (!c.factored) throw std::runtime_error("coral_amd_set_dt must be called on every run before tuning");
+            if (c.timing) throw std::runtime_error("tuning needs timers off on every run");
+            HIP_OK(hipStreamSynchronize(c.stream));
+            saved[r].save(c);
+        }
+        // launch shapes of each run: x pass wavefronts per workgroup and the
+        // z transforms' arithmetic (MFMA products or vector GEMMs)
+        for (size_t r = 0; r < nr; ++r) {
+            coral_amd_ctx &c = *e->runs[r];
+            double best = 1e300;
+            int bw = 0, bz = 0;
+            for (int z = 0; z < 2; ++z)
+                for (int w : {4, 2, 1}) {
+                    c.xpass_wpb = w;
+                    c.z_plain = z != 0;
+                    ++c.gen;
+                    const double ms = time_run_steps(c, steps);
+                    if (ms < best) {
+                        best = ms;
+                        bw = w;
+                        bz = z;
+                    }
+                }
+            c.xpass_wpb = bw;
+            c.z_plain = bz != 0;
+            ++c.gen;
+            e->run_ms[r] = best;
+            saved[r].restore(c);
+        }
+        // branches: LPT schedules over 1, 2, 4, ... branches
+        double best = 1e300;
+        int bnb = 1;
+        for (int nb = 1; nb <= int(nr) && nb <= 16; nb *= 2) {
+            ens_set_branches(*e, nb, lpt_assign(e->run_ms, nb));
+            const double ms = time_ens_steps(*e, steps);
+            if (ms < best) {
+                best = ms;
+                bnb = nb;
+            }
+        }
+        HIP_OK(hipStreamSynchronize(e->master));
+        ens_set_branches(*e, bnb, lpt_assign(e->run_ms, bnb));
+        e->tuned_ms = best;
+        for (size_t r = 0; r < nr; ++r) saved[r].restore(*e->runs[r]);
+        if (nbranch_out) *nbranch_out = bnb;
+        if (ms_out) *ms_out = best;
+        return 0;
+    } catch (const std::exception &ex) {
+        return fail(ex);
     }
 }
 

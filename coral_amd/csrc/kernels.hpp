@@ -130,6 +130,7 @@ struct XPassArgs {
     const int *qa, *qb;  // quadratic factors (-1 = ONE)
     const double *zweight;
     unsigned long long *cflmax;  // [3] bit patterns of non-negative doubles
+    int max_wpb;                 // wavefront kernels: at most this many wavefronts per workgroup (0 = 4)
 };
 void launch_xpass(hipStream_t st, const FFTPlan &px, const XPassArgs &a);
 // wavefront-per-line x pass for N in {96, 144, 192, 384, 768} and up to 6 full

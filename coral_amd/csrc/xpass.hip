@@ -502,7 +502,7 @@ static bool launch_wave(hipStream_t st, const C *tw, const XPassArgs &a) {
     using X = XShape<N, NPI, XGP>;
     const long long nlines = (long long)a.NZAA * a.NYAA;
     const size_t per_wave = size_t(X::SMAX) * N * sizeof(C);
-    int wpb = 4;
+    int wpb = a.max_wpb >= 1 && a.max_wpb <= 4 ? a.max_wpb : 4;
     while (wpb > 1 && wpb * per_wave > 65536) wpb /= 2;
     if (wpb * per_wave > 65536) return false;
     const long long per_block = (long long)wpb * X::LPW;

@@ -301,6 +301,17 @@ class Solver:
         supercritical runs."""
         check(lib.coral_amd_set_kx0_projection(self._ctx, int(bool(on))), "coral_amd_set_kx0_projection")
 
+    def set_launch_shape(self, xpass_wpb: int = 0, z_plain: bool = False):
+        """Launch shapes (``coral_amd_set_launch_shape``): x pass wavefronts per
+        workgroup (0 = most that fit) and vector-GEMM z transforms."""
+        check(lib.coral_amd_set_launch_shape(self._ctx, int(xpass_wpb), int(bool(z_plain))),
+              "coral_amd_set_launch_shape")
+
+    def launch_shape(self) -> Tuple[int, bool]:
+        w, z = ctypes.c_int32(), ctypes.c_int32()
+        check(lib.coral_amd_get_launch_shape(self._ctx, ctypes.byref(w), ctypes.byref(z)), "coral_amd_get_launch_shape")
+        return w.value, bool(z.value)
+
     def use_graphs(self, on: bool = True):
         """Replay steps as HIP graphs (captured at the next step); results are
         the eager ones bit for bit."""

@@ -291,6 +291,42 @@ int coral_amd_set_kx0_projection(coral_amd_ctx *ctx, int32_t enable);
  * plane_layer_main.f90:157).  While timers are enabled steps run eagerly. */
 int coral_amd_use_graphs(coral_amd_ctx *ctx, int32_t enable);
 
+/* Launch shapes of a run (no counterpart in the reference): xpass_wpb is the
+ * largest number of wavefronts per workgroup of the x pass (1..4; 0 = the most
+ * that fit its LDS), z_plain = 1 runs the z transforms as vector GEMMs
+ * instead of MFMA products (results equal to rounding).  The ensemble tuner
+ * sets them per run. */
+int coral_amd_set_launch_shape(coral_amd_ctx *ctx, int32_t xpass_wpb, int32_t z_plain);
+int coral_amd_get_launch_shape(coral_amd_ctx *ctx, int32_t *xpass_wpb, int32_t *z_plain);
+
+/* Ensembles of independent runs (no counterpart in the reference, which runs
+ * one problem per executable).  Each run is a context of its own (its recipe,
+ * grid, operators, dt and state; contexts stay owned by the caller, must
+ * outlive the ensemble and keep working on their own).  The runs are spread
+ * over nbranch branches; each branch's step (its runs one after the other) is
+ * one captured HIP graph replayed on a stream of its own, so one
+ * coral_amd_ensemble_step advances every run by one time step with nbranch
+ * graph launches, and the branches overlap on the device.  Each run's results
+ * are bit for bit those of its own coral_amd_step with the same launch
+ * shapes.  The graphs are recaptured when a run's dt, precision, projection or
+ * launch shapes change or when a run was stepped on its own.  All runs must be
+ * on one device and have timers off. */
+typedef struct coral_amd_ensemble coral_amd_ensemble;
+int coral_amd_ensemble_create(coral_amd_ctx *const *runs, int32_t nrun, coral_amd_ensemble **out);
+int coral_amd_ensemble_destroy(coral_amd_ensemble *ens);
+/* nbranch parallel branches; assign[r] the branch of run r (NULL: longest
+ * step first onto the least loaded branch, by the tuner's step times). */
+int coral_amd_ensemble_set_schedule(coral_amd_ensemble *ens, int32_t nbranch, const int32_t *assign);
+/* The tuner: for every run, each launch shape (x pass 4, 2 or 1 wavefronts per
+ * workgroup; z transforms on MFMA or as vector GEMMs) timed over trial_steps
+ * graph replays, the fastest kept; then 1, 2, 4, ... 16 branches with
+ * longest-first schedules, timed as ensemble steps, the fastest kept.  Every
+ * run's state is restored afterwards.  Returns the branch count and the
+ * ensemble's milliseconds per step. */
+int coral_amd_ensemble_tune(coral_amd_ensemble *ens, int32_t trial_steps, int32_t *nbranch, double *ms_per_step);
+int coral_amd_ensemble_step(coral_amd_ensemble *ens, int32_t nsteps);
+int coral_amd_ensemble_synchronize(coral_amd_ensemble *ens);
+
 /* Per-kernel timing with HIP events on the context's stream.  enable=1 resets
  * and starts accumulating; names are filled by coral_amd_timer_read. */
 int coral_amd_timers_enable(coral_amd_ctx *ctx, int32_t enable);

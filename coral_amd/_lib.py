@@ -68,6 +68,9 @@ EXPORTS = (
     "coral_amd_banded_solve_noalloc_bytes", "coral_amd_set_launch_shape", "coral_amd_get_launch_shape",
     "coral_amd_ensemble_create", "coral_amd_ensemble_destroy", "coral_amd_ensemble_set_schedule",
     "coral_amd_ensemble_tune", "coral_amd_ensemble_step", "coral_amd_ensemble_synchronize",
+    "coral_amd_nn_gemm", "coral_amd_nn_cast_bf16", "coral_amd_nn_bias_act", "coral_amd_nn_gelu_bwd",
+    "coral_amd_nn_colsum_work", "coral_amd_nn_colsum", "coral_amd_nn_mode_scale", "coral_amd_nn_mode_scale_grad",
+    "coral_amd_nn_axpy", "coral_amd_nn_mse_partials", "coral_amd_nn_mse", "coral_amd_nn_adam",
 )
 
 
@@ -117,6 +120,8 @@ def _load():
     for name in EXPORTS:
         getattr(lib, name).restype = ctypes.c_int
     lib.coral_amd_banded_solve_noalloc_bytes.restype = ctypes.c_size_t
+    lib.coral_amd_nn_colsum_work.restype = ctypes.c_int64
+    lib.coral_amd_nn_mse_partials.restype = ctypes.c_int64
     return lib
 
 

@@ -31,7 +31,7 @@ HIP_FLAGS = [
 HOST_FLAGS = ["-O2", "-fPIC", "-std=c++17", "-Wall"]
 
 HIP_SOURCES = ["kernels.hip", "xpass.hip", "ypass.hip", "ztransform.hip", "capi.hip", "ztransform32.hip",
-               "noalloc.hip"]
+               "noalloc.hip", "nn.hip"]
 HOST_SOURCES = ["assembly.cpp"]
 
 

@@ -1,0 +1,324 @@
+"""Learned subgrid closure: a per-z-level spectral-convolution network that
+corrects the quadratic variables of a coarse run, trained on the GPU with the
+library's BF16-MFMA / FP32-accumulate kernels (``coral_amd_nn_*``, nn.hip).
+
+The reference Coral has no closure: a coarse Coral run is the plain method at
+that resolution.  Here the network maps the resolved full variables on one
+horizontal plane of the coarse grid (u, v, w, t and the height z) to the
+subgrid part of every quadratic variable on that plane,
+
+    tau_q = filter(F1 F2) - filter(F1) filter(F2),
+
+the difference between the filtered product of a fine run and the product of
+its filtered fields, and the coarse step adds its spectrum to the quadratic
+variables before the NL terms are assembled (``coral_amd_set_closure``).
+
+Network, per plane of N2 = NXAA NYAA points (rows of a plane contiguous):
+
+    H = X W0 + b0                                    lift, Cin -> width
+    4 times:  H <- H + gelu(H W1 + b1) W2 + b2       pointwise MLP, width -> 4 width -> width
+                     + B ((B^T H) * D)               spectral convolution
+    Y = H Wp + bp                                    projection, width -> Cout
+
+B holds the horizontal Fourier modes |kx|, |ky| <= M of the plane as
+orthonormal cosine and sine columns, so B^T H is the plane's low-mode
+spectrum per channel and D scales each (mode, channel).  Inputs and targets
+are normalised per channel with the training set's statistics, which travel
+with the weights.  Every GEMM reads BF16 operands and accumulates in FP32;
+master weights, biases, activations' pre-images and Adam's moments are FP32.
+
+Device memory comes from coral_amd.devmem (the HIP runtime the library
+links); every arithmetic operation on the device is a kernel of
+libcoral_amd.so, on the null stream.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+from dataclasses import dataclass, field
+from typing import Dict, List, Optional, Tuple
+
+import numpy as np
+
+from ._lib import check, lib
+from .devmem import DevArray
+
+c_int32, c_int64, c_float, c_void_p = ctypes.c_int32, ctypes.c_int64, ctypes.c_float, ctypes.c_void_p
+
+
+def _bind():
+    P = c_void_p
+    sig = {
+        "coral_amd_nn_gemm": [c_int32, c_int32, c_int32, c_int32, c_int32, c_float, P, c_int32, c_int64, P, c_int32,
+                              c_int64, c_float, P, c_int32, c_int64, c_int32, P],
+        "coral_amd_nn_cast_bf16": [P, P, c_int64, P],
+        "coral_amd_nn_bias_act": [P, P, c_int32, c_int64, c_int32, P, P, P],
+        "coral_amd_nn_gelu_bwd": [P, P, P, P, c_int64, P],
+        "coral_amd_nn_colsum_work": [c_int64, c_int32],
+        "coral_amd_nn_colsum": [P, c_int64, c_int32, P, P, c_float, P],
+        "coral_amd_nn_mode_scale": [P, P, P, P, c_int32, c_int32, P],
+        "coral_amd_nn_mode_scale_grad": [P, P, P, c_int32, c_int32, c_float, P],
+        "coral_amd_nn_axpy": [P, P, c_float, c_int64, P, P],
+        "coral_amd_nn_mse_partials": [c_int64],
+        "coral_amd_nn_mse": [P, P, P, P, c_float, c_int64, P, P],
+        "coral_amd_nn_adam": [P, P, P, P, P, c_int64, c_float, c_float, c_float, c_float, c_int32, P],
+    }
+    for name, args in sig.items():
+        f = getattr(lib, name)
+        f.argtypes = args
+        f.restype = c_int64 if name.endswith(("_work", "_partials")) else ctypes.c_int
+
+
+_bind()
+
+
+def fourier_basis(NXAA: int, NYAA: int, M: int) -> np.ndarray:
+    """Orthonormal real Fourier columns of a periodic NYAA x NXAA plane (point
+    i = y NXAA + x): cos and sin of kx x + ky y for the half plane of
+    (kx, ky) with |kx|, |ky| <= M (kx > 0, or kx = 0 and ky >= 0; no sine at
+    (0, 0)).  Returns (N2, nb)."""
+    x = 2 * np.pi * np.arange(NXAA) / NXAA
+    y = 2 * np.pi * np.arange(NYAA) / NYAA
+    Y, X = np.meshgrid(y, x, indexing="ij")
+    cols = []
+    for kx in range(0, M + 1):
+        for ky in range(-M, M + 1):
+            if kx == 0 and ky < 0:
+                continue
+            ph = (kx * X + ky * Y).ravel()
+            c = np.cos(ph)
+            cols.append(c / np.linalg.norm(c))
+            if not (kx == 0 and ky == 0):
+                s = np.sin(ph)
+                cols.append(s / np.linalg.norm(s))
+    return np.stack(cols, axis=1)
+
+
+@dataclass
+class Layout:
+    """Shapes of the parameters, in the order of the flat parameter buffer."""
+    cin: int
+    cout: int
+    width: int = 64
+    hidden: int = 256
+    layers: int = 4
+    nb: int = 0
+    names: List[Tuple[str, Tuple[int, ...]]] = field(default_factory=list)
+
+    def __post_init__(self):
+        W, Hd = self.width, self.hidden
+        self.names = [("W0", (self.cin, W)), ("b0", (W,))]
+        for l in range(self.layers):
+            self.names += [(f"W1.{l}", (W, Hd)), (f"b1.{l}", (Hd,)), (f"W2.{l}", (Hd, W)), (f"b2.{l}", (W,)),
+                           (f"D.{l}", (self.nb, W))]
+        self.names += [("Wp", (W, self.cout)), ("bp", (self.cout,))]
+
+    def offsets(self) -> Dict[str, Tuple[int, Tuple[int, ...]]]:
+        out, off = {}, 0
+        for n, shp in self.names:
+            out[n] = (off, shp)
+            off += int(np.prod(shp))
+        return out
+
+    @property
+    def size(self) -> int:
+        return sum(int(np.prod(s)) for _, s in self.names)
+
+
+def init_params(layout: Layout, seed: int) -> np.ndarray:
+    """Flat FP32 initial parameters: He-normal lift and MLP weights, the
+    second MLP weight and the projection scaled down (the residual branches
+    start near zero), zero biases, spectral scales of 1 / (2 width)."""
+    rng = np.random.default_rng(seed)
+    flat = np.zeros(layout.size, np.float32)
+    for n, (off, shp) in layout.offsets().items():
+        k = int(np.prod(shp))
+        if n.startswith("W0") or n.startswith("W1"):
+            v = rng.normal(size=k) * math.sqrt(2.0 / shp[0])
+        elif n.startswith("W2") or n.startswith("Wp"):
+            v = rng.normal(size=k) * math.sqrt(1.0 / shp[0]) * 0.1
+        elif n.startswith("D"):
+            v = np.full(k, 1.0 / (2 * layout.width))
+        else:
+            v = np.zeros(k)
+        flat[off:off + k] = v
+    return flat
+
+
+class _Work:
+    """Device buffers of one batch size P (activations kept for the backward
+    pass and its scratch), allocated once and reused."""
+
+    def __init__(self, L: Layout, N2: int, P: int):
+        W, Hd, nb, R = L.width, L.hidden, L.nb, P * N2
+        self.P, self.R = P, R
+        self.H = DevArray(R * W)
+        self.H16 = [DevArray(R * W, "bf16") for _ in range(L.layers + 1)]
+        self.Z = [DevArray(R * Hd) for _ in range(L.layers)]
+        self.A16 = [DevArray(R * Hd, "bf16") for _ in range(L.layers)]
+        self.U = DevArray(R * W)
+        self.Cm = [DevArray(P * nb * W) for _ in range(L.layers)]
+        self.Cs16 = DevArray(P * nb * W, "bf16")
+        self.Y = DevArray(R * L.cout)
+        self.dY = DevArray(R * L.cout)
+        self.dY16 = DevArray(R * L.cout, "bf16")
+        self.part = DevArray(int(lib.coral_amd_nn_mse_partials(R * L.cout)))
+        self.dH = DevArray(R * W)
+        self.dH16 = DevArray(R * W, "bf16")
+        self.dA = DevArray(R * Hd)
+        self.dZ16 = DevArray(R * Hd, "bf16")
+        self.dCs = DevArray(P * nb * W)
+        self.dCm16 = DevArray(P * nb * W, "bf16")
+        self.wpart = DevArray(P * max(Hd * W, L.cin * W, W * L.cout))
+        self.work = DevArray(max(int(lib.coral_amd_nn_colsum_work(R, max(Hd, W, L.cout))),
+                                 int(lib.coral_amd_nn_colsum_work(P, Hd * W))))
+
+
+class ClosureNet:
+    """The network on the GPU: forward, backward and Adam on device buffers
+    (the null stream, which the library's other kernels use too)."""
+
+    def __init__(self, layout: Layout, basis: np.ndarray, params: Optional[np.ndarray] = None, seed: int = 0):
+        self.L = layout
+        self.N2, nb = basis.shape
+        assert nb == layout.nb
+        self.basis32 = DevArray.from_numpy(basis)
+        self.basis16 = DevArray(basis.size, "bf16")
+        self._cast(self.basis32, self.basis16)
+        p = init_params(layout, seed) if params is None else np.asarray(params, np.float32)
+        self.w = DevArray.from_numpy(p)
+        self.w16 = DevArray(layout.size, "bf16")
+        self._cast(self.w, self.w16)
+        self.g = DevArray(layout.size, zero=True)
+        self.m = DevArray(layout.size, zero=True)
+        self.v = DevArray(layout.size, zero=True)
+        self.t = 0
+        self.off = layout.offsets()
+        self.work: Dict[int, _Work] = {}
+        self.last = None
+
+    # --- helpers -------------------------------------------------------------
+    stream = None
+
+    def _cast(self, x, y, n=None):
+        check(lib.coral_amd_nn_cast_bf16(x.ptr, y.ptr, x.n if n is None else n, None), "nn cast")
+
+    def _p(self, name, t):
+        return t.at(self.off[name][0])
+
+    def _gemm(self, tA, tB, M, N, K, A, lda, sA, B, ldb, sB, C, ldc, sC, batch=1, beta=0.0, alpha=1.0):
+        check(lib.coral_amd_nn_gemm(tA, tB, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch, None),
+              "nn gemm")
+
+    def _ws(self, P) -> _Work:
+        if P not in self.work:
+            self.work = {P: _Work(self.L, self.N2, P)}   # one batch size at a time
+        return self.work[P]
+
+    def _wgrad(self, A16, lda, B16, ldb, M, N, ws, out_ptr):
+        """out (M x N) = A^T B summed over the P planes of N2 rows each (A, B
+        row-major with P N2 rows): one GEMM per plane into partials, then
+        their sum (a split of the long row dimension over workgroups)."""
+        P = ws.P
+        self._gemm(1, 0, M, N, self.N2, A16.ptr, lda, self.N2 * lda, B16.ptr, ldb, self.N2 * ldb,
+                   ws.wpart.ptr, N, M * N, batch=P)
+        check(lib.coral_amd_nn_colsum(ws.wpart.ptr, P, M * N, out_ptr, ws.work.ptr, 0.0, None), "nn colsum")
+
+    def params(self) -> np.ndarray:
+        return self.w.numpy().copy()
+
+    def param_dict(self, flat: Optional[np.ndarray] = None) -> Dict[str, np.ndarray]:
+        flat = self.params() if flat is None else flat
+        return {n: flat[o:o + int(np.prod(s))].reshape(s).astype(np.float64) for n, (o, s) in self.off.items()}
+
+    # --- forward / backward --------------------------------------------------
+    def forward(self, X16: DevArray, P: int) -> DevArray:
+        """X16: bf16 (P N2, cin) on the device.  Returns Y: f32 (P N2, cout)
+        (a buffer of the workspace, valid until the next call)."""
+        L, N2, W, Hd, nb = self.L, self.N2, self.L.width, self.L.hidden, self.L.nb
+        ws = self._ws(P)
+        R = ws.R
+        self._gemm(0, 0, R, W, L.cin, X16.ptr, L.cin, 0, self._p("W0", self.w16), W, 0, ws.H.ptr, W, 0)
+        check(lib.coral_amd_nn_bias_act(ws.H.ptr, self._p("b0", self.w), W, R * W, 0, None, ws.H16[0].ptr, None),
+              "nn bias")
+        for l in range(L.layers):
+            Hin16, Hout16 = ws.H16[l], ws.H16[l + 1]
+            Z, A16, Cm = ws.Z[l], ws.A16[l], ws.Cm[l]
+            self._gemm(0, 0, R, Hd, W, Hin16.ptr, W, 0, self._p(f"W1.{l}", self.w16), Hd, 0, Z.ptr, Hd, 0)
+            check(lib.coral_amd_nn_bias_act(Z.ptr, self._p(f"b1.{l}", self.w), Hd, R * Hd, 1, None, A16.ptr, None),
+                  "nn bias gelu")
+            self._gemm(0, 0, R, W, Hd, A16.ptr, Hd, 0, self._p(f"W2.{l}", self.w16), W, 0, ws.U.ptr, W, 0)
+            self._gemm(1, 0, nb, W, N2, self.basis16.ptr, nb, 0, Hin16.ptr, W, N2 * W, Cm.ptr, W, nb * W, batch=P)
+            check(lib.coral_amd_nn_mode_scale(Cm.ptr, self._p(f"D.{l}", self.w), None, ws.Cs16.ptr, P, nb * W, None),
+                  "nn mode scale")
+            # H <- H + B Cs (the GEMM accumulates, beta = 1), then + U + b2
+            self._gemm(0, 0, N2, W, nb, self.basis16.ptr, nb, 0, ws.Cs16.ptr, W, nb * W, ws.H.ptr, W, N2 * W,
+                       batch=P, beta=1.0)
+            check(lib.coral_amd_nn_bias_act(ws.U.ptr, self._p(f"b2.{l}", self.w), W, R * W, 0, None, None, None),
+                  "nn bias")
+            check(lib.coral_amd_nn_axpy(ws.H.ptr, ws.U.ptr, 1.0, R * W, Hout16.ptr, None), "nn add")
+        self._gemm(0, 0, R, L.cout, W, ws.H16[L.layers].ptr, W, 0, self._p("Wp", self.w16), L.cout, 0, ws.Y.ptr,
+                   L.cout, 0)
+        check(lib.coral_amd_nn_bias_act(ws.Y.ptr, self._p("bp", self.w), L.cout, R * L.cout, 0, None, None, None),
+              "nn bias")
+        self.last = (P, X16)
+        return ws.Y
+
+    def loss_grad(self, Y: DevArray, T: DevArray) -> float:
+        """MSE sum (Y - T)^2 / n of the last forward pass; its gradient in the workspace."""
+        ws = self._ws(self.last[0])
+        n = Y.n
+        check(lib.coral_amd_nn_mse(Y.ptr, T.ptr, ws.dY.ptr, ws.dY16.ptr, 2.0 / n, n, ws.part.ptr, None), "nn mse")
+        return float(ws.part.numpy().astype(np.float64).sum()) / n
+
+    def backward(self):
+        """Gradients of the last forward pass and loss into self.g (flat, FP32)."""
+        L, N2, W, Hd, nb = self.L, self.N2, self.L.width, self.L.hidden, self.L.nb
+        P, X16 = self.last
+        ws = self._ws(P)
+        R = ws.R
+        g = self.g
+
+        def colsum(x, ncol, name):
+            check(lib.coral_amd_nn_colsum(x.ptr, R, ncol, self._p(name, g), ws.work.ptr, 0.0, None), "nn colsum")
+        # projection
+        self._wgrad(ws.H16[L.layers], W, ws.dY16, L.cout, W, L.cout, ws, self._p("Wp", g))
+        colsum(ws.dY, L.cout, "bp")
+        self._gemm(0, 1, R, W, L.cout, ws.dY16.ptr, L.cout, 0, self._p("Wp", self.w16), L.cout, 0, ws.dH.ptr, W, 0)
+        for l in reversed(range(L.layers)):
+            self._cast(ws.dH, ws.dH16)
+            # MLP branch
+            self._wgrad(ws.A16[l], Hd, ws.dH16, W, Hd, W, ws, self._p(f"W2.{l}", g))
+            colsum(ws.dH, W, f"b2.{l}")
+            self._gemm(0, 1, R, Hd, W, ws.dH16.ptr, W, 0, self._p(f"W2.{l}", self.w16), W, 0, ws.dA.ptr, Hd, 0)
+            check(lib.coral_amd_nn_gelu_bwd(ws.Z[l].ptr, ws.dA.ptr, ws.dA.ptr, ws.dZ16.ptr, R * Hd, None),
+                  "nn gelu bwd")
+            self._wgrad(ws.H16[l], W, ws.dZ16, Hd, W, Hd, ws, self._p(f"W1.{l}", g))
+            colsum(ws.dA, Hd, f"b1.{l}")
+            # spectral branch: dCs = B^T dH per plane; dD = sum_p Cm dCs; dCm = dCs D
+            self._gemm(1, 0, nb, W, N2, self.basis16.ptr, nb, 0, ws.dH16.ptr, W, N2 * W, ws.dCs.ptr, W, nb * W,
+                       batch=P)
+            check(lib.coral_amd_nn_mode_scale_grad(ws.Cm[l].ptr, ws.dCs.ptr, self._p(f"D.{l}", g), P, nb * W, 0.0,
+                                                   None), "nn mode grad")
+            check(lib.coral_amd_nn_mode_scale(ws.dCs.ptr, self._p(f"D.{l}", self.w), None, ws.dCm16.ptr, P, nb * W,
+                                              None), "nn mode scale")
+            # dH (residual) += dZ W1^T + B dCm
+            self._gemm(0, 1, R, W, Hd, ws.dZ16.ptr, Hd, 0, self._p(f"W1.{l}", self.w16), Hd, 0, ws.dH.ptr, W, 0,
+                       beta=1.0)
+            self._gemm(0, 0, N2, W, nb, self.basis16.ptr, nb, 0, ws.dCm16.ptr, W, nb * W, ws.dH.ptr, W, N2 * W,
+                       batch=P, beta=1.0)
+        self._cast(ws.dH, ws.dH16)
+        self._wgrad(X16, L.cin, ws.dH16, W, L.cin, W, ws, self._p("W0", g))
+        colsum(ws.dH, W, "b0")
+
+    def adam_step(self, lr: float, b1: float = 0.9, b2: float = 0.999, eps: float = 1e-8):
+        self.t += 1
+        check(lib.coral_amd_nn_adam(self.w.ptr, self.g.ptr, self.m.ptr, self.v.ptr, self.w16.ptr, self.L.size, lr, b1,
+                                    b2, eps, self.t, None), "nn adam")
+
+    def train_step(self, X16: DevArray, T: DevArray, P: int, lr: float) -> float:
+        Y = self.forward(X16, P)
+        loss = self.loss_grad(Y, T)
+        self.backward()
+        self.adam_step(lr)
+        return loss

@@ -327,6 +327,43 @@ int coral_amd_ensemble_tune(coral_amd_ensemble *ens, int32_t trial_steps, int32_
 int coral_amd_ensemble_step(coral_amd_ensemble *ens, int32_t nsteps);
 int coral_amd_ensemble_synchronize(coral_amd_ensemble *ens);
 
+/* Kernels of the learned subgrid closure (no counterpart in the reference,
+ * which has no closure), on DEVICE pointers, enqueued on `stream` (a
+ * hipStream_t, NULL for the null stream).  BF16 arrays hold 16-bit BF16
+ * patterns; conversions round to nearest even.
+ * coral_amd_nn_gemm: C[b] = alpha op(A[b]) op(B[b]) + beta C[b] for b < batch
+ * on MFMA (BF16 operands, FP32 accumulation and C): op(A) is M x K (A stored M
+ * x K row-major with leading dimension lda, or K x M when transA), op(B) K x N
+ * (B stored K x N, or N x K when transB); sA, sB, sC are the batch strides in
+ * elements (0 shares an operand). */
+int coral_amd_nn_gemm(int32_t transA, int32_t transB, int32_t M, int32_t N, int32_t K, float alpha, const void *A,
+                      int32_t lda, int64_t sA, const void *B, int32_t ldb, int64_t sB, float beta, float *C,
+                      int32_t ldc, int64_t sC, int32_t batch, void *stream);
+int coral_amd_nn_cast_bf16(const float *x, void *y, int64_t n, void *stream);
+/* z += bias (per column of ncol); out = act(z) (act 0: identity, 1: GELU, tanh
+ * form), out16 its BF16 copy (either may be NULL) */
+int coral_amd_nn_bias_act(float *z, const float *bias, int32_t ncol, int64_t n, int32_t act, float *out, void *out16,
+                          void *stream);
+int coral_amd_nn_gelu_bwd(const float *z, const float *da, float *dz, void *dz16, int64_t n, void *stream);
+/* column sums of x [nrow][ncol] into out (out = sums + beta out), deterministic;
+ * work holds coral_amd_nn_colsum_work(nrow, ncol) floats */
+int64_t coral_amd_nn_colsum_work(int64_t nrow, int32_t ncol);
+int coral_amd_nn_colsum(const float *x, int64_t nrow, int32_t ncol, float *out, float *work, float beta, void *stream);
+/* out[p][e] = c[p][e] d[e] for p < P, e < ne; its gradient dd[e] = sum_p c[p][e] g[p][e] (+ beta dd[e]) */
+int coral_amd_nn_mode_scale(const float *c, const float *d, float *out, void *out16, int32_t P, int32_t ne,
+                            void *stream);
+int coral_amd_nn_mode_scale_grad(const float *c, const float *g, float *dd, int32_t P, int32_t ne, float beta,
+                                 void *stream);
+int coral_amd_nn_axpy(float *y, const float *x, float alpha, int64_t n, void *y16, void *stream);
+/* grad = scale (y - t) (and its BF16 copy); partial[b] = sum of (y - t)^2 over
+ * block b of 256 elements (coral_amd_nn_mse_partials(n) blocks) */
+int64_t coral_amd_nn_mse_partials(int64_t n);
+int coral_amd_nn_mse(const float *y, const float *t, float *grad, void *grad16, float scale, int64_t n, float *partial,
+                     void *stream);
+/* Adam with bias-corrected moments at step t (1-based) on FP32 weights; w16 their BF16 copy */
+int coral_amd_nn_adam(float *w, const float *g, float *m, float *v, void *w16, int64_t n, float lr, float b1, float b2,
+                      float eps, int32_t t, void *stream);
+
 /* Per-kernel timing with HIP events on the context's stream.  enable=1 resets
  * and starts accumulating; names are filled by coral_amd_timer_read. */
 int coral_amd_timers_enable(coral_amd_ctx *ctx, int32_t enable);

@@ -1,0 +1,61 @@
+"""The closure network's CPU reference (oracle/closure_ref.py) against finite
+differences of its own loss, and the Fourier basis (no GPU)."""
+import numpy as np
+import pytest
+
+from oracle import closure_ref as C
+
+
+def small_problem(seed=0, P=2, N2=16, nb=5, cin=3, cout=2, width=4, hidden=6, layers=2):
+    rng = np.random.default_rng(seed)
+    p = {"W0": rng.normal(size=(cin, width)) * 0.5, "b0": rng.normal(size=width) * 0.1,
+         "W1": [rng.normal(size=(width, hidden)) * 0.5 for _ in range(layers)],
+         "b1": [rng.normal(size=hidden) * 0.1 for _ in range(layers)],
+         "W2": [rng.normal(size=(hidden, width)) * 0.5 for _ in range(layers)],
+         "b2": [rng.normal(size=width) * 0.1 for _ in range(layers)],
+         "D": [rng.normal(size=(nb, width)) * 0.3 for _ in range(layers)],
+         "Wp": rng.normal(size=(width, cout)) * 0.5, "bp": rng.normal(size=cout) * 0.1}
+    basis = np.linalg.qr(rng.normal(size=(N2, nb)))[0]
+    X = rng.normal(size=(P * N2, cin))
+    T = rng.normal(size=(P * N2, cout))
+    return p, basis, X, T
+
+
+def test_reference_gradients_match_finite_differences():
+    """Central differences of the float64 loss (no BF16 rounding) against
+    every parameter block's analytic gradient."""
+    p, basis, X, T = small_problem()
+    P = 2
+    _, g, _ = C.loss_and_grads(p, X, T, basis, P, round_bf16=False)
+    rng = np.random.default_rng(3)
+    h = 1e-6
+    for key in ("W0", "b0", "Wp", "bp", "W1", "b1", "W2", "b2", "D"):
+        arrs = p[key] if isinstance(p[key], list) else [p[key]]
+        grads = g[key] if isinstance(g[key], list) else [g[key]]
+        for a, ga in zip(arrs, grads):
+            for _ in range(3):
+                idx = tuple(rng.integers(0, s) for s in a.shape)
+                old = a[idx]
+                a[idx] = old + h
+                lp = C.loss_and_grads(p, X, T, basis, P, round_bf16=False)[0]
+                a[idx] = old - h
+                lm = C.loss_and_grads(p, X, T, basis, P, round_bf16=False)[0]
+                a[idx] = old
+                fd = (lp - lm) / (2 * h)
+                assert fd == pytest.approx(ga[idx], rel=1e-6, abs=1e-9), (key, idx)
+
+
+def test_bf16_rounding_is_nearest_even():
+    x = np.array([1.0, 1.0 + 2 ** -8, 1.0 + 3 * 2 ** -8, -2.5, 3.0e38, 1e-40], np.float32)
+    r = C.bf16(x)
+    assert r[0] == 1.0 and r[1] == 1.0            # tie to even (mantissa bit 0)
+    assert r[2] == np.float32(1.0 + 2 ** -6)       # tie to even (rounds up)
+    assert r[3] == -2.5
+    assert np.isfinite(r[4])
+
+
+def test_fourier_basis_is_orthonormal(native_lib):
+    from coral_amd.closure import fourier_basis
+    B = fourier_basis(24, 18, 4)
+    assert B.shape == (24 * 18, 2 * (4 * 9 + 5) - 1)
+    np.testing.assert_allclose(B.T @ B, np.eye(B.shape[1]), atol=1e-12)

@@ -46,6 +46,13 @@ class QuadVar(ctypes.Structure):
     _fields_ = [("ivar1", c_int32), ("ivar2", c_int32), ("remove_z_integral", c_int32)]
 
 
+class ClosureDesc(ctypes.Structure):
+    _fields_ = [("width", c_int32), ("hidden", c_int32), ("layers", c_int32), ("modes", c_int32),
+                ("weights", ctypes.POINTER(ctypes.c_float)), ("nweights", ctypes.c_int64),
+                ("in_mean", ctypes.POINTER(ctypes.c_float)), ("in_std", ctypes.POINTER(ctypes.c_float)),
+                ("out_mean", ctypes.POINTER(ctypes.c_float)), ("out_std", ctypes.POINTER(ctypes.c_float))]
+
+
 class Problem(ctypes.Structure):
     _fields_ = [("NX", c_int32), ("NY", c_int32), ("NZ", c_int32),
                 ("Lx", c_double), ("Ly", c_double), ("gap", c_double), ("center", c_double),
@@ -68,7 +75,7 @@ EXPORTS = (
     "coral_amd_banded_solve_noalloc_bytes", "coral_amd_set_launch_shape", "coral_amd_get_launch_shape",
     "coral_amd_ensemble_create", "coral_amd_ensemble_destroy", "coral_amd_ensemble_set_schedule",
     "coral_amd_ensemble_tune", "coral_amd_ensemble_step", "coral_amd_ensemble_synchronize",
-    "coral_amd_nn_gemm", "coral_amd_nn_cast_bf16", "coral_amd_nn_bias_act", "coral_amd_nn_gelu_bwd",
+    "coral_amd_set_closure", "coral_amd_nn_gemm", "coral_amd_nn_cast_bf16", "coral_amd_nn_bias_act", "coral_amd_nn_gelu_bwd",
     "coral_amd_nn_colsum_work", "coral_amd_nn_colsum", "coral_amd_nn_mode_scale", "coral_amd_nn_mode_scale_grad",
     "coral_amd_nn_axpy", "coral_amd_nn_mse_partials", "coral_amd_nn_mse", "coral_amd_nn_adam",
 )
@@ -110,6 +117,7 @@ def _load():
     lib.coral_amd_banded_solve_noalloc.argtypes = [c_int32, c_int32, c_int32, c_int32, c_int32, P, P, P, P]
     lib.coral_amd_banded_solve_noalloc_bytes.argtypes = [c_int32, c_int32, c_int32, c_int32]
     lib.coral_amd_set_launch_shape.argtypes = [P, c_int32, c_int32]
+    lib.coral_amd_set_closure.argtypes = [P, ctypes.POINTER(ClosureDesc)]
     lib.coral_amd_get_launch_shape.argtypes = [P, P, P]
     lib.coral_amd_ensemble_create.argtypes = [P, c_int32, PP]
     lib.coral_amd_ensemble_destroy.argtypes = [P]

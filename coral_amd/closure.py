@@ -145,6 +145,51 @@ def init_params(layout: Layout, seed: int) -> np.ndarray:
     return flat
 
 
+@dataclass
+class Model:
+    """A trained closure: the network's layout and flat FP32 parameters, the
+    Fourier modes of its spectral convolution, and the per-channel statistics
+    of its inputs (the full variables, then the height) and outputs (the
+    quadratic variables' corrections)."""
+    layout: Layout
+    modes: int
+    params: np.ndarray
+    in_mean: np.ndarray
+    in_std: np.ndarray
+    out_mean: np.ndarray
+    out_std: np.ndarray
+
+    def save(self, path: str):
+        np.savez(path, params=self.params, in_mean=self.in_mean, in_std=self.in_std, out_mean=self.out_mean,
+                 out_std=self.out_std, shape=np.array([self.layout.cin, self.layout.cout, self.layout.width,
+                                                       self.layout.hidden, self.layout.layers, self.layout.nb,
+                                                       self.modes]))
+
+    @classmethod
+    def load(cls, path: str) -> "Model":
+        d = np.load(path, allow_pickle=False)
+        cin, cout, W, Hd, L, nb, M = (int(x) for x in d["shape"])
+        return cls(Layout(cin=cin, cout=cout, width=W, hidden=Hd, layers=L, nb=nb), M, d["params"], d["in_mean"],
+                   d["in_std"], d["out_mean"], d["out_std"])
+
+
+def plane_rows(fields: np.ndarray, zgrid: np.ndarray) -> np.ndarray:
+    """Network input rows of a coarse state: fields (nfull, NXAA, NYAA, NZAA) in
+    the solver's physical layout -> rows (NZAA * NYAA * NXAA, nfull + 1), plane
+    by plane (z slowest, then y, then x), the height last."""
+    nf, NX, NY, NZ = fields.shape
+    f = np.transpose(fields, (3, 2, 1, 0)).reshape(NZ * NY * NX, nf)
+    z = np.repeat(zgrid, NY * NX)[:, None]
+    return np.concatenate([f, z], axis=1)
+
+
+def normalise(rows: np.ndarray, mean: np.ndarray, std: np.ndarray) -> np.ndarray:
+    """(rows - mean) / std in float64, rounded to float32 (the values the
+    device rounds to BF16)."""
+    return ((rows - np.asarray(mean, np.float32).astype(np.float64)) /
+            np.asarray(std, np.float32).astype(np.float64)).astype(np.float32)
+
+
 class _Work:
     """Device buffers of one batch size P (activations kept for the backward
     pass and its scratch), allocated once and reused."""

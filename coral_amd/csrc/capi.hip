@@ -14,6 +14,7 @@
 #include "../../include/coral_amd.h"
 #include "assembly.hpp"
 #include "kernels.hpp"
+#include "nn.hpp"
 
 using namespace coral;
 
@@ -154,6 +155,18 @@ struct coral_amd_ctx {
     // the z transforms as vector GEMMs instead of MFMA products
     int xpass_wpb = 0;
     bool z_plain = false;
+    // learned subgrid closure (coral_amd_set_closure): the network, its
+    // normalisation, inference workspace and the correction's buffers
+    struct Closure {
+        bool on = false;
+        nn::NetParams net;
+        nn::NetWork ws;
+        int chunk = 0;   // planes per network launch
+        DevBuf<float> w, mu_in, sd_in, mu_out, sd_out, Y, H, Z, U, Cm;
+        DevBuf<unsigned short> w16, basis16, X16, H16, A16, Cs16;
+        DevBuf<double> phys_full, phys_tau, zgrid;
+        DevBuf<double2> Qtau;
+    } closure;
     // incremented whenever the launches of a step change (dt, precision,
     // projection, launch shapes): graphs that hold this run's step (its own or
     // an ensemble's) are stale when it differs from their capture's
@@ -728,9 +741,89 @@ void products_to_Qhat(coral_amd_ctx &c, const std::vector<const double2 *> &kst,
         if (P.quad_filter[q]) launch_remove_zmean(c.stream, c.Qhat.p + size_t(q) * P.nsys * P.NZ, P.NZ, P.nsys);
 }
 
+// ---- learned closure: the network's correction of the quadratic variables ----
+__device__ inline unsigned short bf16_of(float f) {
+    unsigned u = __float_as_uint(f);
+    if ((u & 0x7F800000u) == 0x7F800000u) return (unsigned short)((u >> 16) | ((u & 0xFFFFu) ? 0x40u : 0u));
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (unsigned short)(u >> 16);
+}
+
+// network input rows of planes z0 .. z0 + P - 1: the full variables at the
+// point, normalised ((v - mean) / std, in FP64, rounded to FP32 then BF16),
+// and the height of the plane, normalised likewise
+__global__ void k_closure_pack(const double *__restrict__ phys, const double *__restrict__ zgrid, int z0, int P,
+                               int N2, int NZAA, int nfull, const float *__restrict__ mu, const float *__restrict__ sd,
+                               unsigned short *__restrict__ X16) {
+    const long long r = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= (long long)P * N2) return;
+    const int z = z0 + int(r / N2), i = int(r % N2), cin = nfull + 1;
+    for (int f = 0; f < nfull; ++f) {
+        const double v = (phys[((size_t)f * NZAA + z) * N2 + i] - double(mu[f])) / double(sd[f]);
+        X16[r * cin + f] = bf16_of(float(v));
+    }
+    X16[r * cin + nfull] = bf16_of(float((zgrid[z] - double(mu[nfull])) / double(sd[nfull])));
+}
+
+// the network's output rows of planes z0 .. back to physical corrections
+// tau[q][z][y][x] = Y * std + mean
+__global__ void k_closure_unpack(const float *__restrict__ Y, int z0, int P, int N2, int NZAA, int nquad,
+                                 const float *__restrict__ mu, const float *__restrict__ sd, double *__restrict__ tau) {
+    const long long r = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= (long long)P * N2) return;
+    const int z = z0 + int(r / N2), i = int(r % N2);
+    for (int q = 0; q < nquad; ++q)
+        tau[((size_t)q * NZAA + z) * N2 + i] = double(Y[r * nquad + q]) * double(sd[q]) + double(mu[q]);
+}
+
+__global__ void k_add_c(double2 *__restrict__ y, const double2 *__restrict__ x, size_t n) {
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) y[i] = cadd(y[i], x[i]);
+}
+
+// Qhat += spectrum of the network's correction for the full variables in G
+// (the stage's y-physical fields, as the products left them): x c2r of the
+// full variables, the network plane by plane, the forward transforms of the
+// corrections with the step's kernels (z-integral removal as for the
+// variables themselves)
+void closure_correction(coral_amd_ctx &c) {
+    const Problem &P = c.P;
+    auto &cl = c.closure;
+    const long long nlines = (long long)P.NZAA * P.NYAA;
+    const int N2 = P.NYAA * P.NXAA;
+    c.timed("closure", [&] {
+        launch_xbwd(c.stream, c.px, c.G.p, cl.phys_full.p, P.nfull, nlines, P.nkx);
+        for (int z0 = 0; z0 < P.NZAA; z0 += cl.chunk) {
+            const int np = std::min(cl.chunk, P.NZAA - z0);
+            const unsigned g = unsigned(((long long)np * N2 + 255) / 256);
+            hipLaunchKernelGGL(k_closure_pack, dim3(g), dim3(256), 0, c.stream, cl.phys_full.p, cl.zgrid.p, z0, np, N2,
+                               P.NZAA, P.nfull, cl.mu_in.p, cl.sd_in.p, cl.X16.p);
+            check_launch("closure pack");
+            nn::net_forward(c.stream, cl.net, cl.X16.p, np, cl.Y.p, cl.ws);
+            hipLaunchKernelGGL(k_closure_unpack, dim3(g), dim3(256), 0, c.stream, cl.Y.p, z0, np, N2, P.NZAA, P.nquad,
+                               cl.mu_out.p, cl.sd_out.p, cl.phys_tau.p);
+            check_launch("closure unpack");
+        }
+        launch_xfwd(c.stream, c.px, cl.phys_tau.p, c.H.p, P.nquad, nlines, P.nkx);
+        launch_yfwd(c.stream, c.py, c.H.p, c.Hy.p, P.nquad, P.NZAA, P.nkx, P.nky, P.NY);
+        zfwd(c, c.Hy.p, cl.Qtau.p, P.nquad);
+        for (int q = 0; q < P.nquad; ++q)
+            if (P.quad_filter[q]) launch_remove_zmean(c.stream, cl.Qtau.p + size_t(q) * P.nsys * P.NZ, P.NZ, P.nsys);
+        const size_t n = size_t(P.nquad) * P.nsys * P.NZ;
+        hipLaunchKernelGGL(k_add_c, dim3(unsigned((n + 255) / 256)), dim3(256), 0, c.stream, c.Qhat.p, cl.Qtau.p, n);
+        check_launch("closure add");
+    });
+}
+
+// the quadratic variables' spectra in Qhat (or Qhat32), with the closure's correction
+void quad_spectra(coral_amd_ctx &c, const std::vector<const double2 *> &kst, const std::vector<const double2 *> &zst) {
+    products_to_Qhat(c, kst, zst, c.precision == 1);
+    if (c.closure.on) closure_correction(c);
+}
+
 void nl_terms(coral_amd_ctx &c, const std::vector<const double2 *> &kst, const std::vector<const double2 *> &zst,
               int stage) {
-    products_to_Qhat(c, kst, zst, c.precision == 1);
+    quad_spectra(c, kst, zst);
     for (auto &d : c.ks) nl_assemble(c, *d, d->Khat[stage]->p);
     if (!c.zs.empty()) HIP_OK(hipEventRecord(c.ev_qhat, c.stream));
 }
@@ -1565,7 +1658,7 @@ int coral_amd_products(coral_amd_ctx *c, double *out) {
         for (auto &d : c->ks) kst.push_back(d->field.p);
         for (auto &d : c->zs) zst.push_back(d->field.p);
         const bool fp32 = c->precision == 1;
-        products_to_Qhat(*c, kst, zst, fp32);
+        quad_spectra(*c, kst, zst);
         join_mean(*c);
         HIP_OK(hipStreamSynchronize(c->stream));
         std::vector<double2> h(size_t(P.nquad) * P.NZ * P.nsys);
@@ -1831,6 +1924,7 @@ int coral_amd_set_precision(coral_amd_ctx *c, int32_t mode, int32_t refine) {
         if (!c) throw std::runtime_error("null context");
         if (mode < 0 || mode > 1) throw std::runtime_error("precision mode must be 0 (FP64) or 1 (FP32 transforms)");
         if (refine < 0 || refine > 2) throw std::runtime_error("refine must be 0, 1 (last stage) or 2 (every stage)");
+        if (mode == 1 && c->closure.on) throw std::runtime_error("the closure runs with the FP64 transform chain only");
         HIP_OK(hipSetDevice(c->device));
         HIP_OK(hipStreamSynchronize(c->stream));
         HIP_OK(hipStreamSynchronize(c->side));
@@ -1955,6 +2049,105 @@ int coral_amd_timer_read(coral_amd_ctx *c, int32_t i, char *name, int32_t name_l
     }
 }
 
+
+int coral_amd_set_closure(coral_amd_ctx *c, const coral_amd_closure *cl) {
+    try {
+        if (!c) throw std::runtime_error("null context");
+        HIP_OK(hipSetDevice(c->device));
+        HIP_OK(hipStreamSynchronize(c->stream));
+        c->drop_graphs();
+        ++c->gen;
+        auto &C = c->closure;
+        C.on = false;
+        if (!cl) return 0;
+        const Problem &P = c->P;
+        if (c->precision == 1) throw std::runtime_error("the closure runs with the FP64 transform chain only");
+        if (P.nfull < 1 || P.nquad < 1) throw std::runtime_error("the closure needs full and quadratic variables");
+        if (!cl->weights || !cl->in_mean || !cl->in_std || !cl->out_mean || !cl->out_std)
+            throw std::runtime_error("closure: null array");
+        if (cl->width < 1 || cl->hidden < 1 || cl->layers < 0 || cl->modes < 0)
+            throw std::runtime_error("closure: bad network shape");
+        const int NXAA = P.NXAA, NYAA = P.NYAA, N2 = NXAA * NYAA, M = cl->modes;
+        if (2 * M >= NXAA || 2 * M >= NYAA) throw std::runtime_error("closure: modes must stay below the grid's Nyquist");
+        // the plane's Fourier columns (coral_amd.closure.fourier_basis): cos and
+        // sin of kx x + ky y over the half plane |kx|, |ky| <= M, each of unit
+        // norm, rounded to FP32 and then BF16 like the trainer's copy
+        std::vector<std::vector<double>> cols;
+        for (int kx = 0; kx <= M; ++kx)
+            for (int ky = -M; ky <= M; ++ky) {
+                if (kx == 0 && ky < 0) continue;
+                for (int sn = 0; sn < 2; ++sn) {
+                    if (sn == 1 && kx == 0 && ky == 0) continue;
+                    std::vector<double> v(N2);
+                    double nrm = 0.0;
+                    for (int y = 0; y < NYAA; ++y)
+                        for (int x = 0; x < NXAA; ++x) {
+                            const double ph = kx * (2.0 * M_PI * x / NXAA) + ky * (2.0 * M_PI * y / NYAA);
+                            const double e = sn == 0 ? std::cos(ph) : std::sin(ph);
+                            v[size_t(y) * NXAA + x] = e;
+                            nrm += e * e;
+                        }
+                    nrm = std::sqrt(nrm);
+                    for (auto &e : v) e /= nrm;
+                    cols.push_back(std::move(v));
+                }
+            }
+        const int nb = int(cols.size());
+        std::vector<unsigned short> b16(size_t(N2) * nb);
+        for (int i = 0; i < N2; ++i)
+            for (int k = 0; k < nb; ++k) b16[size_t(i) * nb + k] = nn::host_bf16(float(cols[k][i]));
+        nn::NetParams &n = C.net;
+        n = nn::NetParams{};
+        n.cin = P.nfull + 1;
+        n.cout = P.nquad;
+        n.width = cl->width;
+        n.hidden = cl->hidden;
+        n.layers = cl->layers;
+        n.nb = nb;
+        n.N2 = N2;
+        nn::net_layout(n);
+        if (cl->nweights != n.size)
+            throw std::runtime_error("closure: " + std::to_string(cl->nweights) + " weights given, the network has " +
+                                     std::to_string(n.size));
+        std::vector<float> wh(cl->weights, cl->weights + n.size);
+        std::vector<unsigned short> w16(n.size);
+        for (long long i = 0; i < n.size; ++i) w16[i] = nn::host_bf16(wh[i]);
+        C.w.upload(wh);
+        C.w16.upload(w16);
+        C.basis16.upload(b16);
+        C.mu_in.upload(std::vector<float>(cl->in_mean, cl->in_mean + n.cin));
+        C.sd_in.upload(std::vector<float>(cl->in_std, cl->in_std + n.cin));
+        C.mu_out.upload(std::vector<float>(cl->out_mean, cl->out_mean + n.cout));
+        C.sd_out.upload(std::vector<float>(cl->out_std, cl->out_std + n.cout));
+        n.w = C.w.p;
+        n.w16 = C.w16.p;
+        n.basis16 = C.basis16.p;
+        std::vector<double> zg(P.NZAA);
+        for (int k = 0; k < P.NZAA; ++k) zg[k] = P.center + 0.5 * P.gap * std::cos(M_PI * (2 * k + 1) / (2.0 * P.NZAA));
+        C.zgrid.upload(zg);
+        // planes per network launch: about 600k rows of activations at a time
+        C.chunk = std::max(1, std::min(P.NZAA, int(600000 / N2)));
+        const size_t R = size_t(C.chunk) * N2;
+        C.X16.alloc(R * n.cin);
+        C.Y.alloc(R * n.cout);
+        C.H.alloc(R * n.width);
+        C.H16.alloc(R * n.width);
+        C.Z.alloc(R * n.hidden);
+        C.A16.alloc(R * n.hidden);
+        C.U.alloc(R * n.width);
+        C.Cm.alloc(size_t(C.chunk) * nb * n.width);
+        C.Cs16.alloc(size_t(C.chunk) * nb * n.width);
+        C.ws = nn::NetWork{C.H.p, C.Z.p, C.U.p, C.Cm.p, C.H16.p, C.A16.p, C.Cs16.p};
+        C.phys_full.alloc(size_t(P.nfull) * P.NZAA * N2);
+        C.phys_tau.alloc(size_t(P.nquad) * P.NZAA * N2);
+        C.Qtau.alloc(size_t(P.nquad) * P.nsys * P.NZ);
+        C.on = true;
+        return 0;
+    } catch (const std::exception &e) {
+        if (c) c->closure.on = false;
+        return fail(e);
+    }
+}
 
 int coral_amd_set_launch_shape(coral_amd_ctx *c, int32_t xpass_wpb, int32_t z_plain) {
     try {

@@ -4,6 +4,7 @@
 // (unsigned short); FP32 -> BF16 rounds to nearest even.
 #include "nn.hpp"
 
+#include <cstring>
 #include <stdexcept>
 #include <string>
 
@@ -307,6 +308,60 @@ void adam(hipStream_t st, float *w, const float *g, float *m, float *v, unsigned
     const float c1 = 1.f - powf(b1, float(t)), c2 = 1.f - powf(b2, float(t));
     hipLaunchKernelGGL(k_adam, dim3(blocks(n)), dim3(256), 0, st, w, g, m, v, w16, n, lr, b1, b2, eps, c1, c2);
     check_launch("adam");
+}
+
+
+// ---------------------------------------------------------------------------
+// inference of a trained network (the same sequence of kernels as the
+// trainer's forward pass, without keeping activations)
+// ---------------------------------------------------------------------------
+void net_layout(NetParams &n) {
+    if (n.layers < 0 || n.layers > 16) throw std::runtime_error("closure network: at most 16 layers");
+    const long long W = n.width, Hd = n.hidden;
+    long long o = 0;
+    n.off_W0 = o, o += (long long)n.cin * W;
+    n.off_b0 = o, o += W;
+    for (int l = 0; l < n.layers; ++l) {
+        n.off_W1[l] = o, o += W * Hd;
+        n.off_b1[l] = o, o += Hd;
+        n.off_W2[l] = o, o += Hd * W;
+        n.off_b2[l] = o, o += W;
+        n.off_D[l] = o, o += (long long)n.nb * W;
+    }
+    n.off_Wp = o, o += W * n.cout;
+    n.off_bp = o, o += n.cout;
+    n.size = o;
+}
+
+void net_forward(hipStream_t st, const NetParams &n, const unsigned short *X16, int P, float *Y, const NetWork &w) {
+    const int W = n.width, Hd = n.hidden, nb = n.nb, N2 = n.N2;
+    const long long R = (long long)P * N2;
+    if (R > 2147483647LL) throw std::runtime_error("closure network: too many rows");
+    const int Ri = int(R);
+    gemm(st, 0, 0, Ri, W, n.cin, 1.f, X16, n.cin, 0, n.w16 + n.off_W0, W, 0, 0.f, w.H, W, 0, 1);
+    bias_act(st, w.H, n.w + n.off_b0, W, R * W, 0, nullptr, w.H16);
+    for (int l = 0; l < n.layers; ++l) {
+        gemm(st, 0, 0, Ri, Hd, W, 1.f, w.H16, W, 0, n.w16 + n.off_W1[l], Hd, 0, 0.f, w.Z, Hd, 0, 1);
+        bias_act(st, w.Z, n.w + n.off_b1[l], Hd, R * Hd, 1, nullptr, w.A16);
+        gemm(st, 0, 0, Ri, W, Hd, 1.f, w.A16, Hd, 0, n.w16 + n.off_W2[l], W, 0, 0.f, w.U, W, 0, 1);
+        gemm(st, 1, 0, nb, W, N2, 1.f, n.basis16, nb, 0, w.H16, W, (long long)N2 * W, 0.f, w.Cm, W, (long long)nb * W,
+             P);
+        mode_scale(st, w.Cm, n.w + n.off_D[l], nullptr, w.Cs16, P, nb * W);
+        gemm(st, 0, 0, N2, W, nb, 1.f, n.basis16, nb, 0, w.Cs16, W, (long long)nb * W, 1.f, w.H, W, (long long)N2 * W,
+             P);
+        bias_act(st, w.U, n.w + n.off_b2[l], W, R * W, 0, nullptr, nullptr);
+        axpy(st, w.H, w.U, 1.f, R * W, w.H16);
+    }
+    gemm(st, 0, 0, Ri, n.cout, W, 1.f, w.H16, W, 0, n.w16 + n.off_Wp, n.cout, 0, 0.f, Y, n.cout, 0, 1);
+    bias_act(st, Y, n.w + n.off_bp, n.cout, R * n.cout, 0, nullptr, nullptr);
+}
+
+unsigned short host_bf16(float f) {
+    unsigned u;
+    std::memcpy(&u, &f, 4);
+    if ((u & 0x7F800000u) == 0x7F800000u) return (unsigned short)((u >> 16) | ((u & 0xFFFFu) ? 0x40u : 0u));
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (unsigned short)(u >> 16);
 }
 
 }  // namespace nn

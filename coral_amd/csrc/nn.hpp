@@ -32,3 +32,33 @@ void adam(hipStream_t st, float *w, const float *g, float *m, float *v, unsigned
 
 }  // namespace nn
 }  // namespace coral
+
+namespace coral {
+namespace nn {
+
+// A trained network on the device, in the flat layout of coral_amd.closure.Layout:
+// W0 (cin x W), b0 (W), per layer W1 (W x Hd), b1 (Hd), W2 (Hd x W), b2 (W),
+// D (nb x W), then Wp (W x cout), bp (cout).
+struct NetParams {
+    int cin = 0, cout = 0, width = 0, hidden = 0, layers = 0, nb = 0, N2 = 0;
+    const float *w = nullptr;              // FP32 weights (biases and mode scales are read in FP32)
+    const unsigned short *w16 = nullptr;   // BF16 copy (GEMM operands)
+    const unsigned short *basis16 = nullptr;  // N2 x nb, BF16
+    long long off_W0 = 0, off_b0 = 0, off_Wp = 0, off_bp = 0;
+    long long off_W1[16] = {}, off_b1[16] = {}, off_W2[16] = {}, off_b2[16] = {}, off_D[16] = {};
+    long long size = 0;                    // total number of parameters
+};
+// offsets of the blocks for the given shapes (throws for more than 16 layers)
+void net_layout(NetParams &n);
+// inference workspace for up to Pmax planes of N2 points
+struct NetWork {
+    float *H = nullptr, *Z = nullptr, *U = nullptr, *Cm = nullptr;
+    unsigned short *H16 = nullptr, *A16 = nullptr, *Cs16 = nullptr;
+};
+// Y (P N2 x cout, FP32) = network(X16 (P N2 x cin, BF16)), P <= the workspace's planes
+void net_forward(hipStream_t st, const NetParams &n, const unsigned short *X16, int P, float *Y, const NetWork &w);
+// host rounding FP32 -> BF16 (nearest even), the same as the device kernels'
+unsigned short host_bf16(float f);
+
+}  // namespace nn
+}  // namespace coral

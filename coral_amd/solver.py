@@ -301,6 +301,22 @@ class Solver:
         supercritical runs."""
         check(lib.coral_amd_set_kx0_projection(self._ctx, int(bool(on))), "coral_amd_set_kx0_projection")
 
+    def set_closure(self, model=None):
+        """Attach a trained closure (``coral_amd.closure.Model``: layout, flat
+        weights and normalisation) or remove it (None): every stage then adds
+        the network's correction to the quadratic variables
+        (``coral_amd_set_closure``)."""
+        if model is None:
+            check(lib.coral_amd_set_closure(self._ctx, None), "coral_amd_set_closure")
+            return
+        f32 = lambda a: np.ascontiguousarray(a, np.float32)
+        arrs = [f32(model.params), f32(model.in_mean), f32(model.in_std), f32(model.out_mean), f32(model.out_std)]
+        P = ctypes.POINTER(ctypes.c_float)
+        d = _lib.ClosureDesc(model.layout.width, model.layout.hidden, model.layout.layers, model.modes,
+                             arrs[0].ctypes.data_as(P), arrs[0].size, arrs[1].ctypes.data_as(P),
+                             arrs[2].ctypes.data_as(P), arrs[3].ctypes.data_as(P), arrs[4].ctypes.data_as(P))
+        check(lib.coral_amd_set_closure(self._ctx, ctypes.byref(d)), "coral_amd_set_closure")
+
     def set_launch_shape(self, xpass_wpb: int = 0, z_plain: bool = False):
         """Launch shapes (``coral_amd_set_launch_shape``): x pass wavefronts per
         workgroup (0 = most that fit) and vector-GEMM z transforms."""

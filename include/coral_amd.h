@@ -291,6 +291,27 @@ int coral_amd_set_kx0_projection(coral_amd_ctx *ctx, int32_t enable);
  * plane_layer_main.f90:157).  While timers are enabled steps run eagerly. */
 int coral_amd_use_graphs(coral_amd_ctx *ctx, int32_t enable);
 
+/* Learned subgrid closure of a coarse run (no counterpart in the reference).
+ * The network of coral_amd/closure.py (per horizontal plane: a lift from the
+ * nfull full variables and the height to `width` channels, `layers` residual
+ * layers of a GELU MLP of `hidden` channels plus a spectral convolution over
+ * the plane's Fourier modes |kx|, |ky| <= modes, a projection to the nquad
+ * quadratic variables) maps the full variables of each stage to corrections
+ * of the quadratic variables, whose spectra are added to theirs before the NL
+ * terms are assembled (every stage, and in coral_amd_products).  weights: the
+ * flat FP32 parameters in that layout (nweights of them); in_mean / in_std
+ * (nfull + 1) normalise the inputs, out_mean / out_std (nquad) scale the
+ * outputs back.  GEMMs run in BF16 on MFMA with FP32 accumulation.  NULL
+ * removes the closure.  FP64 transform chain only (precision mode 0). */
+typedef struct {
+    int32_t width, hidden, layers, modes;
+    const float *weights;
+    int64_t nweights;
+    const float *in_mean, *in_std;
+    const float *out_mean, *out_std;
+} coral_amd_closure;
+int coral_amd_set_closure(coral_amd_ctx *ctx, const coral_amd_closure *closure);
+
 /* Launch shapes of a run (no counterpart in the reference): xpass_wpb is the
  * largest number of wavefronts per workgroup of the x pass (1..4; 0 = the most
  * that fit its LDS), z_plain = 1 runs the z transforms as vector GEMMs

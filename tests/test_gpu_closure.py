@@ -108,3 +108,48 @@ def test_training_lowers_the_loss(nnmod):
     losses = [net.train_step(X16, Td, P, 2e-3) for _ in range(300)]
     print("loss first/last:", losses[0], losses[-1], flush=True)
     assert np.isfinite(losses[-1]) and losses[-1] < losses[0] / 10
+
+
+def test_step_closure_matches_network(nnmod):
+    """coral_amd_set_closure: the correction the step adds to the quadratic
+    variables (coral_amd_products with the closure minus without) equals the
+    forward transform of the trainer's network applied to the same physical
+    full variables (ClosureNet.forward, then coral_amd_fwd_transform), within
+    FP64 rounding of the transforms (both paths run the same BF16 network
+    kernels on the same BF16 inputs)."""
+    closure, devmem = nnmod
+    from conftest import product_recipe, seeded_state
+    from coral_amd.diagnostics import gauss_chebyshev_grid
+    from coral_amd.solver import Solver
+    r = product_recipe("coral.equations.rbc_ra1e5")
+    NX, NZ, M = 32, 17, 3
+    NXAA, NZAA = 3 * NX // 2, 3 * NZ // 2
+    basis = closure.fourier_basis(NXAA, NXAA, M)
+    lay = closure.Layout(cin=5, cout=9, width=64, hidden=256, layers=4, nb=basis.shape[1])
+    rng = np.random.default_rng(4)
+    model = closure.Model(lay, M, closure.init_params(lay, 11) * 3.0, rng.normal(size=5) * 0.1,
+                          1.0 + rng.random(5), rng.normal(size=9) * 0.1, 0.5 + rng.random(9))
+    k0, z0 = seeded_state(r, NX, NX, NZ, 6, 6, 9, amps=(("u", 1.0), ("v", 1.0), ("w", 1.0), ("t", 0.1)))
+    with Solver(r, NX, NX, NZ, 2 * np.pi, 2 * np.pi) as g:
+        g.set_state(k0, z0)
+        q0 = g.products()
+        fields = np.stack([g.physical(i) for i in range(4)])
+        g.set_closure(model)
+        q1 = g.products()
+        # the network on the host side's copy of the same fields
+        rows = closure.plane_rows(fields, gauss_chebyshev_grid(NZAA, 0.5, 1.0))
+        net = closure.ClosureNet(lay, basis, model.params)
+        X16 = to_dev_bf16(closure, devmem, closure.normalise(rows, model.in_mean, model.in_std))
+        Y = net.forward(X16, NZAA).numpy().reshape(NZAA, NXAA, NXAA, 9).astype(np.float64)
+        tau = Y * model.out_std.astype(np.float32) + model.out_mean.astype(np.float32)
+        worst = 0.0
+        for q in range(9):
+            tq = np.transpose(tau[..., q], (2, 1, 0))            # [x][y][z]
+            spec = g.fwd_transform(tq)
+            d = q1[q] - q0[q]
+            worst = max(worst, float(np.linalg.norm(d - spec) / np.linalg.norm(spec)))
+        g.set_closure(None)
+        q2 = g.products()
+    print("closure correction vs network + transform, worst rel L2:", worst, flush=True)
+    assert worst <= 1e-10
+    assert np.array_equal(q2, q0)

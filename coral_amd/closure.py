@@ -13,7 +13,10 @@ the difference between the filtered product of a fine run and the product of
 its filtered fields, and the coarse step adds its spectrum to the quadratic
 variables before the NL terms are assembled (``coral_amd_set_closure``).
 
-Network, per plane of N2 = NXAA NYAA points (rows of a plane contiguous):
+Network, per plane of N2 = NXAA NYAA points (rows of a plane contiguous); a
+point's input X is every full variable at the point and its differences to
+the 26 neighbours of its 3 x 3 x 3 neighbourhood (the local gradients that
+subgrid products depend on, formed before the BF16 rounding) and the height:
 
     H = X W0 + b0                                    lift, Cin -> width
     4 times:  H <- H + gelu(H W1 + b1) W2 + b2       pointwise MLP, width -> 4 width -> width
@@ -173,14 +176,32 @@ class Model:
                    d["in_std"], d["out_mean"], d["out_std"])
 
 
-def plane_rows(fields: np.ndarray, zgrid: np.ndarray) -> np.ndarray:
-    """Network input rows of a coarse state: fields (nfull, NXAA, NYAA, NZAA) in
-    the solver's physical layout -> rows (NZAA * NYAA * NXAA, nfull + 1), plane
-    by plane (z slowest, then y, then x), the height last."""
+def plane_rows(fields: np.ndarray, zgrid: np.ndarray, stencil: bool = True) -> np.ndarray:
+    """Rows of fields (nfield, NXAA, NYAA, NZAA) in the solver's physical layout,
+    plane by plane (z slowest, then y, then x).  stencil: the network's input,
+    every field on the point's 3 x 3 x 3 neighbourhood (periodic in x and y,
+    clamped at the walls in z; offsets dz, dy, dx = -1, 0, 1 in that nesting,
+    the fields innermost: 27 nfield columns), the point's own values as they
+    are and each neighbour as its difference from them (the BF16 inputs then
+    keep the digits of the local differences), then the height; without
+    stencil the fields at the point and the height (nfield + 1 columns)."""
     nf, NX, NY, NZ = fields.shape
-    f = np.transpose(fields, (3, 2, 1, 0)).reshape(NZ * NY * NX, nf)
-    z = np.repeat(zgrid, NY * NX)[:, None]
-    return np.concatenate([f, z], axis=1)
+    f = np.transpose(fields, (3, 2, 1, 0))              # [z][y][x][field]
+    cols = []
+    if stencil:
+        zi = np.arange(NZ)
+        for dz in (-1, 0, 1):
+            fz = f[np.clip(zi + dz, 0, NZ - 1)]
+            for dy in (-1, 0, 1):
+                for dx in (-1, 0, 1):
+                    v = np.roll(fz, shift=(-dy, -dx), axis=(1, 2))
+                    if (dz, dy, dx) != (0, 0, 0):
+                        v = v - f
+                    cols.append(v.reshape(NZ * NY * NX, nf))
+    else:
+        cols.append(f.reshape(NZ * NY * NX, nf))
+    cols.append(np.repeat(zgrid, NY * NX)[:, None])
+    return np.concatenate(cols, axis=1)
 
 
 def normalise(rows: np.ndarray, mean: np.ndarray, std: np.ndarray) -> np.ndarray:
@@ -367,3 +388,145 @@ class ClosureNet:
         self.backward()
         self.adam_step(lr)
         return loss
+
+
+# ---------------------------------------------------------------------------
+# training data: filtered fine-grid states and their subgrid products
+# ---------------------------------------------------------------------------
+class DevView:
+    """A window of a DevArray (pointer and length), for batches of planes."""
+
+    def __init__(self, base: DevArray, offset: int, n: int):
+        self.ptr = base.at(offset)
+        self.n = n
+
+
+class BoxFilter:
+    """Sharp spectral (box) filter from a fine grid to a coarse one: the
+    horizontal modes the coarse grid keeps, matched by integer wavenumber,
+    and for every coupled variable its first Galerkin coefficients (as many as
+    the coarse NZ gives it); quadratic spectra are cut to the coarse kx, ky and
+    Chebyshev degrees n < NZ_coarse."""
+
+    def __init__(self, recipe, fine: Tuple[int, int, int], coarse: Tuple[int, int, int]):
+        from .ic import interleaved_positions, wavenumber_indices
+        (NXf, NYf, NZf), (NXc, NYc, NZc) = fine, coarse
+        if NXc > NXf or NYc > NYf or NZc > NZf:
+            raise ValueError("the coarse grid must not exceed the fine one")
+        self.fine, self.coarse = fine, coarse
+        kxf, kyf, _ = wavenumber_indices(NXf, NYf)
+        kxc, kyc, keepc = wavenumber_indices(NXc, NYc)
+        self.keep = keepc
+        slot_f = {int(k): i for i, k in enumerate(kyf)}
+        self.iy = np.array([slot_f[int(k)] for k in kyc])
+        self.ix = np.array([int(k) for k in kxc])
+        self.kpos, self.zpos = [], []
+        for systems, out in ((recipe.kxky_systems, self.kpos), (recipe.zero_systems, self.zpos)):
+            for s in systems:
+                sf = [NZf - bc // 10 for bc in s.bc_codes]
+                sc = [NZc - bc // 10 for bc in s.bc_codes]
+                pf = np.concatenate([interleaved_positions(len(sf), sf, v)[: sc[v]] for v in range(len(sf))])
+                pc = np.concatenate([interleaved_positions(len(sc), sc, v) for v in range(len(sc))])
+                out.append((pf, pc, sum(sc)))
+
+    def state(self, k_fine: List[np.ndarray], z_fine: List[np.ndarray]):
+        NXc, NYc, _ = self.coarse
+        ks, zs = [], []
+        for (pf, pc, total), kf in zip(self.kpos, k_fine):
+            kc = np.zeros((NXc // 2, 3 * NYc // 2, total), np.complex128)
+            sub = kf[self.ix][:, self.iy]
+            kc[:, :, pc] = sub[:, :, pf]
+            kc[~self.keep] = 0.0
+            ks.append(kc)
+        for (pf, pc, total), zf in zip(self.zpos, z_fine):
+            zc = np.zeros(total)
+            zc[pc] = zf[pf]
+            zs.append(zc)
+        return ks, zs
+
+    def spectra(self, q_fine: np.ndarray) -> np.ndarray:
+        """(nquad, NX/2, NYAA, NZ) fine spectra -> the coarse grid's."""
+        NZc = self.coarse[2]
+        out = q_fine[:, self.ix][:, :, self.iy][..., :NZc].copy()
+        out[:, ~self.keep] = 0.0
+        return out
+
+
+def subgrid_sample(fine_solver, coarse_solver, box: BoxFilter, zgrid: np.ndarray):
+    """One training sample from the fine solver's current state: the coarse
+    network inputs (rows of plane_rows) of the filtered state, and the
+    subgrid part of every quadratic variable on the coarse grid,
+    tau = filter(F1 F2) - filter(F1) filter(F2), as rows (z, y, x) x nquad."""
+    k, z = fine_solver.get_state()
+    qf = box.spectra(fine_solver.products())
+    kc, zc = box.state(k, z)
+    coarse_solver.set_state(kc, zc)
+    qc = coarse_solver.products()
+    nf = len(coarse_solver.recipe.full_vars)
+    fields = np.stack([coarse_solver.physical(i) for i in range(nf)])
+    tau = np.stack([coarse_solver.bwd_transform(qf[q] - qc[q]) for q in range(qf.shape[0])])
+    rows_in = plane_rows(fields, zgrid)
+    rows_out = plane_rows(tau, zgrid, stencil=False)[:, :-1]
+    return rows_in.astype(np.float32), rows_out.astype(np.float32)
+
+
+def train(X: np.ndarray, T: np.ndarray, N2: int, basis: np.ndarray, modes: int, width: int = 64, hidden: int = 256,
+          layers: int = 4, epochs: int = 8, planes_per_batch: int = 16, lr: float = 1e-3, seed: int = 0,
+          log=None) -> Tuple[Model, List[float]]:
+    """Train a closure on rows X (nplanes N2, cin) -> T (nplanes N2, cout), whole
+    planes in random order, normalised per channel; Adam with a cosine
+    learning-rate decay.  Returns the model and the mean loss of each epoch
+    (in normalised units)."""
+    nplanes = X.shape[0] // N2
+    cin, cout = X.shape[1], T.shape[1]
+    in_mean, in_std = X.mean(axis=0), X.std(axis=0) + 1e-30
+    out_mean, out_std = T.mean(axis=0), T.std(axis=0) + 1e-30
+    in_std[in_std < 1e-12 * max(1.0, float(np.abs(in_mean).max()))] = 1.0
+    rng = np.random.default_rng(seed)
+    order = rng.permutation(nplanes)
+    Xn = normalise(X.reshape(nplanes, N2, cin)[order].reshape(-1, cin), in_mean, in_std)
+    Tn = normalise(T.reshape(nplanes, N2, cout)[order].reshape(-1, cout), out_mean, out_std)
+    lay = Layout(cin=cin, cout=cout, width=width, hidden=hidden, layers=layers, nb=basis.shape[1])
+    net = ClosureNet(lay, basis, seed=seed)
+    X32 = DevArray.from_numpy(Xn)
+    X16 = DevArray(X32.n, "bf16")
+    check(lib.coral_amd_nn_cast_bf16(X32.ptr, X16.ptr, X32.n, None), "nn cast")
+    del X32
+    Td = DevArray.from_numpy(Tn)
+    P = planes_per_batch
+    nb_ = nplanes // P
+    if nb_ < 1:
+        raise ValueError("fewer planes than one batch")
+    total = epochs * nb_
+    hist = []
+    step = 0
+    for ep in range(epochs):
+        acc = 0.0
+        for b in range(nb_):
+            a = 0.5 * lr * (1.0 + math.cos(math.pi * step / total))
+            acc += net.train_step(DevView(X16, b * P * N2 * cin, P * N2 * cin), DevView(Td, b * P * N2 * cout,
+                                                                                      P * N2 * cout), P, a)
+            step += 1
+        hist.append(acc / nb_)
+        if log:
+            log(f"closure epoch {ep + 1}/{epochs}: loss {hist[-1]:.4e}")
+    model = Model(lay, modes, net.params(), in_mean.astype(np.float32), in_std.astype(np.float32),
+                  out_mean.astype(np.float32), out_std.astype(np.float32))
+    return model, hist
+
+
+def predict(model: Model, X: np.ndarray, N2: int, basis: np.ndarray, planes_per_batch: int = 16) -> np.ndarray:
+    """The model's outputs (physical units) for rows X (nplanes N2, cin)."""
+    nplanes = X.shape[0] // N2
+    net = ClosureNet(model.layout, basis, model.params)
+    out = np.empty((X.shape[0], model.layout.cout), np.float64)
+    P = planes_per_batch
+    for p0 in range(0, nplanes, P):
+        np_ = min(P, nplanes - p0)
+        rows = X[p0 * N2:(p0 + np_) * N2]
+        x32 = DevArray.from_numpy(normalise(rows, model.in_mean, model.in_std))
+        x16 = DevArray(x32.n, "bf16")
+        check(lib.coral_amd_nn_cast_bf16(x32.ptr, x16.ptr, x32.n, None), "nn cast")
+        y = net.forward(x16, np_).numpy()[: np_ * N2 * model.layout.cout].reshape(-1, model.layout.cout)
+        out[p0 * N2:(p0 + np_) * N2] = y * model.out_std.astype(np.float64) + model.out_mean.astype(np.float64)
+    return out

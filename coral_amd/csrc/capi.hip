@@ -749,20 +749,38 @@ __device__ inline unsigned short bf16_of(float f) {
     return (unsigned short)(u >> 16);
 }
 
-// network input rows of planes z0 .. z0 + P - 1: the full variables at the
-// point, normalised ((v - mean) / std, in FP64, rounded to FP32 then BF16),
-// and the height of the plane, normalised likewise
+// network input rows of planes z0 .. z0 + P - 1: the full variables on the
+// point's 3 x 3 x 3 neighbourhood (periodic in x and y, clamped at the walls
+// in z; offsets dz, dy, dx = -1, 0, 1 in that nesting, the variables
+// innermost), the point's own values as they are and every neighbour as its
+// difference from them (formed in FP64, so that the BF16 inputs keep the
+// local differences' digits), normalised ((v - mean) / std in FP64, rounded
+// to FP32 then BF16), and the height of the plane, normalised likewise
 __global__ void k_closure_pack(const double *__restrict__ phys, const double *__restrict__ zgrid, int z0, int P,
-                               int N2, int NZAA, int nfull, const float *__restrict__ mu, const float *__restrict__ sd,
-                               unsigned short *__restrict__ X16) {
+                               int NXAA, int NYAA, int NZAA, int nfull, const float *__restrict__ mu,
+                               const float *__restrict__ sd, unsigned short *__restrict__ X16) {
+    const int N2 = NXAA * NYAA;
     const long long r = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= (long long)P * N2) return;
-    const int z = z0 + int(r / N2), i = int(r % N2), cin = nfull + 1;
-    for (int f = 0; f < nfull; ++f) {
-        const double v = (phys[((size_t)f * NZAA + z) * N2 + i] - double(mu[f])) / double(sd[f]);
-        X16[r * cin + f] = bf16_of(float(v));
+    const int z = z0 + int(r / N2), i = int(r % N2), y = i / NXAA, x = i % NXAA, cin = 27 * nfull + 1;
+    int c = 0;
+    for (int dz = -1; dz <= 1; ++dz) {
+        const int zz = min(max(z + dz, 0), NZAA - 1);
+        for (int dy = -1; dy <= 1; ++dy) {
+            const int yy = (y + dy + NYAA) % NYAA;
+            for (int dx = -1; dx <= 1; ++dx) {
+                const int xx = (x + dx + NXAA) % NXAA;
+                const bool centre = dz == 0 && dy == 0 && dx == 0;
+                for (int f = 0; f < nfull; ++f, ++c) {
+                    const double v0 = phys[((size_t)f * NZAA + z) * N2 + i];
+                    const double v = phys[((size_t)f * NZAA + zz) * N2 + (size_t)yy * NXAA + xx];
+                    const double d = centre ? v0 : v - v0;
+                    X16[r * cin + c] = bf16_of(float((d - double(mu[c])) / double(sd[c])));
+                }
+            }
+        }
     }
-    X16[r * cin + nfull] = bf16_of(float((zgrid[z] - double(mu[nfull])) / double(sd[nfull])));
+    X16[r * cin + c] = bf16_of(float((zgrid[z] - double(mu[c])) / double(sd[c])));
 }
 
 // the network's output rows of planes z0 .. back to physical corrections
@@ -796,8 +814,8 @@ void closure_correction(coral_amd_ctx &c) {
         for (int z0 = 0; z0 < P.NZAA; z0 += cl.chunk) {
             const int np = std::min(cl.chunk, P.NZAA - z0);
             const unsigned g = unsigned(((long long)np * N2 + 255) / 256);
-            hipLaunchKernelGGL(k_closure_pack, dim3(g), dim3(256), 0, c.stream, cl.phys_full.p, cl.zgrid.p, z0, np, N2,
-                               P.NZAA, P.nfull, cl.mu_in.p, cl.sd_in.p, cl.X16.p);
+            hipLaunchKernelGGL(k_closure_pack, dim3(g), dim3(256), 0, c.stream, cl.phys_full.p, cl.zgrid.p, z0, np,
+                               P.NXAA, P.NYAA, P.NZAA, P.nfull, cl.mu_in.p, cl.sd_in.p, cl.X16.p);
             check_launch("closure pack");
             nn::net_forward(c.stream, cl.net, cl.X16.p, np, cl.Y.p, cl.ws);
             hipLaunchKernelGGL(k_closure_unpack, dim3(g), dim3(256), 0, c.stream, cl.Y.p, z0, np, N2, P.NZAA, P.nquad,
@@ -2098,7 +2116,7 @@ int coral_amd_set_closure(coral_amd_ctx *c, const coral_amd_closure *cl) {
             for (int k = 0; k < nb; ++k) b16[size_t(i) * nb + k] = nn::host_bf16(float(cols[k][i]));
         nn::NetParams &n = C.net;
         n = nn::NetParams{};
-        n.cin = P.nfull + 1;
+        n.cin = 27 * P.nfull + 1;
         n.cout = P.nquad;
         n.width = cl->width;
         n.hidden = cl->hidden;

@@ -4,6 +4,7 @@
 // (unsigned short); FP32 -> BF16 rounds to nearest even.
 #include "nn.hpp"
 
+#include <cstdint>
 #include <cstring>
 #include <stdexcept>
 #include <string>
@@ -54,38 +55,74 @@ __global__ __launch_bounds__(256) void k_gemm_bf16(int transA, int transB, int M
     for (int i = 0; i < 2; ++i)
 #pragma unroll
         for (int j = 0; j < 2; ++j) acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
+    // each operand's 64 x 32 slice is 256 vectors of 8 consecutive elements
+    // along its contiguous dimension, one per thread; a vector inside the
+    // matrix with a 16-byte aligned address is one load, others element by
+    // element with bounds checks (ragged edges, odd leading dimensions)
+    const bool alA = (lda % 8 == 0) && ((reinterpret_cast<uintptr_t>(A) & 15) == 0);
+    const bool alB = (ldb % 8 == 0) && ((reinterpret_cast<uintptr_t>(B) & 15) == 0);
     for (int k0 = 0; k0 < K; k0 += TK) {
-        // stage: 64 x 32 elements of each operand, 8 per thread
+        {
+            // A: rows m of the tile x k; contiguous along k (or along m when transA)
+            const int a0 = transA ? (tid >> 3) : (tid >> 2);        // k (transA) or m
+            const int a1 = transA ? 8 * (tid & 7) : 8 * (tid & 3);  // first m (transA) or k of the vector
+            const int gm = m0 + (transA ? a1 : a0), gk = k0 + (transA ? a0 : a1);
+            unsigned short v[8];
+            const bool full = transA ? (gk < K && gm + 7 < M) : (gm < M && gk + 7 < K);
+            if (alA && full) {
+                const uint4 q = *reinterpret_cast<const uint4 *>(A + (transA ? (size_t)gk * lda + gm
+                                                                             : (size_t)gm * lda + gk));
+                const unsigned short *e = reinterpret_cast<const unsigned short *>(&q);
 #pragma unroll
-        for (int e = 0; e < 8; ++e) {
-            const int idx = tid + 256 * e;
-            // rows of the tile x k: with transA the k index runs fastest in
-            // memory along the tile's rows, so let consecutive threads take
-            // consecutive m (coalesced) in that case and consecutive k otherwise
-            int r, kk;
+                for (int j = 0; j < 8; ++j) v[j] = e[j];
+            } else {
+#pragma unroll
+                for (int j = 0; j < 8; ++j) {
+                    const int mm = transA ? gm + j : gm, kk = transA ? gk : gk + j;
+                    v[j] = (mm < M && kk < K) ? (transA ? A[(size_t)kk * lda + mm] : A[(size_t)mm * lda + kk]) : 0;
+                }
+            }
             if (transA) {
-                r = idx & 63;
-                kk = idx >> 6;
+#pragma unroll
+                for (int j = 0; j < 8; ++j) As[(a1 + j) * LDK + a0] = v[j];
             } else {
-                kk = idx & 31;
-                r = idx >> 5;
+                uint4 q;
+                unsigned short *e = reinterpret_cast<unsigned short *>(&q);
+#pragma unroll
+                for (int j = 0; j < 8; ++j) e[j] = v[j];
+                *reinterpret_cast<uint4 *>(&As[a0 * LDK + a1]) = q;
             }
-            const int gm = m0 + r, gk = k0 + kk;
-            unsigned short v = 0;
-            if (gm < M && gk < K) v = transA ? A[(size_t)gk * lda + gm] : A[(size_t)gm * lda + gk];
-            As[r * LDK + kk] = v;
-            int c, kb;
+        }
+        {
+            // B: k x columns n of the tile; contiguous along n (or along k when transB)
+            const int b0 = transB ? (tid >> 2) : (tid >> 3);        // n (transB) or k
+            const int b1 = transB ? 8 * (tid & 3) : 8 * (tid & 7);  // first k (transB) or n of the vector
+            const int gn = n0 + (transB ? b0 : b1), gk = k0 + (transB ? b1 : b0);
+            unsigned short v[8];
+            const bool full = transB ? (gn < N && gk + 7 < K) : (gk < K && gn + 7 < N);
+            if (alB && full) {
+                const uint4 q = *reinterpret_cast<const uint4 *>(B + (transB ? (size_t)gn * ldb + gk
+                                                                             : (size_t)gk * ldb + gn));
+                const unsigned short *e = reinterpret_cast<const unsigned short *>(&q);
+#pragma unroll
+                for (int j = 0; j < 8; ++j) v[j] = e[j];
+            } else {
+#pragma unroll
+                for (int j = 0; j < 8; ++j) {
+                    const int nn_ = transB ? gn : gn + j, kk = transB ? gk + j : gk;
+                    v[j] = (nn_ < N && kk < K) ? (transB ? B[(size_t)nn_ * ldb + kk] : B[(size_t)kk * ldb + nn_]) : 0;
+                }
+            }
             if (transB) {
-                kb = idx & 31;
-                c = idx >> 5;
+                uint4 q;
+                unsigned short *e = reinterpret_cast<unsigned short *>(&q);
+#pragma unroll
+                for (int j = 0; j < 8; ++j) e[j] = v[j];
+                *reinterpret_cast<uint4 *>(&Bs[b0 * LDK + b1]) = q;
             } else {
-                c = idx & 63;
-                kb = idx >> 6;
+#pragma unroll
+                for (int j = 0; j < 8; ++j) Bs[(b1 + j) * LDK + b0] = v[j];
             }
-            const int gn = n0 + c, gkb = k0 + kb;
-            unsigned short w = 0;
-            if (gn < N && gkb < K) w = transB ? B[(size_t)gn * ldb + gkb] : B[(size_t)gkb * ldb + gn];
-            Bs[c * LDK + kb] = w;
         }
         __syncthreads();
         const int fr = lane & 15, fk = 8 * (lane >> 4);

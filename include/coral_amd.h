@@ -293,14 +293,15 @@ int coral_amd_use_graphs(coral_amd_ctx *ctx, int32_t enable);
 
 /* Learned subgrid closure of a coarse run (no counterpart in the reference).
  * The network of coral_amd/closure.py (per horizontal plane: a lift from the
- * nfull full variables and the height to `width` channels, `layers` residual
+ * nfull full variables on each point's 3 x 3 x 3 neighbourhood (periodic in x
+ * and y, clamped at the walls) and the height to `width` channels, `layers` residual
  * layers of a GELU MLP of `hidden` channels plus a spectral convolution over
  * the plane's Fourier modes |kx|, |ky| <= modes, a projection to the nquad
  * quadratic variables) maps the full variables of each stage to corrections
  * of the quadratic variables, whose spectra are added to theirs before the NL
  * terms are assembled (every stage, and in coral_amd_products).  weights: the
  * flat FP32 parameters in that layout (nweights of them); in_mean / in_std
- * (nfull + 1) normalise the inputs, out_mean / out_std (nquad) scale the
+ * (27 nfull + 1) normalise the inputs, out_mean / out_std (nquad) scale the
  * outputs back.  GEMMs run in BF16 on MFMA with FP32 accumulation.  NULL
  * removes the closure.  FP64 transform chain only (precision mode 0). */
 typedef struct {

@@ -59,3 +59,40 @@ def test_fourier_basis_is_orthonormal(native_lib):
     B = fourier_basis(24, 18, 4)
     assert B.shape == (24 * 18, 2 * (4 * 9 + 5) - 1)
     np.testing.assert_allclose(B.T @ B, np.eye(B.shape[1]), atol=1e-12)
+
+
+def test_box_filter_is_exact_on_coarse_states(native_lib):
+    """A coarse state embedded in the fine grid (same integer wavenumbers,
+    same leading Galerkin coefficients of every variable) filters back to
+    itself; the filter of a fine state zeroes nothing the coarse grid keeps
+    that the fine state has, and quadratic spectra keep exactly the coarse
+    wavenumbers and degrees."""
+    from coral_amd import ic
+    from coral_amd.closure import BoxFilter
+    from coral_amd.equations import read_equations
+    from conftest import ROOT
+    import os
+    r = read_equations(os.path.join(ROOT, "coral_amd", "inputs", "coral.equations.rbc_ra1e5"))
+    fine, coarse = (32, 24, 17), (16, 12, 9)
+    box = BoxFilter(r, fine, coarse)
+    kc, zc = ic.rbc_initial_state(r, *coarse, 1.0, 50, 50, 3)
+    rng = np.random.default_rng(2)
+    zc = [rng.normal(size=a.shape) for a in zc]
+    # embed: inverse of the filter's index maps
+    kf = [np.zeros((fine[0] // 2, 3 * fine[1] // 2, sum(fine[2] - bc // 10 for bc in s.bc_codes)), np.complex128)
+          for s in r.kxky_systems]
+    zf = [np.zeros(sum(fine[2] - bc // 10 for bc in s.bc_codes)) for s in r.zero_systems]
+    for (pf, pc, _), a, b in zip(box.kpos, kf, kc):
+        for i, ix in enumerate(box.ix):
+            for j, iy in enumerate(box.iy):
+                a[ix, iy, pf] = b[i, j, pc]
+    for (pf, pc, _), a, b in zip(box.zpos, zf, zc):
+        a[pf] = b[pc]
+    k2, z2 = box.state(kf, zf)
+    for a, b in zip(k2 + z2, kc + zc):
+        assert np.array_equal(a, b)
+    q = rng.normal(size=(9, fine[0] // 2, 3 * fine[1] // 2, fine[2])) + 0j
+    qc = box.spectra(q)
+    assert qc.shape == (9, coarse[0] // 2, 3 * coarse[1] // 2, coarse[2])
+    assert np.array_equal(qc[:, box.keep], q[:, box.ix][:, :, box.iy][..., :coarse[2]][:, box.keep])
+    assert not qc[:, ~box.keep].any()

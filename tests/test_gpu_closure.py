@@ -125,10 +125,11 @@ def test_step_closure_matches_network(nnmod):
     NX, NZ, M = 32, 17, 3
     NXAA, NZAA = 3 * NX // 2, 3 * NZ // 2
     basis = closure.fourier_basis(NXAA, NXAA, M)
-    lay = closure.Layout(cin=5, cout=9, width=64, hidden=256, layers=4, nb=basis.shape[1])
+    cin = 27 * 4 + 1
+    lay = closure.Layout(cin=cin, cout=9, width=64, hidden=256, layers=4, nb=basis.shape[1])
     rng = np.random.default_rng(4)
-    model = closure.Model(lay, M, closure.init_params(lay, 11) * 3.0, rng.normal(size=5) * 0.1,
-                          1.0 + rng.random(5), rng.normal(size=9) * 0.1, 0.5 + rng.random(9))
+    model = closure.Model(lay, M, closure.init_params(lay, 11) * 3.0, rng.normal(size=cin) * 0.1,
+                          1.0 + rng.random(cin), rng.normal(size=9) * 0.1, 0.5 + rng.random(9))
     k0, z0 = seeded_state(r, NX, NX, NZ, 6, 6, 9, amps=(("u", 1.0), ("v", 1.0), ("w", 1.0), ("t", 0.1)))
     with Solver(r, NX, NX, NZ, 2 * np.pi, 2 * np.pi) as g:
         g.set_state(k0, z0)

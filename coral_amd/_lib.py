@@ -78,6 +78,7 @@ EXPORTS = (
     "coral_amd_set_closure", "coral_amd_nn_gemm", "coral_amd_nn_cast_bf16", "coral_amd_nn_bias_act", "coral_amd_nn_gelu_bwd",
     "coral_amd_nn_colsum_work", "coral_amd_nn_colsum", "coral_amd_nn_mode_scale", "coral_amd_nn_mode_scale_grad",
     "coral_amd_nn_axpy", "coral_amd_nn_mse_partials", "coral_amd_nn_mse", "coral_amd_nn_adam",
+    "coral_amd_nn_gemm_work", "coral_amd_nn_transpose_bf16", "coral_amd_nn_mlp",
 )
 
 
@@ -130,6 +131,7 @@ def _load():
     lib.coral_amd_banded_solve_noalloc_bytes.restype = ctypes.c_size_t
     lib.coral_amd_nn_colsum_work.restype = ctypes.c_int64
     lib.coral_amd_nn_mse_partials.restype = ctypes.c_int64
+    lib.coral_amd_nn_gemm_work.restype = ctypes.c_int64
     return lib
 
 

@@ -16,7 +16,8 @@ variables before the NL terms are assembled (``coral_amd_set_closure``).
 Network, per plane of N2 = NXAA NYAA points (rows of a plane contiguous); a
 point's input X is every full variable at the point and its differences to
 the 26 neighbours of its 3 x 3 x 3 neighbourhood (the local gradients that
-subgrid products depend on, formed before the BF16 rounding) and the height:
+subgrid products depend on, formed before the BF16 rounding), the height,
+and zero columns up to a multiple of 8 (16-byte BF16 rows):
 
     H = X W0 + b0                                    lift, Cin -> width
     4 times:  H <- H + gelu(H W1 + b1) W2 + b2       pointwise MLP, width -> 4 width -> width
@@ -29,6 +30,11 @@ spectrum per channel and D scales each (mode, channel).  Inputs and targets
 are normalised per channel with the training set's statistics, which travel
 with the weights.  Every GEMM reads BF16 operands and accumulates in FP32;
 master weights, biases, activations' pre-images and Adam's moments are FP32.
+The MLP branch runs as one fused kernel (coral_amd_nn_mlp: the hidden
+activations stay on chip at inference); B is kept transposed (nb x N2) and
+the weights W1, W2 also transposed, so that every GEMM operand row is 16-byte
+aligned; the per-plane projections onto the modes are split along the plane's
+points.
 
 Device memory comes from coral_amd.devmem (the HIP runtime the library
 links); every arithmetic operation on the device is a kernel of
@@ -53,7 +59,10 @@ def _bind():
     P = c_void_p
     sig = {
         "coral_amd_nn_gemm": [c_int32, c_int32, c_int32, c_int32, c_int32, c_float, P, c_int32, c_int64, P, c_int32,
-                              c_int64, c_float, P, c_int32, c_int64, c_int32, P],
+                              c_int64, c_float, P, c_int32, c_int64, c_int32, P, c_int64, P],
+        "coral_amd_nn_gemm_work": [c_int32, c_int32, c_int32, c_int32],
+        "coral_amd_nn_transpose_bf16": [P, c_int32, c_int32, P, P],
+        "coral_amd_nn_mlp": [c_int64, c_int32, c_int32, P, P, P, P, P, P, P, P, P, P],
         "coral_amd_nn_cast_bf16": [P, P, c_int64, P],
         "coral_amd_nn_bias_act": [P, P, c_int32, c_int64, c_int32, P, P, P],
         "coral_amd_nn_gelu_bwd": [P, P, P, P, c_int64, P],
@@ -183,8 +192,9 @@ def plane_rows(fields: np.ndarray, zgrid: np.ndarray, stencil: bool = True) -> n
     clamped at the walls in z; offsets dz, dy, dx = -1, 0, 1 in that nesting,
     the fields innermost: 27 nfield columns), the point's own values as they
     are and each neighbour as its difference from them (the BF16 inputs then
-    keep the digits of the local differences), then the height; without
-    stencil the fields at the point and the height (nfield + 1 columns)."""
+    keep the digits of the local differences), then the height and zero
+    columns up to input_width(nfield); without stencil the fields at the point
+    and the height (nfield + 1 columns)."""
     nf, NX, NY, NZ = fields.shape
     f = np.transpose(fields, (3, 2, 1, 0))              # [z][y][x][field]
     cols = []
@@ -201,7 +211,16 @@ def plane_rows(fields: np.ndarray, zgrid: np.ndarray, stencil: bool = True) -> n
     else:
         cols.append(f.reshape(NZ * NY * NX, nf))
     cols.append(np.repeat(zgrid, NY * NX)[:, None])
+    if stencil:
+        ncol = 27 * nf + 1
+        cols.append(np.zeros((NZ * NY * NX, input_width(nf) - ncol)))
     return np.concatenate(cols, axis=1)
+
+
+def input_width(nfull: int) -> int:
+    """Columns of a network input row for nfull full variables: the 3 x 3 x 3
+    neighbourhood's values and the height, padded to a multiple of 8."""
+    return (27 * nfull + 1 + 7) // 8 * 8
 
 
 def normalise(rows: np.ndarray, mean: np.ndarray, std: np.ndarray) -> np.ndarray:
@@ -222,7 +241,6 @@ class _Work:
         self.H16 = [DevArray(R * W, "bf16") for _ in range(L.layers + 1)]
         self.Z = [DevArray(R * Hd) for _ in range(L.layers)]
         self.A16 = [DevArray(R * Hd, "bf16") for _ in range(L.layers)]
-        self.U = DevArray(R * W)
         self.Cm = [DevArray(P * nb * W) for _ in range(L.layers)]
         self.Cs16 = DevArray(P * nb * W, "bf16")
         self.Y = DevArray(R * L.cout)
@@ -238,6 +256,11 @@ class _Work:
         self.wpart = DevArray(P * max(Hd * W, L.cin * W, W * L.cout))
         self.work = DevArray(max(int(lib.coral_amd_nn_colsum_work(R, max(Hd, W, L.cout))),
                                  int(lib.coral_amd_nn_colsum_work(P, Hd * W))))
+        # split-K partials of the long-K products (mode projections and the
+        # per-plane weight gradients)
+        shapes = [(nb, W, N2, P), (W, L.cout, N2, P), (Hd, W, N2, P), (W, Hd, N2, P), (L.cin, W, N2, P)]
+        self.ngwork = max(int(lib.coral_amd_nn_gemm_work(*sh)) for sh in shapes)
+        self.gwork = DevArray(max(self.ngwork, 1))
 
 
 class ClosureNet:
@@ -248,13 +271,17 @@ class ClosureNet:
         self.L = layout
         self.N2, nb = basis.shape
         assert nb == layout.nb
-        self.basis32 = DevArray.from_numpy(basis)
-        self.basis16 = DevArray(basis.size, "bf16")
-        self._cast(self.basis32, self.basis16)
+        if layout.width not in (32, 64) or layout.hidden not in (128, 256):
+            raise ValueError("width must be 32 or 64 and hidden 128 or 256")
+        basisT32 = DevArray.from_numpy(np.ascontiguousarray(basis.T))
+        self.basisT16 = DevArray(basis.size, "bf16")      # nb x N2
+        self._cast(basisT32, self.basisT16)
+        del basisT32
         p = init_params(layout, seed) if params is None else np.asarray(params, np.float32)
         self.w = DevArray.from_numpy(p)
         self.w16 = DevArray(layout.size, "bf16")
         self._cast(self.w, self.w16)
+        self.wt16 = DevArray(2 * layout.layers * layout.width * layout.hidden, "bf16")
         self.g = DevArray(layout.size, zero=True)
         self.m = DevArray(layout.size, zero=True)
         self.v = DevArray(layout.size, zero=True)
@@ -262,6 +289,7 @@ class ClosureNet:
         self.off = layout.offsets()
         self.work: Dict[int, _Work] = {}
         self.last = None
+        self._transpose_weights()
 
     # --- helpers -------------------------------------------------------------
     stream = None
@@ -272,9 +300,23 @@ class ClosureNet:
     def _p(self, name, t):
         return t.at(self.off[name][0])
 
-    def _gemm(self, tA, tB, M, N, K, A, lda, sA, B, ldb, sB, C, ldc, sC, batch=1, beta=0.0, alpha=1.0):
-        check(lib.coral_amd_nn_gemm(tA, tB, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch, None),
-              "nn gemm")
+    def _gemm(self, tA, tB, M, N, K, A, lda, sA, B, ldb, sB, C, ldc, sC, batch=1, beta=0.0, alpha=1.0, ws=None):
+        gw, ngw = (ws.gwork.ptr, ws.ngwork) if ws is not None else (None, 0)
+        check(lib.coral_amd_nn_gemm(tA, tB, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch, gw, ngw,
+                                    None), "nn gemm")
+
+    def _wt(self, l):
+        """Device pointers of W1^T (hidden x width) and W2^T (width x hidden) of layer l."""
+        W, Hd = self.L.width, self.L.hidden
+        o = 2 * l * W * Hd
+        return self.wt16.at(o), self.wt16.at(o + W * Hd)
+
+    def _transpose_weights(self):
+        W, Hd = self.L.width, self.L.hidden
+        for l in range(self.L.layers):
+            t1, t2 = self._wt(l)
+            check(lib.coral_amd_nn_transpose_bf16(self._p(f"W1.{l}", self.w16), W, Hd, t1, None), "nn transpose")
+            check(lib.coral_amd_nn_transpose_bf16(self._p(f"W2.{l}", self.w16), Hd, W, t2, None), "nn transpose")
 
     def _ws(self, P) -> _Work:
         if P not in self.work:
@@ -287,7 +329,7 @@ class ClosureNet:
         their sum (a split of the long row dimension over workgroups)."""
         P = ws.P
         self._gemm(1, 0, M, N, self.N2, A16.ptr, lda, self.N2 * lda, B16.ptr, ldb, self.N2 * ldb,
-                   ws.wpart.ptr, N, M * N, batch=P)
+                   ws.wpart.ptr, N, M * N, batch=P, ws=ws)
         check(lib.coral_amd_nn_colsum(ws.wpart.ptr, P, M * N, out_ptr, ws.work.ptr, 0.0, None), "nn colsum")
 
     def params(self) -> np.ndarray:
@@ -309,20 +351,19 @@ class ClosureNet:
               "nn bias")
         for l in range(L.layers):
             Hin16, Hout16 = ws.H16[l], ws.H16[l + 1]
-            Z, A16, Cm = ws.Z[l], ws.A16[l], ws.Cm[l]
-            self._gemm(0, 0, R, Hd, W, Hin16.ptr, W, 0, self._p(f"W1.{l}", self.w16), Hd, 0, Z.ptr, Hd, 0)
-            check(lib.coral_amd_nn_bias_act(Z.ptr, self._p(f"b1.{l}", self.w), Hd, R * Hd, 1, None, A16.ptr, None),
-                  "nn bias gelu")
-            self._gemm(0, 0, R, W, Hd, A16.ptr, Hd, 0, self._p(f"W2.{l}", self.w16), W, 0, ws.U.ptr, W, 0)
-            self._gemm(1, 0, nb, W, N2, self.basis16.ptr, nb, 0, Hin16.ptr, W, N2 * W, Cm.ptr, W, nb * W, batch=P)
+            Cm = ws.Cm[l]
+            # spectral branch: Cm = B^T H16 per plane, Cs = Cm D, H <- H + B Cs
+            self._gemm(0, 0, nb, W, N2, self.basisT16.ptr, N2, 0, Hin16.ptr, W, N2 * W, Cm.ptr, W, nb * W, batch=P,
+                       ws=ws)
             check(lib.coral_amd_nn_mode_scale(Cm.ptr, self._p(f"D.{l}", self.w), None, ws.Cs16.ptr, P, nb * W, None),
                   "nn mode scale")
-            # H <- H + B Cs (the GEMM accumulates, beta = 1), then + U + b2
-            self._gemm(0, 0, N2, W, nb, self.basis16.ptr, nb, 0, ws.Cs16.ptr, W, nb * W, ws.H.ptr, W, N2 * W,
+            self._gemm(1, 0, N2, W, nb, self.basisT16.ptr, N2, 0, ws.Cs16.ptr, W, nb * W, ws.H.ptr, W, N2 * W,
                        batch=P, beta=1.0)
-            check(lib.coral_amd_nn_bias_act(ws.U.ptr, self._p(f"b2.{l}", self.w), W, R * W, 0, None, None, None),
-                  "nn bias")
-            check(lib.coral_amd_nn_axpy(ws.H.ptr, ws.U.ptr, 1.0, R * W, Hout16.ptr, None), "nn add")
+            # MLP branch, fused: H <- H + gelu(H16 W1 + b1) W2 + b2, keeping Z and A for the backward pass
+            t1, t2 = self._wt(l)
+            check(lib.coral_amd_nn_mlp(R, W, Hd, Hin16.ptr, t1, self._p(f"b1.{l}", self.w), t2,
+                                       self._p(f"b2.{l}", self.w), ws.H.ptr, Hout16.ptr, ws.Z[l].ptr, ws.A16[l].ptr,
+                                       None), "nn mlp")
         self._gemm(0, 0, R, L.cout, W, ws.H16[L.layers].ptr, W, 0, self._p("Wp", self.w16), L.cout, 0, ws.Y.ptr,
                    L.cout, 0)
         check(lib.coral_amd_nn_bias_act(ws.Y.ptr, self._p("bp", self.w), L.cout, R * L.cout, 0, None, None, None),
@@ -362,8 +403,8 @@ class ClosureNet:
             self._wgrad(ws.H16[l], W, ws.dZ16, Hd, W, Hd, ws, self._p(f"W1.{l}", g))
             colsum(ws.dA, Hd, f"b1.{l}")
             # spectral branch: dCs = B^T dH per plane; dD = sum_p Cm dCs; dCm = dCs D
-            self._gemm(1, 0, nb, W, N2, self.basis16.ptr, nb, 0, ws.dH16.ptr, W, N2 * W, ws.dCs.ptr, W, nb * W,
-                       batch=P)
+            self._gemm(0, 0, nb, W, N2, self.basisT16.ptr, N2, 0, ws.dH16.ptr, W, N2 * W, ws.dCs.ptr, W, nb * W,
+                       batch=P, ws=ws)
             check(lib.coral_amd_nn_mode_scale_grad(ws.Cm[l].ptr, ws.dCs.ptr, self._p(f"D.{l}", g), P, nb * W, 0.0,
                                                    None), "nn mode grad")
             check(lib.coral_amd_nn_mode_scale(ws.dCs.ptr, self._p(f"D.{l}", self.w), None, ws.dCm16.ptr, P, nb * W,
@@ -371,7 +412,7 @@ class ClosureNet:
             # dH (residual) += dZ W1^T + B dCm
             self._gemm(0, 1, R, W, Hd, ws.dZ16.ptr, Hd, 0, self._p(f"W1.{l}", self.w16), Hd, 0, ws.dH.ptr, W, 0,
                        beta=1.0)
-            self._gemm(0, 0, N2, W, nb, self.basis16.ptr, nb, 0, ws.dCm16.ptr, W, nb * W, ws.dH.ptr, W, N2 * W,
+            self._gemm(1, 0, N2, W, nb, self.basisT16.ptr, N2, 0, ws.dCm16.ptr, W, nb * W, ws.dH.ptr, W, N2 * W,
                        batch=P, beta=1.0)
         self._cast(ws.dH, ws.dH16)
         self._wgrad(X16, L.cin, ws.dH16, W, L.cin, W, ws, self._p("W0", g))
@@ -381,6 +422,7 @@ class ClosureNet:
         self.t += 1
         check(lib.coral_amd_nn_adam(self.w.ptr, self.g.ptr, self.m.ptr, self.v.ptr, self.w16.ptr, self.L.size, lr, b1,
                                     b2, eps, self.t, None), "nn adam")
+        self._transpose_weights()
 
     def train_step(self, X16: DevArray, T: DevArray, P: int, lr: float) -> float:
         Y = self.forward(X16, P)
@@ -402,24 +444,35 @@ class DevView:
 
 
 class BoxFilter:
-    """Sharp spectral (box) filter from a fine grid to a coarse one: the
-    horizontal modes the coarse grid keeps, matched by integer wavenumber,
-    and for every coupled variable its first Galerkin coefficients (as many as
-    the coarse NZ gives it); quadratic spectra are cut to the coarse kx, ky and
-    Chebyshev degrees n < NZ_coarse."""
+    """Filter from a fine grid to a coarse one.  Horizontally (the periodic
+    directions) kind "tophat" is the box filter proper, the average over one
+    coarse cell (width Lx / NXc by Ly / NYc), whose transfer function on mode
+    (kx, ky) (integer wavenumbers) is sinc(kx / NXc) sinc(ky / NYc) with
+    sinc(s) = sin(pi s) / (pi s); kind "sharp" passes the coarse modes
+    unchanged.  Either way the result keeps the horizontal modes the coarse
+    grid holds, matched by integer wavenumber, and for every coupled variable
+    its first Galerkin coefficients (as many as the coarse NZ gives it); the
+    vertical direction is cut sharply.  Quadratic spectra are weighted alike
+    and cut to the coarse kx, ky and Chebyshev degrees n < NZ_coarse."""
 
-    def __init__(self, recipe, fine: Tuple[int, int, int], coarse: Tuple[int, int, int]):
+    def __init__(self, recipe, fine: Tuple[int, int, int], coarse: Tuple[int, int, int], kind: str = "tophat"):
         from .ic import interleaved_positions, wavenumber_indices
         (NXf, NYf, NZf), (NXc, NYc, NZc) = fine, coarse
         if NXc > NXf or NYc > NYf or NZc > NZf:
             raise ValueError("the coarse grid must not exceed the fine one")
-        self.fine, self.coarse = fine, coarse
+        if kind not in ("tophat", "sharp"):
+            raise ValueError("kind must be 'tophat' or 'sharp'")
+        self.fine, self.coarse, self.kind = fine, coarse, kind
         kxf, kyf, _ = wavenumber_indices(NXf, NYf)
         kxc, kyc, keepc = wavenumber_indices(NXc, NYc)
         self.keep = keepc
         slot_f = {int(k): i for i, k in enumerate(kyf)}
         self.iy = np.array([slot_f[int(k)] for k in kyc])
         self.ix = np.array([int(k) for k in kxc])
+        if kind == "tophat":
+            self.weight = np.sinc(np.asarray(kxc, float) / NXc)[:, None] * np.sinc(np.asarray(kyc, float) / NYc)[None]
+        else:
+            self.weight = np.ones((len(kxc), len(kyc)))
         self.kpos, self.zpos = [], []
         for systems, out in ((recipe.kxky_systems, self.kpos), (recipe.zero_systems, self.zpos)):
             for s in systems:
@@ -435,7 +488,7 @@ class BoxFilter:
         for (pf, pc, total), kf in zip(self.kpos, k_fine):
             kc = np.zeros((NXc // 2, 3 * NYc // 2, total), np.complex128)
             sub = kf[self.ix][:, self.iy]
-            kc[:, :, pc] = sub[:, :, pf]
+            kc[:, :, pc] = sub[:, :, pf] * self.weight[:, :, None]
             kc[~self.keep] = 0.0
             ks.append(kc)
         for (pf, pc, total), zf in zip(self.zpos, z_fine):
@@ -447,7 +500,7 @@ class BoxFilter:
     def spectra(self, q_fine: np.ndarray) -> np.ndarray:
         """(nquad, NX/2, NYAA, NZ) fine spectra -> the coarse grid's."""
         NZc = self.coarse[2]
-        out = q_fine[:, self.ix][:, :, self.iy][..., :NZc].copy()
+        out = q_fine[:, self.ix][:, :, self.iy][..., :NZc] * self.weight[None, :, :, None]
         out[:, ~self.keep] = 0.0
         return out
 

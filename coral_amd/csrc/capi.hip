@@ -162,8 +162,8 @@ struct coral_amd_ctx {
         nn::NetParams net;
         nn::NetWork ws;
         int chunk = 0;   // planes per network launch
-        DevBuf<float> w, mu_in, sd_in, mu_out, sd_out, Y, H, Z, U, Cm;
-        DevBuf<unsigned short> w16, basis16, X16, H16, A16, Cs16;
+        DevBuf<float> w, mu_in, sd_in, mu_out, sd_out, Y, H, Cm, gwork;
+        DevBuf<unsigned short> w16, wt16, basisT16, X16, H16, Cs16;
         DevBuf<double> phys_full, phys_tau, zgrid;
         DevBuf<double2> Qtau;
     } closure;
@@ -755,32 +755,65 @@ __device__ inline unsigned short bf16_of(float f) {
 // innermost), the point's own values as they are and every neighbour as its
 // difference from them (formed in FP64, so that the BF16 inputs keep the
 // local differences' digits), normalised ((v - mean) / std in FP64, rounded
-// to FP32 then BF16), and the height of the plane, normalised likewise
-__global__ void k_closure_pack(const double *__restrict__ phys, const double *__restrict__ zgrid, int z0, int P,
-                               int NXAA, int NYAA, int NZAA, int nfull, const float *__restrict__ mu,
-                               const float *__restrict__ sd, unsigned short *__restrict__ X16) {
-    const int N2 = NXAA * NYAA;
-    const long long r = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= (long long)P * N2) return;
-    const int z = z0 + int(r / N2), i = int(r % N2), y = i / NXAA, x = i % NXAA, cin = 27 * nfull + 1;
-    int c = 0;
-    for (int dz = -1; dz <= 1; ++dz) {
-        const int zz = min(max(z + dz, 0), NZAA - 1);
-        for (int dy = -1; dy <= 1; ++dy) {
-            const int yy = (y + dy + NYAA) % NYAA;
-            for (int dx = -1; dx <= 1; ++dx) {
-                const int xx = (x + dx + NXAA) % NXAA;
-                const bool centre = dz == 0 && dy == 0 && dx == 0;
-                for (int f = 0; f < nfull; ++f, ++c) {
-                    const double v0 = phys[((size_t)f * NZAA + z) * N2 + i];
-                    const double v = phys[((size_t)f * NZAA + zz) * N2 + (size_t)yy * NXAA + xx];
-                    const double d = centre ? v0 : v - v0;
-                    X16[r * cin + c] = bf16_of(float((d - double(mu[c])) / double(sd[c])));
-                }
-            }
-        }
+// to FP32 then BF16), the height of the plane, normalised likewise, and zero
+// columns up to cin (a multiple of 8, for 16-byte rows).  A workgroup fills
+// PACK_ROWS consecutive rows (each neighbouring line read once into the
+// caches): the rows' positions are decoded once into LDS, and each thread
+// keeps one column (its neighbour offset and variable decoded once) over
+// rows rs apart, so that the stores of a sweep are contiguous and no
+// division remains per element.
+constexpr int PACK_ROWS = 64;
+__global__ __launch_bounds__(256) void k_closure_pack(const double *__restrict__ phys,
+                                                      const double *__restrict__ zgrid, int z0, int P, int NXAA,
+                                                      int NYAA, int NZAA, int nfull, int cin,
+                                                      const float *__restrict__ mu, const float *__restrict__ sd,
+                                                      unsigned short *__restrict__ X16) {
+    __shared__ int rz[PACK_ROWS], ry[PACK_ROWS], rx[PACK_ROWS];
+    const int N2 = NXAA * NYAA, ncore = 27 * nfull, nrow = P * N2;
+    const int r0 = blockIdx.x * PACK_ROWS, nr = min(PACK_ROWS, nrow - r0);
+    for (int rl = threadIdx.x; rl < nr; rl += blockDim.x) {
+        const int r = r0 + rl, zl = r / N2, i = r - zl * N2, y = i / NXAA;
+        rz[rl] = z0 + zl;
+        ry[rl] = y;
+        rx[rl] = i - y * NXAA;
     }
-    X16[r * cin + c] = bf16_of(float((zgrid[z] - double(mu[c])) / double(sd[c])));
+    __syncthreads();
+    unsigned short *out = X16 + (size_t)r0 * cin;
+    const int nt = blockDim.x, rs = cin <= nt ? nt / cin : 1;
+    for (int cbase = 0; cbase < cin; cbase += nt) {
+        int c, ro;
+        if (cin <= nt) {
+            c = threadIdx.x % cin;
+            ro = threadIdx.x / cin;
+            if (ro >= rs) break;
+        } else {
+            c = cbase + threadIdx.x;
+            ro = 0;
+            if (c >= cin) break;
+        }
+        const int o = c < ncore ? c / nfull : 0, f = c < ncore ? c - o * nfull : 0;
+        const int dz = o / 9 - 1, dy = (o / 3) % 3 - 1, dx = o % 3 - 1;
+        const double m = double(mu[c]), s = double(sd[c]);
+        for (int rl = ro; rl < nr; rl += rs) {
+            const int z = rz[rl], y = ry[rl], x = rx[rl];
+            double d = 0.0;
+            if (c < ncore) {
+                const double v0 = phys[((size_t)f * NZAA + z) * N2 + (size_t)y * NXAA + x];
+                if (o == 13) {
+                    d = v0;
+                } else {
+                    const int zz = min(max(z + dz, 0), NZAA - 1);
+                    const int yy = y + dy < 0 ? NYAA - 1 : (y + dy == NYAA ? 0 : y + dy);
+                    const int xx = x + dx < 0 ? NXAA - 1 : (x + dx == NXAA ? 0 : x + dx);
+                    d = phys[((size_t)f * NZAA + zz) * N2 + (size_t)yy * NXAA + xx] - v0;
+                }
+            } else if (c == ncore) {
+                d = zgrid[z];
+            }
+            out[(size_t)rl * cin + c] = bf16_of(float((d - m) / s));
+        }
+        if (cin <= nt) break;
+    }
 }
 
 // the network's output rows of planes z0 .. back to physical corrections
@@ -814,8 +847,9 @@ void closure_correction(coral_amd_ctx &c) {
         for (int z0 = 0; z0 < P.NZAA; z0 += cl.chunk) {
             const int np = std::min(cl.chunk, P.NZAA - z0);
             const unsigned g = unsigned(((long long)np * N2 + 255) / 256);
-            hipLaunchKernelGGL(k_closure_pack, dim3(g), dim3(256), 0, c.stream, cl.phys_full.p, cl.zgrid.p, z0, np,
-                               P.NXAA, P.NYAA, P.NZAA, P.nfull, cl.mu_in.p, cl.sd_in.p, cl.X16.p);
+            const unsigned gp = unsigned(((long long)np * N2 + PACK_ROWS - 1) / PACK_ROWS);
+            hipLaunchKernelGGL(k_closure_pack, dim3(gp), dim3(256), 0, c.stream, cl.phys_full.p, cl.zgrid.p, z0, np,
+                               P.NXAA, P.NYAA, P.NZAA, P.nfull, cl.net.cin, cl.mu_in.p, cl.sd_in.p, cl.X16.p);
             check_launch("closure pack");
             nn::net_forward(c.stream, cl.net, cl.X16.p, np, cl.Y.p, cl.ws);
             hipLaunchKernelGGL(k_closure_unpack, dim3(g), dim3(256), 0, c.stream, cl.Y.p, z0, np, N2, P.NZAA, P.nquad,
@@ -2083,8 +2117,8 @@ int coral_amd_set_closure(coral_amd_ctx *c, const coral_amd_closure *cl) {
         if (P.nfull < 1 || P.nquad < 1) throw std::runtime_error("the closure needs full and quadratic variables");
         if (!cl->weights || !cl->in_mean || !cl->in_std || !cl->out_mean || !cl->out_std)
             throw std::runtime_error("closure: null array");
-        if (cl->width < 1 || cl->hidden < 1 || cl->layers < 0 || cl->modes < 0)
-            throw std::runtime_error("closure: bad network shape");
+        if (cl->layers < 0 || cl->modes < 0 || !nn::mlp_supported(cl->width, cl->hidden))
+            throw std::runtime_error("closure: bad network shape (width 32 or 64, hidden 128 or 256)");
         const int NXAA = P.NXAA, NYAA = P.NYAA, N2 = NXAA * NYAA, M = cl->modes;
         if (2 * M >= NXAA || 2 * M >= NYAA) throw std::runtime_error("closure: modes must stay below the grid's Nyquist");
         // the plane's Fourier columns (coral_amd.closure.fourier_basis): cos and
@@ -2111,12 +2145,12 @@ int coral_amd_set_closure(coral_amd_ctx *c, const coral_amd_closure *cl) {
                 }
             }
         const int nb = int(cols.size());
-        std::vector<unsigned short> b16(size_t(N2) * nb);
-        for (int i = 0; i < N2; ++i)
-            for (int k = 0; k < nb; ++k) b16[size_t(i) * nb + k] = nn::host_bf16(float(cols[k][i]));
+        std::vector<unsigned short> b16(size_t(N2) * nb);   // transposed: nb x N2
+        for (int k = 0; k < nb; ++k)
+            for (int i = 0; i < N2; ++i) b16[size_t(k) * N2 + i] = nn::host_bf16(float(cols[k][i]));
         nn::NetParams &n = C.net;
         n = nn::NetParams{};
-        n.cin = 27 * P.nfull + 1;
+        n.cin = (27 * P.nfull + 1 + 7) / 8 * 8;
         n.cout = P.nquad;
         n.width = cl->width;
         n.hidden = cl->hidden;
@@ -2130,32 +2164,53 @@ int coral_amd_set_closure(coral_amd_ctx *c, const coral_amd_closure *cl) {
         std::vector<float> wh(cl->weights, cl->weights + n.size);
         std::vector<unsigned short> w16(n.size);
         for (long long i = 0; i < n.size; ++i) w16[i] = nn::host_bf16(wh[i]);
+        // per layer W1^T (Hd x W) and W2^T (W x Hd), the fused MLP's operands
+        const int W = n.width, Hd = n.hidden;
+        std::vector<unsigned short> wt(size_t(n.layers) * 2 * W * Hd);
+        for (int l = 0; l < n.layers; ++l) {
+            unsigned short *t1 = wt.data() + size_t(l) * 2 * W * Hd, *t2 = t1 + size_t(W) * Hd;
+            for (int k = 0; k < W; ++k)
+                for (int j = 0; j < Hd; ++j) {
+                    t1[size_t(j) * W + k] = w16[n.off_W1[l] + size_t(k) * Hd + j];
+                    t2[size_t(k) * Hd + j] = w16[n.off_W2[l] + size_t(j) * W + k];
+                }
+        }
         C.w.upload(wh);
         C.w16.upload(w16);
-        C.basis16.upload(b16);
+        C.wt16.upload(wt);
+        C.basisT16.upload(b16);
         C.mu_in.upload(std::vector<float>(cl->in_mean, cl->in_mean + n.cin));
         C.sd_in.upload(std::vector<float>(cl->in_std, cl->in_std + n.cin));
         C.mu_out.upload(std::vector<float>(cl->out_mean, cl->out_mean + n.cout));
         C.sd_out.upload(std::vector<float>(cl->out_std, cl->out_std + n.cout));
         n.w = C.w.p;
         n.w16 = C.w16.p;
-        n.basis16 = C.basis16.p;
+        n.wt16 = C.wt16.p;
+        n.basisT16 = C.basisT16.p;
         std::vector<double> zg(P.NZAA);
         for (int k = 0; k < P.NZAA; ++k) zg[k] = P.center + 0.5 * P.gap * std::cos(M_PI * (2 * k + 1) / (2.0 * P.NZAA));
         C.zgrid.upload(zg);
-        // planes per network launch: about 600k rows of activations at a time
-        C.chunk = std::max(1, std::min(P.NZAA, int(600000 / N2)));
+        // planes per network launch: about 4M rows at a time (the fused MLP
+        // keeps the hidden activations on chip; a row costs 2 cin + 4 cout +
+        // 6 width bytes)
+        C.chunk = std::max(1, std::min(P.NZAA, int(4000000 / N2)));
         const size_t R = size_t(C.chunk) * N2;
         C.X16.alloc(R * n.cin);
         C.Y.alloc(R * n.cout);
         C.H.alloc(R * n.width);
         C.H16.alloc(R * n.width);
-        C.Z.alloc(R * n.hidden);
-        C.A16.alloc(R * n.hidden);
-        C.U.alloc(R * n.width);
         C.Cm.alloc(size_t(C.chunk) * nb * n.width);
         C.Cs16.alloc(size_t(C.chunk) * nb * n.width);
-        C.ws = nn::NetWork{C.H.p, C.Z.p, C.U.p, C.Cm.p, C.H16.p, C.A16.p, C.Cs16.p};
+        long long ngw = 0;
+        for (int p = 1; p <= C.chunk; ++p) ngw = std::max(ngw, nn::gemm_work(nb, n.width, N2, p));
+        C.gwork.alloc(size_t(std::max(ngw, 1LL)));
+        C.ws = nn::NetWork{};
+        C.ws.H = C.H.p;
+        C.ws.Cm = C.Cm.p;
+        C.ws.gwork = C.gwork.p;
+        C.ws.ngwork = ngw;
+        C.ws.H16 = C.H16.p;
+        C.ws.Cs16 = C.Cs16.p;
         C.phys_full.alloc(size_t(P.nfull) * P.NZAA * N2);
         C.phys_tau.alloc(size_t(P.nquad) * P.NZAA * N2);
         C.Qtau.alloc(size_t(P.nquad) * P.nsys * P.NZ);

@@ -11,9 +11,21 @@ namespace nn {
 // C[b] = alpha op(A[b]) op(B[b]) + beta C[b], b < batch (strides in elements;
 // 0 shares an operand): op(A) M x K (A is M x K row-major, or K x M when
 // transA), op(B) K x N (B is K x N, or N x K when transB), C FP32 M x N.
+// Products with few output tiles and a long K are split along K into
+// partials in work (gemm_work(M, N, K, batch) floats; throws if smaller).
 void gemm(hipStream_t st, int transA, int transB, int M, int N, int K, float alpha, const unsigned short *A, int lda,
           long long sA, const unsigned short *B, int ldb, long long sB, float beta, float *C, int ldc, long long sC,
-          int batch);
+          int batch, float *work = nullptr, long long nwork = 0);
+long long gemm_work(int M, int N, int K, int batch);
+// dst (cols x rows) = src (rows x cols)^T, BF16
+void transpose16(hipStream_t st, const unsigned short *src, int rows, int cols, unsigned short *dst);
+// fused MLP branch: H += bf16(gelu(Hin16 W1 + b1)) W2 + b2, Hout16 = bf16(H);
+// W1t = W1^T (HD x W), W2t = W2^T (W x HD), BF16; Zc (R x HD, the
+// pre-activation) and A16c (R x HD, the BF16 activation) are written when given
+bool mlp_supported(int W, int HD);
+void mlp(hipStream_t st, long long R, int W, int HD, const unsigned short *Hin16, const unsigned short *W1t,
+         const float *b1, const unsigned short *W2t, const float *b2, float *H, unsigned short *Hout16, float *Zc,
+         unsigned short *A16c);
 void cast_bf16(hipStream_t st, const float *x, unsigned short *y, long long n);
 // z += bias (per column of ncol), out = act(z) (act: 0 identity, 1 GELU), out16 its bf16 copy
 void bias_act(hipStream_t st, float *z, const float *bias, int ncol, long long n, int act, float *out,
@@ -43,17 +55,21 @@ struct NetParams {
     int cin = 0, cout = 0, width = 0, hidden = 0, layers = 0, nb = 0, N2 = 0;
     const float *w = nullptr;              // FP32 weights (biases and mode scales are read in FP32)
     const unsigned short *w16 = nullptr;   // BF16 copy (GEMM operands)
-    const unsigned short *basis16 = nullptr;  // N2 x nb, BF16
+    const unsigned short *wt16 = nullptr;  // per layer W1^T (Hd x W) then W2^T (W x Hd), BF16
+    const unsigned short *basisT16 = nullptr;  // the Fourier basis transposed, nb x N2, BF16
     long long off_W0 = 0, off_b0 = 0, off_Wp = 0, off_bp = 0;
     long long off_W1[16] = {}, off_b1[16] = {}, off_W2[16] = {}, off_b2[16] = {}, off_D[16] = {};
     long long size = 0;                    // total number of parameters
 };
 // offsets of the blocks for the given shapes (throws for more than 16 layers)
 void net_layout(NetParams &n);
-// inference workspace for up to Pmax planes of N2 points
+// inference workspace for up to Pmax planes of N2 points: H (Pmax N2 x W),
+// H16 (the same, BF16), Cm (Pmax nb x W), Cs16 (the same, BF16), and the
+// split-K work space of the mode projection (gemm_work(nb, W, N2, Pmax))
 struct NetWork {
-    float *H = nullptr, *Z = nullptr, *U = nullptr, *Cm = nullptr;
-    unsigned short *H16 = nullptr, *A16 = nullptr, *Cs16 = nullptr;
+    float *H = nullptr, *Cm = nullptr, *gwork = nullptr;
+    unsigned short *H16 = nullptr, *Cs16 = nullptr;
+    long long ngwork = 0;
 };
 // Y (P N2 x cout, FP32) = network(X16 (P N2 x cin, BF16)), P <= the workspace's planes
 void net_forward(hipStream_t st, const NetParams &n, const unsigned short *X16, int P, float *Y, const NetWork &w);

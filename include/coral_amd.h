@@ -301,9 +301,11 @@ int coral_amd_use_graphs(coral_amd_ctx *ctx, int32_t enable);
  * of the quadratic variables, whose spectra are added to theirs before the NL
  * terms are assembled (every stage, and in coral_amd_products).  weights: the
  * flat FP32 parameters in that layout (nweights of them); in_mean / in_std
- * (27 nfull + 1) normalise the inputs, out_mean / out_std (nquad) scale the
- * outputs back.  GEMMs run in BF16 on MFMA with FP32 accumulation.  NULL
- * removes the closure.  FP64 transform chain only (precision mode 0). */
+ * (cin = 27 nfull + 1 rounded up to a multiple of 8: the neighbourhood's
+ * values, the height, zero columns) normalise the inputs, out_mean / out_std
+ * (nquad) scale the outputs back.  width 32 or 64, hidden 128 or 256.  GEMMs
+ * run in BF16 on MFMA with FP32 accumulation.  NULL removes the closure.
+ * FP64 transform chain only (precision mode 0). */
 typedef struct {
     int32_t width, hidden, layers, modes;
     const float *weights;
@@ -357,10 +359,25 @@ int coral_amd_ensemble_synchronize(coral_amd_ensemble *ens);
  * on MFMA (BF16 operands, FP32 accumulation and C): op(A) is M x K (A stored M
  * x K row-major with leading dimension lda, or K x M when transA), op(B) K x N
  * (B stored K x N, or N x K when transB); sA, sB, sC are the batch strides in
- * elements (0 shares an operand). */
+ * elements (0 shares an operand).  Products with few output tiles and a long
+ * K are split along K: work then holds coral_amd_nn_gemm_work(M, N, K, batch)
+ * floats (0: no split, work may be NULL); the partials are added in a fixed
+ * order, so results do not depend on timing. */
 int coral_amd_nn_gemm(int32_t transA, int32_t transB, int32_t M, int32_t N, int32_t K, float alpha, const void *A,
                       int32_t lda, int64_t sA, const void *B, int32_t ldb, int64_t sB, float beta, float *C,
-                      int32_t ldc, int64_t sC, int32_t batch, void *stream);
+                      int32_t ldc, int64_t sC, int32_t batch, float *work, int64_t nwork, void *stream);
+int64_t coral_amd_nn_gemm_work(int32_t M, int32_t N, int32_t K, int32_t batch);
+/* dst (cols x rows) = src (rows x cols) transposed, BF16 */
+int coral_amd_nn_transpose_bf16(const void *src, int32_t rows, int32_t cols, void *dst, void *stream);
+/* One MLP branch of a layer, fused, for R rows of `width` channels:
+ * H += bf16(gelu(Hin16 W1 + b1)) W2 + b2 and Hout16 = bf16(H) (Hout16 may be
+ * Hin16), the hidden activations kept on chip; W1t = W1^T (hidden x width)
+ * and W2t = W2^T (width x hidden) in BF16.  Zc (R x hidden, Hin16 W1 + b1)
+ * and A16c (R x hidden, the BF16 activations) are written for a backward pass
+ * when not NULL.  width 32 or 64, hidden 128 or 256.  Same results as
+ * coral_amd_nn_gemm, _bias_act (GELU), _gemm, _bias_act, _axpy. */
+int coral_amd_nn_mlp(int64_t R, int32_t width, int32_t hidden, const void *Hin16, const void *W1t, const float *b1,
+                     const void *W2t, const float *b2, float *H, void *Hout16, float *Zc, void *A16c, void *stream);
 int coral_amd_nn_cast_bf16(const float *x, void *y, int64_t n, void *stream);
 /* z += bias (per column of ncol); out = act(z) (act 0: identity, 1: GELU, tanh
  * form), out16 its BF16 copy (either may be NULL) */

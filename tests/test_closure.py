@@ -74,7 +74,7 @@ def test_box_filter_is_exact_on_coarse_states(native_lib):
     import os
     r = read_equations(os.path.join(ROOT, "coral_amd", "inputs", "coral.equations.rbc_ra1e5"))
     fine, coarse = (32, 24, 17), (16, 12, 9)
-    box = BoxFilter(r, fine, coarse)
+    box = BoxFilter(r, fine, coarse, kind="sharp")
     kc, zc = ic.rbc_initial_state(r, *coarse, 1.0, 50, 50, 3)
     rng = np.random.default_rng(2)
     zc = [rng.normal(size=a.shape) for a in zc]
@@ -96,3 +96,41 @@ def test_box_filter_is_exact_on_coarse_states(native_lib):
     assert qc.shape == (9, coarse[0] // 2, 3 * coarse[1] // 2, coarse[2])
     assert np.array_equal(qc[:, box.keep], q[:, box.ix][:, :, box.iy][..., :coarse[2]][:, box.keep])
     assert not qc[:, ~box.keep].any()
+
+
+def test_tophat_filter_is_the_cell_average(native_lib):
+    """The top-hat kind weights mode (kx, ky) by the average of exp(i (kx x +
+    ky y)) over one coarse cell centred on the point, computed here by
+    quadrature of the cell (sinc closed form pinned independently), and is
+    otherwise the sharp filter: the mean passes unchanged, every kept mode is
+    the sharp filter's times its weight."""
+    from coral_amd import ic
+    from coral_amd.closure import BoxFilter
+    from coral_amd.equations import read_equations
+    from conftest import ROOT
+    import os
+    r = read_equations(os.path.join(ROOT, "coral_amd", "inputs", "coral.equations.rbc_ra1e5"))
+    fine, coarse = (32, 24, 17), (16, 12, 9)
+    Lx, Ly = 2.0, 3.0
+    top, sharp = BoxFilter(r, fine, coarse), BoxFilter(r, fine, coarse, kind="sharp")
+    kxc, kyc, _ = ic.wavenumber_indices(coarse[0], coarse[1])
+    dx, dy = Lx / coarse[0], Ly / coarse[1]
+    s = (np.arange(4000) + 0.5) / 4000 - 0.5           # midpoint rule over the cell
+    for i, kx in enumerate(kxc):
+        for j, ky in enumerate(kyc):
+            ax = np.mean(np.exp(1j * 2 * np.pi * kx / Lx * s * dx))
+            ay = np.mean(np.exp(1j * 2 * np.pi * ky / Ly * s * dy))
+            assert abs(top.weight[i, j] - (ax * ay).real) < 1e-6 and abs((ax * ay).imag) < 1e-12
+    assert top.weight[0, list(kyc).index(0)] == 1.0
+    rng = np.random.default_rng(5)
+    kf = [rng.normal(size=(fine[0] // 2, 3 * fine[1] // 2, sum(fine[2] - bc // 10 for bc in sy.bc_codes)))
+          + 1j * rng.normal(size=(fine[0] // 2, 3 * fine[1] // 2, sum(fine[2] - bc // 10 for bc in sy.bc_codes)))
+          for sy in r.kxky_systems]
+    zf = [rng.normal(size=sum(fine[2] - bc // 10 for bc in sy.bc_codes)) for sy in r.zero_systems]
+    (kt, zt), (ks, zs) = top.state(kf, zf), sharp.state(kf, zf)
+    for a, b in zip(kt, ks):
+        np.testing.assert_allclose(a, b * top.weight[:, :, None], rtol=0, atol=1e-15)
+    for a, b in zip(zt, zs):
+        assert np.array_equal(a, b)
+    q = rng.normal(size=(9, fine[0] // 2, 3 * fine[1] // 2, fine[2])) + 0j
+    np.testing.assert_allclose(top.spectra(q), sharp.spectra(q) * top.weight[None, :, :, None], rtol=0, atol=1e-15)

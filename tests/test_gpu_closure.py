@@ -42,7 +42,7 @@ def test_bf16_gemm_matches_numpy(nnmod, tA, tB):
     Cd = devmem.DevArray.from_numpy(C0)
     lda, ldb = A.shape[2], B.shape[2]
     closure.check(closure.lib.coral_amd_nn_gemm(tA, tB, M, N, K, 0.5, Ad.ptr, lda, A[0].size, Bd.ptr,
-                                                ldb, B[0].size, 1.0, Cd.ptr, N, M * N, batch, None), "gemm")
+                                                ldb, B[0].size, 1.0, Cd.ptr, N, M * N, batch, None, 0, None), "gemm")
     got = Cd.numpy().astype(np.float64).reshape(batch, M, N)
     Aq, Bq = C.bf16(A), C.bf16(B)
     for b in range(batch):
@@ -50,6 +50,89 @@ def test_bf16_gemm_matches_numpy(nnmod, tA, tB):
         opB = Bq[b].T if tB else Bq[b]
         ref = 0.5 * opA @ opB + C0[b].astype(np.float32)
         assert np.linalg.norm(got[b] - ref) / np.linalg.norm(ref) < 1e-5
+
+
+@pytest.mark.parametrize("tA", [0, 1])
+def test_split_k_gemm_matches_numpy(nnmod, tA):
+    """A long K with few output tiles (the per-plane projection onto the
+    Fourier modes): split along K into partials, summed in a fixed order;
+    equal to numpy to FP32 summation, the same bits on a second call, and
+    refused without its work space."""
+    closure, devmem = nnmod
+    rng = np.random.default_rng(10 + tA)
+    M, N, K, batch = 81, 64, 9216, 3
+    A = rng.normal(size=(batch, K, M) if tA else (batch, M, K))
+    B = rng.normal(size=(batch, K, N))
+    C0 = rng.normal(size=(batch, M, N))
+    need = int(closure.lib.coral_amd_nn_gemm_work(M, N, K, batch))
+    assert need > 0
+    Ad, Bd = to_dev_bf16(closure, devmem, A), to_dev_bf16(closure, devmem, B)
+    work = devmem.DevArray(need)
+    outs = []
+    for _ in range(2):
+        Cd = devmem.DevArray.from_numpy(C0)
+        closure.check(closure.lib.coral_amd_nn_gemm(tA, 0, M, N, K, 0.5, Ad.ptr, A.shape[2], A[0].size, Bd.ptr, N,
+                                                    B[0].size, 1.0, Cd.ptr, N, M * N, batch, work.ptr, need, None),
+                      "gemm")
+        outs.append(Cd.numpy())
+    assert np.array_equal(outs[0], outs[1])
+    got = outs[0].astype(np.float64).reshape(batch, M, N)
+    Aq, Bq = C.bf16(A), C.bf16(B)
+    for b in range(batch):
+        ref = 0.5 * (Aq[b].T if tA else Aq[b]) @ Bq[b] + C0[b].astype(np.float32)
+        assert np.linalg.norm(got[b] - ref) / np.linalg.norm(ref) < 1e-5
+    Cd = devmem.DevArray.from_numpy(C0)
+    rc = closure.lib.coral_amd_nn_gemm(tA, 0, M, N, K, 0.5, Ad.ptr, A.shape[2], A[0].size, Bd.ptr, N, B[0].size, 1.0,
+                                       Cd.ptr, N, M * N, batch, None, 0, None)
+    assert rc < 0
+
+
+@pytest.mark.parametrize("W,Hd,R", [(64, 256, 1000), (32, 128, 64 * 7)])
+def test_fused_mlp_matches_unfused_sequence(nnmod, W, Hd, R):
+    """coral_amd_nn_mlp gives the bits of gemm, bias_act (GELU), gemm,
+    bias_act, axpy on the same operands, its caches those of the unfused
+    pre-activation and activation, also in place (Hout16 = Hin16) and with a
+    ragged last tile of rows."""
+    closure, devmem = nnmod
+    lib, check = closure.lib, closure.check
+    rng = np.random.default_rng(W + Hd)
+    H0 = rng.normal(size=(R, W)).astype(np.float32)
+    W1, W2 = rng.normal(size=(W, Hd)) / 8, rng.normal(size=(Hd, W)) / 16
+    b1, b2 = rng.normal(size=Hd).astype(np.float32), rng.normal(size=W).astype(np.float32)
+    W1d, W2d = to_dev_bf16(closure, devmem, W1), to_dev_bf16(closure, devmem, W2)
+    W1t, W2t = devmem.DevArray(W * Hd, "bf16"), devmem.DevArray(W * Hd, "bf16")
+    check(lib.coral_amd_nn_transpose_bf16(W1d.ptr, W, Hd, W1t.ptr, None), "t")
+    check(lib.coral_amd_nn_transpose_bf16(W2d.ptr, Hd, W, W2t.ptr, None), "t")
+    b1d, b2d = devmem.DevArray.from_numpy(b1), devmem.DevArray.from_numpy(b2)
+    Hin16 = to_dev_bf16(closure, devmem, H0)
+    # unfused
+    Hu = devmem.DevArray.from_numpy(H0)
+    Z, A16, U = devmem.DevArray(R * Hd), devmem.DevArray(R * Hd, "bf16"), devmem.DevArray(R * W)
+    Hu16 = devmem.DevArray(R * W, "bf16")
+    check(lib.coral_amd_nn_gemm(0, 0, R, Hd, W, 1.0, Hin16.ptr, W, 0, W1d.ptr, Hd, 0, 0.0, Z.ptr, Hd, 0, 1, None, 0,
+                                None), "g")
+    check(lib.coral_amd_nn_bias_act(Z.ptr, b1d.ptr, Hd, R * Hd, 1, None, A16.ptr, None), "b")
+    check(lib.coral_amd_nn_gemm(0, 0, R, W, Hd, 1.0, A16.ptr, Hd, 0, W2d.ptr, W, 0, 0.0, U.ptr, W, 0, 1, None, 0,
+                                None), "g")
+    check(lib.coral_amd_nn_bias_act(U.ptr, b2d.ptr, W, R * W, 0, None, None, None), "b")
+    check(lib.coral_amd_nn_axpy(Hu.ptr, U.ptr, 1.0, R * W, Hu16.ptr, None), "a")
+    # fused, caches on, then in place without caches
+    Hf, Hf16 = devmem.DevArray.from_numpy(H0), devmem.DevArray(R * W, "bf16")
+    Zf, Af16 = devmem.DevArray(R * Hd), devmem.DevArray(R * Hd, "bf16")
+    check(lib.coral_amd_nn_mlp(R, W, Hd, Hin16.ptr, W1t.ptr, b1d.ptr, W2t.ptr, b2d.ptr, Hf.ptr, Hf16.ptr, Zf.ptr,
+                               Af16.ptr, None), "mlp")
+    Hi, Hi16 = devmem.DevArray.from_numpy(H0), to_dev_bf16(closure, devmem, H0)
+    check(lib.coral_amd_nn_mlp(R, W, Hd, Hi16.ptr, W1t.ptr, b1d.ptr, W2t.ptr, b2d.ptr, Hi.ptr, Hi16.ptr, None, None,
+                               None), "mlp")
+    assert np.array_equal(Hf.numpy(), Hu.numpy())
+    assert np.array_equal(Hf16.numpy(), Hu16.numpy())
+    assert np.array_equal(Zf.numpy(), Z.numpy())
+    assert np.array_equal(Af16.numpy(), A16.numpy())
+    assert np.array_equal(Hi.numpy(), Hu.numpy())
+    assert np.array_equal(Hi16.numpy(), Hu16.numpy())
+    # and the math: float64 on the BF16 operands
+    ref = H0 + (C.bf16(C.gelu(C.bf16(H0) @ C.bf16(W1) + b1)) @ C.bf16(W2) + b2)
+    assert np.linalg.norm(Hu.numpy().reshape(R, W) - ref) / np.linalg.norm(ref) < 1e-3
 
 
 def test_network_forward_backward_match_reference(nnmod):
@@ -125,7 +208,7 @@ def test_step_closure_matches_network(nnmod):
     NX, NZ, M = 32, 17, 3
     NXAA, NZAA = 3 * NX // 2, 3 * NZ // 2
     basis = closure.fourier_basis(NXAA, NXAA, M)
-    cin = 27 * 4 + 1
+    cin = closure.input_width(4)
     lay = closure.Layout(cin=cin, cout=9, width=64, hidden=256, layers=4, nb=basis.shape[1])
     rng = np.random.default_rng(4)
     model = closure.Model(lay, M, closure.init_params(lay, 11) * 3.0, rng.normal(size=cin) * 0.1,

@@ -124,21 +124,6 @@ def cpu_baseline(wl, recipe, k0, z0, nsteps=1):
     return nsteps / dt, t_setup, dt
 
 
-def horizontal_spectrum(k_state, positions, NX, NY, Lx, Ly):
-    """Energy of the state's Galerkin coefficients of the given positions (e.g.
-    the velocity variables) summed per horizontal wavenumber shell
-    |k| in [m, m+1) (integer shells of k / (2 pi / Lx)); kx > 0 counted twice
-    (the conjugate half of the spectrum)."""
-    import numpy as np
-    from coral_amd.ic import wavenumber_indices
-    kx, ky, kept = wavenumber_indices(NX, NY)
-    e = np.sum(np.abs(k_state[:, :, positions]) ** 2, axis=2)
-    e = np.where(kept, e, 0.0) * np.where(kx[:, None] > 0, 2.0, 1.0)
-    kk = np.sqrt((kx[:, None] * 2 * np.pi / Lx) ** 2 + (ky[None, :] * 2 * np.pi / Ly) ** 2) / (2 * np.pi / Lx)
-    shell = np.floor(kk + 0.5).astype(int)
-    return np.bincount(shell.ravel(), weights=e.ravel())
-
-
 def bench_ensemble(args):
     """Config 3: 48 independent Rayleigh-Benard runs drawn from a seed (NX = NY
     in {64, 96, 128}, NZ in {33, 49, 65}, Ra log-uniform in [1e5, 1e7], Pr in
@@ -269,7 +254,7 @@ def bench_mixed(args):
             prof = g.variable_physical("zero", tsys, tvar)
             nu = D.nusselt_wall_differences(prof, g.gap)
             k, _ = g.get_state()
-            spec = horizontal_spectrum(k[0], vel, NX, NX, 2 * math.pi, 2 * math.pi)
+            spec = D.horizontal_spectrum(k[0], vel, NX, NX, 2 * math.pi, 2 * math.pi)
             res[mode] = dict(elapsed=el, ke=ke, nu=nu, spec=spec, cfl=g.cfl_dt(), finite=bool(np.isfinite(k[0]).all()))
             log(f"{mode}: {steps} steps in {el:.3f} s, KE {ke:.6e}, Nu (top, bottom) {nu}, CFL dt {res[mode]['cfl']:.3e}")
             g.use_graphs(False)
@@ -306,14 +291,168 @@ def bench_mixed(args):
     print(json.dumps(out), flush=True)
 
 
+CLOSURE_WORKLOADS = {
+    # learned closure, evaluated at 128^2 x 65 (the coarse grid of the
+    # closure's target configuration) against a 256^2 x 129 fine run
+    "closure128": dict(fine=(256, 129), coarse=(128, 65), spin=(128, 65, 12000, 3000), snapshots=64, every=20,
+                       epochs=12, t_eval=0.02),
+    # the same at half the size in every direction
+    "closure64": dict(fine=(128, 65), coarse=(64, 33), spin=(128, 65, 12000, 0), snapshots=150, every=25,
+                      epochs=20, t_eval=0.05),
+}
+
+
+def bench_closure(args, name):
+    """Config 5 at reduced scale: Rayleigh-Benard convection at Ra = 1e6, Pr = 1
+    (2pi x 2pi x 1 box, noisy start, adaptive dt: dt_max 1e-3, CFL 0.2).  A fine
+    run spun up to the statistically steady state (on a coarser grid first
+    when the workload says so, then embedded); snapshots of the fine run are
+    box-filtered (top-hat over one coarse cell) onto the coarse grid and give
+    the training pairs (the filtered full variables on each point's 3 x 3 x 3
+    neighbourhood -> the subgrid part of every quadratic variable); the
+    network (width 32, hidden 128, 4 layers, modes |k| <= 4) is trained on the
+    first 5/6 of the snapshots and scored on the rest (relative L2 error; 1 =
+    no closure).  Then, from the filtered final fine state, the fine run, the
+    plain coarse run and the coarse run with the closure each advance t_eval
+    time units: Nusselt numbers (mean of the wall estimator every 50 steps) and
+    simulated time per second of each.  The metric is the closure run's
+    sim-time/s; its Nu error against the fine run beside the plain coarse
+    run's."""
+    import numpy as np
+    from coral_amd import closure as CL
+    from coral_amd import ic
+    from coral_amd.diagnostics import gauss_chebyshev_grid
+    from coral_amd.ensemble import rbc_recipe
+    from coral_amd.solver import Solver
+    from coral_amd.timekeeping import Timings
+    wl = CLOSURE_WORKLOADS[name]
+    Ra, L = 1e6, 2 * math.pi
+    r = rbc_recipe(Ra, 1.0)
+    (NXf, NZf), (NXc, NZc) = wl["fine"], wl["coarse"]
+    NXs, NZs, nspin, nspin_fine = wl["spin"]
+    tsys = [i for i, z in enumerate(r.zero_systems) if "tMean" in z.var_names][0]
+    tvar = r.zero_systems[tsys].var_names.index("tMean")
+
+    def nusselt(g):
+        p = g.variable_physical("zero", tsys, tvar)
+        dz = 0.5 * (1 - math.cos(math.pi / (2 * p.size)))
+        return 0.5 * (p[0] - p[-1]) / dz
+
+    def advance(g, tk, n):
+        for _ in range(n):
+            tk.check_for_update()
+            if tk.refactorization_is_due:
+                g.factorize_operators(tk.dt_current)
+                tk.refactorization_is_due = False
+            g.march_forward(tk.dt_current, 1)
+            tk.apply_cfl(g.cfl_dt())
+            tk.increment()
+
+    t_all = time.perf_counter()
+    tk = Timings(1e-3, 0.3, 0.2)
+    gs = Solver(r, NXs, NXs, NZs, L, L)
+    gs.use_graphs(True)
+    gs.set_kx0_projection(True)
+    k0, z0 = ic.noisy_initial_state(r, NXs, NXs, NZs, L, L, 1e-2, 1e-2, 5)
+    gs.set_state(k0, z0)
+    for i in range(nspin // 1000):
+        advance(gs, tk, 1000)
+        log(f"closure: spin-up {1000 * (i + 1)} steps at {NXs}x{NZs}, t = {tk.absolute_time:.4f}, Nu = {nusselt(gs):.3f}")
+    if (NXs, NZs) != (NXf, NZf):
+        k, z = gs.get_state()
+        gs.close()
+        gf = Solver(r, NXf, NXf, NZf, L, L)
+        kf, zf = CL.BoxFilter(r, (NXf, NXf, NZf), (NXs, NXs, NZs), kind="sharp").embed(k, z)
+        gf.set_state(kf, zf)
+        tk.refactorization_is_due = True
+    else:
+        gf = gs
+    gf.use_graphs(True)
+    gf.set_kx0_projection(True)
+    for i in range(nspin_fine // 500):
+        advance(gf, tk, 500)
+        log(f"closure: spin-up {500 * (i + 1)} steps at {NXf}x{NZf}, t = {tk.absolute_time:.4f}, Nu = {nusselt(gf):.3f}")
+    gc = Solver(r, NXc, NXc, NZc, L, L)
+    box = CL.BoxFilter(r, (NXf, NXf, NZf), (NXc, NXc, NZc))
+    zg = gauss_chebyshev_grid(3 * NZc // 2, 0.5, 1.0)
+    X, T = [], []
+    t0 = time.perf_counter()
+    for i in range(wl["snapshots"]):
+        advance(gf, tk, wl["every"])
+        xi, ti = CL.subgrid_sample(gf, gc, box, zg)
+        X.append(xi)
+        T.append(ti)
+        if (i + 1) % 16 == 0:
+            log(f"closure: {i + 1} snapshots, t = {tk.absolute_time:.4f} ({time.perf_counter() - t0:.0f} s)")
+    N2 = (3 * NXc // 2) ** 2
+    nval = max(1, len(X) // 6)
+    Xtr, Ttr = np.concatenate(X[:-nval]), np.concatenate(T[:-nval])
+    Xva, Tva = np.concatenate(X[-nval:]), np.concatenate(T[-nval:])
+    del X, T
+    basis = CL.fourier_basis(3 * NXc // 2, 3 * NXc // 2, 4)
+    t0 = time.perf_counter()
+    model, hist = CL.train(Xtr, Ttr, N2, basis, 4, width=32, hidden=128, epochs=wl["epochs"], planes_per_batch=16,
+                           lr=2e-3, log=log)
+    t_train = time.perf_counter() - t0
+    pred = CL.predict(model, Xva, N2, basis)
+    err = np.linalg.norm(pred - Tva, axis=0) / np.linalg.norm(Tva, axis=0)
+    err_all = float(np.linalg.norm(pred - Tva) / np.linalg.norm(Tva))
+    log(f"closure: trained in {t_train:.1f} s; held-out relative L2 error {err_all:.3f} per variable {np.round(err, 3)}")
+    kf, zf = gf.get_state()
+    kc, zc = box.state(kf, zf)
+    res = {}
+    for run, g, use in (("fine", gf, None), ("coarse", gc, False), ("coarse_closure", gc, True)):
+        if run != "fine":
+            g.set_closure(model if use else None)
+            g.set_kx0_projection(True)
+            g.use_graphs(True)
+            g.set_state(kc, zc)
+        tke = Timings(1e-3, 0.3, 0.2)
+        tke.absolute_time = 0.0
+        nus, n = [], 0
+        g.synchronize()
+        t0 = time.perf_counter()
+        while tke.absolute_time < wl["t_eval"]:
+            advance(g, tke, 50)
+            n += 50
+            nus.append(nusselt(g))
+            if not np.isfinite(nus[-1]):
+                break
+        el = time.perf_counter() - t0
+        res[run] = dict(nu=float(np.mean(nus)), nu_std=float(np.std(nus)), steps=n, sim_per_s=tke.absolute_time / el,
+                        ms_per_step=1e3 * el / n, finite=bool(np.all(np.isfinite(nus))))
+        log(f"closure: {run}: Nu {res[run]['nu']:.4f} +- {res[run]['nu_std']:.4f} over {n} steps, "
+            f"{res[run]['sim_per_s']:.5f} sim-time/s, {res[run]['ms_per_step']:.3f} ms/step")
+    nu_f = res["fine"]["nu"]
+    c = res["coarse_closure"]
+    out = {
+        "metric": "sim-time/s of the coarse run with the learned closure, vs the fine GPU run, at its Nu error",
+        "value": round(c["sim_per_s"], 6), "unit": "simulated time units/s", "n_gpus": 1, "steps": c["steps"],
+        "ms_per_step": round(c["ms_per_step"], 4), "higher_is_better": True,
+        "dtype": "f64 solver, bf16 network (f32 accumulate)", "data": "synthetic (noisy start, self-generated training set)",
+        "config": {"workload": f"RBC Ra=1e6 Pr=1, fine {NXf}^2x{NZf} -> coarse {NXc}^2x{NZc}, top-hat box filter, "
+                               f"{wl['snapshots']} snapshots every {wl['every']} steps, {wl['epochs']} epochs, "
+                               f"evaluation over {wl['t_eval']} time units",
+                   "network": {"width": 32, "hidden": 128, "layers": 4, "modes": 4, "params": int(model.layout.size)}},
+        "vs_fine": round(c["sim_per_s"] / res["fine"]["sim_per_s"], 4),
+        "nu_rel_error": {"coarse_closure": abs(c["nu"] / nu_f - 1), "coarse": abs(res["coarse"]["nu"] / nu_f - 1)},
+        "a_priori_rel_l2": {"overall": err_all, "per_quadratic_variable": [round(float(e), 4) for e in err]},
+        "runs": res, "training_s": round(t_train, 1), "wall_s": round(time.perf_counter() - t_all, 1),
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--workload", default="rrbc256", choices=sorted(WORKLOADS) + ["ensemble48", "mixed512"],
+    ap.add_argument("--workload", default="rrbc256",
+                    choices=sorted(WORKLOADS) + ["ensemble48", "mixed512"] + sorted(CLOSURE_WORKLOADS),
                     help="ensemble48: 48 uneven runs as one ensemble against solo runs; mixed512: mixed precision "
-                         "against FP64 at 512^2 x 257 (sim-time/s and KE, Nu, spectrum errors)")
+                         "against FP64 at 512^2 x 257 (sim-time/s and KE, Nu, spectrum errors); closure128 / "
+                         "closure64: a learned closure trained on a filtered fine run, coarse sim-time/s and Nu "
+                         "error against the fine and the plain coarse runs")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--eager-only", action="store_true",
                     help="skip the second timed pass of the same steps replayed as HIP graphs")
@@ -345,6 +484,8 @@ def main():
         return bench_ensemble(args)
     if args.workload == "mixed512":
         return bench_mixed(args)
+    if args.workload in CLOSURE_WORKLOADS:
+        return bench_closure(args, args.workload)
     wl = WORKLOADS[args.workload]
     steps = args.steps if args.steps is not None else wl["steps"]
 

@@ -473,14 +473,16 @@ class BoxFilter:
             self.weight = np.sinc(np.asarray(kxc, float) / NXc)[:, None] * np.sinc(np.asarray(kyc, float) / NYc)[None]
         else:
             self.weight = np.ones((len(kxc), len(kyc)))
-        self.kpos, self.zpos = [], []
-        for systems, out in ((recipe.kxky_systems, self.kpos), (recipe.zero_systems, self.zpos)):
+        self.kpos, self.zpos, self.kfine_total, self.zfine_total = [], [], [], []
+        for systems, out, tot in ((recipe.kxky_systems, self.kpos, self.kfine_total),
+                                  (recipe.zero_systems, self.zpos, self.zfine_total)):
             for s in systems:
                 sf = [NZf - bc // 10 for bc in s.bc_codes]
                 sc = [NZc - bc // 10 for bc in s.bc_codes]
                 pf = np.concatenate([interleaved_positions(len(sf), sf, v)[: sc[v]] for v in range(len(sf))])
                 pc = np.concatenate([interleaved_positions(len(sc), sc, v) for v in range(len(sc))])
                 out.append((pf, pc, sum(sc)))
+                tot.append(sum(sf))
 
     def state(self, k_fine: List[np.ndarray], z_fine: List[np.ndarray]):
         NXc, NYc, _ = self.coarse
@@ -495,6 +497,26 @@ class BoxFilter:
             zc = np.zeros(total)
             zc[pc] = zf[pf]
             zs.append(zc)
+        return ks, zs
+
+    def embed(self, k_coarse: List[np.ndarray], z_coarse: List[np.ndarray]):
+        """A coarse state on the fine grid: each coarse mode and Galerkin
+        coefficient at the same integer wavenumbers and degree, everything
+        else zero (the inverse of the sharp filter's index maps; filter
+        weights are not undone).  For spinning a fine run up from a coarse
+        one."""
+        NXf, NYf, _ = self.fine
+        ks, zs = [], []
+        for (pf, pc, _), nf, kc in zip(self.kpos, self.kfine_total, k_coarse):
+            sub = np.zeros((len(self.ix), len(self.iy), nf), np.complex128)
+            sub[:, :, pf] = np.where(self.keep[:, :, None], kc[:, :, pc], 0.0)
+            kf = np.zeros((NXf // 2, 3 * NYf // 2, nf), np.complex128)
+            kf[np.ix_(self.ix, self.iy)] = sub
+            ks.append(kf)
+        for (pf, pc, _), nf, zc in zip(self.zpos, self.zfine_total, z_coarse):
+            zf = np.zeros(nf)
+            zf[pf] = zc[pc]
+            zs.append(zf)
         return ks, zs
 
     def spectra(self, q_fine: np.ndarray) -> np.ndarray:

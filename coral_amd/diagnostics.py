@@ -186,3 +186,20 @@ def nusselt_quarters(time: np.ndarray, t_top: np.ndarray, t_bot: np.ndarray, NZA
     mean = float(np.mean(vals))
     std = float(np.sqrt(np.mean((vals - mean) ** 2)))
     return quarters, mean, std
+
+
+def horizontal_spectrum(k_state: np.ndarray, positions, NX: int, NY: int, Lx: float, Ly: float) -> np.ndarray:
+    """Horizontal spectrum of a kxky system's state (NX/2, NYAA, total): the
+    squared moduli of the Galerkin coefficients at `positions` (e.g. the
+    velocity variables), summed over them and per wavenumber shell
+    |k| / (2 pi / Lx) rounded to the nearest integer; kx > 0 counted twice
+    (the conjugate half of the real fields' spectrum); dealiased slots
+    excluded.  Entry m is shell m."""
+    from .ic import wavenumber_indices
+    kx, ky, kept = wavenumber_indices(NX, NY)
+    e = np.sum(np.abs(k_state[:, :, positions]) ** 2, axis=2)
+    e = np.where(kept, e, 0.0) * np.where(np.asarray(kx)[:, None] > 0, 2.0, 1.0)
+    kk = np.sqrt((np.asarray(kx)[:, None] * 2 * np.pi / Lx) ** 2 +
+                 (np.asarray(ky)[None, :] * 2 * np.pi / Ly) ** 2) / (2 * np.pi / Lx)
+    shell = np.floor(kk + 0.5).astype(int)
+    return np.bincount(shell.ravel(), weights=e.ravel())

@@ -26,6 +26,8 @@ INPUTS = {
     "rrbc_julien.coral.equations.toroidalPoloidal": "etc/benchmarks/02_RRBC_julien_JFM96/coral.equations.toroidalPoloidal",
     "rbc_vanilla.coral.equations": "etc/coral_equations_examples/coral.equations.RBC_primitiveVars_Vanilla",
     "example.coral.parameters.in": "etc/coral_equations_examples/coral.parameters.in",
+    "rrbc_julien.coral.usrOutput": "etc/benchmarks/02_RRBC_julien_JFM96/coral.usrOutput",
+    "example.coral.usrOutput": "etc/coral_equations_examples/coral.usrOutput",
 }
 
 

@@ -91,6 +91,10 @@ def test_box_filter_is_exact_on_coarse_states(native_lib):
     k2, z2 = box.state(kf, zf)
     for a, b in zip(k2 + z2, kc + zc):
         assert np.array_equal(a, b)
+    # embed is that embedding (dealiased coarse slots stay zero)
+    ke, ze = box.embed(kc, zc)
+    for a, b in zip(ke + ze, kf + zf):
+        assert np.array_equal(a, b)
     q = rng.normal(size=(9, fine[0] // 2, 3 * fine[1] // 2, fine[2])) + 0j
     qc = box.spectra(q)
     assert qc.shape == (9, coarse[0] // 2, 3 * coarse[1] // 2, coarse[2])

@@ -78,3 +78,23 @@ def test_nusselt_quarters_follows_compute_nusselt():
     ramp = np.arange(n, dtype=float) * dz
     quarters, _, _ = D.nusselt_quarters(t2, ramp, -ramp, NZAA, gap)
     assert quarters[1][0] == pytest.approx(float(q), rel=1e-3)
+
+
+def test_horizontal_spectrum_shells():
+    """Single modes land in the shell of their |k| with weight 2 for kx > 0
+    and 1 for kx = 0; dealiased slots are excluded; other positions do not
+    count; a box with Ly = 2 Lx halves the y wavenumbers."""
+    NX, NY, tot = 16, 16, 5
+    kx, ky, kept = __import__("coral_amd.ic", fromlist=["x"]).wavenumber_indices(NX, NY)
+    iy = {int(k): i for i, k in enumerate(ky)}
+    s = np.zeros((NX // 2, 3 * NY // 2, tot), complex)
+    s[3, iy[-4], 1] = 2.0 + 1.0j          # |k| = 5, kx > 0: 2 * 5 = 10
+    s[0, iy[2], 3] = 3.0                  # |k| = 2, kx = 0: 9
+    s[1, iy[1], 0] = 100.0                # position not counted
+    s[0, iy[-8], 1] = 50.0                # |ky| = 8 = NY / 2: dealiased
+    sp = D.horizontal_spectrum(s, [1, 3], NX, NY, 2 * np.pi, 2 * np.pi)
+    assert sp[5] == pytest.approx(10.0) and sp[2] == pytest.approx(9.0)
+    assert sp.sum() == pytest.approx(19.0)
+    sp2 = D.horizontal_spectrum(s, [1, 3], NX, NY, 2 * np.pi, 4 * np.pi)
+    assert sp2[1] == pytest.approx(9.0)   # ky = 2 on a box twice as long in y: |k| = 1
+    assert sp2[round(np.hypot(3, 2))] == pytest.approx(10.0)

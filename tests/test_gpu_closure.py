@@ -237,3 +237,60 @@ def test_step_closure_matches_network(nnmod):
     print("closure correction vs network + transform, worst rel L2:", worst, flush=True)
     assert worst <= 1e-10
     assert np.array_equal(q2, q0)
+
+
+def test_closure_learns_filtered_subgrid_terms(nnmod):
+    """End to end at small size: Rayleigh-Benard at Ra = 1e6, Pr = 1 on
+    64^2 x 33 spun up from noise (adaptive dt), 48 snapshots top-hat filtered
+    onto 32^2 x 17; the network trained on the first 40 predicts the subgrid
+    part of the quadratic variables of the last 8 with a relative L2 error
+    well below 1, the error of no closure (a model that learned nothing, or
+    inputs and targets paired wrongly, stays near 1)."""
+    closure, devmem = nnmod
+    import math
+    from coral_amd import ic
+    from coral_amd.diagnostics import gauss_chebyshev_grid
+    from coral_amd.ensemble import rbc_recipe
+    from coral_amd.solver import Solver
+    from coral_amd.timekeeping import Timings
+    r = rbc_recipe(1e6, 1.0)
+    L = 2 * math.pi
+    (NXf, NZf), (NXc, NZc) = (64, 33), (32, 17)
+    tk = Timings(1e-3, 0.3, 0.2)
+
+    def advance(g, n):
+        for _ in range(n):
+            tk.check_for_update()
+            if tk.refactorization_is_due:
+                g.factorize_operators(tk.dt_current)
+                tk.refactorization_is_due = False
+            g.march_forward(tk.dt_current, 1)
+            tk.apply_cfl(g.cfl_dt())
+            tk.increment()
+    with Solver(r, NXf, NXf, NZf, L, L) as gf, Solver(r, NXc, NXc, NZc, L, L) as gc:
+        gf.use_graphs(True)
+        gf.set_kx0_projection(True)
+        k0, z0 = ic.noisy_initial_state(r, NXf, NXf, NZf, L, L, 1e-2, 1e-2, 5)
+        gf.set_state(k0, z0)
+        advance(gf, 6000)
+        print(f"spun up to t = {tk.absolute_time:.4f}", flush=True)
+        box = closure.BoxFilter(r, (NXf, NXf, NZf), (NXc, NXc, NZc))
+        zg = gauss_chebyshev_grid(3 * NZc // 2, 0.5, 1.0)
+        X, T = [], []
+        for _ in range(48):
+            advance(gf, 20)
+            xi, ti = closure.subgrid_sample(gf, gc, box, zg)
+            X.append(xi)
+            T.append(ti)
+    assert all(np.isfinite(x).all() for x in X + T)
+    N2 = (3 * NXc // 2) ** 2
+    basis = closure.fourier_basis(3 * NXc // 2, 3 * NXc // 2, 4)
+    model, hist = closure.train(np.concatenate(X[:40]), np.concatenate(T[:40]), N2, basis, 4, width=32, hidden=128,
+                                epochs=10, planes_per_batch=16, lr=2e-3)
+    Tva = np.concatenate(T[40:])
+    pred = closure.predict(model, np.concatenate(X[40:]), N2, basis)
+    err = np.linalg.norm(pred - Tva, axis=0) / np.linalg.norm(Tva, axis=0)
+    overall = float(np.linalg.norm(pred - Tva) / np.linalg.norm(Tva))
+    print("training loss by epoch:", np.round(hist, 4), "held-out error:", overall, np.round(err, 3), flush=True)
+    assert hist[-1] < hist[0]
+    assert overall < 0.85

@@ -178,3 +178,47 @@ def test_adaptive_time_step_matches_oracle(solver_mod):
     assert len(change(used_gpu)) >= 2, "the run should refactorize at least twice"
     assert rel(k[0], ora.field[0]) <= 1e-10
     assert abs(t_gpu.absolute_time / t_ora.absolute_time - 1.0) <= 1e-10
+
+
+def test_user_outputs_from_the_device_state(solver_mod, tmp_path):
+    """coral.usrOutput objects written from a GPU run's state: a volume of a
+    full variable holds the oracle's physical field of the same state (in the
+    reference's z-fastest order), a quadratic volume the product of its full
+    variables, a zFiltered quadratic the back transform of its filtered
+    spectrum, a profile the x-y mean, a z slice the level; the
+    Geometry records carry the run's sizes."""
+    from coral_amd import output as OUT
+    from conftest import seeded_state
+    r = product_recipe("coral.equations.rbc_ra1e5")
+    r.quad_vars[8].remove_z_integral = True      # tw, filtered
+    NX, NZ = 16, 17
+    k0, z0 = seeded_state(r, NX, NX, NZ, 4, 4, 7, amps=(("u", 1.0), ("v", 1.0), ("w", 1.0), ("t", 0.1)))
+    text = (">>Volume :: tFull ;; Period= 1 <<\n>>Volume :: uw ;; Period= 1 <<\n>>Volume :: tw ;; Period= 1 <<\n"
+            ">>Profile:: wFull ;; Period= 1 <<\n>>ZSlice :: tFull ;; Period= 1 ;; Position= 3 <<\n>>EOF<<\n")
+    with solver_mod.Solver(r, NX, NX, NZ, 2 * math.pi, 2 * math.pi) as g:
+        g.set_state(k0, z0)
+        objs = OUT.parse_usr_output(text, r, g.NXAA, g.NYAA, g.NZAA)
+        out = OUT.UserOutput(g, objs, str(tmp_path))
+        files = out.before_step()
+        OUT.write_geometry(g, str(tmp_path))
+        full = [g.physical(i) for i in range(4)]
+        tw_ref = g.bwd_transform(g.products()[8])
+    assert len(files) == 5
+    ora = O.OracleSolver(r, O.Geometry(NX, NX, NZ, 2 * math.pi, 2 * math.pi), "ARS_222")
+    ora.field, ora.zfield = [a.copy() for a in k0], [a.copy() for a in z0]
+    ref = ora.physical_full_variables()
+    NXAA, NZAA = 3 * NX // 2, 3 * NZ // 2
+
+    def vol(name):
+        return np.fromfile(os.path.join(tmp_path, "Volumes", name), "<f8").reshape(NXAA, NXAA, NZAA)   # [x][y][z]
+    assert rel(vol("linear_var04_time00000001_full.dat"), ref[3]) < 1e-12
+    assert np.array_equal(vol("quadra_var03_time00000001_full.dat"), full[0] * full[2])
+    tw = vol("quadra_var09_time00000001_full.dat")
+    assert np.array_equal(tw, tw_ref) and np.abs(tw).max() > 0
+    assert not np.allclose(tw, full[3] * full[2])        # the filtered product, not the raw one
+    prof = np.fromfile(os.path.join(tmp_path, "Profiles", "linear_var03_time00000001_full.dat"), "<f8")
+    assert rel(prof, full[2].mean(axis=(0, 1))) < 1e-14
+    zs = np.fromfile(os.path.join(tmp_path, "Slices", "linear_var04_z00003_time00000001_full.dat"), "<f8")
+    assert np.array_equal(zs, full[3][:, :, 2].ravel())
+    ints = np.fromfile(os.path.join(tmp_path, "Geometry", "domDecmp.core0000"), "<i4")
+    assert list(ints[:6]) == [NX, NX, NZ, NXAA, NXAA, NZAA]

@@ -294,7 +294,7 @@ def bench_mixed(args):
 CLOSURE_WORKLOADS = {
     # learned closure, evaluated at 128^2 x 65 (the coarse grid of the
     # closure's target configuration) against a 256^2 x 129 fine run
-    "closure128": dict(fine=(256, 129), coarse=(128, 65), spin=(128, 65, 12000, 3000), snapshots=64, every=20,
+    "closure128": dict(fine=(256, 129), coarse=(128, 65), spin=(128, 65, 12000, 3000), snapshots=42, every=25,
                        epochs=12, t_eval=0.02),
     # the same at half the size in every direction
     "closure64": dict(fine=(128, 65), coarse=(64, 33), spin=(128, 65, 12000, 0), snapshots=150, every=25,
@@ -386,15 +386,16 @@ def bench_closure(args, name):
             log(f"closure: {i + 1} snapshots, t = {tk.absolute_time:.4f} ({time.perf_counter() - t0:.0f} s)")
     N2 = (3 * NXc // 2) ** 2
     nval = max(1, len(X) // 6)
-    Xtr, Ttr = np.concatenate(X[:-nval]), np.concatenate(T[:-nval])
-    Xva, Tva = np.concatenate(X[-nval:]), np.concatenate(T[-nval:])
-    del X, T
+    # per-snapshot arrays, never concatenated (a 128^2 x 65 snapshot is 1.6 GB of rows)
+    Xtr, Ttr, Xva = X[:-nval], T[:-nval], X[-nval:]
+    Tva = np.concatenate(T[-nval:])
     basis = CL.fourier_basis(3 * NXc // 2, 3 * NXc // 2, 4)
     t0 = time.perf_counter()
     model, hist = CL.train(Xtr, Ttr, N2, basis, 4, width=32, hidden=128, epochs=wl["epochs"], planes_per_batch=16,
                            lr=2e-3, log=log)
     t_train = time.perf_counter() - t0
     pred = CL.predict(model, Xva, N2, basis)
+    del X, T, Xtr, Ttr, Xva
     err = np.linalg.norm(pred - Tva, axis=0) / np.linalg.norm(Tva, axis=0)
     err_all = float(np.linalg.norm(pred - Tva) / np.linalg.norm(Tva))
     log(f"closure: trained in {t_train:.1f} s; held-out relative L2 error {err_all:.3f} per variable {np.round(err, 3)}")

@@ -79,6 +79,7 @@ EXPORTS = (
     "coral_amd_nn_colsum_work", "coral_amd_nn_colsum", "coral_amd_nn_mode_scale", "coral_amd_nn_mode_scale_grad",
     "coral_amd_nn_axpy", "coral_amd_nn_mse_partials", "coral_amd_nn_mse", "coral_amd_nn_adam",
     "coral_amd_nn_gemm_work", "coral_amd_nn_transpose_bf16", "coral_amd_nn_mlp",
+    "coral_amd_set_full_variable_filter",
 )
 
 
@@ -118,6 +119,7 @@ def _load():
     lib.coral_amd_banded_solve_noalloc.argtypes = [c_int32, c_int32, c_int32, c_int32, c_int32, P, P, P, P]
     lib.coral_amd_banded_solve_noalloc_bytes.argtypes = [c_int32, c_int32, c_int32, c_int32]
     lib.coral_amd_set_launch_shape.argtypes = [P, c_int32, c_int32]
+    lib.coral_amd_set_full_variable_filter.argtypes = [P, c_int32, c_int32, c_int32]
     lib.coral_amd_set_closure.argtypes = [P, ctypes.POINTER(ClosureDesc)]
     lib.coral_amd_get_launch_shape.argtypes = [P, P, P]
     lib.coral_amd_ensemble_create.argtypes = [P, c_int32, PP]

@@ -545,42 +545,68 @@ def subgrid_sample(fine_solver, coarse_solver, box: BoxFilter, zgrid: np.ndarray
     return rows_in.astype(np.float32), rows_out.astype(np.float32)
 
 
-def train(X: np.ndarray, T: np.ndarray, N2: int, basis: np.ndarray, modes: int, width: int = 64, hidden: int = 256,
+def _as_list(a):
+    return list(a) if isinstance(a, (list, tuple)) else [a]
+
+
+def train(X, T, N2: int, basis: np.ndarray, modes: int, width: int = 64, hidden: int = 256,
           layers: int = 4, epochs: int = 8, planes_per_batch: int = 16, lr: float = 1e-3, seed: int = 0,
           log=None) -> Tuple[Model, List[float]]:
-    """Train a closure on rows X (nplanes N2, cin) -> T (nplanes N2, cout), whole
-    planes in random order, normalised per channel; Adam with a cosine
-    learning-rate decay.  Returns the model and the mean loss of each epoch
-    (in normalised units)."""
-    nplanes = X.shape[0] // N2
-    cin, cout = X.shape[1], T.shape[1]
-    in_mean, in_std = X.mean(axis=0), X.std(axis=0) + 1e-30
-    out_mean, out_std = T.mean(axis=0), T.std(axis=0) + 1e-30
+    """Train a closure on rows X (nplanes N2, cin) -> T (nplanes N2, cout), or on
+    lists of such arrays (one per snapshot, never concatenated on the host):
+    whole planes in random order, normalised per channel with the statistics
+    of all rows (accumulated in float64); Adam with a cosine learning-rate
+    decay.  The normalised rows live on the device (BF16 inputs, FP32
+    targets); each batch gathers its planes there.  Returns the model and the
+    mean loss of each epoch (in normalised units)."""
+    Xs, Ts = _as_list(X), _as_list(T)
+    cin, cout = Xs[0].shape[1], Ts[0].shape[1]
+    nrow = sum(x.shape[0] for x in Xs)
+    if nrow % N2 or any(x.shape[0] != t.shape[0] for x, t in zip(Xs, Ts)):
+        raise ValueError("inputs and targets must hold whole planes, row for row")
+    nplanes = nrow // N2
+
+    def stats(arrs):
+        s1 = sum(a.sum(axis=0, dtype=np.float64) for a in arrs)
+        s2 = sum(np.einsum("ij,ij->j", a, a, dtype=np.float64) for a in arrs)
+        m = s1 / nrow
+        return m, np.sqrt(np.maximum(s2 / nrow - m * m, 0.0))
+    in_mean, in_std = stats(Xs)
+    out_mean, out_std = stats(Ts)
+    in_std, out_std = in_std + 1e-30, out_std + 1e-30
     in_std[in_std < 1e-12 * max(1.0, float(np.abs(in_mean).max()))] = 1.0
-    rng = np.random.default_rng(seed)
-    order = rng.permutation(nplanes)
-    Xn = normalise(X.reshape(nplanes, N2, cin)[order].reshape(-1, cin), in_mean, in_std)
-    Tn = normalise(T.reshape(nplanes, N2, cout)[order].reshape(-1, cout), out_mean, out_std)
     lay = Layout(cin=cin, cout=cout, width=width, hidden=hidden, layers=layers, nb=basis.shape[1])
     net = ClosureNet(lay, basis, seed=seed)
-    X32 = DevArray.from_numpy(Xn)
-    X16 = DevArray(X32.n, "bf16")
-    check(lib.coral_amd_nn_cast_bf16(X32.ptr, X16.ptr, X32.n, None), "nn cast")
-    del X32
-    Td = DevArray.from_numpy(Tn)
+    X16 = DevArray(nrow * cin, "bf16")
+    Td = DevArray(nrow * cout)
+    stage = DevArray(max(x.shape[0] for x in Xs) * cin)
+    off = 0
+    for x, t in zip(Xs, Ts):
+        n = x.shape[0]
+        stage.upload(normalise(x, in_mean, in_std))
+        check(lib.coral_amd_nn_cast_bf16(stage.ptr, X16.at(off * cin), n * cin, None), "nn cast")
+        Td.upload(normalise(t, out_mean, out_std), off * cout)
+        off += n
+    del stage
     P = planes_per_batch
     nb_ = nplanes // P
     if nb_ < 1:
         raise ValueError("fewer planes than one batch")
+    xb = DevArray(P * N2 * cin, "bf16")
+    tb = DevArray(P * N2 * cout)
+    rng = np.random.default_rng(seed)
     total = epochs * nb_
     hist = []
     step = 0
     for ep in range(epochs):
+        order = rng.permutation(nplanes)
         acc = 0.0
         for b in range(nb_):
+            for i, pl in enumerate(order[b * P:(b + 1) * P]):
+                xb.copy_from(X16.at(int(pl) * N2 * cin), N2 * cin, i * N2 * cin)
+                tb.copy_from(Td.at(int(pl) * N2 * cout), N2 * cout, i * N2 * cout)
             a = 0.5 * lr * (1.0 + math.cos(math.pi * step / total))
-            acc += net.train_step(DevView(X16, b * P * N2 * cin, P * N2 * cin), DevView(Td, b * P * N2 * cout,
-                                                                                      P * N2 * cout), P, a)
+            acc += net.train_step(xb, tb, P, a)
             step += 1
         hist.append(acc / nb_)
         if log:
@@ -590,18 +616,24 @@ def train(X: np.ndarray, T: np.ndarray, N2: int, basis: np.ndarray, modes: int, 
     return model, hist
 
 
-def predict(model: Model, X: np.ndarray, N2: int, basis: np.ndarray, planes_per_batch: int = 16) -> np.ndarray:
-    """The model's outputs (physical units) for rows X (nplanes N2, cin)."""
-    nplanes = X.shape[0] // N2
+def predict(model: Model, X, N2: int, basis: np.ndarray, planes_per_batch: int = 16) -> np.ndarray:
+    """The model's outputs (physical units) for rows X (nplanes N2, cin), or a
+    list of such arrays (one output array for their rows in order)."""
+    Xs = _as_list(X)
     net = ClosureNet(model.layout, basis, model.params)
-    out = np.empty((X.shape[0], model.layout.cout), np.float64)
+    cout = model.layout.cout
+    out = np.empty((sum(x.shape[0] for x in Xs), cout), np.float64)
     P = planes_per_batch
-    for p0 in range(0, nplanes, P):
-        np_ = min(P, nplanes - p0)
-        rows = X[p0 * N2:(p0 + np_) * N2]
-        x32 = DevArray.from_numpy(normalise(rows, model.in_mean, model.in_std))
-        x16 = DevArray(x32.n, "bf16")
-        check(lib.coral_amd_nn_cast_bf16(x32.ptr, x16.ptr, x32.n, None), "nn cast")
-        y = net.forward(x16, np_).numpy()[: np_ * N2 * model.layout.cout].reshape(-1, model.layout.cout)
-        out[p0 * N2:(p0 + np_) * N2] = y * model.out_std.astype(np.float64) + model.out_mean.astype(np.float64)
+    r0 = 0
+    for x in Xs:
+        nplanes = x.shape[0] // N2
+        for p0 in range(0, nplanes, P):
+            np_ = min(P, nplanes - p0)
+            rows = x[p0 * N2:(p0 + np_) * N2]
+            x32 = DevArray.from_numpy(normalise(rows, model.in_mean, model.in_std))
+            x16 = DevArray(x32.n, "bf16")
+            check(lib.coral_amd_nn_cast_bf16(x32.ptr, x16.ptr, x32.n, None), "nn cast")
+            y = net.forward(x16, np_).numpy()[: np_ * N2 * cout].reshape(-1, cout)
+            out[r0:r0 + np_ * N2] = y * model.out_std.astype(np.float64) + model.out_mean.astype(np.float64)
+            r0 += np_ * N2
     return out

@@ -155,6 +155,9 @@ struct coral_amd_ctx {
     // the z transforms as vector GEMMs instead of MFMA products
     int xpass_wpb = 0;
     bool z_plain = false;
+    // per full variable: (boundary-value extraction, vertical decomposition),
+    // coral_amd_set_full_variable_filter
+    std::vector<std::pair<int, int>> full_filter;
     // learned subgrid closure (coral_amd_set_closure): the network, its
     // normalisation, inference workspace and the correction's buffers
     struct Closure {
@@ -688,6 +691,16 @@ void backward_to_G(coral_amd_ctx &c, const std::vector<const double2 *> &kst, co
             else
                 launch_extract_mean(c.stream, zst[e->system], P.NZ, P.nsys, e->rowptr.p, e->col.p, e->val.p,
                                     c.spec.p + e->full * fsz);
+        }
+        // boundary values and vertical decompositions of the full variables
+        // (marching.f90:544-577), on their complete coefficients
+        for (int f = 0; f < P.nfull && f < (int)c.full_filter.size(); ++f) {
+            const int ex = c.full_filter[f].first, zd = c.full_filter[f].second;
+            if (!ex && !zd) continue;
+            if (fp32)
+                launch_full_zfilter32(c.stream, c.spec32.p + f * fsz, P.NZ, P.nsys, ex, zd);
+            else
+                launch_full_zfilter(c.stream, c.spec.p + f * fsz, P.NZ, P.nsys, ex, zd);
         }
     });
     if (fp32) {
@@ -2218,6 +2231,25 @@ int coral_amd_set_closure(coral_amd_ctx *c, const coral_amd_closure *cl) {
         return 0;
     } catch (const std::exception &e) {
         if (c) c->closure.on = false;
+        return fail(e);
+    }
+}
+
+int coral_amd_set_full_variable_filter(coral_amd_ctx *c, int32_t ifull, int32_t extract, int32_t zdecomp) {
+    try {
+        if (!c) throw std::runtime_error("null context");
+        if (ifull < 0 || ifull >= c->P.nfull) throw std::runtime_error("full variable index out of range");
+        if (extract < 0 || extract > 2) throw std::runtime_error("extract must be 0 (none), 1 (bottom) or 2 (top)");
+        if (zdecomp < 0 || zdecomp > 2)
+            throw std::runtime_error("vertical decomposition must be 0 (none), 1 (barotropic) or 2 (baroclinic)");
+        HIP_OK(hipSetDevice(c->device));
+        HIP_OK(hipStreamSynchronize(c->stream));
+        c->drop_graphs();
+        ++c->gen;
+        if ((int)c->full_filter.size() < c->P.nfull) c->full_filter.resize(c->P.nfull, {0, 0});
+        c->full_filter[ifull] = {extract, zdecomp};
+        return 0;
+    } catch (const std::exception &e) {
         return fail(e);
     }
 }

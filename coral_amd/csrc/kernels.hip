@@ -710,6 +710,71 @@ void launch_remove_zmean32(hipStream_t st, float2 *Qhat, int NZ, int nsys) {
     CHECK_LAUNCH();
 }
 
+// Boundary-value extraction and vertical decomposition of one full variable's
+// Chebyshev coefficients spec[n][s] (n < NZ), after they are formed and
+// before the z synthesis (compute_full_variables_in_physical_space,
+// PL_IMEX_timestepping_marching.f90:544-577):
+//   extract 1 (BotSurf) / 2 (TopSurf): row 0 <- the series' value at the
+//     bottom / top wall (sum of (-1)^n c_n / of c_n, in the order of n), rows
+//     1 .. NZ-1 zeroed;
+//   zdecomp 1 (barotropic, PL_IMEX_timestepping.f90:252-274): row 0 <- row 0
+//     plus the even rows 2q times their integral weights, rows 2q-1 and 2q
+//     zeroed, q = 1 .. NZ/2-1;
+//   zdecomp 2 (baroclinic, :238-250): row 0 <- minus those weighted sums (the
+//     series' z-integral removed).
+// One thread per system.
+template <class C>
+__global__ void k_full_zfilter(C *spec, int NZ, int nsys, int extract, int zdecomp) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nsys) return;
+    using R = typename CxT<C>::R;
+    const C zero = mkc<C>(R(0), R(0));
+    if (extract) {
+        C acc = spec[s];
+        for (int n = 1; n < NZ; ++n) {
+            C &v = spec[(size_t)n * nsys + s];
+            if (extract == 2 || (n & 1) == 0) {
+                acc.x += v.x;
+                acc.y += v.y;
+            } else {
+                acc.x -= v.x;
+                acc.y -= v.y;
+            }
+            v = zero;
+        }
+        spec[s] = acc;
+    }
+    if (zdecomp) {
+        double2 acc = zdecomp == 1 ? make_double2(double(spec[s].x), double(spec[s].y)) : make_double2(0.0, 0.0);
+        for (int k = 1; k <= NZ / 2 - 1; ++k) {
+            const double w = 0.5 / (2.0 * k + 1.0) - 0.5 / (2.0 * k - 1.0);
+            C &e = spec[(size_t)(2 * k) * nsys + s];
+            if (zdecomp == 1) {
+                acc.x += double(e.x) * w;
+                acc.y += double(e.y) * w;
+                spec[(size_t)(2 * k - 1) * nsys + s] = zero;
+                e = zero;
+            } else {
+                acc.x -= double(e.x) * w;
+                acc.y -= double(e.y) * w;
+            }
+        }
+        spec[s] = mkc<C>(R(acc.x), R(acc.y));
+    }
+}
+
+void launch_full_zfilter(hipStream_t st, double2 *spec, int NZ, int nsys, int extract, int zdecomp) {
+    hipLaunchKernelGGL(k_full_zfilter<double2>, dim3(blocks_for(nsys, 256)), dim3(256), 0, st, spec, NZ, nsys, extract,
+                       zdecomp);
+    CHECK_LAUNCH();
+}
+
+void launch_full_zfilter32(hipStream_t st, float2 *spec, int NZ, int nsys, int extract, int zdecomp) {
+    hipLaunchKernelGGL(k_full_zfilter<float2>, dim3(blocks_for(nsys, 256)), dim3(256), 0, st, spec, NZ, nsys, extract,
+                       zdecomp);
+    CHECK_LAUNCH();
+}
+
 // Galerkin coefficients of one coupled variable from the Chebyshev
 // coefficients of its physical values (import_quicksave,
 // PL_IMEX_timestepping_output.f90:340-356): g[j] = sum_{n < ncol} Sinv[j][n] Q[n][s],

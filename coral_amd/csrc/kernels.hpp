@@ -157,6 +157,10 @@ void launch_insert_var(hipStream_t st, const double2 *Q, int nsys, int ncol, con
 // remove the z-integral of Qhat[s][:] (PL_IMEX_timestepping.f90:223-235)
 void launch_remove_zmean(hipStream_t st, double2 *Qhat, int NZ, int nsys);
 void launch_remove_zmean32(hipStream_t st, float2 *Qhat, int NZ, int nsys);
+// boundary-value extraction (1 bottom, 2 top) and vertical decomposition (1
+// barotropic, 2 baroclinic) of one full variable's Chebyshev coefficients [NZ][nsys]
+void launch_full_zfilter(hipStream_t st, double2 *spec, int NZ, int nsys, int extract, int zdecomp);
+void launch_full_zfilter32(hipStream_t st, float2 *spec, int NZ, int nsys, int extract, int zdecomp);
 
 // ---- NL assembly into K_hat ----
 struct NLArgs {

@@ -82,6 +82,13 @@ class DevArray:
             raise ValueError("upload beyond the buffer")
         _ok(hip().hipMemcpy(self.at(offset_elems), a.ctypes.data, a.nbytes, 1), "hipMemcpy H2D")
 
+    def copy_from(self, src_ptr: int, nelem: int, offset_elems: int = 0):
+        """Device-to-device copy of nelem elements from src_ptr to this
+        buffer at offset_elems (the null stream's order)."""
+        if offset_elems + nelem > self.n:
+            raise ValueError("copy beyond the buffer")
+        _ok(hip().hipMemcpy(self.at(offset_elems), src_ptr, nelem * self.itemsize, 3), "hipMemcpy D2D")
+
     def numpy(self) -> np.ndarray:
         out = np.empty(self.n, np.float32 if self.dtype == "f32" else np.uint16)
         _ok(hip().hipMemcpy(out.ctypes.data, self._p, self.nbytes, 2), "hipMemcpy D2H")

@@ -19,8 +19,8 @@ grammar, the meaning of each token and the bookkeeping follow Coral's parser:
   ``remove_z_integral`` (``:918-925``).
 
 Features of the grammar that the time-stepper does not implement (sources,
-penalisation, boundary-value extraction, barotropic/baroclinic filters) raise
-``NotImplementedError`` naming the keyword, rather than being silently skipped.
+penalisation) raise ``NotImplementedError`` naming the keyword, rather than
+being silently skipped.
 """
 from __future__ import annotations
 
@@ -93,8 +93,15 @@ class FullTerm:
 
 @dataclasses.dataclass
 class FullVariable:
+    """A full variable; extract: "nowhere", or "BotSurf" / "TopSurf" (the
+    z-constant field of its value at the bottom / top wall,
+    ``linear_variable_{bottom,top}_boundary_value``); zdecomp: "", or
+    "barotropic" (only its vertical mean kept) / "baroclinic" (its vertical
+    mean removed), ``vertical_decomposition`` (``PL_equations.f90:1052-1068``)."""
     name: str
     terms: List[FullTerm]
+    extract: str = "nowhere"
+    zdecomp: str = ""
 
 
 @dataclasses.dataclass
@@ -184,9 +191,7 @@ def _atom_tokens(atom: str, dz_token: str):
 
 
 _UNSUPPORTED = (
-    "sourceParam", "enable_penalisation", "penalisation_strength", "penalisation_width",
-    "linear_variable_bottom_boundary_value", "linear_variable_top_boundary_value",
-    "vertical_decomposition", "add_source",
+    "sourceParam", "enable_penalisation", "penalisation_strength", "penalisation_width", "add_source",
 )
 
 
@@ -233,6 +238,17 @@ def parse_equations(text: str) -> Recipe:
             zero_vars.append(payload)
         elif key == "linear_variable_full":
             full_vars.append(FullVariable(payload, []))
+        elif key in ("linear_variable_bottom_boundary_value", "linear_variable_top_boundary_value"):
+            # PL_equations.f90:496-500, 1850-1853
+            full_vars.append(FullVariable(payload, [], "BotSurf" if "bottom" in key else "TopSurf"))
+        elif key == "vertical_decomposition":
+            # PL_equations.f90:501-503 (get_linear_variable_zDecomposition,
+            # PL_string_to_data.f90:322-342: exactly 'baroclinic' or 'barotropic')
+            if not full_vars:
+                raise EquationsError("vertical_decomposition before linear_variable_full")
+            if payload not in ("baroclinic", "barotropic"):
+                raise EquationsError(f"vertical_decomposition must be baroclinic or barotropic, got {payload!r}")
+            full_vars[-1].zdecomp = payload
         elif key == "linear_variable_full_build":
             if not full_vars:
                 raise EquationsError("linear_variable_full_build before linear_variable_full")

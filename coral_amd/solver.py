@@ -98,6 +98,13 @@ class Solver:
         ctx = ctypes.c_void_p()
         check(lib.coral_amd_create(ctypes.byref(prob), device, ctypes.byref(ctx)), "coral_amd_create")
         self._ctx = ctx
+        # boundary values and vertical decompositions of the full variables
+        for i, fv in enumerate(recipe.full_vars):
+            ex = {"nowhere": 0, "BotSurf": 1, "TopSurf": 2}[fv.extract]
+            zd = {"": 0, "barotropic": 1, "baroclinic": 2}[fv.zdecomp]
+            if ex or zd:
+                check(lib.coral_amd_set_full_variable_filter(self._ctx, i, ex, zd),
+                      "coral_amd_set_full_variable_filter")
         self.dt: Optional[float] = None
         self.totals = [self.system_info(i)[0] for i in range(len(recipe.kxky_systems))]
         self.zero_totals = [sum(NZ - bc // 10 for bc in s.bc_codes) for s in recipe.zero_systems]

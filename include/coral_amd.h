@@ -315,6 +315,16 @@ typedef struct {
 } coral_amd_closure;
 int coral_amd_set_closure(coral_amd_ctx *ctx, const coral_amd_closure *closure);
 
+/* Boundary-value extraction and vertical decomposition of full variable
+ * ifull (the recipe's extract_value_at and baroclinic_or_barotropic,
+ * fullVars_recipe_T, PL_equations.f90:114-118; applied where
+ * compute_full_variables_in_physical_space applies them,
+ * PL_IMEX_timestepping_marching.f90:544-577): extract 0 none, 1 BotSurf (the
+ * z-constant field of the variable's value at the bottom wall), 2 TopSurf;
+ * zdecomp 0 none, 1 barotropic (only its vertical mean kept), 2 baroclinic
+ * (its vertical mean removed).  Set after coral_amd_create. */
+int coral_amd_set_full_variable_filter(coral_amd_ctx *ctx, int32_t ifull, int32_t extract, int32_t zdecomp);
+
 /* Launch shapes of a run (no counterpart in the reference): xpass_wpb is the
  * largest number of wavefronts per workgroup of the x pass (1..4; 0 = the most
  * that fit its LDS), z_plain = 1 runs the z transforms as vector GEMMs

@@ -96,6 +96,26 @@ module coral_amd_binding
        type(c_ptr), value :: ctx
        integer(c_int32_t), value :: enable
      end function
+     ! boundary-value extraction (1 BotSurf, 2 TopSurf) and vertical
+     ! decomposition (1 barotropic, 2 baroclinic) of full variable ifull (0-based)
+     integer(c_int) function coral_amd_set_full_variable_filter(ctx, ifull, extract, zdecomp) bind(C)
+       import
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: ifull, extract, zdecomp
+     end function
+     ! precision 0: FP64; 1: FP32 transforms and products, FP64 solves
+     ! (refine = 1: one FP64 residual-refinement sweep per stage)
+     integer(c_int) function coral_amd_set_precision(ctx, precision, refine) bind(C)
+       import
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: precision, refine
+     end function
+     ! Hermitian projection of the kx = 0 plane after every step
+     integer(c_int) function coral_amd_set_kx0_projection(ctx, enable) bind(C)
+       import
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: enable
+     end function
      integer(c_int) function coral_amd_get_cfl_dt(ctx, dt) bind(C)
        import
        type(c_ptr), value :: ctx

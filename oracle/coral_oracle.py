@@ -482,6 +482,33 @@ def remove_z_integral(s, NZ):
     return s
 
 
+def keep_z_integral_only(s, NZ):
+    """In place: only the z-integral of the series kept, on row 0: row 0 plus
+    the even rows 2q times their integral weights, rows 2q - 1 and 2q zeroed,
+    q = 1 .. NZ/2 - 1 (remove_vertical_fluctuations_of_linear_var,
+    PL_IMEX_timestepping.f90:252-274; rows past 2 (NZ/2 - 1) are left as they
+    are, as there)."""
+    for k in range(1, NZ // 2):
+        s[..., 0] += s[..., 2 * k] * (0.5 / (2.0 * k + 1.0) - 0.5 / (2.0 * k - 1.0))
+        s[..., 2 * k - 1] = 0.0
+        s[..., 2 * k] = 0.0
+    return s
+
+
+def boundary_value(s, NZ, where):
+    """In place: the series replaced by the constant of its value at the
+    bottom (T_n(-1) = (-1)^n) or top (T_n(1) = 1) wall, on row 0, rows
+    1 .. NZ - 1 zeroed, summed in the order of n
+    (compute_full_variables_in_physical_space, PL_IMEX_timestepping_marching.f90:544-562)."""
+    for n in range(1, NZ):
+        if where == "TopSurf" or n % 2 == 0:
+            s[..., 0] += s[..., n]
+        else:
+            s[..., 0] -= s[..., n]
+        s[..., n] = 0.0
+    return s
+
+
 def project_kx0_plane(f, geo):
     """In place: the kx = 0 plane of a kxky state (nkx, NYAA, total) replaced by
     its Hermitian part, c(0, ky) = (c(0, ky) + conj c(0, -ky)) / 2 over the
@@ -634,6 +661,16 @@ class OracleSolver:
                     for _ in range(t.dz):
                         buf = self.D @ buf
                     spec[0, 0, :] += buf * t.dsca
+            # boundary-value extraction, then the vertical decomposition
+            # (PL_IMEX_timestepping_marching.f90:544-577)
+            where = getattr(fv, "extract", "nowhere")
+            if where in ("BotSurf", "TopSurf"):
+                boundary_value(spec, g.NZ, where)
+            zd = getattr(fv, "zdecomp", "")
+            if zd == "barotropic":
+                keep_z_integral_only(spec, g.NZ)
+            elif zd == "baroclinic":
+                remove_z_integral(spec, g.NZ)
             out.append(spec)
         return out
 

@@ -73,3 +73,35 @@ def seeded_state(r, NX, NY, NZ, kmax, nmodes, seed, amps=(("u", 1.0), ("v", 1.0)
         k0[0] = k0[0] + ic.seeded_perturbation(NX, NY, NZ, s.bc_codes, s.var_names.index(name), amp,
                                                kmax, nmodes, seed + 17 * i)
     return k0, z0
+
+
+def full_filter_recipe(extract="TopSurf", zdecomp="baroclinic"):
+    """The RBC recipe (coral_amd/inputs/coral.equations.rbc_ra1e5) with two
+    more full variables: tX, the boundary value (extract: TopSurf or BotSurf)
+    of tMean + t, and uZ, uMean + u under the given vertical decomposition
+    (barotropic or baroclinic); two quadratic variables built on them
+    (wFull.tX, uZ.wFull) enter the temperature equation with a factor 0.1."""
+    from coral_amd.equations import parse_equations
+    text = open(os.path.join(ROOT, "coral_amd", "inputs", "coral.equations.rbc_ra1e5")).read()
+    key = "top" if extract == "TopSurf" else "bottom"
+    text = text.replace(">>Param :: RayleighPr := 1.d5 <<",
+                        ">>Param :: RayleighPr := 1.d5 <<\n>>Param :: small := 1.d-1 <<")
+    text = text.replace(">>quadratic_variable :: uu := uFull.uFull <<",
+                        f">>linear_variable_{key}_boundary_value :: tX <<\n"
+                        ">>linear_variable_full_build :: + tMean <<\n>>linear_variable_full_build :: + t <<\n"
+                        ">>====================================<<\n"
+                        ">>linear_variable_full       :: uZ <<\n"
+                        f">>vertical_decomposition :: {zdecomp} <<\n"
+                        ">>linear_variable_full_build :: + uMean <<\n>>linear_variable_full_build :: + u <<\n"
+                        ">>====================================<<\n"
+                        ">>quadratic_variable :: uu := uFull.uFull <<")
+    text = text.replace(">>quadratic_variable :: tw := tFull.wFull <<",
+                        ">>quadratic_variable :: tw := tFull.wFull <<\n>>quadratic_variable :: wtX := wFull.tX <<\n"
+                        ">>quadratic_variable :: uZw := uZ.wFull <<")
+    text = text.replace(">>add_rhs_NL     :: - Iz.Iz.dx.tu - Iz.Iz.dy.tv - Iz.tw <<",
+                        ">>add_rhs_NL     :: - Iz.Iz.dx.tu - Iz.Iz.dy.tv - Iz.tw <<\n"
+                        ">>add_rhs_NL     :: - small.Iz.wtX - small.Iz.Iz.dx.uZw <<")
+    r = parse_equations(text)
+    assert [f.name for f in r.full_vars][-2:] == ["tX", "uZ"]
+    assert r.full_vars[-2].extract == extract and r.full_vars[-1].zdecomp == zdecomp
+    return r

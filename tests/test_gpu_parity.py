@@ -692,3 +692,47 @@ def test_kx0_projection_matches_oracle(solver_mod):
                         assert np.array_equal(k[0][0, jy], np.conj(k[0][0, iy]))
     for a, b in zip(phys[0], phys[1]):
         assert rel(a, b) <= 1e-12
+
+
+@pytest.mark.parametrize("extract,zdecomp,NZ", [("TopSurf", "baroclinic", 17), ("BotSurf", "barotropic", 24)])
+def test_full_variable_filters_match_oracle(solver_mod, extract, zdecomp, NZ):
+    """Boundary-value extraction (linear_variable_{top,bottom}_boundary_value)
+    and vertical decompositions (vertical_decomposition :: baroclinic /
+    barotropic) of full variables (k_full_zfilter, coral_amd_set_full_variable_filter;
+    PL_IMEX_timestepping_marching.f90:544-577), in a recipe whose temperature
+    equation takes products of the filtered variables: the physical full
+    variables, the quadratic spectra and three steps against the oracle; the
+    extracted variable is constant in z and the decomposed one has the
+    z-structure the filter leaves (zero vertical mean, or none but the mean)."""
+    from conftest import full_filter_recipe
+    r = full_filter_recipe(extract, zdecomp)
+    NX, NY = 16, 16
+    k0, z0 = seeded_state(r, NX, NY, NZ, 6, 6, 5, amps=(("u", 1e-2), ("v", 1e-2), ("w", 1e-2), ("t", 1e-2)))
+    rng = np.random.default_rng(NZ)
+    z0 = [a + 1e-2 * rng.normal(size=a.shape) for a in z0]
+    geo = O.Geometry(NX, NY, NZ, 2 * np.pi, 2 * np.pi)
+    ora = O.OracleSolver(r, geo, "ARS_222")
+    ora.field, ora.zfield = [a.copy() for a in k0], [a.copy() for a in z0]
+    phys_ref = ora.physical_full_variables()
+    qref = ora.quad_spectra([a.copy() for a in k0], [a.copy() for a in z0])
+    with solver_mod.Solver(r, NX, NY, NZ, 2 * np.pi, 2 * np.pi) as g:
+        g.set_state(k0, z0)
+        phys = [g.physical(f) for f in range(len(r.full_vars))]
+        q = g.products()
+        for it in range(3):
+            ora.step(1e-4)
+            g.march_forward(1e-4)
+        k, z = g.get_state()
+    for f in range(len(phys)):
+        assert rel(phys[f], phys_ref[f]) <= TOL, r.full_vars[f].name
+    tx, uz = phys[-2], phys[-1]
+    assert np.abs(tx - tx[:, :, :1]).max() <= 1e-12 * np.abs(tx).max()     # constant in z
+    assert np.abs(tx).max() > 0 and np.abs(uz).max() > 0
+    wn = np.zeros(q.shape[2], bool)
+    wn[: NY // 2] = True
+    wn[q.shape[2] - (NY // 2 - 1):] = True
+    for i in range(len(r.quad_vars)):
+        assert rel(q[i][:, wn], qref[i][:, wn]) <= TOL, r.quad_vars[i].name
+    assert rel(k[0], ora.field[0]) <= TOL
+    for a, b in zip(z, ora.zfield):
+        assert rel(a, b) <= TOL

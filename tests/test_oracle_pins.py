@@ -490,3 +490,47 @@ def test_kx0_projection_keeps_physical_fields():
             jy = int(np.nonzero(ky == -ky[iy])[0][0])
             assert np.array_equal(f[0, jy], np.conj(f[0, iy]))
     assert not np.allclose(f[0], k0[0][0])
+
+
+# --- boundary values and vertical decompositions of full variables -----------
+@pytest.mark.parametrize("NZ", [17, 16])
+def test_full_variable_boundary_values_and_decompositions(NZ):
+    """The oracle's boundary_value / keep_z_integral_only / remove_z_integral
+    against the mathematics of a Chebyshev series: the extracted row 0 is the
+    series' value at z = -1 (BotSurf) or z = +1 (TopSurf) (numpy chebval);
+    keeping only the z-integral leaves the series' mean over [-1, 1] (numpy
+    chebint), removing it leaves a series of mean zero, for series whose
+    degree stays below 2 (NZ/2 - 1), where the reference's loops end."""
+    from numpy.polynomial import chebyshev as C
+    rng = np.random.default_rng(NZ)
+    top = 2 * (NZ // 2 - 1)
+    c = np.zeros((3, NZ), complex)
+    c[:, :top + 1] = rng.normal(size=(3, top + 1)) + 1j * rng.normal(size=(3, top + 1))
+
+    def mean(cf):
+        i = C.chebint(cf)
+        return (C.chebval(1.0, i) - C.chebval(-1.0, i)) / 2.0
+    for where, z in (("BotSurf", -1.0), ("TopSurf", 1.0)):
+        full = rng.normal(size=(3, NZ)) + 1j * rng.normal(size=(3, NZ))   # every row, the loops run to NZ - 1
+        s = O.boundary_value(full.copy(), NZ, where)
+        for k in range(3):
+            assert s[k, 0] == pytest.approx(C.chebval(z, full[k]), rel=1e-13)
+        assert not s[:, 1:].any()
+    s = O.keep_z_integral_only(c.copy(), NZ)
+    for k in range(3):
+        assert s[k, 0] == pytest.approx(mean(c[k]), rel=1e-13)
+    assert not s[:, 1:top + 1].any()
+    s = O.remove_z_integral(c.copy(), NZ)
+    for k in range(3):
+        assert abs(mean(s[k])) < 1e-13 * np.abs(c[k]).max()
+        np.testing.assert_array_equal(s[k, 1:], c[k, 1:])
+
+
+def test_parser_reads_boundary_values_and_decompositions():
+    from conftest import full_filter_recipe
+    from coral_amd.equations import EquationsError, parse_equations
+    r = full_filter_recipe("BotSurf", "barotropic")
+    assert r.full_vars[-2].extract == "BotSurf" and r.full_vars[-1].zdecomp == "barotropic"
+    assert [len(f.terms) for f in r.full_vars[-2:]] == [2, 2]
+    with pytest.raises(EquationsError):
+        parse_equations(">>linear_variable_full :: a <<\n>>vertical_decomposition :: baroclinicX <<\n")

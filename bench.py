@@ -310,7 +310,7 @@ def bench_closure(args, name):
     box-filtered (top-hat over one coarse cell) onto the coarse grid and give
     the training pairs (the filtered full variables on each point's 3 x 3 x 3
     neighbourhood -> the subgrid part of every quadratic variable); the
-    network (width 32, hidden 128, 4 layers, modes |k| <= 4) is trained on the
+    network (--closure-width, --closure-hidden; 4 layers, modes |k| <= 4) is trained on the
     first 5/6 of the snapshots and scored on the rest (relative L2 error; 1 =
     no closure).  Then, from the filtered final fine state, the fine run, the
     plain coarse run and the coarse run with the closure each advance t_eval
@@ -391,8 +391,8 @@ def bench_closure(args, name):
     Tva = np.concatenate(T[-nval:])
     basis = CL.fourier_basis(3 * NXc // 2, 3 * NXc // 2, 4)
     t0 = time.perf_counter()
-    model, hist = CL.train(Xtr, Ttr, N2, basis, 4, width=32, hidden=128, epochs=wl["epochs"], planes_per_batch=16,
-                           lr=2e-3, log=log)
+    model, hist = CL.train(Xtr, Ttr, N2, basis, 4, width=args.closure_width, hidden=args.closure_hidden,
+                           epochs=wl["epochs"], planes_per_batch=16, lr=2e-3, log=log)
     t_train = time.perf_counter() - t0
     pred = CL.predict(model, Xva, N2, basis)
     del X, T, Xtr, Ttr, Xva
@@ -434,12 +434,64 @@ def bench_closure(args, name):
         "config": {"workload": f"RBC Ra=1e6 Pr=1, fine {NXf}^2x{NZf} -> coarse {NXc}^2x{NZc}, top-hat box filter, "
                                f"{wl['snapshots']} snapshots every {wl['every']} steps, {wl['epochs']} epochs, "
                                f"evaluation over {wl['t_eval']} time units",
-                   "network": {"width": 32, "hidden": 128, "layers": 4, "modes": 4, "params": int(model.layout.size)}},
+                   "network": {"width": args.closure_width, "hidden": args.closure_hidden, "layers": 4, "modes": 4,
+                               "params": int(model.layout.size)}},
         "vs_fine": round(c["sim_per_s"] / res["fine"]["sim_per_s"], 4),
         "nu_rel_error": {"coarse_closure": abs(c["nu"] / nu_f - 1), "coarse": abs(res["coarse"]["nu"] / nu_f - 1)},
         "a_priori_rel_l2": {"overall": err_all, "per_quadratic_variable": [round(float(e), 4) for e in err]},
         "runs": res, "training_s": round(t_train, 1), "wall_s": round(time.perf_counter() - t_all, 1),
     }
+    print(json.dumps(out), flush=True)
+
+
+def bench_closure_step(args):
+    """The closure's inference cost inside the coarse step at 128^2 x 65 (the
+    coarse grid of the closure configuration): RBC Ra = 1e6, the network with
+    random weights (its cost does not depend on them), width and hidden from
+    --closure-width / --closure-hidden (4 layers, modes |k| <= 4), 20 eager
+    steps with per-kernel events after 3 warmup steps, against the same
+    steps without the closure.  Reports ms per step with and without it and
+    the closure's share by kernel group."""
+    import numpy as np
+    from coral_amd import closure as CL
+    from coral_amd import ic
+    from coral_amd.ensemble import rbc_recipe
+    from coral_amd.solver import Solver
+    NX, NZ, M = 128, 65, 4
+    W, Hd = args.closure_width, args.closure_hidden
+    steps = args.steps if args.steps is not None else 20
+    r = rbc_recipe(1e6, 1.0)
+    L = 2 * math.pi
+    NXAA = 3 * NX // 2
+    basis = CL.fourier_basis(NXAA, NXAA, M)
+    cin = CL.input_width(len(r.full_vars))
+    lay = CL.Layout(cin=cin, cout=len(r.quad_vars), width=W, hidden=Hd, layers=4, nb=basis.shape[1])
+    model = CL.Model(lay, M, CL.init_params(lay, 1), np.zeros(cin), np.ones(cin), np.zeros(lay.cout),
+                     np.full(lay.cout, 1e-3))
+    res = {}
+    with Solver(r, NX, NX, NZ, L, L) as g:
+        k0, z0 = ic.noisy_initial_state(r, NX, NX, NZ, L, L, 1e-3, 1e-3, 5)
+        g.factorize_operators(1e-5)
+        for name, use in (("plain", False), ("closure", True)):
+            g.set_state(k0, z0)
+            g.set_closure(model if use else None)
+            g.march_forward(1e-5, 3)
+            g.synchronize()
+            g.timers_enable(True)
+            t0 = time.perf_counter()
+            g.march_forward(1e-5, steps)
+            g.synchronize()
+            el = time.perf_counter() - t0
+            res[name] = {"ms_per_step": round(1e3 * el / steps, 4),
+                         "kernels_ms_per_step": {k: round(v[0] / steps, 4) for k, v in g.timers().items()}}
+            g.timers_enable(False)
+            log(f"{name}: {res[name]['ms_per_step']:.3f} ms/step, {res[name]['kernels_ms_per_step']}")
+    out = {"metric": "coarse-step time with the learned closure, RBC 128^2x65", "value": res["closure"]["ms_per_step"],
+           "unit": "ms/step", "n_gpus": 1, "steps": steps, "higher_is_better": False,
+           "dtype": "f64 solver, bf16 network (f32 accumulate)", "data": "synthetic (random weights)",
+           "config": {"workload": "RBC Ra=1e6 Pr=1, 128x128x65 (192x192x97 padded), ARS_222, eager with events",
+                      "network": {"width": W, "hidden": Hd, "layers": 4, "modes": M, "params": int(lay.size)}},
+           "plain_ms_per_step": res["plain"]["ms_per_step"], "runs": res}
     print(json.dumps(out), flush=True)
 
 
@@ -449,11 +501,13 @@ def main():
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="rrbc256",
-                    choices=sorted(WORKLOADS) + ["ensemble48", "mixed512"] + sorted(CLOSURE_WORKLOADS),
+                    choices=sorted(WORKLOADS) + ["ensemble48", "mixed512", "closure_step"] + sorted(CLOSURE_WORKLOADS),
                     help="ensemble48: 48 uneven runs as one ensemble against solo runs; mixed512: mixed precision "
                          "against FP64 at 512^2 x 257 (sim-time/s and KE, Nu, spectrum errors); closure128 / "
                          "closure64: a learned closure trained on a filtered fine run, coarse sim-time/s and Nu "
                          "error against the fine and the plain coarse runs")
+    ap.add_argument("--closure-width", type=int, default=32, help="closure_step / closure*: network width")
+    ap.add_argument("--closure-hidden", type=int, default=128, help="closure_step / closure*: MLP hidden width")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--eager-only", action="store_true",
                     help="skip the second timed pass of the same steps replayed as HIP graphs")
@@ -487,6 +541,8 @@ def main():
         return bench_mixed(args)
     if args.workload in CLOSURE_WORKLOADS:
         return bench_closure(args, args.workload)
+    if args.workload == "closure_step":
+        return bench_closure_step(args)
     wl = WORKLOADS[args.workload]
     steps = args.steps if args.steps is not None else wl["steps"]
 

@@ -79,7 +79,9 @@ EXPORTS = (
     "coral_amd_nn_colsum_work", "coral_amd_nn_colsum", "coral_amd_nn_mode_scale", "coral_amd_nn_mode_scale_grad",
     "coral_amd_nn_axpy", "coral_amd_nn_mse_partials", "coral_amd_nn_mse", "coral_amd_nn_adam",
     "coral_amd_nn_gemm_work", "coral_amd_nn_transpose_bf16", "coral_amd_nn_mlp",
-    "coral_amd_set_full_variable_filter",
+    "coral_amd_set_full_variable_filter", "coral_amd_nn_mode_scale_t", "coral_amd_nn_modes_supported",
+    "coral_amd_nn_modes_work", "coral_amd_nn_modes", "coral_amd_nn_tile_transpose",
+    "coral_amd_nn_gemm_bias",
 )
 
 
@@ -134,6 +136,8 @@ def _load():
     lib.coral_amd_nn_colsum_work.restype = ctypes.c_int64
     lib.coral_amd_nn_mse_partials.restype = ctypes.c_int64
     lib.coral_amd_nn_gemm_work.restype = ctypes.c_int64
+    lib.coral_amd_nn_modes_work.restype = ctypes.c_int64
+    lib.coral_amd_nn_modes_supported.restype = ctypes.c_int32
     return lib
 
 

@@ -62,7 +62,13 @@ def _bind():
                               c_int64, c_float, P, c_int32, c_int64, c_int32, P, c_int64, P],
         "coral_amd_nn_gemm_work": [c_int32, c_int32, c_int32, c_int32],
         "coral_amd_nn_transpose_bf16": [P, c_int32, c_int32, P, P],
-        "coral_amd_nn_mlp": [c_int64, c_int32, c_int32, P, P, P, P, P, P, P, P, P, P],
+        "coral_amd_nn_mlp": [c_int64, c_int32, c_int32, P, P, P, P, P, P, P, P, P, P, P, c_int32, c_int32, P, P],
+        "coral_amd_nn_modes_supported": [c_int32, c_int32, c_int32],
+        "coral_amd_nn_tile_transpose": [P, c_int64, c_int32, P, P],
+        "coral_amd_nn_gemm_bias": [c_int32, c_int32, c_int32, P, c_int32, P, c_int32, P, P, P, P, P],
+        "coral_amd_nn_modes_work": [c_int32, c_int32, c_int32, c_int32],
+        "coral_amd_nn_modes": [P, c_int32, c_int32, c_int32, P, c_int32, c_int32, P, P, c_int64, P],
+        "coral_amd_nn_mode_scale_t": [P, P, P, c_int32, c_int32, c_int32, c_int32, P],
         "coral_amd_nn_cast_bf16": [P, P, c_int64, P],
         "coral_amd_nn_bias_act": [P, P, c_int32, c_int64, c_int32, P, P, P],
         "coral_amd_nn_gelu_bwd": [P, P, P, P, c_int64, P],
@@ -78,7 +84,8 @@ def _bind():
     for name, args in sig.items():
         f = getattr(lib, name)
         f.argtypes = args
-        f.restype = c_int64 if name.endswith(("_work", "_partials")) else ctypes.c_int
+        f.restype = c_int64 if name.endswith(("_work", "_partials")) else (
+            c_int32 if name.endswith("_supported") else ctypes.c_int)
 
 
 _bind()
@@ -224,17 +231,18 @@ def input_width(nfull: int) -> int:
 
 
 def normalise(rows: np.ndarray, mean: np.ndarray, std: np.ndarray) -> np.ndarray:
-    """(rows - mean) / std in float64, rounded to float32 (the values the
-    device rounds to BF16)."""
-    return ((rows - np.asarray(mean, np.float32).astype(np.float64)) /
-            np.asarray(std, np.float32).astype(np.float64)).astype(np.float32)
+    """(rows - mean) * (1 / std) in float64 (the reciprocal rounded once per
+    channel, as the step's input packing forms it), rounded to float32 (the
+    values the device rounds to BF16)."""
+    rinv = 1.0 / np.asarray(std, np.float32).astype(np.float64)
+    return ((rows - np.asarray(mean, np.float32).astype(np.float64)) * rinv).astype(np.float32)
 
 
 class _Work:
     """Device buffers of one batch size P (activations kept for the backward
     pass and its scratch), allocated once and reused."""
 
-    def __init__(self, L: Layout, N2: int, P: int):
+    def __init__(self, L: Layout, N2: int, P: int, nbp_modes: int = 0):
         W, Hd, nb, R = L.width, L.hidden, L.nb, P * N2
         self.P, self.R = P, R
         self.H = DevArray(R * W)
@@ -242,7 +250,7 @@ class _Work:
         self.Z = [DevArray(R * Hd) for _ in range(L.layers)]
         self.A16 = [DevArray(R * Hd, "bf16") for _ in range(L.layers)]
         self.Cm = [DevArray(P * nb * W) for _ in range(L.layers)]
-        self.Cs16 = DevArray(P * nb * W, "bf16")
+        self.Cs16 = DevArray(P * ((nb + 31) // 32 * 32) * W, "bf16")
         self.Y = DevArray(R * L.cout)
         self.dY = DevArray(R * L.cout)
         self.dY16 = DevArray(R * L.cout, "bf16")
@@ -261,6 +269,10 @@ class _Work:
         shapes = [(nb, W, N2, P), (W, L.cout, N2, P), (Hd, W, N2, P), (W, Hd, N2, P), (L.cin, W, N2, P)]
         self.ngwork = max(int(lib.coral_amd_nn_gemm_work(*sh)) for sh in shapes)
         self.gwork = DevArray(max(self.ngwork, 1))
+        # the mode kernel's operands: H16 transposed (W x R) and its partials
+        self.H16T = DevArray(R * W, "bf16") if nbp_modes else None
+        self.nmwork = int(lib.coral_amd_nn_modes_work(nbp_modes, N2, W, P)) if nbp_modes else 0
+        self.mwork = DevArray(max(self.nmwork, 1)) if nbp_modes else None
 
 
 class ClosureNet:
@@ -273,10 +285,25 @@ class ClosureNet:
         assert nb == layout.nb
         if layout.width not in (32, 64) or layout.hidden not in (128, 256):
             raise ValueError("width must be 32 or 64 and hidden 128 or 256")
-        basisT32 = DevArray.from_numpy(np.ascontiguousarray(basis.T))
-        self.basisT16 = DevArray(basis.size, "bf16")      # nb x N2
+        # the basis transposed, nbp x N2 (zero rows past nb), and N2 x nbp (modes
+        # contiguous, zero-padded to a multiple of 32): the operands of the
+        # mode projections and of the fused MLP kernel's spectral synthesis,
+        # used when the planes split into its 32-row tiles
+        self.nbp = (nb + 31) // 32 * 32
+        bt = np.zeros((self.nbp, self.N2))
+        bt[:nb] = basis.T
+        basisT32 = DevArray.from_numpy(bt)
+        self.basisT16 = DevArray(bt.size, "bf16")
         self._cast(basisT32, self.basisT16)
         del basisT32
+        self.fused_spectral = self.N2 % 32 == 0
+        self.tmodes = self.fused_spectral and bool(lib.coral_amd_nn_modes_supported(self.nbp, self.N2, layout.width))
+        bp = np.zeros((self.N2, self.nbp))
+        bp[:, :nb] = basis
+        bp32 = DevArray.from_numpy(bp)
+        self.basis16p = DevArray(bp.size, "bf16")
+        self._cast(bp32, self.basis16p)
+        del bp32
         p = init_params(layout, seed) if params is None else np.asarray(params, np.float32)
         self.w = DevArray.from_numpy(p)
         self.w16 = DevArray(layout.size, "bf16")
@@ -320,7 +347,7 @@ class ClosureNet:
 
     def _ws(self, P) -> _Work:
         if P not in self.work:
-            self.work = {P: _Work(self.L, self.N2, P)}   # one batch size at a time
+            self.work = {P: _Work(self.L, self.N2, P, self.nbp if self.tmodes else 0)}   # one batch size at a time
         return self.work[P]
 
     def _wgrad(self, A16, lda, B16, ldb, M, N, ws, out_ptr):
@@ -346,24 +373,36 @@ class ClosureNet:
         L, N2, W, Hd, nb = self.L, self.N2, self.L.width, self.L.hidden, self.L.nb
         ws = self._ws(P)
         R = ws.R
-        self._gemm(0, 0, R, W, L.cin, X16.ptr, L.cin, 0, self._p("W0", self.w16), W, 0, ws.H.ptr, W, 0)
-        check(lib.coral_amd_nn_bias_act(ws.H.ptr, self._p("b0", self.w), W, R * W, 0, None, ws.H16[0].ptr, None),
-              "nn bias")
+        # lift with its bias, BF16 copy and (for the mode kernel) tiled transposed copy
+        check(lib.coral_amd_nn_gemm_bias(R, W, L.cin, X16.ptr, L.cin, self._p("W0", self.w16), W, self._p("b0", self.w),
+                                         ws.H.ptr, ws.H16[0].ptr, ws.H16T.ptr if self.tmodes else None, None),
+              "nn gemm bias")
         for l in range(L.layers):
             Hin16, Hout16 = ws.H16[l], ws.H16[l + 1]
             Cm = ws.Cm[l]
             # spectral branch: Cm = B^T H16 per plane, Cs = Cm D, H <- H + B Cs
-            self._gemm(0, 0, nb, W, N2, self.basisT16.ptr, N2, 0, Hin16.ptr, W, N2 * W, Cm.ptr, W, nb * W, batch=P,
-                       ws=ws)
-            check(lib.coral_amd_nn_mode_scale(Cm.ptr, self._p(f"D.{l}", self.w), None, ws.Cs16.ptr, P, nb * W, None),
-                  "nn mode scale")
-            self._gemm(1, 0, N2, W, nb, self.basisT16.ptr, N2, 0, ws.Cs16.ptr, W, nb * W, ws.H.ptr, W, N2 * W,
-                       batch=P, beta=1.0)
-            # MLP branch, fused: H <- H + gelu(H16 W1 + b1) W2 + b2, keeping Z and A for the backward pass
+            if self.tmodes:
+                check(lib.coral_amd_nn_modes(self.basisT16.ptr, self.nbp, nb, N2, ws.H16T.ptr, W, P, Cm.ptr,
+                                             ws.mwork.ptr, ws.nmwork, None), "nn modes")
+            else:
+                self._gemm(0, 0, nb, W, N2, self.basisT16.ptr, N2, 0, Hin16.ptr, W, N2 * W, Cm.ptr, W, nb * W,
+                           batch=P, ws=ws)
             t1, t2 = self._wt(l)
+            if self.fused_spectral:
+                # B Cs inside the MLP kernel, from Cs transposed and padded
+                check(lib.coral_amd_nn_mode_scale_t(Cm.ptr, self._p(f"D.{l}", self.w), ws.Cs16.ptr, P, nb, W,
+                                                    self.nbp, None), "nn mode scale")
+                bs, cs = self.basis16p.ptr, ws.Cs16.ptr
+            else:
+                check(lib.coral_amd_nn_mode_scale(Cm.ptr, self._p(f"D.{l}", self.w), None, ws.Cs16.ptr, P, nb * W,
+                                                  None), "nn mode scale")
+                self._gemm(1, 0, N2, W, nb, self.basisT16.ptr, N2, 0, ws.Cs16.ptr, W, nb * W, ws.H.ptr, W, N2 * W,
+                           batch=P, beta=1.0)
+                bs, cs = None, None
+            # MLP branch, fused: H <- H + gelu(H16 W1 + b1) W2 + b2, keeping Z and A for the backward pass
             check(lib.coral_amd_nn_mlp(R, W, Hd, Hin16.ptr, t1, self._p(f"b1.{l}", self.w), t2,
                                        self._p(f"b2.{l}", self.w), ws.H.ptr, Hout16.ptr, ws.Z[l].ptr, ws.A16[l].ptr,
-                                       None), "nn mlp")
+                                       bs, cs, N2, self.nbp, ws.H16T.ptr if self.tmodes else None, None), "nn mlp")
         self._gemm(0, 0, R, L.cout, W, ws.H16[L.layers].ptr, W, 0, self._p("Wp", self.w16), L.cout, 0, ws.Y.ptr,
                    L.cout, 0)
         check(lib.coral_amd_nn_bias_act(ws.Y.ptr, self._p("bp", self.w), L.cout, R * L.cout, 0, None, None, None),

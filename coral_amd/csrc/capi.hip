@@ -165,8 +165,8 @@ struct coral_amd_ctx {
         nn::NetParams net;
         nn::NetWork ws;
         int chunk = 0;   // planes per network launch
-        DevBuf<float> w, mu_in, sd_in, mu_out, sd_out, Y, H, Cm, gwork;
-        DevBuf<unsigned short> w16, wt16, basisT16, X16, H16, Cs16;
+        DevBuf<float> w, mu_in, sd_in, mu_out, sd_out, Y, H, Cm, gwork, mwork;
+        DevBuf<unsigned short> w16, wt16, basisT16, basis16p, X16, H16, Cs16, H16T;
         DevBuf<double> phys_full, phys_tau, zgrid;
         DevBuf<double2> Qtau;
     } closure;
@@ -767,8 +767,8 @@ __device__ inline unsigned short bf16_of(float f) {
 // in z; offsets dz, dy, dx = -1, 0, 1 in that nesting, the variables
 // innermost), the point's own values as they are and every neighbour as its
 // difference from them (formed in FP64, so that the BF16 inputs keep the
-// local differences' digits), normalised ((v - mean) / std in FP64, rounded
-// to FP32 then BF16), the height of the plane, normalised likewise, and zero
+// local differences' digits), normalised ((v - mean) * (1 / std) in FP64,
+// rounded to FP32 then BF16), the height of the plane, normalised likewise, and zero
 // columns up to cin (a multiple of 8, for 16-byte rows).  A workgroup fills
 // PACK_ROWS consecutive rows (each neighbouring line read once into the
 // caches): the rows' positions are decoded once into LDS, and each thread
@@ -806,7 +806,7 @@ __global__ __launch_bounds__(256) void k_closure_pack(const double *__restrict__
         }
         const int o = c < ncore ? c / nfull : 0, f = c < ncore ? c - o * nfull : 0;
         const int dz = o / 9 - 1, dy = (o / 3) % 3 - 1, dx = o % 3 - 1;
-        const double m = double(mu[c]), s = double(sd[c]);
+        const double m = double(mu[c]), rinv = 1.0 / double(sd[c]);
         for (int rl = ro; rl < nr; rl += rs) {
             const int z = rz[rl], y = ry[rl], x = rx[rl];
             double d = 0.0;
@@ -823,7 +823,67 @@ __global__ __launch_bounds__(256) void k_closure_pack(const double *__restrict__
             } else if (c == ncore) {
                 d = zgrid[z];
             }
-            out[(size_t)rl * cin + c] = bf16_of(float((d - m) / s));
+            out[(size_t)rl * cin + c] = bf16_of(float((d - m) * rinv));
+        }
+        if (cin <= nt) break;
+    }
+}
+
+// The same rows, one workgroup per x-line (z, y) of NXAA points: the 3 x 3
+// neighbouring lines (dz, dy = -1, 0, 1, periodic in y, clamped in z) of
+// every full variable staged in LDS with coalesced loads, then the line's
+// NXAA rows written whole (each thread keeps one column over rows rs apart).
+// Values and rounding are those of k_closure_pack.  LDS: 9 nfull NXAA doubles.
+__global__ __launch_bounds__(256) void k_closure_pack_lines(const double *__restrict__ phys,
+                                                            const double *__restrict__ zgrid, int z0, int NXAA,
+                                                            int NYAA, int NZAA, int nfull, int cin,
+                                                            const float *__restrict__ mu,
+                                                            const float *__restrict__ sd,
+                                                            unsigned short *__restrict__ X16) {
+    extern __shared__ double tile[];   // [dz][dy][f][x]
+    const int N2 = NXAA * NYAA, ncore = 27 * nfull;
+    const int line = blockIdx.x, zl = line / NYAA, y = line - zl * NYAA, z = z0 + zl;
+    for (int e = threadIdx.x; e < 9 * nfull * NXAA; e += blockDim.x) {
+        const int x = e % NXAA, t = e / NXAA, f = t % nfull, o = t / nfull;   // o = 3 (dz+1) + (dy+1)
+        const int dz = o / 3 - 1, dy = o % 3 - 1;
+        const int zz = min(max(z + dz, 0), NZAA - 1);
+        const int yy = y + dy < 0 ? NYAA - 1 : (y + dy == NYAA ? 0 : y + dy);
+        tile[e] = phys[((size_t)f * NZAA + zz) * N2 + (size_t)yy * NXAA + x];
+    }
+    __syncthreads();
+    unsigned short *out = X16 + (size_t)line * NXAA * cin;
+    const int nt = blockDim.x, rs = cin <= nt ? nt / cin : 1;
+    for (int cbase = 0; cbase < cin; cbase += nt) {
+        int c, ro;
+        if (cin <= nt) {
+            c = threadIdx.x % cin;
+            ro = threadIdx.x / cin;
+            if (ro >= rs) break;
+        } else {
+            c = cbase + threadIdx.x;
+            ro = 0;
+            if (c >= cin) break;
+        }
+        const int o = c < ncore ? c / nfull : 0, f = c < ncore ? c - o * nfull : 0;
+        const int dz = o / 9 - 1, dy = (o / 3) % 3 - 1, dx = o % 3 - 1;
+        const double m = double(mu[c]), rinv = 1.0 / double(sd[c]);
+        const double *lc = tile + (size_t)(4 * nfull + f) * NXAA;                       // dz = dy = 0
+        const double *ln = tile + (size_t)((3 * (dz + 1) + (dy + 1)) * nfull + f) * NXAA;
+        const double hz = zgrid[z];
+        for (int x = ro; x < NXAA; x += rs) {
+            double d = 0.0;
+            if (c < ncore) {
+                const double v0 = lc[x];
+                if (o == 13) {
+                    d = v0;
+                } else {
+                    const int xx = x + dx < 0 ? NXAA - 1 : (x + dx == NXAA ? 0 : x + dx);
+                    d = ln[xx] - v0;
+                }
+            } else if (c == ncore) {
+                d = hz;
+            }
+            out[(size_t)x * cin + c] = bf16_of(float((d - m) * rinv));
         }
         if (cin <= nt) break;
     }
@@ -860,9 +920,16 @@ void closure_correction(coral_amd_ctx &c) {
         for (int z0 = 0; z0 < P.NZAA; z0 += cl.chunk) {
             const int np = std::min(cl.chunk, P.NZAA - z0);
             const unsigned g = unsigned(((long long)np * N2 + 255) / 256);
-            const unsigned gp = unsigned(((long long)np * N2 + PACK_ROWS - 1) / PACK_ROWS);
-            hipLaunchKernelGGL(k_closure_pack, dim3(gp), dim3(256), 0, c.stream, cl.phys_full.p, cl.zgrid.p, z0, np,
-                               P.NXAA, P.NYAA, P.NZAA, P.nfull, cl.net.cin, cl.mu_in.p, cl.sd_in.p, cl.X16.p);
+            const size_t tile = size_t(9) * P.nfull * P.NXAA * sizeof(double);
+            if (tile <= 65536) {
+                hipLaunchKernelGGL(k_closure_pack_lines, dim3(unsigned(np * P.NYAA)), dim3(256), tile, c.stream,
+                                   cl.phys_full.p, cl.zgrid.p, z0, P.NXAA, P.NYAA, P.NZAA, P.nfull, cl.net.cin,
+                                   cl.mu_in.p, cl.sd_in.p, cl.X16.p);
+            } else {
+                const unsigned gp = unsigned(((long long)np * N2 + PACK_ROWS - 1) / PACK_ROWS);
+                hipLaunchKernelGGL(k_closure_pack, dim3(gp), dim3(256), 0, c.stream, cl.phys_full.p, cl.zgrid.p, z0,
+                                   np, P.NXAA, P.NYAA, P.NZAA, P.nfull, cl.net.cin, cl.mu_in.p, cl.sd_in.p, cl.X16.p);
+            }
             check_launch("closure pack");
             nn::net_forward(c.stream, cl.net, cl.X16.p, np, cl.Y.p, cl.ws);
             hipLaunchKernelGGL(k_closure_unpack, dim3(g), dim3(256), 0, c.stream, cl.Y.p, z0, np, N2, P.NZAA, P.nquad,
@@ -2157,10 +2224,13 @@ int coral_amd_set_closure(coral_amd_ctx *c, const coral_amd_closure *cl) {
                     cols.push_back(std::move(v));
                 }
             }
-        const int nb = int(cols.size());
-        std::vector<unsigned short> b16(size_t(N2) * nb);   // transposed: nb x N2
+        const int nb = int(cols.size()), nbp = (nb + 31) / 32 * 32;
+        std::vector<unsigned short> b16(size_t(N2) * nbp, 0);   // transposed: nbp x N2, zero rows past nb
         for (int k = 0; k < nb; ++k)
             for (int i = 0; i < N2; ++i) b16[size_t(k) * N2 + i] = nn::host_bf16(float(cols[k][i]));
+        std::vector<unsigned short> b16p(size_t(N2) * nbp, 0);   // N2 x nbp, the fused synthesis' operand
+        for (int i = 0; i < N2; ++i)
+            for (int k = 0; k < nb; ++k) b16p[size_t(i) * nbp + k] = b16[size_t(k) * N2 + i];
         nn::NetParams &n = C.net;
         n = nn::NetParams{};
         n.cin = (27 * P.nfull + 1 + 7) / 8 * 8;
@@ -2192,6 +2262,7 @@ int coral_amd_set_closure(coral_amd_ctx *c, const coral_amd_closure *cl) {
         C.w16.upload(w16);
         C.wt16.upload(wt);
         C.basisT16.upload(b16);
+        C.basis16p.upload(b16p);
         C.mu_in.upload(std::vector<float>(cl->in_mean, cl->in_mean + n.cin));
         C.sd_in.upload(std::vector<float>(cl->in_std, cl->in_std + n.cin));
         C.mu_out.upload(std::vector<float>(cl->out_mean, cl->out_mean + n.cout));
@@ -2200,6 +2271,8 @@ int coral_amd_set_closure(coral_amd_ctx *c, const coral_amd_closure *cl) {
         n.w16 = C.w16.p;
         n.wt16 = C.wt16.p;
         n.basisT16 = C.basisT16.p;
+        n.basis16p = C.basis16p.p;
+        n.nbp = nbp;
         std::vector<double> zg(P.NZAA);
         for (int k = 0; k < P.NZAA; ++k) zg[k] = P.center + 0.5 * P.gap * std::cos(M_PI * (2 * k + 1) / (2.0 * P.NZAA));
         C.zgrid.upload(zg);
@@ -2213,7 +2286,7 @@ int coral_amd_set_closure(coral_amd_ctx *c, const coral_amd_closure *cl) {
         C.H.alloc(R * n.width);
         C.H16.alloc(R * n.width);
         C.Cm.alloc(size_t(C.chunk) * nb * n.width);
-        C.Cs16.alloc(size_t(C.chunk) * nb * n.width);
+        C.Cs16.alloc(size_t(C.chunk) * nbp * n.width);
         long long ngw = 0;
         for (int p = 1; p <= C.chunk; ++p) ngw = std::max(ngw, nn::gemm_work(nb, n.width, N2, p));
         C.gwork.alloc(size_t(std::max(ngw, 1LL)));
@@ -2224,6 +2297,13 @@ int coral_amd_set_closure(coral_amd_ctx *c, const coral_amd_closure *cl) {
         C.ws.ngwork = ngw;
         C.ws.H16 = C.H16.p;
         C.ws.Cs16 = C.Cs16.p;
+        if (N2 % nn::MLP_ROWS == 0 && nn::modes_supported(nbp, N2, n.width)) {
+            C.H16T.alloc(R * n.width);
+            C.mwork.alloc(size_t(nn::modes_work(nbp, N2, n.width, C.chunk)));
+            C.ws.H16T = C.H16T.p;
+            C.ws.mwork = C.mwork.p;
+            C.ws.nmwork = nn::modes_work(nbp, N2, n.width, C.chunk);
+        }
         C.phys_full.alloc(size_t(P.nfull) * P.NZAA * N2);
         C.phys_tau.alloc(size_t(P.nquad) * P.NZAA * N2);
         C.Qtau.alloc(size_t(P.nquad) * P.nsys * P.NZ);

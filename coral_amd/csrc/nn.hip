@@ -37,11 +37,18 @@ __device__ inline unsigned short f2bf(float f) { return __builtin_bit_cast(unsig
 // ---------------------------------------------------------------------------
 constexpr int TM = 64, TN = 64, TK = 32, LDK = TK + 8;   // LDS row stride in bf16 (80 B)
 
+// Optional epilogue of an unsplit, unbatched product: C += bias (per
+// column), C16 = bf16(C) (same layout), C16T = bf16(C) in the tiled
+// transposed layout of tile_transpose (M % 32 == 0).
+struct GemmEpi {
+    const float *bias = nullptr;
+    unsigned short *C16 = nullptr, *C16T = nullptr;
+};
 __global__ __launch_bounds__(256) void k_gemm_bf16(int transA, int transB, int M, int N, int K, float alpha,
                                                    const unsigned short *__restrict__ A, int lda, long long sA,
                                                    const unsigned short *__restrict__ B, int ldb, long long sB,
                                                    float beta, float *__restrict__ C, int ldc, long long sC,
-                                                   int kc, int nsplit) {
+                                                   int kc, int nsplit, GemmEpi epi) {
     __shared__ __align__(16) unsigned short As[TM * LDK];
     __shared__ __align__(16) unsigned short Bs[TN * LDK];
     const int bz = blockIdx.z / nsplit, ks = blockIdx.z - bz * nsplit;
@@ -149,8 +156,15 @@ __global__ __launch_bounds__(256) void k_gemm_bf16(int transA, int transB, int M
                 const int gm = m0 + wm + 16 * i + rq + q, gn = n0 + wn + 16 * j + col;
                 if (gm < M && gn < N) {
                     float *p = C + (size_t)gm * ldc + gn;
-                    const float v = alpha * acc[i][j][q];
-                    *p = beta == 0.f ? v : v + beta * *p;
+                    const float v0 = alpha * acc[i][j][q];
+                    float v = beta == 0.f ? v0 : v0 + beta * *p;
+                    if (epi.bias) v = v + epi.bias[gn];
+                    *p = v;
+                    if (epi.C16 || epi.C16T) {
+                        const unsigned short h = f2bf(v);
+                        if (epi.C16) epi.C16[(size_t)gm * ldc + gn] = h;
+                        if (epi.C16T) epi.C16T[((size_t)(gm / 32) * N + gn) * 32 + (gm & 31)] = h;
+                    }
                 }
             }
 }
@@ -202,7 +216,7 @@ void gemm(hipStream_t st, int transA, int transB, int M, int N, int K, float alp
     if (grid.y > 65535u || (long long)batch * ns > 65535) throw std::runtime_error("gemm: grid too large (M or batch)");
     if (ns == 1) {
         hipLaunchKernelGGL(k_gemm_bf16, grid, dim3(256), 0, st, transA, transB, M, N, K, alpha, A, lda, sA, B, ldb,
-                           sB, beta, C, ldc, sC, K, 1);
+                           sB, beta, C, ldc, sC, K, 1, GemmEpi{});
         check_launch("bf16 gemm");
         return;
     }
@@ -210,26 +224,56 @@ void gemm(hipStream_t st, int transA, int transB, int M, int N, int K, float alp
     if (!work || nwork < need)
         throw std::runtime_error("gemm: split-K work space of " + std::to_string(need) + " floats required");
     hipLaunchKernelGGL(k_gemm_bf16, grid, dim3(256), 0, st, transA, transB, M, N, K, 1.f, A, lda, sA, B, ldb, sB, 0.f,
-                       work, N, (long long)M * N, kc, ns);
+                       work, N, (long long)M * N, kc, ns, GemmEpi{});
     const long long n = (long long)batch * M * N;
     hipLaunchKernelGGL(k_splitk_sum, dim3(unsigned((n + 255) / 256)), dim3(256), 0, st, work, ns, M, N, batch, alpha,
                        beta, C, ldc, sC);
     check_launch("bf16 gemm (split K)");
 }
 
-// dst (cols x rows) = src (rows x cols)^T, BF16, for the MLP kernel's weight
-// operands (a column of W1 or W2 contiguous along k)
-__global__ void k_transpose16(const unsigned short *__restrict__ src, int rows, int cols,
-                              unsigned short *__restrict__ dst) {
-    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= (long long)rows * cols) return;
-    const int c = int(i / rows), r = int(i - (long long)c * rows);
-    dst[i] = src[(long long)r * cols + c];
+// C (M x N, ldc = N) = A B + bias, C16 = bf16(C), C16T = bf16(C) tiled and
+// transposed (when given; M % 32 == 0): the sequence gemm, bias_act and
+// tile_transpose in one kernel, with their arithmetic
+void gemm_bias(hipStream_t st, int M, int N, int K, const unsigned short *A, int lda, const unsigned short *B, int ldb,
+               const float *bias, float *C, unsigned short *C16, unsigned short *C16T) {
+    if (M <= 0 || N <= 0) return;
+    if (K <= 0) throw std::runtime_error("gemm: K must be positive");
+    if (C16T && M % 32) throw std::runtime_error("gemm: the tiled copy needs M % 32 == 0");
+    const dim3 grid(unsigned((N + TN - 1) / TN), unsigned((M + TM - 1) / TM), 1);
+    if (grid.y > 65535u) throw std::runtime_error("gemm: grid too large (M)");
+    GemmEpi e;
+    e.bias = bias;
+    e.C16 = C16;
+    e.C16T = C16T;
+    hipLaunchKernelGGL(k_gemm_bf16, grid, dim3(256), 0, st, 0, 0, M, N, K, 1.f, A, lda, 0LL, B, ldb, 0LL, 0.f, C, N,
+                       0LL, K, 1, e);
+    check_launch("bf16 gemm + bias");
+}
+
+// dst (cols x rows) = src (rows x cols)^T, BF16 (the MLP kernel's weight
+// operands, a column of W1 or W2 contiguous along k; the layer input for the
+// mode projections): 64 x 64 tiles through LDS, both sides read and written
+// along their rows
+__global__ __launch_bounds__(256) void k_transpose16(const unsigned short *__restrict__ src, int rows, int cols,
+                                                     unsigned short *__restrict__ dst) {
+    __shared__ unsigned short t[64][65];
+    const long long r0 = (long long)blockIdx.x * 64;
+    const int c0 = blockIdx.y * 64;
+    for (int e = threadIdx.x; e < 64 * 64; e += 256) {
+        const int r = e >> 6, c = e & 63;
+        if (r0 + r < rows && c0 + c < cols) t[r][c] = src[(size_t)(r0 + r) * cols + c0 + c];
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < 64 * 64; e += 256) {
+        const int c = e >> 6, r = e & 63;
+        if (r0 + r < rows && c0 + c < cols) dst[(size_t)(c0 + c) * rows + r0 + r] = t[r][c];
+    }
 }
 void transpose16(hipStream_t st, const unsigned short *src, int rows, int cols, unsigned short *dst) {
     if (rows <= 0 || cols <= 0) return;
-    const long long n = (long long)rows * cols;
-    hipLaunchKernelGGL(k_transpose16, dim3(unsigned((n + 255) / 256)), dim3(256), 0, st, src, rows, cols, dst);
+    const long long gx = (rows + 63) / 64, gy = (cols + 63) / 64;
+    if (gx > 2147483647LL || gy > 65535) throw std::runtime_error("transpose: too large");
+    hipLaunchKernelGGL(k_transpose16, dim3(unsigned(gx), unsigned(gy)), dim3(256), 0, st, src, rows, cols, dst);
     check_launch("bf16 transpose");
 }
 
@@ -250,14 +294,15 @@ void cast_bf16(hipStream_t st, const float *x, unsigned short *y, long long n) {
 
 // tanh form of GELU, 0.5 z (1 + tanh u) with u = c (z + 0.044715 z^3),
 // c = sqrt(2/pi), evaluated through the identity 0.5 (1 + tanh u) =
-// 1 / (1 + exp(-2 u)): one exponential and one hardware reciprocal (1 ulp),
+// 1 / (1 + exp(-2 u)): one hardware exponential and one hardware reciprocal (1 ulp),
 // no branches (exp overflows to inf for very negative z, whose reciprocal 0
 // gives z s = -0, and underflows to 0 for large z, giving z).  Derivative:
 // s + 2 z s (1 - s) du/dz with s = 1 / (1 + exp(-2 u)).
 __device__ inline float gelu_sig(float z) {
-    const float c = 0.7978845608028654f;
-    const float u = c * (z + 0.044715f * z * z * z);
-    return __builtin_amdgcn_rcpf(1.f + __expf(-2.f * u));
+    // exp(-2 u) = 2^(z (a + b z^2)), a = -2 c log2(e), b = 0.044715 a: one
+    // multiply-add and one multiply feed the hardware exponential (v_exp_f32)
+    const float a = -2.f * 0.7978845608028654f * 1.4426950408889634f, b = 0.044715f * a;
+    return __builtin_amdgcn_rcpf(1.f + __builtin_amdgcn_exp2f(z * __builtin_fmaf(b, z * z, a)));
 }
 __device__ inline float gelu(float z) { return z * gelu_sig(z); }
 __device__ inline float gelu_grad(float z) {
@@ -302,13 +347,17 @@ void bias_act(hipStream_t st, float *z, const float *bias, int ncol, long long n
 // every read of a row's Hin16 precedes the barrier between the stages and
 // every write follows it.
 // ---------------------------------------------------------------------------
-constexpr int MLP_ROWS = 32;
-template <int W, int HD>
+// FAST: the inference form, with every tile whole (R % 32 == 0) and no
+// caches: no row checks and no cache stores in the instruction stream, and
+// addresses as 32-bit offsets from the workgroup's base pointers.
+template <int W, int HD, bool FAST>
 __global__ __launch_bounds__(256) void k_mlp(long long R, const unsigned short *Hin16,
                                              const unsigned short *__restrict__ W1t, const float *__restrict__ b1,
                                              const unsigned short *__restrict__ W2t, const float *__restrict__ b2,
                                              float *__restrict__ H, unsigned short *Hout16, float *__restrict__ Zc,
-                                             unsigned short *__restrict__ A16c) {
+                                             unsigned short *__restrict__ A16c, const unsigned short *__restrict__ B16,
+                                             const unsigned short *__restrict__ CsT16, int N2, int nbp,
+                                             unsigned short *__restrict__ H16T) {
     static_assert(W % 32 == 0 && HD % 64 == 0, "MLP shape");
     constexpr int RB = MLP_ROWS / 16;    // 16-row blocks
     constexpr int LDA = HD + 8;          // LDS row stride of the activation tile (bf16)
@@ -317,35 +366,85 @@ __global__ __launch_bounds__(256) void k_mlp(long long R, const unsigned short *
     const long long r0 = (long long)blockIdx.x * MLP_ROWS;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int fr = lane & 15, fk = 8 * (lane >> 4), rq = 4 * (lane >> 4);
+    const int sr = 16 * (wv >> 1), sc = (W / 2) * (wv & 1), c0 = wv * (HD / 4);
     const bf8 zero8 = {};
-    // stage 1
-    bf8 a[RB][KS1];
+    // Every global read is issued up front, so that a workgroup waits for
+    // memory once: the stage-1 fragments (Hin16 rows from HBM, W1t from L2),
+    // the old H values of the epilogue's rows, then the synthesis operands.
+    const unsigned short *Hin_b = Hin16 + r0 * W;
+    float *H_b = H + r0 * W;
+    unsigned short *Hout_b = Hout16 + r0 * W;
+    bf8 a[RB][KS1], w1[KS1][CB1];
 #pragma unroll
     for (int rb = 0; rb < RB; ++rb) {
-        const long long row = r0 + 16 * rb + fr;
+        const int rl = 16 * rb + fr;
 #pragma unroll
         for (int k = 0; k < KS1; ++k)
-            a[rb][k] = row < R ? *reinterpret_cast<const bf8 *>(Hin16 + row * W + 32 * k + fk) : zero8;
+            a[rb][k] = (FAST || r0 + rl < R) ? *reinterpret_cast<const bf8 *>(Hin_b + rl * W + 32 * k + fk) : zero8;
     }
+#pragma unroll
+    for (int k = 0; k < KS1; ++k)
+#pragma unroll
+        for (int cb = 0; cb < CB1; ++cb)
+            w1[k][cb] = *reinterpret_cast<const bf8 *>(W1t + (size_t)(c0 + 16 * cb + fr) * W + 32 * k + fk);
+    float hold[CB2][4], bias1[CB1], bias2[CB2];
+#pragma unroll
+    for (int cb = 0; cb < CB2; ++cb)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int rl = sr + rq + q;
+            hold[cb][q] = (FAST || r0 + rl < R) ? H_b[rl * W + sc + 16 * cb + fr] : 0.f;
+        }
+#pragma unroll
+    for (int cb = 0; cb < CB1; ++cb) bias1[cb] = b1[c0 + 16 * cb + fr];
+#pragma unroll
+    for (int cb = 0; cb < CB2; ++cb) bias2[cb] = b2[sc + 16 * cb + fr];
+    // the spectral branch's synthesis S = B Cs for the rows of stage 2 when
+    // B16 is given (all MLP_ROWS rows in one plane: N2 % MLP_ROWS == 0): A
+    // fragments from the plane's basis rows (N2 x nbp, modes contiguous, zero
+    // past nb), B fragments from Cs transposed (W x nbp per plane)
+    f4 sacc[CB2];
+#pragma unroll
+    for (int cb = 0; cb < CB2; ++cb) sacc[cb] = f4{0.f, 0.f, 0.f, 0.f};
+    if (B16) {
+        const long long p = r0 / N2;
+        const int i = int(r0 - p * N2) + sr + fr;
+        const unsigned short *brow = B16 + (size_t)i * nbp + fk;
+        const unsigned short *cst = CsT16 + (size_t)p * W * nbp + fk;
+        for (int k = 0; k < nbp; k += 32) {
+            const bf8 av = *reinterpret_cast<const bf8 *>(brow + k);
+#pragma unroll
+            for (int cb = 0; cb < CB2; ++cb) {
+                const bf8 bv = *reinterpret_cast<const bf8 *>(cst + (size_t)(sc + 16 * cb + fr) * nbp + k);
+                sacc[cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, sacc[cb], 0, 0, 0);
+            }
+        }
+    }
+    // stage 1
     f4 acc[RB][CB1];
 #pragma unroll
     for (int rb = 0; rb < RB; ++rb)
 #pragma unroll
         for (int cb = 0; cb < CB1; ++cb) acc[rb][cb] = f4{0.f, 0.f, 0.f, 0.f};
-    const int c0 = wv * (HD / 4);
 #pragma unroll
     for (int k = 0; k < KS1; ++k)
 #pragma unroll
-        for (int cb = 0; cb < CB1; ++cb) {
-            const bf8 b = *reinterpret_cast<const bf8 *>(W1t + (size_t)(c0 + 16 * cb + fr) * W + 32 * k + fk);
+        for (int cb = 0; cb < CB1; ++cb)
 #pragma unroll
             for (int rb = 0; rb < RB; ++rb)
-                acc[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[rb][k], b, acc[rb][cb], 0, 0, 0);
-        }
+                acc[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[rb][k], w1[k][cb], acc[rb][cb], 0, 0, 0);
+    // stage 2's weight fragments, loaded now: their L2 latency hides behind
+    // the GELU epilogue instead of stalling the stage-2 k loop
+    bf8 w2[KS2][CB2];
+#pragma unroll
+    for (int k = 0; k < KS2; ++k)
+#pragma unroll
+        for (int cb = 0; cb < CB2; ++cb)
+            w2[k][cb] = *reinterpret_cast<const bf8 *>(W2t + (size_t)(sc + 16 * cb + fr) * HD + 32 * k + fk);
 #pragma unroll
     for (int cb = 0; cb < CB1; ++cb) {
         const int col = c0 + 16 * cb + fr;
-        const float bias = b1[col];
+        const float bias = bias1[cb];
 #pragma unroll
         for (int rb = 0; rb < RB; ++rb)
 #pragma unroll
@@ -354,14 +453,15 @@ __global__ __launch_bounds__(256) void k_mlp(long long R, const unsigned short *
                 const float v = acc[rb][cb][q] + bias;
                 const unsigned short h = f2bf(gelu(v));
                 As[rl * LDA + col] = h;
-                const long long row = r0 + rl;
-                if (Zc && row < R) Zc[row * HD + col] = v;
-                if (A16c && row < R) A16c[row * HD + col] = h;
+                if constexpr (!FAST) {
+                    const long long row = r0 + rl;
+                    if (Zc && row < R) Zc[row * HD + col] = v;
+                    if (A16c && row < R) A16c[row * HD + col] = h;
+                }
             }
     }
     __syncthreads();
     // stage 2
-    const int sr = 16 * (wv >> 1), sc = (W / 2) * (wv & 1);
     f4 u[CB2];
 #pragma unroll
     for (int cb = 0; cb < CB2; ++cb) u[cb] = f4{0.f, 0.f, 0.f, 0.f};
@@ -369,40 +469,193 @@ __global__ __launch_bounds__(256) void k_mlp(long long R, const unsigned short *
     for (int k = 0; k < KS2; ++k) {
         const bf8 av = *reinterpret_cast<const bf8 *>(&As[(sr + fr) * LDA + 32 * k + fk]);
 #pragma unroll
-        for (int cb = 0; cb < CB2; ++cb) {
-            const bf8 b = *reinterpret_cast<const bf8 *>(W2t + (size_t)(sc + 16 * cb + fr) * HD + 32 * k + fk);
-            u[cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, b, u[cb], 0, 0, 0);
-        }
+        for (int cb = 0; cb < CB2; ++cb) u[cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, w2[k][cb], u[cb], 0, 0, 0);
     }
 #pragma unroll
     for (int cb = 0; cb < CB2; ++cb) {
         const int col = sc + 16 * cb + fr;
-        const float bias = b2[col];
+        const float bias = bias2[cb];
+        unsigned short ht[4];
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-            const long long row = r0 + sr + rq + q;
-            if (row < R) {
-                const float h = H[row * W + col] + (u[cb][q] + bias);
-                H[row * W + col] = h;
-                Hout16[row * W + col] = f2bf(h);
+            const int rl = sr + rq + q;
+            // (S + H) + (U + b2): the order of the unfused GEMM (beta = 1), bias and axpy
+            const float hs = B16 ? sacc[cb][q] + hold[cb][q] : hold[cb][q];
+            const float h = hs + (u[cb][q] + bias);
+            ht[q] = f2bf(h);
+            if (FAST || r0 + rl < R) {
+                H_b[rl * W + col] = h;
+                Hout_b[rl * W + col] = ht[q];
+            }
+        }
+        // the new BF16 values in the tiled transposed layout (tile_index: this
+        // workgroup's 32 rows are one W x 32 block), four consecutive rows in
+        // one 8-byte store, for the next layer's mode projection
+        if (H16T && (FAST || r0 + sr + rq + 3 < R)) {
+            uint2 v;
+            v.x = unsigned(ht[0]) | (unsigned(ht[1]) << 16);
+            v.y = unsigned(ht[2]) | (unsigned(ht[3]) << 16);
+            *reinterpret_cast<uint2 *>(H16T + ((size_t)(r0 / MLP_ROWS) * W + col) * MLP_ROWS + sr + rq) = v;
+        }
+    }
+}
+
+// Tiled transposed layout of an R x W BF16 array (R % 32 == 0): row r, column
+// c at ((r / 32) W + c) 32 + r % 32, i.e. each 32-row tile stored as its W
+// columns of 32 consecutive rows (2 KB at W = 32): contiguous along the rows,
+// as MFMA B fragments read them, and local to a tile (a layout with whole
+// columns of R rows would put every lane of a fragment on another page).
+__global__ void k_tile_transpose(const unsigned short *__restrict__ src, long long R, int W,
+                                 unsigned short *__restrict__ dst) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= R * W) return;
+    const long long t = i / ((long long)W * MLP_ROWS);
+    const int e = int(i - t * W * MLP_ROWS), c = e / MLP_ROWS, rr = e - c * MLP_ROWS;
+    dst[i] = src[(t * MLP_ROWS + rr) * W + c];
+}
+void tile_transpose(hipStream_t st, const unsigned short *src, long long R, int W, unsigned short *dst) {
+    if (R <= 0 || W <= 0) return;
+    if (R % MLP_ROWS) throw std::runtime_error("tile transpose: R % 32 != 0");
+    const long long n = R * W;
+    hipLaunchKernelGGL(k_tile_transpose, dim3(unsigned((n + 255) / 256)), dim3(256), 0, st, src, R, W, dst);
+    check_launch("tile transpose");
+}
+
+// Mode projections of a layer's input, per plane p: Cm[p] (nb x W) =
+// B^T H16[p], from the basis transposed (BT: nbp x N2, zero rows past nb) and
+// H16 in the tiled transposed layout (HT, R = P N2 rows), both contiguous
+// along the points.  A
+// workgroup of four wavefronts takes chunk c of a plane's points (MODE_CHUNK
+// of them) and all nbp x W outputs as 16 x 16 MFMA tiles, tile t on wavefront
+// t % 4, fragments straight from L2 (B) and HBM (H16); its partial sums go to
+// part[p][c]; k_modes_sum adds the chunks in order (deterministic).
+constexpr int MODE_CHUNK = 1024, MODE_TILES = 8;   // tiles per wavefront at most
+template <int TPW>
+__global__ __launch_bounds__(256) void k_modes(const unsigned short *__restrict__ BT, int nbp, int N2,
+                                               const unsigned short *__restrict__ HT, int W,
+                                               float *__restrict__ part) {
+    const int c = blockIdx.x, p = blockIdx.y, nchunk = gridDim.x;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int fr = lane & 15, fk = 8 * (lane >> 4), rq = 4 * (lane >> 4);
+    const int NB = W / 16, T = (nbp / 16) * NB;
+    const int i0 = c * MODE_CHUNK, i1 = min(N2, i0 + MODE_CHUNK);
+    // the wavefront's tiles: mode block and channel block of t = wv + 4 j
+    int aoff[TPW], boff[TPW];
+    bool on[TPW];
+#pragma unroll
+    for (int j = 0; j < TPW; ++j) {
+        const int t = wv + 4 * j;
+        on[j] = t < T;
+        const int mb = on[j] ? t / NB : 0, nb_ = on[j] ? t - mb * NB : 0;
+        aoff[j] = (16 * mb + fr) * N2 + fk;
+        boff[j] = (16 * nb_ + fr) * 32 + fk;
+    }
+    f4 acc[TPW];
+#pragma unroll
+    for (int j = 0; j < TPW; ++j) acc[j] = f4{0.f, 0.f, 0.f, 0.f};
+    // software pipeline: the next 32 points' fragments are loaded while the
+    // current ones feed the MFMAs (the points k .. k + 31 are one tile of HT)
+    bf8 av[TPW], bv[TPW];
+    auto load = [&](int k, bf8 (&a)[TPW], bf8 (&b)[TPW]) {
+        const unsigned short *hp = HT + ((size_t)p * N2 + k) * W;
+#pragma unroll
+        for (int j = 0; j < TPW; ++j) {
+            a[j] = *reinterpret_cast<const bf8 *>(BT + (size_t)aoff[j] + k);
+            b[j] = *reinterpret_cast<const bf8 *>(hp + boff[j]);
+        }
+    };
+    load(i0, av, bv);
+    for (int k = i0; k < i1; k += 32) {
+        bf8 an[TPW], bn[TPW];
+        if (k + 32 < i1) load(k + 32, an, bn);
+#pragma unroll
+        for (int j = 0; j < TPW; ++j)
+            if (on[j]) acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[j], bv[j], acc[j], 0, 0, 0);
+        if (k + 32 < i1) {
+#pragma unroll
+            for (int j = 0; j < TPW; ++j) {
+                av[j] = an[j];
+                bv[j] = bn[j];
             }
         }
     }
+    float *out = part + ((size_t)p * nchunk + c) * nbp * W;
+#pragma unroll
+    for (int j = 0; j < TPW; ++j) {
+        const int t = wv + 4 * j;
+        if (on[j]) {
+            const int mb = t / NB, nb_ = t - mb * NB;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) out[(size_t)(16 * mb + rq + q) * W + 16 * nb_ + fr] = acc[j][q];
+        }
+    }
+}
+__global__ void k_modes_sum(const float *__restrict__ part, int nchunk, int nbp, int nb, int W, int P,
+                            float *__restrict__ Cm) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long long)P * nb * W) return;
+    const long long p = i / ((long long)nb * W);
+    const int e = int(i - p * nb * W);
+    const float *src = part + (size_t)p * nchunk * nbp * W + e;
+    float s = 0.f;
+    for (int c = 0; c < nchunk; ++c) s += src[(size_t)c * nbp * W];
+    Cm[i] = s;
+}
+bool modes_supported(int nbp, int N2, int W) {
+    return nbp > 0 && W > 0 && nbp % 16 == 0 && W % 16 == 0 && N2 > 0 && N2 % 32 == 0 &&
+           (nbp / 16) * (W / 16) <= 4 * MODE_TILES;
+}
+long long modes_work(int nbp, int N2, int W, int P) {
+    return (long long)P * ((N2 + MODE_CHUNK - 1) / MODE_CHUNK) * nbp * W;
+}
+void modes(hipStream_t st, const unsigned short *BT, int nbp, int nb, int N2, const unsigned short *HT, int W, int P,
+           float *Cm, float *work, long long nwork) {
+    if (P <= 0) return;
+    if (!modes_supported(nbp, N2, W) || nb > nbp)
+        throw std::runtime_error("modes: shapes (nbp, W multiples of 16, N2 of 32, at most 32 tiles)");
+    const int nchunk = (N2 + MODE_CHUNK - 1) / MODE_CHUNK;
+    if (!work || nwork < modes_work(nbp, N2, W, P)) throw std::runtime_error("modes: work space too small");
+    const int tpw = ((nbp / 16) * (W / 16) + 3) / 4;
+    const dim3 g(nchunk, P);
+    switch (tpw) {
+    case 1: hipLaunchKernelGGL((k_modes<1>), g, dim3(256), 0, st, BT, nbp, N2, HT, W, work); break;
+    case 2: hipLaunchKernelGGL((k_modes<2>), g, dim3(256), 0, st, BT, nbp, N2, HT, W, work); break;
+    case 3: hipLaunchKernelGGL((k_modes<3>), g, dim3(256), 0, st, BT, nbp, N2, HT, W, work); break;
+    case 4: hipLaunchKernelGGL((k_modes<4>), g, dim3(256), 0, st, BT, nbp, N2, HT, W, work); break;
+    case 5: hipLaunchKernelGGL((k_modes<5>), g, dim3(256), 0, st, BT, nbp, N2, HT, W, work); break;
+    case 6: hipLaunchKernelGGL((k_modes<6>), g, dim3(256), 0, st, BT, nbp, N2, HT, W, work); break;
+    case 7: hipLaunchKernelGGL((k_modes<7>), g, dim3(256), 0, st, BT, nbp, N2, HT, W, work); break;
+    default: hipLaunchKernelGGL((k_modes<8>), g, dim3(256), 0, st, BT, nbp, N2, HT, W, work); break;
+    }
+    const long long n = (long long)P * nb * W;
+    hipLaunchKernelGGL(k_modes_sum, dim3(unsigned((n + 255) / 256)), dim3(256), 0, st, work, nchunk, nbp, nb, W, P,
+                       Cm);
+    check_launch("mode projection");
 }
 
 bool mlp_supported(int W, int HD) { return (W == 64 || W == 32) && (HD == 256 || HD == 128); }
 void mlp(hipStream_t st, long long R, int W, int HD, const unsigned short *Hin16, const unsigned short *W1t,
          const float *b1, const unsigned short *W2t, const float *b2, float *H, unsigned short *Hout16, float *Zc,
-         unsigned short *A16c) {
+         unsigned short *A16c, const unsigned short *B16, const unsigned short *CsT16, int N2, int nbp,
+         unsigned short *H16T) {
     if (R <= 0) return;
+    if (H16T && R % MLP_ROWS != 0) throw std::runtime_error("mlp: the transposed copy needs R % 32 == 0");
+    if (B16 && (!CsT16 || N2 <= 0 || N2 % MLP_ROWS != 0 || nbp <= 0 || nbp % 32 != 0))
+        throw std::runtime_error("mlp: the spectral term needs N2 % 32 == 0 and nbp a positive multiple of 32");
     const long long nblk = (R + MLP_ROWS - 1) / MLP_ROWS;
     if (nblk > 2147483647LL) throw std::runtime_error("mlp: too many rows");
     const dim3 g{unsigned(nblk)}, b{256};
-#define MLP_CASE(w_, hd_)                                                                                     \
-    if (W == w_ && HD == hd_) {                                                                               \
-        hipLaunchKernelGGL((k_mlp<w_, hd_>), g, b, 0, st, R, Hin16, W1t, b1, W2t, b2, H, Hout16, Zc, A16c); \
-        check_launch("fused mlp");                                                                            \
-        return;                                                                                               \
+    const bool fast = !Zc && !A16c && R % MLP_ROWS == 0;
+#define MLP_CASE(w_, hd_)                                                                                       \
+    if (W == w_ && HD == hd_) {                                                                                 \
+        if (fast)                                                                                               \
+            hipLaunchKernelGGL((k_mlp<w_, hd_, true>), g, b, 0, st, R, Hin16, W1t, b1, W2t, b2, H, Hout16, Zc,  \
+                               A16c, B16, CsT16, N2, nbp, H16T);                                                \
+        else                                                                                                    \
+            hipLaunchKernelGGL((k_mlp<w_, hd_, false>), g, b, 0, st, R, Hin16, W1t, b1, W2t, b2, H, Hout16, Zc, \
+                               A16c, B16, CsT16, N2, nbp, H16T);                                                \
+        check_launch("fused mlp");                                                                              \
+        return;                                                                                                 \
     }
     MLP_CASE(64, 256)
     MLP_CASE(64, 128)
@@ -470,6 +723,26 @@ void mode_scale(hipStream_t st, const float *c, const float *d, float *out, unsi
     if (P <= 0 || ne <= 0) return;
     hipLaunchKernelGGL(k_mode_scale, dim3(blocks((long long)P * ne)), dim3(256), 0, st, c, d, out, out16, P, ne);
     check_launch("mode scale");
+}
+// out16T[p][c][k] = bf16(c[p][k][c] d[k][c]) for k < nb, 0 for nb <= k < nbp
+// (the scaled modes transposed and padded, the fused MLP kernel's operand)
+__global__ void k_mode_scale_t(const float *__restrict__ c, const float *__restrict__ d,
+                               unsigned short *__restrict__ out16T, int P, int nb, int W, int nbp) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long long)P * W * nbp) return;
+    const int k = int(i % nbp);
+    const long long pc = i / nbp;
+    const int ch = int(pc % W);
+    const long long p = pc / W;
+    out16T[i] = k < nb ? f2bf(c[(p * nb + k) * W + ch] * d[(long long)k * W + ch]) : (unsigned short)0;
+}
+void mode_scale_t(hipStream_t st, const float *c, const float *d, unsigned short *out16T, int P, int nb, int W,
+                  int nbp) {
+    if (P <= 0 || nb <= 0 || W <= 0) return;
+    if (nbp < nb) throw std::runtime_error("mode_scale_t: nbp < nb");
+    const long long n = (long long)P * W * nbp;
+    hipLaunchKernelGGL(k_mode_scale_t, dim3(blocks(n)), dim3(256), 0, st, c, d, out16T, P, nb, W, nbp);
+    check_launch("mode scale (transposed)");
 }
 // dd[e] (+)= sum_p c[p][e] g[p][e], one thread per e, planes in order
 __global__ void k_mode_scale_grad(const float *__restrict__ c, const float *__restrict__ g, float *__restrict__ dd,
@@ -579,19 +852,32 @@ void net_forward(hipStream_t st, const NetParams &n, const unsigned short *X16, 
     const long long R = (long long)P * N2;
     if (R > 2147483647LL) throw std::runtime_error("closure network: too many rows");
     const int Ri = int(R);
-    gemm(st, 0, 0, Ri, W, n.cin, 1.f, X16, n.cin, 0, n.w16 + n.off_W0, W, 0, 0.f, w.H, W, 0, 1);
-    bias_act(st, w.H, n.w + n.off_b0, W, R * W, 0, nullptr, w.H16);
+    const bool fused = n.basis16p && N2 % MLP_ROWS == 0;
+    const bool tmodes = fused && w.H16T && modes_supported(n.nbp, N2, W);
+    // lift with its bias, BF16 copy and (for the mode kernel) tiled transposed copy
+    gemm_bias(st, Ri, W, n.cin, X16, n.cin, n.w16 + n.off_W0, W, n.w + n.off_b0, w.H, w.H16,
+              tmodes ? w.H16T : nullptr);
     for (int l = 0; l < n.layers; ++l) {
-        // spectral branch: Cm = B^T H16 per plane, Cs = Cm D, H += B Cs
-        gemm(st, 0, 0, nb, W, N2, 1.f, n.basisT16, N2, 0, w.H16, W, (long long)N2 * W, 0.f, w.Cm, W,
-             (long long)nb * W, P, w.gwork, w.ngwork);
-        mode_scale(st, w.Cm, n.w + n.off_D[l], nullptr, w.Cs16, P, nb * W);
-        gemm(st, 1, 0, N2, W, nb, 1.f, n.basisT16, N2, 0, w.Cs16, W, (long long)nb * W, 1.f, w.H, W,
-             (long long)N2 * W, P);
-        // MLP branch and the residual sum, H16 updated in place
+        // spectral branch: Cm = B^T H16 per plane (from H16 transposed when the
+        // mode kernel takes the shapes), Cs = Cm D, H += B Cs (inside the MLP
+        // kernel when the planes split into its row tiles)
+        if (tmodes)
+            modes(st, n.basisT16, n.nbp, nb, N2, w.H16T, W, P, w.Cm, w.mwork, w.nmwork);
+        else
+            gemm(st, 0, 0, nb, W, N2, 1.f, n.basisT16, N2, 0, w.H16, W, (long long)N2 * W, 0.f, w.Cm, W,
+                 (long long)nb * W, P, w.gwork, w.ngwork);
         const unsigned short *wt = n.wt16 + (long long)l * 2 * W * Hd;
-        mlp(st, R, W, Hd, w.H16, wt, n.w + n.off_b1[l], wt + (long long)W * Hd, n.w + n.off_b2[l], w.H, w.H16,
-            nullptr, nullptr);
+        if (fused) {
+            mode_scale_t(st, w.Cm, n.w + n.off_D[l], w.Cs16, P, nb, W, n.nbp);
+            mlp(st, R, W, Hd, w.H16, wt, n.w + n.off_b1[l], wt + (long long)W * Hd, n.w + n.off_b2[l], w.H, w.H16,
+                nullptr, nullptr, n.basis16p, w.Cs16, N2, n.nbp, tmodes ? w.H16T : nullptr);
+        } else {
+            mode_scale(st, w.Cm, n.w + n.off_D[l], nullptr, w.Cs16, P, nb * W);
+            gemm(st, 1, 0, N2, W, nb, 1.f, n.basisT16, N2, 0, w.Cs16, W, (long long)nb * W, 1.f, w.H, W,
+                 (long long)N2 * W, P);
+            mlp(st, R, W, Hd, w.H16, wt, n.w + n.off_b1[l], wt + (long long)W * Hd, n.w + n.off_b2[l], w.H, w.H16,
+                nullptr, nullptr, nullptr, nullptr, 0, 0, nullptr);
+        }
     }
     gemm(st, 0, 0, Ri, n.cout, W, 1.f, w.H16, W, 0, n.w16 + n.off_Wp, n.cout, 0, 0.f, Y, n.cout, 0, 1);
     bias_act(st, Y, n.w + n.off_bp, n.cout, R * n.cout, 0, nullptr, nullptr);
@@ -640,10 +926,36 @@ int coral_amd_nn_transpose_bf16(const void *src, int32_t rows, int32_t cols, voi
     NN_ABI(coral::nn::transpose16((hipStream_t)stream, (const unsigned short *)src, rows, cols, (unsigned short *)dst))
 }
 int coral_amd_nn_mlp(int64_t R, int32_t width, int32_t hidden, const void *Hin16, const void *W1t, const float *b1,
-                     const void *W2t, const float *b2, float *H, void *Hout16, float *Zc, void *A16c, void *stream) {
+                     const void *W2t, const float *b2, float *H, void *Hout16, float *Zc, void *A16c,
+                     const void *basis16, const void *CsT16, int32_t N2, int32_t nbp, void *H16T, void *stream) {
     NN_ABI(coral::nn::mlp((hipStream_t)stream, R, width, hidden, (const unsigned short *)Hin16,
                           (const unsigned short *)W1t, b1, (const unsigned short *)W2t, b2, H,
-                          (unsigned short *)Hout16, Zc, (unsigned short *)A16c))
+                          (unsigned short *)Hout16, Zc, (unsigned short *)A16c, (const unsigned short *)basis16,
+                          (const unsigned short *)CsT16, N2, nbp, (unsigned short *)H16T))
+}
+int coral_amd_nn_gemm_bias(int32_t M, int32_t N, int32_t K, const void *A, int32_t lda, const void *B, int32_t ldb,
+                           const float *bias, float *C, void *C16, void *C16T, void *stream) {
+    NN_ABI(coral::nn::gemm_bias((hipStream_t)stream, M, N, K, (const unsigned short *)A, lda,
+                                (const unsigned short *)B, ldb, bias, C, (unsigned short *)C16,
+                                (unsigned short *)C16T))
+}
+int coral_amd_nn_tile_transpose(const void *src, int64_t R, int32_t width, void *dst, void *stream) {
+    NN_ABI(coral::nn::tile_transpose((hipStream_t)stream, (const unsigned short *)src, R, width, (unsigned short *)dst))
+}
+int32_t coral_amd_nn_modes_supported(int32_t nbp, int32_t N2, int32_t width) {
+    return coral::nn::modes_supported(nbp, N2, width) ? 1 : 0;
+}
+int64_t coral_amd_nn_modes_work(int32_t nbp, int32_t N2, int32_t width, int32_t P) {
+    return coral::nn::modes_work(nbp, N2, width, P);
+}
+int coral_amd_nn_modes(const void *BT, int32_t nbp, int32_t nb, int32_t N2, const void *HT, int32_t width, int32_t P,
+                       float *Cm, float *work, int64_t nwork, void *stream) {
+    NN_ABI(coral::nn::modes((hipStream_t)stream, (const unsigned short *)BT, nbp, nb, N2, (const unsigned short *)HT,
+                            width, P, Cm, work, nwork))
+}
+int coral_amd_nn_mode_scale_t(const float *c, const float *d, void *out16T, int32_t P, int32_t nb, int32_t width,
+                              int32_t nbp, void *stream) {
+    NN_ABI(coral::nn::mode_scale_t((hipStream_t)stream, c, d, (unsigned short *)out16T, P, nb, width, nbp))
 }
 int coral_amd_nn_cast_bf16(const float *x, void *y, int64_t n, void *stream) {
     NN_ABI(coral::nn::cast_bf16((hipStream_t)stream, x, (unsigned short *)y, n))

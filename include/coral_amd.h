@@ -387,7 +387,36 @@ int coral_amd_nn_transpose_bf16(const void *src, int32_t rows, int32_t cols, voi
  * when not NULL.  width 32 or 64, hidden 128 or 256.  Same results as
  * coral_amd_nn_gemm, _bias_act (GELU), _gemm, _bias_act, _axpy. */
 int coral_amd_nn_mlp(int64_t R, int32_t width, int32_t hidden, const void *Hin16, const void *W1t, const float *b1,
-                     const void *W2t, const float *b2, float *H, void *Hout16, float *Zc, void *A16c, void *stream);
+                     const void *W2t, const float *b2, float *H, void *Hout16, float *Zc, void *A16c,
+                     const void *basis16, const void *CsT16, int32_t N2, int32_t nbp, void *H16T, void *stream);
+/* With basis16 (the Fourier basis, N2 x nbp BF16, modes contiguous, zero past
+ * nb) and CsT16 (per plane the scaled modes transposed, width x nbp, from
+ * coral_amd_nn_mode_scale_t), coral_amd_nn_mlp also adds the spectral branch's
+ * synthesis: H <- (B Cs + H) + (U + b2), the bits of the unfused GEMM with
+ * beta = 1 followed by the MLP; needs N2 % 32 == 0 and nbp % 32 == 0 (NULL
+ * basis16: no spectral term).  H16T, when not NULL, receives Hout16 in the
+ * tiled transposed layout (R % 32 == 0): row r, column c at ((r / 32) width +
+ * c) 32 + r % 32, which coral_amd_nn_tile_transpose also writes.
+ * coral_amd_nn_mode_scale_t: out16T[p][c][k] =
+ * bf16(c[p][k][c] d[k][c]) for k < nb, zero up to nbp.
+ * coral_amd_nn_modes: the mode projections Cm[p] (nb x width) = B^T H16[p] of
+ * P planes of N2 points from BT (nbp x N2, the basis transposed, zero rows
+ * past nb) and HT (H16 in the tiled transposed layout), on MFMA, chunk partials in
+ * work (coral_amd_nn_modes_work floats) added in a fixed order; for the
+ * shapes coral_amd_nn_modes_supported accepts. */
+int coral_amd_nn_mode_scale_t(const float *c, const float *d, void *out16T, int32_t P, int32_t nb, int32_t width,
+                              int32_t nbp, void *stream);
+int coral_amd_nn_tile_transpose(const void *src, int64_t R, int32_t width, void *dst, void *stream);
+/* C (M x N, row-major) = A (M x K, lda) B (K x N, ldb) + bias (per column),
+ * C16 = bf16(C), C16T = bf16(C) in the tiled transposed layout (M % 32 == 0);
+ * C16 and C16T optional: the bits of coral_amd_nn_gemm, _bias_act and
+ * _tile_transpose in one kernel. */
+int coral_amd_nn_gemm_bias(int32_t M, int32_t N, int32_t K, const void *A, int32_t lda, const void *B, int32_t ldb,
+                           const float *bias, float *C, void *C16, void *C16T, void *stream);
+int32_t coral_amd_nn_modes_supported(int32_t nbp, int32_t N2, int32_t width);
+int64_t coral_amd_nn_modes_work(int32_t nbp, int32_t N2, int32_t width, int32_t P);
+int coral_amd_nn_modes(const void *BT, int32_t nbp, int32_t nb, int32_t N2, const void *HT, int32_t width, int32_t P,
+                       float *Cm, float *work, int64_t nwork, void *stream);
 int coral_amd_nn_cast_bf16(const float *x, void *y, int64_t n, void *stream);
 /* z += bias (per column of ncol); out = act(z) (act 0: identity, 1: GELU, tanh
  * form), out16 its BF16 copy (either may be NULL) */

@@ -87,15 +87,18 @@ def test_split_k_gemm_matches_numpy(nnmod, tA):
     assert rc < 0
 
 
-@pytest.mark.parametrize("W,Hd,R", [(64, 256, 1000), (32, 128, 64 * 7)])
-def test_fused_mlp_matches_unfused_sequence(nnmod, W, Hd, R):
+@pytest.mark.parametrize("W,Hd,R,spectral", [(64, 256, 1000, False), (32, 128, 64 * 7, False),
+                                              (64, 256, 3 * 288, True), (32, 128, 2 * 288, True)])
+def test_fused_mlp_matches_unfused_sequence(nnmod, W, Hd, R, spectral):
     """coral_amd_nn_mlp gives the bits of gemm, bias_act (GELU), gemm,
     bias_act, axpy on the same operands, its caches those of the unfused
     pre-activation and activation, also in place (Hout16 = Hin16) and with a
-    ragged last tile of rows."""
+    ragged last tile of rows; with the spectral synthesis (planes of N2 = 288
+    points, 17 modes padded to 32) the bits of the unfused H += B Cs GEMM
+    (beta = 1) before the MLP."""
     closure, devmem = nnmod
     lib, check = closure.lib, closure.check
-    rng = np.random.default_rng(W + Hd)
+    rng = np.random.default_rng(W + Hd + R)
     H0 = rng.normal(size=(R, W)).astype(np.float32)
     W1, W2 = rng.normal(size=(W, Hd)) / 8, rng.normal(size=(Hd, W)) / 16
     b1, b2 = rng.normal(size=Hd).astype(np.float32), rng.normal(size=W).astype(np.float32)
@@ -105,10 +108,26 @@ def test_fused_mlp_matches_unfused_sequence(nnmod, W, Hd, R):
     check(lib.coral_amd_nn_transpose_bf16(W2d.ptr, Hd, W, W2t.ptr, None), "t")
     b1d, b2d = devmem.DevArray.from_numpy(b1), devmem.DevArray.from_numpy(b2)
     Hin16 = to_dev_bf16(closure, devmem, H0)
+    N2, nb, nbp = 288, 17, 32
+    P = R // N2
+    Bp16 = CsT16 = None
+    if spectral:
+        B = rng.normal(size=(N2, nb)) / 10
+        Cs = rng.normal(size=(P, nb, W))
+        BT16 = to_dev_bf16(closure, devmem, B.T.copy())
+        Cs16 = to_dev_bf16(closure, devmem, Cs)
+        Bpad = np.zeros((N2, nbp))
+        Bpad[:, :nb] = B
+        CsT = np.zeros((P, W, nbp))
+        CsT[:, :, :nb] = np.transpose(Cs, (0, 2, 1))
+        Bp16, CsT16 = to_dev_bf16(closure, devmem, Bpad), to_dev_bf16(closure, devmem, CsT)
     # unfused
     Hu = devmem.DevArray.from_numpy(H0)
     Z, A16, U = devmem.DevArray(R * Hd), devmem.DevArray(R * Hd, "bf16"), devmem.DevArray(R * W)
     Hu16 = devmem.DevArray(R * W, "bf16")
+    if spectral:
+        check(lib.coral_amd_nn_gemm(1, 0, N2, W, nb, 1.0, BT16.ptr, N2, 0, Cs16.ptr, W, nb * W, 1.0, Hu.ptr, W, N2 * W,
+                                    P, None, 0, None), "g")
     check(lib.coral_amd_nn_gemm(0, 0, R, Hd, W, 1.0, Hin16.ptr, W, 0, W1d.ptr, Hd, 0, 0.0, Z.ptr, Hd, 0, 1, None, 0,
                                 None), "g")
     check(lib.coral_amd_nn_bias_act(Z.ptr, b1d.ptr, Hd, R * Hd, 1, None, A16.ptr, None), "b")
@@ -116,14 +135,19 @@ def test_fused_mlp_matches_unfused_sequence(nnmod, W, Hd, R):
                                 None), "g")
     check(lib.coral_amd_nn_bias_act(U.ptr, b2d.ptr, W, R * W, 0, None, None, None), "b")
     check(lib.coral_amd_nn_axpy(Hu.ptr, U.ptr, 1.0, R * W, Hu16.ptr, None), "a")
+    sp = (Bp16.ptr, CsT16.ptr, N2, nbp) if spectral else (None, None, 0, 0)
     # fused, caches on, then in place without caches
     Hf, Hf16 = devmem.DevArray.from_numpy(H0), devmem.DevArray(R * W, "bf16")
     Zf, Af16 = devmem.DevArray(R * Hd), devmem.DevArray(R * Hd, "bf16")
+    HT = devmem.DevArray(R * W, "bf16") if spectral else None
     check(lib.coral_amd_nn_mlp(R, W, Hd, Hin16.ptr, W1t.ptr, b1d.ptr, W2t.ptr, b2d.ptr, Hf.ptr, Hf16.ptr, Zf.ptr,
-                               Af16.ptr, None), "mlp")
+                               Af16.ptr, *sp, HT.ptr if spectral else None, None), "mlp")
     Hi, Hi16 = devmem.DevArray.from_numpy(H0), to_dev_bf16(closure, devmem, H0)
     check(lib.coral_amd_nn_mlp(R, W, Hd, Hi16.ptr, W1t.ptr, b1d.ptr, W2t.ptr, b2d.ptr, Hi.ptr, Hi16.ptr, None, None,
-                               None), "mlp")
+                               *sp, None, None), "mlp")
+    if spectral:   # the new BF16 values in the tiled transposed layout
+        tiled = Hu16.numpy().reshape(R // 32, 32, W).transpose(0, 2, 1).ravel()
+        assert np.array_equal(HT.numpy(), tiled)
     assert np.array_equal(Hf.numpy(), Hu.numpy())
     assert np.array_equal(Hf16.numpy(), Hu16.numpy())
     assert np.array_equal(Zf.numpy(), Z.numpy())
@@ -132,7 +156,63 @@ def test_fused_mlp_matches_unfused_sequence(nnmod, W, Hd, R):
     assert np.array_equal(Hi16.numpy(), Hu16.numpy())
     # and the math: float64 on the BF16 operands
     ref = H0 + (C.bf16(C.gelu(C.bf16(H0) @ C.bf16(W1) + b1)) @ C.bf16(W2) + b2)
+    if spectral:
+        ref = ref + np.einsum("in,pnc->pic", C.bf16(B), C.bf16(Cs)).reshape(R, W)
     assert np.linalg.norm(Hu.numpy().reshape(R, W) - ref) / np.linalg.norm(ref) < 1e-3
+
+
+def test_gemm_bias_matches_unfused_sequence(nnmod):
+    """coral_amd_nn_gemm_bias (the lift: product, bias, BF16 copy, tiled
+    transposed copy in one kernel) gives the bits of coral_amd_nn_gemm,
+    _bias_act and _tile_transpose."""
+    closure, devmem = nnmod
+    lib, check = closure.lib, closure.check
+    rng = np.random.default_rng(21)
+    M, N, K = 32 * 45, 32, 112
+    A = to_dev_bf16(closure, devmem, rng.normal(size=(M, K)))
+    B = to_dev_bf16(closure, devmem, rng.normal(size=(K, N)) / 10)
+    bias = devmem.DevArray.from_numpy(rng.normal(size=N))
+    C1, C2 = devmem.DevArray(M * N), devmem.DevArray(M * N)
+    h1, h2 = devmem.DevArray(M * N, "bf16"), devmem.DevArray(M * N, "bf16")
+    t1, t2 = devmem.DevArray(M * N, "bf16"), devmem.DevArray(M * N, "bf16")
+    check(lib.coral_amd_nn_gemm(0, 0, M, N, K, 1.0, A.ptr, K, 0, B.ptr, N, 0, 0.0, C1.ptr, N, 0, 1, None, 0, None), "g")
+    check(lib.coral_amd_nn_bias_act(C1.ptr, bias.ptr, N, M * N, 0, None, h1.ptr, None), "b")
+    check(lib.coral_amd_nn_tile_transpose(h1.ptr, M, N, t1.ptr, None), "t")
+    check(lib.coral_amd_nn_gemm_bias(M, N, K, A.ptr, K, B.ptr, N, bias.ptr, C2.ptr, h2.ptr, t2.ptr, None), "gb")
+    for a, b in ((C1, C2), (h1, h2), (t1, t2)):
+        assert np.array_equal(a.numpy(), b.numpy())
+
+
+@pytest.mark.parametrize("W,nb,N2,P", [(32, 81, 36864 // 16, 3), (64, 17, 9216, 2)])
+def test_mode_projection_matches_numpy(nnmod, W, nb, N2, P):
+    """coral_amd_nn_modes: Cm[p] = B^T H[p] per plane from the transposed
+    operands (chunk partials added in a fixed order), against numpy on the same
+    BF16 values to FP32 summation, and the same bits on a second call."""
+    closure, devmem = nnmod
+    lib, check = closure.lib, closure.check
+    nbp = (nb + 31) // 32 * 32
+    assert lib.coral_amd_nn_modes_supported(nbp, N2, W) == 1
+    rng = np.random.default_rng(nb + W)
+    B = rng.normal(size=(N2, nb)) / 30
+    H = rng.normal(size=(P, N2, W))
+    BT = np.zeros((nbp, N2))
+    BT[:nb] = B.T
+    BTd = to_dev_bf16(closure, devmem, BT)
+    Hd16 = to_dev_bf16(closure, devmem, H.reshape(P * N2, W))
+    HTd = devmem.DevArray(P * N2 * W, "bf16")
+    check(lib.coral_amd_nn_tile_transpose(Hd16.ptr, P * N2, W, HTd.ptr, None), "tile transpose")
+    assert np.array_equal(HTd.numpy(), Hd16.numpy().reshape(P * N2 // 32, 32, W).transpose(0, 2, 1).ravel())
+    nwork = int(lib.coral_amd_nn_modes_work(nbp, N2, W, P))
+    work = devmem.DevArray(nwork)
+    outs = []
+    for _ in range(2):
+        Cm = devmem.DevArray(P * nb * W)
+        check(lib.coral_amd_nn_modes(BTd.ptr, nbp, nb, N2, HTd.ptr, W, P, Cm.ptr, work.ptr, nwork, None), "modes")
+        outs.append(Cm.numpy())
+    assert np.array_equal(outs[0], outs[1])
+    ref = np.einsum("in,pic->pnc", C.bf16(B), C.bf16(H))
+    got = outs[0].astype(np.float64).reshape(P, nb, W)
+    assert np.linalg.norm(got - ref) / np.linalg.norm(ref) < 1e-5
 
 
 def test_network_forward_backward_match_reference(nnmod):

@@ -12,7 +12,7 @@ from conftest import ROOT, recipe
 
 def header_symbols():
     text = open(os.path.join(ROOT, "include", "coral_amd.h")).read()
-    return sorted(set(re.findall(r"^(?:int|size_t|int64_t)\s+(coral_amd_\w+)\s*\(", text, flags=re.M)))
+    return sorted(set(re.findall(r"^(?:int|size_t|int64_t|int32_t)\s+(coral_amd_\w+)\s*\(", text, flags=re.M)))
 
 
 def test_library_exports_every_header_symbol(native_lib):

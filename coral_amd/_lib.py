@@ -79,7 +79,8 @@ EXPORTS = (
     "coral_amd_nn_colsum_work", "coral_amd_nn_colsum", "coral_amd_nn_mode_scale", "coral_amd_nn_mode_scale_grad",
     "coral_amd_nn_axpy", "coral_amd_nn_mse_partials", "coral_amd_nn_mse", "coral_amd_nn_adam",
     "coral_amd_nn_gemm_work", "coral_amd_nn_transpose_bf16", "coral_amd_nn_mlp",
-    "coral_amd_set_full_variable_filter", "coral_amd_nn_mode_scale_t", "coral_amd_nn_modes_supported",
+    "coral_amd_set_full_variable_filter", "coral_amd_add_source_term", "coral_amd_clear_source_terms",
+    "coral_amd_set_source", "coral_amd_nn_mode_scale_t", "coral_amd_nn_modes_supported",
     "coral_amd_nn_modes_work", "coral_amd_nn_modes", "coral_amd_nn_tile_transpose",
     "coral_amd_nn_gemm_bias",
 )
@@ -122,6 +123,9 @@ def _load():
     lib.coral_amd_banded_solve_noalloc_bytes.argtypes = [c_int32, c_int32, c_int32, c_int32]
     lib.coral_amd_set_launch_shape.argtypes = [P, c_int32, c_int32]
     lib.coral_amd_set_full_variable_filter.argtypes = [P, c_int32, c_int32, c_int32]
+    lib.coral_amd_add_source_term.argtypes = [P, c_int32, c_int32, c_int32, c_double, c_int32]
+    lib.coral_amd_clear_source_terms.argtypes = [P]
+    lib.coral_amd_set_source.argtypes = [P, c_int32, P]
     lib.coral_amd_set_closure.argtypes = [P, ctypes.POINTER(ClosureDesc)]
     lib.coral_amd_get_launch_shape.argtypes = [P, P, P]
     lib.coral_amd_ensemble_create.argtypes = [P, c_int32, PP]

@@ -95,6 +95,9 @@ struct SysDev {
     } mass, stif;
     DevBuf<int> pos_eqn, pos_row, eqn_begin, t_qvar, t_kx, t_ky, t_w, t_order, t_woff;
     DevBuf<double> t_dsca, wv;
+    // mean systems: the sources' constant part of K_hat, one value per
+    // position (zero without sources; coral_amd_set_source)
+    DevBuf<double> src;
     // kxky NL program (NLProgArgs)
     DevBuf<int> nl_pos_ptr, nl_blk_term, nl_blk_ptr, nl_ent_qm;
     DevBuf<double> nl_ent_w;
@@ -158,6 +161,16 @@ struct coral_amd_ctx {
     // per full variable: (boundary-value extraction, vertical decomposition),
     // coral_amd_set_full_variable_filter
     std::vector<std::pair<int, int>> full_filter;
+    // mean-system sources: the terms (coral_amd_add_source_term) and the
+    // sources' Chebyshev coefficients by index (coral_amd_set_source);
+    // src_missing is the lowest index a term names with no definition (0: none)
+    struct SourceTerm {
+        int isys, eqn, iz, index;
+        double dsca;
+    };
+    std::vector<SourceTerm> src_terms;
+    std::map<int, std::vector<double>> src_cheb;
+    int src_missing = 0;
     // learned subgrid closure (coral_amd_set_closure): the network, its
     // normalisation, inference workspace and the correction's buffers
     struct Closure {
@@ -608,6 +621,7 @@ void nl_assemble(coral_amd_ctx &c, SysDev &d, double2 *Khat) {
     a.t_woff = d.t_woff.p;
     a.t_dsca = d.t_dsca.p;
     a.wv = d.wv.p;
+    a.src = d.kxky ? nullptr : d.src.p;
     a.km = c.km;
     a.Khat = Khat;
     c.timed(d.kxky ? "nl_assemble" : "nl_mean", [&] { launch_nl(c.stream, a); });
@@ -1057,6 +1071,9 @@ void project_kx0(coral_amd_ctx &c) {
 // + sum_{j<=i} dt ah_{i+1,j} K_hat_j, and solves (M - dt a_ii L) x_i = rhs_i;
 // x^{n+1} = x_s.
 void one_step(coral_amd_ctx &c) {
+    if (c.src_missing)
+        throw std::runtime_error("source" + std::to_string(c.src_missing / 10) + std::to_string(c.src_missing % 10) +
+                                 " is used by a mean equation but has no definition (coral_amd_set_source)");
     const Problem &P = c.P;
     const int s = P.nstages;
     const double dt = c.dt;
@@ -1240,6 +1257,7 @@ int coral_amd_create(const coral_amd_problem *prob, int device, coral_amd_ctx **
             d.kxky = false;
             d.nsys = 1;
             fill_system_device(d, P.nstages, c->km, c->stream, P.NZ);
+            d.src.upload(std::vector<double>(std::max(t.total, 1), 0.0));
         }
         c->full_has_kxky.assign(P.nfull, 0);
         for (auto &e : P.extract) {
@@ -2311,6 +2329,110 @@ int coral_amd_set_closure(coral_amd_ctx *c, const coral_amd_closure *cl) {
         return 0;
     } catch (const std::exception &e) {
         if (c) c->closure.on = false;
+        return fail(e);
+    }
+}
+
+namespace {
+// The sources' part of each mean system's K_hat, the same at every stage:
+// for each term, its source's first NZ Chebyshev coefficients times dsca,
+// through trunc(I^iz)[order:, :] into the rows of its equation
+// (PL_IMEX_timestepping_marching.f90:399-416).  Terms whose source has no
+// definition are left out and the lowest such index kept in src_missing, on
+// which stepping stops.
+void rebuild_sources(coral_amd_ctx &c) {
+    const Problem &P = c.P;
+    const int NZ = P.NZ;
+    c.src_missing = 0;
+    std::vector<std::vector<double>> v(c.zs.size());
+    for (size_t i = 0; i < c.zs.size(); ++i) v[i].assign(c.zs[i]->src.n, 0.0);
+    std::map<int, Dense> im;
+    for (const auto &st : c.src_terms) {
+        const auto it = c.src_cheb.find(st.index);
+        if (it == c.src_cheb.end()) {
+            if (!c.src_missing || st.index < c.src_missing) c.src_missing = st.index;
+            continue;
+        }
+        const SystemTables &t = *c.zs[st.isys]->t;
+        auto m = im.find(st.iz);
+        if (m == im.end()) m = im.emplace(st.iz, cheb_IM(NZ, P.center, P.gap, st.iz, 0)).first;
+        const Dense &I = m->second;
+        const int order = t.eqn_orders[st.eqn];
+        std::vector<double> b(NZ);
+        for (int k = 0; k < NZ; ++k) b[k] = it->second[k] * st.dsca;
+        for (int p = 0; p < t.total; ++p) {
+            if (t.pos_eqn[p] != st.eqn) continue;
+            const int r = t.pos_row[p];
+            double acc = 0.0;
+            for (int k = 0; k < NZ; ++k) acc += I(order + r, k) * b[k];
+            v[st.isys][p] += acc;
+        }
+    }
+    // the vectors are read by every step in flight, including an ensemble's
+    HIP_OK(hipDeviceSynchronize());
+    for (size_t i = 0; i < c.zs.size(); ++i)
+        HIP_OK(hipMemcpy(c.zs[i]->src.p, v[i].data(), v[i].size() * sizeof(double), hipMemcpyHostToDevice));
+}
+}  // namespace
+
+int coral_amd_add_source_term(coral_amd_ctx *c, int32_t isys, int32_t eqn, int32_t iz, double dsca,
+                              int32_t source_index) {
+    try {
+        if (!c) throw std::runtime_error("null context");
+        if (isys < 0 || isys >= (int)c->zs.size()) throw std::runtime_error("mean system index out of range");
+        if (eqn < 0 || eqn >= c->zs[isys]->t->nvars) throw std::runtime_error("equation index out of range");
+        if (iz < 0) throw std::runtime_error("negative Iz exponent");
+        if (source_index < 1 || source_index > 99) throw std::runtime_error("source index must be in 1..99");
+        HIP_OK(hipSetDevice(c->device));
+        c->src_terms.push_back({isys, eqn, iz, source_index, dsca});
+        rebuild_sources(*c);
+        return 0;
+    } catch (const std::exception &e) {
+        return fail(e);
+    }
+}
+
+int coral_amd_clear_source_terms(coral_amd_ctx *c) {
+    try {
+        if (!c) throw std::runtime_error("null context");
+        HIP_OK(hipSetDevice(c->device));
+        c->src_terms.clear();
+        rebuild_sources(*c);
+        return 0;
+    } catch (const std::exception &e) {
+        return fail(e);
+    }
+}
+
+int coral_amd_set_source(coral_amd_ctx *c, int32_t source_index, const double *physical) {
+    try {
+        if (!c) throw std::runtime_error("null context");
+        if (source_index < 1 || source_index > 99) throw std::runtime_error("source index must be in 1..99");
+        HIP_OK(hipSetDevice(c->device));
+        if (!physical) {
+            c->src_cheb.erase(source_index);
+        } else {
+            // Compute_1dCheby_transform (PL_IMEX_timestepping_marching.f90:845-866):
+            // REDFT10 of the NZAA Gauss-point values over NZAA, the first halved;
+            // the first NZ coefficients are used.  In the cosine's argument
+            // pi k (2j+1) / (2 NZAA), k (2j+1) is reduced modulo 4 NZAA (its period).
+            const int N = c->P.NZAA, NZ = c->P.NZ;
+            std::vector<double> coef(NZ, 0.0);
+            const double h = 3.14159265358979323846 / (2.0 * N);
+            for (int k = 0; k < NZ; ++k) {
+                double acc = 0.0;
+                for (int j = 0; j < N; ++j) {
+                    const long long m = (long long)k * (2 * j + 1) % (4LL * N);
+                    acc += physical[j] * std::cos(h * double(m));
+                }
+                coef[k] = 2.0 * acc / N;
+            }
+            coef[0] *= 0.5;
+            c->src_cheb[source_index] = coef;
+        }
+        rebuild_sources(*c);
+        return 0;
+    } catch (const std::exception &e) {
         return fail(e);
     }
 }

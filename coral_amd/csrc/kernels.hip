@@ -947,7 +947,9 @@ void launch_term_monomials(hipStream_t st, const int *t_kx, const int *t_ky, int
     CHECK_LAUNCH();
 }
 
-// mean systems: real part of the (0,0) column
+// mean systems: real part of the (0,0) column, then the sources' constant part
+// (PL_IMEX_timestepping_marching.f90:369-417: the NL terms, then the sources,
+// accumulate into K_hat)
 __global__ void k_nl_mean(NLArgs a) {
     const int pos = blockIdx.x * blockDim.x + threadIdx.x;
     if (pos >= a.total) return;
@@ -966,6 +968,7 @@ __global__ void k_nl_mean(NLArgs a) {
         }
         acc += a.t_dsca[t] * sum;
     }
+    if (a.src) acc += a.src[pos];
     a.Khat[pos] = make_double2(acc, 0.0);
 }
 

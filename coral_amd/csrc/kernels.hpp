@@ -170,6 +170,7 @@ struct NLArgs {
     const int *pos_eqn, *pos_row, *eqn_begin;
     const int *t_qvar, *t_kx, *t_ky, *t_w, *t_order, *t_woff;
     const double *t_dsca, *wv;
+    const double *src;        // mean systems: added to each position's sum (the sources' part), or null
     KeptModes km;
     double2 *Khat;            // [nsys][total]
 };

@@ -18,8 +18,13 @@ grammar, the meaning of each token and the bookkeeping follow Coral's parser:
   for 1 (``PL_equations.f90:890-954``); ``zFiltered_quadratic_variable`` sets
   ``remove_z_integral`` (``:918-925``).
 
-Features of the grammar that the time-stepper does not implement (sources,
-penalisation) raise ``NotImplementedError`` naming the keyword, rather than
+* ``sourceParam`` values and ``add_rhs_source`` terms of the mean systems
+  (``PL_equations.f90:985-996,1156-1230``); the sources themselves are
+  profiles that the caller defines (``Solver.set_sources``), as the
+  reference's user include ``PL_user_sources_definitions.f90`` does.
+
+Features of the grammar that the time-stepper does not implement
+(penalisation) raise ``NotImplementedError`` naming the keyword, rather than
 being silently skipped.
 """
 from __future__ import annotations
@@ -58,6 +63,18 @@ class NLTerm:
     mz: int
     qvar: int
     eqn: int
+
+
+@dataclasses.dataclass
+class SourceTerm:
+    """``dsca * Iz^iz sourceNN`` added to equation ``eqn`` of mean system
+    ``system`` (0-based); ``index`` = NN.  Mirrors ``source_term_T``
+    (``PL_equations.f90:37-45``)."""
+    dsca: float
+    iz: int
+    index: int
+    eqn: int
+    system: int
 
 
 @dataclasses.dataclass
@@ -124,6 +141,10 @@ class Recipe:
     quad_vars: List[QuadraticVariable]
     kxky_systems: List[CoupledSystem]
     zero_systems: List[CoupledSystem]
+    # sourceParam values by name, the name cut to its first six characters as
+    # the reference's sourceParams_T%str (character(len=6), PL_equations.f90:47-50)
+    source_params: Dict[str, float] = dataclasses.field(default_factory=dict)
+    sources: List[SourceTerm] = dataclasses.field(default_factory=list)
 
     def var_location(self, name: str) -> Tuple[str, int, int]:
         for kind, systems in (("kxky", self.kxky_systems), ("zero", self.zero_systems)):
@@ -191,13 +212,42 @@ def _atom_tokens(atom: str, dz_token: str):
 
 
 _UNSUPPORTED = (
-    "sourceParam", "enable_penalisation", "penalisation_strength", "penalisation_width", "add_source",
+    "enable_penalisation", "penalisation_strength", "penalisation_width",
 )
+
+
+def _source_term(atom: str, params: Dict[str, float], sgn: float, eqn: int, system: int) -> SourceTerm:
+    """One ``add_rhs_source`` atom (``build_zero_sources_in_full_recipe``,
+    ``PL_equations.f90:1156-1230``): ``Iz`` tokens count the exponent, an
+    eight-character ``sourceNN`` names the source, any other name is a
+    parameter multiplying the term.  As in the reference, the names after
+    ``sourceNN`` in an atom are not read, and an ``mz`` token has no effect
+    (the source enters through trunc(I^iz) alone, ``PL_IMEX_timestepping_marching.f90:407-409``)."""
+    kx, ky, iz, _mz, names = _atom_tokens(atom, "Iz")
+    if kx or ky:
+        raise EquationsError(f"horizontal derivative in a mean-field source: {atom!r}")
+    dsca, index = sgn, 0
+    for nm in names:
+        if len(nm) == 8 and nm[:6] == "source":
+            try:
+                index = int(nm[6:8])
+            except ValueError as exc:
+                raise EquationsError("Source terms must be of the form sourceNN, where NN are 2 integers") from exc
+            break
+        if nm in params:
+            dsca = params[nm] * dsca
+        else:
+            raise EquationsError(f"unknown string in equations: {nm!r}")
+    if not 1 <= index <= 99:
+        raise EquationsError(f"invalid source_index in {atom!r}")
+    return SourceTerm(dsca, iz, index, eqn, system)
 
 
 def parse_equations(text: str) -> Recipe:
     """Parse the text of a ``coral.equations`` file into a :class:`Recipe`."""
     params: Dict[str, float] = {}
+    source_params: Dict[str, float] = {}
+    sources: List[SourceTerm] = []
     kxky_vars: List[str] = []
     zero_vars: List[str] = []
     full_vars: List[FullVariable] = []
@@ -232,6 +282,18 @@ def parse_equations(text: str) -> Recipe:
         if key == "Param":
             name, val = payload.split(":=", 1)
             params[name.strip()] = _fortran_float(val)
+        elif key == "sourceParam":
+            # add_source_parameter_to_full_recipe (PL_equations.f90:985-996)
+            name, val = payload.split(":=", 1)
+            source_params[name.strip()[:6]] = _fortran_float(val)
+        elif key == "add_rhs_source":
+            if current is None or not current.eqn_orders:
+                raise EquationsError(f"{key} outside an equation")
+            if current_kind != "zero":
+                raise EquationsError("add_rhs_source in a kxky coupled set: sources drive the mean systems only")
+            eqn = len(current.eqn_orders) - 1
+            for sgn, atom in _split_signed_atoms(payload):
+                sources.append(_source_term(atom, params, sgn, eqn, len(zero_systems) - 1))
         elif key == "linear_variable_kxky":
             kxky_vars.append(payload)
         elif key == "linear_variable_mean":
@@ -310,7 +372,7 @@ def parse_equations(text: str) -> Recipe:
     if not saw_eof:
         raise EquationsError("missing >>EOF<< line")
 
-    recipe = Recipe(params, kxky_vars, zero_vars, full_vars, [], kxky_systems, zero_systems)
+    recipe = Recipe(params, kxky_vars, zero_vars, full_vars, [], kxky_systems, zero_systems, source_params, sources)
 
     # full variables are resolved once all coupled sets are known
     for ifull, expr in full_build_pending:

@@ -24,8 +24,9 @@ from typing import Dict, List, Optional, Tuple
 
 import numpy as np
 
-from . import _lib
+from . import _lib, sources
 from ._lib import check, lib
+from .diagnostics import gauss_chebyshev_grid
 from .equations import Recipe
 from .schemes import tableau
 
@@ -105,6 +106,10 @@ class Solver:
             if ex or zd:
                 check(lib.coral_amd_set_full_variable_filter(self._ctx, i, ex, zd),
                       "coral_amd_set_full_variable_filter")
+        # the mean systems' source terms; the sources themselves: set_sources
+        for t in recipe.sources:
+            check(lib.coral_amd_add_source_term(self._ctx, t.system, t.eqn, t.iz, t.dsca, t.index),
+                  "coral_amd_add_source_term")
         self.dt: Optional[float] = None
         self.totals = [self.system_info(i)[0] for i in range(len(recipe.kxky_systems))]
         self.zero_totals = [sum(NZ - bc // 10 for bc in s.bc_codes) for s in recipe.zero_systems]
@@ -291,6 +296,24 @@ class Solver:
         out = np.zeros((nq, self.NX // 2, self.NYAA, self.NZ), np.complex128)
         check(lib.coral_amd_products(self._ctx, out.ctypes.data), "coral_amd_products")
         return out
+
+    def set_sources(self, definitions: Dict[int, "sources.Definition"]):
+        """Define sources by index NN (the ``sourceNN`` of ``add_rhs_source``
+        lines): a callable ``f(z, params)`` of the NZAA Gauss-Chebyshev points
+        (top wall first) and the recipe's ``sourceParams`` lookup, an array of
+        the NZAA values, or a constant (``init_all_sources``,
+        ``PL_IMEX_timestepping_marching.f90:829-843``).  Only the sources that
+        the recipe's terms name are evaluated; ``coral_amd.sources.SHIPPED``
+        is the reference's shipped definitions file.  Stepping fails while a
+        named source has no definition."""
+        z = gauss_chebyshev_grid(self.NZAA, self.center, self.gap)
+        params = sources.SourceParams(self.recipe.source_params)
+        used = {t.index for t in self.recipe.sources}
+        for nn, d in definitions.items():
+            if nn not in used:
+                continue
+            vals = sources.evaluate(d, z, params)
+            check(lib.coral_amd_set_source(self._ctx, int(nn), vals.ctypes.data), "coral_amd_set_source")
 
     def set_precision(self, mode: str = "fp64", refine: int = 0):
         """``"fp64"``: the FP64 step; ``"mixed"``: FP32 transforms and products

@@ -325,6 +325,24 @@ int coral_amd_set_closure(coral_amd_ctx *ctx, const coral_amd_closure *closure);
  * (its vertical mean removed).  Set after coral_amd_create. */
 int coral_amd_set_full_variable_filter(coral_amd_ctx *ctx, int32_t ifull, int32_t extract, int32_t zdecomp);
 
+/* Sources of the mean systems.  A term (an add_rhs_source atom, built by
+ * build_zero_sources_in_full_recipe, PL_equations.f90:1156-1230) adds
+ * dsca Iz^iz sourceNN to equation eqn of mean system isys (0-based), with
+ * source_index = NN in 1..99; coral_amd_clear_source_terms removes all terms.
+ * coral_amd_set_source defines source NN from its NZAA values on the
+ * Gauss-Chebyshev grid z_j = center + gap/2 cos(pi (2j+1) / (2 NZAA)),
+ * j = 0..NZAA-1 (top wall first; the sourceDef array of
+ * PL_user_sources_definitions.f90 and add_source_array,
+ * PL_IMEX_timestepping_marching.f90:754-766), or removes it (NULL).  Every
+ * stage adds trunc(I^iz)[order:, :] (dsca c) to the equation's rows of K_hat,
+ * c the first NZ Chebyshev coefficients of the source (REDFT10 / NZAA, the
+ * first halved; PL_IMEX_timestepping_marching.f90:399-416, 845-866).
+ * Stepping fails while a term names a source with no definition. */
+int coral_amd_add_source_term(coral_amd_ctx *ctx, int32_t isys, int32_t eqn, int32_t iz, double dsca,
+                              int32_t source_index);
+int coral_amd_clear_source_terms(coral_amd_ctx *ctx);
+int coral_amd_set_source(coral_amd_ctx *ctx, int32_t source_index, const double *physical);
+
 /* Launch shapes of a run (no counterpart in the reference): xpass_wpb is the
  * largest number of wavefronts per workgroup of the x pass (1..4; 0 = the most
  * that fit its LDS), z_plain = 1 runs the z transforms as vector GEMMs

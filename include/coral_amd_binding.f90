@@ -103,6 +103,26 @@ module coral_amd_binding
        type(c_ptr), value :: ctx
        integer(c_int32_t), value :: ifull, extract, zdecomp
      end function
+     ! mean-system sources: a term dsca Iz^iz sourceNN of equation eqn of
+     ! mean system isys (0-based); source NN from its NZAA Gauss-point values
+     ! (top wall first; c_null_ptr removes it)
+     integer(c_int) function coral_amd_add_source_term(ctx, isys, eqn, iz, dsca, source_index) bind(C)
+       import
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: isys, eqn, iz
+       real(c_double), value :: dsca
+       integer(c_int32_t), value :: source_index
+     end function
+     integer(c_int) function coral_amd_clear_source_terms(ctx) bind(C)
+       import
+       type(c_ptr), value :: ctx
+     end function
+     integer(c_int) function coral_amd_set_source(ctx, source_index, physical) bind(C)
+       import
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: source_index
+       type(c_ptr), value :: physical
+     end function
      ! precision 0: FP64; 1: FP32 transforms and products, FP64 solves
      ! (refine = 1: one FP64 residual-refinement sweep per stage)
      integer(c_int) function coral_amd_set_precision(ctx, precision, refine) bind(C)

@@ -534,6 +534,7 @@ class OracleSolver:
     def __init__(self, recipe, geo: Geometry, scheme: str = "ARS_222"):
         self.recipe, self.geo = recipe, geo
         self.A, self.Ah = imex_scheme(scheme)
+        self.source_cheb: Dict[int, np.ndarray] = {}
         self.s = self.A.shape[0]
         NZ = geo.NZ
         self.kops = [SystemOps(s, NZ, geo.center, geo.gap) for s in recipe.kxky_systems]
@@ -731,8 +732,26 @@ class OracleSolver:
                 order = ops.sys.eqn_orders[t.eqn]
                 W = ops.IM(t.iz, t.mz)[order:, :]
                 K[ops.insert_rows(t.eqn)] += W @ (t.dsca * qhat[t.qvar][0, 0, : g.NZ].real)
+            # sources (marching.f90:399-416): dsca times the source's first NZ
+            # Chebyshev coefficients through trunc(I^iz)[order:, :]
+            for t in getattr(self.recipe, "sources", []):
+                if t.system != isys:
+                    continue
+                if t.index not in self.source_cheb:
+                    raise ValueError(f"source{t.index:02d} has no definition")
+                order = ops.sys.eqn_orders[t.eqn]
+                W = ops.IM(t.iz, 0)[order:, :]
+                K[ops.insert_rows(t.eqn)] += W @ (self.source_cheb[t.index][: g.NZ] * t.dsca)
             zkhat.append(K)
         return khat, zkhat
+
+    def set_source(self, index: int, physical: np.ndarray):
+        """Source ``index`` from its NZAA values on the Gauss-Chebyshev grid
+        (top wall first): Compute_1dCheby_transform (marching.f90:845-866),
+        REDFT10 / NZAA with the first coefficient halved."""
+        c = sfft.dct(np.asarray(physical, np.float64), type=2) / self.geo.NZAA
+        c[0] *= 0.5
+        self.source_cheb[index] = c
 
     # --- one_step_beyond (marching.f90:3-48) -------------------------------
     def step(self, dt: float):

@@ -28,6 +28,12 @@ INPUTS = {
     "example.coral.parameters.in": "etc/coral_equations_examples/coral.parameters.in",
     "rrbc_julien.coral.usrOutput": "etc/benchmarks/02_RRBC_julien_JFM96/coral.usrOutput",
     "example.coral.usrOutput": "etc/coral_equations_examples/coral.usrOutput",
+    "stratified_channel.coral.equations.velocityPressure":
+        "etc/benchmarks/coral.equations.stratifiedChannel.velocityPressure",
+    "stratified_channel.coral.equations.toroidalPoloidal":
+        "etc/benchmarks/coral.equations.stratifiedChannel.toroidalPoloidal",
+    "baroclinic_turbulence.coral.equations": "etc/baroclinic_turbulence/coral.equations",
+    "radiative_heating.coral.equations": "etc/coral_equations_examples/coral.equations.radiative_heating_potentials",
 }
 
 

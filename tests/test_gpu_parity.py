@@ -736,3 +736,83 @@ def test_full_variable_filters_match_oracle(solver_mod, extract, zdecomp, NZ):
     assert rel(k[0], ora.field[0]) <= TOL
     for a, b in zip(z, ora.zfield):
         assert rel(a, b) <= TOL
+
+
+# --- mean-system sources ------------------------------------------------------
+SOURCE_CASES = [
+    # reference equation file, definitions, kxky perturbations
+    ("stratified_channel.coral.equations.velocityPressure", "SHIPPED",
+     (("u", 1e-2), ("v", 1e-2), ("w", 1e-2), ("theta", 1e-2))),
+    ("radiative_heating.coral.equations", "SHIPPED", (("psi", 1e-2), ("phi", 1e-2), ("theta", 1e-2))),
+    ("baroclinic_turbulence.coral.equations", "BAROCLINIC_TURBULENCE", (("psi", 1e-2), ("phi", 1e-2), ("b", 1e-2))),
+]
+
+
+@pytest.mark.parametrize("case", SOURCE_CASES, ids=[c[0].split(".")[0] for c in SOURCE_CASES])
+def test_mean_sources_match_oracle(solver_mod, case):
+    """Recipes of the reference with add_rhs_source terms (the stratified
+    channel of etc/benchmarks, the radiative-heating example, the baroclinic
+    turbulence case with its own definitions file), their sources from the
+    definitions files restated in coral_amd.sources: three steps from a
+    seeded state against the oracle, every system."""
+    from coral_amd import sources as S
+    name, defs, amps = case
+    print("case", name, flush=True)
+    r = recipe(name)
+    defs = getattr(S, defs)
+    NX, NY, NZ = 16, 16, 17
+    k0, z0 = seeded_state(r, NX, NY, NZ, 6, 6, 5, amps=amps)
+    rng = np.random.default_rng(7)
+    z0 = [a + 1e-3 * rng.normal(size=a.shape) for a in z0]
+    geo = O.Geometry(NX, NY, NZ, 2 * np.pi, 2 * np.pi)
+    ora = O.OracleSolver(r, geo, "ARS_222")
+    ora.field, ora.zfield = [a.copy() for a in k0], [a.copy() for a in z0]
+    zg = geo.gauss_chebyshev_grid()
+    for t in r.sources:
+        ora.set_source(t.index, S.evaluate(defs[t.index], zg, S.SourceParams(r.source_params)))
+    with solver_mod.Solver(r, NX, NY, NZ, 2 * np.pi, 2 * np.pi) as g:
+        g.set_state(k0, z0)
+        g.set_sources(defs)
+        for it in range(3):
+            ora.step(1e-4)
+            g.march_forward(1e-4)
+        k, z = g.get_state()
+    for a, b in zip(k, ora.field):
+        assert rel(a, b) <= TOL
+    for i, (a, b) in enumerate(zip(z, ora.zfield)):
+        assert rel(a, b) <= TOL, r.zero_systems[i].var_names
+
+
+def test_mean_sources_closed_forms_on_gpu(solver_mod):
+    """Without fluctuations the stratified channel's mean flow steps to
+    u = (z - z^2/2) / nu (bottom no-slip, top stress-free, unit forcing from
+    source02 = -1); the baroclinic case's algebraic mean equations
+    0 = -UShear + sourceNN hold the sources' Chebyshev coefficients,
+    exp(-(1 - z)/L) = e^{-1/(2L)} (I_0(a) + 2 sum I_k(a) T_k(2z - 1)), a = 1/(2L),
+    times 1/L and 1/L^2 for sources 4 and 5.  Stepping with a source that has
+    no definition fails."""
+    import scipy.special as ss
+    from coral_amd import sources as S
+    r = recipe("stratified_channel.coral.equations.velocityPressure")
+    NX, NY, NZ = 8, 8, 17
+    with solver_mod.Solver(r, NX, NY, NZ, 1.0, 1.0) as g:
+        with pytest.raises(Exception, match="source02"):
+            g.march_forward(2e-3)
+        g.set_sources(S.SHIPPED)
+        g.march_forward(2e-3, 80)
+        u = g.physical(0)
+    zg = O.Geometry(NX, NY, NZ, 1.0, 1.0).gauss_chebyshev_grid()
+    want = (zg - zg ** 2 / 2) / r.params["invReTau"]
+    assert np.abs(u - want[None, None, :]).max() <= 1e-10 * np.abs(want).max()
+    r = recipe("baroclinic_turbulence.coral.equations")
+    L = r.source_params["shearL"]
+    a = 1.0 / (2 * L)
+    k = np.arange(NZ)
+    series = np.exp(-a) * 2 * ss.iv(k, a)
+    series[0] /= 2
+    with solver_mod.Solver(r, NX, NY, NZ, 2 * np.pi, 2 * np.pi) as g:
+        g.set_sources(S.BAROCLINIC_TURBULENCE)
+        g.march_forward(1e-4)
+        _, z = g.get_state()
+    for isys, scale in ((1, 1.0), (2, 1.0 / L), (3, 1.0 / L ** 2)):
+        np.testing.assert_allclose(z[isys], scale * series, rtol=1e-12, atol=1e-15)

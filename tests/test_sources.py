@@ -1,0 +1,140 @@
+"""Mean-system sources on the CPU: the parser on the reference's own equation
+files that use them, the sourceParams lookup, the oracle's Chebyshev
+coefficients of a source against a closed-form series, and the oracle's
+time stepping of the stratified channel's mean flow to the closed-form steady
+state that its source fixes.  No GPU needed."""
+import numpy as np
+import pytest
+import scipy.special as ss
+
+from conftest import recipe
+from coral_amd import sources as S
+from coral_amd.equations import EquationsError, SourceTerm, parse_equations
+from oracle import coral_oracle as O
+
+
+def test_parser_reads_the_reference_source_files():
+    r = recipe("stratified_channel.coral.equations.velocityPressure")
+    assert r.sources == [SourceTerm(-1.0, 2, 2, 0, 0)] and r.source_params == {}
+    r = recipe("stratified_channel.coral.equations.toroidalPoloidal")
+    assert [(t.dsca, t.iz, t.index, t.eqn) for t in r.sources] == [(-1.0, 2, 2, 0)]
+    r = recipe("radiative_heating.coral.equations")
+    assert r.source_params == {"HovEll": 41.66666}
+    assert [(t.dsca, t.iz, t.index, t.eqn, t.system) for t in r.sources] == [(1.0, 2, 1, 0, 1), (1.0, 2, 2, 0, 1)]
+    r = recipe("baroclinic_turbulence.coral.equations")
+    assert r.source_params == {"HovEll": 0.5, "shearL": 0.5}
+    assert [(t.dsca, t.iz, t.index, t.eqn, t.system) for t in r.sources] == [
+        (1.0, 0, 3, 0, 1), (1.0, 0, 4, 0, 2), (1.0, 0, 5, 0, 3)]
+
+
+def _mean_recipe(source_line, extra=""):
+    return parse_equations(
+        ">>Param :: nu := 2.d0 <<\n>>Param :: half := 0.5d0 <<\n" + extra +
+        ">>linear_variable_kxky :: a <<\n>>linear_variable_mean :: m <<\n"
+        ">>add_set_of_coupled_kxky_equations <<\n>>linearly_coupled_var :: a <<\n>>add_BC_for_this_var :: 20 <<\n"
+        ">>new_equation :: 2 <<\n>>add_d/dt_term :: + Iz.Iz.a <<\n>>add_rhs_linear :: + nu.a <<\n"
+        ">>add_set_of_coupled_zero_equations <<\n>>linearly_coupled_var :: m <<\n>>add_BC_for_this_var :: 20 <<\n"
+        ">>new_equation :: 2 <<\n>>add_d/dt_term :: + Iz.Iz.m <<\n>>add_rhs_linear :: + nu.m <<\n"
+        f">>add_rhs_source :: {source_line} <<\n>>EOF<<\n")
+
+
+def test_parser_source_atoms():
+    """build_zero_sources_in_full_recipe (PL_equations.f90:1156-1230):
+    parameters before sourceNN multiply the term, names after it are not
+    read, mz has no effect; a seven-character 'source1', a non-numeric NN,
+    source00, a horizontal derivative and a source in a kxky set are errors."""
+    r = _mean_recipe("- half.Iz.Iz.source07 + nu.source12.unknownName")
+    assert r.sources == [SourceTerm(-0.5, 2, 7, 0, 0), SourceTerm(2.0, 0, 12, 0, 0)]
+    assert _mean_recipe("+ mz.Iz.source03").sources == [SourceTerm(1.0, 1, 3, 0, 0)]
+    for bad in ("+ Iz.source1", "+ Iz.sourceAB", "+ Iz.source00", "+ dx.source01", "+ Iz.half"):
+        with pytest.raises(EquationsError):
+            _mean_recipe(bad)
+    assert _mean_recipe("+ Iz.source01").sources[0].index == 1
+    with pytest.raises(EquationsError):
+        parse_equations(
+            ">>linear_variable_kxky :: a <<\n>>add_set_of_coupled_kxky_equations <<\n>>linearly_coupled_var :: a <<\n"
+            ">>add_BC_for_this_var :: 20 <<\n>>new_equation :: 2 <<\n>>add_d/dt_term :: + Iz.Iz.a <<\n"
+            ">>add_rhs_source :: + Iz.Iz.source01 <<\n>>EOF<<\n")
+    # sourceParam names keep their first six characters (character(len=6))
+    r = _mean_recipe("+ source01", ">>sourceParam :: shearLength := 3.d0 <<\n")
+    assert r.source_params == {"shearL": 3.0}
+
+
+def test_source_params_lookup():
+    """sourceParams (PL_IMEX_timestepping_marching.f90:711-734): 0 with no
+    sourceParam line; the value whose first six characters match; a miss
+    stops."""
+    assert S.SourceParams({})("HovEll") == 0.0
+    p = S.SourceParams({"HovEll": 0.5, "shearL": 2.0})
+    assert p("HovEll") == 0.5 and p("shearLength") == 2.0
+    with pytest.raises(KeyError):
+        p("other")
+    z = np.linspace(0, 1, 5)
+    np.testing.assert_allclose(S.evaluate(S.BAROCLINIC_TURBULENCE[4], z, p), np.exp(-(1 - z) / 2.0) / 2.0)
+    assert np.array_equal(S.evaluate(-1.0, z, p), -np.ones(5))
+    with pytest.raises(ValueError):
+        S.evaluate(np.ones(4), z, p)
+    # the shipped bottom heating integrates to zero over the unit layer
+    zz = np.linspace(0, 1, 200001)
+    f = S.evaluate(S.SHIPPED[1], zz, S.SourceParams({"HovEll": 41.66666}))
+    assert abs(np.trapezoid(f, zz)) < 1e-6
+
+
+@pytest.mark.parametrize("a", [0.7, 2.5])
+def test_oracle_source_coefficients_are_the_chebyshev_series(a):
+    """exp(a x) = I_0(a) + 2 sum_k I_k(a) T_k(x) (modified Bessel functions):
+    the oracle's coefficients of that profile on the Gauss-Chebyshev grid
+    (Compute_1dCheby_transform: REDFT10 / N, the first halved) are the series'
+    coefficients; a missing halving or 1/N fails."""
+    geo = O.Geometry(4, 4, 24, 1.0, 1.0, gap=2.0, center=0.0)
+    s = O.OracleSolver(_mean_recipe("+ source01"), geo)
+    x = geo.gauss_chebyshev_grid()
+    s.set_source(1, np.exp(a * x))
+    k = np.arange(geo.NZ)
+    want = 2 * ss.iv(k, a)
+    want[0] /= 2
+    np.testing.assert_allclose(s.source_cheb[1][: geo.NZ], want, rtol=1e-12, atol=1e-15)
+
+
+@pytest.mark.parametrize("name", ["stratified_channel.coral.equations.velocityPressure",
+                                  "stratified_channel.coral.equations.toroidalPoloidal"])
+def test_stratified_channel_mean_flow_reaches_the_closed_form(name):
+    """The stratified channel of etc/benchmarks: uMean obeys
+    Iz Iz d/dt uMean = nu uMean - Iz Iz source02 with source02 = -1 (the shipped
+    definitions), bottom no-slip and top stress-free (BC 23), i.e.
+    d/dt u = nu u'' + 1; without fluctuations its steady state is
+    u = (z - z^2/2) / nu on [0, 1].  The oracle's time stepping reaches it to
+    round-off (a wrong sign, Iz power, row offset or coefficient scaling of
+    the source does not)."""
+    r = recipe(name)
+    nu = r.params["invReTau"]
+    geo = O.Geometry(8, 8, 17, 1.0, 1.0)
+    s = O.OracleSolver(r, geo)
+    z = geo.gauss_chebyshev_grid()
+    p = S.SourceParams(r.source_params)
+    s.set_source(2, S.evaluate(S.SHIPPED[2], z, p))
+    s.run(2e-3, 80)
+    u = s.physical_full_variables()[0]
+    want = (z - z ** 2 / 2) / nu
+    assert np.abs(u - want[None, None, :]).max() <= 1e-12 * np.abs(want).max()
+
+
+def test_oracle_algebraic_mean_equations_hold_the_sources():
+    """The baroclinic turbulence case's mean equations 0 = -UShear + sourceNN
+    (order 0, no boundary condition): one oracle step leaves each UShear
+    system equal to its source's first NZ Chebyshev coefficients, the Bessel
+    series of exp(-(1 - z)/L) and its derivatives' scalings 1/L, 1/L^2."""
+    r = recipe("baroclinic_turbulence.coral.equations")
+    L = r.source_params["shearL"]
+    geo = O.Geometry(8, 8, 17, 2 * np.pi, 2 * np.pi)
+    s = O.OracleSolver(r, geo)
+    z = geo.gauss_chebyshev_grid()
+    p = S.SourceParams(r.source_params)
+    for t in r.sources:
+        s.set_source(t.index, S.evaluate(S.BAROCLINIC_TURBULENCE[t.index], z, p))
+    s.run(1e-4, 1)
+    a = 1.0 / (2 * L)
+    series = np.exp(-a) * 2 * ss.iv(np.arange(geo.NZ), a)
+    series[0] /= 2
+    for isys, scale in ((1, 1.0), (2, 1.0 / L), (3, 1.0 / L ** 2)):
+        np.testing.assert_allclose(s.zfield[isys], scale * series, rtol=1e-12, atol=1e-15)

@@ -80,7 +80,7 @@ EXPORTS = (
     "coral_amd_nn_axpy", "coral_amd_nn_mse_partials", "coral_amd_nn_mse", "coral_amd_nn_adam",
     "coral_amd_nn_gemm_work", "coral_amd_nn_transpose_bf16", "coral_amd_nn_mlp",
     "coral_amd_set_full_variable_filter", "coral_amd_add_source_term", "coral_amd_clear_source_terms",
-    "coral_amd_set_source", "coral_amd_nn_mode_scale_t", "coral_amd_nn_modes_supported",
+    "coral_amd_set_source", "coral_amd_set_fixed_physical", "coral_amd_set_smagorinsky", "coral_amd_nn_mode_scale_t", "coral_amd_nn_modes_supported",
     "coral_amd_nn_modes_work", "coral_amd_nn_modes", "coral_amd_nn_tile_transpose",
     "coral_amd_nn_gemm_bias",
 )
@@ -126,6 +126,8 @@ def _load():
     lib.coral_amd_add_source_term.argtypes = [P, c_int32, c_int32, c_int32, c_double, c_int32]
     lib.coral_amd_clear_source_terms.argtypes = [P]
     lib.coral_amd_set_source.argtypes = [P, c_int32, P]
+    lib.coral_amd_set_fixed_physical.argtypes = [P, c_int32, P]
+    lib.coral_amd_set_smagorinsky.argtypes = [P, c_int32, P, c_int32, c_double]
     lib.coral_amd_set_closure.argtypes = [P, ctypes.POINTER(ClosureDesc)]
     lib.coral_amd_get_launch_shape.argtypes = [P, P, P]
     lib.coral_amd_ensemble_create.argtypes = [P, c_int32, PP]

@@ -171,6 +171,24 @@ struct coral_amd_ctx {
     std::vector<SourceTerm> src_terms;
     std::map<int, std::vector<double>> src_cheb;
     int src_missing = 0;
+    // full variables whose physical values do not come from the spectral
+    // state: fixed fields (penalisation masks, coral_amd_set_fixed_physical;
+    // fixed_slot[ifull] >= 0 is the field's plane block in fixed_all) and the
+    // Smagorinsky viscosity (coral_amd_set_smagorinsky: the z-profile nu_prof
+    // from the squares of the nu_grads variables).  The quadratic variables
+    // with such a factor are formed in physical space (side_products).
+    std::vector<int> fixed_slot;
+    int nfixed = 0;
+    DevBuf<double> fixed_all;
+    int nu_full = -1;
+    std::vector<int> nu_grads;
+    double nu_prefactor = 1.0;
+    DevBuf<double> nu_prof, nu_tmp;
+    bool side_advance = true;   // a step keeps the new nuSmago profile; the products seam does not
+    std::vector<int> side_q, side_reg;   // quadratic variables formed aside; full variables they transform
+    DevBuf<int> side_desc;               // per side variable: kind and slot of both factors
+    DevBuf<double> side_phys, side_tau;
+    DevBuf<double2> side_Q;
     // learned subgrid closure (coral_amd_set_closure): the network, its
     // normalisation, inference workspace and the correction's buffers
     struct Closure {
@@ -732,6 +750,154 @@ void backward_to_G(coral_amd_ctx &c, const std::vector<const double2 *> &kst, co
 // variables to the 3/2 grid, products, forward transforms
 // (marching.f90:279-300), z-integral removal (PL_IMEX_timestepping.f90:223-235);
 // the CFL maxima of the full variables on the way.
+// ---- full variables with physical values of their own -----------------------
+// nuSmago's profile (compute_full_variables_in_physical_space,
+// PL_IMEX_timestepping_marching.f90:514-541): out[z] = pf sqrt(prev[z] + the
+// plane's mean of the summed squares of the gradient fields), prev its last
+// values.  One workgroup per plane; a fixed-order tree keeps it deterministic.
+__global__ __launch_bounds__(256) void k_smago_profile(const double *__restrict__ phys, int4 slots, int nslot,
+                                                       long long plane, int N2, double pf,
+                                                       const double *__restrict__ prev, double *__restrict__ out) {
+    __shared__ double red[256];
+    const int z = blockIdx.x;
+    const int sl[4] = {slots.x, slots.y, slots.z, slots.w};
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < N2; i += 256) {
+        double t = 0.0;
+        for (int k = 0; k < nslot; ++k) {
+            const double v = phys[sl[k] * plane + (long long)z * N2 + i];
+            t = fma(v, v, t);
+        }
+        acc += t;
+    }
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[z] = pf * sqrt(prev[z] + red[0] / double(N2));
+}
+
+// tau[k] = a * b over a plane block for each side quadratic variable k; a
+// factor is desc kind 0: a transformed full variable (its slot in reg), 1: a
+// fixed field (its slot in fixed), 2: the nuSmago profile, 3: ONE
+__device__ inline double side_factor(int kind, int slot, const double *reg, const double *fixed, const double *nu,
+                                     long long plane, long long i, int N2) {
+    if (kind == 0) return reg[slot * plane + i];
+    if (kind == 1) return fixed[slot * plane + i];
+    if (kind == 2) return nu[i / N2];
+    return 1.0;
+}
+__global__ __launch_bounds__(256) void k_side_products(const double *__restrict__ reg, const double *__restrict__ fixed,
+                                                       const double *__restrict__ nu, const int4 *__restrict__ desc,
+                                                       long long plane, int N2, double *__restrict__ tau) {
+    const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (i >= plane) return;
+    const int4 d = desc[blockIdx.y];
+    const double a = side_factor(d.x, d.y, reg, fixed, nu, plane, i, N2);
+    const double b = side_factor(d.z, d.w, reg, fixed, nu, plane, i, N2);
+    tau[blockIdx.y * plane + i] = d.z == 3 ? a : (d.x == 3 ? b : a * b);
+}
+
+__global__ void k_copy_c(double2 *__restrict__ y, const double2 *__restrict__ x, size_t n) {
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) y[i] = x[i];
+}
+
+__global__ void k_broadcast_profile(const double *__restrict__ prof, long long plane, int N2, double *__restrict__ out) {
+    const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (i < plane) out[i] = prof[i / N2];
+}
+
+int side_kind(const coral_amd_ctx &c, int f) {
+    if (f < 0) return 3;
+    if (c.fixed_slot.size() > size_t(f) && c.fixed_slot[f] >= 0) return 1;
+    if (f == c.nu_full) return 2;
+    return 0;
+}
+
+// The side path's tables and buffers for the current fixed fields and
+// Smagorinsky setting (changes the launches of a step: graphs are dropped)
+void rebuild_side(coral_amd_ctx &c) {
+    const Problem &P = c.P;
+    const long long plane = (long long)P.NZAA * P.NYAA * P.NXAA;
+    c.side_q.clear();
+    c.side_reg.clear();
+    auto reg_slot = [&](int f) {
+        for (size_t k = 0; k < c.side_reg.size(); ++k)
+            if (c.side_reg[k] == f) return int(k);
+        c.side_reg.push_back(f);
+        return int(c.side_reg.size()) - 1;
+    };
+    std::vector<int4> desc;
+    if (c.nu_full >= 0)
+        for (int f : c.nu_grads) reg_slot(f);
+    for (int q = 0; q < P.nquad; ++q) {
+        const int ka = side_kind(c, P.quad_a[q]), kb = side_kind(c, P.quad_b[q]);
+        if (ka != 1 && ka != 2 && kb != 1 && kb != 2) continue;
+        auto slot = [&](int kind, int f) { return kind == 0 ? reg_slot(f) : kind == 1 ? c.fixed_slot[f] : 0; };
+        const int sa = slot(ka, P.quad_a[q]), sb = slot(kb, P.quad_b[q]);
+        c.side_q.push_back(q);
+        desc.push_back(make_int4(ka, sa, kb, sb));
+    }
+    HIP_OK(hipDeviceSynchronize());
+    c.drop_graphs();
+    ++c.gen;
+    const size_t ns = c.side_q.size();
+    c.side_desc.upload(std::vector<int>(reinterpret_cast<const int *>(desc.data()),
+                                        reinterpret_cast<const int *>(desc.data()) + 4 * ns));
+    c.side_phys.alloc(c.side_reg.size() * plane);
+    c.side_tau.alloc(ns * plane);
+    c.side_Q.alloc(ns * size_t(P.nsys) * P.NZ);
+}
+
+// The quadratic variables with a fixed or Smagorinsky factor, formed in
+// physical space as the reference forms every product
+// (PL_IMEX_timestepping_marching.f90:279-300): x c2r of the full variables
+// they use (from G, as the products left it), nuSmago's profile, the
+// products, and the step's forward transforms; their spectra replace what
+// the fused x pass produced for them.
+void side_products(coral_amd_ctx &c) {
+    if (c.side_q.empty() && c.nu_full < 0) return;
+    const Problem &P = c.P;
+    const long long nlines = (long long)P.NZAA * P.NYAA, plane = nlines * P.NXAA;
+    const int N2 = P.NYAA * P.NXAA;
+    c.timed("side_products", [&] {
+        for (size_t k = 0; k < c.side_reg.size(); ++k)
+            launch_xbwd(c.stream, c.px, c.G.p + (size_t)c.side_reg[k] * nlines * P.nkx, c.side_phys.p + k * plane, 1,
+                        nlines, P.nkx);
+        const double *nu = c.nu_prof.p;
+        if (c.nu_full >= 0) {
+            int sl[4] = {0, 0, 0, 0};
+            for (size_t k = 0; k < c.nu_grads.size(); ++k) sl[k] = int(k);   // the gradients take the first slots
+            double *out = c.side_advance ? c.nu_prof.p : c.nu_tmp.p;
+            hipLaunchKernelGGL(k_smago_profile, dim3(unsigned(P.NZAA)), dim3(256), 0, c.stream, c.side_phys.p,
+                               make_int4(sl[0], sl[1], sl[2], sl[3]), int(c.nu_grads.size()), plane, N2,
+                               c.nu_prefactor, c.nu_prof.p, out);
+            check_launch("smagorinsky profile");
+            nu = out;
+        }
+        const int ns = int(c.side_q.size());
+        if (!ns) return;
+        hipLaunchKernelGGL(k_side_products, dim3(unsigned((plane + 255) / 256), unsigned(ns)), dim3(256), 0, c.stream,
+                           c.side_phys.p, c.fixed_all.p, nu, reinterpret_cast<const int4 *>(c.side_desc.p), plane,
+                           N2, c.side_tau.p);
+        check_launch("side products");
+        launch_xfwd(c.stream, c.px, c.side_tau.p, c.H.p, ns, nlines, P.nkx);
+        launch_yfwd(c.stream, c.py, c.H.p, c.Hy.p, ns, P.NZAA, P.nkx, P.nky, P.NY);
+        zfwd(c, c.Hy.p, c.side_Q.p, ns);
+        const size_t per = size_t(P.nsys) * P.NZ;
+        for (int k = 0; k < ns; ++k) {
+            const int q = c.side_q[k];
+            if (P.quad_filter[q]) launch_remove_zmean(c.stream, c.side_Q.p + k * per, P.NZ, P.nsys);
+            hipLaunchKernelGGL(k_copy_c, dim3(unsigned((per + 255) / 256)), dim3(256), 0, c.stream,
+                               c.Qhat.p + q * per, c.side_Q.p + k * per, per);
+            check_launch("side copy");
+        }
+    });
+}
+
 void products_to_Qhat(coral_amd_ctx &c, const std::vector<const double2 *> &kst,
                       const std::vector<const double2 *> &zst, bool fp32 = false) {
     const Problem &P = c.P;
@@ -766,6 +932,7 @@ void products_to_Qhat(coral_amd_ctx &c, const std::vector<const double2 *> &kst,
     c.timed("zfwd", [&] { zfwd(c, c.Hy.p, c.Qhat.p, P.nquad); });
     for (int q = 0; q < P.nquad; ++q)
         if (P.quad_filter[q]) launch_remove_zmean(c.stream, c.Qhat.p + size_t(q) * P.nsys * P.NZ, P.NZ, P.nsys);
+    side_products(c);
 }
 
 // ---- learned closure: the network's correction of the quadratic variables ----
@@ -1479,6 +1646,14 @@ int coral_amd_get_cfl_dt(coral_amd_ctx *c, double *out) {
         double w = std::max(v, m[2]);
         double mx = std::max(u, std::max(v, w));
         *out = mx > 0 ? 1.0 / mx : INFINITY;
+        if (c->nu_full >= 0) {
+            // the Smagorinsky bound (PL_IMEX_timestepping_marching.f90:241-253)
+            std::vector<double> prof(P.NZAA);
+            HIP_OK(hipMemcpy(prof.data(), c->nu_prof.p, prof.size() * sizeof(double), hipMemcpyDeviceToHost));
+            const double numax = *std::max_element(prof.begin(), prof.end());
+            const double kx = P.NXAA / P.Lx, ky = P.NYAA / P.Ly;
+            *out = std::min(*out, 0.95 / ((2 * 3.141592) * (2 * 3.141592) * numax * (kx * kx + ky * ky)));
+        }
         return 0;
     } catch (const std::exception &e) {
         return fail(e);
@@ -1562,10 +1737,20 @@ int coral_amd_get_physical(coral_amd_ctx *c, int32_t ifull, double *out) {
         std::vector<const double2 *> kst, zst;
         for (auto &d : c->ks) kst.push_back(d->field.p);
         for (auto &d : c->zs) zst.push_back(d->field.p);
-        backward_to_G(*c, kst, zst);
-        const long long nlines = (long long)P.NZAA * P.NYAA;
+        const long long nlines = (long long)P.NZAA * P.NYAA, plane = nlines * P.NXAA;
         ensure_phys(*c);
-        launch_xbwd(c->stream, c->px, c->G.p + (size_t)ifull * nlines * P.nkx, c->physd.p, 1, nlines, P.nkx);
+        const int kind = side_kind(*c, ifull);
+        if (kind == 1) {
+            HIP_OK(hipMemcpyAsync(c->physd.p, c->fixed_all.p + (size_t)c->fixed_slot[ifull] * plane,
+                                  plane * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+        } else if (kind == 2) {
+            hipLaunchKernelGGL(k_broadcast_profile, dim3(unsigned((plane + 255) / 256)), dim3(256), 0, c->stream,
+                               c->nu_prof.p, plane, P.NYAA * P.NXAA, c->physd.p);
+            check_launch("broadcast profile");
+        } else {
+            backward_to_G(*c, kst, zst);
+            launch_xbwd(c->stream, c->px, c->G.p + (size_t)ifull * nlines * P.nkx, c->physd.p, 1, nlines, P.nkx);
+        }
         download_phys(*c, out);
         return 0;
     } catch (const std::exception &e) {
@@ -1808,7 +1993,14 @@ int coral_amd_products(coral_amd_ctx *c, double *out) {
         for (auto &d : c->ks) kst.push_back(d->field.p);
         for (auto &d : c->zs) zst.push_back(d->field.p);
         const bool fp32 = c->precision == 1;
-        quad_spectra(*c, kst, zst);
+        c->side_advance = false;
+        try {
+            quad_spectra(*c, kst, zst);
+        } catch (...) {
+            c->side_advance = true;
+            throw;
+        }
+        c->side_advance = true;
         join_mean(*c);
         HIP_OK(hipStreamSynchronize(c->stream));
         std::vector<double2> h(size_t(P.nquad) * P.NZ * P.nsys);
@@ -2075,6 +2267,8 @@ int coral_amd_set_precision(coral_amd_ctx *c, int32_t mode, int32_t refine) {
         if (mode < 0 || mode > 1) throw std::runtime_error("precision mode must be 0 (FP64) or 1 (FP32 transforms)");
         if (refine < 0 || refine > 2) throw std::runtime_error("refine must be 0, 1 (last stage) or 2 (every stage)");
         if (mode == 1 && c->closure.on) throw std::runtime_error("the closure runs with the FP64 transform chain only");
+        if (mode == 1 && (c->nfixed > 0 || c->nu_full >= 0))
+            throw std::runtime_error("fixed physical fields and the Smagorinsky viscosity run with the FP64 transform chain only");
         HIP_OK(hipSetDevice(c->device));
         HIP_OK(hipStreamSynchronize(c->stream));
         HIP_OK(hipStreamSynchronize(c->side));
@@ -2431,6 +2625,83 @@ int coral_amd_set_source(coral_amd_ctx *c, int32_t source_index, const double *p
             c->src_cheb[source_index] = coef;
         }
         rebuild_sources(*c);
+        return 0;
+    } catch (const std::exception &e) {
+        return fail(e);
+    }
+}
+
+int coral_amd_set_fixed_physical(coral_amd_ctx *c, int32_t ifull, const double *phys) {
+    try {
+        if (!c) throw std::runtime_error("null context");
+        const Problem &P = c->P;
+        if (ifull < 0 || ifull >= P.nfull) throw std::runtime_error("full variable index out of range");
+        if (ifull < 3 && phys)
+            throw std::runtime_error("the first three full variables are the velocities of the CFL bound and cannot be fixed");
+        if (ifull == c->nu_full) throw std::runtime_error("the Smagorinsky variable cannot be fixed");
+        if (phys && c->precision == 1)
+            throw std::runtime_error("fixed physical fields run with the FP64 transform chain only");
+        HIP_OK(hipSetDevice(c->device));
+        HIP_OK(hipDeviceSynchronize());
+        const size_t plane = size_t(P.NZAA) * P.NYAA * P.NXAA;
+        if ((int)c->fixed_slot.size() < P.nfull) c->fixed_slot.resize(P.nfull, -1);
+        if (!phys) {
+            c->fixed_slot[ifull] = -1;   // its block stays allocated, unused
+        } else {
+            if (c->fixed_slot[ifull] < 0) {
+                // grow the block array by one plane block, keeping the others
+                DevBuf<double> grown;
+                grown.alloc((c->nfixed + 1) * plane);
+                if (c->nfixed)
+                    HIP_OK(hipMemcpy(grown.p, c->fixed_all.p, c->nfixed * plane * sizeof(double), hipMemcpyDeviceToDevice));
+                std::swap(grown.p, c->fixed_all.p);
+                std::swap(grown.n, c->fixed_all.n);
+                c->fixed_slot[ifull] = c->nfixed++;
+            }
+            // host [x][y][z] -> device [z][y][x]
+            std::vector<double> h(plane);
+            for (int z = 0; z < P.NZAA; ++z)
+                for (int y = 0; y < P.NYAA; ++y)
+                    for (int x = 0; x < P.NXAA; ++x)
+                        h[(size_t(z) * P.NYAA + y) * P.NXAA + x] = phys[(size_t(x) * P.NYAA + y) * P.NZAA + z];
+            HIP_OK(hipMemcpy(c->fixed_all.p + size_t(c->fixed_slot[ifull]) * plane, h.data(), plane * sizeof(double),
+                             hipMemcpyHostToDevice));
+        }
+        rebuild_side(*c);
+        return 0;
+    } catch (const std::exception &e) {
+        return fail(e);
+    }
+}
+
+int coral_amd_set_smagorinsky(coral_amd_ctx *c, int32_t ifull, const int32_t *grads, int32_t ngrads,
+                              double prefactor) {
+    try {
+        if (!c) throw std::runtime_error("null context");
+        const Problem &P = c->P;
+        if (ifull >= P.nfull) throw std::runtime_error("full variable index out of range");
+        if (ifull >= 0 && ifull != P.nfull - 1)
+            throw std::runtime_error("the Smagorinsky variable is the last full variable");
+        if (ngrads < 0 || ngrads > 4 || (ngrads && !grads)) throw std::runtime_error("0 to 4 gradient variables");
+        if (ifull >= 0 && c->precision == 1)
+            throw std::runtime_error("the Smagorinsky viscosity runs with the FP64 transform chain only");
+        std::vector<int> g;
+        for (int k = 0; k < ngrads; ++k) {
+            if (grads[k] < 0 || grads[k] >= P.nfull || grads[k] == ifull)
+                throw std::runtime_error("bad gradient variable index");
+            if (c->fixed_slot.size() > size_t(grads[k]) && c->fixed_slot[grads[k]] >= 0)
+                throw std::runtime_error("a gradient variable has fixed physical values");
+            g.push_back(grads[k]);
+        }
+        HIP_OK(hipSetDevice(c->device));
+        HIP_OK(hipDeviceSynchronize());
+        c->nu_full = ifull;
+        c->nu_grads = g;
+        c->nu_prefactor = prefactor;
+        // the profile starts at zero, and restarts at zero when set again
+        c->nu_prof.upload(std::vector<double>(P.NZAA, 0.0));
+        c->nu_tmp.upload(std::vector<double>(P.NZAA, 0.0));
+        rebuild_side(*c);
         return 0;
     } catch (const std::exception &e) {
         return fail(e);

@@ -21,11 +21,12 @@ grammar, the meaning of each token and the bookkeeping follow Coral's parser:
 * ``sourceParam`` values and ``add_rhs_source`` terms of the mean systems
   (``PL_equations.f90:985-996,1156-1230``); the sources themselves are
   profiles that the caller defines (``Solver.set_sources``), as the
-  reference's user include ``PL_user_sources_definitions.f90`` does.
-
-Features of the grammar that the time-stepper does not implement
-(penalisation) raise ``NotImplementedError`` naming the keyword, rather than
-being silently skipped.
+  reference's user include ``PL_user_sources_definitions.f90`` does;
+* penalisation variables (``enable_penalisation``, ``penalisation_strength``,
+  ``penalisation_width``, ``PL_equations.f90:509-517``), whose masks the
+  caller defines (``Solver.set_penalisation_mask``), as the user include
+  ``PL_user_penalisation_mask.f90`` does, and the Smagorinsky variable
+  ``nuSmago`` of a run with ``--withHorizontalSmagorinsky``.
 """
 from __future__ import annotations
 
@@ -119,6 +120,13 @@ class FullVariable:
     terms: List[FullTerm]
     extract: str = "nowhere"
     zdecomp: str = ""
+    # penalisation variable (enable_penalisation, PL_equations.f90:1035-1050):
+    # its physical values are fixed, strength (1 + tanh(width m)) / 2 of a mask m
+    penalisation: bool = False
+    penalisation_strength: float = 0.0
+    penalisation_width: float = 0.0
+    # nuSmago, the horizontal Smagorinsky viscosity (PL_equations.f90:519-525)
+    smagorinsky: bool = False
 
 
 @dataclasses.dataclass
@@ -145,6 +153,7 @@ class Recipe:
     # the reference's sourceParams_T%str (character(len=6), PL_equations.f90:47-50)
     source_params: Dict[str, float] = dataclasses.field(default_factory=dict)
     sources: List[SourceTerm] = dataclasses.field(default_factory=list)
+    smagorinsky: bool = False
 
     def var_location(self, name: str) -> Tuple[str, int, int]:
         for kind, systems in (("kxky", self.kxky_systems), ("zero", self.zero_systems)):
@@ -211,11 +220,6 @@ def _atom_tokens(atom: str, dz_token: str):
     return kx, ky, iz, mz, names
 
 
-_UNSUPPORTED = (
-    "enable_penalisation", "penalisation_strength", "penalisation_width",
-)
-
-
 def _source_term(atom: str, params: Dict[str, float], sgn: float, eqn: int, system: int) -> SourceTerm:
     """One ``add_rhs_source`` atom (``build_zero_sources_in_full_recipe``,
     ``PL_equations.f90:1156-1230``): ``Iz`` tokens count the exponent, an
@@ -243,8 +247,20 @@ def _source_term(atom: str, params: Dict[str, float], sgn: float, eqn: int, syst
     return SourceTerm(dsca, iz, index, eqn, system)
 
 
-def parse_equations(text: str) -> Recipe:
-    """Parse the text of a ``coral.equations`` file into a :class:`Recipe`."""
+def _one_float(payload: str) -> float:
+    """extract_one_float: the number after ``::``."""
+    fields = payload.split()
+    if not fields:
+        raise EquationsError("missing number")
+    return _fortran_float(fields[0])
+
+
+def parse_equations(text: str, smagorinsky: bool = False) -> Recipe:
+    """Parse the text of a ``coral.equations`` file into a :class:`Recipe`.
+    ``smagorinsky`` is the run's ``--withHorizontalSmagorinsky`` flag
+    (``plane_layer_main.f90:82``): a full variable ``nuSmago`` is then added
+    where the first ``quadratic_variable`` line starts the quadratic
+    variables (``PL_equations.f90:519-525``)."""
     params: Dict[str, float] = {}
     source_params: Dict[str, float] = {}
     sources: List[SourceTerm] = []
@@ -258,6 +274,7 @@ def parse_equations(text: str) -> Recipe:
     current_kind = ""
     full_build_pending: List[Tuple[int, str]] = []  # (full var index, expression)
     saw_eof = False
+    smagorinsky_pending = True
 
     for raw in text.splitlines():
         line = raw.rstrip("\n")
@@ -276,9 +293,6 @@ def parse_equations(text: str) -> Recipe:
             key, payload = body, ""
         key = key.strip()
         payload = payload.strip()
-        if any(key.startswith(u) for u in _UNSUPPORTED):
-            raise NotImplementedError(f"coral.equations keyword '{key}' is not supported by coral_amd")
-
         if key == "Param":
             name, val = payload.split(":=", 1)
             params[name.strip()] = _fortran_float(val)
@@ -315,7 +329,17 @@ def parse_equations(text: str) -> Recipe:
             if not full_vars:
                 raise EquationsError("linear_variable_full_build before linear_variable_full")
             full_build_pending.append((len(full_vars) - 1, payload))
+        elif key == "enable_penalisation":
+            full_vars.append(FullVariable(payload, [], penalisation=True))
+        elif key in ("penalisation_strength", "penalisation_width"):
+            if not full_vars or not full_vars[-1].penalisation:
+                raise EquationsError(f"{key} does not follow enable_penalisation")
+            setattr(full_vars[-1], key, _one_float(payload))
         elif key in ("quadratic_variable", "zFiltered_quadratic_variable"):
+            if key == "quadratic_variable" and smagorinsky_pending:
+                if smagorinsky:
+                    full_vars.append(FullVariable("nuSmago", [], smagorinsky=True))
+                smagorinsky_pending = False
             qv, rhs = payload.split(":=", 1)
             parts = rhs.strip().split(".")
             if len(parts) != 2:
@@ -372,7 +396,8 @@ def parse_equations(text: str) -> Recipe:
     if not saw_eof:
         raise EquationsError("missing >>EOF<< line")
 
-    recipe = Recipe(params, kxky_vars, zero_vars, full_vars, [], kxky_systems, zero_systems, source_params, sources)
+    recipe = Recipe(params, kxky_vars, zero_vars, full_vars, [], kxky_systems, zero_systems, source_params, sources,
+                    smagorinsky)
 
     # full variables are resolved once all coupled sets are known
     for ifull, expr in full_build_pending:
@@ -414,6 +439,6 @@ def parse_equations(text: str) -> Recipe:
     return recipe
 
 
-def read_equations(path: str) -> Recipe:
+def read_equations(path: str, smagorinsky: bool = False) -> Recipe:
     with open(path, "r") as f:
-        return parse_equations(f.read())
+        return parse_equations(f.read(), smagorinsky)

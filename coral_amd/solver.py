@@ -88,7 +88,8 @@ def assembled_operator(recipe: Recipe, NX: int, NY: int, NZ: int, Lx: float, Ly:
 
 class Solver:
     def __init__(self, recipe: Recipe, NX: int, NY: int, NZ: int, Lx: float, Ly: float,
-                 gap: float = 1.0, center: float = 0.5, scheme: str = "ARS_222", device: int = 0):
+                 gap: float = 1.0, center: float = 0.5, scheme: str = "ARS_222", device: int = 0,
+                 smagorinsky_prefactor: float = 1.0):
         self.recipe = recipe
         self.NX, self.NY, self.NZ = NX, NY, NZ
         self.NXAA, self.NYAA, self.NZAA = 3 * NX // 2, 3 * NY // 2, 3 * NZ // 2
@@ -106,6 +107,16 @@ class Solver:
             if ex or zd:
                 check(lib.coral_amd_set_full_variable_filter(self._ctx, i, ex, zd),
                       "coral_amd_set_full_variable_filter")
+        # nuSmago (a recipe read with smagorinsky=True): its profile from the
+        # squares of the full variables the reference names
+        # (PL_IMEX_timestepping_marching.f90:519-534); --smagorinsky-prefactor
+        nu = [i for i, f in enumerate(recipe.full_vars) if f.smagorinsky]
+        if nu:
+            grads = [i for i, f in enumerate(recipe.full_vars)
+                     if f.name in ("dxUfull", "dyUfull", "dxVfull", "dyVfull")]
+            g = _arr(ctypes.c_int32, grads)
+            check(lib.coral_amd_set_smagorinsky(self._ctx, nu[0], g, len(grads), float(smagorinsky_prefactor)),
+                  "coral_amd_set_smagorinsky")
         # the mean systems' source terms; the sources themselves: set_sources
         for t in recipe.sources:
             check(lib.coral_amd_add_source_term(self._ctx, t.system, t.eqn, t.iz, t.dsca, t.index),
@@ -314,6 +325,33 @@ class Solver:
                 continue
             vals = sources.evaluate(d, z, params)
             check(lib.coral_amd_set_source(self._ctx, int(nn), vals.ctypes.data), "coral_amd_set_source")
+
+    def set_penalisation_mask(self, definition=None, name: Optional[str] = None):
+        """Fix the physical values of a penalisation variable (the one named,
+        or every one of the recipe) to strength (1 + tanh(width max_i m_i)) / 2
+        of the masks m_i of ``definition`` (``coral_amd.penalisation``; the
+        shipped example masks by default), as ``init_penalisation`` does
+        (``PL_IMEX_timestepping_marching.f90:778-822``)."""
+        from . import penalisation
+        definition = definition or penalisation.SHIPPED
+        for i, f in enumerate(self.recipe.full_vars):
+            if not f.penalisation or (name is not None and f.name != name):
+                continue
+            field = penalisation.mask_field(definition, f.penalisation_strength, f.penalisation_width, self.NXAA,
+                                            self.NYAA, self.NZAA, self.Lx, self.Ly, self.center, self.gap)
+            self.set_fixed_physical(i, field)
+
+    def set_fixed_physical(self, ifull: int, values: Optional[np.ndarray]):
+        """Fixed physical values [x, y, z] of full variable ``ifull``
+        (``coral_amd_set_fixed_physical``); None returns it to its spectral
+        definition."""
+        if values is None:
+            check(lib.coral_amd_set_fixed_physical(self._ctx, ifull, None), "coral_amd_set_fixed_physical")
+            return
+        v = np.ascontiguousarray(values, np.float64)
+        if v.shape != (self.NXAA, self.NYAA, self.NZAA):
+            raise ValueError(f"fixed physical values have shape {(self.NXAA, self.NYAA, self.NZAA)}, got {v.shape}")
+        check(lib.coral_amd_set_fixed_physical(self._ctx, ifull, v.ctypes.data), "coral_amd_set_fixed_physical")
 
     def set_precision(self, mode: str = "fp64", refine: int = 0):
         """``"fp64"``: the FP64 step; ``"mixed"``: FP32 transforms and products

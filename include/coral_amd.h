@@ -325,6 +325,29 @@ int coral_amd_set_closure(coral_amd_ctx *ctx, const coral_amd_closure *closure);
  * (its vertical mean removed).  Set after coral_amd_create. */
 int coral_amd_set_full_variable_filter(coral_amd_ctx *ctx, int32_t ifull, int32_t extract, int32_t zdecomp);
 
+/* Fixed physical values of full variable ifull (a penalisation variable:
+ * init_penalisation, PL_IMEX_timestepping_marching.f90:778-822, and the
+ * penalisation guard of compute_full_variables_in_physical_space, :515):
+ * NXAA x NYAA x NZAA values in the layout of coral_amd_get_physical (z
+ * fastest, then y, then x), or NULL to return the variable to its spectral
+ * definition.  Quadratic variables with such a factor are formed in
+ * physical space, as the reference forms all of them (:279-300).  Not for the
+ * first three full variables (the CFL bound's velocities).  FP64 transform
+ * chain only. */
+int coral_amd_set_fixed_physical(coral_amd_ctx *ctx, int32_t ifull, const double *phys);
+
+/* Horizontal Smagorinsky viscosity (the --withHorizontalSmagorinsky run;
+ * PL_IMEX_timestepping_marching.f90:514-541, 241-253): full variable ifull
+ * (the last one, nuSmago; -1 switches it off) takes at every stage the
+ * z-profile prefactor * sqrt(previous profile + the horizontal mean of the
+ * summed squares of full variables grads[0..ngrads-1], ngrads <= 4: the
+ * reference's dxUfull, dyUfull, dxVfull, dyVfull); the profile starts at
+ * zero.  coral_amd_get_cfl_dt then also bounds dt by
+ * 0.95 / ((2 pi)^2 max nuSmago ((NXAA/Lx)^2 + (NYAA/Ly)^2)).  FP64 transform
+ * chain only. */
+int coral_amd_set_smagorinsky(coral_amd_ctx *ctx, int32_t ifull, const int32_t *grads, int32_t ngrads,
+                              double prefactor);
+
 /* Sources of the mean systems.  A term (an add_rhs_source atom, built by
  * build_zero_sources_in_full_recipe, PL_equations.f90:1156-1230) adds
  * dsca Iz^iz sourceNN to equation eqn of mean system isys (0-based), with

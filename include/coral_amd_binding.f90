@@ -103,6 +103,25 @@ module coral_amd_binding
        type(c_ptr), value :: ctx
        integer(c_int32_t), value :: ifull, extract, zdecomp
      end function
+     ! fixed physical values of full variable ifull (a penalisation mask),
+     ! NXAA x NYAA x NZAA in the layout of coral_amd_get_physical; c_null_ptr
+     ! returns it to its spectral definition
+     integer(c_int) function coral_amd_set_fixed_physical(ctx, ifull, phys) bind(C)
+       import
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: ifull
+       type(c_ptr), value :: phys
+     end function
+     ! horizontal Smagorinsky viscosity in full variable ifull (the last; -1
+     ! off) from the squares of full variables grads(1:ngrads)
+     integer(c_int) function coral_amd_set_smagorinsky(ctx, ifull, grads, ngrads, prefactor) bind(C)
+       import
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: ifull
+       integer(c_int32_t), intent(in) :: grads(*)
+       integer(c_int32_t), value :: ngrads
+       real(c_double), value :: prefactor
+     end function
      ! mean-system sources: a term dsca Iz^iz sourceNN of equation eqn of
      ! mean system isys (0-based); source NN from its NZAA Gauss-point values
      ! (top wall first; c_null_ptr removes it)

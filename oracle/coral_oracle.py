@@ -535,6 +535,13 @@ class OracleSolver:
         self.recipe, self.geo = recipe, geo
         self.A, self.Ah = imex_scheme(scheme)
         self.source_cheb: Dict[int, np.ndarray] = {}
+        # penalisation variables: fixed physical values (init_penalisation,
+        # marching.f90:778-822); nuSmago: its z-profile as last computed
+        # (it starts at zero) and prefactor (--smagorinsky-prefactor)
+        self.fixed_phys: Dict[int, np.ndarray] = {}
+        self.nu_index = next((i for i, f in enumerate(recipe.full_vars) if getattr(f, "smagorinsky", False)), None)
+        self.nu_prof = np.zeros(geo.NZAA)
+        self.smagorinsky_prefactor = 1.0
         self.s = self.A.shape[0]
         NZ = geo.NZ
         self.kops = [SystemOps(s, NZ, geo.center, geo.gap) for s in recipe.kxky_systems]
@@ -686,16 +693,54 @@ class OracleSolver:
         return 1.0 / m if m > 0 else math.inf
 
     # --- NL terms (marching.f90:257-418) -----------------------------------
-    def quad_spectra(self, aux, zaux):
+    def set_fixed_physical(self, ifull: int, phys: np.ndarray):
+        """Fixed physical values [x, y, z] of a full variable (a penalisation
+        variable's mask)."""
+        self.fixed_phys[ifull] = np.asarray(phys, np.float64).copy()
+
+    def _physical(self, specs, advance: bool):
+        """compute_full_variables_in_physical_space (marching.f90:420-588):
+        spec_to_phys of every full variable but the penalisation variables
+        (their fixed values) and nuSmago, whose values are
+        prefactor sqrt(<nuSmago + sum of the squares of dxUfull, dyUfull,
+        dxVfull, dyVfull>_xy) (:514-541) with nuSmago its previous values
+        (the reference's phys array is not reset between calls); ``advance``
+        keeps the new profile."""
+        g = self.geo
+        cdt = np.complex64 if self.transform_dtype == np.float32 else np.complex128
+        phys = []
+        for i, s in enumerate(specs):
+            if i in self.fixed_phys:
+                phys.append(self.fixed_phys[i])
+            elif i == self.nu_index:
+                phys.append(None)
+            else:
+                phys.append(spec_to_phys(s.astype(cdt), g))
+        if self.nu_index is not None:
+            arr = np.broadcast_to(self.nu_prof, (g.NXAA, g.NYAA, g.NZAA)).copy()
+            for j, fv in enumerate(self.recipe.full_vars):
+                if fv.name in ("dxUfull", "dyUfull", "dxVfull", "dyVfull"):
+                    arr = arr + phys[j] ** 2
+            avg = arr.sum(axis=(0, 1)) / g.NXAA / g.NYAA
+            prof = self.smagorinsky_prefactor * np.sqrt(avg)
+            if advance:
+                self.nu_prof = prof
+            phys[self.nu_index] = np.broadcast_to(prof, (g.NXAA, g.NYAA, g.NZAA)).copy()
+        return phys
+
+    def quad_spectra(self, aux, zaux, advance: bool = True):
         """Spectra of the quadratic variables (marching.f90:279-300 and the
         z-integral removal of PL_IMEX_timestepping.f90:223-235), complex
-        (nkx, NYAA, NZ) each; sets cfl_dt from the full variables."""
+        (nkx, NYAA, NZ) each; sets cfl_dt from the full variables, with the
+        Smagorinsky bound (marching.f90:241-253) when nuSmago is present."""
         g = self.geo
         specs = self.full_variables_spec(aux, zaux)
-        cdt = np.complex64 if self.transform_dtype == np.float32 else np.complex128
-        phys = [spec_to_phys(s.astype(cdt), g) for s in specs]
+        phys = self._physical(specs, advance)
         if len(phys) >= 3:
             self.cfl_dt = self.cfl(phys)
+            if self.nu_index is not None:
+                self.cfl_dt = min(self.cfl_dt, 0.95 / ((2 * 3.141592) ** 2 * float(np.max(phys[-1]))
+                                                       * ((g.NXAA / g.Lx) ** 2 + (g.NYAA / g.Ly) ** 2)))
         qhat = []
         for q in self.recipe.quad_vars:
             if q.ivar1 == -1:
@@ -812,5 +857,10 @@ class OracleSolver:
 
     # --- diagnostics ---------------------------------------------------------
     def physical_full_variables(self) -> List[np.ndarray]:
+        """The full variables of the current state [x, y, z]; nuSmago as last
+        computed by a step."""
         specs = self.full_variables_spec(self.field, self.zfield)
-        return [spec_to_phys(s, self.geo) for s in specs]
+        phys = self._physical(specs, advance=False)
+        if self.nu_index is not None:
+            phys[self.nu_index] = np.broadcast_to(self.nu_prof, phys[self.nu_index].shape).copy()
+        return phys

@@ -105,3 +105,37 @@ def full_filter_recipe(extract="TopSurf", zdecomp="baroclinic"):
     assert [f.name for f in r.full_vars][-2:] == ["tX", "uZ"]
     assert r.full_vars[-2].extract == extract and r.full_vars[-1].zdecomp == zdecomp
     return r
+
+
+def smagorinsky_recipe(prefactor_param="0.3d0"):
+    """The RBC recipe (coral_amd/inputs/coral.equations.rbc_ra1e5) with the
+    horizontal velocity gradients as full variables (dxUfull, dyUfull,
+    dxVfull, dyVfull, the names the reference's Smagorinsky code reads), read
+    with smagorinsky=True (nuSmago added), and an eddy-viscosity term
+    nu dx(nuSmago dxUfull) + nu dy(nuSmago dyUfull) in the u equation and the
+    like in the v equation, nu = 0.3."""
+    from coral_amd.equations import parse_equations
+    text = open(os.path.join(ROOT, "coral_amd", "inputs", "coral.equations.rbc_ra1e5")).read()
+    text = text.replace(">>Param :: RayleighPr := 1.d5 <<",
+                        f">>Param :: RayleighPr := 1.d5 <<\n>>Param :: nuT := {prefactor_param} <<")
+    grads = "".join(f">>linear_variable_full :: d{a}{b.upper()}full <<\n>>linear_variable_full_build :: + d{a}.{b} <<\n"
+                    for b in ("u", "v") for a in ("x", "y"))
+    text = text.replace(">>quadratic_variable :: uu := uFull.uFull <<",
+                        grads + ">>quadratic_variable :: uu := uFull.uFull <<")
+    text = text.replace(">>quadratic_variable :: tw := tFull.wFull <<",
+                        ">>quadratic_variable :: tw := tFull.wFull <<\n"
+                        ">>quadratic_variable :: nxu := nuSmago.dxUfull <<\n"
+                        ">>quadratic_variable :: nyu := nuSmago.dyUfull <<\n"
+                        ">>quadratic_variable :: nxv := dxVfull.nuSmago <<\n"
+                        ">>quadratic_variable :: nyv := nuSmago.dyVfull <<\n"
+                        ">>quadratic_variable :: nn := nuSmago.ONE <<")
+    text = text.replace(">>add_rhs_NL     :: - Iz.Iz.dx.uu - Iz.Iz.dy.uv - Iz.uw <<",
+                        ">>add_rhs_NL     :: - Iz.Iz.dx.uu - Iz.Iz.dy.uv - Iz.uw <<\n"
+                        ">>add_rhs_NL     :: + nuT.Iz.Iz.dx.nxu + nuT.Iz.Iz.dy.nyu <<", 1)
+    text = text.replace(">>add_rhs_NL     :: - Iz.Iz.dx.uv - Iz.Iz.dy.vv - Iz.vw <<",
+                        ">>add_rhs_NL     :: - Iz.Iz.dx.uv - Iz.Iz.dy.vv - Iz.vw <<\n"
+                        ">>add_rhs_NL     :: + nuT.Iz.Iz.dx.nxv + nuT.Iz.Iz.dy.nyv - nuT.Iz.Iz.nn <<", 1)
+    r = parse_equations(text, smagorinsky=True)
+    assert [f.name for f in r.full_vars][-5:] == ["dxUfull", "dyUfull", "dxVfull", "dyVfull", "nuSmago"]
+    assert sum(len(s.nl) for s in r.kxky_systems) > 0
+    return r

@@ -34,6 +34,7 @@ INPUTS = {
         "etc/benchmarks/coral.equations.stratifiedChannel.toroidalPoloidal",
     "baroclinic_turbulence.coral.equations": "etc/baroclinic_turbulence/coral.equations",
     "radiative_heating.coral.equations": "etc/coral_equations_examples/coral.equations.radiative_heating_potentials",
+    "rbc_penalised.coral.equations": "etc/coral_equations_examples/coral.equations.RBC_penalised",
 }
 
 

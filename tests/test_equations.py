@@ -66,8 +66,8 @@ def test_parse_errors():
                   ">>new_equation :: 0 <<\n>>add_rhs_linear :: + b.u <<\n>>EOF<<\n")
     with pytest.raises(EquationsError, match="unknown string"):
         parse_equations(bad)
-    with pytest.raises(NotImplementedError):
-        parse_equations(">>enable_penalisation <<\n>>EOF<<\n")
+    with pytest.raises(EquationsError, match="enable_penalisation"):
+        parse_equations(">>linear_variable_full :: f <<\n>>penalisation_strength :: 1.e3 <<\n>>EOF<<\n")
     mean_dx = (">>linear_variable_mean :: m <<\n>>add_set_of_coupled_zero_equations <<\n"
                ">>linearly_coupled_var :: m <<\n>>new_equation :: 0 <<\n>>add_rhs_linear :: + dx.m <<\n>>EOF<<\n")
     with pytest.raises(EquationsError, match="horizontal derivative"):

@@ -816,3 +816,90 @@ def test_mean_sources_closed_forms_on_gpu(solver_mod):
         _, z = g.get_state()
     for isys, scale in ((1, 1.0), (2, 1.0 / L), (3, 1.0 / L ** 2)):
         np.testing.assert_allclose(z[isys], scale * series, rtol=1e-12, atol=1e-15)
+
+
+# --- penalisation and the horizontal Smagorinsky viscosity ----------------------
+def test_penalisation_matches_oracle(solver_mod):
+    """The reference's penalised RBC recipe (coral.equations.RBC_penalised)
+    with the shipped masks (PL_user_penalisation_mask.f90): GAMMA's physical
+    values are the mask field as set, the products Gu, Gv, Gw (formed in
+    physical space with it) and three steps match the oracle."""
+    from coral_amd import penalisation as PN
+    r = recipe("rbc_penalised.coral.equations")
+    NX, NY, NZ = 16, 16, 17
+    L = 2 * np.pi
+    k0, z0 = seeded_state(r, NX, NY, NZ, 6, 6, 5, amps=(("u", 1e-2), ("v", 1e-2), ("w", 1e-2), ("t", 1e-2)))
+    geo = O.Geometry(NX, NY, NZ, L, L)
+    gvar = r.full_vars[4]
+    field = PN.mask_field(PN.SHIPPED, gvar.penalisation_strength, gvar.penalisation_width, geo.NXAA, geo.NYAA,
+                          geo.NZAA, L, L, geo.center, geo.gap)
+    assert 0 <= field.min() < field.max() <= 1e3 and 0 < np.median(field) < 1e3
+    ora = O.OracleSolver(r, geo, "ARS_222")
+    ora.field, ora.zfield = [a.copy() for a in k0], [a.copy() for a in z0]
+    ora.set_fixed_physical(4, field)
+    qref = ora.quad_spectra([a.copy() for a in k0], [a.copy() for a in z0])
+    with solver_mod.Solver(r, NX, NY, NZ, L, L) as g:
+        g.set_state(k0, z0)
+        g.set_penalisation_mask()
+        assert np.array_equal(g.physical(4), field)
+        q = g.products()
+        for it in range(3):
+            ora.step(1e-4)
+            g.march_forward(1e-4)
+        k, z = g.get_state()
+    wn = np.zeros(q.shape[2], bool)
+    wn[: NY // 2] = True
+    wn[q.shape[2] - (NY // 2 - 1):] = True
+    for i in range(len(r.quad_vars)):
+        assert rel(q[i][:, wn], qref[i][:, wn]) <= TOL, r.quad_vars[i].name
+    for a, b in zip(k, ora.field):
+        assert rel(a, b) <= TOL
+    for a, b in zip(z, ora.zfield):
+        assert rel(a, b) <= TOL
+
+
+def test_smagorinsky_matches_oracle(solver_mod):
+    """An RBC recipe with the horizontal velocity gradients as full variables
+    and eddy-viscosity terms built on nuSmago (read with the
+    --withHorizontalSmagorinsky flag, prefactor 0.7): the products seam leaves
+    the profile as it was; two eager steps and two steps replayed as HIP
+    graphs match the oracle, and so do nuSmago's physical values (the profile
+    of the last stage) and the CFL step with the Smagorinsky bound."""
+    from conftest import smagorinsky_recipe
+    r = smagorinsky_recipe()
+    NX, NY, NZ = 16, 16, 17
+    L = 2 * np.pi
+    nu = len(r.full_vars) - 1
+    k0, z0 = seeded_state(r, NX, NY, NZ, 6, 6, 5, amps=(("u", 1e-1), ("v", 1e-1), ("w", 1e-1), ("t", 1e-2)))
+    geo = O.Geometry(NX, NY, NZ, L, L)
+    ora = O.OracleSolver(r, geo, "ARS_222")
+    ora.smagorinsky_prefactor = 0.7
+    ora.field, ora.zfield = [a.copy() for a in k0], [a.copy() for a in z0]
+    qref = ora.quad_spectra([a.copy() for a in k0], [a.copy() for a in z0], advance=False)
+    with solver_mod.Solver(r, NX, NY, NZ, L, L, smagorinsky_prefactor=0.7) as g:
+        g.set_state(k0, z0)
+        q = g.products()
+        assert np.array_equal(g.products(), q)
+        assert not g.physical(nu).any()          # the profile starts at zero
+        for it in range(2):
+            ora.step(1e-4)
+            g.march_forward(1e-4)
+        g.use_graphs(True)
+        for it in range(2):
+            ora.step(1e-4)
+            g.march_forward(1e-4)
+        k, z = g.get_state()
+        pnu = g.physical(nu)
+        cdt = g.cfl_dt()
+    wn = np.zeros(q.shape[2], bool)
+    wn[: NY // 2] = True
+    wn[q.shape[2] - (NY // 2 - 1):] = True
+    for i in range(len(r.quad_vars)):
+        assert rel(q[i][:, wn], qref[i][:, wn]) <= TOL, r.quad_vars[i].name
+    for a, b in zip(k, ora.field):
+        assert rel(a, b) <= TOL
+    for a, b in zip(z, ora.zfield):
+        assert rel(a, b) <= TOL
+    assert ora.nu_prof.max() > 0
+    assert rel(pnu, np.broadcast_to(ora.nu_prof, pnu.shape)) <= TOL
+    assert cdt == pytest.approx(ora.cfl_dt, rel=1e-10)

@@ -247,6 +247,53 @@ def _source_term(atom: str, params: Dict[str, float], sgn: float, eqn: int, syst
     return SourceTerm(dsca, iz, index, eqn, system)
 
 
+# The keywords each keyword may be followed by (determine_kind_of_information,
+# PL_equations.f90:1758-2043): a file that breaks the order stops the
+# reference ("A different prefix was expected"), and raises here.  Keywords
+# inside a coupled set are told apart by the set's kind; "term" stands for
+# the add_d/dt_term / add_rhs_* lines of an equation.
+_START = ("Param", "sourceParam", "linear_variable_kxky")
+_QUADS = ("add_set_of_coupled_kxky_equations", "quadratic_variable", "zFiltered_quadratic_variable")
+_SET_VAR = ("linearly_coupled_var", "new_equation", "add_BC_for_this_var")
+_KXKY_TERMS = ("add_d/dt_term", "add_rhs_linear", "add_rhs_NL")
+_ZERO_TERMS = _KXKY_TERMS + ("add_rhs_source",)
+_GRAMMAR = {
+    "start": _START, "Param": _START, "sourceParam": _START,
+    "linear_variable_kxky": ("linear_variable_kxky", "linear_variable_mean"),
+    "linear_variable_mean": ("linear_variable_mean", "linear_variable_full"),
+    "linear_variable_full": ("linear_variable_full_build", "vertical_decomposition"),
+    "linear_variable_bottom_boundary_value": ("linear_variable_full_build",),
+    "linear_variable_top_boundary_value": ("linear_variable_full_build",),
+    "vertical_decomposition": ("linear_variable_full_build",),
+    "linear_variable_full_build": ("linear_variable_full_build", "linear_variable_full",
+                                   "linear_variable_bottom_boundary_value", "linear_variable_top_boundary_value",
+                                   "enable_penalisation", "quadratic_variable"),
+    "enable_penalisation": ("penalisation_strength",),
+    "penalisation_strength": ("penalisation_width",),
+    "penalisation_width": ("quadratic_variable",),
+    "quadratic_variable": _QUADS, "zFiltered_quadratic_variable": _QUADS,
+    "add_set_of_coupled_kxky_equations": ("linearly_coupled_var",),
+    "kxky:linearly_coupled_var": _SET_VAR,
+    "kxky:add_BC_for_this_var": ("linearly_coupled_var", "new_equation"),
+    "kxky:new_equation": _KXKY_TERMS,
+    "kxky:term": _KXKY_TERMS + ("new_equation", "add_set_of_coupled_kxky_equations",
+                                "add_set_of_coupled_zero_equations"),
+    "add_set_of_coupled_zero_equations": ("linearly_coupled_var",),
+    "zero:linearly_coupled_var": _SET_VAR,
+    "zero:add_BC_for_this_var": ("linearly_coupled_var", "new_equation"),
+    "zero:new_equation": _ZERO_TERMS,
+    "zero:term": _ZERO_TERMS + ("new_equation", "add_set_of_coupled_zero_equations", "EOF"),
+}
+
+
+def _grammar_state(key: str, kind: str) -> str:
+    if key in _ZERO_TERMS:
+        return f"{kind}:term"
+    if key in ("linearly_coupled_var", "add_BC_for_this_var", "new_equation"):
+        return f"{kind}:{key}"
+    return key
+
+
 def _one_float(payload: str) -> float:
     """extract_one_float: the number after ``::``."""
     fields = payload.split()
@@ -276,23 +323,29 @@ def parse_equations(text: str, smagorinsky: bool = False) -> Recipe:
     saw_eof = False
     smagorinsky_pending = True
 
-    for raw in text.splitlines():
-        line = raw.rstrip("\n")
+    state = "start"
+    for lineno, raw in enumerate(text.splitlines(), 1):
+        line = raw.rstrip("\r\n")
+        if line.startswith(">>="):
+            continue
         m = _LINE.match(line.strip())
         if not m:
-            continue
+            raise EquationsError(f"line {lineno}: {line!r} is not a '>>keyword :: ... <<' line")
         body = m.group(1)
-        if body.strip().startswith("====") or body.strip() == "":
-            continue
-        if body.strip() == "EOF":
-            saw_eof = True
-            break
         if "::" in body:
             key, payload = body.split("::", 1)
         else:
             key, payload = body, ""
         key = key.strip()
         payload = payload.strip()
+        if key not in _GRAMMAR[state]:
+            raise EquationsError(f"line {lineno}: {key!r} where one of {', '.join(_GRAMMAR[state])} was expected "
+                                 "(a different prefix was expected)")
+        if key == "EOF":
+            saw_eof = True
+            break
+        state = _grammar_state(key, "zero" if key == "add_set_of_coupled_zero_equations" else
+                               "kxky" if key == "add_set_of_coupled_kxky_equations" else current_kind)
         if key == "Param":
             name, val = payload.split(":=", 1)
             params[name.strip()] = _fortran_float(val)

@@ -58,20 +58,61 @@ def test_heat_equation_recipe():
     assert [(t.kind, t.system) for t in r.full_vars[2].terms] == [("kxky", 2), ("zero", 2)]
 
 
+def skeleton(kxky=">>add_d/dt_term :: + Iz.Iz.u <<\n>>add_rhs_linear :: + a.u <<\n",
+             zero=">>add_d/dt_term :: + Iz.Iz.m <<\n>>add_rhs_linear :: + a.m <<\n", head="", full=None):
+    """A minimal recipe in the reference's keyword order, with the given
+    equation lines for its kxky and mean systems."""
+    full = full if full is not None else ">>linear_variable_full :: f <<\n>>linear_variable_full_build :: + u <<\n"
+    return (">>Param :: a := 1.d0 <<\n" + head + ">>linear_variable_kxky :: u <<\n>>linear_variable_mean :: m <<\n"
+            + full + ">>quadratic_variable :: ff := f.f <<\n"
+            ">>add_set_of_coupled_kxky_equations <<\n>>linearly_coupled_var :: u <<\n>>add_BC_for_this_var :: 20 <<\n"
+            ">>new_equation :: 2 <<\n" + kxky +
+            ">>add_set_of_coupled_zero_equations <<\n>>linearly_coupled_var :: m <<\n>>add_BC_for_this_var :: 20 <<\n"
+            ">>new_equation :: 2 <<\n" + zero + ">>EOF<<\n")
+
+
 def test_parse_errors():
-    base = ">>Param :: a := 1.d0 <<\n>>linear_variable_kxky :: u <<\n"
+    r = parse_equations(skeleton())
+    assert r.kxky_systems[0].var_names == ["u"] and r.zero_systems[0].var_names == ["m"]
     with pytest.raises(EquationsError):
-        parse_equations(base)  # no EOF
-    bad = base + (">>add_set_of_coupled_kxky_equations <<\n>>linearly_coupled_var :: u <<\n"
-                  ">>new_equation :: 0 <<\n>>add_rhs_linear :: + b.u <<\n>>EOF<<\n")
+        parse_equations(skeleton()[: -len(">>EOF<<\n")])  # no EOF
     with pytest.raises(EquationsError, match="unknown string"):
-        parse_equations(bad)
-    with pytest.raises(EquationsError, match="enable_penalisation"):
-        parse_equations(">>linear_variable_full :: f <<\n>>penalisation_strength :: 1.e3 <<\n>>EOF<<\n")
-    mean_dx = (">>linear_variable_mean :: m <<\n>>add_set_of_coupled_zero_equations <<\n"
-               ">>linearly_coupled_var :: m <<\n>>new_equation :: 0 <<\n>>add_rhs_linear :: + dx.m <<\n>>EOF<<\n")
+        parse_equations(skeleton(kxky=">>add_d/dt_term :: + Iz.Iz.u <<\n>>add_rhs_linear :: + b.u <<\n"))
+    with pytest.raises(EquationsError):   # penalisation_strength only follows enable_penalisation
+        parse_equations(skeleton(full=">>linear_variable_full :: f <<\n>>linear_variable_full_build :: + u <<\n"
+                                      ">>penalisation_strength :: 1.e3 <<\n"))
     with pytest.raises(EquationsError, match="horizontal derivative"):
-        parse_equations(mean_dx)
+        parse_equations(skeleton(zero=">>add_d/dt_term :: + Iz.Iz.m <<\n>>add_rhs_linear :: + dx.m <<\n"))
+
+
+@pytest.mark.parametrize("change", [
+    ("blank line", lambda t: t.replace(">>linear_variable_mean", "\n>>linear_variable_mean", 1)),
+    ("no mean variable", lambda t: t.replace(">>linear_variable_mean :: m <<\n", "", 1)),
+    ("parameter after the variables", lambda t: t.replace(">>linear_variable_mean :: m <<\n",
+                                                          ">>linear_variable_mean :: m <<\n>>Param :: b := 2.d0 <<\n")),
+    ("vertical decomposition after a build line", lambda t: t.replace(
+        ">>quadratic_variable", ">>vertical_decomposition :: baroclinic <<\n>>quadratic_variable", 1)),
+    ("zFiltered first", lambda t: t.replace(">>quadratic_variable :: ff", ">>zFiltered_quadratic_variable :: ff", 1)),
+    ("kxky set after a mean set", lambda t: t.replace(
+        ">>EOF<<", ">>add_set_of_coupled_kxky_equations <<\n>>linearly_coupled_var :: u <<\n"
+                   ">>new_equation :: 0 <<\n>>add_rhs_linear :: + u <<\n>>EOF<<")),
+    ("new equation with no term", lambda t: t.replace(">>new_equation :: 2 <<\n>>add_d/dt_term :: + Iz.Iz.m",
+                                                      ">>new_equation :: 2 <<\n>>new_equation :: 2 <<\n"
+                                                      ">>add_d/dt_term :: + Iz.Iz.m")),
+    ("source in a kxky set", lambda t: t.replace(">>add_rhs_linear :: + a.u <<",
+                                                 ">>add_rhs_linear :: + a.u <<\n>>add_rhs_source :: + source01 <<")),
+    ("no closing marks", lambda t: t.replace(">>linear_variable_mean :: m <<", ">>linear_variable_mean :: m")),
+], ids=lambda c: c[0] if isinstance(c, tuple) else None)
+def test_keyword_order_of_the_reference_grammar(change):
+    """determine_kind_of_information (PL_equations.f90:1758-2043): each
+    keyword may only follow certain keywords; the reference stops on any other
+    line ('A different prefix was expected'), blank lines included, and only
+    reads up to an EOF line that closes a mean system's equation."""
+    parse_equations(skeleton())
+    with pytest.raises(EquationsError):
+        parse_equations(change[1](skeleton()))
+    # lines starting with '>>=' are skipped anywhere
+    parse_equations(skeleton().replace(">>linear_variable_mean", ">>=== a comment <<\n>>linear_variable_mean", 1))
 
 
 def test_parameters_file():

@@ -31,6 +31,7 @@ def _mean_recipe(source_line, extra=""):
     return parse_equations(
         ">>Param :: nu := 2.d0 <<\n>>Param :: half := 0.5d0 <<\n" + extra +
         ">>linear_variable_kxky :: a <<\n>>linear_variable_mean :: m <<\n"
+        ">>linear_variable_full :: f <<\n>>linear_variable_full_build :: + a <<\n>>quadratic_variable :: ff := f.f <<\n"
         ">>add_set_of_coupled_kxky_equations <<\n>>linearly_coupled_var :: a <<\n>>add_BC_for_this_var :: 20 <<\n"
         ">>new_equation :: 2 <<\n>>add_d/dt_term :: + Iz.Iz.a <<\n>>add_rhs_linear :: + nu.a <<\n"
         ">>add_set_of_coupled_zero_equations <<\n>>linearly_coupled_var :: m <<\n>>add_BC_for_this_var :: 20 <<\n"
@@ -51,8 +52,11 @@ def test_parser_source_atoms():
             _mean_recipe(bad)
     assert _mean_recipe("+ Iz.source01").sources[0].index == 1
     with pytest.raises(EquationsError):
+        # a source in the kxky set
         parse_equations(
-            ">>linear_variable_kxky :: a <<\n>>add_set_of_coupled_kxky_equations <<\n>>linearly_coupled_var :: a <<\n"
+            ">>linear_variable_kxky :: a <<\n>>linear_variable_mean :: m <<\n>>linear_variable_full :: f <<\n"
+            ">>linear_variable_full_build :: + a <<\n>>quadratic_variable :: ff := f.f <<\n"
+            ">>add_set_of_coupled_kxky_equations <<\n>>linearly_coupled_var :: a <<\n"
             ">>add_BC_for_this_var :: 20 <<\n>>new_equation :: 2 <<\n>>add_d/dt_term :: + Iz.Iz.a <<\n"
             ">>add_rhs_source :: + Iz.Iz.source01 <<\n>>EOF<<\n")
     # sourceParam names keep their first six characters (character(len=6))

@@ -193,11 +193,13 @@ struct coral_amd_ctx {
     // normalisation, inference workspace and the correction's buffers
     struct Closure {
         bool on = false;
+        bool fused_io = false;   // lift and projection fused with the packing (k_closure_lift / _project)
         nn::NetParams net;
         nn::NetWork ws;
         int chunk = 0;   // planes per network launch
         DevBuf<float> w, mu_in, sd_in, mu_out, sd_out, Y, H, Cm, gwork, mwork;
         DevBuf<unsigned short> w16, wt16, basisT16, basis16p, X16, H16, Cs16, H16T;
+        DevBuf<unsigned short> W0T;   // the lift's W0 transposed, W x kp (zero past cin), for k_closure_lift
         DevBuf<double> phys_full, phys_tau, zgrid;
         DevBuf<double2> Qtau;
     } closure;
@@ -1070,6 +1072,241 @@ __global__ __launch_bounds__(256) void k_closure_pack_lines(const double *__rest
     }
 }
 
+// ---- the network's lift and projection fused with the packing and unpacking ----
+typedef __bf16 cbf8 __attribute__((ext_vector_type(8)));
+typedef float cf4 __attribute__((ext_vector_type(4)));
+// FP32 -> BF16 with gfx950's conversion (as the network kernels' epilogues)
+__device__ inline unsigned short f2bf_hw(float f) { return __builtin_bit_cast(unsigned short, static_cast<__bf16>(f)); }
+
+// Rows per workgroup of the fused lift and projection: a tile lies inside one
+// x-line, so NXAA % LIFT_ROWS == 0 is required.
+constexpr int LIFT_ROWS = 64;
+
+struct LiftShape {
+    int kp = 0, ldk = 0;   // K rounded up to the MFMA depth; LDS row stride (bf16)
+    size_t bytes = 0;      // dynamic LDS
+};
+LiftShape lift_shape(int nfull, int cin, int W) {
+    LiftShape s;
+    s.kp = (cin + 31) / 32 * 32;
+    s.ldk = s.kp + 8;
+    // the input tile (bf16 [LIFT_ROWS][ldk]), later reused by the epilogue's
+    // FP32 tile [LIFT_ROWS][W + 4]
+    const size_t ops = sizeof(unsigned short) * size_t(LIFT_ROWS) * s.ldk;
+    const size_t epi = sizeof(float) * size_t(LIFT_ROWS) * (W + 4);
+    s.bytes = sizeof(double) * (size_t(9) * nfull * (LIFT_ROWS + 2) + 2 * size_t(s.kp)) + std::max(ops, epi);
+    return s;
+}
+
+// H = X W0 + b0 (FP32), H16 = bf16(H), H16T its tiled transposed copy (when
+// given), for LIFT_ROWS rows of the chunk at a time, X formed on chip: the
+// input rows of k_closure_pack_lines (same values and rounding) are built in
+// LDS from the 3 x 3 neighbouring lines of every full variable and multiplied
+// on MFMA with the K order, zero padding and epilogue of nn::gemm_bias, so H,
+// H16 and H16T equal the pack-then-GEMM sequence's bit for bit without the
+// input rows ever reaching HBM.  One wavefront per 16 rows, all W columns.
+template <int W>
+__global__ __launch_bounds__(256) void k_closure_lift(const double *__restrict__ phys,
+                                                      const double *__restrict__ zgrid, int z0, int NXAA, int NYAA,
+                                                      int NZAA, int nfull, int cin, int kp, int ldk,
+                                                      const float *__restrict__ mu, const float *__restrict__ sd,
+                                                      const unsigned short *__restrict__ W0T,
+                                                      const float *__restrict__ b0, float *__restrict__ H,
+                                                      unsigned short *__restrict__ H16,
+                                                      unsigned short *__restrict__ H16T) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    constexpr int LR = LIFT_ROWS, NT = W / 16, span = LR + 2;
+    const int N2 = NXAA * NYAA, ncore = 27 * nfull, ntile = 9 * nfull * span;
+    double *tile = reinterpret_cast<double *>(lds);   // [dz][dy][f][x0-1 .. x0+LR]
+    double *mv = tile + ntile, *rv = mv + kp;
+    unsigned short *As = reinterpret_cast<unsigned short *>(rv + kp);   // [LR][ldk]
+    const long long r0 = (long long)blockIdx.x * LR;
+    const int zl = int(r0 / N2), rem = int(r0 - (long long)zl * N2), y = rem / NXAA, x0 = rem - y * NXAA;
+    const int z = z0 + zl, tid = threadIdx.x;
+    const int lane = tid & 63, wv = tid >> 6, fr = lane & 15, fk = 8 * (lane >> 4), rq = 4 * (lane >> 4);
+    // the B fragments (W0^T rows, zero past cin) straight from memory, issued first
+    constexpr int KMAX = 8;   // kp <= 256
+    cbf8 bfr[KMAX][NT];
+#pragma unroll
+    for (int k = 0; k < KMAX; ++k)
+#pragma unroll
+        for (int j = 0; j < NT; ++j)
+            if (32 * k < kp) bfr[k][j] = *reinterpret_cast<const cbf8 *>(W0T + (size_t)(16 * j + fr) * kp + 32 * k + fk);
+    // the neighbouring lines: wavefront wv takes lines wv, wv + 4, ... (line
+    // L = o nfull + f, o = 3 (dz+1) + (dy+1)), a lane the points lane and 64 +
+    // lane of the line's span; up to four lines' loads in flight before their
+    // LDS stores
+    {
+        const int nlines = 9 * nfull;
+        int xa = x0 - 1 + lane, xb = x0 + 63 + lane;
+        xa = xa < 0 ? xa + NXAA : xa;
+        xb = xb >= NXAA ? xb - NXAA : xb;
+        const bool hasb = lane < span - 64;
+        for (int L0 = wv; L0 < nlines; L0 += 16) {
+            double va[4], vb[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int L = L0 + 4 * i;
+                if (L < nlines) {
+                    const int o = L / nfull, f = L - o * nfull;
+                    const int dz = o / 3 - 1, dy = o - 3 * (o / 3) - 1;
+                    const int zz = min(max(z + dz, 0), NZAA - 1);
+                    const int yy = y + dy < 0 ? NYAA - 1 : (y + dy == NYAA ? 0 : y + dy);
+                    const double *row = phys + ((size_t)f * NZAA + zz) * N2 + (size_t)yy * NXAA;
+                    va[i] = row[xa];
+                    vb[i] = hasb ? row[xb] : 0.0;
+                }
+            }
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int L = L0 + 4 * i;
+                if (L < nlines) {
+                    tile[L * span + lane] = va[i];
+                    if (hasb) tile[L * span + 64 + lane] = vb[i];
+                }
+            }
+        }
+    }
+    for (int c = tid; c < kp; c += 256) {
+        mv[c] = c < cin ? double(mu[c]) : 0.0;
+        rv[c] = c < cin ? 1.0 / double(sd[c]) : 0.0;
+    }
+    __syncthreads();
+    {
+        // each thread keeps one column (its neighbour offset and variable
+        // decoded once) over rows 256 / kp apart (kp <= 256)
+        const int cpt = 256 / kp, c = tid % kp, rs = tid / kp;
+        if (rs < cpt) {
+            const bool core = c < ncore;
+            const int o = core ? c / nfull : 13, f = core ? c - o * nfull : 0;
+            const int dz = o / 9 - 1, dy = (o / 3) % 3 - 1, dx = o % 3 - 1;
+            const double *lc = tile + (4 * nfull + f) * span + 1;
+            const double *ln = tile + ((3 * (dz + 1) + (dy + 1)) * nfull + f) * span + 1 + dx;
+            const double m = mv[c], ri = rv[c], hz = zgrid[z];
+            for (int rl = rs; rl < LR; rl += cpt) {
+                unsigned short h = 0;
+                if (c < cin) {
+                    double d = 0.0;
+                    if (core) {
+                        const double v0 = lc[rl];
+                        d = o == 13 ? v0 : ln[rl] - v0;
+                    } else if (c == ncore) {
+                        d = hz;
+                    }
+                    h = bf16_of(float((d - m) * ri));
+                }
+                As[rl * ldk + c] = h;
+            }
+        }
+    }
+    __syncthreads();
+    cf4 acc[NT];
+#pragma unroll
+    for (int j = 0; j < NT; ++j) acc[j] = cf4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int k = 0; k < KMAX; ++k) {
+        if (32 * k < kp) {
+            const cbf8 a = *reinterpret_cast<const cbf8 *>(&As[(16 * wv + fr) * ldk + 32 * k + fk]);
+#pragma unroll
+            for (int j = 0; j < NT; ++j) acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bfr[k][j], acc[j], 0, 0, 0);
+        }
+    }
+    // epilogue through LDS (the operand tiles are done with): the tile's H
+    // rows, their BF16 copy and its two 32-row blocks of the tiled transposed
+    // layout are each one contiguous run of memory, written with 16-byte stores
+    __syncthreads();
+    constexpr int LDH = W + 4;
+    float *Hs = reinterpret_cast<float *>(As);   // [LR][LDH]
+#pragma unroll
+    for (int j = 0; j < NT; ++j) {
+        const int gn = 16 * j + fr;
+        const float bias = b0[gn];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) Hs[(16 * wv + rq + q) * LDH + gn] = acc[j][q] + bias;
+    }
+    __syncthreads();
+    float4 *Ho = reinterpret_cast<float4 *>(H + r0 * W);
+    for (int e = tid; e < LR * W / 4; e += 256) {
+        const int rl = (4 * e) / W, n = 4 * e - rl * W;
+        const float *v = Hs + rl * LDH + n;
+        Ho[e] = make_float4(v[0], v[1], v[2], v[3]);
+    }
+    uint4 *H16o = reinterpret_cast<uint4 *>(H16 + r0 * W);
+    for (int e = tid; e < LR * W / 8; e += 256) {
+        const int rl = (8 * e) / W, n = 8 * e - rl * W;
+        const float *v = Hs + rl * LDH + n;
+        uint4 u;
+        u.x = unsigned(f2bf_hw(v[0])) | (unsigned(f2bf_hw(v[1])) << 16);
+        u.y = unsigned(f2bf_hw(v[2])) | (unsigned(f2bf_hw(v[3])) << 16);
+        u.z = unsigned(f2bf_hw(v[4])) | (unsigned(f2bf_hw(v[5])) << 16);
+        u.w = unsigned(f2bf_hw(v[6])) | (unsigned(f2bf_hw(v[7])) << 16);
+        H16o[e] = u;
+    }
+    if (H16T) {
+        // block t (32 rows) holds W columns of 32 consecutive rows
+        uint4 *Ht = reinterpret_cast<uint4 *>(H16T + r0 * W);
+        for (int e = tid; e < LR * W / 8; e += 256) {
+            const int o = 8 * e, t = o / (32 * W), n = (o - t * 32 * W) / 32, rr = o & 31;
+            const float *v = Hs + (32 * t + rr) * LDH + n;
+            uint4 u;
+            u.x = unsigned(f2bf_hw(v[0])) | (unsigned(f2bf_hw(v[LDH])) << 16);
+            u.y = unsigned(f2bf_hw(v[2 * LDH])) | (unsigned(f2bf_hw(v[3 * LDH])) << 16);
+            u.z = unsigned(f2bf_hw(v[4 * LDH])) | (unsigned(f2bf_hw(v[5 * LDH])) << 16);
+            u.w = unsigned(f2bf_hw(v[6 * LDH])) | (unsigned(f2bf_hw(v[7 * LDH])) << 16);
+            Ht[e] = u;
+        }
+    }
+}
+
+// tau[q][z][y][x] = (H16 Wp + bp) std + mean for LIFT_ROWS rows of the chunk:
+// the network's projection on MFMA with the K order and epilogue of nn::gemm
+// and nn::bias_act, the unpacking of k_closure_unpack, in one pass (the
+// outputs of a tile are staged in LDS so that each variable's 64 values are
+// one contiguous store).  cout <= 16.
+template <int W>
+__global__ __launch_bounds__(256) void k_closure_project(const unsigned short *__restrict__ H16,
+                                                         const unsigned short *__restrict__ Wp,
+                                                         const float *__restrict__ bp, int cout, int z0, int N2,
+                                                         int NZAA, const float *__restrict__ mu,
+                                                         const float *__restrict__ sd, double *__restrict__ tau) {
+    constexpr int LR = LIFT_ROWS, LDB = W + 8, KS = W / 32;
+    __shared__ __align__(16) unsigned short Bs[16 * LDB];
+    __shared__ double outs[16 * LR];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, fr = lane & 15, fk = 8 * (lane >> 4);
+    const int rq = 4 * (lane >> 4);
+    const long long r0 = (long long)blockIdx.x * LR;
+    cbf8 a[KS];
+#pragma unroll
+    for (int k = 0; k < KS; ++k)
+        a[k] = *reinterpret_cast<const cbf8 *>(H16 + (r0 + 16 * wv + fr) * W + 32 * k + fk);
+    for (int e = tid; e < 16 * W; e += 256) {
+        const int n = e / W, k = e - n * W;
+        Bs[n * LDB + k] = n < cout ? Wp[k * cout + n] : (unsigned short)0;
+    }
+    __syncthreads();
+    cf4 acc = cf4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int k = 0; k < KS; ++k) {
+        const cbf8 b = *reinterpret_cast<const cbf8 *>(&Bs[fr * LDB + 32 * k + fk]);
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[k], b, acc, 0, 0, 0);
+    }
+    if (fr < cout) {
+        const float bias = bp[fr];
+        const double s = double(sd[fr]), m = double(mu[fr]);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const float yv = acc[q] + bias;
+            outs[fr * LR + 16 * wv + rq + q] = double(yv) * s + m;
+        }
+    }
+    __syncthreads();
+    const int zl = int(r0 / N2), i0 = int(r0 - (long long)zl * N2);
+    for (int e = tid; e < cout * LR; e += 256) {
+        const int q = e / LR, rr = e - q * LR;
+        tau[((size_t)q * NZAA + z0 + zl) * N2 + i0 + rr] = outs[e];
+    }
+}
+
 // the network's output rows of planes z0 .. back to physical corrections
 // tau[q][z][y][x] = Y * std + mean
 __global__ void k_closure_unpack(const float *__restrict__ Y, int z0, int P, int N2, int NZAA, int nquad,
@@ -1101,6 +1338,33 @@ void closure_correction(coral_amd_ctx &c) {
         for (int z0 = 0; z0 < P.NZAA; z0 += cl.chunk) {
             const int np = std::min(cl.chunk, P.NZAA - z0);
             const unsigned g = unsigned(((long long)np * N2 + 255) / 256);
+            if (cl.fused_io) {
+                const LiftShape ls = lift_shape(P.nfull, cl.net.cin, cl.net.width);
+                const unsigned nt = unsigned((long long)np * N2 / LIFT_ROWS);
+                unsigned short *h16t = nn::net_tiled(cl.net, cl.ws) ? cl.ws.H16T : nullptr;
+                const unsigned short *W0 = cl.W0T.p;
+                const float *b0 = cl.net.w + cl.net.off_b0;
+                if (cl.net.width == 32)
+                    hipLaunchKernelGGL(k_closure_lift<32>, dim3(nt), dim3(256), ls.bytes, c.stream, cl.phys_full.p,
+                                       cl.zgrid.p, z0, P.NXAA, P.NYAA, P.NZAA, P.nfull, cl.net.cin, ls.kp, ls.ldk,
+                                       cl.mu_in.p, cl.sd_in.p, W0, b0, cl.ws.H, cl.ws.H16, h16t);
+                else
+                    hipLaunchKernelGGL(k_closure_lift<64>, dim3(nt), dim3(256), ls.bytes, c.stream, cl.phys_full.p,
+                                       cl.zgrid.p, z0, P.NXAA, P.NYAA, P.NZAA, P.nfull, cl.net.cin, ls.kp, ls.ldk,
+                                       cl.mu_in.p, cl.sd_in.p, W0, b0, cl.ws.H, cl.ws.H16, h16t);
+                check_launch("closure lift");
+                nn::net_forward(c.stream, cl.net, nullptr, np, nullptr, cl.ws);
+                const unsigned short *Wp = cl.net.w16 + cl.net.off_Wp;
+                const float *bp = cl.net.w + cl.net.off_bp;
+                if (cl.net.width == 32)
+                    hipLaunchKernelGGL(k_closure_project<32>, dim3(nt), dim3(256), 0, c.stream, cl.ws.H16, Wp, bp,
+                                       cl.net.cout, z0, N2, P.NZAA, cl.mu_out.p, cl.sd_out.p, cl.phys_tau.p);
+                else
+                    hipLaunchKernelGGL(k_closure_project<64>, dim3(nt), dim3(256), 0, c.stream, cl.ws.H16, Wp, bp,
+                                       cl.net.cout, z0, N2, P.NZAA, cl.mu_out.p, cl.sd_out.p, cl.phys_tau.p);
+                check_launch("closure projection");
+                continue;
+            }
             const size_t tile = size_t(9) * P.nfull * P.NXAA * sizeof(double);
             if (tile <= 65536) {
                 hipLaunchKernelGGL(k_closure_pack_lines, dim3(unsigned(np * P.NYAA)), dim3(256), tile, c.stream,
@@ -2515,6 +2779,31 @@ int coral_amd_set_closure(coral_amd_ctx *c, const coral_amd_closure *cl) {
             C.ws.H16T = C.H16T.p;
             C.ws.mwork = C.mwork.p;
             C.ws.nmwork = nn::modes_work(nbp, N2, n.width, C.chunk);
+        }
+        // the fused lift / projection: tiles inside x-lines, at most 16 outputs,
+        // their dynamic LDS within the device limit (CORAL_AMD_CLOSURE_UNFUSED=1
+        // keeps the pack, GEMM and unpack kernels)
+        {
+            const char *ev = std::getenv("CORAL_AMD_CLOSURE_UNFUSED");
+            const LiftShape ls = lift_shape(P.nfull, n.cin, n.width);
+            C.fused_io = !(ev && ev[0] == '1') && P.NXAA % LIFT_ROWS == 0 && n.cout <= 16 &&
+                         (n.width == 32 || n.width == 64) && ls.kp <= 256 && ls.bytes <= 65536;
+            if (C.fused_io && ls.bytes > 49152) {
+                if (n.width == 32)
+                    HIP_OK(hipFuncSetAttribute(reinterpret_cast<const void *>(k_closure_lift<32>),
+                                               hipFuncAttributeMaxDynamicSharedMemorySize, int(ls.bytes)));
+                else
+                    HIP_OK(hipFuncSetAttribute(reinterpret_cast<const void *>(k_closure_lift<64>),
+                                               hipFuncAttributeMaxDynamicSharedMemorySize, int(ls.bytes)));
+            }
+            if (C.fused_io) {   // no packed input rows or FP32 outputs to keep
+                C.X16.alloc(0);
+                C.Y.alloc(0);
+                std::vector<unsigned short> w0t(size_t(n.width) * ls.kp, 0);
+                for (int k = 0; k < n.cin; ++k)
+                    for (int j = 0; j < n.width; ++j) w0t[size_t(j) * ls.kp + k] = w16[n.off_W0 + size_t(k) * n.width + j];
+                C.W0T.upload(w0t);
+            }
         }
         C.phys_full.alloc(size_t(P.nfull) * P.NZAA * N2);
         C.phys_tau.alloc(size_t(P.nquad) * P.NZAA * N2);

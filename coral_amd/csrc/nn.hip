@@ -4,7 +4,9 @@
 // (unsigned short); FP32 -> BF16 rounds to nearest even.
 #include "nn.hpp"
 
+#include <algorithm>
 #include <cstdint>
+#include <cstdlib>
 #include <cstring>
 #include <stdexcept>
 #include <string>
@@ -500,6 +502,194 @@ __global__ __launch_bounds__(256) void k_mlp(long long R, const unsigned short *
     }
 }
 
+// The inference form of k_mlp as a persistent, software-pipelined kernel:
+// each workgroup walks row tiles blockIdx.x, + gridDim.x, ... and issues the
+// global reads of its next tile (input fragments, old H, basis and mode
+// fragments) before computing the current one, so that loads stay in flight
+// while the MFMAs, the GELU and the stores run; the weight fragments and
+// biases are read once per workgroup.  Two register sets alternate (no copies
+// between iterations, which would wait for the loads).  The new H rows go
+// through LDS and leave with 16-byte stores.  Arithmetic, order and outputs
+// are k_mlp<W, HD, true>'s; requires B16 (the spectral term),
+// R % MLP_ROWS == 0, N2 % MLP_ROWS == 0 and nbp == 32 NBK.
+template <int W, int HD, int NBK>
+struct MlpTile {
+    static constexpr int RB = MLP_ROWS / 16, KS1 = W / 32, CB2 = W / 32;
+    bf8 a[RB][KS1];
+    float hold[CB2][4];
+    bf8 bas[NBK];
+    bf8 cst[NBK][CB2];
+};
+
+template <int W, int HD, int NBK>
+__device__ inline void mlp_tile_load(MlpTile<W, HD, NBK> &T, long long t, const unsigned short *Hin16,
+                                     const float *H, const unsigned short *B16, const unsigned short *CsT16, int N2,
+                                     int fr, int fk, int rq, int sr, int sc) {
+    using TT = MlpTile<W, HD, NBK>;
+    const long long r0 = t * MLP_ROWS;
+    const unsigned short *Hin_b = Hin16 + r0 * W;
+#pragma unroll
+    for (int rb = 0; rb < TT::RB; ++rb)
+#pragma unroll
+        for (int k = 0; k < TT::KS1; ++k)
+            T.a[rb][k] = *reinterpret_cast<const bf8 *>(Hin_b + (16 * rb + fr) * W + 32 * k + fk);
+#pragma unroll
+    for (int cb = 0; cb < TT::CB2; ++cb)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) T.hold[cb][q] = H[(r0 + sr + rq + q) * W + sc + 16 * cb + fr];
+    const long long p = r0 / N2;
+    const int i = int(r0 - p * N2) + sr + fr;
+    const unsigned short *brow = B16 + (size_t)i * (32 * NBK) + fk;
+    const unsigned short *cst = CsT16 + (size_t)p * W * (32 * NBK) + fk;
+#pragma unroll
+    for (int k = 0; k < NBK; ++k) {
+        T.bas[k] = *reinterpret_cast<const bf8 *>(brow + 32 * k);
+#pragma unroll
+        for (int cb = 0; cb < TT::CB2; ++cb)
+            T.cst[k][cb] = *reinterpret_cast<const bf8 *>(cst + (size_t)(sc + 16 * cb + fr) * (32 * NBK) + 32 * k);
+    }
+}
+
+template <int W, int HD, int NBK>
+__device__ inline void mlp_tile_compute(const MlpTile<W, HD, NBK> &T, long long t, unsigned short *As,
+                                        const bf8 (&w1)[W / 32][HD / 64], const bf8 (&w2)[HD / 32][W / 32],
+                                        const float (&bias1)[HD / 64], const float (&bias2)[W / 32], float *H,
+                                        unsigned short *Hout16, unsigned short *H16T, int fr, int fk, int rq,
+                                        int sr, int sc, int c0) {
+    constexpr int RB = MLP_ROWS / 16, LDA = HD + 8, KS1 = W / 32, CB1 = HD / 64, KS2 = HD / 32, CB2 = W / 32;
+    const long long r0 = t * MLP_ROWS;
+    f4 sacc[CB2];
+#pragma unroll
+    for (int cb = 0; cb < CB2; ++cb) sacc[cb] = f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int k = 0; k < NBK; ++k)
+#pragma unroll
+        for (int cb = 0; cb < CB2; ++cb)
+            sacc[cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(T.bas[k], T.cst[k][cb], sacc[cb], 0, 0, 0);
+    f4 acc[RB][CB1];
+#pragma unroll
+    for (int rb = 0; rb < RB; ++rb)
+#pragma unroll
+        for (int cb = 0; cb < CB1; ++cb) acc[rb][cb] = f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int k = 0; k < KS1; ++k)
+#pragma unroll
+        for (int cb = 0; cb < CB1; ++cb)
+#pragma unroll
+            for (int rb = 0; rb < RB; ++rb)
+                acc[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(T.a[rb][k], w1[k][cb], acc[rb][cb], 0, 0, 0);
+#pragma unroll
+    for (int cb = 0; cb < CB1; ++cb) {
+        const int col = c0 + 16 * cb + fr;
+#pragma unroll
+        for (int rb = 0; rb < RB; ++rb)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) As[(16 * rb + rq + q) * LDA + col] = f2bf(gelu(acc[rb][cb][q] + bias1[cb]));
+    }
+    __syncthreads();
+    f4 u[CB2];
+#pragma unroll
+    for (int cb = 0; cb < CB2; ++cb) u[cb] = f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int k = 0; k < KS2; ++k) {
+        const bf8 av = *reinterpret_cast<const bf8 *>(&As[(sr + fr) * LDA + 32 * k + fk]);
+#pragma unroll
+        for (int cb = 0; cb < CB2; ++cb) u[cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, w2[k][cb], u[cb], 0, 0, 0);
+    }
+    // the activation tile is free once every wavefront has read it: the new
+    // H values go through it, so that the tile's H and H16 rows (each one
+    // contiguous run) leave with 16-byte stores
+    __syncthreads();
+    constexpr int LDH = W + 4;
+    float *Hs = reinterpret_cast<float *>(As);   // [MLP_ROWS][LDH], within the activation tile
+    static_assert(MLP_ROWS * LDH * 4 <= MLP_ROWS * LDA * 2, "H staging fits the activation tile");
+#pragma unroll
+    for (int cb = 0; cb < CB2; ++cb) {
+        const int col = sc + 16 * cb + fr;
+        unsigned short ht[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const float h = (sacc[cb][q] + T.hold[cb][q]) + (u[cb][q] + bias2[cb]);
+            ht[q] = f2bf(h);
+            Hs[(sr + rq + q) * LDH + col] = h;
+        }
+        if (H16T) {
+            uint2 v;
+            v.x = unsigned(ht[0]) | (unsigned(ht[1]) << 16);
+            v.y = unsigned(ht[2]) | (unsigned(ht[3]) << 16);
+            *reinterpret_cast<uint2 *>(H16T + ((size_t)t * W + col) * MLP_ROWS + sr + rq) = v;
+        }
+    }
+    __syncthreads();
+    float4 *Ho = reinterpret_cast<float4 *>(H + r0 * W);
+    for (int e = threadIdx.x; e < MLP_ROWS * W / 4; e += 256) {
+        const int rl = (4 * e) / W, n = 4 * e - rl * W;
+        const float *v = Hs + rl * LDH + n;
+        Ho[e] = make_float4(v[0], v[1], v[2], v[3]);
+    }
+    uint4 *H16o = reinterpret_cast<uint4 *>(Hout16 + r0 * W);
+    for (int e = threadIdx.x; e < MLP_ROWS * W / 8; e += 256) {
+        const int rl = (8 * e) / W, n = 8 * e - rl * W;
+        const float *v = Hs + rl * LDH + n;
+        uint4 o;
+        o.x = unsigned(f2bf(v[0])) | (unsigned(f2bf(v[1])) << 16);
+        o.y = unsigned(f2bf(v[2])) | (unsigned(f2bf(v[3])) << 16);
+        o.z = unsigned(f2bf(v[4])) | (unsigned(f2bf(v[5])) << 16);
+        o.w = unsigned(f2bf(v[6])) | (unsigned(f2bf(v[7])) << 16);
+        H16o[e] = o;
+    }
+    // the staging area is the next tile's activation tile
+    __syncthreads();
+}
+
+template <int W, int HD, int NBK>
+__global__ __launch_bounds__(256) void k_mlp_pipe(long long ntiles, const unsigned short *Hin16,
+                                                  const unsigned short *__restrict__ W1t,
+                                                  const float *__restrict__ b1,
+                                                  const unsigned short *__restrict__ W2t,
+                                                  const float *__restrict__ b2, float *H, unsigned short *Hout16,
+                                                  const unsigned short *__restrict__ B16,
+                                                  const unsigned short *__restrict__ CsT16, int N2,
+                                                  unsigned short *__restrict__ H16T) {
+    static_assert(W % 32 == 0 && HD % 64 == 0 && NBK >= 1, "MLP shape");
+    constexpr int LDA = HD + 8, KS1 = W / 32, CB1 = HD / 64, KS2 = HD / 32, CB2 = W / 32;
+    __shared__ __align__(16) unsigned short As[MLP_ROWS * LDA];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int fr = lane & 15, fk = 8 * (lane >> 4), rq = 4 * (lane >> 4);
+    const int sr = 16 * (wv >> 1), sc = (W / 2) * (wv & 1), c0 = wv * (HD / 4);
+    const long long G = gridDim.x;
+    long long t = blockIdx.x;
+    if (t >= ntiles) return;
+    MlpTile<W, HD, NBK> A, B;
+    mlp_tile_load<W, HD, NBK>(A, t, Hin16, H, B16, CsT16, N2, fr, fk, rq, sr, sc);
+    bf8 w1[KS1][CB1], w2[KS2][CB2];
+    float bias1[CB1], bias2[CB2];
+#pragma unroll
+    for (int k = 0; k < KS1; ++k)
+#pragma unroll
+        for (int cb = 0; cb < CB1; ++cb)
+            w1[k][cb] = *reinterpret_cast<const bf8 *>(W1t + (size_t)(c0 + 16 * cb + fr) * W + 32 * k + fk);
+#pragma unroll
+    for (int k = 0; k < KS2; ++k)
+#pragma unroll
+        for (int cb = 0; cb < CB2; ++cb)
+            w2[k][cb] = *reinterpret_cast<const bf8 *>(W2t + (size_t)(sc + 16 * cb + fr) * HD + 32 * k + fk);
+#pragma unroll
+    for (int cb = 0; cb < CB1; ++cb) bias1[cb] = b1[c0 + 16 * cb + fr];
+#pragma unroll
+    for (int cb = 0; cb < CB2; ++cb) bias2[cb] = b2[sc + 16 * cb + fr];
+    for (;;) {
+        if (t + G < ntiles) mlp_tile_load<W, HD, NBK>(B, t + G, Hin16, H, B16, CsT16, N2, fr, fk, rq, sr, sc);
+        mlp_tile_compute<W, HD, NBK>(A, t, As, w1, w2, bias1, bias2, H, Hout16, H16T, fr, fk, rq, sr, sc, c0);
+        t += G;
+        if (t >= ntiles) break;
+        if (t + G < ntiles) mlp_tile_load<W, HD, NBK>(A, t + G, Hin16, H, B16, CsT16, N2, fr, fk, rq, sr, sc);
+        mlp_tile_compute<W, HD, NBK>(B, t, As, w1, w2, bias1, bias2, H, Hout16, H16T, fr, fk, rq, sr, sc, c0);
+        t += G;
+        if (t >= ntiles) break;
+    }
+}
+
 // Tiled transposed layout of an R x W BF16 array (R % 32 == 0): row r, column
 // c at ((r / 32) W + c) 32 + r % 32, i.e. each 32-row tile stored as its W
 // columns of 32 consecutive rows (2 KB at W = 32): contiguous along the rows,
@@ -646,6 +836,33 @@ void mlp(hipStream_t st, long long R, int W, int HD, const unsigned short *Hin16
     if (nblk > 2147483647LL) throw std::runtime_error("mlp: too many rows");
     const dim3 g{unsigned(nblk)}, b{256};
     const bool fast = !Zc && !A16c && R % MLP_ROWS == 0;
+    // the pipelined inference kernel: spectral term present, up to four 32-mode blocks
+    const bool pipe = fast && B16 && nbp <= 128 && !std::getenv("CORAL_AMD_MLP_UNPIPELINED");
+    if (pipe) {
+        int dev = 0, ncu = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || ncu <= 0)
+            throw std::runtime_error("mlp: cannot read the device's compute-unit count");
+        // as many workgroups as are resident at once: a persistent workgroup
+        // that had to wait for a slot would start its share late
+#define MLP_PIPE(w_, hd_, nbk_)                                                                                   \
+        if (W == w_ && HD == hd_ && nbp == 32 * nbk_) {                                                          \
+            int per = 0;                                                                                         \
+            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(                                                    \
+                    &per, reinterpret_cast<const void *>(k_mlp_pipe<w_, hd_, nbk_>), 256, 0) != hipSuccess ||   \
+                per < 1)                                                                                         \
+                per = 1;                                                                                         \
+            const unsigned gp = unsigned(std::min<long long>(nblk, (long long)per * ncu));                      \
+            hipLaunchKernelGGL((k_mlp_pipe<w_, hd_, nbk_>), dim3(gp), b, 0, st, nblk, Hin16, W1t, b1, W2t, b2, H, \
+                               Hout16, B16, CsT16, N2, H16T);                                                    \
+            check_launch("pipelined mlp");                                                                       \
+            return;                                                                                              \
+        }
+        // (width 64 needs more registers than leave two wavefronts per SIMD: it
+        // keeps the one-tile kernel)
+        MLP_PIPE(32, 128, 1) MLP_PIPE(32, 128, 2) MLP_PIPE(32, 128, 3) MLP_PIPE(32, 128, 4)
+#undef MLP_PIPE
+    }
 #define MLP_CASE(w_, hd_)                                                                                       \
     if (W == w_ && HD == hd_) {                                                                                 \
         if (fast)                                                                                               \
@@ -847,16 +1064,23 @@ void net_layout(NetParams &n) {
     n.size = o;
 }
 
+bool net_tiled(const NetParams &n, const NetWork &w) {
+    return n.basis16p && n.N2 % MLP_ROWS == 0 && w.H16T && modes_supported(n.nbp, n.N2, n.width);
+}
+
 void net_forward(hipStream_t st, const NetParams &n, const unsigned short *X16, int P, float *Y, const NetWork &w) {
     const int W = n.width, Hd = n.hidden, nb = n.nb, N2 = n.N2;
     const long long R = (long long)P * N2;
     if (R > 2147483647LL) throw std::runtime_error("closure network: too many rows");
     const int Ri = int(R);
     const bool fused = n.basis16p && N2 % MLP_ROWS == 0;
-    const bool tmodes = fused && w.H16T && modes_supported(n.nbp, N2, W);
-    // lift with its bias, BF16 copy and (for the mode kernel) tiled transposed copy
-    gemm_bias(st, Ri, W, n.cin, X16, n.cin, n.w16 + n.off_W0, W, n.w + n.off_b0, w.H, w.H16,
-              tmodes ? w.H16T : nullptr);
+    const bool tmodes = net_tiled(n, w);
+    // lift with its bias, BF16 copy and (for the mode kernel) tiled transposed
+    // copy; without X16 the caller has filled H, H16 (and H16T when
+    // net_tiled) itself
+    if (X16)
+        gemm_bias(st, Ri, W, n.cin, X16, n.cin, n.w16 + n.off_W0, W, n.w + n.off_b0, w.H, w.H16,
+                  tmodes ? w.H16T : nullptr);
     for (int l = 0; l < n.layers; ++l) {
         // spectral branch: Cm = B^T H16 per plane (from H16 transposed when the
         // mode kernel takes the shapes), Cs = Cm D, H += B Cs (inside the MLP
@@ -879,6 +1103,7 @@ void net_forward(hipStream_t st, const NetParams &n, const unsigned short *X16, 
                 nullptr, nullptr, nullptr, nullptr, 0, 0, nullptr);
         }
     }
+    if (!Y) return;   // the caller projects H16 itself
     gemm(st, 0, 0, Ri, n.cout, W, 1.f, w.H16, W, 0, n.w16 + n.off_Wp, n.cout, 0, 0.f, Y, n.cout, 0, 1);
     bias_act(st, Y, n.w + n.off_bp, n.cout, R * n.cout, 0, nullptr, nullptr);
 }

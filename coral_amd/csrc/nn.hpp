@@ -99,8 +99,13 @@ struct NetWork {
     unsigned short *H16 = nullptr, *Cs16 = nullptr, *H16T = nullptr;   // H16T: W x Pmax N2
     long long ngwork = 0, nmwork = 0;
 };
-// Y (P N2 x cout, FP32) = network(X16 (P N2 x cin, BF16)), P <= the workspace's planes
+// Y (P N2 x cout, FP32) = network(X16 (P N2 x cin, BF16)), P <= the workspace's planes.
+// X16 null: the lift's outputs (H, H16, and H16T when net_tiled) are already
+// in the workspace; Y null: the last layer's H16 is left for the caller to
+// project (Wp, bp).
 void net_forward(hipStream_t st, const NetParams &n, const unsigned short *X16, int P, float *Y, const NetWork &w);
+// whether net_forward keeps the tiled transposed copy H16T of each layer's input
+bool net_tiled(const NetParams &n, const NetWork &w);
 // host rounding FP32 -> BF16 (nearest even), the same as the device kernels'
 unsigned short host_bf16(float f);
 

@@ -374,3 +374,58 @@ def test_closure_learns_filtered_subgrid_terms(nnmod):
     print("training loss by epoch:", np.round(hist, 4), "held-out error:", overall, np.round(err, 3), flush=True)
     assert hist[-1] < hist[0]
     assert overall < 0.85
+
+
+@pytest.mark.parametrize("width,hidden", [(32, 128), (64, 256)])
+def test_fused_lift_and_projection_match_the_pack_gemm_sequence(nnmod, monkeypatch, width, hidden):
+    """On grids whose x-lines split into 64-point tiles (here 128^2 x 17, a
+    192-point padded line) the closure's lift runs fused with the packing of
+    its inputs (k_closure_lift) and its projection with the unpacking
+    (k_closure_project), and the MLP layers on the persistent pipelined kernel
+    (k_mlp_pipe).  The corrected quadratic spectra equal those of the pack,
+    GEMM, bias and unpack kernels and the one-tile MLP kernel
+    (CORAL_AMD_CLOSURE_UNFUSED=1, CORAL_AMD_MLP_UNPIPELINED=1) bit for bit,
+    and the network's own forward pass on the host's rows (within FP64
+    rounding of the transforms)."""
+    closure, devmem = nnmod
+    from conftest import product_recipe, seeded_state
+    from coral_amd.diagnostics import gauss_chebyshev_grid
+    from coral_amd.solver import Solver
+    r = product_recipe("coral.equations.rbc_ra1e5")
+    NX, NZ, M = 128, 17, 4
+    NXAA, NZAA = 3 * NX // 2, 3 * NZ // 2
+    basis = closure.fourier_basis(NXAA, NXAA, M)
+    cin = closure.input_width(4)
+    lay = closure.Layout(cin=cin, cout=9, width=width, hidden=hidden, layers=4, nb=basis.shape[1])
+    rng = np.random.default_rng(5)
+    model = closure.Model(lay, M, closure.init_params(lay, 12) * 3.0, rng.normal(size=cin) * 0.1,
+                          1.0 + rng.random(cin), rng.normal(size=9) * 0.1, 0.5 + rng.random(9))
+    k0, z0 = seeded_state(r, NX, NX, NZ, 6, 6, 9, amps=(("u", 1.0), ("v", 1.0), ("w", 1.0), ("t", 0.1)))
+    out = {}
+    for mode in ("fused", "unfused"):
+        if mode == "unfused":
+            monkeypatch.setenv("CORAL_AMD_CLOSURE_UNFUSED", "1")
+            monkeypatch.setenv("CORAL_AMD_MLP_UNPIPELINED", "1")
+        with Solver(r, NX, NX, NZ, 2 * np.pi, 2 * np.pi) as g:
+            g.set_state(k0, z0)
+            if mode == "fused":
+                q0 = g.products()
+                fields = np.stack([g.physical(i) for i in range(4)])
+            g.set_closure(model)
+            out[mode] = g.products()
+            if mode == "fused":
+                rows = closure.plane_rows(fields, gauss_chebyshev_grid(NZAA, 0.5, 1.0))
+                net = closure.ClosureNet(lay, basis, model.params)
+                X16 = to_dev_bf16(closure, devmem, closure.normalise(rows, model.in_mean, model.in_std))
+                Y = net.forward(X16, NZAA).numpy().reshape(NZAA, NXAA, NXAA, 9).astype(np.float64)
+                tau = Y * model.out_std.astype(np.float32) + model.out_mean.astype(np.float32)
+                worst = 0.0
+                for q in range(9):
+                    spec = g.fwd_transform(np.transpose(tau[..., q], (2, 1, 0)))
+                    d = out[mode][q] - q0[q]
+                    worst = max(worst, float(np.linalg.norm(d - spec) / np.linalg.norm(spec)))
+    monkeypatch.delenv("CORAL_AMD_CLOSURE_UNFUSED")
+    monkeypatch.delenv("CORAL_AMD_MLP_UNPIPELINED")
+    print("fused closure vs network + transform, worst rel L2:", worst, flush=True)
+    assert worst <= 1e-10
+    assert np.array_equal(out["fused"], out["unfused"])

@@ -16,6 +16,7 @@ namespace nn {
 
 typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
 typedef float f4 __attribute__((ext_vector_type(4)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
 __device__ inline float bf2f(unsigned short h) { return __uint_as_float(unsigned(h) << 16); }
 // FP32 -> BF16, round to nearest even (gfx950's conversion instruction)
@@ -720,15 +721,23 @@ void tile_transpose(hipStream_t st, const unsigned short *src, long long R, int 
 // t % 4, fragments straight from L2 (B) and HBM (H16); its partial sums go to
 // part[p][c]; k_modes_sum adds the chunks in order (deterministic).
 constexpr int MODE_CHUNK = 1024, MODE_TILES = 8;   // tiles per wavefront at most
-template <int TPW>
+constexpr int MODE_LDP = 40, MODE_AHEAD = 4;   // LDS row stride (bf16); steps loaded ahead
+template <int TPW, int NCH>
 __global__ __launch_bounds__(256) void k_modes(const unsigned short *__restrict__ BT, int nbp, int N2,
                                                const unsigned short *__restrict__ HT, int W,
                                                float *__restrict__ part) {
+    // per step of 32 points the workgroup copies the step's basis columns
+    // (nbp x 32) and H columns (W x 32) into LDS once, and every tile reads its
+    // fragments there (a global byte is read once per workgroup, not once per
+    // tile that uses it).  Double-buffered LDS; the global loads run
+    // MODE_AHEAD steps ahead in a ring of registers that are never copied into
+    // one another.  One barrier per step.
+    extern __shared__ __align__(16) unsigned short sm[];   // [2][nbp + W][MODE_LDP]
     const int c = blockIdx.x, p = blockIdx.y, nchunk = gridDim.x;
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int fr = lane & 15, fk = 8 * (lane >> 4), rq = 4 * (lane >> 4);
-    const int NB = W / 16, T = (nbp / 16) * NB;
-    const int i0 = c * MODE_CHUNK, i1 = min(N2, i0 + MODE_CHUNK);
+    const int NB = W / 16, T = (nbp / 16) * NB, R = nbp + W, bufsz = R * MODE_LDP;
+    const int i0 = c * MODE_CHUNK, i1 = min(N2, i0 + MODE_CHUNK), nst = (i1 - i0) / 32;
     // the wavefront's tiles: mode block and channel block of t = wv + 4 j
     int aoff[TPW], boff[TPW];
     bool on[TPW];
@@ -737,38 +746,59 @@ __global__ __launch_bounds__(256) void k_modes(const unsigned short *__restrict_
         const int t = wv + 4 * j;
         on[j] = t < T;
         const int mb = on[j] ? t / NB : 0, nb_ = on[j] ? t - mb * NB : 0;
-        aoff[j] = (16 * mb + fr) * N2 + fk;
-        boff[j] = (16 * nb_ + fr) * 32 + fk;
+        aoff[j] = (16 * mb + fr) * MODE_LDP + fk;
+        boff[j] = (nbp + 16 * nb_ + fr) * MODE_LDP + fk;
+    }
+    // this thread's 16-byte pieces of a step: row r = q / 4, points 8 (q % 4) ..
+    // (a basis row advances 32 points per step, the tiled H 32 W elements)
+    const unsigned short *src[NCH];
+    int dst[NCH], sstr[NCH];
+    bool has[NCH];
+#pragma unroll
+    for (int i = 0; i < NCH; ++i) {
+        const int q = tid + 256 * i, r = q >> 2, part8 = 8 * (q & 3);
+        has[i] = r < R;
+        src[i] = !has[i] ? BT
+                 : r < nbp ? BT + (size_t)r * N2 + i0 + part8
+                           : HT + ((size_t)p * N2 + i0) * W + (size_t)(r - nbp) * 32 + part8;
+        sstr[i] = r < nbp ? 32 : 32 * W;
+        dst[i] = r * MODE_LDP + part8;
     }
     f4 acc[TPW];
 #pragma unroll
     for (int j = 0; j < TPW; ++j) acc[j] = f4{0.f, 0.f, 0.f, 0.f};
-    // software pipeline: the next 32 points' fragments are loaded while the
-    // current ones feed the MFMAs (the points k .. k + 31 are one tile of HT)
-    bf8 av[TPW], bv[TPW];
-    auto load = [&](int k, bf8 (&a)[TPW], bf8 (&b)[TPW]) {
-        const unsigned short *hp = HT + ((size_t)p * N2 + k) * W;
+    u32x4 ring[MODE_AHEAD][NCH];
 #pragma unroll
-        for (int j = 0; j < TPW; ++j) {
-            a[j] = *reinterpret_cast<const bf8 *>(BT + (size_t)aoff[j] + k);
-            b[j] = *reinterpret_cast<const bf8 *>(hp + boff[j]);
-        }
-    };
-    load(i0, av, bv);
-    for (int k = i0; k < i1; k += 32) {
-        bf8 an[TPW], bn[TPW];
-        if (k + 32 < i1) load(k + 32, an, bn);
+    for (int d = 0; d < MODE_AHEAD; ++d)
 #pragma unroll
-        for (int j = 0; j < TPW; ++j)
-            if (on[j]) acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[j], bv[j], acc[j], 0, 0, 0);
-        if (k + 32 < i1) {
-#pragma unroll
-            for (int j = 0; j < TPW; ++j) {
-                av[j] = an[j];
-                bv[j] = bn[j];
-            }
-        }
+        for (int i = 0; i < NCH; ++i)
+            if (d < nst && has[i]) ring[d][i] = *reinterpret_cast<const u32x4 *>(src[i] + (size_t)d * sstr[i]);
+    // one step: store the ring's slot D into LDS buffer D % 2, refill the slot
+    // MODE_AHEAD steps ahead, wait for the workgroup, feed the MFMAs (written
+    // out per slot so that every ring index is a constant)
+#define MODE_STEP(D)                                                                                            \
+    if (s0 + D < nst) {                                                                                         \
+        unsigned short *buf = sm + (D & 1) * bufsz;                                                             \
+        _Pragma("unroll") for (int i = 0; i < NCH; ++i) if (has[i])                                             \
+            *reinterpret_cast<u32x4 *>(buf + dst[i]) = ring[D][i];                                              \
+        const int sn = s0 + D + MODE_AHEAD;                                                                     \
+        _Pragma("unroll") for (int i = 0; i < NCH; ++i) if (sn < nst && has[i])                                 \
+            ring[D][i] = *reinterpret_cast<const u32x4 *>(src[i] + (size_t)sn * sstr[i]);                       \
+        __syncthreads();                                                                                        \
+        _Pragma("unroll") for (int j = 0; j < TPW; ++j) if (on[j]) {                                            \
+            const bf8 a = *reinterpret_cast<const bf8 *>(buf + aoff[j]);                                        \
+            const bf8 b = *reinterpret_cast<const bf8 *>(buf + boff[j]);                                        \
+            acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[j], 0, 0, 0);                            \
+        }                                                                                                       \
     }
+    static_assert(MODE_AHEAD == 4, "MODE_STEP is written out for four slots");
+    for (int s0 = 0; s0 < nst; s0 += MODE_AHEAD) {
+        MODE_STEP(0)
+        MODE_STEP(1)
+        MODE_STEP(2)
+        MODE_STEP(3)
+    }
+#undef MODE_STEP
     float *out = part + ((size_t)p * nchunk + c) * nbp * W;
 #pragma unroll
     for (int j = 0; j < TPW; ++j) {
@@ -792,8 +822,10 @@ __global__ void k_modes_sum(const float *__restrict__ part, int nchunk, int nbp,
     Cm[i] = s;
 }
 bool modes_supported(int nbp, int N2, int W) {
+    // nbp + W <= 256: a step's rows in LDS (40 KB double-buffered), at most
+    // four 16-byte pieces per thread
     return nbp > 0 && W > 0 && nbp % 16 == 0 && W % 16 == 0 && N2 > 0 && N2 % 32 == 0 &&
-           (nbp / 16) * (W / 16) <= 4 * MODE_TILES;
+           (nbp / 16) * (W / 16) <= 4 * MODE_TILES && nbp + W <= 256;
 }
 long long modes_work(int nbp, int N2, int W, int P) {
     return (long long)P * ((N2 + MODE_CHUNK - 1) / MODE_CHUNK) * nbp * W;
@@ -805,18 +837,20 @@ void modes(hipStream_t st, const unsigned short *BT, int nbp, int nb, int N2, co
         throw std::runtime_error("modes: shapes (nbp, W multiples of 16, N2 of 32, at most 32 tiles)");
     const int nchunk = (N2 + MODE_CHUNK - 1) / MODE_CHUNK;
     if (!work || nwork < modes_work(nbp, N2, W, P)) throw std::runtime_error("modes: work space too small");
-    const int tpw = ((nbp / 16) * (W / 16) + 3) / 4;
+    const int tpw = ((nbp / 16) * (W / 16) + 3) / 4, nch = (4 * (nbp + W) + 255) / 256;
+    const size_t lds = sizeof(unsigned short) * 2 * size_t(nbp + W) * MODE_LDP;
     const dim3 g(nchunk, P);
-    switch (tpw) {
-    case 1: hipLaunchKernelGGL((k_modes<1>), g, dim3(256), 0, st, BT, nbp, N2, HT, W, work); break;
-    case 2: hipLaunchKernelGGL((k_modes<2>), g, dim3(256), 0, st, BT, nbp, N2, HT, W, work); break;
-    case 3: hipLaunchKernelGGL((k_modes<3>), g, dim3(256), 0, st, BT, nbp, N2, HT, W, work); break;
-    case 4: hipLaunchKernelGGL((k_modes<4>), g, dim3(256), 0, st, BT, nbp, N2, HT, W, work); break;
-    case 5: hipLaunchKernelGGL((k_modes<5>), g, dim3(256), 0, st, BT, nbp, N2, HT, W, work); break;
-    case 6: hipLaunchKernelGGL((k_modes<6>), g, dim3(256), 0, st, BT, nbp, N2, HT, W, work); break;
-    case 7: hipLaunchKernelGGL((k_modes<7>), g, dim3(256), 0, st, BT, nbp, N2, HT, W, work); break;
-    default: hipLaunchKernelGGL((k_modes<8>), g, dim3(256), 0, st, BT, nbp, N2, HT, W, work); break;
+    bool done = false;
+#define MODES_CASE(tpw_, nch_)                                                                                 \
+    if (!done && tpw == tpw_ && nch == nch_) {                                                                 \
+        hipLaunchKernelGGL((k_modes<tpw_, nch_>), g, dim3(256), lds, st, BT, nbp, N2, HT, W, work);             \
+        done = true;                                                                                           \
     }
+#define MODES_TPW(tpw_) MODES_CASE(tpw_, 1) MODES_CASE(tpw_, 2) MODES_CASE(tpw_, 3) MODES_CASE(tpw_, 4)
+    MODES_TPW(1) MODES_TPW(2) MODES_TPW(3) MODES_TPW(4) MODES_TPW(5) MODES_TPW(6) MODES_TPW(7) MODES_TPW(8)
+#undef MODES_TPW
+#undef MODES_CASE
+    if (!done) throw std::runtime_error("modes: no kernel for these shapes");
     const long long n = (long long)P * nb * W;
     hipLaunchKernelGGL(k_modes_sum, dim3(unsigned((n + 255) / 256)), dim3(256), 0, st, work, nchunk, nbp, nb, W, P,
                        Cm);

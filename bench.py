@@ -169,7 +169,8 @@ def bench_ensemble(args):
     with E.Ensemble([g for g, _, _, _ in runs]) as ens:
         t0 = time.perf_counter()
         nb, tuned_ms = ens.tune(6)
-        log(f"tuner: {nb} branches, {tuned_ms:.3f} ms per ensemble step, {time.perf_counter() - t0:.1f} s")
+        sched = f"lockstep, fronts over {-nb} branches" if nb <= 0 else f"{nb} branches"
+        log(f"tuner: {sched}, {tuned_ms:.3f} ms per ensemble step, {time.perf_counter() - t0:.1f} s")
         ens.step(2)
         for g, _, k0, z0 in runs:
             g.set_state(k0, z0)
@@ -197,7 +198,7 @@ def bench_ensemble(args):
         "config": {"workload": f"{nrun} RBC runs drawn from seed {seed}: NX=NY in {{64,96,128}}, NZ in {{33,49,65}}, "
                                "Ra log-uniform [1e5,1e7], Pr in {0.7,1,7}, dt = 5e-5 (Ra Pr/1e5)^-1/2, ARS_222",
                    "runs": [[sp.NX, sp.NZ, round(sp.Ra, 1), sp.Pr, sp.dt] for _, sp, _, _ in runs],
-                   "branches": nb, "launch_shapes": shapes},
+                   "branches": nb, "schedule": sched, "launch_shapes": shapes},
         "run_steps_per_s": round(nrun * value, 1),
         "solo_sequential": {"ms_per_step_all_runs": round(solo_total, 4), "ensemble_steps_per_s": round(1e3 / solo_total, 3),
                             "per_run_ms": [round(x, 4) for x in solo_ms]},

@@ -161,6 +161,8 @@ struct coral_amd_ctx {
     // per full variable: (boundary-value extraction, vertical decomposition),
     // coral_amd_set_full_variable_filter
     std::vector<std::pair<int, int>> full_filter;
+    // the current step's stage coefficients (step_begin): M x^n, M x_j, L x_j, K_hat_j
+    std::vector<std::vector<double>> stepRm, stepRx, stepRk;
     // mean-system sources: the terms (coral_amd_add_source_term) and the
     // sources' Chebyshev coefficients by index (coral_amd_set_source);
     // src_missing is the lowest index a term names with no definition (0: none)
@@ -1414,31 +1416,39 @@ void nl_terms_mean(coral_amd_ctx &c, int stage) {
 
 // (M - dt a11 L) out = rhs for every wavenumber of a system, with the factors
 // of the last set_dt (backsolve_to_aux, PL_IMEX_timestepping_marching.f90:57-58)
+// the 16-lane solve's arguments for system d (false: d uses another kernel)
+bool solve16_args_of(const coral_amd_ctx &c, const SysDev &d, const double2 *rhs, double2 *out, Solve16Args &sa) {
+    if (d.ncol16 <= 0) return false;
+    const SystemTables &t = *d.t;
+    sa = Solve16Args{};
+    sa.L16 = d.real ? (const void *)d.Lr.p : (const void *)d.Lf.p;
+    sa.U = d.real ? (const void *)d.Urt.p : (const void *)d.Uft.p;
+    sa.ucol = d.ucol.p;
+    sa.ustride = d.ustride;
+    sa.ncol16 = d.ncol16;
+    sa.piv = d.piv.p;
+    sa.words = d.words.p;
+    sa.rotate = d.real ? 1 : 0;
+    sa.ident0 = d.kxky ? 1 : 0;
+    sa.total = t.total;
+    sa.n = d.blk_n;
+    sa.kl = d.blk_nl;
+    sa.kv = d.blk_nl + d.blk_nu;
+    sa.nsys = d.nsys;
+    sa.nblk = d.nblk;
+    sa.rhs = rhs;
+    sa.x = out;
+    sa.sink = c.sink.p;
+    return true;
+}
+
 void solve_system(coral_amd_ctx &c, SysDev &d, const double2 *rhs, double2 *out, const char *name = nullptr) {
     const SystemTables &t = *d.t;
     c.timed(name ? name : (d.kxky ? "solve" : "solve_mean"), [&] {
         const int kv = t.nl + t.nu, id0 = d.kxky ? 1 : 0;
         bool done = true;
-        if (d.ncol16 > 0) {
-            Solve16Args sa{};
-            sa.L16 = d.real ? (const void *)d.Lr.p : (const void *)d.Lf.p;
-            sa.U = d.real ? (const void *)d.Urt.p : (const void *)d.Uft.p;
-            sa.ucol = d.ucol.p;
-            sa.ustride = d.ustride;
-            sa.ncol16 = d.ncol16;
-            sa.piv = d.piv.p;
-            sa.words = d.words.p;
-            sa.rotate = d.real ? 1 : 0;
-            sa.ident0 = id0;
-            sa.total = t.total;
-            sa.n = d.blk_n;
-            sa.kl = d.blk_nl;
-            sa.kv = d.blk_nl + d.blk_nu;
-            sa.nsys = d.nsys;
-            sa.nblk = d.nblk;
-            sa.rhs = rhs;
-            sa.x = out;
-            sa.sink = c.sink.p;
+        Solve16Args sa;
+        if (solve16_args_of(c, d, rhs, out, sa)) {
             done = d.real ? launch_solve16_real(c.stream, sa) : launch_solve16_complex(c.stream, sa);
         } else if (d.real)
             launch_solve_real(c.stream, d.Lr.p, d.Ur.p + SOLVE16_PAD, d.piv.p, d.rot.p, id0, t.total, t.nl, kv,
@@ -1501,7 +1511,13 @@ void project_kx0(coral_amd_ctx &c) {
 // NL terms of the previous stage solution, rhs_i = M x^n + sum_{j<i} dt a_ij L x_j
 // + sum_{j<=i} dt ah_{i+1,j} K_hat_j, and solves (M - dt a_ii L) x_i = rhs_i;
 // x^{n+1} = x_s.
-void one_step(coral_amd_ctx &c) {
+// A step in phases (one_step_beyond, PL_IMEX_timestepping_marching.f90:3-48):
+// step_begin (the stage coefficients), per stage stage_nl (NL terms, the mean
+// systems on the side stream) and per kxky system stage_rhs then
+// stage_solve, and step_end.  one_step runs them for one run; the ensemble's
+// lockstep step runs every run's stage_nl and stage_rhs, then their solves
+// as one launch.
+void step_begin(coral_amd_ctx &c) {
     if (c.src_missing)
         throw std::runtime_error("source" + std::to_string(c.src_missing / 10) + std::to_string(c.src_missing % 10) +
                                  " is used by a mean equation but has no definition (coral_amd_set_source)");
@@ -1515,8 +1531,10 @@ void one_step(coral_amd_ctx &c) {
     // such a combination, so every rhs_i is a combination of M x^n, M x_j,
     // L x_j (only where a_jj = 0) and K_hat_j alone: no earlier rhs_j is read
     // back.  mc[0] multiplies M x^n, mc[1 + j] M x_j, xc[j] L x_j, kc[j] K_hat_j.
-    std::vector<std::vector<double>> Rm(s, std::vector<double>(s + 1, 0.0)), Rx(s, std::vector<double>(s, 0.0)),
-        Rk(s, std::vector<double>(s, 0.0));
+    auto &Rm = c.stepRm, &Rx = c.stepRx, &Rk = c.stepRk;
+    Rm.assign(s, std::vector<double>(s + 1, 0.0));
+    Rx.assign(s, std::vector<double>(s, 0.0));
+    Rk.assign(s, std::vector<double>(s, 0.0));
     for (int i = 0; i < s; ++i) {
         std::vector<double> &mc = Rm[i], &xc = Rx[i], &kc = Rk[i];
         mc[0] = 1.0;
@@ -1534,38 +1552,63 @@ void one_step(coral_amd_ctx &c) {
         }
         for (int j = 0; j <= i; ++j) kc[j] += dt * Ah(i + 1, j);
     }
-    for (int i = 0; i < s; ++i) {
-        std::vector<const double2 *> kst, zst;
-        for (auto &d : c.ks) kst.push_back(i == 0 ? d->field.p : d->X[i - 1]->p);
-        for (auto &d : c.zs) zst.push_back(i == 0 ? d->field.p : d->X[i - 1]->p);
-        nl_terms(c, kst, zst, i);
-        auto linear = [&](SysDev &d) {
-            std::vector<std::pair<const double2 *, double>> ms, xs, ks;
-            if (Rm[i][0] != 0.0) ms.push_back({d.field.p, Rm[i][0]});
-            for (int j = 0; j < i; ++j) {
-                if (Rm[i][1 + j] != 0.0) ms.push_back({d.X[j]->p, Rm[i][1 + j]});
-                if (Rx[i][j] != 0.0) xs.push_back({d.X[j]->p, Rx[i][j]});
-            }
-            for (int j = 0; j <= i; ++j)
-                if (Rk[i][j] != 0.0) ks.push_back({d.Khat[j]->p, Rk[i][j]});
-            double2 *rhs_i = d.R.p;
-            rhs(c, d, ms, xs, ks, rhs_i);
-            solve_system(c, d, rhs_i, d.X[i]->p);
-            if (c.refine == 2 || (c.refine == 1 && i == s - 1)) refine_solution(c, d, rhs_i, d.X[i]->p, A(i, i));
-        };
-        if (!c.zs.empty()) {
-            on_side(c, [&] {
-                nl_terms_mean(c, i);
-                for (auto &d : c.zs) linear(*d);
-                HIP_OK(hipEventRecord(c.ev_mean, c.stream));
-            });
-            c.mean_pending = true;
-        }
-        for (auto &d : c.ks) linear(*d);
+}
+
+// stage i's right-hand side of system d into d.R
+void stage_rhs(coral_amd_ctx &c, int i, SysDev &d) {
+    std::vector<std::pair<const double2 *, double>> ms, xs, ks;
+    if (c.stepRm[i][0] != 0.0) ms.push_back({d.field.p, c.stepRm[i][0]});
+    for (int j = 0; j < i; ++j) {
+        if (c.stepRm[i][1 + j] != 0.0) ms.push_back({d.X[j]->p, c.stepRm[i][1 + j]});
+        if (c.stepRx[i][j] != 0.0) xs.push_back({d.X[j]->p, c.stepRx[i][j]});
     }
+    for (int j = 0; j <= i; ++j)
+        if (c.stepRk[i][j] != 0.0) ks.push_back({d.Khat[j]->p, c.stepRk[i][j]});
+    rhs(c, d, ms, xs, ks, d.R.p);
+}
+
+// stage i's solution of system d into d.X[i] (with the refinement sweep when set)
+void stage_solve(coral_amd_ctx &c, int i, SysDev &d) {
+    solve_system(c, d, d.R.p, d.X[i]->p);
+    const int s = c.P.nstages;
+    if (c.refine == 2 || (c.refine == 1 && i == s - 1)) refine_solution(c, d, d.R.p, d.X[i]->p, c.P.a_arr[i * s + i]);
+}
+
+// stage i's NL terms, and the mean systems' whole stage on the side stream
+void stage_nl(coral_amd_ctx &c, int i) {
+    std::vector<const double2 *> kst, zst;
+    for (auto &d : c.ks) kst.push_back(i == 0 ? d->field.p : d->X[i - 1]->p);
+    for (auto &d : c.zs) zst.push_back(i == 0 ? d->field.p : d->X[i - 1]->p);
+    nl_terms(c, kst, zst, i);
+    if (!c.zs.empty()) {
+        on_side(c, [&] {
+            nl_terms_mean(c, i);
+            for (auto &d : c.zs) {
+                stage_rhs(c, i, *d);
+                stage_solve(c, i, *d);
+            }
+            HIP_OK(hipEventRecord(c.ev_mean, c.stream));
+        });
+        c.mean_pending = true;
+    }
+}
+
+void step_end(coral_amd_ctx &c) {
     join_mean(c);
     swap_step_buffers(c);
     if (c.project_kx0) project_kx0(c);
+}
+
+void one_step(coral_amd_ctx &c) {
+    step_begin(c);
+    for (int i = 0; i < c.P.nstages; ++i) {
+        stage_nl(c, i);
+        for (auto &d : c.ks) {
+            stage_rhs(c, i, *d);
+            stage_solve(c, i, *d);
+        }
+    }
+    step_end(c);
 }
 
 // Two graphs of one step each, captured from the stream (the side stream
@@ -3070,6 +3113,16 @@ struct coral_amd_ensemble {
     std::vector<unsigned long long> cap_gen;
     std::vector<double> run_ms;       // per-run step time measured by the tuner (0 = not measured)
     double tuned_ms = 0.0;            // ensemble step time of the tuned schedule
+    // lockstep schedule (set_schedule with 0 branches): one graph per step
+    // holding every run's NL terms and right-hand sides of a stage, then the
+    // stage's kxky solves of all runs as one launch per kernel variant; the
+    // launches' argument tables are filled during the capture and uploaded
+    // after it
+    bool lockstep = false;
+    std::vector<Solve16Args> lk_args_h;
+    std::vector<int> lk_start_h;
+    DevBuf<Solve16Args> lk_args;
+    DevBuf<int> lk_start;
 
     void drop() {
         // an executable graph is destroyed only once its launches have ended
@@ -3099,6 +3152,7 @@ namespace {
 
 void ens_set_branches(coral_amd_ensemble &e, int nb, const std::vector<int> &assign) {
     e.drop();
+    e.lockstep = false;
     while (int(e.branch.size()) < nb) {
         hipStream_t st = nullptr;
         HIP_OK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
@@ -3177,9 +3231,152 @@ void ens_capture(coral_amd_ensemble &e) {
     e.epar = 0;
 }
 
+// whether the runs can step in lockstep: the same number of stages and no
+// residual refinement (the batched solves leave it out)
+bool ens_lockstep_ok(const coral_amd_ensemble &e, std::string *why = nullptr) {
+    for (auto *c : e.runs) {
+        if (c->P.nstages != e.runs[0]->P.nstages) {
+            if (why) *why = "lockstep steps need the same IMEX scheme on every run";
+            return false;
+        }
+        if (c->refine) {
+            if (why) *why = "lockstep steps do not take residual refinement";
+            return false;
+        }
+    }
+    return true;
+}
+
+// The lockstep graphs (one per buffer assignment): per stage every run's NL
+// terms (its mean systems forked onto its side stream) and kxky right-hand
+// sides, then the stage's kxky solves grouped by kernel variant, each group
+// one k_solve16_batch launch; systems the 16-lane solve does not take are
+// solved on their own.  Every run's arithmetic is its own step's.
+void ens_capture_lockstep(coral_amd_ensemble &e) {
+    e.drop();
+    std::string why;
+    if (!ens_lockstep_ok(e, &why)) throw std::runtime_error(why);
+    for (auto *c : e.runs) {
+        if (c->timing) throw std::runtime_error("ensemble steps need timers off on every run");
+        if (!c->factored) throw std::runtime_error("coral_amd_set_dt must be called on every run before stepping");
+    }
+    const int s = e.runs[0]->P.nstages;
+    size_t nks = 0;
+    for (auto *c : e.runs) nks += c->ks.size();
+    const size_t cap_args = 2 * size_t(s) * std::max<size_t>(nks, 1), cap_start = 2 * size_t(s) * (nks + 8);
+    e.lk_args_h.assign(cap_args, Solve16Args{});
+    e.lk_start_h.assign(cap_start, 0);
+    e.lk_args.alloc(cap_args);
+    e.lk_start.alloc(cap_start);
+    size_t ao = 0, so = 0;
+    hipStream_t st = e.branch[0];
+    for (int p = 0; p < 2; ++p) {
+        e.gexec[p].assign(1, nullptr);
+        std::vector<hipStream_t> own(e.runs.size());
+        for (size_t r = 0; r < e.runs.size(); ++r) own[r] = e.runs[r]->stream;
+        auto restore = [&] {
+            for (size_t r = 0; r < e.runs.size(); ++r) e.runs[r]->stream = own[r];
+        };
+        HIP_OK(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed));
+        hipGraph_t g = nullptr;
+        try {
+            for (auto *c : e.runs) {
+                c->stream = st;
+                step_begin(*c);
+            }
+            for (int i = 0; i < s; ++i) {
+                // the runs' NL terms and right-hand sides, branch b's runs on
+                // its stream (branch 0 on the capture's own), joined before the solves
+                HIP_OK(hipEventRecord(e.ev_fork, st));
+                for (int b = 0; b < e.nbranch; ++b) {
+                    hipStream_t bs = b == 0 ? st : e.branch[b];
+                    if (b > 0) HIP_OK(hipStreamWaitEvent(bs, e.ev_fork, 0));
+                    for (size_t r = 0; r < e.runs.size(); ++r) {
+                        if (e.assign[r] != b) continue;
+                        coral_amd_ctx &c = *e.runs[r];
+                        c.stream = bs;
+                        stage_nl(c, i);
+                        for (auto &d : c.ks) stage_rhs(c, i, *d);
+                    }
+                    if (b > 0) {
+                        HIP_OK(hipEventRecord(e.ev_join[b], bs));
+                        HIP_OK(hipStreamWaitEvent(st, e.ev_join[b], 0));
+                    }
+                }
+                for (auto *c : e.runs) c->stream = st;
+                // (real form, NQ) -> the group's arguments, workgroups and LDS
+                std::map<std::pair<int, int>, std::vector<std::pair<Solve16Args, int>>> groups;
+                std::map<std::pair<int, int>, size_t> glds;
+                for (auto *c : e.runs)
+                    for (auto &d : c->ks) {
+                        Solve16Args sa;
+                        int nwg = 0, nq = 0;
+                        size_t lds = 0;
+                        if (solve16_args_of(*c, *d, d->R.p, d->X[i]->p, sa) && solve16_shape(sa, nwg, lds, nq)) {
+                            if (!d->real) sa.rotate = 0;
+                            const auto key = std::make_pair(d->real ? 1 : 0, nq);
+                            groups[key].push_back({sa, nwg});
+                            glds[key] = std::max(glds[key], lds);
+                        } else {
+                            stage_solve(*c, i, *d);
+                        }
+                    }
+                for (auto &gp : groups) {
+                    const auto &v = gp.second;
+                    if (ao + v.size() > cap_args || so + v.size() + 1 > cap_start)
+                        throw std::runtime_error("lockstep argument tables too small");
+                    int tot = 0;
+                    for (size_t k = 0; k < v.size(); ++k) {
+                        e.lk_args_h[ao + k] = v[k].first;
+                        e.lk_start_h[so + k] = tot;
+                        tot += v[k].second;
+                    }
+                    e.lk_start_h[so + v.size()] = tot;
+                    if (!launch_solve16_batch(st, gp.first.first != 0, gp.first.second, tot, glds[gp.first],
+                                              e.lk_args.p + ao, e.lk_start.p + so, int(v.size())))
+                        throw std::runtime_error("batched 16-lane solve rejected its launches");
+                    ao += v.size();
+                    so += v.size() + 1;
+                }
+            }
+            for (auto *c : e.runs) step_end(*c);
+        } catch (...) {
+            restore();
+            (void)hipStreamEndCapture(st, &g);
+            if (g) (void)hipGraphDestroy(g);
+            e.drop();
+            throw;
+        }
+        restore();
+        HIP_OK(hipStreamEndCapture(st, &g));
+        const hipError_t err = hipGraphInstantiate(&e.gexec[p][0], g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (err != hipSuccess) {
+            e.drop();
+            throw HipError(std::string("hipGraphInstantiate: ") + hipGetErrorString(err));
+        }
+    }
+    // the graphs read their argument tables from now on
+    HIP_OK(hipMemcpy(e.lk_args.p, e.lk_args_h.data(), ao * sizeof(Solve16Args), hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(e.lk_start.p, e.lk_start_h.data(), so * sizeof(int), hipMemcpyHostToDevice));
+    e.cap_nswap.resize(e.runs.size());
+    e.cap_gen.resize(e.runs.size());
+    for (size_t r = 0; r < e.runs.size(); ++r) {
+        e.cap_nswap[r] = e.runs[r]->nswap;
+        e.cap_gen[r] = e.runs[r]->gen;
+    }
+    e.epar = 0;
+}
+
 // one ensemble step: every branch's graph on its own stream, forked from and
-// joined into the ensemble's stream
+// joined into the ensemble's stream (lockstep: its one graph on that stream)
 void ens_launch(coral_amd_ensemble &e) {
+    if (e.lockstep) {
+        HIP_OK(hipGraphLaunch(e.gexec[e.epar][0], e.master));
+        for (auto *c : e.runs) swap_step_buffers(*c);
+        e.epar ^= 1;
+        return;
+    }
     HIP_OK(hipEventRecord(e.ev_fork, e.master));
     for (int b = 0; b < e.nbranch; ++b) {
         HIP_OK(hipStreamWaitEvent(e.branch[b], e.ev_fork, 0));
@@ -3207,7 +3404,12 @@ void ens_steps(coral_amd_ensemble &e, int nsteps) {
         HIP_OK(hipStreamWaitEvent(e.master, e.ev_run[r], 0));
     }
     for (int n = 0; n < nsteps; ++n) {
-        if (!ens_valid(e)) ens_capture(e);
+        if (!ens_valid(e)) {
+            if (e.lockstep)
+                ens_capture_lockstep(e);
+            else
+                ens_capture(e);
+        }
         ens_launch(e);
     }
     HIP_OK(hipEventRecord(e.ev_done, e.master));
@@ -3334,9 +3536,28 @@ int coral_amd_ensemble_destroy(coral_amd_ensemble *e) {
 int coral_amd_ensemble_set_schedule(coral_amd_ensemble *e, int32_t nbranch, const int32_t *assign) {
     try {
         if (!e) throw std::runtime_error("null ensemble");
-        if (nbranch < 1 || nbranch > int(e->runs.size())) throw std::runtime_error("nbranch must be 1..nrun");
+        if (nbranch < -int(e->runs.size()) || nbranch > int(e->runs.size()))
+            throw std::runtime_error("nbranch must be -nrun..nrun");
         HIP_OK(hipSetDevice(e->device));
         HIP_OK(hipStreamSynchronize(e->master));
+        if (nbranch <= 0) {
+            // lockstep, the stage fronts over max(1, -nbranch) branches
+            std::string why;
+            if (!ens_lockstep_ok(*e, &why)) throw std::runtime_error(why);
+            const int nb = std::max(1, -nbranch);
+            std::vector<int> a(e->runs.size());
+            if (assign) {
+                for (size_t r = 0; r < a.size(); ++r) {
+                    if (assign[r] < 0 || assign[r] >= nb) throw std::runtime_error("branch index out of range");
+                    a[r] = assign[r];
+                }
+            } else {
+                a = lpt_assign(e->run_ms, nb);
+            }
+            ens_set_branches(*e, nb, a);
+            e->lockstep = true;
+            return 0;
+        }
         std::vector<int> a(e->runs.size());
         if (assign) {
             for (size_t r = 0; r < a.size(); ++r) {
@@ -3427,8 +3648,22 @@ int coral_amd_ensemble_tune(coral_amd_ensemble *e, int32_t trial_steps, int32_t 
                 bnb = nb;
             }
         }
+        // and lockstep schedules with their stage fronts over 1, 2, 4, ...
+        // branches (reported as minus that count)
+        if (ens_lockstep_ok(*e)) {
+            for (int nb = 1; nb <= int(nr) && nb <= 16; nb *= 2) {
+                ens_set_branches(*e, nb, lpt_assign(e->run_ms, nb));
+                e->lockstep = true;
+                const double ms = time_ens_steps(*e, steps);
+                if (ms < best) {
+                    best = ms;
+                    bnb = -nb;
+                }
+            }
+        }
         HIP_OK(hipStreamSynchronize(e->master));
-        ens_set_branches(*e, bnb, lpt_assign(e->run_ms, bnb));
+        ens_set_branches(*e, std::abs(bnb), lpt_assign(e->run_ms, std::abs(bnb)));
+        e->lockstep = bnb < 0;
         e->tuned_ms = best;
         for (size_t r = 0; r < nr; ++r) saved[r].restore(*e->runs[r]);
         if (nbranch_out) *nbranch_out = bnb;

@@ -1827,8 +1827,9 @@ struct Solve16 {
     }
 };
 
+// the solve of one 64-lane workgroup wg of a launch (four block matrices)
 template <typename T, int NQ, int PFB>
-__global__ __launch_bounds__(64) void k_solve16(Solve16Args a) {
+__device__ __forceinline__ void solve16_body(const Solve16Args &a, int wg) {
     using S = Solve16<T, NQ, PFB>;
     extern __shared__ unsigned sW[];   // position words of every block
     const T *__restrict__ L16 = reinterpret_cast<const T *>(a.L16);
@@ -1844,7 +1845,7 @@ __global__ __launch_bounds__(64) void k_solve16(Solve16Args a) {
     for (int e = lane; e < nbc; e += 64) sU[e] = a.ucol[e];
     // factored matrix (structural block) of this 16-lane row
     const int nmat = a.nsys * a.nblk;
-    const int sg = blockIdx.x * 4 + (lane >> 4);
+    const int sg = wg * 4 + (lane >> 4);
     const bool live = sg < nmat;
     const int M = live ? sg : nmat - 1;
     const int sys = M / a.nblk, blk = M - sys * a.nblk;
@@ -1951,6 +1952,24 @@ __global__ __launch_bounds__(64) void k_solve16(Solve16Args a) {
         for (int q = 0; q < NQ - 1; ++q) r[q] = r[q + 1];
         r[NQ - 1] = fix_y(ynx, wy);
     }
+}
+
+template <typename T, int NQ, int PFB>
+__global__ __launch_bounds__(64) void k_solve16(Solve16Args a) {
+    solve16_body<T, NQ, PFB>(a, blockIdx.x);
+}
+
+// Solves of several launches in one: workgroup b belongs to the launch r with
+// start[r] <= b < start[r + 1] (nrun launches, their arguments in args) and
+// runs as that launch's workgroup b - start[r]
+template <typename T, int NQ, int PFB>
+__global__ __launch_bounds__(64) void k_solve16_batch(const Solve16Args *__restrict__ args,
+                                                      const int *__restrict__ start, int nrun) {
+    const int b = blockIdx.x;
+    int r = 0;
+    while (r + 1 < nrun && start[r + 1] <= b) ++r;
+    const Solve16Args a = args[r];
+    solve16_body<T, NQ, PFB>(a, b - start[r]);
 }
 
 template <typename T>
@@ -2061,6 +2080,37 @@ int solve16_ncol(int n, int kl, int kv) {
     // (positions are packed in 15 bits of the position words)
     if (kl > 16 || n < 32 || n >= SOLVE16_NONE || 1 + (kv + 15) / 16 > 4) return 0;
     return 16 * ((n + 15) / 16 + 1);
+}
+
+bool solve16_shape(const Solve16Args &a, int &nwg, size_t &lds, int &nq) {
+    if (a.kl > 16 || a.n < 32 || a.ncol16 < 16 * ((a.n + 15) / 16 + 1) || !a.ucol || !a.sink) return false;
+    nq = 1 + (a.kv + 15) / 16;
+    if (nq < 2 || nq > 4) return false;
+    nwg = (a.nsys * a.nblk + 3) / 4;
+    lds = (size_t(a.nblk) * (a.n + 2 * SOLVE16_WPAD) + 1) / 2 * 2 * sizeof(unsigned) +
+          size_t(16 * ((a.n + 15) / 16)) * sizeof(int2);
+    return lds <= 32768;
+}
+
+bool launch_solve16_batch(hipStream_t st, bool real, int nq, int nwg, size_t lds, const Solve16Args *args,
+                          const int *start, int nrun) {
+    if (nwg <= 0 || nrun <= 0 || lds > 32768) return false;
+    const dim3 grid{unsigned(nwg)};
+#define SOLVE16_BATCH(T_)                                                                                         \
+    switch (nq) {                                                                                                 \
+    case 2: hipLaunchKernelGGL((k_solve16_batch<T_, 2, 8>), grid, dim3(64), lds, st, args, start, nrun); break;  \
+    case 3: hipLaunchKernelGGL((k_solve16_batch<T_, 3, 8>), grid, dim3(64), lds, st, args, start, nrun); break;  \
+    case 4: hipLaunchKernelGGL((k_solve16_batch<T_, 4, 8>), grid, dim3(64), lds, st, args, start, nrun); break;  \
+    default: return false;                                                                                        \
+    }
+    if (real) {
+        SOLVE16_BATCH(double)
+    } else {
+        SOLVE16_BATCH(double2)
+    }
+#undef SOLVE16_BATCH
+    CHECK_LAUNCH();
+    return true;
 }
 
 bool launch_solve16_real(hipStream_t st, const Solve16Args &a) { return launch_solve16<double>(st, a); }

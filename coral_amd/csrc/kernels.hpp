@@ -334,6 +334,15 @@ constexpr int SOLVE16_WPAD = 128;
 // b*(blk_n + 2*SOLVE16_WPAD) + SOLVE16_WPAD, padding SOLVE16_NOWORD.
 std::vector<unsigned> solve16_words(int total, int nblk, int blk_n, const int *blk_row, const int *blk_col,
                                     const unsigned char *rot);
+// workgroups, LDS bytes and NQ of the 16-lane solve of a (false: it does not take a)
+bool solve16_shape(const Solve16Args &a, int &nwg, size_t &lds, int &nq);
+// the 16-lane solves of nrun launches of one element type (real: the real
+// form; complex entries must carry rotate = 0) and NQ in one launch: device
+// copies of their arguments and of start[0..nrun] (cumulative workgroups),
+// nwg = start[nrun] workgroups, lds the largest of theirs; results equal each
+// launch's own
+bool launch_solve16_batch(hipStream_t st, bool real, int nq, int nwg, size_t lds, const Solve16Args *args,
+                          const int *start, int nrun);
 bool launch_solve16_real(hipStream_t st, const Solve16Args &a);
 bool launch_solve16_complex(hipStream_t st, const Solve16Args &a);
 // (ident0: system 0 is the identity and is not rotated)

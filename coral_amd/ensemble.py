@@ -108,13 +108,18 @@ class Ensemble:
         self.close()
 
     def tune(self, trial_steps: int = 10) -> Tuple[int, float]:
-        """Run the library's tuner; returns (branches, ms per ensemble step)."""
+        """Run the library's tuner; returns (branches, ms per ensemble step);
+        a negative branch count is the lockstep schedule over that many
+        branches."""
         nb, ms = ctypes.c_int32(), ctypes.c_double()
         check(lib.coral_amd_ensemble_tune(self._h, int(trial_steps), ctypes.byref(nb), ctypes.byref(ms)),
               "coral_amd_ensemble_tune")
         return nb.value, ms.value
 
     def set_schedule(self, nbranch: int, assign: Optional[Sequence[int]] = None):
+        """nbranch > 0: parallel branches of whole run steps; nbranch <= 0:
+        the lockstep schedule (stage fronts over max(1, -nbranch) branches,
+        each stage's solves of all runs batched)."""
         a = None if assign is None else (ctypes.c_int32 * len(assign))(*assign)
         check(lib.coral_amd_ensemble_set_schedule(self._h, int(nbranch), a), "coral_amd_ensemble_set_schedule")
 

@@ -390,14 +390,21 @@ typedef struct coral_amd_ensemble coral_amd_ensemble;
 int coral_amd_ensemble_create(coral_amd_ctx *const *runs, int32_t nrun, coral_amd_ensemble **out);
 int coral_amd_ensemble_destroy(coral_amd_ensemble *ens);
 /* nbranch parallel branches; assign[r] the branch of run r (NULL: longest
- * step first onto the least loaded branch, by the tuner's step times). */
+ * step first onto the least loaded branch, by the tuner's step times).
+ * nbranch <= 0 selects the lockstep schedule: one graph per ensemble step in
+ * which, stage by stage, every run's NL terms and right-hand sides (spread
+ * over max(1, -nbranch) parallel branches) are followed by the stage's kxky
+ * solves of all runs as one batched launch per kernel variant (same results
+ * bit for bit); it needs the same IMEX scheme on every run and no residual
+ * refinement. */
 int coral_amd_ensemble_set_schedule(coral_amd_ensemble *ens, int32_t nbranch, const int32_t *assign);
 /* The tuner: for every run, each launch shape (x pass 4, 2 or 1 wavefronts per
  * workgroup; z transforms on MFMA or as vector GEMMs) timed over trial_steps
  * graph replays, the fastest kept; then 1, 2, 4, ... 16 branches with
- * longest-first schedules, timed as ensemble steps, the fastest kept.  Every
- * run's state is restored afterwards.  Returns the branch count and the
- * ensemble's milliseconds per step. */
+ * longest-first schedules, and the lockstep schedule with its stage fronts
+ * over as many branches, timed as ensemble steps, the fastest kept.  Every
+ * run's state is restored afterwards.  Returns the branch count (negative:
+ * lockstep over that many branches) and the ensemble's milliseconds per step. */
 int coral_amd_ensemble_tune(coral_amd_ensemble *ens, int32_t trial_steps, int32_t *nbranch, double *ms_per_step);
 int coral_amd_ensemble_step(coral_amd_ensemble *ens, int32_t nsteps);
 int coral_amd_ensemble_synchronize(coral_amd_ensemble *ens);

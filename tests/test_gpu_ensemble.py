@@ -86,7 +86,7 @@ def test_ensemble_matches_solo_runs(mods):
         nb, ms = ens.tune(4)
         print("tuned:", nb, "branches,", ms, "ms per ensemble step;",
               [g.launch_shape() for g, _ in runs], flush=True)
-        assert 1 <= nb <= len(RUNS) and ms > 0
+        assert nb != 0 and -len(RUNS) <= nb <= len(RUNS) and ms > 0   # negative: lockstep
         ens.step(200)
         got = states(runs)
     worst = 0.0
@@ -95,5 +95,45 @@ def test_ensemble_matches_solo_runs(mods):
             worst = max(worst, rel(a, b))
     print("tuned ensemble vs solo after 200 steps: worst rel L2", worst, flush=True)
     assert worst <= 1e-12
+    for g, _ in runs:
+        g.close()
+
+
+def test_lockstep_ensemble_matches_solo_runs(mods):
+    """The lockstep schedule (set_schedule(-nb), 0 for one branch): one graph
+    per ensemble step in which every run's NL terms and right-hand sides of a
+    stage (over nb branches) are followed by the stage's kxky solves of all
+    runs as one batched launch (k_solve16_batch): bit for bit the solo steps
+    after 200 steps, also when one run is stepped on its own in between and
+    the schedule changes."""
+    solver, ensemble = mods
+    solo = make_runs(solver, ensemble)
+    for g, dt in solo:
+        g.use_graphs(True)
+        g.march_forward(dt, 200)
+    ref = states(solo)
+    for g, _ in solo:
+        g.close()
+    runs = make_runs(solver, ensemble)
+    with ensemble.Ensemble([g for g, _ in runs]) as ens:
+        ens.set_schedule(-3)          # stage fronts over three branches
+        ens.step(120)
+        g0, dt0 = runs[0]
+        g0.march_forward(dt0, 1)
+        ens.set_schedule(0)           # all on one stream
+        ens.step(80)
+        got = states(runs)
+    print("lockstep ensemble vs solo after 200 steps: bit-exact check", flush=True)
+    for i, ((k, z), (kr, zr)) in enumerate(zip(got, ref)):
+        if i == 0:
+            continue
+        for a, b in zip(k + z, kr + zr):
+            assert np.array_equal(a, b), f"run {i} differs"
+    r0 = solver.Solver(runs[0][0].recipe, RUNS[0][0], RUNS[0][0], RUNS[0][1], 2 * np.pi, 2 * np.pi)
+    r0.set_state(*ref[0])
+    r0.march_forward(runs[0][1], 1)
+    for a, b in zip(sum(got[0], []), sum(r0.get_state(), [])):
+        assert np.array_equal(a, b)
+    r0.close()
     for g, _ in runs:
         g.close()

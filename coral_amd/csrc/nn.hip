@@ -873,18 +873,25 @@ void mlp(hipStream_t st, long long R, int W, int HD, const unsigned short *Hin16
     // the pipelined inference kernel: spectral term present, up to four 32-mode blocks
     const bool pipe = fast && B16 && nbp <= 128 && !std::getenv("CORAL_AMD_MLP_UNPIPELINED");
     if (pipe) {
-        int dev = 0, ncu = 0;
-        if (hipGetDevice(&dev) != hipSuccess ||
-            hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || ncu <= 0)
+        // compute units of the current device (read once per device)
+        int dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess) throw std::runtime_error("mlp: no current device");
+        static int ncu_of[64] = {};
+        if (dev < 0 || dev >= 64) throw std::runtime_error("mlp: device index out of range");
+        if (!ncu_of[dev] &&
+            (hipDeviceGetAttribute(&ncu_of[dev], hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+             ncu_of[dev] <= 0))
             throw std::runtime_error("mlp: cannot read the device's compute-unit count");
-        // as many workgroups as are resident at once: a persistent workgroup
-        // that had to wait for a slot would start its share late
+        const int ncu = ncu_of[dev];
+        // as many workgroups as are resident at once (read once per kernel): a
+        // persistent workgroup that had to wait for a slot would start its share late
 #define MLP_PIPE(w_, hd_, nbk_)                                                                                   \
         if (W == w_ && HD == hd_ && nbp == 32 * nbk_) {                                                          \
-            int per = 0;                                                                                         \
-            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(                                                    \
-                    &per, reinterpret_cast<const void *>(k_mlp_pipe<w_, hd_, nbk_>), 256, 0) != hipSuccess ||   \
-                per < 1)                                                                                         \
+            static int per = 0;                                                                                  \
+            if (!per &&                                                                                          \
+                (hipOccupancyMaxActiveBlocksPerMultiprocessor(                                                   \
+                     &per, reinterpret_cast<const void *>(k_mlp_pipe<w_, hd_, nbk_>), 256, 0) != hipSuccess ||  \
+                 per < 1))                                                                                       \
                 per = 1;                                                                                         \
             const unsigned gp = unsigned(std::min<long long>(nblk, (long long)per * ncu));                      \
             hipLaunchKernelGGL((k_mlp_pipe<w_, hd_, nbk_>), dim3(gp), b, 0, st, nblk, Hin16, W1t, b1, W2t, b2, H, \

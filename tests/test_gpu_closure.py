@@ -429,3 +429,37 @@ def test_fused_lift_and_projection_match_the_pack_gemm_sequence(nnmod, monkeypat
     print("fused closure vs network + transform, worst rel L2:", worst, flush=True)
     assert worst <= 1e-10
     assert np.array_equal(out["fused"], out["unfused"])
+
+
+def test_closure_steps_as_graph_replays_match_eager_steps(nnmod):
+    """Steps with the closure (fused lift and projection, pipelined MLP,
+    LDS-staged mode projections at 128^2 x 17) replayed as HIP graphs equal
+    the same steps launched one by one, bit for bit, replay after replay."""
+    closure, devmem = nnmod
+    from conftest import product_recipe, seeded_state
+    from coral_amd.solver import Solver
+    r = product_recipe("coral.equations.rbc_ra1e5")
+    NX, NZ, M = 128, 17, 4
+    NXAA = 3 * NX // 2
+    basis = closure.fourier_basis(NXAA, NXAA, M)
+    cin = closure.input_width(4)
+    lay = closure.Layout(cin=cin, cout=9, width=32, hidden=128, layers=4, nb=basis.shape[1])
+    rng = np.random.default_rng(6)
+    model = closure.Model(lay, M, closure.init_params(lay, 13) * 0.3, rng.normal(size=cin) * 0.1,
+                          1.0 + rng.random(cin), rng.normal(size=9) * 0.01, 0.05 + 0.1 * rng.random(9))
+    k0, z0 = seeded_state(r, NX, NX, NZ, 6, 6, 9, amps=(("u", 1e-2), ("v", 1e-2), ("w", 1e-2), ("t", 1e-3)))
+    with Solver(r, NX, NX, NZ, 2 * np.pi, 2 * np.pi) as g:
+        g.set_closure(model)
+        g.set_state(k0, z0)
+        eager = []
+        for _ in range(4):
+            g.march_forward(1e-5)
+            eager.append(g.get_state())
+        g.set_state(k0, z0)
+        g.use_graphs(True)
+        for n in range(4):
+            g.march_forward(1e-5)
+            k, z = g.get_state()
+            for a, b in zip(k + z, eager[n][0] + eager[n][1]):
+                assert np.array_equal(a, b), f"replay {n}"
+        assert all(np.isfinite(a).all() for a in eager[-1][0])

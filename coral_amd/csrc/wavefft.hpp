@@ -2,7 +2,10 @@
 //
 // One wavefront (64 lanes) owns S complex sequences of length N held in its
 // own slice of LDS; stages exchange data through that slice with wavefront
-// ordering only (no workgroup barrier).  Lengths, radices and stage strides
+// ordering only (no workgroup barrier).  A group of LN = 128, 192, ... lanes
+// (consecutive wavefronts of a workgroup) can share the sequences instead:
+// its stages are ordered by the workgroup barrier, so every wavefront of the
+// workgroup runs the same stages.  Lengths, radices and stage strides
 // are template constants, so every index is resolved at compile time.
 //
 // Stage with radix R after Ns points have been combined (Ns = product of the
@@ -31,6 +34,16 @@ __device__ inline void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// orders the LDS traffic of a group of LN lanes between stages
+template <int LN>
+__device__ inline void group_sync() {
+    static_assert(LN % 64 == 0, "lane groups are whole wavefronts");
+    if constexpr (LN > 64)
+        __syncthreads();
+    else
+        wave_sync();
 }
 
 // z * (SIGN i)
@@ -177,21 +190,21 @@ template <int R, int NS>
 constexpr int stage_swz() { return (NS == 1 && R == 4) ? 1 : (NS == 4 && R == 4) ? 2 : (NS == 1 && R == 6) ? 3 : 0; }
 
 // One LDS -> LDS stage over S sequences (sequence s at buf + s*LS); the
-// S*T butterflies are spread over the 64 lanes round by round.  The input was
+// S*T butterflies are spread over the LN lanes of the group round by round.  The input was
 // written with swizzle WI, the output is written with stage_swz<R, NS>() when
 // SW is set, unswizzled otherwise.
-template <int N, int R, int NS, int SIGN, int S, int LS, int WI = 0, bool SW = false, class C>
+template <int N, int R, int NS, int SIGN, int S, int LS, int WI = 0, bool SW = false, int LN = 64, class C>
 __device__ inline void stage_lds(C *buf, const C *__restrict__ tw, int lane) {
     constexpr int WO = SW ? stage_swz<R, NS>() : 0;
     constexpr int T = N / R;
     constexpr int TOT = S * T;
-    constexpr int ROUNDS = (TOT + 63) / 64;
+    constexpr int ROUNDS = (TOT + LN - 1) / LN;
     static_assert(N % R == 0, "radix must divide the length");
     C v[ROUNDS][R];
 #pragma unroll
     for (int c = 0; c < ROUNDS; ++c) {
-        const int b = lane + 64 * c;
-        if (TOT % 64 == 0 || b < TOT) {
+        const int b = lane + LN * c;
+        if (TOT % LN == 0 || b < TOT) {
             const int s = b / T, j = b - s * T;
             const C *L = buf + s * LS;
             const int jm = j % NS;
@@ -206,11 +219,11 @@ __device__ inline void stage_lds(C *buf, const C *__restrict__ tw, int lane) {
             Dft<R, SIGN>::run(v[c]);
         }
     }
-    wave_sync();
+    group_sync<LN>();
 #pragma unroll
     for (int c = 0; c < ROUNDS; ++c) {
-        const int b = lane + 64 * c;
-        if (TOT % 64 == 0 || b < TOT) {
+        const int b = lane + LN * c;
+        if (TOT % LN == 0 || b < TOT) {
             const int s = b / T, j = b - s * T;
             C *L = buf + s * LS;
             const int base = (j / NS) * NS * R + j % NS;
@@ -218,33 +231,40 @@ __device__ inline void stage_lds(C *buf, const C *__restrict__ tw, int lane) {
             for (int r = 0; r < R; ++r) L[swz<WO>(base + r * NS)] = v[c][r];
         }
     }
-    wave_sync();
+    group_sync<LN>();
 }
 
 // A run of LDS stages with radices Rs..., starting after NS points on input
 // written with swizzle WI; sequences LS elements apart; SW: the stages write
-// swizzled.  OUT_SWZ is the swizzle of the run's output (what its reader must
-// apply).
-template <int N, int SIGN, int S, int LS, int WI, bool SW, int NS, int... Rs>
-struct StagesLSW;
+// swizzled; LN lanes share the work.  OUT_SWZ is the swizzle of the run's
+// output (what its reader must apply).
+template <int LN, int N, int SIGN, int S, int LS, int WI, bool SW, int NS, int... Rs>
+struct GStagesLSW;
 
-template <int N, int SIGN, int S, int LS, int WI, bool SW, int NS>
-struct StagesLSW<N, SIGN, S, LS, WI, SW, NS> {
+template <int LN, int N, int SIGN, int S, int LS, int WI, bool SW, int NS>
+struct GStagesLSW<LN, N, SIGN, S, LS, WI, SW, NS> {
     static constexpr int OUT_SWZ = WI;
     template <class C>
     __device__ static inline void run(C *, const C *__restrict__, int) {}
 };
 
-template <int N, int SIGN, int S, int LS, int WI, bool SW, int NS, int R, int... Rs>
-struct StagesLSW<N, SIGN, S, LS, WI, SW, NS, R, Rs...> {
-    using Next = StagesLSW<N, SIGN, S, LS, (SW ? stage_swz<R, NS>() : 0), SW, NS * R, Rs...>;
+template <int LN, int N, int SIGN, int S, int LS, int WI, bool SW, int NS, int R, int... Rs>
+struct GStagesLSW<LN, N, SIGN, S, LS, WI, SW, NS, R, Rs...> {
+    using Next = GStagesLSW<LN, N, SIGN, S, LS, (SW ? stage_swz<R, NS>() : 0), SW, NS * R, Rs...>;
     static constexpr int OUT_SWZ = Next::OUT_SWZ;
     template <class C>
     __device__ static inline void run(C *buf, const C *__restrict__ tw, int lane) {
-        stage_lds<N, R, NS, SIGN, S, LS, WI, SW>(buf, tw, lane);
+        stage_lds<N, R, NS, SIGN, S, LS, WI, SW, LN>(buf, tw, lane);
         Next::run(buf, tw, lane);
     }
 };
+
+// one wavefront per group of sequences
+template <int N, int SIGN, int S, int LS, int WI, bool SW, int NS, int... Rs>
+using StagesLSW = GStagesLSW<64, N, SIGN, S, LS, WI, SW, NS, Rs...>;
+// contiguous sequences shared by LN lanes, unswizzled
+template <int LN, int N, int SIGN, int S, int NS, int... Rs>
+using GStages = GStagesLSW<LN, N, SIGN, S, N, 0, false, NS, Rs...>;
 
 // runs on unswizzled input, without (StagesLS) or with (StagesLSZ) swizzled writes
 template <int N, int SIGN, int S, int LS, int NS, int... Rs>

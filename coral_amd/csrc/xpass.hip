@@ -1,4 +1,5 @@
-// x pass of the nonlinear step on one wavefront per (z, y) line.
+// x pass of the nonlinear step on one wavefront (two for 768-point lines)
+// per (z, y) line.
 //
 // For each line: inverse real FFTs of the full variables (two real lines are
 // packed into one complex sequence, f0 + i f1), the quadratic products and
@@ -16,32 +17,33 @@
 namespace coral {
 
 using wfft::Dft;
+using wfft::group_sync;
 using wfft::Stages;
 
 // Radix plans.  Inverse: first stage R0 reads the packed spectrum from HBM,
 // InvMid runs in LDS, the last stage RL leaves each lane holding the points
 // j + r*N/RL of its lines.  Forward: first stage RL starts from those
-// registers, FwdRest finishes in LDS in natural order.  LPW lines per
-// wavefront make LPW * N / RL a multiple of 64.  W0 / WL: the swizzles
-// (wfft::swz) of the first inverse and first forward stages' LDS writes,
-// which InvMid / FwdRest read through.
+// registers, FwdRest finishes in LDS in natural order.  LN lanes (one or
+// more wavefronts) share LPW lines, LPW * N / RL a multiple of LN.  W0 / WL:
+// the swizzles (wfft::swz) of the first inverse and first forward stages' LDS
+// writes, which InvMid / FwdRest read through.
 template <int N>
 struct XPlan;
 template <>
 struct XPlan<96> {
-    static constexpr int R0 = 4, RL = 6, LPW = 4, W0 = 0, WL = 0;
+    static constexpr int R0 = 4, RL = 6, LPW = 4, W0 = 0, WL = 0, LN = 64;
     template <int S> using InvMid = Stages<96, +1, S, 4, 4>;
     template <int S> using FwdRest = Stages<96, -1, S, 6, 4, 4>;
 };
 template <>
 struct XPlan<144> {
-    static constexpr int R0 = 4, RL = 3, LPW = 4, W0 = 0, WL = 0;
+    static constexpr int R0 = 4, RL = 3, LPW = 4, W0 = 0, WL = 0, LN = 64;
     template <int S> using InvMid = Stages<144, +1, S, 4, 4, 3>;
     template <int S> using FwdRest = Stages<144, -1, S, 3, 3, 4, 4>;
 };
 template <>
 struct XPlan<192> {
-    static constexpr int R0 = 4, RL = 3, LPW = 1, W0 = 0, WL = 0;
+    static constexpr int R0 = 4, RL = 3, LPW = 1, W0 = 0, WL = 0, LN = 64;
     template <int S> using InvMid = Stages<192, +1, S, 4, 4, 4>;
     template <int S> using FwdRest = Stages<192, -1, S, 3, 4, 4, 4>;
 };
@@ -51,28 +53,31 @@ struct XPlan<384> {
     // row of eight on the same banks: swizzled (0.613 -> 0.599 ms per
     // launch); the 2-way conflicts of the first forward stage's stride-6
     // writes cost less than their swizzle (0.624 ms with it)
-    static constexpr int R0 = 8, RL = 6, LPW = 1, W0 = 4, WL = 0;
+    static constexpr int R0 = 8, RL = 6, LPW = 1, W0 = 4, WL = 0, LN = 64;
     template <int S> using InvMid = wfft::StagesLSW<384, +1, S, 384, W0, false, 8, 8>;
     template <int S> using FwdRest = wfft::StagesLSW<384, -1, S, 384, WL, false, 6, 8, 8>;
 };
 template <>
 struct XPlan<768> {
-    static constexpr int R0 = 8, RL = 6, LPW = 1, W0 = 0, WL = 0;
-    template <int S> using InvMid = Stages<768, +1, S, 8, 4, 4>;
-    template <int S> using FwdRest = Stages<768, -1, S, 6, 4, 4, 8>;
+    // two wavefronts per line: one lane holds 6 points of each full variable
+    // pair (96 VGPRs of them per lane in FP64 on one wavefront, which left
+    // two wavefronts per SIMD)
+    static constexpr int R0 = 8, RL = 6, LPW = 1, W0 = 0, WL = 0, LN = 128;
+    template <int S> using InvMid = wfft::GStages<LN, 768, +1, S, 8, 4, 4>;
+    template <int S> using FwdRest = wfft::GStages<LN, 768, -1, S, 6, 4, 4, 8>;
 };
 
 // XGP: product pairs per forward batch
 template <int N, int NPI, int XGP>
 struct XShape {
     using P = XPlan<N>;
-    static constexpr int LPW = P::LPW, R0 = P::R0, RL = P::RL;
+    static constexpr int LPW = P::LPW, R0 = P::R0, RL = P::RL, LN = P::LN;
     static constexpr int T0 = N / R0, TL = N / RL;
-    static constexpr int NT = LPW * TL / 64;  // (line, j) tasks per lane at the register stages
+    static constexpr int NT = LPW * TL / LN;  // (line, j) tasks per lane at the register stages
     static constexpr int SIN = LPW;           // inverse sequences in LDS at once (one pair per line)
     static constexpr int SF = LPW * XGP;      // forward sequences per batch
     static constexpr int SMAX = SIN > SF ? SIN : SF;
-    static_assert((LPW * TL) % 64 == 0, "register stage must fill the wavefront");
+    static_assert((LPW * TL) % LN == 0, "register stage must fill the lane group");
 };
 
 // value of full variable A (1 for A < 0) at point r of a task
@@ -129,7 +134,7 @@ template <>
 struct Balanced<float2> {
     static constexpr bool value = true;
 };
-constexpr int XBAL_SLOTS = 16;   // per wavefront: 2 lines x (lines per wavefront x pairs per batch)
+constexpr int XBAL_SLOTS = 16;   // per lane group: 2 lines x (lines per group x pairs per batch)
 
 __device__ inline unsigned absmax_bits(float2 v) { return __float_as_uint(fmaxf(fabsf(v.x), fabsf(v.y))); }
 __device__ inline unsigned absmax_bits(float v) { return __float_as_uint(fabsf(v)); }
@@ -161,17 +166,17 @@ __device__ inline void xfwd_batch(const XPassArgs &a, const C (&fv)[XShape<N, NP
     using X = XShape<N, NPI, XGP>;
     using T = typename CxT<C>::R;
     C *H = reinterpret_cast<C *>(a.H);
-    constexpr int RL = X::RL, TL = X::TL, LPW = X::LPW;
+    constexpr int RL = X::RL, TL = X::TL, LPW = X::LPW, LN = X::LN;
     if constexpr (Balanced<C>::value) {
         static_assert(2 * LPW * G <= XBAL_SLOTS, "balancing slots");
         // products of every task first, their per-line maxima, then the
         // scaled FFTs (slot 2 (l G + gp) + {0, 1}: the pair's two lines)
         T p0[X::NT][G][RL], p1[X::NT][G][RL];
         if (lane < 2 * LPW * G) xmax[lane] = 0u;
-        wfft::wave_sync();
+        group_sync<LN>();
 #pragma unroll
         for (int c = 0; c < X::NT; ++c) {
-            const int t = lane + 64 * c, l = t / TL;
+            const int t = lane + LN * c, l = t / TL;
 #pragma unroll
             for (int gp = 0; gp < G; ++gp) {
                 const int q0 = 2 * (pq0 + gp), q1 = q0 + 1;
@@ -195,10 +200,10 @@ __device__ inline void xfwd_batch(const XPassArgs &a, const C (&fv)[XShape<N, NP
                 atomicMax(&xmax[2 * (l * G + gp) + 1], m1);
             }
         }
-        wfft::wave_sync();
+        group_sync<LN>();
 #pragma unroll
         for (int c = 0; c < X::NT; ++c) {
-            const int t = lane + 64 * c, l = t / TL, j = t - l * TL;
+            const int t = lane + LN * c, l = t / TL, j = t - l * TL;
 #pragma unroll
             for (int gp = 0; gp < G; ++gp) {
                 const T s0 = bal_scale(xmax[2 * (l * G + gp)]), s1 = bal_scale(xmax[2 * (l * G + gp) + 1]);
@@ -214,7 +219,7 @@ __device__ inline void xfwd_batch(const XPassArgs &a, const C (&fv)[XShape<N, NP
     } else {
 #pragma unroll
     for (int c = 0; c < X::NT; ++c) {
-        const int t = lane + 64 * c, l = t / TL, j = t - l * TL;
+        const int t = lane + LN * c, l = t / TL, j = t - l * TL;
 #pragma unroll
         for (int gp = 0; gp < G; ++gp) {
             const int q0 = 2 * (pq0 + gp), q1 = q0 + 1;
@@ -239,7 +244,7 @@ __device__ inline void xfwd_batch(const XPassArgs &a, const C (&fv)[XShape<N, NP
         }
     }
     }
-    wfft::wave_sync();
+    group_sync<LN>();
     XPlan<N>::template FwdRest<LPW * G>::run(buf, tw, lane);
     const int nkx = a.nkx;
 #pragma unroll
@@ -257,7 +262,7 @@ __device__ inline void xfwd_batch(const XPassArgs &a, const C (&fv)[XShape<N, NP
                     hf0 *= bal_unscale(xmax[2 * (l * G + gp)]);
                     hf1 *= bal_unscale(xmax[2 * (l * G + gp) + 1]);
                 }
-                for (int k = lane; k < nkx; k += 64) {
+                for (int k = lane; k < nkx; k += LN) {
                     const C y = L[k], yc = cconj(L[k == 0 ? 0 : N - k]);
                     // p0 = (Y + conj Y(-k)) / 2,  p1 = -i (Y - conj Y(-k)) / 2
                     h0[k] = mkc<C>(hf0 * (y.x + yc.x), hf0 * (y.y + yc.y));
@@ -266,7 +271,7 @@ __device__ inline void xfwd_batch(const XPassArgs &a, const C (&fv)[XShape<N, NP
             }
         }
     }
-    wfft::wave_sync();
+    group_sync<LN>();
 }
 
 // NK: the kept kx count when it is known at compile time (N/3, the 3/2 rule),
@@ -276,16 +281,17 @@ template <int N, int NPI, int XGP, int NK, class C>
 __global__ __launch_bounds__(256) void k_xpass_wave(XPassArgs a, const C *__restrict__ tw, long long nlines) {
     using X = XShape<N, NPI, XGP>;
     using T = typename CxT<C>::R;
-    constexpr int R0 = X::R0, RL = X::RL, T0 = X::T0, TL = X::TL, LPW = X::LPW, SIN = X::SIN;
+    constexpr int R0 = X::R0, RL = X::RL, T0 = X::T0, TL = X::TL, LPW = X::LPW, SIN = X::SIN, LN = X::LN;
     extern __shared__ unsigned char lds_raw[];
     C *lds = reinterpret_cast<C *>(lds_raw);
     __shared__ double cfl_part[4][3];
     __shared__ unsigned xmax_all[4][XBAL_SLOTS];   // per-line maxima of packed pairs (FP32 only)
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    unsigned *xmax = xmax_all[wid];
-    C *buf = lds + wid * X::SMAX * N;
+    // lane in the group of LN lanes that shares LPW lines, group in the workgroup
+    const int lane = threadIdx.x % LN, grp = threadIdx.x / LN, wid = threadIdx.x >> 6;
+    unsigned *xmax = xmax_all[grp];
+    C *buf = lds + grp * X::SMAX * N;
     const C *G = reinterpret_cast<const C *>(a.G);
-    const long long line0 = ((long long)blockIdx.x * (blockDim.x >> 6) + wid) * LPW;
+    const long long line0 = ((long long)blockIdx.x * (blockDim.x / LN) + grp) * LPW;
     const int nkx = NK > 0 ? NK : a.nkx;
 
     // ---- inverse transforms, one packed pair of full variables at a time
@@ -297,16 +303,16 @@ __global__ __launch_bounds__(256) void k_xpass_wave(XPassArgs a, const C *__rest
         // first stage: packed Hermitian spectra straight from HBM
         if constexpr (Balanced<C>::value) {
             static_assert(2 * LPW <= XBAL_SLOTS, "balancing slots");
-            constexpr int TOT = SIN * T0, ROUNDS = (TOT + 63) / 64;
+            constexpr int TOT = SIN * T0, ROUNDS = (TOT + LN - 1) / LN;
             // the two lines' values first (kept slots only, kx = 0 imaginary
             // parts dropped), their per-line maxima, then the scaled packing
             C va[ROUNDS][R0], vb[ROUNDS][R0];
             if (lane < 2 * LPW) xmax[lane] = 0u;
-            wfft::wave_sync();
+            group_sync<LN>();
 #pragma unroll
             for (int c = 0; c < ROUNDS; ++c) {
-                const int b = lane + 64 * c;
-                if (TOT % 64 == 0 || b < TOT) {
+                const int b = lane + LN * c;
+                if (TOT % LN == 0 || b < TOT) {
                     const int l = b / T0, j = b - l * T0;
                     const long long line = line0 + l;
                     const bool valid = line < nlines;
@@ -342,11 +348,11 @@ __global__ __launch_bounds__(256) void k_xpass_wave(XPassArgs a, const C *__rest
                     atomicMax(&xmax[2 * l + 1], mb);
                 }
             }
-            wfft::wave_sync();
+            group_sync<LN>();
 #pragma unroll
             for (int c = 0; c < ROUNDS; ++c) {
-                const int b = lane + 64 * c;
-                if (TOT % 64 == 0 || b < TOT) {
+                const int b = lane + LN * c;
+                if (TOT % LN == 0 || b < TOT) {
                     const int l = b / T0, j = b - l * T0;
                     const T sa = bal_scale(xmax[2 * l]), sb = bal_scale(xmax[2 * l + 1]);
 #pragma unroll
@@ -366,14 +372,14 @@ __global__ __launch_bounds__(256) void k_xpass_wave(XPassArgs a, const C *__rest
                     for (int r = 0; r < R0; ++r) buf[l * N + wfft::swz<XPlan<N>::W0>(j * R0 + r)] = va[c][r];
                 }
             }
-            wfft::wave_sync();
+            group_sync<LN>();
         } else {
-            constexpr int TOT = SIN * T0, ROUNDS = (TOT + 63) / 64;
+            constexpr int TOT = SIN * T0, ROUNDS = (TOT + LN - 1) / LN;
             C v[ROUNDS][R0];
 #pragma unroll
             for (int c = 0; c < ROUNDS; ++c) {
-                const int b = lane + 64 * c;
-                if (TOT % 64 == 0 || b < TOT) {
+                const int b = lane + LN * c;
+                if (TOT % LN == 0 || b < TOT) {
                     const int l = b / T0, j = b - l * T0;
                     const long long line = line0 + l;
                     const bool valid = line < nlines;
@@ -414,20 +420,20 @@ __global__ __launch_bounds__(256) void k_xpass_wave(XPassArgs a, const C *__rest
             }
 #pragma unroll
             for (int c = 0; c < ROUNDS; ++c) {
-                const int b = lane + 64 * c;
-                if (TOT % 64 == 0 || b < TOT) {
+                const int b = lane + LN * c;
+                if (TOT % LN == 0 || b < TOT) {
                     const int l = b / T0, j = b - l * T0;
 #pragma unroll
                     for (int r = 0; r < R0; ++r) buf[l * N + wfft::swz<XPlan<N>::W0>(j * R0 + r)] = v[c][r];
                 }
             }
-            wfft::wave_sync();
+            group_sync<LN>();
         }
         XPlan<N>::template InvMid<SIN>::run(buf, tw, lane);
         // last stage into registers
 #pragma unroll
         for (int c = 0; c < X::NT; ++c) {
-            const int t = lane + 64 * c, l = t / TL, j = t - l * TL;
+            const int t = lane + LN * c, l = t / TL, j = t - l * TL;
             const C *L = buf + l * N;
             C wp[RL];
             wfft::twiddle_powers<RL, +1>(tw, j, wp);
@@ -444,7 +450,7 @@ __global__ __launch_bounds__(256) void k_xpass_wave(XPassArgs a, const C *__rest
                 for (int r = 0; r < RL; ++r) fv[c][p][r] = mkc<C>(fv[c][p][r].x * ua, fv[c][p][r].y * ub);
             }
         }
-        wfft::wave_sync();
+        group_sync<LN>();
     }
 
     // ---- CFL maxima over u, v, w / weight_z (full variables 0, 1, 2) ----
@@ -453,7 +459,7 @@ __global__ __launch_bounds__(256) void k_xpass_wave(XPassArgs a, const C *__rest
         double mu = 0.0, mv = 0.0, mw = 0.0;
 #pragma unroll
         for (int c = 0; c < X::NT; ++c) {
-            const int t = lane + 64 * c, l = t / TL;
+            const int t = lane + LN * c, l = t / TL;
             const long long line = line0 + l;
             if (line < nlines) {
                 // max |w| / zw = (max |w|) / zw: rounded division by a
@@ -473,7 +479,7 @@ __global__ __launch_bounds__(256) void k_xpass_wave(XPassArgs a, const C *__rest
             mv = fmax(mv, __shfl_xor(mv, off));
             mw = fmax(mw, __shfl_xor(mw, off));
         }
-        if (lane == 0) {
+        if ((threadIdx.x & 63) == 0) {   // per wavefront, not per lane group
             cfl_part[wid][0] = mu;
             cfl_part[wid][1] = mv;
             cfl_part[wid][2] = mw;
@@ -500,15 +506,18 @@ __global__ __launch_bounds__(256) void k_xpass_wave(XPassArgs a, const C *__rest
 template <int N, int NPI, int XGP, int NK, class C>
 static bool launch_wave(hipStream_t st, const C *tw, const XPassArgs &a) {
     using X = XShape<N, NPI, XGP>;
+    constexpr int WPG = X::LN / 64;   // wavefronts per lane group
+    static_assert(4 % WPG == 0, "a workgroup holds whole lane groups");
     const long long nlines = (long long)a.NZAA * a.NYAA;
-    const size_t per_wave = size_t(X::SMAX) * N * sizeof(C);
-    int wpb = a.max_wpb >= 1 && a.max_wpb <= 4 ? a.max_wpb : 4;
-    while (wpb > 1 && wpb * per_wave > 65536) wpb /= 2;
-    if (wpb * per_wave > 65536) return false;
-    const long long per_block = (long long)wpb * X::LPW;
+    const size_t per_group = size_t(X::SMAX) * N * sizeof(C);
+    const int wpb = a.max_wpb >= 1 && a.max_wpb <= 4 ? a.max_wpb : 4;
+    int gpb = wpb / WPG > 0 ? wpb / WPG : 1;   // lane groups per workgroup
+    while (gpb > 1 && gpb * per_group > 65536) gpb /= 2;
+    if (gpb * per_group > 65536) return false;
+    const long long per_block = (long long)gpb * X::LPW;
     const unsigned blocks = unsigned((nlines + per_block - 1) / per_block);
-    hipLaunchKernelGGL((k_xpass_wave<N, NPI, XGP, NK, C>), dim3(blocks), dim3(64 * wpb), wpb * per_wave, st, a, tw,
-                       nlines);
+    hipLaunchKernelGGL((k_xpass_wave<N, NPI, XGP, NK, C>), dim3(blocks), dim3(X::LN * gpb), gpb * per_group, st, a,
+                       tw, nlines);
     check_launch("x pass");
     return true;
 }

@@ -368,7 +368,8 @@ int coral_amd_set_source(coral_amd_ctx *ctx, int32_t source_index, const double 
 
 /* Launch shapes of a run (no counterpart in the reference): xpass_wpb is the
  * largest number of wavefronts per workgroup of the x pass (1..4; 0 = the most
- * that fit its LDS), z_plain = 1 runs the z transforms as vector GEMMs
+ * that fit its LDS; 768-point lines take two wavefronts each, so they run on
+ * two at least), z_plain = 1 runs the z transforms as vector GEMMs
  * instead of MFMA products (results equal to rounding).  The ensemble tuner
  * sets them per run. */
 int coral_amd_set_launch_shape(coral_amd_ctx *ctx, int32_t xpass_wpb, int32_t z_plain);

@@ -36,6 +36,8 @@ PRODUCT_CASES = [
     ("coral.equations.rbc_ra1e5", 64, 64, 17, 2 * np.pi, 2 * np.pi),
     ("coral.equations.rrbc_ra5e7_ek1e-4", 128, 96, 33, 2 * np.pi, 3.0),
     ("coral.equations.rrbc_ra5e7_ek1e-4", 64, 64, 129, 2 * np.pi, 2 * np.pi),
+    # 768-point x lines (two wavefronts per line in the x pass)
+    ("coral.equations.rrbc_ra5e7_ek1e-4", 512, 512, 17, 2 * np.pi, 2 * np.pi),
 ]
 
 

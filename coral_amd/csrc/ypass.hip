@@ -2,8 +2,8 @@
 //
 // A workgroup owns YL consecutive kx columns of one (variable, z) plane, so
 // every global access covers YL consecutive kx (YL complex = 128 B) or a whole
-// kept-ky line; each wavefront transforms its own YL/4 lines in LDS without
-// workgroup barriers.  These are the y-direction parts of c2r_transform
+// kept-ky line; each wavefront transforms its own lines in LDS without
+// workgroup barriers (YL/4 of them, one at 768 points).  These are the y-direction parts of c2r_transform
 // (backward c2c in y) and r2c_transform (forward c2c in y, truncated to the
 // kept ky), slabs.fftw3mpi/transforms.f90:155-301.
 #include "kernels.hpp"
@@ -16,8 +16,6 @@ namespace coral {
 using wfft::StagesLSZ;
 using wfft::swz;
 
-constexpr int YWAVES = 4;    // wavefronts per workgroup (YL / YWAVES lines each)
-constexpr int YT = 64 * YWAVES;   // threads per workgroup
 
 template <int N>
 struct YPlan;
@@ -49,12 +47,22 @@ __device__ inline int ky_slot(int jy, int NY, int N) { return jy < NY / 2 ? jy :
 template <class C>
 constexpr int ycols() { return int(128 / sizeof(C)); }
 
+// wavefronts per workgroup.  At 768 points (and 384 in FP64) one line per
+// wavefront (8 wavefronts in FP64, 16 in FP32) rather than four wavefronts
+// of two or four lines each: the tile's LDS (98 KB at 768, 49 KB at 384)
+// bounds the workgroups per CU, so four wavefronts per workgroup left one to
+// three per SIMD, each with two or four lines' butterflies in its registers
+// (768 points: FP64 yfwd 5.90 -> 5.34 ms, FP32 3.89 -> 3.00 ms per launch).
+// The FP32 384-point pass measured slower with 16 wavefronts and keeps four.
+template <int N, class C>
+constexpr int ywaves() { return N == 768 || (N == 384 && sizeof(C) == 16) ? ycols<C>() : 4; }
+
 // G[b][z][y][ix] = sum_jy Fz[b][z][ix*nky + jy] exp(+i ky y)
 template <int N, class C>
-__global__ __launch_bounds__(64 * YWAVES) void k_ybwd_w(const C *__restrict__ Fz, C *__restrict__ G,
-                                                       const C *__restrict__ tw, int nkx, int nky, int NY) {
-    constexpr int YL = ycols<C>();
-    constexpr int LS = N + 1, LPW = YL / YWAVES;
+__global__ __launch_bounds__((64 * ywaves<N, C>())) void k_ybwd_w(const C *__restrict__ Fz, C *__restrict__ G,
+                                                               const C *__restrict__ tw, int nkx, int nky, int NY) {
+    constexpr int YL = ycols<C>(), YT = 64 * ywaves<N, C>();
+    constexpr int LS = N + 1, LPW = YL / ywaves<N, C>();
     extern __shared__ unsigned char buf_raw[];   // YL * LS elements (beyond 64 KiB for N = 768)
     C *buf = reinterpret_cast<C *>(buf_raw);
     const int ntile = (nkx + YL - 1) / YL;
@@ -115,10 +123,10 @@ __global__ __launch_bounds__(64 * YWAVES) void k_ybwd_w(const C *__restrict__ Fz
 
 // Hy[b][z][ix*nky + jy] = sum_y H[b][z][y][ix] exp(-i ky y)
 template <int N, class C>
-__global__ __launch_bounds__(64 * YWAVES) void k_yfwd_w(const C *__restrict__ H, C *__restrict__ Hy,
-                                                       const C *__restrict__ tw, int nkx, int nky, int NY) {
-    constexpr int YL = ycols<C>();
-    constexpr int LS = N + 1, LPW = YL / YWAVES;
+__global__ __launch_bounds__((64 * ywaves<N, C>())) void k_yfwd_w(const C *__restrict__ H, C *__restrict__ Hy,
+                                                               const C *__restrict__ tw, int nkx, int nky, int NY) {
+    constexpr int YL = ycols<C>(), YT = 64 * ywaves<N, C>();
+    constexpr int LS = N + 1, LPW = YL / ywaves<N, C>();
     extern __shared__ unsigned char buf_raw[];   // YL * LS elements (beyond 64 KiB for N = 768)
     C *buf = reinterpret_cast<C *>(buf_raw);
     const int ntile = (nkx + YL - 1) / YL;
@@ -181,10 +189,10 @@ static void go(hipStream_t st, bool fwd, const C *tw, const C *in, C *out, int n
     const size_t shm = ywave_lds<C>(N);
     if (fwd) {
         if (shm > 65536) set_dynamic_lds((const void *)k_yfwd_w<N, C>, shm, "y pass (forward)");
-        hipLaunchKernelGGL((k_yfwd_w<N, C>), grid, dim3(64 * YWAVES), shm, st, in, out, tw, nkx, nky, NY);
+        hipLaunchKernelGGL((k_yfwd_w<N, C>), grid, dim3(64 * ywaves<N, C>()), shm, st, in, out, tw, nkx, nky, NY);
     } else {
         if (shm > 65536) set_dynamic_lds((const void *)k_ybwd_w<N, C>, shm, "y pass (inverse)");
-        hipLaunchKernelGGL((k_ybwd_w<N, C>), grid, dim3(64 * YWAVES), shm, st, in, out, tw, nkx, nky, NY);
+        hipLaunchKernelGGL((k_ybwd_w<N, C>), grid, dim3(64 * ywaves<N, C>()), shm, st, in, out, tw, nkx, nky, NY);
     }
     check_launch("y pass");
 }

@@ -1,9 +1,10 @@
 // y passes of the horizontal transforms on wavefront-local FFTs.
 //
 // A workgroup owns YL consecutive kx columns of one (variable, z) plane, so
-// every global access covers YL consecutive kx (YL complex = 128 B) or a whole
-// kept-ky line; each wavefront transforms its own lines in LDS without
-// workgroup barriers (YL/4 of them, one at 768 points).  These are the y-direction parts of c2r_transform
+// every global access covers YL consecutive kx (YL complex = 128 B, 64 B at
+// 768 points) or a whole kept-ky line; each wavefront transforms its own
+// lines in LDS without workgroup barriers (YL/4 of them, one at 768 points
+// and at 384 in FP64).  These are the y-direction parts of c2r_transform
 // (backward c2c in y) and r2c_transform (forward c2c in y, truncated to the
 // kept ky), slabs.fftw3mpi/transforms.f90:155-301.
 #include "kernels.hpp"
@@ -43,25 +44,29 @@ struct YPlan<768> {
 // kept ky index jy -> slot of the NYAA-long line
 __device__ inline int ky_slot(int jy, int NY, int N) { return jy < NY / 2 ? jy : N - (NY - 1) + jy; }
 
-// kx columns per workgroup for element type C: one 128-byte segment
-template <class C>
-constexpr int ycols() { return int(128 / sizeof(C)); }
+// kx columns per workgroup for element type C: one 128-byte segment, 64
+// bytes at 768 points, where a 128-byte tile of lines fills 98 KB of LDS and
+// leaves one workgroup per CU; the 49 KB tile fits three (512^2 x 257: FP64
+// yfwd 5.34 -> 4.61 ms, ybwd 2.30 -> 2.06 ms per launch)
+constexpr int ycol_bytes(int N) { return N == 768 ? 64 : 128; }
+template <int N, class C>
+constexpr int ycols() { return int(ycol_bytes(N) / sizeof(C)); }
 
 // wavefronts per workgroup.  At 768 points (and 384 in FP64) one line per
-// wavefront (8 wavefronts in FP64, 16 in FP32) rather than four wavefronts
-// of two or four lines each: the tile's LDS (98 KB at 768, 49 KB at 384)
-// bounds the workgroups per CU, so four wavefronts per workgroup left one to
-// three per SIMD, each with two or four lines' butterflies in its registers
-// (768 points: FP64 yfwd 5.90 -> 5.34 ms, FP32 3.89 -> 3.00 ms per launch).
-// The FP32 384-point pass measured slower with 16 wavefronts and keeps four.
+// wavefront rather than four wavefronts of two or four lines each: the
+// tile's LDS bounds the workgroups per CU, so four wavefronts per workgroup
+// left one to three per SIMD, each with two or four lines' butterflies in
+// its registers (768 points, 128-byte tiles: FP64 yfwd 5.90 -> 5.34 ms, FP32
+// 3.89 -> 3.00 ms per launch).  The FP32 384-point pass measured slower with
+// 16 wavefronts and keeps four.
 template <int N, class C>
-constexpr int ywaves() { return N == 768 || (N == 384 && sizeof(C) == 16) ? ycols<C>() : 4; }
+constexpr int ywaves() { return N == 768 || (N == 384 && sizeof(C) == 16) ? ycols<N, C>() : 4; }
 
 // G[b][z][y][ix] = sum_jy Fz[b][z][ix*nky + jy] exp(+i ky y)
 template <int N, class C>
 __global__ __launch_bounds__((64 * ywaves<N, C>())) void k_ybwd_w(const C *__restrict__ Fz, C *__restrict__ G,
                                                                const C *__restrict__ tw, int nkx, int nky, int NY) {
-    constexpr int YL = ycols<C>(), YT = 64 * ywaves<N, C>();
+    constexpr int YL = ycols<N, C>(), YT = 64 * ywaves<N, C>();
     constexpr int LS = N + 1, LPW = YL / ywaves<N, C>();
     extern __shared__ unsigned char buf_raw[];   // YL * LS elements (beyond 64 KiB for N = 768)
     C *buf = reinterpret_cast<C *>(buf_raw);
@@ -125,7 +130,7 @@ __global__ __launch_bounds__((64 * ywaves<N, C>())) void k_ybwd_w(const C *__res
 template <int N, class C>
 __global__ __launch_bounds__((64 * ywaves<N, C>())) void k_yfwd_w(const C *__restrict__ H, C *__restrict__ Hy,
                                                                const C *__restrict__ tw, int nkx, int nky, int NY) {
-    constexpr int YL = ycols<C>(), YT = 64 * ywaves<N, C>();
+    constexpr int YL = ycols<N, C>(), YT = 64 * ywaves<N, C>();
     constexpr int LS = N + 1, LPW = YL / ywaves<N, C>();
     extern __shared__ unsigned char buf_raw[];   // YL * LS elements (beyond 64 KiB for N = 768)
     C *buf = reinterpret_cast<C *>(buf_raw);
@@ -178,12 +183,12 @@ __global__ __launch_bounds__((64 * ywaves<N, C>())) void k_yfwd_w(const C *__res
 
 // LDS of a workgroup of the wavefront y pass at length N
 template <class C>
-static size_t ywave_lds(int N) { return size_t(ycols<C>()) * (N + 1) * sizeof(C); }
+static size_t ywave_lds(int N) { return size_t(ycol_bytes(N) / sizeof(C)) * (N + 1) * sizeof(C); }
 
 template <int N, class C>
 static void go(hipStream_t st, bool fwd, const C *tw, const C *in, C *out, int nbatch, int NZAA, int nkx, int nky,
                int NY) {
-    constexpr int YL = ycols<C>();
+    constexpr int YL = ycols<N, C>();
     const int ntile = (nkx + YL - 1) / YL;
     const dim3 grid(ntile * nbatch * NZAA);
     const size_t shm = ywave_lds<C>(N);

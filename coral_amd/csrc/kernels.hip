@@ -607,7 +607,8 @@ void launch_xpass(hipStream_t st, const FFTPlan &px, const XPassArgs &a) {
     CHECK_LAUNCH();
 }
 
-// c2r only (diagnostics and the transform seam): G[b][line][kx] -> phys[b][line][x]
+// c2r only (diagnostics and the transform seam) for lengths without a
+// wavefront plan: G[b][line][kx] -> phys[b][line][x]
 __global__ __launch_bounds__(256) void k_xbwd(FFTPlanDev p, const double2 *__restrict__ G, double *__restrict__ phys,
                                               long long nlines, int nkx, int LX) {
     extern __shared__ double2 sbuf[];
@@ -637,7 +638,8 @@ __global__ __launch_bounds__(256) void k_xbwd(FFTPlanDev p, const double2 *__res
     }
 }
 
-// r2c only: phys[b][line][x] -> H[b][line][kx] (kept kx)
+// r2c only, for lengths without a wavefront plan: phys[b][line][x] ->
+// H[b][line][kx] (kept kx)
 __global__ __launch_bounds__(256) void k_xfwd(FFTPlanDev p, const double *__restrict__ phys, double2 *__restrict__ H,
                                               long long nlines, int nkx, int LX) {
     extern __shared__ double2 sbuf[];
@@ -667,6 +669,7 @@ static int xlines_for(int N) {
 
 void launch_xbwd(hipStream_t st, const FFTPlan &px, const double2 *G, double *phys, int nbatch, long long nlines,
                  int nkx) {
+    if (launch_xbwd_wave(st, px, G, phys, nbatch, nlines, nkx)) return;
     int LX = xlines_for(px.N);
     if (!fft_fits(LX, px.N)) return;
     size_t shmem = size_t(LX) * (px.N + 1) * sizeof(double2);
@@ -677,6 +680,7 @@ void launch_xbwd(hipStream_t st, const FFTPlan &px, const double2 *G, double *ph
 
 void launch_xfwd(hipStream_t st, const FFTPlan &px, const double *phys, double2 *H, int nbatch, long long nlines,
                  int nkx) {
+    if (launch_xfwd_wave(st, px, phys, H, nbatch, nlines, nkx)) return;
     int LX = xlines_for(px.N);
     if (!fft_fits(LX, px.N)) return;
     size_t shmem = size_t(LX) * (px.N + 1) * sizeof(double2);

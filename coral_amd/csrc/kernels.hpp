@@ -136,6 +136,12 @@ void launch_xpass(hipStream_t st, const FFTPlan &px, const XPassArgs &a);
 // wavefront-per-line x pass for N in {96, 144, 192, 384, 768} and up to 6 full
 // variables; returns false (nothing launched) for other shapes
 bool launch_xpass_wave(hipStream_t st, const FFTPlan &px, const XPassArgs &a);
+// the x transforms alone on the x pass's wavefront plans (FP64, N = 96, 144,
+// 192, 384, 768); false (nothing launched) for other lengths
+bool launch_xbwd_wave(hipStream_t st, const FFTPlan &px, const double2 *G, double *phys, int nbatch, long long nlines,
+                      int nkx);
+bool launch_xfwd_wave(hipStream_t st, const FFTPlan &px, const double *phys, double2 *H, int nbatch, long long nlines,
+                      int nkx);
 // wavefront-FFT y passes for NYAA in {96, 144, 192, 384, 768}; false (nothing launched) otherwise
 bool launch_ypass_wave(hipStream_t st, bool fwd, const FFTPlan &py, const double2 *in, double2 *out, int nbatch,
                        int NZAA, int nkx, int nky, int NY);

@@ -544,6 +544,223 @@ static bool launch_wave_n(hipStream_t st, const FFTPlan &px, const XPassArgs &a)
     return launch_wave_c<N>(st, px.tw, a);
 }
 
+// ---- the x transforms alone (FP64) ----
+// c2r of nbatch variables, G[b][line][kx] -> phys[b][line][x], and r2c,
+// phys -> H[b][line][kx] (kept kx), on the x pass's plans: two variables of a
+// line (b = 2p, 2p + 1, blockIdx.y = p) share one complex sequence, the inverse
+// from the packed Hermitian spectra as in the x pass, the forward separated
+// with Hermitian symmetry as its products are.  The c2r / r2c of
+// slabs.fftw3mpi/transforms.f90:155-301 in x, for the transform seam, the
+// physical outputs, the side products and the closure's inputs and outputs.
+template <int N, int NK>
+__global__ __launch_bounds__(256) void k_xbwd_wave(const double2 *__restrict__ G, double *__restrict__ phys,
+                                                   const double2 *__restrict__ tw, long long nlines, int nkx_arg,
+                                                   int nbatch) {
+    using X = XShape<N, 1, 1>;
+    using C = double2;
+    constexpr int R0 = X::R0, RL = X::RL, T0 = X::T0, TL = X::TL, LPW = X::LPW, SIN = X::SIN, LN = X::LN;
+    extern __shared__ unsigned char lds_raw[];
+    C *buf = reinterpret_cast<C *>(lds_raw) + (threadIdx.x / LN) * X::SMAX * N;
+    const int lane = threadIdx.x % LN;
+    const long long line0 = ((long long)blockIdx.x * (blockDim.x / LN) + threadIdx.x / LN) * LPW;
+    const int nkx = NK > 0 ? NK : nkx_arg;
+    const int f0 = 2 * blockIdx.y, f1 = f0 + 1;
+    const bool has1 = f1 < nbatch;
+    {
+        constexpr int TOT = SIN * T0, ROUNDS = (TOT + LN - 1) / LN;
+        C v[ROUNDS][R0];
+#pragma unroll
+        for (int c = 0; c < ROUNDS; ++c) {
+            const int b = lane + LN * c;
+            if (TOT % LN == 0 || b < TOT) {
+                const int l = b / T0, j = b - l * T0;
+                const long long line = line0 + l;
+                const bool valid = line < nlines;
+                const size_t lo = valid ? (size_t)line : 0;
+                const C *g0 = G + ((size_t)f0 * nlines + lo) * nkx;
+                const C *g1 = G + ((size_t)(has1 ? f1 : f0) * nlines + lo) * nkx;
+#pragma unroll
+                for (int r = 0; r < R0; ++r) {
+                    const int e = j + r * T0;
+                    const int lo_e = r * T0, hi_e = lo_e + T0 - 1;
+                    const bool all_pos = NK > 0 && hi_e < NK;
+                    const bool all_neg = NK > 0 && lo_e > N - NK;
+                    const bool all_zero = NK > 0 && lo_e >= NK && hi_e <= N - NK;
+                    if (all_zero) {
+                        v[c][r] = mkc<C>(0, 0);
+                        continue;
+                    }
+                    const bool pos = all_pos || (!all_neg && e < nkx);
+                    const bool neg = all_neg || (!all_pos && e > N - nkx);
+                    const int k = pos ? e : (neg ? N - e : 0);
+                    const C A = g0[k], B = g1[k];
+                    const bool e0 = r == 0 && j == 0;   // kx = 0: imaginary parts dropped
+                    const double bx = has1 ? B.x : 0.0, by = (has1 && !e0) ? B.y : 0.0;
+                    const double ay = !e0 ? A.y : 0.0;
+                    // pos: A + iB;  neg: conj(A) + i conj(B)
+                    const double zx = pos ? A.x - by : A.x + by;
+                    const double zy = pos ? ay + bx : bx - ay;
+                    const bool keep = valid && (all_pos || all_neg || pos || neg);
+                    v[c][r] = mkc<C>(keep ? zx : 0.0, keep ? zy : 0.0);
+                }
+                Dft<R0, +1>::run(v[c]);
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < ROUNDS; ++c) {
+            const int b = lane + LN * c;
+            if (TOT % LN == 0 || b < TOT) {
+                const int l = b / T0, j = b - l * T0;
+#pragma unroll
+                for (int r = 0; r < R0; ++r) buf[l * N + wfft::swz<XPlan<N>::W0>(j * R0 + r)] = v[c][r];
+            }
+        }
+        group_sync<LN>();
+    }
+    XPlan<N>::template InvMid<SIN>::run(buf, tw, lane);
+    // last stage in registers: x = j + r TL of both lines, stored to consecutive x
+#pragma unroll
+    for (int c = 0; c < X::NT; ++c) {
+        const int t = lane + LN * c, l = t / TL, j = t - l * TL;
+        const long long line = line0 + l;
+        const C *L = buf + l * N;
+        C wp[RL], x[RL];
+        wfft::twiddle_powers<RL, +1>(tw, j, wp);
+#pragma unroll
+        for (int r = 0; r < RL; ++r) {
+            x[r] = L[j + r * TL];
+            if (r > 0) x[r] = cmul(x[r], wp[r]);
+        }
+        Dft<RL, +1>::run(x);
+        if (line < nlines) {
+            double *o0 = phys + ((size_t)f0 * nlines + line) * N + j;
+            double *o1 = phys + ((size_t)f1 * nlines + line) * N + j;
+#pragma unroll
+            for (int r = 0; r < RL; ++r) {
+                o0[r * TL] = x[r].x;
+                if (has1) o1[r * TL] = x[r].y;
+            }
+        }
+    }
+}
+
+template <int N>
+__global__ __launch_bounds__(256) void k_xfwd_wave(const double *__restrict__ phys, double2 *__restrict__ H,
+                                                   const double2 *__restrict__ tw, long long nlines, int nkx,
+                                                   int nbatch) {
+    using X = XShape<N, 1, 1>;
+    using C = double2;
+    constexpr int RL = X::RL, TL = X::TL, LPW = X::LPW, LN = X::LN;
+    extern __shared__ unsigned char lds_raw[];
+    C *buf = reinterpret_cast<C *>(lds_raw) + (threadIdx.x / LN) * X::SMAX * N;
+    const int lane = threadIdx.x % LN;
+    const long long line0 = ((long long)blockIdx.x * (blockDim.x / LN) + threadIdx.x / LN) * LPW;
+    const int f0 = 2 * blockIdx.y, f1 = f0 + 1;
+    const bool has1 = f1 < nbatch;
+    // first forward stage from the two lines' values at x = j + r TL
+#pragma unroll
+    for (int c = 0; c < X::NT; ++c) {
+        const int t = lane + LN * c, l = t / TL, j = t - l * TL;
+        const long long line = line0 + l;
+        const bool valid = line < nlines;
+        const double *i0 = phys + ((size_t)f0 * nlines + (valid ? line : 0)) * N + j;
+        const double *i1 = phys + ((size_t)(has1 ? f1 : f0) * nlines + (valid ? line : 0)) * N + j;
+        C y[RL];
+#pragma unroll
+        for (int r = 0; r < RL; ++r) {
+            const double a = i0[r * TL], b = i1[r * TL];
+            y[r] = mkc<C>(valid ? a : 0.0, (valid && has1) ? b : 0.0);
+        }
+        Dft<RL, -1>::run(y);
+#pragma unroll
+        for (int r = 0; r < RL; ++r) buf[l * N + wfft::swz<XPlan<N>::WL>(j * RL + r)] = y[r];
+    }
+    group_sync<LN>();
+    XPlan<N>::template FwdRest<LPW>::run(buf, tw, lane);
+#pragma unroll
+    for (int l = 0; l < LPW; ++l) {
+        const long long line = line0 + l;
+        if (line < nlines) {
+            const C *L = buf + l * N;
+            C *h0 = H + ((size_t)f0 * nlines + line) * nkx;
+            C *h1 = H + ((size_t)f1 * nlines + line) * nkx;
+            for (int k = lane; k < nkx; k += LN) {
+                const C y = L[k], yc = cconj(L[k == 0 ? 0 : N - k]);
+                // f0 = (Y + conj Y(-k)) / 2,  f1 = -i (Y - conj Y(-k)) / 2
+                h0[k] = mkc<C>(0.5 * (y.x + yc.x), 0.5 * (y.y + yc.y));
+                if (has1) h1[k] = mkc<C>(0.5 * (y.y - yc.y), -0.5 * (y.x - yc.x));
+            }
+        }
+    }
+}
+
+// grid of the x transforms: lane groups per workgroup within 64 KiB of LDS,
+// blockIdx.y over the variable pairs
+template <int N>
+static bool xt_shape(long long nlines, int nbatch, dim3 &grid, dim3 &block, size_t &shm) {
+    using X = XShape<N, 1, 1>;
+    const size_t per_group = size_t(X::SMAX) * N * sizeof(double2);
+    int gpb = 4 / (X::LN / 64);
+    while (gpb > 1 && gpb * per_group > 65536) gpb /= 2;
+    if (gpb * per_group > 65536) return false;
+    const long long per_block = (long long)gpb * X::LPW;
+    grid = dim3(unsigned((nlines + per_block - 1) / per_block), unsigned((nbatch + 1) / 2));
+    block = dim3(X::LN * gpb);
+    shm = gpb * per_group;
+    return true;
+}
+
+template <int N>
+static bool xbwd_n(hipStream_t st, const double2 *tw, const double2 *G, double *phys, int nbatch, long long nlines,
+                   int nkx) {
+    dim3 grid, block;
+    size_t shm;
+    if (!xt_shape<N>(nlines, nbatch, grid, block, shm)) return false;
+    if (3 * nkx == N)
+        hipLaunchKernelGGL((k_xbwd_wave<N, N / 3>), grid, block, shm, st, G, phys, tw, nlines, nkx, nbatch);
+    else
+        hipLaunchKernelGGL((k_xbwd_wave<N, 0>), grid, block, shm, st, G, phys, tw, nlines, nkx, nbatch);
+    check_launch("x c2r");
+    return true;
+}
+
+template <int N>
+static bool xfwd_n(hipStream_t st, const double2 *tw, const double *phys, double2 *H, int nbatch, long long nlines,
+                   int nkx) {
+    dim3 grid, block;
+    size_t shm;
+    if (!xt_shape<N>(nlines, nbatch, grid, block, shm)) return false;
+    hipLaunchKernelGGL((k_xfwd_wave<N>), grid, block, shm, st, phys, H, tw, nlines, nkx, nbatch);
+    check_launch("x r2c");
+    return true;
+}
+
+bool launch_xbwd_wave(hipStream_t st, const FFTPlan &px, const double2 *G, double *phys, int nbatch, long long nlines,
+                      int nkx) {
+    if (nbatch < 1 || nlines < 1 || 2 * nkx > px.N || !px.tw) return false;
+    switch (px.N) {
+    case 96: return xbwd_n<96>(st, px.tw, G, phys, nbatch, nlines, nkx);
+    case 144: return xbwd_n<144>(st, px.tw, G, phys, nbatch, nlines, nkx);
+    case 192: return xbwd_n<192>(st, px.tw, G, phys, nbatch, nlines, nkx);
+    case 384: return xbwd_n<384>(st, px.tw, G, phys, nbatch, nlines, nkx);
+    case 768: return xbwd_n<768>(st, px.tw, G, phys, nbatch, nlines, nkx);
+    default: return false;
+    }
+}
+
+bool launch_xfwd_wave(hipStream_t st, const FFTPlan &px, const double *phys, double2 *H, int nbatch, long long nlines,
+                      int nkx) {
+    if (nbatch < 1 || nlines < 1 || 2 * nkx > px.N || !px.tw) return false;
+    switch (px.N) {
+    case 96: return xfwd_n<96>(st, px.tw, phys, H, nbatch, nlines, nkx);
+    case 144: return xfwd_n<144>(st, px.tw, phys, H, nbatch, nlines, nkx);
+    case 192: return xfwd_n<192>(st, px.tw, phys, H, nbatch, nlines, nkx);
+    case 384: return xfwd_n<384>(st, px.tw, phys, H, nbatch, nlines, nkx);
+    case 768: return xfwd_n<768>(st, px.tw, phys, H, nbatch, nlines, nkx);
+    default: return false;
+    }
+}
+
 bool launch_xpass_wave(hipStream_t st, const FFTPlan &px, const XPassArgs &a) {
     if (a.nfull < 1 || 2 * a.nkx > px.N) return false;
     switch (px.N) {

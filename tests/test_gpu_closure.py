@@ -273,19 +273,24 @@ def test_training_lowers_the_loss(nnmod):
     assert np.isfinite(losses[-1]) and losses[-1] < losses[0] / 10
 
 
-def test_step_closure_matches_network(nnmod):
+@pytest.mark.parametrize("NX", [32, 64], ids=["x48", "x96"])
+def test_step_closure_matches_network(nnmod, NX):
     """coral_amd_set_closure: the correction the step adds to the quadratic
     variables (coral_amd_products with the closure minus without) equals the
     forward transform of the trainer's network applied to the same physical
     full variables (ClosureNet.forward, then coral_amd_fwd_transform), within
     FP64 rounding of the transforms (both paths run the same BF16 network
-    kernels on the same BF16 inputs)."""
+    kernels on the same BF16 inputs).  At 96-point x lines the step's own x
+    transforms take two variables per complex sequence (four inputs, nine
+    corrections) while coral_amd_physical and coral_amd_fwd_transform take
+    one, so the paired kernels are checked against the single ones (which
+    test_transforms_match_oracle checks against the oracle)."""
     closure, devmem = nnmod
     from conftest import product_recipe, seeded_state
     from coral_amd.diagnostics import gauss_chebyshev_grid
     from coral_amd.solver import Solver
     r = product_recipe("coral.equations.rbc_ra1e5")
-    NX, NZ, M = 32, 17, 3
+    NZ, M = 17, 3
     NXAA, NZAA = 3 * NX // 2, 3 * NZ // 2
     basis = closure.fourier_basis(NXAA, NXAA, M)
     cin = closure.input_width(4)

@@ -394,7 +394,10 @@ def test_zfiltered_quadratics_match_oracle(solver_mod, NX, NY, NZ):
 
 def test_transforms_match_oracle(solver_mod):
     r = product_recipe("coral.equations.rbc_ra1e5")
-    for NX, NY, NZ, Lx, Ly in [(16, 16, 17, 2 * np.pi, 2 * np.pi), (64, 48, 33, 2.0, 3.0)]:
+    # x lines of 24 (generic kernels), 96, 144, 192, 384 and 768 points
+    # (wavefront kernels)
+    for NX, NY, NZ, Lx, Ly in [(16, 16, 17, 2 * np.pi, 2 * np.pi), (64, 48, 33, 2.0, 3.0), (96, 16, 9, 2.0, 1.0),
+                               (128, 16, 9, 2.0, 1.0), (256, 16, 9, 2.0, 1.0), (512, 16, 9, 2.0, 1.0)]:
         geo = O.Geometry(NX, NY, NZ, Lx, Ly)
         rng = np.random.default_rng(NX)
         S = rng.normal(size=(geo.nkx, geo.nky, NZ)) + 1j * rng.normal(size=(geo.nkx, geo.nky, NZ))
